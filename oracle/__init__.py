@@ -1,0 +1,366 @@
+"""Test infrastructure only: ctypes front-end to the CPU checkers.
+
+* ``Oracle``  -- oracle/_ref/liboracle.so, the plain-C restatement of the
+  reference hot path (tc_oracle.c), FP32 reference order and FP64 yardstick.
+* ``RefLib``  -- oracle/_ref/libtcref.so, the reference's own hot-path sources
+  compiled by oracle/Makefile (present only where the reference tree was
+  available at build time).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
+import this package, and only to check or to time the CPU side; the product
+never routes through it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from ctypes import POINTER, c_double, c_float, c_int, c_int32, c_uint32, c_void_p
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(HERE, "_ref")
+LIB_ORACLE = os.path.join(REF_DIR, "liboracle.so")
+LIB_REF = os.path.join(REF_DIR, "libtcref.so")
+
+LOSS_MSE, LOSS_LOGLOSS = 0, 1
+
+
+def build(reference_root: str = "/root/reference") -> None:
+    """Compile the checkers (make -C oracle); libtcref only if the reference exists."""
+    subprocess.run(["make", "-s", "-C", HERE, f"REF={reference_root}", "oracle"], check=True)
+    if os.path.exists(os.path.join(reference_root, "src", "tree", "symmetric_tree.cpp")):
+        subprocess.run(["make", "-s", "-j8", "-C", HERE, f"REF={reference_root}", "ref"], check=True)
+
+
+def _p(a):
+    return a.ctypes.data_as(c_void_p) if a is not None else None
+
+
+class TreeParams(ctypes.Structure):
+    _fields_ = [
+        ("max_depth", c_uint32),
+        ("max_bins", c_uint32),
+        ("lambda_l2", c_float),
+        ("lambda_l1", c_float),
+        ("max_delta_step", c_float),
+        ("min_samples_leaf", c_uint32),
+        ("min_child_weight", c_float),
+        ("min_split_gain", c_float),
+    ]
+
+
+def tree_params(max_depth=6, max_bins=255, lambda_l2=1.0, lambda_l1=0.0, max_delta_step=0.0,
+                min_samples_leaf=1, min_child_weight=1.0, min_split_gain=0.0) -> TreeParams:
+    return TreeParams(max_depth, max_bins, lambda_l2, lambda_l1, max_delta_step,
+                      min_samples_leaf, min_child_weight, min_split_gain)
+
+
+class Oracle:
+    def __init__(self, path: str = LIB_ORACLE):
+        if not os.path.exists(path):
+            build()
+        self.L = L = ctypes.CDLL(path)
+        V = c_void_p
+        L.tco_quantize.argtypes = [V, c_uint32, c_uint32, c_uint32, c_int, V, V, V]
+        L.tco_apply_bins.argtypes = [V, c_uint32, c_uint32, V, V, V]
+        L.tco_init_prediction.argtypes = [c_int, V, c_uint32]
+        L.tco_init_prediction.restype = c_float
+        for fn in ("tco_gradients_f32", "tco_gradients_f64"):
+            getattr(L, fn).argtypes = [c_int, V, V, V, V, c_uint32]
+        L.tco_gradients_mc_f64.argtypes = [V, V, V, V, c_uint32, c_uint32]
+        L.tco_hist_f32.argtypes = [V, c_uint32, c_uint32, V, V, V, c_uint32, V, V]
+        L.tco_hist_f64.argtypes = [V, c_uint32, c_uint32, V, V, V, c_uint32, V, V, V]
+        L.tco_split_f32.argtypes = [V, V, V, V, c_uint32, c_uint32, c_uint32, c_float, c_int, V, V, V]
+        L.tco_split_f64.argtypes = [V, V, V, V, c_uint32, c_uint32, c_uint32, c_double, c_int, V, V, V, V]
+        L.tco_tree_build_f32.argtypes = [V, c_uint32, c_uint32, V, V, V, V, POINTER(TreeParams),
+                                         V, V, V, V, V, V]
+        L.tco_tree_build_f64.argtypes = [V, c_uint32, c_uint32, V, V, V, V, POINTER(TreeParams),
+                                         V, V, V, V, V, V, V]
+        L.tco_tree_build_mc_f64.argtypes = [V, c_uint32, c_uint32, c_uint32, V, V, V, V,
+                                            POINTER(TreeParams), V, V, V, V, V, V, V]
+        L.tco_leaf_index_binned.argtypes = [V, c_uint32, c_uint32, V, V, V]
+        L.tco_predict_f32.argtypes = [V, V, V, V, V, c_uint32, c_uint32, c_float, V, c_uint32,
+                                      c_uint32, V, V]
+        L.tco_predict_f64.argtypes = [V, V, V, V, V, c_uint32, c_uint32, c_double, V, c_uint32,
+                                      c_uint32, V, V]
+        L.tco_fit_f32.argtypes = [V, c_uint32, c_uint32, V, V, V, c_int, c_uint32, c_float,
+                                  POINTER(TreeParams), V, V, V, V, V, V, V]
+        L.tco_fit_f32.restype = c_float
+        for fn in ("tco_tree_build_f32", "tco_tree_build_f64", "tco_tree_build_mc_f64"):
+            getattr(L, fn).restype = c_int
+
+    # ---------------------------------------------------------------- A2/A3
+    def quantize(self, X, max_bins=255, wrap=True):
+        X = np.ascontiguousarray(X, np.float32)
+        n, F = X.shape
+        bins = np.zeros((F, n), np.uint8)
+        edges = np.zeros((F, 255), np.float32)
+        ne = np.zeros(F, np.uint32)
+        self.L.tco_quantize(_p(X), n, F, max_bins, int(wrap), _p(bins), _p(edges), _p(ne))
+        return bins, edges, ne
+
+    def apply_bins(self, X, edges, n_edges):
+        X = np.ascontiguousarray(X, np.float32)
+        n, F = X.shape
+        bins = np.zeros((F, n), np.uint8)
+        self.L.tco_apply_bins(_p(X), n, F, _p(np.ascontiguousarray(edges, np.float32)),
+                              _p(np.ascontiguousarray(n_edges, np.uint32)), _p(bins))
+        return bins
+
+    # ---------------------------------------------------------------- A5-A8
+    def init_prediction(self, loss, y):
+        y = np.ascontiguousarray(y, np.float32)
+        return float(self.L.tco_init_prediction(loss, _p(y), len(y)))
+
+    def gradients_f32(self, loss, y, p):
+        y = np.ascontiguousarray(y, np.float32); p = np.ascontiguousarray(p, np.float32)
+        g = np.zeros_like(y); h = np.zeros_like(y)
+        self.L.tco_gradients_f32(loss, _p(y), _p(p), _p(g), _p(h), len(y))
+        return g, h
+
+    def gradients_f64(self, loss, y, p):
+        y = np.ascontiguousarray(y, np.float32); p = np.ascontiguousarray(p, np.float32)
+        g = np.zeros(len(y)); h = np.zeros(len(y))
+        self.L.tco_gradients_f64(loss, _p(y), _p(p), _p(g), _p(h), len(y))
+        return g, h
+
+    def gradients_mc_f64(self, y, p):
+        y = np.ascontiguousarray(y, np.float32); p = np.ascontiguousarray(p, np.float32)
+        n, K = p.shape
+        g = np.zeros((n, K)); h = np.zeros((n, K))
+        self.L.tco_gradients_mc_f64(_p(y), _p(p), _p(g), _p(h), n, K)
+        return g, h
+
+    # ---------------------------------------------------------------- A9
+    def hist_f32(self, bins, g, h, rows=None):
+        F, n = bins.shape
+        gh = np.zeros((F, 256, 2), np.float32); cnt = np.zeros((F, 256), np.uint32)
+        r = None if rows is None else np.ascontiguousarray(rows, np.uint32)
+        self.L.tco_hist_f32(_p(bins), n, F, _p(np.ascontiguousarray(g, np.float32)),
+                            _p(np.ascontiguousarray(h, np.float32)), _p(r),
+                            0 if r is None else len(r), _p(gh), _p(cnt))
+        return gh, cnt
+
+    def hist_f64(self, bins, g, h, rows=None):
+        F, n = bins.shape
+        gh = np.zeros((F, 256, 2)); ab = np.zeros((F, 256)); cnt = np.zeros((F, 256), np.uint32)
+        r = None if rows is None else np.ascontiguousarray(rows, np.uint32)
+        self.L.tco_hist_f64(_p(bins), n, F, _p(np.ascontiguousarray(g, np.float32)),
+                            _p(np.ascontiguousarray(h, np.float32)), _p(r),
+                            0 if r is None else len(r), _p(gh), _p(ab), _p(cnt))
+        return gh, ab, cnt
+
+    # ---------------------------------------------------------------- A10
+    def split_f32(self, hist, node_g, node_h, node_count, max_bins=255, lam=1.0, is_root=False):
+        hist = np.ascontiguousarray(hist, np.float32)
+        nn, F = hist.shape[0], hist.shape[1]
+        f = c_int32(); b = c_int32(); gn = c_float()
+        self.L.tco_split_f32(_p(hist), _p(np.ascontiguousarray(node_g, np.float32)),
+                             _p(np.ascontiguousarray(node_h, np.float32)),
+                             _p(np.ascontiguousarray(node_count, np.uint32)), nn, F, max_bins,
+                             lam, int(is_root), ctypes.byref(f), ctypes.byref(b), ctypes.byref(gn))
+        return f.value, b.value, gn.value
+
+    def split_f64(self, hist, node_g, node_h, node_count, max_bins=255, lam=1.0, is_root=False):
+        hist = np.ascontiguousarray(hist, np.float64)
+        nn, F = hist.shape[0], hist.shape[1]
+        f = c_int32(); b = c_int32(); gn = c_double(); sec = c_double()
+        self.L.tco_split_f64(_p(hist), _p(np.ascontiguousarray(node_g, np.float64)),
+                             _p(np.ascontiguousarray(node_h, np.float64)),
+                             _p(np.ascontiguousarray(node_count, np.uint32)), nn, F, max_bins,
+                             lam, int(is_root), ctypes.byref(f), ctypes.byref(b), ctypes.byref(gn),
+                             ctypes.byref(sec))
+        return f.value, b.value, gn.value, sec.value
+
+    # ---------------------------------------------------------------- A9-A14
+    def tree_f32(self, bins, g, h, edges, n_edges, params: TreeParams):
+        F, n = bins.shape
+        D = params.max_depth
+        feat = np.zeros(D, np.uint16); b = np.zeros(D, np.uint8); ft = np.zeros(D, np.float32)
+        gn = np.zeros(D, np.float32); lv = np.zeros(1 << D, np.float32); li = np.zeros(n, np.uint32)
+        d = self.L.tco_tree_build_f32(_p(bins), n, F, _p(np.ascontiguousarray(g, np.float32)),
+                                      _p(np.ascontiguousarray(h, np.float32)), _p(edges),
+                                      _p(n_edges), ctypes.byref(params), _p(feat), _p(b), _p(ft),
+                                      _p(gn), _p(lv), _p(li))
+        return dict(depth=d, feature=feat[:d], bin=b[:d], fthr=ft[:d], gain=gn[:d],
+                    leaves=lv[:1 << d], leaf_idx=li)
+
+    def tree_f64(self, bins, g, h, edges, n_edges, params: TreeParams):
+        F, n = bins.shape
+        D = params.max_depth
+        feat = np.zeros(D, np.uint16); b = np.zeros(D, np.uint8); ft = np.zeros(D, np.float32)
+        gn = np.zeros(D); gap = np.zeros(D); lv = np.zeros(1 << D); li = np.zeros(n, np.uint32)
+        d = self.L.tco_tree_build_f64(_p(bins), n, F, _p(np.ascontiguousarray(g, np.float32)),
+                                      _p(np.ascontiguousarray(h, np.float32)), _p(edges),
+                                      _p(n_edges), ctypes.byref(params), _p(feat), _p(b), _p(ft),
+                                      _p(gn), _p(gap), _p(lv), _p(li))
+        return dict(depth=d, feature=feat[:d], bin=b[:d], fthr=ft[:d], gain=gn[:d], gap=gap[:d],
+                    leaves=lv[:1 << d], leaf_idx=li)
+
+    def tree_mc_f64(self, bins, g, h, edges, n_edges, params: TreeParams):
+        F, n = bins.shape
+        K = g.shape[1]
+        D = params.max_depth
+        feat = np.zeros(D, np.uint16); b = np.zeros(D, np.uint8); ft = np.zeros(D, np.float32)
+        gn = np.zeros(D); gap = np.zeros(D); lv = np.zeros((1 << D) * K); li = np.zeros(n, np.uint32)
+        d = self.L.tco_tree_build_mc_f64(_p(bins), n, F, K, _p(np.ascontiguousarray(g, np.float64)),
+                                         _p(np.ascontiguousarray(h, np.float64)), _p(edges),
+                                         _p(n_edges), ctypes.byref(params), _p(feat), _p(b),
+                                         _p(ft), _p(gn), _p(gap), _p(lv), _p(li))
+        return dict(depth=d, feature=feat[:d], bin=b[:d], fthr=ft[:d], gain=gn[:d], gap=gap[:d],
+                    leaves=lv[:(1 << d) * K].reshape(1 << d, K), leaf_idx=li)
+
+    def leaf_index_binned(self, bins, feature, bin_):
+        F, n = bins.shape
+        li = np.zeros(n, np.uint32)
+        self.L.tco_leaf_index_binned(_p(bins), n, len(feature),
+                                     _p(np.ascontiguousarray(feature, np.uint16)),
+                                     _p(np.ascontiguousarray(bin_, np.uint8)), _p(li))
+        return li
+
+    # ---------------------------------------------------------------- A16
+    def predict_f32(self, model, X, want_leaf_idx=False):
+        X = np.ascontiguousarray(X, np.float32)
+        n, F = X.shape
+        T, D = model["T"], model["D"]
+        out = np.zeros(n, np.float32)
+        li = np.zeros((n, T), np.uint32) if want_leaf_idx else None
+        self.L.tco_predict_f32(_p(model["feature"]), _p(model["fthr"]), _p(model["depth"]),
+                               _p(model["leaves"]), _p(model["weight"]), T, D, model["base"],
+                               _p(X), n, F, _p(out), _p(li))
+        return (out, li) if want_leaf_idx else out
+
+    def predict_f64(self, model, X):
+        X = np.ascontiguousarray(X, np.float32)
+        n, F = X.shape
+        out = np.zeros(n); ab = np.zeros(n)
+        self.L.tco_predict_f64(_p(model["feature"]), _p(model["fthr"]), _p(model["depth"]),
+                               _p(model["leaves"]), _p(model["weight"]), model["T"], model["D"],
+                               model["base"], _p(X), n, F, _p(out), _p(ab))
+        return out, ab
+
+    # ---------------------------------------------------------------- fit
+    def fit_f32(self, bins, edges, n_edges, y, loss, n_trees, lr, params: TreeParams):
+        F, n = bins.shape
+        D = params.max_depth
+        m = new_model(n_trees, D)
+        tp = np.zeros(n, np.float32)
+        base = self.L.tco_fit_f32(_p(bins), n, F, _p(edges), _p(n_edges),
+                                  _p(np.ascontiguousarray(y, np.float32)), loss, n_trees, lr,
+                                  ctypes.byref(params), _p(m["feature"]), _p(m["bin"]),
+                                  _p(m["fthr"]), _p(m["gain"]), _p(m["depth"]), _p(m["leaves"]),
+                                  _p(tp))
+        m["base"] = float(base)
+        m["weight"][:] = lr
+        return m, tp
+
+
+def new_model(T, D):
+    return dict(T=T, D=D, feature=np.zeros(T * D, np.uint16), bin=np.zeros(T * D, np.uint8),
+                fthr=np.zeros(T * D, np.float32), gain=np.zeros(T * D, np.float32),
+                depth=np.zeros(T, np.uint16), leaves=np.zeros(T * (1 << D), np.float32),
+                weight=np.zeros(T, np.float32), base=0.0)
+
+
+class RefLib:
+    """The reference's own hot path, compiled from its sources (oracle/Makefile)."""
+
+    def __init__(self, path: str = LIB_REF):
+        self.L = L = ctypes.CDLL(path)
+        V = c_void_p
+        L.tcref_dataset_create.restype = V
+        L.tcref_dataset_create.argtypes = [V, V, c_uint32, c_uint32, c_uint32, V]
+        L.tcref_dataset_free.argtypes = [V]
+        L.tcref_dataset_bins.argtypes = [V, V, V, V]
+        L.tcref_gradients.argtypes = [c_int, V, V, V, V, c_uint32]
+        L.tcref_init_prediction.argtypes = [c_int, V, c_uint32]
+        L.tcref_init_prediction.restype = c_float
+        L.tcref_hist.argtypes = [V, V, V, V, c_uint32, V, V]
+        L.tcref_tree_build.argtypes = [V, V, V, c_uint32, c_float, c_float, V, V, V, V, V, V]
+        L.tcref_tree_build.restype = c_int
+        L.tcref_fit.argtypes = [V, c_int, c_uint32, c_uint32, c_float, c_float, c_float,
+                                V, V, V, V, V, V, V, V]
+        L.tcref_fit.restype = c_float
+        L.tcref_predict.argtypes = [V, V, V, V, V, c_uint32, c_uint32, c_float, c_float, V,
+                                    c_uint32, c_uint32, V]
+
+    @staticmethod
+    def available(path: str = LIB_REF) -> bool:
+        return os.path.exists(path)
+
+    def dataset(self, X, y=None, max_bins=255):
+        X = np.ascontiguousarray(X, np.float32)
+        n, F = X.shape
+        y = np.zeros(n, np.float32) if y is None else np.ascontiguousarray(y, np.float32)
+        sec = np.zeros(1)
+        h = self.L.tcref_dataset_create(_p(X), _p(y), n, F, max_bins, _p(sec))
+        return RefDataset(self, h, n, F, float(sec[0]), y)
+
+    def gradients(self, loss, y, p):
+        y = np.ascontiguousarray(y, np.float32); p = np.ascontiguousarray(p, np.float32)
+        g = np.zeros_like(y); h = np.zeros_like(y)
+        self.L.tcref_gradients(loss, _p(y), _p(p), _p(g), _p(h), len(y))
+        return g, h
+
+    def init_prediction(self, loss, y):
+        y = np.ascontiguousarray(y, np.float32)
+        return float(self.L.tcref_init_prediction(loss, _p(y), len(y)))
+
+    def predict(self, model, X):
+        X = np.ascontiguousarray(X, np.float32)
+        n, F = X.shape
+        out = np.zeros(n, np.float32)
+        self.L.tcref_predict(_p(model["feature"]), _p(model["fthr"]), _p(model["bin"]),
+                             _p(model["depth"]), _p(model["leaves"]), model["T"], model["D"],
+                             float(model["weight"][0]) if model["T"] else 0.0, model["base"],
+                             _p(X), n, F, _p(out))
+        return out
+
+
+class RefDataset:
+    def __init__(self, lib, h, n, F, seconds, y):
+        self.lib, self.h, self.n, self.F, self.seconds, self.y = lib, h, n, F, seconds, y
+
+    def __del__(self):
+        try:
+            self.lib.L.tcref_dataset_free(self.h)
+        except Exception:
+            pass
+
+    def bins(self):
+        bins = np.zeros((self.F, self.n), np.uint8)
+        edges = np.zeros((self.F, 255), np.float32)
+        ne = np.zeros(self.F, np.uint32)
+        self.lib.L.tcref_dataset_bins(self.h, _p(bins), _p(edges), _p(ne))
+        return bins, edges, ne
+
+    def hist(self, g, h, rows=None):
+        gh = np.zeros((self.F, 255, 2), np.float32); cnt = np.zeros((self.F, 255), np.uint32)
+        r = None if rows is None else np.ascontiguousarray(rows, np.uint32)
+        self.lib.L.tcref_hist(self.h, _p(np.ascontiguousarray(g, np.float32)),
+                              _p(np.ascontiguousarray(h, np.float32)), _p(r),
+                              0 if r is None else len(r), _p(gh), _p(cnt))
+        return gh, cnt
+
+    def tree(self, g, h, max_depth=6, lambda_l2=1.0, min_child_weight=1.0):
+        D = max_depth
+        feat = np.zeros(D, np.uint16); b = np.zeros(D, np.uint8); ft = np.zeros(D, np.float32)
+        gn = np.zeros(D, np.float32); lv = np.zeros(1 << D, np.float32); li = np.zeros(self.n, np.uint32)
+        d = self.lib.L.tcref_tree_build(self.h, _p(np.ascontiguousarray(g, np.float32)),
+                                        _p(np.ascontiguousarray(h, np.float32)), D, lambda_l2,
+                                        min_child_weight, _p(feat), _p(b), _p(ft), _p(gn), _p(lv),
+                                        _p(li))
+        return dict(depth=d, feature=feat[:d], bin=b[:d], fthr=ft[:d], gain=gn[:d],
+                    leaves=lv[:1 << d], leaf_idx=li)
+
+    def fit(self, loss, n_trees, max_depth=6, lr=0.1, lambda_l2=1.0, min_child_weight=1.0):
+        m = new_model(n_trees, max_depth)
+        tp = np.zeros(self.n, np.float32); sec = np.zeros(3)
+        base = self.lib.L.tcref_fit(self.h, loss, n_trees, max_depth, lr, lambda_l2,
+                                    min_child_weight, _p(m["feature"]), _p(m["bin"]), _p(m["fthr"]),
+                                    _p(m["gain"]), _p(m["depth"]), _p(m["leaves"]), _p(tp), _p(sec))
+        m["base"] = float(base)
+        m["weight"][:] = lr
+        return m, tp, sec

@@ -1,0 +1,247 @@
+// Test infrastructure only: a C-ABI harness over the reference's own
+// hot-path translation units (compiled in place from the read-only reference
+// tree by oracle/Makefile, output in oracle/_ref/).  Used by tests/ to pin the
+// oracle and write golden vectors, and by bench.py's cpu_baseline leg.  Nothing
+// in the product links or loads this.
+//
+// The boosting loop below restates src/boosting/booster.cpp:92-100, :173-174,
+// :438-465, :505-519 and :551-583 (binary/regression, symmetric branch, no
+// sampling) because booster.cpp itself does not compile here (it pulls in
+// Eigen through tree.hpp).  Everything else is the reference's code:
+//   Dataset::from_dense / compute_bins      src/core/dataset.cpp:50-94, :176-344
+//   Dataset::compute_target_statistics      src/core/dataset.cpp:679-721
+//   Loss::create / compute_gradients        include/turbocat/loss.hpp:511, src/boosting/loss.cpp
+//   CPUHistogramBuilder::build              src/tree/histogram.cpp:62-205
+//   SymmetricTree::build / predict_batch    src/tree/symmetric_tree.cpp:34-368, :813-840
+//   SymmetricEnsemble::predict_batch_raw_fast  src/tree/symmetric_tree.cpp:1133-1149
+
+#include "turbocat/config.hpp"
+#include "turbocat/dataset.hpp"
+#include "turbocat/histogram.hpp"
+#include "turbocat/loss.hpp"
+#include "turbocat/symmetric_tree.hpp"
+
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+using namespace turbocat;
+
+namespace {
+
+struct RefData {
+    std::unique_ptr<Dataset> ds;
+    Config cfg;
+};
+
+double now_s() {
+    return std::chrono::duration<double>(
+        std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+Config make_config(uint32_t max_bins, uint32_t max_depth, float lambda_l2,
+                   float min_child_weight, int loss_id) {
+    Config c;
+    c.verbosity = 0;
+    c.tree.max_bins = static_cast<uint16_t>(max_bins);
+    c.tree.max_depth = static_cast<uint16_t>(max_depth);
+    c.tree.lambda_l2 = lambda_l2;
+    c.tree.min_child_weight = min_child_weight;
+    c.tree.use_symmetric = true;
+    c.boosting.subsample = 1.0f;
+    c.boosting.colsample_bytree = 1.0f;
+    if (loss_id == 0) {
+        c.task = TaskType::Regression;
+        c.loss.loss_type = LossType::MSE;
+    } else {
+        c.task = TaskType::BinaryClassification;
+        c.loss.loss_type = LossType::LogLoss;
+    }
+    return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Bin X (row-major n x F) exactly as the reference fit does.  bins_out is
+// column-major F x n; edges_out holds 255 slots per feature.
+void* tcref_dataset_create(const float* X, const float* y, uint32_t n, uint32_t F,
+                           uint32_t max_bins, double* seconds) {
+    auto* r = new RefData();
+    r->cfg = make_config(max_bins, 6, 1.0f, 1.0f, 0);
+    r->ds = std::make_unique<Dataset>();
+    double t0 = now_s();
+    r->ds->from_dense(X, n, static_cast<FeatureIndex>(F), y, nullptr, {});
+    r->ds->compute_bins(r->cfg);
+    if (seconds) *seconds = now_s() - t0;
+    return r;
+}
+
+void tcref_dataset_free(void* h) { delete static_cast<RefData*>(h); }
+
+void tcref_dataset_bins(void* h, uint8_t* bins_out, float* edges_out, uint32_t* n_edges_out) {
+    auto* r = static_cast<RefData*>(h);
+    const Dataset& ds = *r->ds;
+    const uint32_t n = ds.n_samples(), F = ds.n_features();
+    for (uint32_t f = 0; f < F; ++f) {
+        std::memcpy(bins_out + static_cast<size_t>(f) * n, ds.binned().column(f), n);
+        const auto& e = ds.bin_edges()[f];
+        n_edges_out[f] = static_cast<uint32_t>(e.size());
+        for (size_t k = 0; k < e.size() && k < 255; ++k) edges_out[f * 255 + k] = e[k];
+    }
+}
+
+void tcref_gradients(int loss_id, const float* y, const float* p, float* g, float* h, uint32_t n) {
+    LossConfig lc;
+    lc.loss_type = loss_id == 0 ? LossType::MSE : LossType::LogLoss;
+    auto loss = Loss::create(lc, loss_id == 0 ? TaskType::Regression : TaskType::BinaryClassification, 2);
+    loss->compute_gradients(y, p, g, h, n);
+}
+
+float tcref_init_prediction(int loss_id, const float* y, uint32_t n) {
+    LossConfig lc;
+    lc.loss_type = loss_id == 0 ? LossType::MSE : LossType::LogLoss;
+    auto loss = Loss::create(lc, loss_id == 0 ? TaskType::Regression : TaskType::BinaryClassification, 2);
+    return loss->init_prediction(y, n);
+}
+
+// Histogram over a row list (rows == NULL means all rows in order).
+// out_gh: F x 255 x {grad, hess}; out_count: F x 255.
+void tcref_hist(void* h, const float* g, const float* hs, const uint32_t* rows, uint32_t n_rows,
+                float* out_gh, uint32_t* out_count) {
+    auto* r = static_cast<RefData*>(h);
+    Dataset& ds = *r->ds;
+    const uint32_t n = ds.n_samples(), F = ds.n_features();
+    std::memcpy(ds.gradients(), g, n * sizeof(float));
+    std::memcpy(ds.hessians(), hs, n * sizeof(float));
+    std::vector<Index> idx;
+    if (rows) idx.assign(rows, rows + n_rows);
+    else { idx.resize(n); std::iota(idx.begin(), idx.end(), 0u); }
+    Histogram hist(static_cast<FeatureIndex>(F), 255);
+    CPUHistogramBuilder b;
+    b.build(ds, idx, {}, hist);
+    for (uint32_t f = 0; f < F; ++f) {
+        const GradientPair* bp = hist.bins(static_cast<FeatureIndex>(f));
+        for (uint32_t k = 0; k < 255; ++k) {
+            out_gh[(f * 255 + k) * 2 + 0] = bp[k].grad;
+            out_gh[(f * 255 + k) * 2 + 1] = bp[k].hess;
+            out_count[f * 255 + k] = bp[k].count;
+        }
+    }
+}
+
+// One symmetric tree on given gradients (teacher forcing).
+// split_* arrays hold max_depth entries; leaves 2^max_depth; leaf_idx n.
+int tcref_tree_build(void* h, const float* g, const float* hs, uint32_t max_depth,
+                     float lambda_l2, float min_child_weight,
+                     uint16_t* split_feature, uint8_t* split_bin, float* split_fthr,
+                     float* split_gain, float* leaves, uint32_t* leaf_idx) {
+    auto* r = static_cast<RefData*>(h);
+    Dataset& ds = *r->ds;
+    const uint32_t n = ds.n_samples();
+    std::memcpy(ds.gradients(), g, n * sizeof(float));
+    std::memcpy(ds.hessians(), hs, n * sizeof(float));
+    Config c = make_config(r->cfg.tree.max_bins, max_depth, lambda_l2, min_child_weight, 0);
+    std::vector<Index> idx(n);
+    std::iota(idx.begin(), idx.end(), 0u);
+    CPUHistogramBuilder b;
+    SymmetricTree t(c.tree);
+    t.build(ds, idx, b);
+    const int depth = t.depth();
+    for (int d = 0; d < depth; ++d) {
+        split_feature[d] = t.splits()[d].feature;
+        split_bin[d] = t.splits()[d].threshold;
+        split_fthr[d] = t.splits()[d].float_threshold;
+        split_gain[d] = t.splits()[d].gain;
+    }
+    for (uint32_t k = 0; k < (1u << depth); ++k) leaves[k] = t.leaf_values()[k];
+    if (leaf_idx) {
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t li = 0;
+            for (int d = 0; d < depth; ++d) {
+                uint8_t bin = ds.binned().column(t.splits()[d].feature)[i];
+                li = (li << 1) | ((bin != 255 && bin > t.splits()[d].threshold) ? 1u : 0u);
+            }
+            leaf_idx[i] = li;
+        }
+    }
+    return depth;
+}
+
+// Full fit on the dataset (symmetric trees, no sampling), mirroring
+// booster.cpp's loop.  Model arrays: feature/bin/fthr/gain [T*D], depth [T],
+// leaves [T*2^D].  train_preds [n] gets the final training predictions.
+// seconds[0] = target statistics + init, seconds[1] = boosting loop,
+// seconds[2] = time inside CPUHistogramBuilder-driven SymmetricTree::build.
+float tcref_fit(void* h, int loss_id, uint32_t n_trees, uint32_t max_depth, float lr,
+                float lambda_l2, float min_child_weight,
+                uint16_t* feat, uint8_t* bin, float* fthr, float* gain, uint16_t* depths,
+                float* leaves, float* train_preds, double* seconds) {
+    auto* r = static_cast<RefData*>(h);
+    Dataset& ds = *r->ds;
+    const uint32_t n = ds.n_samples();
+    Config c = make_config(r->cfg.tree.max_bins, max_depth, lambda_l2, min_child_weight, loss_id);
+    double t0 = now_s();
+    auto loss = Loss::create(c.loss, c.task, c.n_classes);
+    const float base = loss->init_prediction(ds.labels().data(), n);
+    auto hb = HistogramBuilder::create(c.device);
+    ds.compute_target_statistics(c);
+    double t1 = now_s();
+    std::vector<float> preds(n, base), buf(n);
+    std::vector<Index> all(n);
+    std::iota(all.begin(), all.end(), 0u);
+    const uint32_t L = 1u << max_depth;
+    double tb = 0.0;
+    for (uint32_t it = 0; it < n_trees; ++it) {
+        ds.ensure_gradients_allocated(n);
+        loss->compute_gradients(ds.labels().data(), preds.data(), ds.gradients(), ds.hessians(), n);
+        SymmetricTree t(c.tree);
+        double a = now_s();
+        t.build(ds, all, *hb);
+        tb += now_s() - a;
+        t.predict_batch(ds, buf.data());
+        #pragma omp parallel for
+        for (int64_t i = 0; i < static_cast<int64_t>(n); ++i) preds[i] += lr * buf[i];
+        depths[it] = t.depth();
+        for (int d = 0; d < t.depth(); ++d) {
+            feat[it * max_depth + d] = t.splits()[d].feature;
+            bin[it * max_depth + d] = t.splits()[d].threshold;
+            fthr[it * max_depth + d] = t.splits()[d].float_threshold;
+            gain[it * max_depth + d] = t.splits()[d].gain;
+        }
+        for (uint32_t k = 0; k < (1u << t.depth()); ++k) leaves[it * L + k] = t.leaf_values()[k];
+    }
+    double t2 = now_s();
+    if (train_preds) std::memcpy(train_preds, preds.data(), n * sizeof(float));
+    if (seconds) { seconds[0] = t1 - t0; seconds[1] = t2 - t1; seconds[2] = tb; }
+    return base;
+}
+
+// predict_raw_nobinning_fast without the Booster: flat model -> raw scores
+// (base added, booster.cpp:921-924).
+void tcref_predict(const uint16_t* feat, const float* fthr, const uint8_t* bin, const uint16_t* depths,
+                   const float* leaves, uint32_t n_trees, uint32_t max_depth, float lr, float base,
+                   const float* X, uint32_t n, uint32_t F, float* out) {
+    SymmetricEnsemble ens;
+    const uint32_t L = 1u << max_depth;
+    for (uint32_t t = 0; t < n_trees; ++t) {
+        auto tree = std::make_unique<SymmetricTree>();
+        std::vector<SymmetricSplit> sp(depths[t]);
+        for (int d = 0; d < depths[t]; ++d) {
+            sp[d].feature = feat[t * max_depth + d];
+            sp[d].threshold = bin[t * max_depth + d];
+            sp[d].float_threshold = fthr[t * max_depth + d];
+        }
+        std::vector<float> lv(leaves + t * L, leaves + t * L + (1u << depths[t]));
+        tree->set_data(depths[t], std::move(sp), std::move(lv));
+        ens.add_tree(std::move(tree), lr);
+    }
+    ens.predict_batch_raw_fast(X, n, static_cast<FeatureIndex>(F), out);
+    for (uint32_t i = 0; i < n; ++i) out[i] += base;
+}
+
+}  // extern "C"

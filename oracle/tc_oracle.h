@@ -1,0 +1,135 @@
+/* Test infrastructure only.  CPU oracle for the TurboCat symmetric-tree hot
+ * path: a plain-C restatement of the reference algorithm, in two precisions.
+ *
+ *   *_f32  follow the reference's FP32 arithmetic and summation order so that
+ *          they can be checked bit-for-bit against the reference itself
+ *          (oracle/_ref/libtcref.so, built from the reference sources).
+ *   *_f64  the same algorithm in double precision: the yardstick for the
+ *          1e-5 relative tolerance on sums, leaf values and predictions.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load this library, and only as the checker.  Every function cites the
+ * reference file:line it restates.  Layouts:
+ *   X       row-major n x F float32
+ *   bins    column-major F x n uint8 (reference BinnedData, dataset.hpp:27-82)
+ *   edges   255 float slots per feature, n_edges[f] of them valid
+ *   hist    F x 256 x {grad, hess}; slot 255 holds NaN rows (see below)
+ */
+#ifndef TC_ORACLE_H
+#define TC_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { TCO_LOSS_MSE = 0, TCO_LOSS_LOGLOSS = 1 };
+
+/* A2: quantile binning.  wrap != 0 reproduces the reference's uint8 cast of
+ * the unique count (src/core/dataset.cpp:308); wrap == 0 is the "fixed" mode
+ * A = min(max_bins, U).  Returns 0. */
+int tco_quantize(const float* X, uint32_t n, uint32_t F, uint32_t max_bins, int wrap,
+                 uint8_t* bins, float* edges, uint32_t* n_edges);
+
+/* A3: bin new rows with existing edges (src/core/dataset.cpp:346-379). */
+void tco_apply_bins(const float* X, uint32_t n, uint32_t F, const float* edges,
+                    const uint32_t* n_edges, uint8_t* bins);
+
+/* A5: base score (src/boosting/loss.cpp:217-226, :364-370). */
+float tco_init_prediction(int loss, const float* y, uint32_t n);
+
+/* A6/A7: gradients, FP32 as the reference AVX2 build computes them
+ * (src/boosting/loss.cpp:123-163, :348-362). */
+void tco_gradients_f32(int loss, const float* y, const float* p, float* g, float* h, uint32_t n);
+/* A6/A7 in double precision with an exact exp (yardstick). */
+void tco_gradients_f64(int loss, const float* y, const float* p, double* g, double* h, uint32_t n);
+/* A8: softmax gradients, K channels, row-major n x K (src/boosting/loss.cpp:273-318). */
+void tco_gradients_mc_f64(const float* y, const float* p, double* g, double* h, uint32_t n, uint32_t K);
+
+/* A9: histogram of a row list (rows == NULL: all rows in order).
+ * f32: sequential row-order sums (src/tree/histogram.cpp:93-133, :307-378).
+ * f64: double sums, plus abs_g = sum |g| per slot for the normalized
+ * tolerance.  out_count may be NULL. */
+void tco_hist_f32(const uint8_t* bins, uint32_t n, uint32_t F, const float* g, const float* h,
+                  const uint32_t* rows, uint32_t n_rows, float* out_gh, uint32_t* out_count);
+void tco_hist_f64(const uint8_t* bins, uint32_t n, uint32_t F, const float* g, const float* h,
+                  const uint32_t* rows, uint32_t n_rows, double* out_gh, double* abs_g,
+                  uint32_t* out_count);
+
+/* A10: level split scan over n_nodes histograms (layout n_nodes x F x 256 x 2)
+ * and node totals.  is_root selects the d == 0 rules.  Writes the argmax
+ * (feature, bin, gain) with ties to the lowest (feature, bin), and the best
+ * gain among candidates with a different (feature, bin) as *second.
+ * src/tree/symmetric_tree.cpp:130-262. */
+void tco_split_f32(const float* hist, const float* node_g, const float* node_h,
+                   const uint32_t* node_count, uint32_t n_nodes, uint32_t F, uint32_t max_bins,
+                   float lambda, int is_root, int32_t* feature, int32_t* bin, float* gain);
+void tco_split_f64(const double* hist, const double* node_g, const double* node_h,
+                   const uint32_t* node_count, uint32_t n_nodes, uint32_t F, uint32_t max_bins,
+                   double lambda, int is_root, int32_t* feature, int32_t* bin, double* gain,
+                   double* second);
+
+typedef struct {
+    uint32_t max_depth;
+    uint32_t max_bins;         /* 255 in every reference preset */
+    float lambda_l2;
+    float lambda_l1;
+    float max_delta_step;
+    uint32_t min_samples_leaf;
+    float min_child_weight;
+    float min_split_gain;
+} tco_tree_params;
+
+/* A9-A14: one symmetric tree on given gradients (SymmetricTree::build,
+ * src/tree/symmetric_tree.cpp:34-368; leaves :753-775).  Returns the depth.
+ * Per level d < depth: feature[d], bin[d], fthr[d] (A11), gain[d]; for the
+ * f64 variant also gap[d] = best - second best candidate gain at that level.
+ * leaves: 2^depth values.  leaf_idx (optional): per-row MSB-first index. */
+int tco_tree_build_f32(const uint8_t* bins, uint32_t n, uint32_t F, const float* g, const float* h,
+                       const float* edges, const uint32_t* n_edges, const tco_tree_params* p,
+                       uint16_t* feature, uint8_t* bin, float* fthr, float* gain,
+                       float* leaves, uint32_t* leaf_idx);
+int tco_tree_build_f64(const uint8_t* bins, uint32_t n, uint32_t F, const float* g, const float* h,
+                       const float* edges, const uint32_t* n_edges, const tco_tree_params* p,
+                       uint16_t* feature, uint8_t* bin, float* fthr, double* gain, double* gap,
+                       double* leaves, uint32_t* leaf_idx);
+
+/* A18 (builder-defined, no reference counterpart): K-channel symmetric tree,
+ * gradients row-major n x K.  gain = sum over nodes and channels; leaves are
+ * 2^depth x K.  K == 1 reduces to tco_tree_build_f64. */
+int tco_tree_build_mc_f64(const uint8_t* bins, uint32_t n, uint32_t F, uint32_t K,
+                          const double* g, const double* h, const float* edges,
+                          const uint32_t* n_edges, const tco_tree_params* p,
+                          uint16_t* feature, uint8_t* bin, float* fthr, double* gain,
+                          double* gap, double* leaves, uint32_t* leaf_idx);
+
+/* A15: binned per-tree leaf index (src/tree/symmetric_tree.cpp:777-791). */
+void tco_leaf_index_binned(const uint8_t* bins, uint32_t n, uint32_t depth,
+                           const uint16_t* feature, const uint8_t* bin, uint32_t* leaf_idx);
+
+/* A16: raw-float ensemble prediction.  Model arrays: feature/fthr [T*D],
+ * depth [T], leaves [T * 2^D], weight [T].  out = sum_t w_t * leaf + base.
+ * f32: the reference's 8-row-block order -- groups of four trees summed
+ * w0*l0 + w1*l1 + w2*l2 + w3*l3 then added, remaining trees added one by one
+ * (include/turbocat/fast_float_ensemble.hpp:388-541).  f64: double sum.
+ * leaf_idx (optional): n x T indices. */
+void tco_predict_f32(const uint16_t* feature, const float* fthr, const uint16_t* depth,
+                     const float* leaves, const float* weight, uint32_t T, uint32_t D,
+                     float base, const float* X, uint32_t n, uint32_t F, float* out,
+                     uint32_t* leaf_idx);
+void tco_predict_f64(const uint16_t* feature, const float* fthr, const uint16_t* depth,
+                     const float* leaves, const float* weight, uint32_t T, uint32_t D,
+                     double base, const float* X, uint32_t n, uint32_t F, double* out,
+                     double* abs_sum);
+
+/* A5-A15: the whole boosting loop in FP32 reference order (booster.cpp:92-465),
+ * bins already computed.  Returns the base score.  train_preds: n floats. */
+float tco_fit_f32(const uint8_t* bins, uint32_t n, uint32_t F, const float* edges,
+                  const uint32_t* n_edges, const float* y, int loss, uint32_t n_trees,
+                  float lr, const tco_tree_params* p, uint16_t* feature, uint8_t* bin,
+                  float* fthr, float* gain, uint16_t* depth, float* leaves, float* train_preds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
