@@ -1,0 +1,372 @@
+#!/usr/bin/env python
+"""Benchmark of the TurboCat symmetric-tree hot path on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload train|predict|serve|batched]
+
+A step is one boosting iteration (gradients -> per-level histograms ->
+oblivious split -> partition -> leaves -> prediction update) over the whole
+synthetic training matrix, already resident in HBM.  The default workload is
+binary LogLoss on 1,000,000 rows x 100 float32 features (80 N(0,1), 20
+U[0,1) with 5% exact duplicates per column; label ~ Bernoulli(sigmoid(w.x)),
+10 nonzero weights), 255-bin quantization, depth-6 symmetric trees, lr 0.1,
+lambda 1.  `value` is trees/s; the histogram kernel's roofline fraction and
+predict rows/s ride along, and the reference's own CPU path (compiled from
+its sources into oracle/_ref) is timed beside it on a bounded sample.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "turbo-cat_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "hist GB/s, trees/s train, rows/s predict + p50/p99 serve latency vs plain"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------------ data
+def binary_workload(n, F, seed, n_uniform=None, dup=0.05):
+    """80% N(0,1) columns, 20% U[0,1) columns with `dup` of rows overwritten by
+    another seeded row's value in the same column; y ~ Bernoulli(sigmoid(w.x))
+    with 10 nonzero N(0,1) weights."""
+    rng = np.random.default_rng(seed)
+    nu = F // 5 if n_uniform is None else n_uniform
+    X = np.empty((n, F), np.float32)
+    X[:, :F - nu] = rng.standard_normal((n, F - nu), dtype=np.float32)
+    X[:, F - nu:] = rng.random((n, nu), dtype=np.float32)
+    k = int(n * dup)
+    for c in range(F - nu, F):
+        dst = rng.choice(n, k, replace=False)
+        src = rng.integers(0, n, k)
+        X[dst, c] = X[src, c]
+    w = np.zeros(F, np.float32)
+    nz = rng.choice(F, min(10, F), replace=False)
+    w[nz] = rng.standard_normal(len(nz)).astype(np.float32)
+    z = X @ w
+    y = (rng.random(n) < 1.0 / (1.0 + np.exp(-z))).astype(np.float32)
+    return X, y
+
+
+def regression_workload(n, F, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((n, F), dtype=np.float32)
+    y = (X[:, 0] + 2 * np.sin(X[:, 1]) + 0.5 * X[:, 2] * X[:, 3]
+         + 0.1 * rng.standard_normal(n)).astype(np.float32)
+    return X, y
+
+
+# ------------------------------------------------------------------ CPU baseline
+def cpu_baseline_train(X, y, loss, depth, lr, lam, budget_s):
+    """Time the reference's own boosting loop (compiled from its sources) on
+    the same matrix for a bounded number of trees."""
+    from oracle import RefLib
+    if not RefLib.available():
+        return None
+
+    def body():
+        R = RefLib()
+        t0 = time.perf_counter()
+        ds = R.dataset(X, y)
+        q = time.perf_counter() - t0
+        # one tree to estimate the rate, then as many as fit the budget
+        _, _, s1 = ds.fit(loss, 1, depth, lr, lam, 1.0)
+        per = max(s1[1], 1e-3)
+        T = int(max(2, min(50, (budget_s - q) / per)))
+        _, _, sec = ds.fit(loss, T, depth, lr, lam, 1.0)
+        return dict(quantize_s=q, trees=T, loop_s=sec[1], build_s=sec[2])
+
+    r = RefLib.isolated(body)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=r["trees"] / r["loop_s"], unit="trees/s", cores=cores, kind="reference",
+                sample=(f"reference hot path (oracle/_ref, -O3 -mavx2 -mfma, OpenMP) on the full "
+                        f"matrix: compute_bins {r['quantize_s']:.2f} s, then {r['trees']} trees "
+                        f"in {r['loop_s']:.2f} s"),
+                quantize_s=r["quantize_s"])
+
+
+def cpu_predict_rate(model, X, budget_rows=200_000):
+    from oracle import RefLib
+    if not RefLib.available():
+        return None
+    R = RefLib()
+    Xs = X[:budget_rows]
+    t0 = time.perf_counter()
+    R.predict(model, Xs)
+    return len(Xs) / (time.perf_counter() - t0)
+
+
+# ------------------------------------------------------------------ workloads
+def run_train(args):
+    from turbocat import capi
+    n, F, T_timed = args.rows, args.features, args.steps
+    log(f"[bench] generating {n} x {F} binary workload")
+    X, y = binary_workload(n, F, seed=1001)
+    ctx = capi.Context(0)
+    t0 = time.perf_counter()
+    ds, edges, ne = ctx.quantize_fit(X)
+    quant_s = time.perf_counter() - t0
+    log(f"[bench] quantize_fit {quant_s:.2f} s; bins per feature {int(ne.min()) + 1}..{int(ne.max()) + 1}")
+    R = 5  # trees timed kernel by kernel for the roofline
+    prm = capi.train_params(capi.TC_LOSS_LOGLOSS, n_trees=args.warmup + T_timed + R,
+                            learning_rate=0.1, max_depth=6, lambda_l2=1.0,
+                            hist_chunk=args.hist_chunk)
+    tr = capi.Trainer(ctx, [ds], [y], prm)
+    tr.step(args.warmup)
+    tr.sync()
+    log(f"[bench] warmup {args.warmup} trees done; timing {T_timed}")
+    t0 = time.perf_counter()
+    tr.step(T_timed)
+    tr.sync()
+    dt = time.perf_counter() - t0
+    trees_per_s = T_timed / dt
+    log(f"[bench] {trees_per_s:.1f} trees/s ({1e3 * dt / T_timed:.3f} ms/tree)")
+    rows0 = tr.hist_rows()
+    hist_ms, launches = tr.step_timed(R)
+    rows1 = tr.hist_rows()
+    alg_bytes = (rows1 - rows0) * (F + 8)
+    achieved = alg_bytes / (hist_ms * 1e-3) / 1e9
+    log(f"[bench] histogram: {launches} launches, {hist_ms / launches * 1e3:.1f} us avg, "
+        f"{alg_bytes / launches / 1e6:.1f} MB algorithmic per launch -> {achieved:.0f} GB/s")
+    model = tr.model()
+    tr.close()
+    # ------------------------------------------------ predict on the resident rows
+    import ctypes
+    ens = ctx.ensemble(model, F)
+    dx, do = ctypes.c_void_p(), ctypes.c_void_p()
+    capi.check(ctx.L.tc_device_alloc(ctx.h, X.nbytes, ctypes.byref(dx)))
+    capi.check(ctx.L.tc_device_alloc(ctx.h, 4 * n, ctypes.byref(do)))
+    capi.check(ctx.L.tc_memcpy_h2d(ctx.h, dx, X.ctypes.data_as(ctypes.c_void_p), X.nbytes))
+    ens.predict_dev(dx, n, do)
+    ctx.sync()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ens.predict_dev(dx, n, do)
+    ctx.sync()
+    pred_kernel = reps * n / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    ens.predict(X)
+    pred_e2e = n / (time.perf_counter() - t0)
+    capi.check(ctx.L.tc_device_free(ctx.h, dx))
+    capi.check(ctx.L.tc_device_free(ctx.h, do))
+    log(f"[bench] predict {pred_kernel / 1e6:.1f} M rows/s resident, {pred_e2e / 1e6:.1f} M rows/s "
+        f"with host copies ({model['T']} trees)")
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] timing the reference CPU path (bounded sample)")
+        cpu = cpu_baseline_train(X, y, 1, 6, 0.1, 1.0, args.cpu_budget)
+        if cpu:
+            cpu["predict_rows_per_s"] = cpu_predict_rate(model, X)
+            log(f"[bench] CPU reference {cpu['value']:.3f} trees/s on {cpu['cores']} threads")
+    out = {
+        "metric": METRIC, "value": round(trees_per_s, 3), "unit": "trees/s", "n_gpus": 1,
+        "steps": T_timed, "warmup": args.warmup, "ms_per_step": round(1e3 * dt / T_timed, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"binary logloss {n} rows x {F} float32 features "
+                               f"({F - F // 5} N(0,1), {F // 5} U[0,1) with 5% duplicates), "
+                               f"255-bin uint8 quantization, symmetric trees depth 6, lr 0.1, "
+                               f"lambda 1", "rows": n, "features": F, "depth": 6,
+                   "trees_timed": T_timed},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "kernel": "k_hist", "avg_launch_us": round(hist_ms / launches * 1e3, 2),
+                     "algorithmic_bytes_per_launch": int(alg_bytes / launches)},
+        "cpu_baseline": cpu,
+        "extra": {"quantize_s": round(quant_s, 3),
+                  "predict_rows_per_s_resident": round(pred_kernel, 1),
+                  "predict_rows_per_s_with_copies": round(pred_e2e, 1),
+                  "predict_trees": model["T"]},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_serve(args):
+    """Poisson request stream against a resident 1000-tree depth-8 ensemble
+    over 200 features; coalesced server vs one launch per request."""
+    from turbocat import capi
+    F, T, D = 200, args.serve_trees, 8
+    rng = np.random.default_rng(3003)
+    log(f"[bench] training the serving ensemble ({args.serve_train_rows} rows, {T} trees depth {D})")
+    X, y = binary_workload(args.serve_train_rows, F, seed=1003, n_uniform=40)
+    ctx = capi.Context(0)
+    ds, _, _ = ctx.quantize_fit(X)
+    tr = capi.Trainer(ctx, [ds], [y], capi.train_params(capi.TC_LOSS_LOGLOSS, n_trees=T,
+                                                         learning_rate=0.1, max_depth=D,
+                                                         lambda_l2=1.0))
+    tr.step(T)
+    tr.sync()
+    model = tr.model()
+    tr.close()
+    ens = ctx.ensemble(model, F)
+    n_req = args.requests
+    sizes = np.floor(np.exp(rng.uniform(0, np.log(65536), n_req))).astype(int)
+    pool = rng.standard_normal((int(sizes.max()) + 1024, F), dtype=np.float32)
+    offs = rng.integers(0, 1024, n_req)
+    reqs = [np.ascontiguousarray(pool[o:o + s]) for o, s in zip(offs, sizes)]
+    gaps = rng.exponential(1.0 / args.qps, n_req)
+
+    def plain():
+        lat = []
+        t_start = time.perf_counter()
+        t_next = t_start
+        for x, g in zip(reqs, gaps):
+            t_next += g
+            now = time.perf_counter()
+            if now < t_next:
+                time.sleep(t_next - now)
+            t0 = max(time.perf_counter(), t_next)
+            ens.predict(x, transform=1)
+            lat.append(time.perf_counter() - t_next)
+        return np.array(lat), time.perf_counter() - t_start
+
+    def served():
+        srv = capi.Server(ens, max_batch_rows=args.max_batch_rows, memory_budget=1 << 31,
+                          n_streams=args.streams, transform=1)
+        outs = [np.empty(len(x), np.float32) for x in reqs]
+        lat = np.zeros(n_req)
+        done = threading.Event()
+        tickets = [None] * n_req
+        submit_t = np.zeros(n_req)
+        t_start = time.perf_counter()
+
+        def waiter():
+            for i in range(n_req):
+                while tickets[i] is None:
+                    time.sleep(1e-5)
+                srv.wait(tickets[i])
+                lat[i] = time.perf_counter() - submit_t[i]
+            done.set()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t_next = t_start
+        for i, (x, g) in enumerate(zip(reqs, gaps)):
+            t_next += g
+            now = time.perf_counter()
+            if now < t_next:
+                time.sleep(t_next - now)
+            submit_t[i] = t_next
+            tickets[i] = srv.submit(x, outs[i])
+        done.wait()
+        wall = time.perf_counter() - t_start
+        b, r = srv.stats()
+        srv.stop()
+        # each row alone and in the mix: identical bits
+        for i in range(0, n_req, max(1, n_req // 20)):
+            assert np.array_equal(outs[i], ens.predict(reqs[i], transform=1))
+        return lat, wall, b
+
+    rows = int(sizes.sum())
+    lat_p, wall_p = plain()
+    lat_s, wall_s, batches = served()
+    p = lambda a, q: float(np.percentile(a, q) * 1e3)
+    res = {
+        "metric": METRIC, "value": round(rows / wall_s, 1), "unit": "rows/s", "n_gpus": 1,
+        "steps": n_req, "warmup": 0, "ms_per_step": round(1e3 * wall_s / n_req, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"serving: {T} symmetric trees depth {D} over {F} features, "
+                               f"Poisson arrivals at {args.qps} QPS, request rows log-uniform "
+                               f"[1, 65536]", "requests": n_req, "qps": args.qps},
+        "extra": {"served_p50_ms": p(lat_s, 50), "served_p99_ms": p(lat_s, 99),
+                  "plain_p50_ms": p(lat_p, 50), "plain_p99_ms": p(lat_p, 99),
+                  "served_rows_per_s": rows / wall_s, "plain_rows_per_s": rows / wall_p,
+                  "batches": int(batches), "requests": n_req, "rows": rows},
+    }
+    print(json.dumps(res), flush=True)
+
+
+def run_batched(args):
+    """64 independent regression jobs (rows log-uniform in [1e4, 2e6],
+    features uniform in [16, 256]) trained in lockstep by one set of launches,
+    against the same jobs trained one after another."""
+    from turbocat import capi
+    rng = np.random.default_rng(2002)
+    J = args.jobs
+    rows = np.floor(np.exp(rng.uniform(np.log(1e4), np.log(args.max_job_rows), J))).astype(int)
+    feats = rng.integers(16, 257, J)
+    ctx = capi.Context(0)
+    dss, ys = [], []
+    for j in range(J):
+        X, y = regression_workload(int(rows[j]), int(feats[j]), seed=5000 + j)
+        ds, _, _ = ctx.quantize_fit(X)
+        dss.append(ds)
+        ys.append(y)
+        del X
+    T = args.steps
+    prm = capi.train_params(capi.TC_LOSS_MSE, n_trees=T, learning_rate=0.1, max_depth=6,
+                            lambda_l2=0.0)
+    log(f"[bench] {J} jobs, {rows.sum()} rows total; batched")
+    trb = capi.Trainer(ctx, dss, ys, prm)
+    t0 = time.perf_counter()
+    trb.step(T)
+    trb.sync()
+    t_batched = time.perf_counter() - t0
+    trb.close()
+    log(f"[bench] batched {J * T / t_batched:.1f} trees/s; now one job at a time")
+    t_plain = 0.0
+    for j in range(J):
+        trj = capi.Trainer(ctx, [dss[j]], [ys[j]], prm)
+        t0 = time.perf_counter()
+        trj.step(T)
+        trj.sync()
+        t_plain += time.perf_counter() - t0
+        trj.close()
+    res = {
+        "metric": METRIC, "value": round(J * T / t_batched, 2), "unit": "trees/s", "n_gpus": 1,
+        "steps": T, "warmup": 0, "ms_per_step": round(1e3 * t_batched / T, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{J} regression jobs, rows log-uniform [1e4, {args.max_job_rows}], "
+                               f"features uniform [16, 256], depth 6, {T} trees each",
+                   "jobs": J, "total_rows": int(rows.sum())},
+        "extra": {"batched_trees_per_s": J * T / t_batched, "plain_trees_per_s": J * T / t_plain,
+                  "speedup_vs_plain": t_plain / t_batched},
+    }
+    print(json.dumps(res), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="train", choices=["train", "serve", "batched"])
+    ap.add_argument("--rows", type=int, default=1_000_000)
+    ap.add_argument("--features", type=int, default=100)
+    ap.add_argument("--hist-chunk", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--qps", type=float, default=200.0)
+    ap.add_argument("--requests", type=int, default=400)
+    ap.add_argument("--serve-trees", type=int, default=1000)
+    ap.add_argument("--serve-train-rows", type=int, default=500_000)
+    ap.add_argument("--max-batch-rows", type=int, default=131072)
+    ap.add_argument("--streams", type=int, default=2)
+    ap.add_argument("--jobs", type=int, default=64)
+    ap.add_argument("--max-job-rows", type=int, default=2_000_000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU "
+                                   "sharding is described in DESIGN.md, not implemented)"}))
+        sys.exit(2)
+    {"train": run_train, "serve": run_serve, "batched": run_batched}[args.workload](args)
+
+
+if __name__ == "__main__":
+    main()

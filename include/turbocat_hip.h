@@ -1,0 +1,227 @@
+/* turbocat_hip.h -- C-ABI of the MI355X (gfx950) TurboCat symmetric-tree hot
+ * path: quantization, gradients, histograms, oblivious split scoring, tree
+ * build, boosting loop, ensemble prediction, continuous-batching serving and
+ * batched multi-job training.
+ *
+ * Conventions
+ *   - Every function returns int32_t: TC_OK (0) or a negative TC_E* code;
+ *     tc_last_error() gives the message of the calling thread's last error.
+ *   - Host pointers are borrowed for the duration of the call; functions whose
+ *     name ends in _dev take device pointers (memory from tc_device_alloc or
+ *     any hipMalloc) and run asynchronously on the context's stream.
+ *   - Device memory is owned by handles and released by their *_destroy.
+ *   - One host thread at a time per context (the reference Booster is not
+ *     re-entrant either); the server runs its own thread and streams.
+ *   - Layouts: X row-major n x F float32; bins column-major F x n uint8;
+ *     edges 255 float slots per feature (n_edges[f] valid); histograms
+ *     F x 256 x {grad, hess} float32 with slot 255 holding the NaN bin.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   Dataset::compute_bins / apply_bins     include/turbocat/dataset.hpp:134,137
+ *   Loss::compute_gradients               include/turbocat/loss.hpp:57-63
+ *   HistogramBuilder::build / create      include/turbocat/histogram.hpp:72-119
+ *   Histogram::subtract_from              src/tree/histogram.cpp:39-42
+ *   SymmetricTree::build (split scan)     include/turbocat/symmetric_tree.hpp:49-53
+ *   Booster::train (symmetric branch)     src/boosting/booster.cpp:47-418
+ *   SymmetricEnsemble::predict_batch_raw_fast  include/turbocat/symmetric_tree.hpp:177
+ *   SymmetricEnsemble::predict_batch (binned)  include/turbocat/symmetric_tree.hpp:171
+ */
+#ifndef TURBOCAT_HIP_H
+#define TURBOCAT_HIP_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TC_OK 0
+#define TC_EINVAL (-1)   /* bad argument (maps to std::invalid_argument) */
+#define TC_ENOMEM (-2)   /* device or host allocation failed */
+#define TC_EDEVICE (-3)  /* HIP runtime error or no usable gfx950 device */
+#define TC_ESTATE (-4)   /* call not valid in the object's state (e.g. not fitted) */
+#define TC_EBUSY (-5)    /* serving: admission refused, retry later */
+#define TC_ETIMEOUT (-6) /* serving: wait timed out */
+
+#define TC_LOSS_MSE 0
+#define TC_LOSS_LOGLOSS 1
+
+typedef struct tc_ctx tc_ctx;
+typedef struct tc_dataset tc_dataset;
+typedef struct tc_ensemble tc_ensemble;
+typedef struct tc_server tc_server;
+
+/* Tree hyper-parameters (TreeConfig, include/turbocat/config.hpp:26-59). */
+typedef struct {
+  uint32_t max_depth;        /* 1..10 */
+  uint32_t max_bins;         /* histogram stride / split candidates b < max_bins-1; 255 */
+  float lambda_l2;
+  float lambda_l1;
+  float max_delta_step;
+  uint32_t min_samples_leaf;
+  float min_child_weight;
+  float min_split_gain;
+} tc_tree_params;
+
+/* Boosting parameters (BoostingConfig, config.hpp:66-96; symmetric trees,
+ * no row/column sampling). */
+typedef struct {
+  int32_t loss;              /* TC_LOSS_MSE or TC_LOSS_LOGLOSS */
+  uint32_t n_trees;
+  float learning_rate;
+  tc_tree_params tree;
+  uint32_t hist_chunk;       /* rows per histogram work item; 0 = autotuned */
+  int32_t use_graph;         /* 1: replay one captured hipGraph per tree */
+} tc_train_params;
+
+/* Host-side model arrays filled by training (caller-allocated).
+ * feature/bin/fthr/gain: n_trees * max_depth; depth: n_trees;
+ * leaves: n_trees << max_depth; train_preds: n rows (optional). */
+typedef struct {
+  uint16_t* feature;
+  uint8_t* bin;
+  float* fthr;
+  float* gain;
+  uint16_t* depth;
+  float* leaves;
+  float* train_preds;
+  float base;                /* out: base score (A5) */
+} tc_model;
+
+/* ---------------------------------------------------------------- context */
+int32_t tc_ctx_create(int32_t device_id, tc_ctx** out);
+int32_t tc_ctx_destroy(tc_ctx* ctx);
+int32_t tc_ctx_synchronize(tc_ctx* ctx);
+const char* tc_last_error(void);
+/* Device memory plumbing for the _dev entry points. */
+int32_t tc_device_alloc(tc_ctx* ctx, size_t bytes, void** out);
+int32_t tc_device_free(tc_ctx* ctx, void* p);
+int32_t tc_memcpy_h2d(tc_ctx* ctx, void* dst, const void* src, size_t bytes);
+int32_t tc_memcpy_d2h(tc_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* ---------------------------------------------------------- quantization */
+/* A2: Dataset::compute_bins -> compute_quantile_bins (src/core/dataset.cpp:
+ * 176-334).  Computes per-feature edges (wrap != 0 reproduces the reference's
+ * uint8 wrap of the unique count, dataset.cpp:308) and bins every row on the
+ * GPU.  Returns a device-resident dataset; edges/n_edges copies optional. */
+int32_t tc_quantize_fit(tc_ctx* ctx, const float* X, uint32_t n, uint32_t F, uint32_t max_bins,
+                        int32_t wrap, float* edges_out, uint32_t* n_edges_out, tc_dataset** out);
+/* A3: Dataset::apply_bins (dataset.cpp:346-379): bin X with given edges. */
+int32_t tc_dataset_create(tc_ctx* ctx, const float* X, uint32_t n, uint32_t F,
+                          const float* edges, const uint32_t* n_edges, uint32_t max_bins,
+                          tc_dataset** out);
+int32_t tc_dataset_bins(tc_dataset* ds, uint8_t* bins_out /* F x n column-major */);
+int32_t tc_dataset_shape(tc_dataset* ds, uint32_t* n, uint32_t* F);
+int32_t tc_dataset_destroy(tc_dataset* ds);
+
+/* ------------------------------------------------------------- gradients */
+/* A6/A7: Loss::compute_gradients (loss.hpp:57-63; loss.cpp:123-211, :348-362). */
+int32_t tc_gradients(tc_ctx* ctx, int32_t loss, const float* y, const float* preds, float* g,
+                     float* h, uint32_t n);
+/* A5: Loss::init_prediction (loss.cpp:217-226, :364-370). */
+int32_t tc_init_prediction(int32_t loss, const float* y, uint32_t n, float* base);
+
+/* ------------------------------------------------------------ histograms */
+/* A9: HistogramBuilder::build(dataset, rows, {}, out) (histogram.hpp:78-83;
+ * histogram.cpp:62-205).  rows == NULL: all rows.  hist_out: F x 256 x 2. */
+int32_t tc_hist_build(tc_dataset* ds, const float* g, const float* h, const uint32_t* rows,
+                      uint32_t n_rows, float* hist_out);
+/* A13: Histogram::subtract_from(parent, other) (histogram.cpp:39-42). */
+int32_t tc_hist_subtract(tc_ctx* ctx, const float* parent, const float* child, float* out,
+                         uint32_t F);
+
+/* --------------------------------------------------------- split scoring */
+/* A10: the level split scan inlined in SymmetricTree::build
+ * (symmetric_tree.cpp:130-262) over n_nodes node histograms (n_nodes x F x
+ * 256 x 2) and node totals.  is_root selects the d == 0 rules.  Ties go to
+ * the lowest (feature, bin); feature = -1 when no candidate is valid. */
+int32_t tc_split_scan(tc_ctx* ctx, const float* hist, const float* node_g, const float* node_h,
+                      const uint32_t* node_count, uint32_t n_nodes, uint32_t F,
+                      uint32_t max_bins, float lambda_l2, int32_t is_root, int32_t* feature,
+                      int32_t* bin, float* gain);
+
+/* ------------------------------------------------------------ tree build */
+/* A9-A14: SymmetricTree::build on given gradients (symmetric_tree.hpp:49-53).
+ * Outputs per level d < *depth: feature, bin, fthr, gain; leaves 2^depth;
+ * leaf_idx (optional) per row, MSB-first. */
+int32_t tc_tree_build(tc_dataset* ds, const float* g, const float* h, const tc_tree_params* p,
+                      uint16_t* feature, uint8_t* bin, float* fthr, float* gain, float* leaves,
+                      uint32_t* leaf_idx, int32_t* depth);
+
+/* ------------------------------------------------------------- boosting */
+/* A5-A15: Booster::train, symmetric branch without sampling
+ * (booster.cpp:47-418, build_tree :438-465). */
+int32_t tc_train(tc_dataset* ds, const float* y, const tc_train_params* p, tc_model* model);
+/* Batched form: n_jobs independent fits in lockstep, one launch per kernel
+ * for all jobs.  All jobs share max_depth and n_trees; y[j], models[j] per
+ * job.  No reference counterpart (the reference fits one model per call). */
+int32_t tc_train_batch(tc_ctx* ctx, uint32_t n_jobs, tc_dataset* const* ds,
+                       const float* const* y, const tc_train_params* p, tc_model* models);
+/* Timing hook for benchmarks: time (ms, device events) of the boosting loop
+ * of the last tc_train / tc_train_batch on this context, and of its
+ * histogram kernels. */
+int32_t tc_last_train_timing(tc_ctx* ctx, double* loop_ms, double* hist_ms);
+
+/* Stepwise form of tc_train / tc_train_batch for callers that drive the
+ * boosting loop themselves (benchmarks, schedulers): create once, then each
+ * tc_trainer_step enqueues n more trees for every job (asynchronously, one
+ * captured hipGraph replay per tree); tc_trainer_sync waits.  hist_ms /
+ * hist_launches of tc_trainer_step_timed report HIP-event time of every
+ * histogram kernel launch of those trees (graph not used).  No reference
+ * counterpart: the reference's Booster::train loop (booster.cpp:160-408)
+ * runs to completion in one call. */
+typedef struct tc_trainer tc_trainer;
+int32_t tc_trainer_create(tc_ctx* ctx, uint32_t n_jobs, tc_dataset* const* ds,
+                          const float* const* y, const tc_train_params* p, tc_trainer** out);
+int32_t tc_trainer_step(tc_trainer* tr, uint32_t n_trees);
+int32_t tc_trainer_step_timed(tc_trainer* tr, uint32_t n_trees, double* hist_ms,
+                              uint32_t* hist_launches);
+int32_t tc_trainer_sync(tc_trainer* tr);
+/* Model of job j so far (trees beyond the ones built are zero). */
+int32_t tc_trainer_model(tc_trainer* tr, uint32_t job, tc_model* model);
+/* Rows fed to histogram kernels for job j since creation (for algorithmic
+ * bytes: rows x (F + 8)). */
+int32_t tc_trainer_hist_rows(tc_trainer* tr, uint32_t job, uint64_t* rows);
+int32_t tc_trainer_destroy(tc_trainer* tr);
+
+/* ------------------------------------------------------------- inference */
+/* A16/A17 model upload: FastFloatEnsemble::from_symmetric_ensemble
+ * (fast_float_ensemble.cpp:23-68).  Arrays as tc_model; weights per tree. */
+int32_t tc_ensemble_create(tc_ctx* ctx, uint32_t n_trees, uint32_t max_depth,
+                           const uint16_t* feature, const float* fthr, const uint8_t* bin,
+                           const uint16_t* depth, const float* leaves, const float* weights,
+                           float base, uint32_t n_features, tc_ensemble** out);
+int32_t tc_ensemble_destroy(tc_ensemble* ens);
+/* A16: predict_batch_raw_fast + base (booster.cpp:899-925).  transform: 0 raw,
+ * 1 sigmoid (LogLoss::transform_prediction, loss.cpp:213-215).  leaf_idx
+ * (optional) n x T. */
+int32_t tc_predict(tc_ensemble* ens, const float* X, uint32_t n, uint32_t F, int32_t transform,
+                   float* out, uint32_t* leaf_idx);
+int32_t tc_predict_dev(tc_ensemble* ens, const float* X_dev, uint32_t n, uint32_t F,
+                       int32_t transform, float* out_dev);
+/* A17: binned prediction (SymmetricEnsemble::predict_batch, symmetric_tree.cpp:
+ * 934-962) over a device dataset's bins. */
+int32_t tc_predict_binned(tc_ensemble* ens, tc_dataset* ds, float* out);
+
+/* --------------------------------------------------------------- serving */
+/* Continuous-batching server over a resident ensemble.  Requests are admitted
+ * while their rows fit the memory budget and a batch slot is free; arrivals
+ * during a launch join the next coalesced batch; up to n_streams batches run
+ * on the device at once.  No reference counterpart. */
+typedef struct {
+  uint32_t max_batch_rows;   /* rows per coalesced launch */
+  uint64_t memory_budget;    /* bytes of request data admitted at once */
+  uint32_t n_streams;        /* batches in flight */
+  int32_t transform;         /* 0 raw, 1 sigmoid */
+} tc_serve_params;
+int32_t tc_serve_start(tc_ensemble* ens, const tc_serve_params* p, tc_server** out);
+/* Submit n rows; out must stay valid until tc_serve_wait returns for ticket. */
+int32_t tc_serve_submit(tc_server* s, const float* X, uint32_t n, float* out, uint64_t* ticket);
+int32_t tc_serve_wait(tc_server* s, uint64_t ticket, int64_t timeout_us);
+int32_t tc_serve_stats(tc_server* s, uint64_t* batches, uint64_t* rows);
+int32_t tc_serve_stop(tc_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

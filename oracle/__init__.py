@@ -88,6 +88,7 @@ class Oracle:
         L.tco_fit_f32.argtypes = [V, c_uint32, c_uint32, V, V, V, c_int, c_uint32, c_float,
                                   POINTER(TreeParams), V, V, V, V, V, V, V]
         L.tco_fit_f32.restype = c_float
+        L.tco_fma_update.argtypes = [V, V, c_float, c_uint32]
         for fn in ("tco_tree_build_f32", "tco_tree_build_f64", "tco_tree_build_mc_f64"):
             getattr(L, fn).restype = c_int
 
@@ -241,6 +242,12 @@ class Oracle:
                                model["base"], _p(X), n, F, _p(out), _p(ab))
         return out, ab
 
+    def fma_update(self, preds, lr, vals):
+        out = np.array(preds, np.float32, copy=True)
+        v = np.ascontiguousarray(vals, np.float32)
+        self.L.tco_fma_update(_p(out), _p(v), lr, len(out))
+        return out
+
     # ---------------------------------------------------------------- fit
     def fit_f32(self, bins, edges, n_edges, y, loss, n_trees, lr, params: TreeParams):
         F, n = bins.shape
@@ -289,6 +296,29 @@ class RefLib:
     @staticmethod
     def available(path: str = LIB_REF) -> bool:
         return os.path.exists(path)
+
+    @staticmethod
+    def isolated(fn, *args, **kw):
+        """Run fn in a fresh OS thread.  SymmetricTree::build keeps static
+        thread_local histogram pools that are regrown only when 2^max_depth
+        grows, not when n_features grows (symmetric_tree.cpp:57-74), so a
+        second dataset with more features on the same thread writes past the
+        pools.  A fresh thread gets fresh pools."""
+        import threading
+        box = {}
+
+        def run():
+            try:
+                box["r"] = fn(*args, **kw)
+            except BaseException as e:  # noqa: BLE001
+                box["e"] = e
+
+        t = threading.Thread(target=run)
+        t.start()
+        t.join()
+        if "e" in box:
+            raise box["e"]
+        return box["r"]
 
     def dataset(self, X, y=None, max_bins=255):
         X = np.ascontiguousarray(X, np.float32)

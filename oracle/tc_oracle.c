@@ -356,10 +356,9 @@ void tco_predict_f32(const uint16_t* feature, const float* fthr, const uint16_t*
         for (uint32_t t = TB * 4; t < T; ++t) {
             const uint32_t l = float_leaf(row, feature + (size_t)t * D, fthr + (size_t)t * D, depth[t]);
             if (leaf_idx) leaf_idx[(size_t)i * T + t] = l;
-            /* rounded product, then a separate add: the reference stores the
-             * products to memory before the vector add, so nothing fuses */
-            volatile float v = weight[t] * leaves[(size_t)t * LP + l];
-            sum = sum + v;
+            /* leftover trees: the reference compiler forwards the stored
+             * product into the vector add and fuses the two */
+            sum = fmaf(weight[t], leaves[(size_t)t * LP + l], sum);
         }
         out[i] = sum + base;
     }
@@ -385,6 +384,10 @@ void tco_predict_f64(const uint16_t* feature, const float* fthr, const uint16_t*
 }
 
 /* ---------------------------------------------------------------- A5 - A15 */
+
+void tco_fma_update(float* preds, const float* vals, float lr, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) preds[i] = fmaf(lr, vals[i], preds[i]);
+}
 
 float tco_fit_f32(const uint8_t* bins, uint32_t n, uint32_t F, const float* edges,
                   const uint32_t* n_edges, const float* y, int loss, uint32_t n_trees,

@@ -110,7 +110,7 @@ void tco_leaf_index_binned(const uint8_t* bins, uint32_t n, uint32_t depth,
 /* A16: raw-float ensemble prediction.  Model arrays: feature/fthr [T*D],
  * depth [T], leaves [T * 2^D], weight [T].  out = sum_t w_t * leaf + base.
  * f32: the reference's 8-row-block order -- groups of four trees summed
- * w0*l0 + w1*l1 + w2*l2 + w3*l3 then added, remaining trees added one by one
+ * w0*l0 + w1*l1 + w2*l2 + w3*l3 then added, remaining trees fused in one by one
  * (include/turbocat/fast_float_ensemble.hpp:388-541).  f64: double sum.
  * leaf_idx (optional): n x T indices. */
 void tco_predict_f32(const uint16_t* feature, const float* fthr, const uint16_t* depth,
@@ -121,6 +121,10 @@ void tco_predict_f64(const uint16_t* feature, const float* fthr, const uint16_t*
                      const float* leaves, const float* weight, uint32_t T, uint32_t D,
                      double base, const float* X, uint32_t n, uint32_t F, double* out,
                      double* abs_sum);
+
+/* preds[i] = fma(lr, vals[i], preds[i]): the training-prediction update as
+ * the reference compiler contracts it (booster.cpp:460-463). */
+void tco_fma_update(float* preds, const float* vals, float lr, uint32_t n);
 
 /* A5-A15: the whole boosting loop in FP32 reference order (booster.cpp:92-465),
  * bins already computed.  Returns the base score.  train_preds: n floats. */

@@ -1,0 +1,58 @@
+"""Write the golden vectors under tests/golden/ from the reference's own hot
+path, compiled from its sources by oracle/Makefile (oracle/_ref/libtcref.so).
+
+Run in a container that has the reference tree:
+    make -C oracle && python tests/golden/make_golden.py
+
+Each fixture is data only: seeded inputs (regenerated from the seed, so only
+the seed and shape are stored), the reference's outputs, as .npz.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from conftest import binary_data, regression_data  # noqa: E402
+from oracle import LOSS_LOGLOSS, LOSS_MSE, RefLib  # noqa: E402
+
+CASES = [
+    # name, kind, n, F, seed, loss, n_trees, depth, lr, lambda
+    ("reg_small", "reg", 4000, 8, 101, LOSS_MSE, 12, 6, 0.1, 0.0),
+    ("bin_small", "bin", 6000, 10, 102, LOSS_LOGLOSS, 12, 6, 0.1, 1.0),
+    ("reg_deep", "reg", 3000, 5, 103, LOSS_MSE, 6, 8, 0.2, 3.0),
+]
+
+
+def make(name, kind, n, F, seed, loss, T, D, lr, lam):
+    X, y = regression_data(n, F, seed, low_card_col=2) if kind == "reg" else binary_data(n, F, seed)
+    R = RefLib()
+    ds = R.dataset(X, y)
+    bins, edges, ne = ds.bins()
+    rng = np.random.default_rng(seed + 7)
+    p = (rng.standard_normal(n) * 2).astype(np.float32)
+    g, h = R.gradients(loss, y, p)
+    t1 = ds.tree(g, h, max_depth=D, lambda_l2=lam, min_child_weight=1.0)
+    m, tp, _ = ds.fit(loss, T, D, lr, lam, 1.0)
+    Xp = rng.standard_normal((512, F)).astype(np.float32)
+    pred = R.predict(m, Xp)
+    np.savez_compressed(
+        os.path.join(HERE, f"{name}.npz"),
+        meta=np.array([n, F, seed, loss, T, D], np.int64), kind=np.array(kind),
+        lr=np.float32(lr), lam=np.float32(lam),
+        bins=bins, edges=edges, n_edges=ne, p=p, g=g, h=h,
+        t_feature=t1["feature"], t_bin=t1["bin"], t_fthr=t1["fthr"], t_gain=t1["gain"],
+        t_leaves=t1["leaves"], t_leaf_idx=t1["leaf_idx"],
+        m_feature=m["feature"], m_bin=m["bin"], m_fthr=m["fthr"], m_gain=m["gain"],
+        m_depth=m["depth"], m_leaves=m["leaves"], m_base=np.float32(m["base"]),
+        train_preds=tp, Xp=Xp, pred=pred)
+
+
+if __name__ == "__main__":
+    for c in CASES:
+        RefLib.isolated(make, *c)  # fresh thread: see RefLib.isolated
+        print("wrote", c[0])

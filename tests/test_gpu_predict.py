@@ -1,0 +1,110 @@
+"""GPU ensemble inference (A15-A17) and the continuous-batching server.
+
+Bit-exact claims: per-row, per-tree leaf indices; raw scores equal the
+oracle's FP32 restatement of the reference's summation order bit for bit;
+every row's result is identical alone and inside a coalesced batch.  Float
+claim: raw scores within 1e-5 of the FP64 sum, normalized by sum |w * leaf|."""
+import threading
+
+import numpy as np
+import pytest
+
+from conftest import regression_data, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def random_model(T, D, F, seed, shallow_every=7):
+    rng = np.random.default_rng(seed)
+    depth = np.full(T, D, np.uint16)
+    depth[::shallow_every] = rng.integers(0, D, len(depth[::shallow_every]))
+    feat = rng.integers(0, F, T * D).astype(np.uint16)
+    thr = rng.standard_normal(T * D).astype(np.float32)
+    leaves = (rng.standard_normal(T << D) * 0.1).astype(np.float32)
+    for t in range(T):   # unused slots stay zero, as the trainer leaves them
+        leaves[(t << D) + (1 << int(depth[t])):(t + 1) << D] = 0
+        feat[t * D + int(depth[t]):(t + 1) * D] = 0
+        thr[t * D + int(depth[t]):(t + 1) * D] = 0
+    return dict(T=T, D=D, feature=feat, fthr=thr, bin=np.zeros(T * D, np.uint8), depth=depth,
+                leaves=leaves, weight=np.full(T, 0.1, np.float32), base=0.25)
+
+
+@pytest.mark.parametrize("T,D,F,n", [(1, 1, 3, 5), (7, 4, 11, 1000), (100, 6, 20, 50000),
+                                     (1003, 8, 200, 3001)])
+def test_predict_bitexact(ctx, oracle, T, D, F, n):
+    m = random_model(T, D, F, seed=T + n)
+    X, _ = regression_data(n, F, seed=T)
+    X[::53, F // 2] = np.nan
+    ens = ctx.ensemble(m, F)
+    out, li = ens.predict(X, want_leaf_idx=True)
+    ref, rli = oracle.predict_f32(m, X, want_leaf_idx=True)
+    assert np.array_equal(li, rli)
+    assert np.array_equal(out, ref)
+    r64, ab = oracle.predict_f64(m, X)
+    assert rel_err(out, r64, ab).max() < 1e-5
+    p = ens.predict(X, transform=1)
+    assert np.allclose(p, 1.0 / (1.0 + np.exp(-r64)), rtol=1e-6, atol=1e-7)
+
+
+def test_predict_binned_matches_float(ctx, oracle):
+    """A.3: with thresholds taken from the training edges, bin > t <=> v >= e[t],
+    so binned and raw-float predictions agree for non-NaN rows."""
+    from turbocat.capi import train_params
+    X, y = regression_data(20000, 9, 5)
+    ds, edges, ne = ctx.quantize_fit(X)
+    m = ds.train(y, train_params(0, n_trees=30, learning_rate=0.1, max_depth=6, lambda_l2=0.0))
+    ens = ctx.ensemble(m, 9)
+    a = ens.predict(X)
+    b = ens.predict_binned(ds)
+    assert np.array_equal(a, b)
+
+
+def test_predict_device_pointers(ctx, oracle):
+    import ctypes
+    from turbocat.capi import check
+    m = random_model(50, 6, 16, seed=3)
+    X, _ = regression_data(4096, 16, seed=3)
+    ens = ctx.ensemble(m, 16)
+    L = ctx.L
+    dx, do = ctypes.c_void_p(), ctypes.c_void_p()
+    check(L.tc_device_alloc(ctx.h, X.nbytes, ctypes.byref(dx)))
+    check(L.tc_device_alloc(ctx.h, 4 * 4096, ctypes.byref(do)))
+    check(L.tc_memcpy_h2d(ctx.h, dx, X.ctypes.data_as(ctypes.c_void_p), X.nbytes))
+    ens.predict_dev(dx, 4096, do)
+    out = np.zeros(4096, np.float32)
+    check(L.tc_memcpy_d2h(ctx.h, out.ctypes.data_as(ctypes.c_void_p), do, 4 * 4096))
+    check(L.tc_device_free(ctx.h, dx)); check(L.tc_device_free(ctx.h, do))
+    assert np.array_equal(out, oracle.predict_f32(m, X))
+
+
+def test_serving_alone_and_in_the_mix(ctx, oracle):
+    """Requests of uneven sizes submitted from several threads are coalesced
+    into shared batches (the batch count is below the request count); each
+    result is bitwise the same as the request predicted alone."""
+    from turbocat.capi import Server
+    F = 24
+    m = random_model(64, 6, F, seed=9)
+    ens = ctx.ensemble(m, F)
+    rng = np.random.default_rng(10)
+    sizes = [int(s) for s in np.exp(rng.uniform(0, np.log(3000), 120))]
+    reqs = [regression_data(s, F, seed=100 + i)[0] for i, s in enumerate(sizes)]
+    alone = [ens.predict(x) for x in reqs]
+    srv = Server(ens, max_batch_rows=2048, memory_budget=4 << 20, n_streams=3)
+    outs = [np.zeros(len(x), np.float32) for x in reqs]
+    tickets = [None] * len(reqs)
+
+    def client(k):
+        for i in range(k, len(reqs), 4):
+            tickets[i] = srv.submit(reqs[i], outs[i])
+
+    th = [threading.Thread(target=client, args=(k,)) for k in range(4)]
+    [t.start() for t in th]
+    [t.join() for t in th]
+    for t in tickets:
+        assert srv.wait(t, timeout_us=60_000_000)
+    batches, rows = srv.stats()
+    srv.stop()
+    assert rows == sum(sizes)
+    assert batches < len(reqs)
+    for a, b in zip(outs, alone):
+        assert np.array_equal(a, b)

@@ -1,0 +1,172 @@
+"""GPU symmetric-tree build (A9-A14) and boosting loop (A5-A15) against the
+CPU oracle.
+
+Bit-exact claims: chosen (feature, bin) per level, tree depth, per-row leaf
+indices, and -- given the GPU's own trees -- the training predictions.
+Float claims: leaf values within 1e-5 of the FP64 oracle, normalized by
+sum |g| / (H + lambda) of the leaf.  Near-tie rule: where the FP64 gap
+between the best and second-best candidate of a level is below
+NEAR_TIE * |best gain|, either choice is accepted and the rest of that tree
+is not compared (the two builds then legitimately diverge); such events are
+counted and must stay rare."""
+import numpy as np
+import pytest
+
+from conftest import binary_data, regression_data, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+NEAR_TIE = 1e-5
+
+
+def compare_tree(gpu, bins, g, h, edges, ne, params, oracle):
+    """Compare one GPU tree with the FP64 oracle on the same gradients.
+    Returns (levels compared, near_tie_hit)."""
+    ref = oracle.tree_f64(bins, g, h, edges, ne, params)
+    lam = params.lambda_l2
+    for d in range(max(gpu["depth"], ref["depth"])):
+        if d >= gpu["depth"] or d >= ref["depth"]:
+            assert False, f"depth differs at level {d}: gpu {gpu['depth']} ref {ref['depth']}"
+        same = (gpu["feature"][d] == ref["feature"][d]) and (gpu["bin"][d] == ref["bin"][d])
+        if not same:
+            assert ref["gap"][d] <= NEAR_TIE * max(1.0, abs(ref["gain"][d])), (
+                f"level {d}: gpu ({gpu['feature'][d]},{gpu['bin'][d]}) ref "
+                f"({ref['feature'][d]},{ref['bin'][d]}) gap {ref['gap'][d]} gain {ref['gain'][d]}")
+            return d, True
+        assert gpu["fthr"][d] == ref["fthr"][d]
+        assert abs(gpu["gain"][d] - ref["gain"][d]) <= 1e-4 * max(1.0, abs(ref["gain"][d]))
+    assert gpu["depth"] == ref["depth"]
+    assert np.array_equal(gpu["leaf_idx"], ref["leaf_idx"])
+    L = 1 << ref["depth"]
+    absg = np.bincount(ref["leaf_idx"], weights=np.abs(g.astype(np.float64)), minlength=L)
+    hs = np.bincount(ref["leaf_idx"], weights=h.astype(np.float64), minlength=L)
+    scale = absg / (hs + lam + 1e-30)
+    assert rel_err(gpu["leaves"], ref["leaves"], scale).max() < TOL
+    return ref["depth"], False
+
+
+def test_kat_eight_rows(ctx, oracle):
+    """Known-answer test: the reference's 8-row toy (tests/test_turbocat.py:51-57)
+    with one depth-3 tree, lr 1, lambda 1 -- splits (0,3) at every level with
+    gains 2, 0, 0 (zero-gain splits applied), leaves [-1, 0 x 6, +1]."""
+    from turbocat.capi import train_params, TC_LOSS_LOGLOSS
+    X = np.array([-2, -1, 1, 2, -1.5, -0.5, 0.5, 1.5], np.float32).reshape(8, 1)
+    y = np.array([0, 0, 1, 1, 0, 0, 1, 1], np.float32)
+    ds, edges, ne = ctx.quantize_fit(X)
+    assert list(ds.bins()[0]) == [0, 2, 5, 7, 1, 3, 4, 6]
+    m = ds.train(y, train_params(TC_LOSS_LOGLOSS, n_trees=1, learning_rate=1.0, max_depth=3,
+                                 lambda_l2=1.0, min_child_weight=0.01))
+    assert m["base"] == 0.0
+    assert list(m["depth"]) == [3]
+    assert list(m["feature"]) == [0, 0, 0] and list(m["bin"]) == [3, 3, 3]
+    assert list(m["gain"]) == [2.0, 0.0, 0.0]
+    assert list(m["fthr"]) == [0.5, 0.5, 0.5]
+    assert list(m["leaves"]) == [-1, 0, 0, 0, 0, 0, 0, 1]
+    ens = ctx.ensemble(m, 1)
+    p = ens.predict(X, transform=1)
+    assert np.allclose(p, np.where(X[:, 0] >= 0.5, 0.7310586, 0.26894143), atol=1e-6)
+
+
+@pytest.mark.parametrize("n,F,depth,lam,seed", [
+    (5000, 6, 4, 1.0, 1),
+    (50000, 20, 6, 0.0, 2),
+    (30011, 33, 6, 1.0, 3),
+    (20000, 10, 8, 1.0, 4),
+])
+def test_tree_teacher_forced(ctx, oracle, n, F, depth, lam, seed):
+    from turbocat.capi import tree_params
+    X, y = regression_data(n, F, seed, low_card_col=3)
+    ds, edges, ne = ctx.quantize_fit(X)
+    bins = ds.bins()
+    rng = np.random.default_rng(seed + 100)
+    g = (rng.standard_normal(n) + 0.3 * X[:, 0] - np.sin(X[:, 1])).astype(np.float32)
+    h = (rng.random(n) * 0.2 + 0.05).astype(np.float32)
+    prm = tree_params(max_depth=depth, lambda_l2=lam)
+    from oracle import tree_params as otp
+    oprm = otp(max_depth=depth, lambda_l2=lam)
+    gpu = ds.tree(g, h, prm)
+    lv, tie = compare_tree(gpu, bins, g, h, edges, ne, oprm, oracle)
+    assert not tie
+
+
+def test_tree_stops_early(ctx, oracle):
+    """min_child_weight large enough that the level stop test fires."""
+    from turbocat.capi import tree_params
+    from oracle import tree_params as otp
+    X, y = regression_data(3000, 5, 9)
+    ds, edges, ne = ctx.quantize_fit(X)
+    g = (X[:, 0] + 0.1).astype(np.float32)
+    h = np.full(3000, 1.0, np.float32)
+    for mcw in (400.0, 1000.0, 2000.0):
+        gpu = ds.tree(g, h, tree_params(max_depth=6, min_child_weight=mcw))
+        ref = oracle.tree_f64(ds.bins(), g, h, edges, ne, otp(max_depth=6, min_child_weight=mcw))
+        assert gpu["depth"] == ref["depth"] < 6
+        assert np.array_equal(gpu["leaf_idx"], ref["leaf_idx"])
+
+
+def fma_update(oracle, preds, lr, leaf):
+    """preds += lr * leaf as one fused multiply-add per row (the reference
+    compiler's contraction of booster.cpp:460-463), computed by the oracle."""
+    return oracle.fma_update(preds, lr, leaf)
+
+
+@pytest.mark.parametrize("loss,lam", [(0, 0.0), (1, 1.0)])
+def test_fit_teacher_forced(ctx, oracle, loss, lam):
+    from turbocat.capi import train_params
+    n, F, T = 20000, 12, 25
+    X, y = (regression_data(n, F, 31) if loss == 0 else binary_data(n, F, 31))
+    ds, edges, ne = ctx.quantize_fit(X)
+    bins = ds.bins()
+    m = ds.train(y, train_params(loss, n_trees=T, learning_rate=0.1, max_depth=6, lambda_l2=lam))
+    assert m["base"] == oracle.init_prediction(loss, y)
+    from oracle import tree_params as otp
+    preds = np.full(n, np.float32(m["base"]), np.float32)
+    ties = 0
+    D = 6
+    for t in range(T):
+        g, h = oracle.gradients_f32(loss, y, preds)
+        d = int(m["depth"][t])
+        gpu = dict(depth=d, feature=m["feature"][t * D:t * D + d], bin=m["bin"][t * D:t * D + d],
+                   fthr=m["fthr"][t * D:t * D + d], gain=m["gain"][t * D:t * D + d],
+                   leaves=m["leaves"][t << D:(t << D) + (1 << d)])
+        gpu["leaf_idx"] = oracle.leaf_index_binned(bins, gpu["feature"], gpu["bin"])
+        _, tie = compare_tree(gpu, bins, g, h, edges, ne, otp(max_depth=D, lambda_l2=lam), oracle)
+        ties += tie
+        preds = fma_update(oracle, preds, np.float32(0.1), gpu["leaves"][gpu["leaf_idx"]])
+    assert ties <= 1
+    # the GPU's training predictions are the same fused updates of the same leaves
+    assert np.array_equal(m["train_preds"], preds)
+
+
+def test_fit_batched_matches_each_job(ctx, oracle):
+    """Three jobs of different shapes and losses trained in one batched call:
+    every job's trees pass the teacher-forced check against the oracle."""
+    from turbocat.capi import train_params
+    from oracle import tree_params as otp
+    shapes = [(9000, 7, 0, 0.0, 41), (25000, 19, 1, 1.0, 42), (4000, 40, 0, 1.0, 43)]
+    dss, ys, prms, data = [], [], [], []
+    for n, F, loss, lam, seed in shapes:
+        X, y = regression_data(n, F, seed) if loss == 0 else binary_data(n, F, seed)
+        ds, edges, ne = ctx.quantize_fit(X)
+        dss.append(ds); ys.append(y)
+        prms.append(train_params(loss, n_trees=8, learning_rate=0.2, max_depth=5, lambda_l2=lam))
+        data.append((ds.bins(), edges, ne, loss, lam))
+    models = ctx.train_batch(dss, ys, prms)
+    D = 5
+    for (bins, edges, ne, loss, lam), y, m in zip(data, ys, models):
+        n = bins.shape[1]
+        preds = np.full(n, np.float32(m["base"]), np.float32)
+        ties = 0
+        for t in range(8):
+            g, h = oracle.gradients_f32(loss, y, preds)
+            d = int(m["depth"][t])
+            gpu = dict(depth=d, feature=m["feature"][t * D:t * D + d], bin=m["bin"][t * D:t * D + d],
+                       fthr=m["fthr"][t * D:t * D + d], gain=m["gain"][t * D:t * D + d],
+                       leaves=m["leaves"][t << D:(t << D) + (1 << d)])
+            gpu["leaf_idx"] = oracle.leaf_index_binned(bins, gpu["feature"], gpu["bin"])
+            _, tie = compare_tree(gpu, bins, g, h, edges, ne, otp(max_depth=D, lambda_l2=lam), oracle)
+            ties += tie
+            preds = fma_update(oracle, preds, np.float32(0.2), gpu["leaves"][gpu["leaf_idx"]])
+        assert ties <= 1
+        assert np.array_equal(m["train_preds"], preds)

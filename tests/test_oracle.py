@@ -1,0 +1,243 @@
+"""Pinning the CPU oracle (tests run without a GPU).
+
+Each oracle function is checked against something other than itself:
+  * known-answer tests derived from the reference source (8-row toy, the
+    quantizer's uint8 wrap, threshold semantics);
+  * brute force on tiny inputs (exhaustive split search written from the
+    gain formula, counting histograms);
+  * the golden vectors in tests/golden/ written by the reference's own
+    hot-path code (tests/golden/make_golden.py), bit for bit;
+  * the compiled reference itself (oracle/_ref/libtcref.so) where it is
+    present, bit for bit.
+"""
+import glob
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+from conftest import binary_data, regression_data
+
+from oracle import LOSS_LOGLOSS, LOSS_MSE, RefLib, tree_params
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+HAVE_REF = RefLib.available()
+
+
+# ---------------------------------------------------------------- known answers
+
+def test_kat_eight_rows_tree(oracle):
+    X = np.array([-2, -1, 1, 2, -1.5, -0.5, 0.5, 1.5], np.float32).reshape(8, 1)
+    y = np.array([0, 0, 1, 1, 0, 0, 1, 1], np.float32)
+    bins, edges, ne = oracle.quantize(X)
+    assert list(bins[0]) == [0, 2, 5, 7, 1, 3, 4, 6]
+    assert list(edges[0, :ne[0]]) == [-1.5, -1, -0.5, 0.5, 1, 1.5, 2]
+    base = oracle.init_prediction(LOSS_LOGLOSS, y)
+    assert base == 0.0
+    g, h = oracle.gradients_f32(LOSS_LOGLOSS, y, np.zeros(8, np.float32))
+    assert list(g) == [0.5, 0.5, -0.5, -0.5, 0.5, 0.5, -0.5, -0.5] and (h == 0.25).all()
+    for build in (oracle.tree_f32, oracle.tree_f64):
+        t = build(bins, g, h, edges, ne, tree_params(3, lambda_l2=1.0, min_child_weight=0.01))
+        assert t["depth"] == 3
+        assert list(t["feature"]) == [0, 0, 0] and list(t["bin"]) == [3, 3, 3]
+        assert list(t["gain"]) == [2.0, 0.0, 0.0]           # zero-gain splits applied
+        assert list(t["fthr"]) == [0.5, 0.5, 0.5]
+        assert list(t["leaves"]) == [-1, 0, 0, 0, 0, 0, 0, 1]
+        assert list(t["leaf_idx"]) == [0, 0, 7, 7, 0, 0, 7, 7]
+
+
+def test_kat_level0_gains(oracle):
+    """Level-0 gains of the toy by bin: 0.1455, 0.5333, 1.1429, 2.0, ... (closed form
+    0.5 (GL^2/(HL+1) + GR^2/(HR+1)))."""
+    hist = np.zeros((1, 1, 256, 2), np.float64)
+    order = [0, 0, 0, 0, 1, 1, 1, 1]                 # y by bin 0..7 for the toy
+    for b, yy in enumerate(order):
+        hist[0, 0, b] = [0.5 - yy, 0.25]
+    for b, exp in enumerate([0.145455, 0.533333, 1.142857, 2.0, 1.142857, 0.533333, 0.145455]):
+        h1 = hist.copy()
+        h1[0, 0, b + 1:] = 0
+        gl, hl = h1[0, 0, :, 0].sum(), h1[0, 0, :, 1].sum()
+        gr, hr = -gl, 2.0 - hl
+        assert abs(0.5 * (gl * gl / (hl + 1) + gr * gr / (hr + 1)) - exp) < 1e-6
+    f, b, gain, second = oracle.split_f64(hist, [0.0], [2.0], [8], lam=1.0, is_root=True)
+    assert (f, b) == (0, 3) and abs(gain - 2.0) < 1e-12 and abs(second - 1.142857142857) < 1e-9
+
+
+@pytest.mark.parametrize("U,A", [(300, 44), (256, 0), (257, 1), (511, 255), (255, 255), (2, 2)])
+def test_kat_quantizer_wrap(oracle, U, A):
+    """A = min(255, U mod 256) bins; A - 1 edges at unique[b*U/A] (dataset.cpp:308-318)."""
+    vals = np.arange(U, dtype=np.float32)
+    X = np.concatenate([vals, vals[:17]]).reshape(-1, 1)       # duplicates do not count
+    bins, edges, ne = oracle.quantize(X)
+    assert ne[0] == max(A - 1, 0)
+    if A > 1:
+        assert list(edges[0, :ne[0]]) == [float(b * U // A) for b in range(1, A)]
+    assert bins.max() == max(A - 1, 0)
+    # the fixed mode keeps min(255, U) bins
+    _, _, ne2 = oracle.quantize(X, wrap=False)
+    assert ne2[0] == min(255, U) - 1
+
+
+def test_kat_threshold_semantics(oracle):
+    """bin > t  <=>  v >= edges[t] (upper_bound binning), NaN -> bin 255 goes left."""
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((3000, 1)).astype(np.float32)
+    bins, edges, ne = oracle.quantize(X)
+    for t in range(ne[0]):
+        assert np.array_equal(bins[0] > t, X[:, 0] >= edges[0, t])
+    Xn = np.array([[np.nan], [edges[0, 5]], [np.nextafter(edges[0, 5], -np.inf)]], np.float32)
+    bn = oracle.apply_bins(Xn, edges, ne)
+    assert list(bn[0]) == [255, 6, 5]
+
+
+def test_kat_all_nan_and_constant(oracle):
+    X = np.full((50, 2), np.nan, np.float32)
+    X[:, 1] = 3.0
+    bins, edges, ne = oracle.quantize(X)
+    assert ne[0] == 1 and edges[0, 0] == 0.0 and (bins[0] == 255).all()
+    assert ne[1] == 0 and (bins[1] == 0).all()
+
+
+# ---------------------------------------------------------------- brute force
+
+def brute_split(hist, G, H, C, lam, is_root, max_bins=255):
+    """Exhaustive search written directly from the gain formula."""
+    best, arg = -1e30, (0, 0)
+    nn, F = hist.shape[:2]
+    for f in range(F):
+        for b in range(max_bins - 1):
+            total, any_ = 0.0, False
+            for n in range(nn):
+                if not is_root and C[n] == 0:
+                    continue
+                gl, hl = hist[n, f, :b + 1, 0].sum(), hist[n, f, :b + 1, 1].sum()
+                gr, hr = G[n] - gl, H[n] - hl
+                if hl < 1e-10 or hr < 1e-10:
+                    continue
+                pg = G[n] ** 2 / (H[n] + lam) if H[n] > 1e-10 else 0.0
+                total += 0.5 * (gl * gl / (hl + lam) + gr * gr / (hr + lam) - pg)
+                any_ = True
+            if is_root and not any_:
+                continue
+            if total > best:
+                best, arg = total, (f, b)
+    return arg, best
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_split_brute_force(oracle, seed):
+    rng = np.random.default_rng(seed)
+    nn = [1, 2, 4][seed % 3]
+    F, nb = 3, 9
+    hist = np.zeros((nn, F, 256, 2))
+    hist[..., :nb, 0] = rng.integers(-3, 4, (nn, F, nb))       # small integers: exact sums
+    hist[..., :nb, 1] = rng.integers(0, 3, (nn, F, nb))
+    for n in range(nn):                                        # same node totals on every feature
+        hist[n, 1:] = hist[n, 0][None]
+        hist[n, 1:, :nb] = hist[n, 0, rng.permutation(nb)][None]
+    G = hist[:, 0, :, 0].sum(1)
+    H = hist[:, 0, :, 1].sum(1)
+    C = (H > 0).astype(np.uint32) * 5
+    for lam in (0.0, 1.0):
+        f, b, gain, _ = oracle.split_f64(hist, G, H, C, max_bins=12, lam=lam, is_root=nn == 1)
+        (ef, eb), eg = brute_split(hist, G, H, C, lam, nn == 1, max_bins=12)
+        assert (f, b) == (ef, eb) and abs(gain - eg) < 1e-9
+
+
+def test_hist_brute_force_and_sibling_identity(oracle):
+    X, _ = regression_data(700, 4, 3, low_card_col=1)
+    bins, _, _ = oracle.quantize(X)
+    rng = np.random.default_rng(1)
+    g = rng.integers(-8, 8, 700).astype(np.float32)            # integers: exact in float32
+    h = rng.integers(1, 4, 700).astype(np.float32)
+    rows = np.sort(rng.choice(700, 250, replace=False)).astype(np.uint32)
+    gh, cnt = oracle.hist_f32(bins, g, h, rows)
+    exp = np.zeros((4, 256, 2), np.float32)
+    for r in rows:
+        for f in range(4):
+            exp[f, bins[f, r]] += [g[r], h[r]]
+    assert np.array_equal(gh, exp)
+    assert (cnt.sum(1) == 250).all()
+    full, _ = oracle.hist_f32(bins, g, h)
+    rest = np.setdiff1d(np.arange(700), rows).astype(np.uint32)
+    other, _ = oracle.hist_f32(bins, g, h, rest)
+    assert np.array_equal(full - gh, other)                    # larger = parent - smaller
+
+
+def test_tree_f32_vs_f64_and_k1_reduction(oracle):
+    X, y = regression_data(3000, 6, 5)
+    bins, edges, ne = oracle.quantize(X)
+    g, h = oracle.gradients_f32(LOSS_MSE, y, np.zeros(3000, np.float32))
+    p = tree_params(5, lambda_l2=1.0)
+    a = oracle.tree_f32(bins, g, h, edges, ne, p)
+    b = oracle.tree_f64(bins, g, h, edges, ne, p)
+    c = oracle.tree_mc_f64(bins, g.astype(np.float64)[:, None], h.astype(np.float64)[:, None],
+                           edges, ne, p)
+    assert np.array_equal(a["feature"], b["feature"]) and np.array_equal(a["bin"], b["bin"])
+    assert np.array_equal(b["feature"], c["feature"]) and np.array_equal(b["bin"], c["bin"])
+    assert np.allclose(a["leaves"], b["leaves"], rtol=1e-5)
+    assert np.array_equal(b["leaves"], c["leaves"][:, 0])
+    assert np.array_equal(a["leaf_idx"], oracle.leaf_index_binned(bins, a["feature"], a["bin"]))
+
+
+def test_predict_f32_vs_f64(oracle):
+    X, y = regression_data(2000, 5, 8)
+    bins, edges, ne = oracle.quantize(X)
+    m, tp = oracle.fit_f32(bins, edges, ne, y, LOSS_MSE, 23, 0.1, tree_params(6, lambda_l2=0.0))
+    p32 = oracle.predict_f32(m, X)
+    p64, ab = oracle.predict_f64(m, X)
+    assert (np.abs(p32 - p64) / np.maximum(ab, 1e-30)).max() < 1e-6
+    # raw-float predict on the training rows == binned training predictions
+    assert np.allclose(p32, tp, rtol=1e-5, atol=1e-6)
+
+
+# ---------------------------------------------------------------- golden vectors
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])
+def test_oracle_matches_golden(oracle, path):
+    z = np.load(path, allow_pickle=False)
+    n, F, seed, loss, T, D = (int(v) for v in z["meta"])
+    kind = str(z["kind"])
+    X, y = regression_data(n, F, seed, low_card_col=2) if kind == "reg" else binary_data(n, F, seed)
+    bins, edges, ne = oracle.quantize(X)
+    assert np.array_equal(bins, z["bins"]) and np.array_equal(ne, z["n_edges"])
+    assert np.array_equal(edges, z["edges"])
+    g, h = oracle.gradients_f32(loss, y, z["p"])
+    assert np.array_equal(g, z["g"]) and np.array_equal(h, z["h"])
+    lam, lr = float(z["lam"]), float(z["lr"])
+    t = oracle.tree_f32(bins, g, h, edges, ne, tree_params(D, lambda_l2=lam))
+    assert np.array_equal(t["feature"], z["t_feature"]) and np.array_equal(t["bin"], z["t_bin"])
+    assert np.array_equal(t["gain"], z["t_gain"]) and np.array_equal(t["fthr"], z["t_fthr"])
+    assert np.array_equal(t["leaves"], z["t_leaves"]) and np.array_equal(t["leaf_idx"], z["t_leaf_idx"])
+    m, tp = oracle.fit_f32(bins, edges, ne, y, loss, T, lr, tree_params(D, lambda_l2=lam))
+    for k in ("feature", "bin", "fthr", "gain", "depth", "leaves"):
+        assert np.array_equal(m[k], z["m_" + k]), k
+    assert np.float32(m["base"]) == z["m_base"]
+    assert np.array_equal(tp, z["train_preds"])
+    assert np.array_equal(oracle.predict_f32(m, z["Xp"]), z["pred"])
+
+
+# ---------------------------------------------------------------- compiled reference
+
+@pytest.mark.skipif(not HAVE_REF, reason="oracle/_ref/libtcref.so not built (no reference tree)")
+@pytest.mark.parametrize("n,F,seed,D", [(2053, 5, 1, 6), (9000, 17, 2, 7), (16, 4, 3, 3)])
+def test_oracle_matches_reference(oracle, n, F, seed, D):
+    def body():
+        R = RefLib()
+        X, y = regression_data(n, F, seed, low_card_col=min(3, F - 1))
+        X[::31, 0] = X[0, 0]                                   # duplicates
+        yb = (y > np.median(y)).astype(np.float32)
+        for loss, yy, lam in ((LOSS_MSE, y, 0.0), (LOSS_LOGLOSS, yb, 1.0)):
+            ds = R.dataset(X, yy)
+            rb, re, rn = ds.bins()
+            ob, oe, on = oracle.quantize(X)
+            assert np.array_equal(rb, ob) and np.array_equal(rn, on) and np.array_equal(re, oe)
+            rm, rtp, _ = ds.fit(loss, 9, D, 0.1, lam, 1.0)
+            om, otp = oracle.fit_f32(ob, oe, on, yy, loss, 9, 0.1, tree_params(D, lambda_l2=lam))
+            for k in ("feature", "bin", "fthr", "gain", "depth", "leaves"):
+                assert np.array_equal(rm[k], om[k]), k
+            assert np.array_equal(rtp, otp)
+            Xp = np.random.default_rng(seed).standard_normal((64 * 8, F)).astype(np.float32)
+            assert np.array_equal(R.predict(rm, Xp), oracle.predict_f32(om, Xp))
+    RefLib.isolated(body)

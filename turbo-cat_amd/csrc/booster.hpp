@@ -1,0 +1,109 @@
+// Host C++ booster over the C-ABI: the MI355X counterpart of the reference's
+// Booster for the symmetric-tree path (src/boosting/booster.cpp:47-465,
+// :899-945, :1049-1203).  Training, binning and prediction run on the GPU
+// through include/turbocat_hip.h; this class owns the model on the host,
+// keeps it resident on the device for prediction, and reads/writes the
+// reference's TCAT v3 model files.
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "turbocat_hip.h"
+
+namespace tcamd {
+
+// Task codes as stored in TCAT files (TaskType, include/turbocat/types.hpp:242-247).
+enum Task : uint8_t { kBinary = 0, kMulticlass = 1, kRegression = 2 };
+
+struct Params {
+  // boosting (BoostingConfig, config.hpp:66-96)
+  uint32_t n_estimators = 500;
+  float learning_rate = 0.1f;
+  float subsample = 1.0f;
+  float colsample_bytree = 1.0f;
+  // tree (TreeConfig, config.hpp:26-59)
+  uint32_t max_depth = 6;
+  uint32_t max_bins = 255;
+  float lambda_l2 = 1.0f;
+  float lambda_l1 = 0.0f;
+  float max_delta_step = 0.0f;
+  uint32_t min_samples_leaf = 1;
+  float min_child_weight = 1.0f;
+  float min_split_gain = 0.0f;
+  // run
+  int32_t loss = TC_LOSS_MSE;
+  Task task = kRegression;
+  uint64_t seed = 42;
+  int32_t verbosity = 0;
+  int32_t device = 0;
+  bool wrap_bins = true;      // reproduce the reference quantizer's uint8 wrap
+  bool use_graph = true;
+};
+
+// Translate a nonzero C-ABI code into the reference's exception types.
+void check(int32_t rc);
+
+class Booster {
+ public:
+  explicit Booster(const Params& p);
+  ~Booster();
+  Booster(const Booster&) = delete;
+  Booster& operator=(const Booster&) = delete;
+
+  // Quantize X (row-major n x F), train n_estimators symmetric trees.
+  void fit(const float* X, const float* y, uint32_t n, uint32_t F);
+  // Raw scores + base (predict_raw_nobinning_fast, booster.cpp:899-925).
+  void predict_raw(const float* X, uint32_t n, uint32_t F, float* out);
+  // sigmoid(raw) (predict_proba_nobinning_fast, booster.cpp:927-945).
+  void predict_proba(const float* X, uint32_t n, uint32_t F, float* out);
+  // Per-feature sum of split gains, 256 entries (symmetric_tree.cpp:892, :1176).
+  std::vector<float> feature_importance() const;
+
+  void save(const std::string& path) const;   // TCAT v3 (booster.cpp:1049-1109)
+  void load(const std::string& path);         // TCAT v1-3 symmetric (booster.cpp:1111-1203)
+
+  bool fitted() const { return fitted_; }
+  uint32_t n_trees() const { return T_; }
+  uint32_t n_features() const { return F_; }
+  float base_prediction() const { return base_; }
+  const Params& params() const { return p_; }
+  Params& params() { return p_; }
+  double last_fit_seconds() const { return fit_seconds_; }
+
+  // model (tree-major, D = max_depth slots per tree, 2^D leaves per tree)
+  uint32_t depth_slots() const { return D_; }
+  const std::vector<uint16_t>& split_feature() const { return feat_; }
+  const std::vector<uint8_t>& split_bin() const { return bin_; }
+  const std::vector<float>& split_threshold() const { return fthr_; }
+  const std::vector<float>& split_gain() const { return gain_; }
+  const std::vector<uint16_t>& tree_depth() const { return depth_; }
+  const std::vector<float>& leaves() const { return leaves_; }
+  const std::vector<float>& tree_weights() const { return weights_; }
+  const std::vector<std::vector<float>>& bin_edges() const { return edges_; }
+
+ private:
+  void ensure_ensemble();
+  void drop_ensemble();
+
+  Params p_;
+  tc_ctx* ctx_ = nullptr;
+  tc_ensemble* ens_ = nullptr;
+  bool fitted_ = false;
+  uint32_t T_ = 0, D_ = 0, F_ = 0;
+  float base_ = 0.f;
+  double fit_seconds_ = 0.0;
+  std::vector<std::vector<float>> edges_;
+  std::vector<uint16_t> feat_;
+  std::vector<uint8_t> bin_;
+  std::vector<float> fthr_, gain_, leaves_, weights_;
+  std::vector<uint16_t> depth_;
+};
+
+// One process-wide context per device (the reference binds one Booster to
+// one thread; contexts here are shared the same way).
+tc_ctx* device_context(int32_t device);
+
+}  // namespace tcamd

@@ -1,0 +1,99 @@
+// Shared device/host definitions for the TurboCat MI355X (gfx950) hot path.
+//
+// Data layout in HBM (one "job" = one boosting run over one dataset):
+//   bins_rm   uint8  [N][Fp]     row-major, Fp = F rounded up to 16 (one
+//                                 16-byte load = one 16-feature tile of a row)
+//   bins_cm   uint8  [F][N]      column-major (split column reads)
+//   gh        float2 [N]         gradient / hessian per row
+//   rows      uint32 [D+1][N]    per level, row ids grouped by node, each
+//                                 node's rows in ascending order (stable)
+//   off       uint32 [D+1][2^D+1] node segment offsets into rows[level]
+//   hist      float2 [2^D-1][F][256]   per node (heap order: level d node n
+//                                 at (2^d - 1 + n)); slot 255 = NaN bin
+//   ng/nh/nc  float/float/uint32 [2^(D+1)-1] node totals, heap order
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tc {
+
+constexpr int kMaxDepth = 10;          // 2^10 leaves; u16 leaf index range
+constexpr int kTileF = 16;             // features per histogram tile (16 B)
+constexpr int kHistThreads = 256;
+constexpr int kPartChunk = 4096;       // rows per partition work item
+constexpr int kScanNodesPerPass = 32;  // node prefixes staged in LDS at a time
+constexpr uint32_t kEmptyKey = 0u;
+
+enum Loss : int32_t { kMSE = 0, kLogLoss = 1 };
+
+// Per-tree state of one job, device resident.
+struct TreeState {
+  int32_t active;       // 1 while levels are still being grown
+  int32_t depth;        // splits applied so far in the current tree
+  int32_t tree;         // index of the tree being built
+  int32_t pad;
+  unsigned long long best[kMaxDepth];  // argmax key per level
+};
+
+// One unit of histogram work: rows [start, start+len) of rows[level] for
+// node `node` (level-local index), feature tile `tile`.
+struct HistWork {
+  uint32_t job, node, start, len, tile, pad0, pad1, pad2;
+};
+
+// One unit of partition work: rows [start, start+len) of rows[level],
+// node `node`, result slot `slot` (per job, node order).
+struct PartWork {
+  uint32_t job, node, start, len, slot, pad0, pad1, pad2;
+};
+
+// Everything a kernel needs to know about one job.  Lives in device memory.
+struct JobDev {
+  // dataset
+  const uint8_t* bins_rm;
+  const uint8_t* bins_cm;
+  const float* edges;       // [F][255]
+  const uint32_t* n_edges;  // [F]
+  uint32_t N, F, Fp, T;     // T = Fp / 16
+  // hyper-parameters
+  int32_t loss;
+  uint32_t D, max_bins, min_samples_leaf;
+  float lr, lambda_l2, lambda_l1, max_delta_step, min_child_weight, min_split_gain;
+  float base;
+  uint32_t hist_chunk;      // rows per histogram work item
+  // training state
+  const float* y;
+  float* preds;
+  float2* gh;
+  uint32_t* rows;           // [D+1][N]
+  uint32_t* off;            // [D+1][2^D + 1]
+  float* ng;                // heap [2^(D+1) - 1]
+  float* nh;
+  uint32_t* nc;
+  float2* hist;             // heap [2^D - 1][F][256]
+  TreeState* st;
+  // partition scratch, per slot (capacity part_cap)
+  uint32_t* part_left;      // left count
+  uint32_t* part_loff;      // exclusive scan of left counts within the node
+  double* part_stats;       // [slot][4] = GL, HL, GR, HR
+  uint32_t part_cap;
+  // root-sum partials written by the gradient kernel, per workgroup
+  double* grad_partials;    // [wg][2]
+  uint32_t grad_wg_off;     // first workgroup of this job in the row grid
+  uint32_t grad_wg_cnt;
+  // model, tree-major
+  uint32_t n_trees;
+  uint16_t* m_feat;         // [n_trees][D]
+  uint8_t* m_bin;
+  float* m_fthr;
+  float* m_gain;
+  uint16_t* m_depth;        // [n_trees]
+  float* m_leaves;          // [n_trees][2^D]
+  unsigned long long* hist_rows_total;  // rows fed to histogram kernels (all trees)
+};
+
+__host__ __device__ inline uint32_t heap_index(uint32_t level, uint32_t node) {
+  return (1u << level) - 1u + node;
+}
+
+}  // namespace tc

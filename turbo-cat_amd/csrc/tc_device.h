@@ -1,0 +1,94 @@
+// Device helpers shared by the HIP translation units.
+#pragma once
+#include "tc_common.h"
+
+namespace tc {
+
+// Order-preserving map of a float to uint32 (larger float -> larger key).
+__device__ inline uint32_t float_order(float x) {
+  const uint32_t u = __float_as_uint(x);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline float float_unorder(uint32_t k) {
+  const uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+  return __uint_as_float(u);
+}
+
+// Split key: max over keys = max gain, ties to the lowest (feature, bin).
+__device__ inline unsigned long long split_key(float gain, uint32_t feature, uint32_t bin) {
+  const uint32_t lo = 0xFFFFFFFFu - (feature * 256u + bin);
+  return (static_cast<unsigned long long>(float_order(gain)) << 32) | lo;
+}
+
+// Exp as the reference's AVX2 build evaluates it inside the sigmoid:
+// round-to-nearest range reduction t = x*log2(e), degree-5 Taylor
+// polynomial of 2^f in Horner form with fused multiply-adds, scaled by
+// 2^round(t) through the exponent bits (src/boosting/loss.cpp:34-69).  The
+// reference compiler fuses the final scale into the sigmoid's 1 + e, so this
+// returns 1 + exp(x) with that single rounding.
+__device__ inline float one_plus_exp_ref(float x) {
+  x = fmaxf(x, -88.0f);
+  x = fminf(x, 88.0f);
+  const float t = __fmul_rn(x, 1.4426950408889634f);
+  const float ti = rintf(t);
+  const float tf = __fsub_rn(t, ti);
+  float p = __fmaf_rn(0.0013333558146428443f, tf, 0.009618129107628477f);
+  p = __fmaf_rn(p, tf, 0.05550410866482046f);
+  p = __fmaf_rn(p, tf, 0.2402265069591007f);
+  p = __fmaf_rn(p, tf, 0.6931471805599453f);
+  p = __fmaf_rn(p, tf, 1.0f);
+  const float pw = __uint_as_float(static_cast<uint32_t>(static_cast<int32_t>(ti) + 127) << 23);
+  return __fmaf_rn(p, pw, 1.0f);
+}
+
+// A6/A7 gradient of one row (src/boosting/loss.cpp:123-163, :348-362).
+// `vec_row` selects the reference's AVX2 formula (rows 0 .. 8*floor(N/8)-1);
+// the scalar tail uses the library exp.
+__device__ inline float2 row_gradient(int loss, float pred, float label, bool vec_row) {
+  if (loss == kMSE) {
+    return make_float2(__fmul_rn(2.0f, __fsub_rn(pred, label)), 2.0f);
+  }
+  float x = fmaxf(pred, -15.0f);
+  x = fminf(x, 15.0f);
+  float s;
+  if (vec_row) {
+    s = __fdiv_rn(1.0f, one_plus_exp_ref(__fsub_rn(0.0f, x)));
+  } else {
+    s = __fdiv_rn(1.0f, __fadd_rn(1.0f, expf(-x)));
+  }
+  const float g = __fsub_rn(s, label);
+  const float hh = __fmul_rn(s, __fsub_rn(1.0f, s));
+  return make_float2(g, hh > 1e-6f ? hh : 1e-6f);
+}
+
+__device__ inline double warp_sum_d(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__device__ inline uint32_t warp_sum_u(uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__device__ inline unsigned long long warp_max_u64(unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long w = __shfl_down(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+// Rank a logical work index so that consecutive indices (the feature tiles
+// of one row chunk) run on the same XCD back to back: workgroups are dealt
+// round-robin to the 8 XCDs, so XCD x gets blockIdx x, x+8, ...  Returns
+// false when this workgroup has no work.
+__device__ inline bool xcd_rank(uint32_t block, uint32_t count, uint32_t* logical) {
+  const uint32_t per = (count + 7u) >> 3;
+  const uint32_t k = block >> 3, x = block & 7u;
+  if (k >= per) return false;
+  const uint32_t l = x * per + k;
+  if (l >= count) return false;
+  *logical = l;
+  return true;
+}
+
+}  // namespace tc

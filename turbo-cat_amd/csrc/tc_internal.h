@@ -1,0 +1,98 @@
+// Host-side internals shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/turbocat_hip.h"
+#include "tc_common.h"
+
+struct tc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double last_loop_ms = 0.0, last_hist_ms = 0.0;
+};
+
+struct tc_dataset {
+  tc_ctx* ctx = nullptr;
+  uint32_t N = 0, F = 0, Fp = 0, max_bins = 255;
+  uint8_t* bins_rm = nullptr;   // [N][Fp]
+  uint8_t* bins_cm = nullptr;   // [F][N]
+  float* edges = nullptr;       // [F][255]
+  uint32_t* n_edges = nullptr;  // [F]
+  std::vector<float> h_edges;
+  std::vector<uint32_t> h_n_edges;
+};
+
+struct tc_ensemble {
+  tc_ctx* ctx = nullptr;
+  uint32_t T = 0, D = 0, F = 0;
+  float base = 0.f;
+  uint16_t* feat = nullptr;     // [T][D]
+  float* thr = nullptr;         // [T][D]
+  uint8_t* bin = nullptr;       // [T][D]
+  uint16_t* depth = nullptr;    // [T]
+  float* leaves = nullptr;      // [T][2^D]
+  float* weight = nullptr;      // [T]
+};
+
+namespace tc {
+
+// Error plumbing: internal code throws, the C-ABI boundary catches.
+struct Error : std::runtime_error {
+  int32_t code;
+  Error(int32_t c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_error(const std::string& msg);
+int32_t fail(int32_t code, const std::string& msg);
+
+#define TC_HIP(call)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      throw ::tc::Error(TC_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+#define TC_REQUIRE(cond, msg)                            \
+  do {                                                   \
+    if (!(cond)) throw ::tc::Error(TC_EINVAL, (msg));    \
+  } while (0)
+
+template <typename F>
+int32_t guard(F&& f) {
+  try {
+    f();
+    return TC_OK;
+  } catch (const Error& e) {
+    return fail(e.code, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(TC_ENOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(TC_EDEVICE, e.what());
+  }
+}
+
+template <typename T>
+T* dmalloc(size_t count) {
+  void* p = nullptr;
+  if (count == 0) count = 1;
+  const hipError_t e = hipMalloc(&p, count * sizeof(T));
+  if (e != hipSuccess) throw Error(TC_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return static_cast<T*>(p);
+}
+
+inline void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+// Host-side edge computation shared by fit and tests (dataset.cpp:278-334).
+void compute_edges(const float* X, uint32_t n, uint32_t F, uint32_t max_bins, bool wrap,
+                   std::vector<float>& edges, std::vector<uint32_t>& n_edges);
+// Bin X (host, row-major) on the device into ds (allocates ds buffers).
+void bin_on_device(tc_dataset* ds, const float* X);
+
+}  // namespace tc

@@ -1,0 +1,730 @@
+// Host side of training: device state for a batch of jobs, the fixed
+// per-tree launch sequence (optionally replayed as one hipGraph per tree),
+// and the C-ABI entry points built on it (tc_train, tc_train_batch,
+// tc_tree_build, tc_hist_build, tc_split_scan, tc_hist_subtract,
+// tc_gradients).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "tc_device.h"
+#include "tc_internal.h"
+#include "tc_train_kernels.hip"
+
+namespace tc {
+
+namespace {
+
+constexpr int kHistUnroll = 4;
+
+uint32_t ceil_div(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
+uint32_t round8(uint32_t x) { return std::max(8u, (x + 7u) & ~7u); }
+
+__global__ void k_iota(uint32_t* rows, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) rows[i] = i;
+}
+
+__global__ void k_init_state(TreeState* st) {
+  st->active = 0;
+  st->depth = 0;
+  st->tree = -1;
+  for (int d = 0; d < kMaxDepth; ++d) st->best[d] = kEmptyKey;
+}
+
+__global__ void k_sub_elementwise(const float* a, const float* b, float* c, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) c[i] = __fsub_rn(a[i], b[i]);
+}
+
+__global__ void k_gradients(int loss, const float* y, const float* p, float* g, float* h, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float2 v = row_gradient(loss, p[i], y[i], i < (n / 8u) * 8u);
+  g[i] = v.x;
+  h[i] = v.y;
+}
+
+}  // namespace
+
+struct JobSpec {
+  const tc_dataset* ds;
+  const float* y_host;   // labels (may be null for teacher forcing)
+  tc_train_params p;
+  float base;
+};
+
+class Trainer {
+ public:
+  Trainer(tc_ctx* ctx, const std::vector<JobSpec>& specs) : ctx_(ctx), specs_(specs) {
+    njobs_ = static_cast<uint32_t>(specs.size());
+    TC_REQUIRE(njobs_ > 0, "no jobs");
+    D_ = specs[0].p.tree.max_depth;
+    TC_REQUIRE(D_ >= 1 && D_ <= static_cast<uint32_t>(kMaxDepth), "max_depth must be in [1, 10]");
+    n_trees_ = specs[0].p.n_trees;
+    for (const auto& s : specs) {
+      TC_REQUIRE(s.p.tree.max_depth == D_, "all jobs of a batch share max_depth");
+      TC_REQUIRE(s.p.n_trees == n_trees_, "all jobs of a batch share n_trees");
+      TC_REQUIRE(s.p.loss == TC_LOSS_MSE || s.p.loss == TC_LOSS_LOGLOSS, "unsupported loss");
+      TC_REQUIRE((s.p.tree.max_bins & 0xFFu) >= 2, "max_bins must be in [2, 255]");
+      fmax_ = std::max(fmax_, s.ds->F);
+    }
+    try {
+      setup();
+    } catch (...) {
+      release();
+      throw;
+    }
+  }
+  ~Trainer() { release(); }
+
+  void init_gradients() {
+    hipLaunchKernelGGL(k_grad_update, dim3(total_grad_wg_), dim3(256), 0, ctx_->stream, d_jobs_,
+                       d_job_wg_off_, njobs_, 0);
+    TC_HIP(hipGetLastError());
+  }
+
+  void set_gradients(const float* g, const float* h) {
+    const uint32_t N = h_jobs_[0].N;
+    float* dg = dmalloc<float>(N);
+    float* dh = dmalloc<float>(N);
+    TC_HIP(hipMemcpyAsync(dg, g, sizeof(float) * N, hipMemcpyHostToDevice, ctx_->stream));
+    TC_HIP(hipMemcpyAsync(dh, h, sizeof(float) * N, hipMemcpyHostToDevice, ctx_->stream));
+    hipLaunchKernelGGL(k_set_gh, dim3(std::max(1u, ceil_div(N, kGradTile))), dim3(256), 0,
+                       ctx_->stream, d_jobs_, dg, dh);
+    TC_HIP(hipGetLastError());
+    TC_HIP(hipStreamSynchronize(ctx_->stream));
+    dfree(dg);
+    dfree(dh);
+  }
+
+  // The per-tree launch sequence.  hist_events (optional) brackets every
+  // histogram launch for live timing.
+  void enqueue_tree(std::vector<std::pair<hipEvent_t, hipEvent_t>>* hist_events) {
+    hipStream_t s = ctx_->stream;
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, lists_);
+    hipLaunchKernelGGL(k_tree_begin, dim3(1, njobs_), dim3(256), 0, s, d_jobs_, lists_);
+    for (uint32_t d = 0; d < D_; ++d) {
+      const uint32_t pairs = d > 0 ? (1u << (d - 1)) : 0u;
+      if (d > 0)
+        hipLaunchKernelGGL(k_zero_smaller, dim3(pairs * fmax_, njobs_), dim3(256), 0, s, d_jobs_, d,
+                           fmax_);
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      if (hist_events) {
+        TC_HIP(hipEventCreate(&e0));
+        TC_HIP(hipEventCreate(&e1));
+        TC_HIP(hipEventRecord(e0, s));
+      }
+      hipLaunchKernelGGL(k_hist<kHistUnroll>, dim3(round8(hist_cap_[d])), dim3(kHistThreads), 0, s,
+                         d_jobs_, lists_.hist[d], lists_.hist_count[d], d);
+      if (hist_events) {
+        TC_HIP(hipEventRecord(e1, s));
+        hist_events->emplace_back(e0, e1);
+      }
+      if (d > 0)
+        hipLaunchKernelGGL(k_subtract, dim3(pairs * fmax_, njobs_), dim3(256), 0, s, d_jobs_, d, fmax_);
+      hipLaunchKernelGGL(k_split_scan, dim3(fmax_, njobs_), dim3(256), 0, s, d_jobs_, d);
+      hipLaunchKernelGGL(k_finalize, dim3(1, njobs_), dim3(64), 0, s, d_jobs_, d);
+      hipLaunchKernelGGL(k_part_count, dim3(part_cap_[d]), dim3(256), 0, s, d_jobs_, lists_.part[d],
+                         lists_.part_count[d], d);
+      hipLaunchKernelGGL(k_part_plan, dim3(1, njobs_), dim3(256), 0, s, d_jobs_, lists_, d);
+      hipLaunchKernelGGL(k_part_scatter, dim3(part_cap_[d]), dim3(256), 0, s, d_jobs_,
+                         lists_.part[d], lists_.part_count[d], d);
+    }
+    hipLaunchKernelGGL(k_leaves, dim3(1, njobs_), dim3(256), 0, s, d_jobs_);
+    hipLaunchKernelGGL(k_grad_update, dim3(total_grad_wg_), dim3(256), 0, s, d_jobs_, d_job_wg_off_,
+                       njobs_, 1);
+    TC_HIP(hipGetLastError());
+  }
+
+  // Enqueue n more trees as graph replays (captured on first use).
+  void step_graph(uint32_t n) {
+    hipStream_t s = ctx_->stream;
+    if (!graph_exec_) {
+      hipGraph_t g;
+      TC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      enqueue_tree(nullptr);
+      TC_HIP(hipStreamEndCapture(s, &g));
+      const hipError_t e = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      TC_HIP(e);
+    }
+    for (uint32_t t = 0; t < n; ++t) TC_HIP(hipGraphLaunch(graph_exec_, s));
+  }
+
+  // Enqueue n trees without a graph, timing every histogram launch.
+  void step_timed(uint32_t n, double* hist_ms, uint32_t* launches) {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> hev;
+    for (uint32_t t = 0; t < n; ++t) enqueue_tree(&hev);
+    TC_HIP(hipStreamSynchronize(ctx_->stream));
+    double hs = 0.0;
+    for (auto& e : hev) {
+      float x = 0.f;
+      TC_HIP(hipEventElapsedTime(&x, e.first, e.second));
+      hs += x;
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+    if (hist_ms) *hist_ms = hs;
+    if (launches) *launches = static_cast<uint32_t>(hev.size());
+  }
+
+  // Run all trees; returns loop time (ms) and histogram time (ms).
+  void run(bool use_graph, bool time_hist, double* loop_ms, double* hist_ms) {
+    hipStream_t s = ctx_->stream;
+    hipEvent_t a, b;
+    TC_HIP(hipEventCreate(&a));
+    TC_HIP(hipEventCreate(&b));
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> hev;
+    if (use_graph && !time_hist) {
+      hipGraph_t g;
+      TC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      enqueue_tree(nullptr);
+      TC_HIP(hipStreamEndCapture(s, &g));
+      hipGraphExec_t ge;
+      TC_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+      TC_HIP(hipGraphDestroy(g));
+      TC_HIP(hipEventRecord(a, s));
+      for (uint32_t t = 0; t < n_trees_; ++t) TC_HIP(hipGraphLaunch(ge, s));
+      TC_HIP(hipEventRecord(b, s));
+      TC_HIP(hipStreamSynchronize(s));
+      TC_HIP(hipGraphExecDestroy(ge));
+    } else {
+      TC_HIP(hipEventRecord(a, s));
+      for (uint32_t t = 0; t < n_trees_; ++t) enqueue_tree(time_hist ? &hev : nullptr);
+      TC_HIP(hipEventRecord(b, s));
+      TC_HIP(hipStreamSynchronize(s));
+    }
+    float ms = 0.f;
+    TC_HIP(hipEventElapsedTime(&ms, a, b));
+    if (loop_ms) *loop_ms = ms;
+    double hs = 0.0;
+    for (auto& e : hev) {
+      float x = 0.f;
+      TC_HIP(hipEventElapsedTime(&x, e.first, e.second));
+      hs += x;
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+    if (hist_ms) *hist_ms = hs;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    TC_HIP(hipGetLastError());
+  }
+
+  void download(uint32_t j, tc_model* m) {
+    const JobDev& J = h_jobs_[j];
+    const size_t TD = static_cast<size_t>(n_trees_) * D_;
+    hipStream_t s = ctx_->stream;
+    if (m->feature) TC_HIP(hipMemcpyAsync(m->feature, J.m_feat, TD * 2, hipMemcpyDeviceToHost, s));
+    if (m->bin) TC_HIP(hipMemcpyAsync(m->bin, J.m_bin, TD, hipMemcpyDeviceToHost, s));
+    if (m->fthr) TC_HIP(hipMemcpyAsync(m->fthr, J.m_fthr, TD * 4, hipMemcpyDeviceToHost, s));
+    if (m->gain) TC_HIP(hipMemcpyAsync(m->gain, J.m_gain, TD * 4, hipMemcpyDeviceToHost, s));
+    if (m->depth) TC_HIP(hipMemcpyAsync(m->depth, J.m_depth, n_trees_ * 2, hipMemcpyDeviceToHost, s));
+    if (m->leaves)
+      TC_HIP(hipMemcpyAsync(m->leaves, J.m_leaves, (static_cast<size_t>(n_trees_) << D_) * 4,
+                            hipMemcpyDeviceToHost, s));
+    if (m->train_preds)
+      TC_HIP(hipMemcpyAsync(m->train_preds, J.preds, sizeof(float) * J.N, hipMemcpyDeviceToHost, s));
+    TC_HIP(hipStreamSynchronize(s));
+    m->base = J.base;
+  }
+
+  // Per-row leaf index of tree 0 (teacher-forcing checks).
+  void leaf_index(uint32_t* out) {
+    const JobDev& J = h_jobs_[0];
+    uint16_t depth = 0;
+    TC_HIP(hipMemcpy(&depth, J.m_depth, 2, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> rows(J.N), off((1u << depth) + 1);
+    TC_HIP(hipMemcpy(rows.data(), J.rows + static_cast<size_t>(depth) * J.N, 4ull * J.N,
+                     hipMemcpyDeviceToHost));
+    TC_HIP(hipMemcpy(off.data(), J.off + static_cast<size_t>(depth) * ((1u << D_) + 1),
+                     4ull * off.size(), hipMemcpyDeviceToHost));
+    for (uint32_t n = 0; n < (1u << depth); ++n)
+      for (uint32_t p = off[n]; p < off[n + 1]; ++p) out[rows[p]] = n;
+  }
+
+  uint64_t hist_rows(uint32_t j) const {
+    uint64_t v = 0;
+    TC_HIP(hipMemcpy(&v, hist_rows_[j], 8, hipMemcpyDeviceToHost));
+    return v;
+  }
+
+  uint32_t njobs() const { return njobs_; }
+  const JobDev& job(uint32_t j) const { return h_jobs_[j]; }
+
+ private:
+  template <typename T>
+  T* alloc(size_t n) {
+    T* p = dmalloc<T>(n);
+    allocs_.push_back(p);
+    return p;
+  }
+
+  void setup() {
+    hipStream_t s = ctx_->stream;
+    h_jobs_.resize(njobs_);
+    hist_rows_.resize(njobs_);
+    std::vector<uint32_t> wg_off(njobs_ + 1, 0);
+    std::fill(hist_cap_, hist_cap_ + kMaxDepth, 0u);
+    std::fill(part_cap_, part_cap_ + kMaxDepth, 0u);
+    for (uint32_t j = 0; j < njobs_; ++j) {
+      const JobSpec& S = specs_[j];
+      const tc_dataset* ds = S.ds;
+      JobDev& J = h_jobs_[j];
+      std::memset(&J, 0, sizeof(J));
+      const uint32_t N = ds->N;
+      J.bins_rm = ds->bins_rm; J.bins_cm = ds->bins_cm; J.edges = ds->edges; J.n_edges = ds->n_edges;
+      J.N = N; J.F = ds->F; J.Fp = ds->Fp; J.T = ds->Fp / kTileF;
+      J.loss = S.p.loss; J.D = D_; J.max_bins = S.p.tree.max_bins;
+      J.min_samples_leaf = S.p.tree.min_samples_leaf;
+      J.lr = S.p.learning_rate; J.lambda_l2 = S.p.tree.lambda_l2; J.lambda_l1 = S.p.tree.lambda_l1;
+      J.max_delta_step = S.p.tree.max_delta_step; J.min_child_weight = S.p.tree.min_child_weight;
+      J.min_split_gain = S.p.tree.min_split_gain;
+      J.base = S.base;
+      uint32_t ch = S.p.hist_chunk;
+      if (ch == 0) {  // heuristic: ~2048 root work items per job batch
+        const uint64_t work = static_cast<uint64_t>(N) * J.T;
+        ch = 1024;
+        while (ch < 65536 && work / ch > 2048) ch <<= 1;
+      }
+      J.hist_chunk = ch;
+      float* y = alloc<float>(N);
+      if (S.y_host) TC_HIP(hipMemcpyAsync(y, S.y_host, 4ull * N, hipMemcpyHostToDevice, s));
+      else TC_HIP(hipMemsetAsync(y, 0, 4ull * N, s));
+      J.y = y;
+      J.preds = alloc<float>(N);
+      J.gh = alloc<float2>(N);
+      J.rows = alloc<uint32_t>(static_cast<size_t>(D_ + 1) * N);
+      hipLaunchKernelGGL(k_iota, dim3(std::max(1u, ceil_div(N, 256))), dim3(256), 0, s, J.rows, N);
+      J.off = alloc<uint32_t>(static_cast<size_t>(D_ + 1) * ((1u << D_) + 1));
+      const size_t nheap = (2ull << D_) - 1;
+      J.ng = alloc<float>(nheap);
+      J.nh = alloc<float>(nheap);
+      J.nc = alloc<uint32_t>(nheap);
+      J.hist = alloc<float2>(((1ull << D_) - 1) * J.F * 256);
+      J.st = alloc<TreeState>(1);
+      hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1), 0, s, J.st);
+      J.part_cap = ceil_div(N, kPartChunk) + (1u << D_);
+      J.part_left = alloc<uint32_t>(J.part_cap);
+      J.part_loff = alloc<uint32_t>(J.part_cap);
+      J.part_stats = alloc<double>(4ull * J.part_cap);
+      J.grad_wg_cnt = std::max(1u, ceil_div(N, kGradTile));
+      J.grad_wg_off = wg_off[j];
+      wg_off[j + 1] = wg_off[j] + J.grad_wg_cnt;
+      J.grad_partials = alloc<double>(2ull * J.grad_wg_cnt);
+      J.n_trees = n_trees_;
+      const size_t TD = static_cast<size_t>(n_trees_) * D_;
+      J.m_feat = alloc<uint16_t>(TD);
+      J.m_bin = alloc<uint8_t>(TD);
+      J.m_fthr = alloc<float>(TD);
+      J.m_gain = alloc<float>(TD);
+      J.m_depth = alloc<uint16_t>(n_trees_);
+      J.m_leaves = alloc<float>(static_cast<size_t>(n_trees_) << D_);
+      TC_HIP(hipMemsetAsync(J.m_feat, 0, TD * 2, s));
+      TC_HIP(hipMemsetAsync(J.m_bin, 0, TD, s));
+      TC_HIP(hipMemsetAsync(J.m_fthr, 0, TD * 4, s));
+      TC_HIP(hipMemsetAsync(J.m_gain, 0, TD * 4, s));
+      hist_rows_[j] = alloc<unsigned long long>(1);
+      TC_HIP(hipMemsetAsync(hist_rows_[j], 0, 8, s));
+      J.hist_rows_total = hist_rows_[j];
+      // work-list capacities (see emit_hist / k_part_plan)
+      for (uint32_t d = 0; d < D_; ++d) {
+        const uint32_t hc = d == 0 ? ceil_div(N, ch) : ceil_div(N / 2, ch) + (1u << (d - 1));
+        hist_cap_[d] += hc * J.T;
+        part_cap_[d] += ceil_div(N, kPartChunk) + (1u << d);
+      }
+    }
+    for (uint32_t d = 0; d < D_; ++d) {
+      lists_.hist[d] = alloc<HistWork>(std::max(1u, hist_cap_[d]));
+      lists_.part[d] = alloc<PartWork>(std::max(1u, part_cap_[d]));
+      part_cap_[d] = std::max(1u, part_cap_[d]);
+    }
+    for (uint32_t d = D_; d < static_cast<uint32_t>(kMaxDepth); ++d) {
+      lists_.hist[d] = lists_.hist[0];
+      lists_.part[d] = lists_.part[0];
+    }
+    uint32_t* counters = alloc<uint32_t>(2 * kMaxDepth);
+    TC_HIP(hipMemsetAsync(counters, 0, 8 * kMaxDepth, s));
+    for (int d = 0; d < kMaxDepth; ++d) {
+      lists_.hist_count[d] = counters + d;
+      lists_.part_count[d] = counters + kMaxDepth + d;
+    }
+    total_grad_wg_ = wg_off[njobs_];
+    d_job_wg_off_ = alloc<uint32_t>(njobs_ + 1);
+    TC_HIP(hipMemcpyAsync(d_job_wg_off_, wg_off.data(), 4ull * (njobs_ + 1), hipMemcpyHostToDevice, s));
+    d_jobs_ = alloc<JobDev>(njobs_);
+    TC_HIP(hipMemcpyAsync(d_jobs_, h_jobs_.data(), sizeof(JobDev) * njobs_, hipMemcpyHostToDevice, s));
+    TC_HIP(hipStreamSynchronize(s));
+  }
+
+  void release() {
+    if (ctx_ && ctx_->stream) (void)hipStreamSynchronize(ctx_->stream);
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
+    for (void* p : allocs_) dfree(p);
+    allocs_.clear();
+  }
+
+  tc_ctx* ctx_;
+  std::vector<JobSpec> specs_;
+  uint32_t njobs_ = 0, D_ = 0, n_trees_ = 0, fmax_ = 0;
+  std::vector<JobDev> h_jobs_;
+  std::vector<unsigned long long*> hist_rows_;
+  JobDev* d_jobs_ = nullptr;
+  uint32_t* d_job_wg_off_ = nullptr;
+  uint32_t total_grad_wg_ = 0;
+  Lists lists_{};
+  uint32_t hist_cap_[kMaxDepth] = {}, part_cap_[kMaxDepth] = {};
+  std::vector<void*> allocs_;
+  hipGraphExec_t graph_exec_ = nullptr;
+};
+
+}  // namespace tc
+
+struct tc_trainer {
+  tc_ctx* ctx = nullptr;
+  std::unique_ptr<tc::Trainer> tr;
+};
+
+namespace tc {
+
+float init_prediction_host(int32_t loss, const float* y, uint32_t n) {
+  // loss.cpp:217-226 (LogLoss) and :364-370 (MSE): sequential float sum
+  float s = 0.f;
+  for (uint32_t i = 0; i < n; ++i) s += y[i];
+  if (loss == TC_LOSS_MSE) return s / static_cast<float>(n);
+  float p = s / static_cast<float>(n);
+  p = std::max(1e-7f, std::min(1.0f - 1e-7f, p));
+  return std::log(p / (1.0f - p));
+}
+
+void check_params(const tc_train_params* p) {
+  TC_REQUIRE(p != nullptr, "params is NULL");
+  TC_REQUIRE(p->tree.max_depth >= 1 && p->tree.max_depth <= static_cast<uint32_t>(kMaxDepth),
+             "max_depth must be in [1, 10]");
+  TC_REQUIRE(p->learning_rate == p->learning_rate, "learning_rate is NaN");
+}
+
+}  // namespace tc
+
+using namespace tc;
+
+extern "C" {
+
+int32_t tc_init_prediction(int32_t loss, const float* y, uint32_t n, float* base) {
+  return guard([&] {
+    TC_REQUIRE(y && base && n > 0, "tc_init_prediction: bad arguments");
+    TC_REQUIRE(loss == TC_LOSS_MSE || loss == TC_LOSS_LOGLOSS, "unsupported loss");
+    *base = init_prediction_host(loss, y, n);
+  });
+}
+
+int32_t tc_train(tc_dataset* ds, const float* y, const tc_train_params* p, tc_model* model) {
+  return guard([&] {
+    TC_REQUIRE(ds && y && model, "tc_train: NULL argument");
+    check_params(p);
+    TC_REQUIRE(ds->N > 0, "tc_train: empty dataset");
+    JobSpec s{ds, y, *p, init_prediction_host(p->loss, y, ds->N)};
+    Trainer tr(ds->ctx, {s});
+    tr.init_gradients();
+    double loop_ms = 0, hist_ms = 0;
+    tr.run(p->use_graph != 0, false, &loop_ms, &hist_ms);
+    ds->ctx->last_loop_ms = loop_ms;
+    ds->ctx->last_hist_ms = 0.0;
+    tr.download(0, model);
+  });
+}
+
+int32_t tc_train_batch(tc_ctx* ctx, uint32_t n_jobs, tc_dataset* const* ds, const float* const* y,
+                       const tc_train_params* p, tc_model* models) {
+  return guard([&] {
+    TC_REQUIRE(ctx && ds && y && p && models && n_jobs > 0, "tc_train_batch: NULL argument");
+    std::vector<JobSpec> specs;
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+      check_params(&p[j]);
+      TC_REQUIRE(ds[j] && ds[j]->ctx == ctx && ds[j]->N > 0, "tc_train_batch: bad dataset");
+      specs.push_back(JobSpec{ds[j], y[j], p[j], init_prediction_host(p[j].loss, y[j], ds[j]->N)});
+    }
+    Trainer tr(ctx, specs);
+    tr.init_gradients();
+    double loop_ms = 0, hist_ms = 0;
+    tr.run(p[0].use_graph != 0, false, &loop_ms, &hist_ms);
+    ctx->last_loop_ms = loop_ms;
+    ctx->last_hist_ms = 0.0;
+    for (uint32_t j = 0; j < n_jobs; ++j) tr.download(j, &models[j]);
+  });
+}
+
+int32_t tc_trainer_create(tc_ctx* ctx, uint32_t n_jobs, tc_dataset* const* ds, const float* const* y,
+                          const tc_train_params* p, tc_trainer** out) {
+  return guard([&] {
+    TC_REQUIRE(ctx && ds && y && p && out && n_jobs > 0, "tc_trainer_create: NULL argument");
+    *out = nullptr;
+    std::vector<JobSpec> specs;
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+      check_params(&p[j]);
+      TC_REQUIRE(ds[j] && ds[j]->ctx == ctx && ds[j]->N > 0, "tc_trainer_create: bad dataset");
+      specs.push_back(JobSpec{ds[j], y[j], p[j], init_prediction_host(p[j].loss, y[j], ds[j]->N)});
+    }
+    auto h = std::make_unique<tc_trainer>();
+    h->ctx = ctx;
+    h->tr = std::make_unique<Trainer>(ctx, specs);
+    h->tr->init_gradients();
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    *out = h.release();
+  });
+}
+
+int32_t tc_trainer_step(tc_trainer* t, uint32_t n_trees) {
+  return guard([&] {
+    TC_REQUIRE(t, "NULL trainer");
+    t->tr->step_graph(n_trees);
+  });
+}
+
+int32_t tc_trainer_step_timed(tc_trainer* t, uint32_t n_trees, double* hist_ms, uint32_t* launches) {
+  return guard([&] {
+    TC_REQUIRE(t, "NULL trainer");
+    t->tr->step_timed(n_trees, hist_ms, launches);
+  });
+}
+
+int32_t tc_trainer_sync(tc_trainer* t) {
+  return guard([&] {
+    TC_REQUIRE(t, "NULL trainer");
+    TC_HIP(hipStreamSynchronize(t->ctx->stream));
+  });
+}
+
+int32_t tc_trainer_model(tc_trainer* t, uint32_t job, tc_model* m) {
+  return guard([&] {
+    TC_REQUIRE(t && m, "tc_trainer_model: NULL argument");
+    TC_REQUIRE(job < t->tr->njobs(), "job index out of range");
+    t->tr->download(job, m);
+  });
+}
+
+int32_t tc_trainer_hist_rows(tc_trainer* t, uint32_t job, uint64_t* rows) {
+  return guard([&] {
+    TC_REQUIRE(t && rows, "tc_trainer_hist_rows: NULL argument");
+    TC_REQUIRE(job < t->tr->njobs(), "job index out of range");
+    *rows = t->tr->hist_rows(job);
+  });
+}
+
+int32_t tc_trainer_destroy(tc_trainer* t) {
+  return guard([&] { delete t; });
+}
+
+int32_t tc_last_train_timing(tc_ctx* ctx, double* loop_ms, double* hist_ms) {
+  return guard([&] {
+    TC_REQUIRE(ctx, "NULL ctx");
+    if (loop_ms) *loop_ms = ctx->last_loop_ms;
+    if (hist_ms) *hist_ms = ctx->last_hist_ms;
+  });
+}
+
+int32_t tc_tree_build(tc_dataset* ds, const float* g, const float* h, const tc_tree_params* tp,
+                      uint16_t* feature, uint8_t* bin, float* fthr, float* gain, float* leaves,
+                      uint32_t* leaf_idx, int32_t* depth) {
+  return guard([&] {
+    TC_REQUIRE(ds && g && h && tp && feature && bin && fthr && gain && leaves && depth,
+               "tc_tree_build: NULL argument");
+    tc_train_params p{};
+    p.loss = TC_LOSS_MSE;
+    p.n_trees = 1;
+    p.learning_rate = 0.f;
+    p.tree = *tp;
+    check_params(&p);
+    TC_REQUIRE(ds->N > 0, "tc_tree_build: empty dataset");
+    JobSpec s{ds, nullptr, p, 0.f};
+    Trainer tr(ds->ctx, {s});
+    tr.set_gradients(g, h);
+    tr.enqueue_tree(nullptr);
+    TC_HIP(hipStreamSynchronize(ds->ctx->stream));
+    const uint32_t D = tp->max_depth;
+    std::vector<uint16_t> dep(1);
+    tc_model m{};
+    m.feature = feature; m.bin = bin; m.fthr = fthr; m.gain = gain; m.depth = dep.data();
+    std::vector<float> lv(1u << D);
+    m.leaves = lv.data();
+    tr.download(0, &m);
+    *depth = dep[0];
+    std::memcpy(leaves, lv.data(), sizeof(float) * (1u << dep[0]));
+    if (leaf_idx) tr.leaf_index(leaf_idx);
+  });
+}
+
+int32_t tc_hist_build(tc_dataset* ds, const float* g, const float* h, const uint32_t* rows,
+                      uint32_t n_rows, float* hist_out) {
+  return guard([&] {
+    TC_REQUIRE(ds && g && h && hist_out, "tc_hist_build: NULL argument");
+    tc_ctx* ctx = ds->ctx;
+    hipStream_t s = ctx->stream;
+    const uint32_t N = ds->N, F = ds->F;
+    const uint32_t nr = rows ? n_rows : N;
+    std::vector<uint32_t> r(nr);
+    for (uint32_t i = 0; i < nr; ++i) {
+      r[i] = rows ? rows[i] : i;
+      TC_REQUIRE(r[i] < N, "tc_hist_build: row index out of range");
+    }
+    std::vector<void*> owned;
+    auto al = [&](size_t bytes) { void* p = dmalloc<uint8_t>(bytes); owned.push_back(p); return p; };
+    try {
+      JobDev J;
+      std::memset(&J, 0, sizeof(J));
+      J.bins_rm = ds->bins_rm; J.bins_cm = ds->bins_cm; J.N = N; J.F = F; J.Fp = ds->Fp;
+      J.T = ds->Fp / kTileF;
+      std::vector<float2> ghh(N);
+      for (uint32_t i = 0; i < N; ++i) ghh[i] = make_float2(g[i], h[i]);
+      J.gh = static_cast<float2*>(al(8ull * N));
+      TC_HIP(hipMemcpyAsync(J.gh, ghh.data(), 8ull * N, hipMemcpyHostToDevice, s));
+      J.rows = static_cast<uint32_t*>(al(4ull * std::max(1u, nr)));
+      if (nr) TC_HIP(hipMemcpyAsync(J.rows, r.data(), 4ull * nr, hipMemcpyHostToDevice, s));
+      J.hist = static_cast<float2*>(al(8ull * F * 256));
+      TC_HIP(hipMemsetAsync(J.hist, 0, 8ull * F * 256, s));
+      TreeState st{};
+      st.active = 1;
+      J.st = static_cast<TreeState*>(al(sizeof(TreeState)));
+      TC_HIP(hipMemcpyAsync(J.st, &st, sizeof(st), hipMemcpyHostToDevice, s));
+      const uint32_t ch = 2048;
+      std::vector<HistWork> wl;
+      for (uint32_t s0 = 0; s0 < nr; s0 += ch)
+        for (uint32_t t = 0; t < J.T; ++t) {
+          HistWork w{};
+          w.job = 0; w.node = 0; w.start = s0; w.len = std::min(ch, nr - s0); w.tile = t;
+          wl.push_back(w);
+        }
+      const uint32_t cnt = static_cast<uint32_t>(wl.size());
+      HistWork* dw = static_cast<HistWork*>(al(sizeof(HistWork) * std::max<size_t>(1, wl.size())));
+      if (cnt) TC_HIP(hipMemcpyAsync(dw, wl.data(), sizeof(HistWork) * cnt, hipMemcpyHostToDevice, s));
+      uint32_t* dc = static_cast<uint32_t*>(al(4));
+      TC_HIP(hipMemcpyAsync(dc, &cnt, 4, hipMemcpyHostToDevice, s));
+      JobDev* dj = static_cast<JobDev*>(al(sizeof(JobDev)));
+      TC_HIP(hipMemcpyAsync(dj, &J, sizeof(J), hipMemcpyHostToDevice, s));
+      if (cnt)
+        hipLaunchKernelGGL(k_hist<4>, dim3(round8(cnt)), dim3(kHistThreads), 0, s, dj, dw, dc, 0u);
+      TC_HIP(hipGetLastError());
+      TC_HIP(hipMemcpyAsync(hist_out, J.hist, 8ull * F * 256, hipMemcpyDeviceToHost, s));
+      TC_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+      for (void* p : owned) dfree(p);
+      throw;
+    }
+    for (void* p : owned) dfree(p);
+  });
+}
+
+int32_t tc_hist_subtract(tc_ctx* ctx, const float* parent, const float* child, float* out,
+                         uint32_t F) {
+  return guard([&] {
+    TC_REQUIRE(ctx && parent && child && out, "tc_hist_subtract: NULL argument");
+    const uint32_t n = F * 512;
+    float *a = dmalloc<float>(n), *b = nullptr, *c = nullptr;
+    try {
+      b = dmalloc<float>(n);
+      c = dmalloc<float>(n);
+      TC_HIP(hipMemcpy(a, parent, 4ull * n, hipMemcpyHostToDevice));
+      TC_HIP(hipMemcpy(b, child, 4ull * n, hipMemcpyHostToDevice));
+      hipLaunchKernelGGL(k_sub_elementwise, dim3(std::max(1u, ceil_div(n, 256))), dim3(256), 0,
+                         ctx->stream, a, b, c, n);
+      TC_HIP(hipGetLastError());
+      TC_HIP(hipStreamSynchronize(ctx->stream));
+      TC_HIP(hipMemcpy(out, c, 4ull * n, hipMemcpyDeviceToHost));
+    } catch (...) {
+      dfree(a); dfree(b); dfree(c);
+      throw;
+    }
+    dfree(a); dfree(b); dfree(c);
+  });
+}
+
+int32_t tc_split_scan(tc_ctx* ctx, const float* hist, const float* node_g, const float* node_h,
+                      const uint32_t* node_count, uint32_t n_nodes, uint32_t F, uint32_t max_bins,
+                      float lambda_l2, int32_t is_root, int32_t* feature, int32_t* bin,
+                      float* gain) {
+  return guard([&] {
+    TC_REQUIRE(ctx && hist && node_g && node_h && node_count && feature && bin && gain,
+               "tc_split_scan: NULL argument");
+    TC_REQUIRE(n_nodes >= 1 && (n_nodes & (n_nodes - 1)) == 0 && n_nodes <= 512,
+               "tc_split_scan: n_nodes must be a power of two <= 512");
+    uint32_t level = 0;
+    while ((1u << level) < n_nodes) ++level;
+    TC_REQUIRE((level == 0) == (is_root != 0), "tc_split_scan: is_root must equal (n_nodes == 1)");
+    hipStream_t s = ctx->stream;
+    std::vector<void*> owned;
+    auto al = [&](size_t bytes) { void* p = dmalloc<uint8_t>(bytes); owned.push_back(p); return p; };
+    try {
+      const size_t nheap = (2ull << level) - 1;
+      const size_t hb = static_cast<size_t>(F) * 256 * 8;
+      JobDev J;
+      std::memset(&J, 0, sizeof(J));
+      J.F = F; J.max_bins = max_bins; J.lambda_l2 = lambda_l2; J.D = level + 1;
+      J.hist = static_cast<float2*>(al(hb * nheap));
+      TC_HIP(hipMemcpyAsync(J.hist + ((1ull << level) - 1) * F * 256, hist, hb * n_nodes,
+                            hipMemcpyHostToDevice, s));
+      J.ng = static_cast<float*>(al(4 * nheap));
+      J.nh = static_cast<float*>(al(4 * nheap));
+      J.nc = static_cast<uint32_t*>(al(4 * nheap));
+      const size_t o = (1ull << level) - 1;
+      TC_HIP(hipMemcpyAsync(J.ng + o, node_g, 4ull * n_nodes, hipMemcpyHostToDevice, s));
+      TC_HIP(hipMemcpyAsync(J.nh + o, node_h, 4ull * n_nodes, hipMemcpyHostToDevice, s));
+      TC_HIP(hipMemcpyAsync(J.nc + o, node_count, 4ull * n_nodes, hipMemcpyHostToDevice, s));
+      TreeState st{};
+      st.active = 1;
+      J.st = static_cast<TreeState*>(al(sizeof(TreeState)));
+      TC_HIP(hipMemcpyAsync(J.st, &st, sizeof(st), hipMemcpyHostToDevice, s));
+      JobDev* dj = static_cast<JobDev*>(al(sizeof(JobDev)));
+      TC_HIP(hipMemcpyAsync(dj, &J, sizeof(J), hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_split_scan, dim3(F, 1), dim3(256), 0, s, dj, level);
+      TC_HIP(hipGetLastError());
+      TC_HIP(hipMemcpyAsync(&st, J.st, sizeof(st), hipMemcpyDeviceToHost, s));
+      TC_HIP(hipStreamSynchronize(s));
+      const unsigned long long key = st.best[level];
+      if (key == kEmptyKey) {
+        *feature = -1; *bin = -1; *gain = -1e30f;
+      } else {
+        const uint32_t k = static_cast<uint32_t>(key >> 32);
+        const uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+        float gv;
+        std::memcpy(&gv, &u, 4);
+        const uint32_t fb = 0xFFFFFFFFu - static_cast<uint32_t>(key & 0xFFFFFFFFu);
+        *feature = static_cast<int32_t>(fb >> 8);
+        *bin = static_cast<int32_t>(fb & 255u);
+        *gain = gv;
+      }
+    } catch (...) {
+      for (void* p : owned) dfree(p);
+      throw;
+    }
+    for (void* p : owned) dfree(p);
+  });
+}
+
+int32_t tc_gradients(tc_ctx* ctx, int32_t loss, const float* y, const float* preds, float* g,
+                     float* h, uint32_t n) {
+  return guard([&] {
+    TC_REQUIRE(ctx && y && preds && g && h, "tc_gradients: NULL argument");
+    TC_REQUIRE(loss == TC_LOSS_MSE || loss == TC_LOSS_LOGLOSS, "unsupported loss");
+    if (n == 0) return;
+    float* buf = dmalloc<float>(4ull * n);
+    try {
+      TC_HIP(hipMemcpy(buf, y, 4ull * n, hipMemcpyHostToDevice));
+      TC_HIP(hipMemcpy(buf + n, preds, 4ull * n, hipMemcpyHostToDevice));
+      hipLaunchKernelGGL(k_gradients, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, loss, buf,
+                         buf + n, buf + 2ull * n, buf + 3ull * n, n);
+      TC_HIP(hipGetLastError());
+      TC_HIP(hipStreamSynchronize(ctx->stream));
+      TC_HIP(hipMemcpy(g, buf + 2ull * n, 4ull * n, hipMemcpyDeviceToHost));
+      TC_HIP(hipMemcpy(h, buf + 3ull * n, 4ull * n, hipMemcpyDeviceToHost));
+    } catch (...) {
+      dfree(buf);
+      throw;
+    }
+    dfree(buf);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,430 @@
+"""ctypes front-end to the C-ABI in include/turbocat_hip.h.
+
+This is the thin Python view of the drop-in boundary used by the parity
+tests and the benchmark; the user-facing classes live in
+``turbocat.TurboCatClassifier`` / ``TurboCatRegressor``.  Every call raises
+``TurboCatError`` (a RuntimeError, like the reference's pybind-translated
+std::runtime_error) when the library returns a nonzero code; the library
+itself refuses to run on anything but a gfx950 device.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import (POINTER, Structure, byref, c_char_p, c_double, c_float, c_int32, c_int64,
+                    c_size_t, c_uint8, c_uint16, c_uint32, c_uint64, c_void_p)
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libturbocat_hip.so")
+
+TC_LOSS_MSE, TC_LOSS_LOGLOSS = 0, 1
+TC_EBUSY, TC_ETIMEOUT = -5, -6
+
+# Every exported symbol of include/turbocat_hip.h (checked by the tests).
+EXPORTS = [
+    "tc_ctx_create", "tc_ctx_destroy", "tc_ctx_synchronize", "tc_last_error",
+    "tc_device_alloc", "tc_device_free", "tc_memcpy_h2d", "tc_memcpy_d2h",
+    "tc_quantize_fit", "tc_dataset_create", "tc_dataset_bins", "tc_dataset_shape",
+    "tc_dataset_destroy", "tc_gradients", "tc_init_prediction", "tc_hist_build",
+    "tc_hist_subtract", "tc_split_scan", "tc_tree_build", "tc_train", "tc_train_batch",
+    "tc_last_train_timing", "tc_trainer_create", "tc_trainer_step", "tc_trainer_step_timed",
+    "tc_trainer_sync", "tc_trainer_model", "tc_trainer_hist_rows", "tc_trainer_destroy",
+    "tc_ensemble_create", "tc_ensemble_destroy", "tc_predict",
+    "tc_predict_dev", "tc_predict_binned", "tc_serve_start", "tc_serve_submit",
+    "tc_serve_wait", "tc_serve_stats", "tc_serve_stop",
+]
+
+
+class TurboCatError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class TreeParams(Structure):
+    _fields_ = [("max_depth", c_uint32), ("max_bins", c_uint32), ("lambda_l2", c_float),
+                ("lambda_l1", c_float), ("max_delta_step", c_float),
+                ("min_samples_leaf", c_uint32), ("min_child_weight", c_float),
+                ("min_split_gain", c_float)]
+
+
+class TrainParams(Structure):
+    _fields_ = [("loss", c_int32), ("n_trees", c_uint32), ("learning_rate", c_float),
+                ("tree", TreeParams), ("hist_chunk", c_uint32), ("use_graph", c_int32)]
+
+
+class Model(Structure):
+    _fields_ = [("feature", c_void_p), ("bin", c_void_p), ("fthr", c_void_p), ("gain", c_void_p),
+                ("depth", c_void_p), ("leaves", c_void_p), ("train_preds", c_void_p),
+                ("base", c_float)]
+
+
+class ServeParams(Structure):
+    _fields_ = [("max_batch_rows", c_uint32), ("memory_budget", c_uint64),
+                ("n_streams", c_uint32), ("transform", c_int32)]
+
+
+def tree_params(max_depth=6, max_bins=255, lambda_l2=1.0, lambda_l1=0.0, max_delta_step=0.0,
+                min_samples_leaf=1, min_child_weight=1.0, min_split_gain=0.0) -> TreeParams:
+    return TreeParams(max_depth, max_bins, lambda_l2, lambda_l1, max_delta_step,
+                      min_samples_leaf, min_child_weight, min_split_gain)
+
+
+def train_params(loss=TC_LOSS_MSE, n_trees=100, learning_rate=0.1, hist_chunk=0, use_graph=1,
+                 **tree) -> TrainParams:
+    return TrainParams(loss, n_trees, learning_rate, tree_params(**tree), hist_chunk, use_graph)
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise TurboCatError(-3, f"{path} not built; run python turbo-cat_amd/build.py")
+    L = ctypes.CDLL(path)
+    V, I, U = c_void_p, c_int32, c_uint32
+    sig = {
+        "tc_ctx_create": [I, V], "tc_ctx_destroy": [V], "tc_ctx_synchronize": [V],
+        "tc_device_alloc": [V, c_size_t, V], "tc_device_free": [V, V],
+        "tc_memcpy_h2d": [V, V, V, c_size_t], "tc_memcpy_d2h": [V, V, V, c_size_t],
+        "tc_quantize_fit": [V, V, U, U, U, I, V, V, V],
+        "tc_dataset_create": [V, V, U, U, V, V, U, V],
+        "tc_dataset_bins": [V, V], "tc_dataset_shape": [V, V, V], "tc_dataset_destroy": [V],
+        "tc_gradients": [V, I, V, V, V, V, U], "tc_init_prediction": [I, V, U, V],
+        "tc_hist_build": [V, V, V, V, U, V], "tc_hist_subtract": [V, V, V, V, U],
+        "tc_split_scan": [V, V, V, V, V, U, U, U, c_float, I, V, V, V],
+        "tc_tree_build": [V, V, V, POINTER(TreeParams), V, V, V, V, V, V, V],
+        "tc_train": [V, V, POINTER(TrainParams), POINTER(Model)],
+        "tc_train_batch": [V, U, V, V, V, V],
+        "tc_last_train_timing": [V, V, V],
+        "tc_trainer_create": [V, U, V, V, V, V], "tc_trainer_step": [V, U],
+        "tc_trainer_step_timed": [V, U, V, V], "tc_trainer_sync": [V],
+        "tc_trainer_model": [V, U, POINTER(Model)], "tc_trainer_hist_rows": [V, U, V],
+        "tc_trainer_destroy": [V],
+        "tc_ensemble_create": [V, U, U, V, V, V, V, V, V, c_float, U, V],
+        "tc_ensemble_destroy": [V], "tc_predict": [V, V, U, U, I, V, V],
+        "tc_predict_dev": [V, V, U, U, I, V], "tc_predict_binned": [V, V, V],
+        "tc_serve_start": [V, POINTER(ServeParams), V], "tc_serve_submit": [V, V, U, V, V],
+        "tc_serve_wait": [V, c_uint64, c_int64], "tc_serve_stats": [V, V, V],
+        "tc_serve_stop": [V],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = c_int32
+    L.tc_last_error.restype = c_char_p
+    L.tc_last_error.argtypes = []
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(c_void_p)
+
+
+def check(rc):
+    if rc != 0:
+        raise TurboCatError(rc, load().tc_last_error().decode(errors="replace"))
+    return rc
+
+
+class Context:
+    """A device context (one HIP stream on one GPU)."""
+
+    def __init__(self, device: int = 0):
+        self.L = load()
+        self.h = c_void_p()
+        check(self.L.tc_ctx_create(device, byref(self.h)))
+
+    def close(self):
+        if self.h:
+            self.L.tc_ctx_destroy(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check(self.L.tc_ctx_synchronize(self.h))
+
+    # ------------------------------------------------------------- datasets
+    def quantize_fit(self, X, max_bins=255, wrap=True):
+        X = np.ascontiguousarray(X, np.float32)
+        n, F = X.shape
+        edges = np.zeros((F, 255), np.float32)
+        ne = np.zeros(F, np.uint32)
+        h = c_void_p()
+        check(self.L.tc_quantize_fit(self.h, _p(X), n, F, max_bins, int(wrap), _p(edges), _p(ne),
+                                     byref(h)))
+        return Dataset(self, h, n, F), edges, ne
+
+    def dataset(self, X, edges, n_edges, max_bins=255):
+        X = np.ascontiguousarray(X, np.float32)
+        n, F = X.shape
+        h = c_void_p()
+        check(self.L.tc_dataset_create(self.h, _p(X), n, F,
+                                       _p(np.ascontiguousarray(edges, np.float32)),
+                                       _p(np.ascontiguousarray(n_edges, np.uint32)), max_bins,
+                                       byref(h)))
+        return Dataset(self, h, n, F)
+
+    # ------------------------------------------------------------ primitives
+    def gradients(self, loss, y, preds):
+        y = np.ascontiguousarray(y, np.float32)
+        p = np.ascontiguousarray(preds, np.float32)
+        g = np.zeros_like(y)
+        h = np.zeros_like(y)
+        check(self.L.tc_gradients(self.h, loss, _p(y), _p(p), _p(g), _p(h), len(y)))
+        return g, h
+
+    def hist_subtract(self, parent, child):
+        parent = np.ascontiguousarray(parent, np.float32)
+        child = np.ascontiguousarray(child, np.float32)
+        out = np.zeros_like(parent)
+        check(self.L.tc_hist_subtract(self.h, _p(parent), _p(child), _p(out), parent.shape[0]))
+        return out
+
+    def split_scan(self, hist, node_g, node_h, node_count, max_bins=255, lambda_l2=1.0,
+                   is_root=None):
+        hist = np.ascontiguousarray(hist, np.float32)
+        nn, F = hist.shape[0], hist.shape[1]
+        if is_root is None:
+            is_root = nn == 1
+        f, b, g = c_int32(), c_int32(), c_float()
+        check(self.L.tc_split_scan(self.h, _p(hist), _p(np.ascontiguousarray(node_g, np.float32)),
+                                   _p(np.ascontiguousarray(node_h, np.float32)),
+                                   _p(np.ascontiguousarray(node_count, np.uint32)), nn, F,
+                                   max_bins, lambda_l2, int(is_root), byref(f), byref(b),
+                                   byref(g)))
+        return f.value, b.value, g.value
+
+    # -------------------------------------------------------------- training
+    def train_batch(self, datasets, ys, params):
+        n = len(datasets)
+        if isinstance(params, TrainParams):
+            params = [params] * n
+        P = (TrainParams * n)(*params)
+        D = (c_void_p * n)(*[d.h for d in datasets])
+        ys = [np.ascontiguousarray(y, np.float32) for y in ys]
+        Y = (c_void_p * n)(*[y.ctypes.data for y in ys])
+        models, arrs = [], []
+        for d, p in zip(datasets, params):
+            m, a = _new_model(p.n_trees, p.tree.max_depth, d.n)
+            models.append(m)
+            arrs.append(a)
+        M = (Model * n)(*models)
+        check(self.L.tc_train_batch(self.h, n, D, Y, byref(P[0]), byref(M[0])))
+        out = []
+        for k in range(n):
+            a = arrs[k]
+            a["base"] = float(M[k].base)
+            a["weight"] = np.full(params[k].n_trees, params[k].learning_rate, np.float32)
+            out.append(a)
+        return out
+
+    def last_train_timing(self):
+        a, b = c_double(), c_double()
+        check(self.L.tc_last_train_timing(self.h, byref(a), byref(b)))
+        return a.value, b.value
+
+    def ensemble(self, model, n_features):
+        return Ensemble(self, model, n_features)
+
+
+def _new_model(T, D, n):
+    a = dict(T=T, D=D, feature=np.zeros(T * D, np.uint16), bin=np.zeros(T * D, np.uint8),
+             fthr=np.zeros(T * D, np.float32), gain=np.zeros(T * D, np.float32),
+             depth=np.zeros(T, np.uint16), leaves=np.zeros(T << D, np.float32),
+             train_preds=np.zeros(n, np.float32))
+    m = Model(_p(a["feature"]), _p(a["bin"]), _p(a["fthr"]), _p(a["gain"]), _p(a["depth"]),
+              _p(a["leaves"]), _p(a["train_preds"]), 0.0)
+    return m, a
+
+
+class Dataset:
+    def __init__(self, ctx: Context, h, n, F):
+        self.ctx, self.h, self.n, self.F = ctx, h, n, F
+
+    def close(self):
+        if self.h:
+            self.ctx.L.tc_dataset_destroy(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bins(self):
+        out = np.zeros((self.F, self.n), np.uint8)
+        check(self.ctx.L.tc_dataset_bins(self.h, _p(out)))
+        return out
+
+    def hist(self, g, h, rows=None):
+        out = np.zeros((self.F, 256, 2), np.float32)
+        r = None if rows is None else np.ascontiguousarray(rows, np.uint32)
+        check(self.ctx.L.tc_hist_build(self.h, _p(np.ascontiguousarray(g, np.float32)),
+                                       _p(np.ascontiguousarray(h, np.float32)), _p(r),
+                                       0 if r is None else len(r), _p(out)))
+        return out
+
+    def tree(self, g, h, params: TreeParams, want_leaf_idx=True):
+        D = params.max_depth
+        feat = np.zeros(D, np.uint16); b = np.zeros(D, np.uint8); ft = np.zeros(D, np.float32)
+        gn = np.zeros(D, np.float32); lv = np.zeros(1 << D, np.float32)
+        li = np.zeros(self.n, np.uint32) if want_leaf_idx else None
+        dep = c_int32()
+        check(self.ctx.L.tc_tree_build(self.h, _p(np.ascontiguousarray(g, np.float32)),
+                                       _p(np.ascontiguousarray(h, np.float32)), byref(params),
+                                       _p(feat), _p(b), _p(ft), _p(gn), _p(lv), _p(li),
+                                       byref(dep)))
+        d = dep.value
+        return dict(depth=d, feature=feat[:d], bin=b[:d], fthr=ft[:d], gain=gn[:d],
+                    leaves=lv[:1 << d], leaf_idx=li)
+
+    def train(self, y, params: TrainParams):
+        m, a = _new_model(params.n_trees, params.tree.max_depth, self.n)
+        y = np.ascontiguousarray(y, np.float32)
+        check(self.ctx.L.tc_train(self.h, _p(y), byref(params), byref(m)))
+        a["base"] = float(m.base)
+        a["weight"] = np.full(params.n_trees, params.learning_rate, np.float32)
+        return a
+
+
+class Trainer:
+    """Stepwise boosting over one or more resident datasets (tc_trainer_*)."""
+
+    def __init__(self, ctx: Context, datasets, ys, params):
+        n = len(datasets)
+        if isinstance(params, TrainParams):
+            params = [params] * n
+        self.ctx, self.params, self.datasets = ctx, list(params), list(datasets)
+        self._ys = [np.ascontiguousarray(y, np.float32) for y in ys]
+        P = (TrainParams * n)(*params)
+        D = (c_void_p * n)(*[d.h for d in datasets])
+        Y = (c_void_p * n)(*[y.ctypes.data for y in self._ys])
+        self.h = c_void_p()
+        check(ctx.L.tc_trainer_create(ctx.h, n, D, Y, byref(P[0]), byref(self.h)))
+
+    def step(self, n_trees):
+        check(self.ctx.L.tc_trainer_step(self.h, n_trees))
+
+    def step_timed(self, n_trees):
+        ms, k = c_double(), c_uint32()
+        check(self.ctx.L.tc_trainer_step_timed(self.h, n_trees, byref(ms), byref(k)))
+        return ms.value, k.value
+
+    def sync(self):
+        check(self.ctx.L.tc_trainer_sync(self.h))
+
+    def hist_rows(self, job=0):
+        v = c_uint64()
+        check(self.ctx.L.tc_trainer_hist_rows(self.h, job, byref(v)))
+        return v.value
+
+    def model(self, job=0):
+        p = self.params[job]
+        m, a = _new_model(p.n_trees, p.tree.max_depth, self.datasets[job].n)
+        check(self.ctx.L.tc_trainer_model(self.h, job, byref(m)))
+        a["base"] = float(m.base)
+        a["weight"] = np.full(p.n_trees, p.learning_rate, np.float32)
+        return a
+
+    def close(self):
+        if self.h:
+            self.ctx.L.tc_trainer_destroy(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ensemble:
+    def __init__(self, ctx: Context, model: dict, n_features: int):
+        self.ctx = ctx
+        self.T, self.D, self.F = model["T"], model["D"], n_features
+        self.h = c_void_p()
+        w = model.get("weight")
+        if w is None:
+            w = np.ones(self.T, np.float32)
+        check(ctx.L.tc_ensemble_create(ctx.h, self.T, self.D, _p(model["feature"]),
+                                       _p(model["fthr"]), _p(model.get("bin")), _p(model["depth"]),
+                                       _p(model["leaves"]), _p(np.ascontiguousarray(w, np.float32)),
+                                       float(model["base"]), n_features, byref(self.h)))
+
+    def close(self):
+        if self.h:
+            self.ctx.L.tc_ensemble_destroy(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def predict(self, X, transform=0, want_leaf_idx=False):
+        X = np.ascontiguousarray(X, np.float32)
+        n = X.shape[0]
+        out = np.zeros(n, np.float32)
+        li = np.zeros((n, self.T), np.uint32) if want_leaf_idx else None
+        check(self.ctx.L.tc_predict(self.h, _p(X), n, self.F, transform, _p(out), _p(li)))
+        return (out, li) if want_leaf_idx else out
+
+    def predict_dev(self, x_dev, n, out_dev, transform=0):
+        check(self.ctx.L.tc_predict_dev(self.h, x_dev, n, self.F, transform, out_dev))
+
+    def predict_binned(self, ds: Dataset):
+        out = np.zeros(ds.n, np.float32)
+        check(self.ctx.L.tc_predict_binned(self.h, ds.h, _p(out)))
+        return out
+
+
+class Server:
+    """Continuous-batching server over a resident ensemble."""
+
+    def __init__(self, ens: Ensemble, max_batch_rows=65536, memory_budget=1 << 30, n_streams=2,
+                 transform=0):
+        self.ens = ens
+        self.L = ens.ctx.L
+        self.h = c_void_p()
+        p = ServeParams(max_batch_rows, memory_budget, n_streams, transform)
+        check(self.L.tc_serve_start(ens.h, byref(p), byref(self.h)))
+        self._keep = {}
+
+    def submit(self, X, out):
+        t = c_uint64()
+        check(self.L.tc_serve_submit(self.h, _p(X), X.shape[0], _p(out), byref(t)))
+        self._keep[t.value] = (X, out)
+        return t.value
+
+    def wait(self, ticket, timeout_us=-1):
+        rc = self.L.tc_serve_wait(self.h, ticket, timeout_us)
+        if rc == TC_ETIMEOUT:
+            return False
+        check(rc)
+        self._keep.pop(ticket, None)
+        return True
+
+    def stats(self):
+        b, r = c_uint64(), c_uint64()
+        check(self.L.tc_serve_stats(self.h, byref(b), byref(r)))
+        return b.value, r.value
+
+    def stop(self):
+        if self.h:
+            check(self.L.tc_serve_stop(self.h))
+            self.h = c_void_p()
