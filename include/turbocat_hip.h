@@ -63,8 +63,12 @@ typedef struct {
   float min_split_gain;
 } tc_tree_params;
 
-/* Boosting parameters (BoostingConfig, config.hpp:66-96; symmetric trees,
- * no row/column sampling). */
+/* Boosting parameters (BoostingConfig, config.hpp:66-96; symmetric trees).
+ * Row sampling follows Booster::train (booster.cpp:198-229): per tree, when
+ * subsample < 1 an xorshift64* draw seeds Dataset::random_subsample
+ * (dataset.cpp:552-562: std::mt19937_64 + std::shuffle of 0..N-1, first
+ * uint32(N * subsample) kept, in shuffled order); when colsample_bytree < 1
+ * one more draw is consumed (symmetric trees ignore the feature subset). */
 typedef struct {
   int32_t loss;              /* TC_LOSS_MSE or TC_LOSS_LOGLOSS */
   uint32_t n_trees;
@@ -72,6 +76,9 @@ typedef struct {
   tc_tree_params tree;
   uint32_t hist_chunk;       /* rows per histogram work item; 0 = autotuned */
   int32_t use_graph;         /* 1: replay one captured hipGraph per tree */
+  float subsample;           /* in (0, 1): row sample per tree; 0 or >= 1: every row */
+  float colsample_bytree;    /* in (0, 1): consumes an RNG draw per tree */
+  uint64_t seed;             /* Config::seed (config.hpp:235) */
 } tc_train_params;
 
 /* Host-side model arrays filled by training (caller-allocated).
@@ -118,6 +125,10 @@ int32_t tc_dataset_destroy(tc_dataset* ds);
 /* A6/A7: Loss::compute_gradients (loss.hpp:57-63; loss.cpp:123-211, :348-362). */
 int32_t tc_gradients(tc_ctx* ctx, int32_t loss, const float* y, const float* preds, float* g,
                      float* h, uint32_t n);
+/* Row sample of one tree: Dataset::random_subsample(ratio, seed)
+ * (dataset.cpp:552-562).  out needs n slots; *n_out = uint32(n * ratio). */
+int32_t tc_random_subsample(uint32_t n, float ratio, uint64_t seed, uint32_t* out,
+                            uint32_t* n_out);
 /* A5: Loss::init_prediction (loss.cpp:217-226, :364-370). */
 int32_t tc_init_prediction(int32_t loss, const float* y, uint32_t n, float* base);
 

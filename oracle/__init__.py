@@ -74,10 +74,10 @@ class Oracle:
         L.tco_hist_f64.argtypes = [V, c_uint32, c_uint32, V, V, V, c_uint32, V, V, V]
         L.tco_split_f32.argtypes = [V, V, V, V, c_uint32, c_uint32, c_uint32, c_float, c_int, V, V, V]
         L.tco_split_f64.argtypes = [V, V, V, V, c_uint32, c_uint32, c_uint32, c_double, c_int, V, V, V, V]
-        L.tco_tree_build_f32.argtypes = [V, c_uint32, c_uint32, V, V, V, V, POINTER(TreeParams),
-                                         V, V, V, V, V, V]
-        L.tco_tree_build_f64.argtypes = [V, c_uint32, c_uint32, V, V, V, V, POINTER(TreeParams),
-                                         V, V, V, V, V, V, V]
+        L.tco_tree_build_f32.argtypes = [V, c_uint32, c_uint32, V, V, V, c_uint32, V, V,
+                                         POINTER(TreeParams), V, V, V, V, V, V]
+        L.tco_tree_build_f64.argtypes = [V, c_uint32, c_uint32, V, V, V, c_uint32, V, V,
+                                         POINTER(TreeParams), V, V, V, V, V, V, V]
         L.tco_tree_build_mc_f64.argtypes = [V, c_uint32, c_uint32, c_uint32, V, V, V, V,
                                             POINTER(TreeParams), V, V, V, V, V, V, V]
         L.tco_leaf_index_binned.argtypes = [V, c_uint32, c_uint32, V, V, V]
@@ -176,25 +176,29 @@ class Oracle:
         return f.value, b.value, gn.value, sec.value
 
     # ---------------------------------------------------------------- A9-A14
-    def tree_f32(self, bins, g, h, edges, n_edges, params: TreeParams):
+    def tree_f32(self, bins, g, h, edges, n_edges, params: TreeParams, rows=None):
         F, n = bins.shape
         D = params.max_depth
         feat = np.zeros(D, np.uint16); b = np.zeros(D, np.uint8); ft = np.zeros(D, np.float32)
         gn = np.zeros(D, np.float32); lv = np.zeros(1 << D, np.float32); li = np.zeros(n, np.uint32)
+        r = None if rows is None else np.ascontiguousarray(rows, np.uint32)
         d = self.L.tco_tree_build_f32(_p(bins), n, F, _p(np.ascontiguousarray(g, np.float32)),
-                                      _p(np.ascontiguousarray(h, np.float32)), _p(edges),
+                                      _p(np.ascontiguousarray(h, np.float32)), _p(r),
+                                      n if r is None else len(r), _p(edges),
                                       _p(n_edges), ctypes.byref(params), _p(feat), _p(b), _p(ft),
                                       _p(gn), _p(lv), _p(li))
         return dict(depth=d, feature=feat[:d], bin=b[:d], fthr=ft[:d], gain=gn[:d],
                     leaves=lv[:1 << d], leaf_idx=li)
 
-    def tree_f64(self, bins, g, h, edges, n_edges, params: TreeParams):
+    def tree_f64(self, bins, g, h, edges, n_edges, params: TreeParams, rows=None):
         F, n = bins.shape
         D = params.max_depth
         feat = np.zeros(D, np.uint16); b = np.zeros(D, np.uint8); ft = np.zeros(D, np.float32)
         gn = np.zeros(D); gap = np.zeros(D); lv = np.zeros(1 << D); li = np.zeros(n, np.uint32)
+        r = None if rows is None else np.ascontiguousarray(rows, np.uint32)
         d = self.L.tco_tree_build_f64(_p(bins), n, F, _p(np.ascontiguousarray(g, np.float32)),
-                                      _p(np.ascontiguousarray(h, np.float32)), _p(edges),
+                                      _p(np.ascontiguousarray(h, np.float32)), _p(r),
+                                      n if r is None else len(r), _p(edges),
                                       _p(n_edges), ctypes.byref(params), _p(feat), _p(b), _p(ft),
                                       _p(gn), _p(gap), _p(lv), _p(li))
         return dict(depth=d, feature=feat[:d], bin=b[:d], fthr=ft[:d], gain=gn[:d], gap=gap[:d],
@@ -288,7 +292,9 @@ class RefLib:
         L.tcref_tree_build.argtypes = [V, V, V, c_uint32, c_float, c_float, V, V, V, V, V, V]
         L.tcref_tree_build.restype = c_int
         L.tcref_fit.argtypes = [V, c_int, c_uint32, c_uint32, c_float, c_float, c_float,
-                                V, V, V, V, V, V, V, V]
+                                c_float, c_float, ctypes.c_uint64, V, V, V, V, V, V, V, V]
+        L.tcref_random_subsample.argtypes = [V, c_float, ctypes.c_uint64, V]
+        L.tcref_random_subsample.restype = c_uint32
         L.tcref_fit.restype = c_float
         L.tcref_predict.argtypes = [V, V, V, V, V, c_uint32, c_uint32, c_float, c_float, V,
                                     c_uint32, c_uint32, V]
@@ -385,11 +391,18 @@ class RefDataset:
         return dict(depth=d, feature=feat[:d], bin=b[:d], fthr=ft[:d], gain=gn[:d],
                     leaves=lv[:1 << d], leaf_idx=li)
 
-    def fit(self, loss, n_trees, max_depth=6, lr=0.1, lambda_l2=1.0, min_child_weight=1.0):
+    def random_subsample(self, ratio, seed):
+        out = np.zeros(self.n, np.uint32)
+        k = self.lib.L.tcref_random_subsample(self.h, ratio, seed, _p(out))
+        return out[:k]
+
+    def fit(self, loss, n_trees, max_depth=6, lr=0.1, lambda_l2=1.0, min_child_weight=1.0,
+            subsample=1.0, colsample=1.0, seed=42):
         m = new_model(n_trees, max_depth)
         tp = np.zeros(self.n, np.float32); sec = np.zeros(3)
         base = self.lib.L.tcref_fit(self.h, loss, n_trees, max_depth, lr, lambda_l2,
-                                    min_child_weight, _p(m["feature"]), _p(m["bin"]), _p(m["fthr"]),
+                                    min_child_weight, subsample, colsample, seed,
+                                    _p(m["feature"]), _p(m["bin"]), _p(m["fthr"]),
                                     _p(m["gain"]), _p(m["depth"]), _p(m["leaves"]), _p(tp), _p(sec))
         m["base"] = float(base)
         m["weight"][:] = lr

@@ -5,8 +5,8 @@
 // in the product links or loads this.
 //
 // The boosting loop below restates src/boosting/booster.cpp:92-100, :173-174,
-// :438-465, :505-519 and :551-583 (binary/regression, symmetric branch, no
-// sampling) because booster.cpp itself does not compile here (it pulls in
+// :198-229, :438-465, :505-519 and :551-583 (binary/regression, symmetric
+// branch, row sampling) because booster.cpp itself does not compile here (it pulls in
 // Eigen through tree.hpp).  Everything else is the reference's code:
 //   Dataset::from_dense / compute_bins      src/core/dataset.cpp:50-94, :176-344
 //   Dataset::compute_target_statistics      src/core/dataset.cpp:679-721
@@ -172,13 +172,30 @@ int tcref_tree_build(void* h, const float* g, const float* hs, uint32_t max_dept
     return depth;
 }
 
-// Full fit on the dataset (symmetric trees, no sampling), mirroring
-// booster.cpp's loop.  Model arrays: feature/bin/fthr/gain [T*D], depth [T],
+// Dataset::random_subsample (src/core/dataset.cpp:552-562) on the handle's
+// row count; returns the number of rows written to out.
+uint32_t tcref_random_subsample(void* h, float ratio, uint64_t seed, uint32_t* out) {
+    auto* r = static_cast<RefData*>(h);
+    auto idx = r->ds->random_subsample(ratio, seed);
+    std::memcpy(out, idx.data(), idx.size() * sizeof(uint32_t));
+    return static_cast<uint32_t>(idx.size());
+}
+
+// Full fit on the dataset (symmetric trees), mirroring
+// booster.cpp's loop, row sampling included.  Model arrays: feature/bin/fthr/gain [T*D], depth [T],
 // leaves [T*2^D].  train_preds [n] gets the final training predictions.
 // seconds[0] = target statistics + init, seconds[1] = boosting loop,
 // seconds[2] = time inside CPUHistogramBuilder-driven SymmetricTree::build.
+static uint64_t xorshift64s(uint64_t& s) {  // Booster::next_random, booster.cpp:1219-1225
+    s ^= s >> 12;
+    s ^= s << 25;
+    s ^= s >> 27;
+    return s * 0x2545F4914F6CDD1DULL;
+}
+
 float tcref_fit(void* h, int loss_id, uint32_t n_trees, uint32_t max_depth, float lr,
-                float lambda_l2, float min_child_weight,
+                float lambda_l2, float min_child_weight, float subsample, float colsample,
+                uint64_t seed,
                 uint16_t* feat, uint8_t* bin, float* fthr, float* gain, uint16_t* depths,
                 float* leaves, float* train_preds, double* seconds) {
     auto* r = static_cast<RefData*>(h);
@@ -192,6 +209,7 @@ float tcref_fit(void* h, int loss_id, uint32_t n_trees, uint32_t max_depth, floa
     ds.compute_target_statistics(c);
     double t1 = now_s();
     std::vector<float> preds(n, base), buf(n);
+    uint64_t rng = seed;
     std::vector<Index> all(n);
     std::iota(all.begin(), all.end(), 0u);
     const uint32_t L = 1u << max_depth;
@@ -199,9 +217,13 @@ float tcref_fit(void* h, int loss_id, uint32_t n_trees, uint32_t max_depth, floa
     for (uint32_t it = 0; it < n_trees; ++it) {
         ds.ensure_gradients_allocated(n);
         loss->compute_gradients(ds.labels().data(), preds.data(), ds.gradients(), ds.hessians(), n);
+        // sample selection and feature draw (booster.cpp:198-229)
+        std::vector<Index> rows = all;
+        if (subsample < 1.0f) rows = ds.random_subsample(subsample, xorshift64s(rng));
+        if (colsample < 1.0f) (void)ds.random_feature_subsample(colsample, xorshift64s(rng));
         SymmetricTree t(c.tree);
         double a = now_s();
-        t.build(ds, all, *hb);
+        t.build(ds, rows, *hb);
         tb += now_s() - a;
         t.predict_batch(ds, buf.data());
         #pragma omp parallel for

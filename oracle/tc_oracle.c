@@ -277,22 +277,24 @@ void tco_split_f64(const double* hist, const double* node_g, const double* node_
 #undef SUF
 
 int tco_tree_build_f32(const uint8_t* bins, uint32_t n, uint32_t F, const float* g, const float* h,
+                       const uint32_t* root, uint32_t n_root,
                        const float* edges, const uint32_t* n_edges, const tco_tree_params* p,
                        uint16_t* feature, uint8_t* bin, float* fthr, float* gain,
                        float* leaves, uint32_t* leaf_idx) {
-    return tree_build_f32(bins, n, F, 1, g, h, edges, n_edges, p, feature, bin, fthr, gain,
-                          NULL, leaves, leaf_idx);
+    return tree_build_f32(bins, n, F, 1, g, h, root, n_root, edges, n_edges, p, feature, bin,
+                          fthr, gain, NULL, leaves, leaf_idx);
 }
 
 int tco_tree_build_f64(const uint8_t* bins, uint32_t n, uint32_t F, const float* g, const float* h,
+                       const uint32_t* root, uint32_t n_root,
                        const float* edges, const uint32_t* n_edges, const tco_tree_params* p,
                        uint16_t* feature, uint8_t* bin, float* fthr, double* gain, double* gap,
                        double* leaves, uint32_t* leaf_idx) {
     double* gd = (double*)malloc(sizeof(double) * (n ? n : 1));
     double* hd = (double*)malloc(sizeof(double) * (n ? n : 1));
     for (uint32_t i = 0; i < n; ++i) { gd[i] = g[i]; hd[i] = h[i]; }
-    const int d = tree_build_f64(bins, n, F, 1, gd, hd, edges, n_edges, p, feature, bin, fthr,
-                                 gain, gap, leaves, leaf_idx);
+    const int d = tree_build_f64(bins, n, F, 1, gd, hd, root, n_root, edges, n_edges, p, feature,
+                                 bin, fthr, gain, gap, leaves, leaf_idx);
     free(gd); free(hd);
     return d;
 }
@@ -302,8 +304,8 @@ int tco_tree_build_mc_f64(const uint8_t* bins, uint32_t n, uint32_t F, uint32_t 
                           const uint32_t* n_edges, const tco_tree_params* p,
                           uint16_t* feature, uint8_t* bin, float* fthr, double* gain,
                           double* gap, double* leaves, uint32_t* leaf_idx) {
-    return tree_build_f64(bins, n, F, K, g, h, edges, n_edges, p, feature, bin, fthr, gain, gap,
-                          leaves, leaf_idx);
+    return tree_build_f64(bins, n, F, K, g, h, NULL, n, edges, n_edges, p, feature, bin, fthr,
+                          gain, gap, leaves, leaf_idx);
 }
 
 /* ---------------------------------------------------------------- A15 - A16 */
@@ -401,7 +403,7 @@ float tco_fit_f32(const uint8_t* bins, uint32_t n, uint32_t F, const float* edge
     for (uint32_t i = 0; i < n; ++i) train_preds[i] = base;
     for (uint32_t t = 0; t < n_trees; ++t) {
         tco_gradients_f32(loss, y, train_preds, g, h, n); /* booster.cpp:505-519 */
-        const int d = tco_tree_build_f32(bins, n, F, g, h, edges, n_edges, p,
+        const int d = tco_tree_build_f32(bins, n, F, g, h, NULL, n, edges, n_edges, p,
                                          feature + (size_t)t * D, bin + (size_t)t * D,
                                          fthr + (size_t)t * D, gain + (size_t)t * D,
                                          leaves + (size_t)t * LP, NULL);

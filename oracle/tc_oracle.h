@@ -82,14 +82,18 @@ typedef struct {
 
 /* A9-A14: one symmetric tree on given gradients (SymmetricTree::build,
  * src/tree/symmetric_tree.cpp:34-368; leaves :753-775).  Returns the depth.
+ * root/n_root: the tree's row list in order (the row sample of
+ * booster.cpp:198-219); NULL = every row.  leaf_idx covers every row.
  * Per level d < depth: feature[d], bin[d], fthr[d] (A11), gain[d]; for the
  * f64 variant also gap[d] = best - second best candidate gain at that level.
  * leaves: 2^depth values.  leaf_idx (optional): per-row MSB-first index. */
 int tco_tree_build_f32(const uint8_t* bins, uint32_t n, uint32_t F, const float* g, const float* h,
+                       const uint32_t* root, uint32_t n_root,
                        const float* edges, const uint32_t* n_edges, const tco_tree_params* p,
                        uint16_t* feature, uint8_t* bin, float* fthr, float* gain,
                        float* leaves, uint32_t* leaf_idx);
 int tco_tree_build_f64(const uint8_t* bins, uint32_t n, uint32_t F, const float* g, const float* h,
+                       const uint32_t* root, uint32_t n_root,
                        const float* edges, const uint32_t* n_edges, const tco_tree_params* p,
                        uint16_t* feature, uint8_t* bin, float* fthr, double* gain, double* gap,
                        double* leaves, uint32_t* leaf_idx);

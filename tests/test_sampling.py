@@ -1,0 +1,65 @@
+"""Row sampling (booster.cpp:198-229, dataset.cpp:552-562) without a GPU.
+
+The library draws each tree's row sample on the host with the reference's
+engine (xorshift64* seeding std::mt19937_64 + std::shuffle).  Pinned two
+ways against the reference's own code (oracle/_ref/libtcref.so):
+  * tc_random_subsample equals Dataset::random_subsample for the same seed;
+  * a whole sampled fit: the oracle, fed the library's per-tree samples,
+    reproduces the reference's sampled fit bit for bit -- which fails if the
+    draw sequence (next_random) or the colsample draw were wrong.
+"""
+import numpy as np
+import pytest
+
+from conftest import binary_data, regression_data
+
+from oracle import LOSS_LOGLOSS, LOSS_MSE, RefLib, tree_params
+
+pytestmark = pytest.mark.skipif(not RefLib.available(), reason="reference harness not built")
+
+
+@pytest.mark.parametrize("n,ratio,seed", [(50, 0.8, 7), (1000, 0.5, 123), (100003, 0.8, 2 ** 63 + 5),
+                                          (10, 0.05, 1), (7, 1.0, 3)])
+def test_random_subsample_matches_reference(n, ratio, seed):
+    from turbocat import capi
+
+    def body():
+        X = np.zeros((n, 1), np.float32)
+        ds = RefLib().dataset(X)
+        return ds.random_subsample(ratio, seed)
+    ref = RefLib.isolated(body)
+    got = capi.random_subsample(n, ratio, seed)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("loss,subsample,colsample,seed", [(LOSS_MSE, 0.8, 0.8, 42),
+                                                           (LOSS_LOGLOSS, 0.5, 1.0, 123)])
+def test_sampled_fit_replays_reference(oracle, loss, subsample, colsample, seed):
+    from turbocat import capi
+    n, F, T, D, lr = 3000, 6, 10, 5, 0.1
+    lam = 0.0 if loss == LOSS_MSE else 1.0
+    X, y = regression_data(n, F, 77) if loss == LOSS_MSE else binary_data(n, F, 77)
+
+    def body():
+        ds = RefLib().dataset(X, y)
+        return ds.fit(loss, T, D, lr, lam, 1.0, subsample=subsample, colsample=colsample, seed=seed)
+    rm, rtp, _ = RefLib.isolated(body)
+    bins, edges, ne = oracle.quantize(X)
+    preds = np.full(n, np.float32(oracle.init_prediction(loss, y)), np.float32)
+    state = seed
+    for t in range(T):
+        s, state = capi.next_random(state)
+        rows = capi.random_subsample(n, subsample, s)
+        if colsample < 1.0:
+            _, state = capi.next_random(state)
+        g, h = oracle.gradients_f32(loss, y, preds)
+        tr = oracle.tree_f32(bins, g, h, edges, ne, tree_params(D, lambda_l2=lam), rows=rows)
+        d = tr["depth"]
+        assert d == rm["depth"][t]
+        assert np.array_equal(tr["feature"], rm["feature"][t * D:t * D + d])
+        assert np.array_equal(tr["bin"], rm["bin"][t * D:t * D + d])
+        assert np.array_equal(tr["gain"], rm["gain"][t * D:t * D + d])
+        assert np.array_equal(tr["leaves"], rm["leaves"][t << D:(t << D) + (1 << d)])
+        li = oracle.leaf_index_binned(bins, tr["feature"], tr["bin"])
+        preds = oracle.fma_update(preds, np.float32(lr), tr["leaves"][li])
+    assert np.array_equal(preds, rtp)

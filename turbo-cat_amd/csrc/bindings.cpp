@@ -4,7 +4,7 @@
 // types and error behaviour (RuntimeError "Model not fitted. Call fit()
 // first." before fit; ValueError for invalid arguments).  Options whose
 // reference code path is not the symmetric-tree hot path (mode="small",
-// GOSS, ordered boosting, row subsampling, categorical features, multiclass,
+// GOSS, ordered boosting, categorical features, multiclass,
 // validation-set early stopping, non-MSE/LogLoss losses) raise ValueError
 // naming the option instead of silently training something else.
 #include <pybind11/numpy.h>

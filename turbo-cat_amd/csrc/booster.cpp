@@ -43,9 +43,8 @@ void Booster::fit(const float* X, const float* y, uint32_t n, uint32_t F) {
   if (p_.max_depth < 1 || p_.max_depth > 10)
     throw std::invalid_argument("max_depth must be in [1, 10] for symmetric trees");
   if (p_.max_bins < 2 || p_.max_bins > 255) throw std::invalid_argument("max_bins must be in [2, 255]");
-  if (p_.subsample < 1.0f)
-    throw std::invalid_argument("subsample < 1 is not supported by the GPU symmetric path; "
-                                "pass subsample=1.0");
+  if (!(p_.subsample > 0.0f && p_.subsample <= 1.0f))
+    throw std::invalid_argument("subsample must be in (0, 1]");
   const auto t0 = std::chrono::steady_clock::now();
   ctx_ = device_context(p_.device);
   drop_ensemble();
@@ -69,6 +68,9 @@ void Booster::fit(const float* X, const float* y, uint32_t n, uint32_t F) {
   tp.tree.min_split_gain = p_.min_split_gain;
   tp.hist_chunk = 0;
   tp.use_graph = p_.use_graph ? 1 : 0;
+  tp.subsample = p_.subsample;
+  tp.colsample_bytree = p_.colsample_bytree;
+  tp.seed = p_.seed;
   const uint32_t T = p_.n_estimators, D = p_.max_depth;
   feat_.assign(static_cast<size_t>(T) * D, 0);
   bin_.assign(static_cast<size_t>(T) * D, 0);

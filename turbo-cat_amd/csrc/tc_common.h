@@ -55,6 +55,7 @@ struct JobDev {
   const float* edges;       // [F][255]
   const uint32_t* n_edges;  // [F]
   uint32_t N, F, Fp, T;     // T = Fp / 16
+  uint32_t n_root;          // rows of the tree's root list (N, or the row sample)
   // hyper-parameters
   int32_t loss;
   uint32_t D, max_bins, min_samples_leaf;
@@ -77,7 +78,7 @@ struct JobDev {
   uint32_t* part_loff;      // exclusive scan of left counts within the node
   double* part_stats;       // [slot][4] = GL, HL, GR, HR
   uint32_t part_cap;
-  // root-sum partials written by the gradient kernel, per workgroup
+  // root-sum partials (sum g, sum h over 1024-row tiles of the root list)
   double* grad_partials;    // [wg][2]
   uint32_t grad_wg_off;     // first workgroup of this job in the row grid
   uint32_t grad_wg_cnt;

@@ -7,6 +7,8 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <numeric>
+#include <random>
 
 #include "tc_device.h"
 #include "tc_internal.h"
@@ -48,6 +50,26 @@ __global__ void k_gradients(int loss, const float* y, const float* p, float* g, 
 
 }  // namespace
 
+// Booster::next_random, xorshift64* (booster.cpp:1219-1225).
+uint64_t next_random(uint64_t& state) {
+  state ^= state >> 12;
+  state ^= state << 25;
+  state ^= state >> 27;
+  return state * 0x2545F4914F6CDD1DULL;
+}
+
+// Dataset::random_subsample (dataset.cpp:552-562): the same engine and the
+// same std::shuffle, so the same rows in the same order as the reference.
+uint32_t random_subsample(uint32_t n, float ratio, uint64_t seed, uint32_t* out) {
+  const uint32_t k = static_cast<uint32_t>(static_cast<float>(n) * ratio);
+  std::vector<uint32_t> idx(n);
+  std::iota(idx.begin(), idx.end(), 0u);
+  std::mt19937_64 rng(seed);
+  std::shuffle(idx.begin(), idx.end(), rng);
+  std::memcpy(out, idx.data(), sizeof(uint32_t) * std::min(k, n));
+  return std::min(k, n);
+}
+
 struct JobSpec {
   const tc_dataset* ds;
   const float* y_host;   // labels (may be null for teacher forcing)
@@ -79,6 +101,31 @@ class Trainer {
   }
   ~Trainer() { release(); }
 
+  // Row sample of the next tree for every sampling job: draw, shuffle on the
+  // host (pinned, double-buffered), copy into rows[0] on the stream.
+  void prepare_samples() {
+    for (uint32_t j = 0; j < njobs_; ++j) {
+      Sampler& S = samplers_[j];
+      if (!S.active && !S.colsample) continue;
+      uint64_t seed = 0;
+      if (S.active) seed = next_random(S.state);
+      if (S.colsample) (void)next_random(S.state);  // feature subset: drawn, unused
+      if (!S.active) continue;
+      const int b = S.next;
+      S.next ^= 1;
+      TC_HIP(hipEventSynchronize(S.ev[b]));
+      random_subsample(h_jobs_[j].N, S.ratio, seed, S.host[b]);
+      TC_HIP(hipMemcpyAsync(h_jobs_[j].rows, S.host[b], 4ull * h_jobs_[j].n_root,
+                            hipMemcpyHostToDevice, ctx_->stream));
+      TC_HIP(hipEventRecord(S.ev[b], ctx_->stream));
+    }
+  }
+
+  bool sampling() const {
+    for (const auto& S : samplers_) if (S.active || S.colsample) return true;
+    return false;
+  }
+
   void init_gradients() {
     hipLaunchKernelGGL(k_grad_update, dim3(total_grad_wg_), dim3(256), 0, ctx_->stream, d_jobs_,
                        d_job_wg_off_, njobs_, 0);
@@ -91,7 +138,7 @@ class Trainer {
     float* dh = dmalloc<float>(N);
     TC_HIP(hipMemcpyAsync(dg, g, sizeof(float) * N, hipMemcpyHostToDevice, ctx_->stream));
     TC_HIP(hipMemcpyAsync(dh, h, sizeof(float) * N, hipMemcpyHostToDevice, ctx_->stream));
-    hipLaunchKernelGGL(k_set_gh, dim3(std::max(1u, ceil_div(N, kGradTile))), dim3(256), 0,
+    hipLaunchKernelGGL(k_set_gh, dim3(std::max(1u, ceil_div(N, 256))), dim3(256), 0,
                        ctx_->stream, d_jobs_, dg, dh);
     TC_HIP(hipGetLastError());
     TC_HIP(hipStreamSynchronize(ctx_->stream));
@@ -104,6 +151,8 @@ class Trainer {
   void enqueue_tree(std::vector<std::pair<hipEvent_t, hipEvent_t>>* hist_events) {
     hipStream_t s = ctx_->stream;
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, lists_);
+    hipLaunchKernelGGL(k_root_partials, dim3(total_grad_wg_), dim3(256), 0, s, d_jobs_,
+                       d_job_wg_off_, njobs_);
     hipLaunchKernelGGL(k_tree_begin, dim3(1, njobs_), dim3(256), 0, s, d_jobs_, lists_);
     for (uint32_t d = 0; d < D_; ++d) {
       const uint32_t pairs = d > 0 ? (1u << (d - 1)) : 0u;
@@ -150,13 +199,20 @@ class Trainer {
       (void)hipGraphDestroy(g);
       TC_HIP(e);
     }
-    for (uint32_t t = 0; t < n; ++t) TC_HIP(hipGraphLaunch(graph_exec_, s));
+    const bool smp = sampling();
+    for (uint32_t t = 0; t < n; ++t) {
+      if (smp) prepare_samples();
+      TC_HIP(hipGraphLaunch(graph_exec_, s));
+    }
   }
 
   // Enqueue n trees without a graph, timing every histogram launch.
   void step_timed(uint32_t n, double* hist_ms, uint32_t* launches) {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> hev;
-    for (uint32_t t = 0; t < n; ++t) enqueue_tree(&hev);
+    for (uint32_t t = 0; t < n; ++t) {
+      if (sampling()) prepare_samples();
+      enqueue_tree(&hev);
+    }
     TC_HIP(hipStreamSynchronize(ctx_->stream));
     double hs = 0.0;
     for (auto& e : hev) {
@@ -186,13 +242,20 @@ class Trainer {
       TC_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
       TC_HIP(hipGraphDestroy(g));
       TC_HIP(hipEventRecord(a, s));
-      for (uint32_t t = 0; t < n_trees_; ++t) TC_HIP(hipGraphLaunch(ge, s));
+      const bool smp = sampling();
+      for (uint32_t t = 0; t < n_trees_; ++t) {
+        if (smp) prepare_samples();
+        TC_HIP(hipGraphLaunch(ge, s));
+      }
       TC_HIP(hipEventRecord(b, s));
       TC_HIP(hipStreamSynchronize(s));
       TC_HIP(hipGraphExecDestroy(ge));
     } else {
       TC_HIP(hipEventRecord(a, s));
-      for (uint32_t t = 0; t < n_trees_; ++t) enqueue_tree(time_hist ? &hev : nullptr);
+      for (uint32_t t = 0; t < n_trees_; ++t) {
+        if (sampling()) prepare_samples();
+        enqueue_tree(time_hist ? &hev : nullptr);
+      }
       TC_HIP(hipEventRecord(b, s));
       TC_HIP(hipStreamSynchronize(s));
     }
@@ -266,6 +329,7 @@ class Trainer {
     hipStream_t s = ctx_->stream;
     h_jobs_.resize(njobs_);
     hist_rows_.resize(njobs_);
+    samplers_.resize(njobs_);
     std::vector<uint32_t> wg_off(njobs_ + 1, 0);
     std::fill(hist_cap_, hist_cap_ + kMaxDepth, 0u);
     std::fill(part_cap_, part_cap_ + kMaxDepth, 0u);
@@ -277,6 +341,21 @@ class Trainer {
       const uint32_t N = ds->N;
       J.bins_rm = ds->bins_rm; J.bins_cm = ds->bins_cm; J.edges = ds->edges; J.n_edges = ds->n_edges;
       J.N = N; J.F = ds->F; J.Fp = ds->Fp; J.T = ds->Fp / kTileF;
+      Sampler& SM = samplers_[j];
+      SM.state = S.p.seed;
+      SM.ratio = S.p.subsample;
+      // 0 (unset) or >= 1 means no sampling
+      SM.active = S.p.subsample > 0.0f && S.p.subsample < 1.0f;
+      SM.colsample = S.p.colsample_bytree > 0.0f && S.p.colsample_bytree < 1.0f;
+      J.n_root = SM.active ? std::min(N, static_cast<uint32_t>(static_cast<float>(N) * S.p.subsample))
+                           : N;
+      if (SM.active) {
+        for (int b = 0; b < 2; ++b) {
+          TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&SM.host[b]), 4ull * std::max(1u, N),
+                               hipHostMallocDefault));
+          TC_HIP(hipEventCreateWithFlags(&SM.ev[b], hipEventDisableTiming));
+        }
+      }
       J.loss = S.p.loss; J.D = D_; J.max_bins = S.p.tree.max_bins;
       J.min_samples_leaf = S.p.tree.min_samples_leaf;
       J.lr = S.p.learning_rate; J.lambda_l2 = S.p.tree.lambda_l2; J.lambda_l1 = S.p.tree.lambda_l1;
@@ -365,7 +444,23 @@ class Trainer {
     graph_exec_ = nullptr;
     for (void* p : allocs_) dfree(p);
     allocs_.clear();
+    for (auto& S : samplers_)
+      for (int b = 0; b < 2; ++b) {
+        if (S.host[b]) (void)hipHostFree(S.host[b]);
+        if (S.ev[b]) (void)hipEventDestroy(S.ev[b]);
+        S.host[b] = nullptr;
+        S.ev[b] = nullptr;
+      }
   }
+
+  struct Sampler {
+    uint64_t state = 42;
+    float ratio = 1.f;
+    bool active = false, colsample = false;
+    uint32_t* host[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int next = 0;
+  };
 
   tc_ctx* ctx_;
   std::vector<JobSpec> specs_;
@@ -379,6 +474,7 @@ class Trainer {
   uint32_t hist_cap_[kMaxDepth] = {}, part_cap_[kMaxDepth] = {};
   std::vector<void*> allocs_;
   hipGraphExec_t graph_exec_ = nullptr;
+  std::vector<Sampler> samplers_;
 };
 
 }  // namespace tc
@@ -412,6 +508,13 @@ void check_params(const tc_train_params* p) {
 using namespace tc;
 
 extern "C" {
+
+int32_t tc_random_subsample(uint32_t n, float ratio, uint64_t seed, uint32_t* out, uint32_t* n_out) {
+  return guard([&] {
+    TC_REQUIRE(out && n_out, "tc_random_subsample: NULL argument");
+    *n_out = random_subsample(n, ratio, seed, out);
+  });
+}
 
 int32_t tc_init_prediction(int32_t loss, const float* y, uint32_t n, float* base) {
   return guard([&] {
@@ -537,6 +640,8 @@ int32_t tc_tree_build(tc_dataset* ds, const float* g, const float* h, const tc_t
     p.n_trees = 1;
     p.learning_rate = 0.f;
     p.tree = *tp;
+    p.subsample = 1.0f;
+    p.colsample_bytree = 1.0f;
     check_params(&p);
     TC_REQUIRE(ds->N > 0, "tc_tree_build: empty dataset");
     JobSpec s{ds, nullptr, p, 0.f};

@@ -7,7 +7,7 @@
 // data-dependent control (level stops, tree count) lives in TreeState and
 // makes later kernels of a stopped job return at once.
 //
-//   reset_counters, tree_begin
+//   reset_counters, root_partials, tree_begin
 //   for level d < D:
 //     [d>0] zero_smaller(d), hist(d, smaller children), subtract(d)
 //     [d=0] hist(0, root)
@@ -56,28 +56,32 @@ __device__ inline uint32_t part_slots(uint32_t len) {
 // Gradients (A6/A7) fused with the training-prediction update (A15 + K9).
 //
 // mode 0: preds = base for every row (before the first tree).
-// mode 1: apply tree st->tree: rows[depth] grouped by leaf, preds[row] +=
-//         lr * leaf (fused multiply-add, as the reference compiler emits
-//         for booster.cpp:460-463), then g/h of the new prediction.
-// Each workgroup covers 1024 positions of one job and leaves its (sum g,
-// sum h) in grad_partials for the next tree's root totals.
+// mode 1: apply tree st->tree to every row: leaf index from the binned
+//         compares go_right = bin != 255 && bin > threshold, MSB first
+//         (SymmetricTree::predict_batch, symmetric_tree.cpp:813-840), then
+//         preds += lr * leaf as one fused multiply-add (the reference
+//         compiler's contraction of booster.cpp:460-463), then g/h.
+// Rows in natural order, 1024 per workgroup, every row of the dataset (also
+// when the tree was grown on a row sample).
 constexpr int kGradTile = 1024;
+
+__device__ inline uint32_t job_of_block(const uint32_t* job_wg_off, uint32_t njobs) {
+  uint32_t lo = 0, hi = njobs;  // last job with job_wg_off[j] <= blockIdx.x
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (job_wg_off[mid] <= blockIdx.x) lo = mid; else hi = mid;
+  }
+  return lo;
+}
 
 __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ jobs,
                                                      const uint32_t* __restrict__ job_wg_off,
                                                      uint32_t njobs, int mode) {
-  __shared__ uint32_t s_off[(1 << kMaxDepth) + 1];
-  __shared__ double s_red[2][4];
   __shared__ uint32_t s_job;
+  __shared__ uint16_t s_f[kMaxDepth];
+  __shared__ uint8_t s_b[kMaxDepth];
   const uint32_t tid = threadIdx.x;
-  if (tid == 0) {
-    uint32_t lo = 0, hi = njobs;  // last job with job_wg_off[j] <= blockIdx.x
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (job_wg_off[mid] <= blockIdx.x) lo = mid; else hi = mid;
-    }
-    s_job = lo;
-  }
+  if (tid == 0) s_job = job_of_block(job_wg_off, njobs);
   __syncthreads();
   const JobDev& J = jobs[s_job];
   const uint32_t lw = blockIdx.x - J.grad_wg_off;
@@ -88,40 +92,58 @@ __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ 
     t = J.st->tree;
     if (t < 0 || t >= static_cast<int>(J.n_trees)) return;
     depth = J.m_depth[t];
-    const uint32_t* offd = J.off + static_cast<size_t>(depth) * ((1u << J.D) + 1u);
-    for (uint32_t i = tid; i <= (1u << depth); i += blockDim.x) s_off[i] = offd[i];
+    if (tid < static_cast<uint32_t>(depth)) {
+      s_f[tid] = J.m_feat[static_cast<size_t>(t) * J.D + tid];
+      s_b[tid] = J.m_bin[static_cast<size_t>(t) * J.D + tid];
+    }
     __syncthreads();
   }
   const float* leaves = J.m_leaves + static_cast<size_t>(t < 0 ? 0 : t) * (1u << J.D);
-  const uint32_t* rows = J.rows + static_cast<size_t>(depth) * N;
-  double sg = 0.0, sh = 0.0;
   for (int k = 0; k < kGradTile / 256; ++k) {
-    const uint32_t p = lw * kGradTile + k * 256 + tid;
-    if (p >= N) break;
-    uint32_t row;
+    const uint32_t row = lw * kGradTile + k * 256 + tid;
+    if (row >= N) break;
     float pred;
     if (mode == 1) {
-      row = rows[p];
-      uint32_t lo = 0, hi = 1u << depth;  // leaf = segment holding p
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (s_off[mid] <= p) lo = mid; else hi = mid;
+      uint32_t li = 0;
+      for (int d = 0; d < depth; ++d) {
+        const uint8_t v = J.bins_cm[static_cast<size_t>(s_f[d]) * N + row];
+        li = (li << 1) | ((v != 255 && v > s_b[d]) ? 1u : 0u);
       }
-      pred = __fmaf_rn(J.lr, leaves[lo], J.preds[row]);
+      pred = __fmaf_rn(J.lr, leaves[li], J.preds[row]);
     } else {
-      row = p;
       pred = J.base;
     }
     J.preds[row] = pred;
-    const float2 gh = row_gradient(J.loss, pred, J.y[row], row < vecN);
-    J.gh[row] = gh;
-    sg += gh.x;
-    sh += gh.y;
+    J.gh[row] = row_gradient(J.loss, pred, J.y[row], row < vecN);
+  }
+}
+
+// Root totals: (sum g, sum h) over 1024-position tiles of the root list
+// rows[0][0 .. n_root), double, fixed order within the tile; k_tree_begin
+// adds the tiles in order.
+__global__ void __launch_bounds__(256) k_root_partials(const JobDev* __restrict__ jobs,
+                                                       const uint32_t* __restrict__ job_wg_off,
+                                                       uint32_t njobs) {
+  __shared__ uint32_t s_job;
+  __shared__ double s_red[2][4];
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) s_job = job_of_block(job_wg_off, njobs);
+  __syncthreads();
+  const JobDev& J = jobs[s_job];
+  const uint32_t lw = blockIdx.x - J.grad_wg_off;
+  double sg = 0.0, sh = 0.0;
+  if (J.st->tree + 1 < static_cast<int>(J.n_trees)) {
+    for (int k = 0; k < kGradTile / 256; ++k) {
+      const uint32_t p = lw * kGradTile + k * 256 + tid;
+      if (p >= J.n_root) break;
+      const float2 v = J.gh[J.rows[p]];
+      sg += v.x;
+      sh += v.y;
+    }
   }
   sg = warp_sum_d(sg);
   sh = warp_sum_d(sh);
-  const uint32_t lane = tid & 63, wv = tid >> 6;
-  if (lane == 0) { s_red[0][wv] = sg; s_red[1][wv] = sh; }
+  if ((tid & 63) == 0) { s_red[0][tid >> 6] = sg; s_red[1][tid >> 6] = sh; }
   __syncthreads();
   if (tid == 0) {
     double a = 0.0, b = 0.0;
@@ -162,11 +184,11 @@ __global__ void __launch_bounds__(256) k_tree_begin(const JobDev* __restrict__ j
       }
       J.ng[0] = static_cast<float>(G);
       J.nh[0] = static_cast<float>(H);
-      J.nc[0] = J.N;
+      J.nc[0] = J.n_root;
       J.off[0] = 0;
-      J.off[1] = J.N;
-      go = (J.N >= 2u * J.min_samples_leaf && J.nh[0] >= J.min_child_weight * 2.0f) ? 1 : 0;
-      if (J.N == 0) go = 0;
+      J.off[1] = J.n_root;
+      go = (J.n_root >= 2u * J.min_samples_leaf && J.nh[0] >= J.min_child_weight * 2.0f) ? 1 : 0;
+      if (J.n_root == 0) go = 0;  // empty sample: depth-0 tree, leaf 0 (:39-43)
     }
     st->active = go;
     s_go = go;
@@ -178,13 +200,13 @@ __global__ void __launch_bounds__(256) k_tree_begin(const JobDev* __restrict__ j
   const uint32_t n4 = J.F * 256u / 2u;
   for (uint32_t i = tid; i < n4; i += blockDim.x) h4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
   if (tid == 0) {
-    emit_hist(J, job, L.hist[0], L.hist_count[0], 0, 0, J.N);
-    const uint32_t ns = part_slots(J.N);
+    emit_hist(J, job, L.hist[0], L.hist_count[0], 0, 0, J.n_root);
+    const uint32_t ns = part_slots(J.n_root);
     const uint32_t base = atomicAdd(L.part_count[0], ns);
     for (uint32_t s = 0; s < ns; ++s) {
       PartWork w;
       w.job = job; w.node = 0; w.start = s * kPartChunk;
-      w.len = min(static_cast<uint32_t>(kPartChunk), J.N - w.start);
+      w.len = min(static_cast<uint32_t>(kPartChunk), J.n_root - w.start);
       w.slot = s; w.pad0 = w.pad1 = w.pad2 = 0;
       L.part[0][base + s] = w;
     }
@@ -601,32 +623,12 @@ __global__ void __launch_bounds__(256) k_leaves(const JobDev* __restrict__ jobs)
   if (threadIdx.x == 0) J.m_depth[t] = static_cast<uint16_t>(depth);
 }
 
-// Copy externally supplied gradients (teacher forcing) into gh and leave
-// root partials as if the gradient kernel had produced them.
+// Copy externally supplied gradients (teacher forcing) into gh.
 __global__ void __launch_bounds__(256) k_set_gh(const JobDev* __restrict__ jobs, const float* g,
                                                 const float* h) {
   const JobDev& J = jobs[0];
-  const uint32_t tid = threadIdx.x;
-  double sg = 0.0, sh = 0.0;
-  for (int k = 0; k < kGradTile / 256; ++k) {
-    const uint32_t p = blockIdx.x * kGradTile + k * 256 + tid;
-    if (p >= J.N) break;
-    const float2 v = make_float2(g[p], h[p]);
-    J.gh[p] = v;
-    sg += v.x;
-    sh += v.y;
-  }
-  __shared__ double s_red[2][4];
-  sg = warp_sum_d(sg);
-  sh = warp_sum_d(sh);
-  if ((tid & 63) == 0) { s_red[0][tid >> 6] = sg; s_red[1][tid >> 6] = sh; }
-  __syncthreads();
-  if (tid == 0) {
-    double a = 0, b = 0;
-    for (int w = 0; w < 4; ++w) { a += s_red[0][w]; b += s_red[1][w]; }
-    J.grad_partials[2 * blockIdx.x] = a;
-    J.grad_partials[2 * blockIdx.x + 1] = b;
-  }
+  const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+  if (p < J.N) J.gh[p] = make_float2(g[p], h[p]);
 }
 
 }  // namespace tc

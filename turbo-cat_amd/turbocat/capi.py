@@ -27,7 +27,8 @@ EXPORTS = [
     "tc_ctx_create", "tc_ctx_destroy", "tc_ctx_synchronize", "tc_last_error",
     "tc_device_alloc", "tc_device_free", "tc_memcpy_h2d", "tc_memcpy_d2h",
     "tc_quantize_fit", "tc_dataset_create", "tc_dataset_bins", "tc_dataset_shape",
-    "tc_dataset_destroy", "tc_gradients", "tc_init_prediction", "tc_hist_build",
+    "tc_dataset_destroy", "tc_gradients", "tc_random_subsample", "tc_init_prediction",
+    "tc_hist_build",
     "tc_hist_subtract", "tc_split_scan", "tc_tree_build", "tc_train", "tc_train_batch",
     "tc_last_train_timing", "tc_trainer_create", "tc_trainer_step", "tc_trainer_step_timed",
     "tc_trainer_sync", "tc_trainer_model", "tc_trainer_hist_rows", "tc_trainer_destroy",
@@ -52,7 +53,8 @@ class TreeParams(Structure):
 
 class TrainParams(Structure):
     _fields_ = [("loss", c_int32), ("n_trees", c_uint32), ("learning_rate", c_float),
-                ("tree", TreeParams), ("hist_chunk", c_uint32), ("use_graph", c_int32)]
+                ("tree", TreeParams), ("hist_chunk", c_uint32), ("use_graph", c_int32),
+                ("subsample", c_float), ("colsample_bytree", c_float), ("seed", c_uint64)]
 
 
 class Model(Structure):
@@ -73,8 +75,26 @@ def tree_params(max_depth=6, max_bins=255, lambda_l2=1.0, lambda_l1=0.0, max_del
 
 
 def train_params(loss=TC_LOSS_MSE, n_trees=100, learning_rate=0.1, hist_chunk=0, use_graph=1,
-                 **tree) -> TrainParams:
-    return TrainParams(loss, n_trees, learning_rate, tree_params(**tree), hist_chunk, use_graph)
+                 subsample=1.0, colsample_bytree=1.0, seed=42, **tree) -> TrainParams:
+    return TrainParams(loss, n_trees, learning_rate, tree_params(**tree), hist_chunk, use_graph,
+                       subsample, colsample_bytree, seed)
+
+
+def random_subsample(n, ratio, seed):
+    """The row sample the trainer draws for one tree (tc_random_subsample)."""
+    out = np.zeros(max(n, 1), np.uint32)
+    k = c_uint32()
+    check(load().tc_random_subsample(n, ratio, seed, _p(out), byref(k)))
+    return out[:k.value]
+
+
+def next_random(state):
+    """xorshift64* step (Booster::next_random); returns (value, new_state)."""
+    m = (1 << 64) - 1
+    state ^= state >> 12
+    state ^= (state << 25) & m
+    state ^= state >> 27
+    return (state * 0x2545F4914F6CDD1D) & m, state
 
 
 _lib = None
@@ -97,6 +117,7 @@ def load(path: str = LIB_PATH):
         "tc_dataset_create": [V, V, U, U, V, V, U, V],
         "tc_dataset_bins": [V, V], "tc_dataset_shape": [V, V, V], "tc_dataset_destroy": [V],
         "tc_gradients": [V, I, V, V, V, V, U], "tc_init_prediction": [I, V, U, V],
+        "tc_random_subsample": [U, c_float, c_uint64, V, V],
         "tc_hist_build": [V, V, V, V, U, V], "tc_hist_subtract": [V, V, V, V, U],
         "tc_split_scan": [V, V, V, V, V, U, U, U, c_float, I, V, V, V],
         "tc_tree_build": [V, V, V, POINTER(TreeParams), V, V, V, V, V, V, V],
