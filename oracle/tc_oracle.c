@@ -349,10 +349,13 @@ void tco_predict_f32(const uint16_t* feature, const float* fthr, const uint16_t*
                 li[k] = float_leaf(row, feature + (size_t)t * D, fthr + (size_t)t * D, depth[t]);
                 if (leaf_idx) leaf_idx[(size_t)i * T + t] = li[k];
             }
-            const float grp = weight[t0] * leaves[(size_t)t0 * LP + li[0]] +
-                              weight[t0 + 1] * leaves[(size_t)(t0 + 1) * LP + li[1]] +
-                              weight[t0 + 2] * leaves[(size_t)(t0 + 2) * LP + li[2]] +
-                              weight[t0 + 3] * leaves[(size_t)(t0 + 3) * LP + li[3]];
+            /* w0*l0 + w1*l1 + w2*l2 + w3*l3 as the reference's compiler
+             * contracts it: w1*l1 rounded, the other products fused onto it
+             * in order (checked against the compiled reference) */
+            float grp = weight[t0 + 1] * leaves[(size_t)(t0 + 1) * LP + li[1]];
+            grp = fmaf(weight[t0], leaves[(size_t)t0 * LP + li[0]], grp);
+            grp = fmaf(weight[t0 + 2], leaves[(size_t)(t0 + 2) * LP + li[2]], grp);
+            grp = fmaf(weight[t0 + 3], leaves[(size_t)(t0 + 3) * LP + li[3]], grp);
             sum = sum + grp;
         }
         for (uint32_t t = TB * 4; t < T; ++t) {

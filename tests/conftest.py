@@ -36,7 +36,8 @@ def regression_data(n, F, seed, low_card_col=None):
     X = rng.standard_normal((n, F)).astype(np.float32)
     if low_card_col is not None:
         X[:, low_card_col] = np.round(X[:, low_card_col] * 3)
-    y = (X[:, 0] + 2 * np.sin(X[:, 1]) + 0.5 * X[:, 2] * X[:, 3]
+    c = [X[:, i % F] for i in range(4)]  # narrow sets (F < 4) reuse columns
+    y = (c[0] + 2 * np.sin(c[1]) + 0.5 * c[2] * c[3]
          + 0.1 * rng.standard_normal(n)).astype(np.float32)
     return X, y
 

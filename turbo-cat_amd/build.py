@@ -25,6 +25,10 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_SOURCES = ["tc_capi.hip", "tc_quantize.hip", "tc_train.hip", "tc_predict.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
+    # no implicit contraction: the kernels fuse exactly where they call __fmaf_rn,
+    # which is where the reference's compiler fuses (bit parity of gradients,
+    # split gains, training and prediction sums)
+    "-ffp-contract=off",
     # keep the compiler from forming packed-FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result",

@@ -8,8 +8,9 @@
 //   rows      uint32 [D+1][N]    per level, row ids grouped by node, each
 //                                 node's rows in ascending order (stable)
 //   off       uint32 [D+1][2^D+1] node segment offsets into rows[level]
-//   hist      float2 [2^D-1][F][256]   per node (heap order: level d node n
-//                                 at (2^d - 1 + n)); slot 255 = NaN bin
+//   hist      HistCell{int64 g, int64 h} [2^D-1][F][256]  fixed-point per
+//                                 node (heap order: level d node n at
+//                                 (2^d - 1 + n)); slot 255 = NaN bin
 //   ng/nh/nc  float/float/uint32 [2^(D+1)-1] node totals, heap order
 #pragma once
 #include <hip/hip_runtime.h>
@@ -31,8 +32,20 @@ struct TreeState {
   int32_t active;       // 1 while levels are still being grown
   int32_t depth;        // splits applied so far in the current tree
   int32_t tree;         // index of the tree being built
-  int32_t pad;
+  uint32_t gmax_next;   // float bits of max |g| of the gradients being written
+  uint32_t hmax_next;   // float bits of max |h| of the gradients being written
+  uint32_t pad;
   unsigned long long best[kMaxDepth];  // argmax key per level
+  // Fixed-point scales of the current tree's histograms: a row's g enters as
+  // round(g * gscale) (int64), h as round(h * hscale).  Powers of two chosen
+  // from max |g| and the root row count so that no sum can overflow.
+  double gscale, hscale;
+  double ginv, hinv;
+};
+
+// Histogram slot in HBM: exact fixed-point sums of one (node, feature, bin).
+struct HistCell {
+  long long g, h;
 };
 
 // One unit of histogram work: rows [start, start+len) of rows[level] for
@@ -56,6 +69,7 @@ struct JobDev {
   const uint32_t* n_edges;  // [F]
   uint32_t N, F, Fp, T;     // T = Fp / 16
   uint32_t n_root;          // rows of the tree's root list (N, or the row sample)
+  uint32_t root_identity;   // 1: the root list is 0..N-1 (no sampling), rows[0] not read
   // hyper-parameters
   int32_t loss;
   uint32_t D, max_bins, min_samples_leaf;
@@ -71,7 +85,7 @@ struct JobDev {
   float* ng;                // heap [2^(D+1) - 1]
   float* nh;
   uint32_t* nc;
-  float2* hist;             // heap [2^D - 1][F][256]
+  HistCell* hist;           // heap [2^D - 1][F][256]
   TreeState* st;
   // partition scratch, per slot (capacity part_cap)
   uint32_t* part_left;      // left count
@@ -95,6 +109,30 @@ struct JobDev {
 
 __host__ __device__ inline uint32_t heap_index(uint32_t level, uint32_t node) {
   return (1u << level) - 1u + node;
+}
+
+// ceil(log2(n)) for n >= 1 (0 for n <= 1).
+__host__ __device__ inline int ceil_log2(uint32_t n) {
+  int b = 0;
+  for (uint32_t v = n > 1u ? n - 1u : 0u; v != 0u; v >>= 1) ++b;
+  return b;
+}
+
+// Exponent e of a histogram's fixed-point scale 2^e for values |v| <= vmax
+// summed over at most n rows.  |round(v * 2^e)| <= 2^(62 - ceil_log2(n)), so
+// any sum of n of them fits int64.  `packed`: also keep a sum over one work
+// item's `chunk` rows inside the signed 48-bit field of the packed cell.
+__host__ __device__ inline int fixed_exponent(float vmax, uint32_t n, uint32_t chunk, bool packed) {
+  int e = 62 - ceil_log2(n);
+  if (packed) {
+    const int ec = 46 - ceil_log2(chunk);
+    e = ec < e ? ec : e;
+  }
+  if (!(vmax > 0.f) || vmax > 3.4e38f) return e < 40 ? e : 40;  // all zero, or not finite
+  int k = 0;
+  (void)frexpf(vmax, &k);  // vmax <= 2^k
+  e -= k;
+  return e < -900 ? -900 : (e > 900 ? 900 : e);
 }
 
 }  // namespace tc

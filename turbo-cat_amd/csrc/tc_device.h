@@ -25,7 +25,8 @@ __device__ inline unsigned long long split_key(float gain, uint32_t feature, uin
 // polynomial of 2^f in Horner form with fused multiply-adds, scaled by
 // 2^round(t) through the exponent bits (src/boosting/loss.cpp:34-69).  The
 // reference compiler fuses the final scale into the sigmoid's 1 + e, so this
-// returns 1 + exp(x) with that single rounding.
+// returns 1 + exp(x) with that single rounding.  t - round(t) is not fused
+// (t has a second use), which needs the build's -ffp-contract=off.
 __device__ inline float one_plus_exp_ref(float x) {
   x = fmaxf(x, -88.0f);
   x = fminf(x, 88.0f);

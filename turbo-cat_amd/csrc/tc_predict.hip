@@ -6,8 +6,8 @@
 // threads are (q, r) = (tree lane, row): in each pass tree lane q evaluates
 // one group of four trees for row r -- leaf index built MSB-first from depth
 // comparisons (v >= threshold, NaN left; fast_float_ensemble.hpp:122-137)
-// -- and the group value w0*l0 + w1*l1 + w2*l2 + w3*l3 (as fused multiply-
-// adds, the reference compiler's contraction).  Then lane 0 of the row adds
+// -- and the group value w0*l0 + w1*l1 + w2*l2 + w3*l3 (as the reference
+// compiler contracts it: w1*l1 rounded, the other three fused onto it).  Then lane 0 of the row adds
 // the groups to the row's sum in tree order, exactly the reference's
 // 8-row-block order (fast_float_ensemble.hpp:480-541): groups added one by
 // one, leftover trees fused in one by one (sum = fma(w, leaf, sum)).  A row's value
@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(256) k_predict(EnsDev E, const void* __restric
   for (uint32_t g0 = 0; g0 < groups; g0 += Q) {
     const uint32_t g = g0 + q;
     if (g < groups && r < nr) {
-      float gv = 0.f;
+      float lv[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const uint32_t t = 4 * g + k;
@@ -90,9 +90,15 @@ __global__ void __launch_bounds__(256) k_predict(EnsDev E, const void* __restric
           li = (li << 1) | right;
         }
         if (leaf_idx) leaf_idx[static_cast<size_t>(row0 + r) * E.T + t] = li;
-        const float lv = E.leaves[static_cast<size_t>(t) * LP + li];
-        gv = (k == 0) ? __fmul_rn(E.weight[t], lv) : __fmaf_rn(E.weight[t], lv, gv);
+        lv[k] = E.leaves[static_cast<size_t>(t) * LP + li];
       }
+      // the reference compiler's contraction of w0*l0 + w1*l1 + w2*l2 + w3*l3:
+      // w1*l1 rounded, then w0*l0, w2*l2, w3*l3 fused onto it in that order
+      const float* w = E.weight + 4 * g;
+      float gv = __fmul_rn(w[1], lv[1]);
+      gv = __fmaf_rn(w[0], lv[0], gv);
+      gv = __fmaf_rn(w[2], lv[2], gv);
+      gv = __fmaf_rn(w[3], lv[3], gv);
       s_gv[q * R + r] = gv;
     }
     __syncthreads();

@@ -19,6 +19,7 @@ namespace tc {
 namespace {
 
 constexpr int kHistUnroll = 4;
+constexpr uint32_t kMaxHistChunk = 32768;
 
 uint32_t ceil_div(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
 uint32_t round8(uint32_t x) { return std::max(8u, (x + 7u) & ~7u); }
@@ -32,7 +33,11 @@ __global__ void k_init_state(TreeState* st) {
   st->active = 0;
   st->depth = 0;
   st->tree = -1;
+  st->gmax_next = 0u;
+  st->hmax_next = 0u;
+  st->pad = 0u;
   for (int d = 0; d < kMaxDepth; ++d) st->best[d] = kEmptyKey;
+  st->gscale = st->hscale = st->ginv = st->hinv = 1.0;
 }
 
 __global__ void k_sub_elementwise(const float* a, const float* b, float* c, uint32_t n) {
@@ -140,6 +145,7 @@ class Trainer {
     TC_HIP(hipMemcpyAsync(dh, h, sizeof(float) * N, hipMemcpyHostToDevice, ctx_->stream));
     hipLaunchKernelGGL(k_set_gh, dim3(std::max(1u, ceil_div(N, 256))), dim3(256), 0,
                        ctx_->stream, d_jobs_, dg, dh);
+    forced_ = true;  // arbitrary h: no packed histogram cells
     TC_HIP(hipGetLastError());
     TC_HIP(hipStreamSynchronize(ctx_->stream));
     dfree(dg);
@@ -153,7 +159,9 @@ class Trainer {
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, lists_);
     hipLaunchKernelGGL(k_root_partials, dim3(total_grad_wg_), dim3(256), 0, s, d_jobs_,
                        d_job_wg_off_, njobs_);
-    hipLaunchKernelGGL(k_tree_begin, dim3(1, njobs_), dim3(256), 0, s, d_jobs_, lists_);
+    const bool packed = packed_hist();
+    hipLaunchKernelGGL(k_tree_begin, dim3(1, njobs_), dim3(256), 0, s, d_jobs_, lists_,
+                       packed ? 1 : 0);
     for (uint32_t d = 0; d < D_; ++d) {
       const uint32_t pairs = d > 0 ? (1u << (d - 1)) : 0u;
       if (d > 0)
@@ -165,15 +173,20 @@ class Trainer {
         TC_HIP(hipEventCreate(&e1));
         TC_HIP(hipEventRecord(e0, s));
       }
-      hipLaunchKernelGGL(k_hist<kHistUnroll>, dim3(round8(hist_cap_[d])), dim3(kHistThreads), 0, s,
-                         d_jobs_, lists_.hist[d], lists_.hist_count[d], d);
+      if (packed)
+        hipLaunchKernelGGL((k_hist<kHistUnroll, true>), dim3(round8(hist_cap_[d])), dim3(kHistThreads),
+                           0, s, d_jobs_, lists_.hist[d], lists_.hist_count[d], d);
+      else
+        hipLaunchKernelGGL((k_hist<kHistUnroll, false>), dim3(round8(hist_cap_[d])), dim3(kHistThreads),
+                           0, s, d_jobs_, lists_.hist[d], lists_.hist_count[d], d);
       if (hist_events) {
         TC_HIP(hipEventRecord(e1, s));
         hist_events->emplace_back(e0, e1);
       }
       if (d > 0)
         hipLaunchKernelGGL(k_subtract, dim3(pairs * fmax_, njobs_), dim3(256), 0, s, d_jobs_, d, fmax_);
-      hipLaunchKernelGGL(k_split_scan, dim3(fmax_, njobs_), dim3(256), 0, s, d_jobs_, d);
+      hipLaunchKernelGGL(k_split_scan<true>, dim3(fmax_, njobs_), dim3(256), 0, s, d_jobs_,
+                         static_cast<const float2*>(nullptr), d);
       hipLaunchKernelGGL(k_finalize, dim3(1, njobs_), dim3(64), 0, s, d_jobs_, d);
       hipLaunchKernelGGL(k_part_count, dim3(part_cap_[d]), dim3(256), 0, s, d_jobs_, lists_.part[d],
                          lists_.part_count[d], d);
@@ -349,6 +362,7 @@ class Trainer {
       SM.colsample = S.p.colsample_bytree > 0.0f && S.p.colsample_bytree < 1.0f;
       J.n_root = SM.active ? std::min(N, static_cast<uint32_t>(static_cast<float>(N) * S.p.subsample))
                            : N;
+      J.root_identity = SM.active ? 0u : 1u;
       if (SM.active) {
         for (int b = 0; b < 2; ++b) {
           TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&SM.host[b]), 4ull * std::max(1u, N),
@@ -366,8 +380,9 @@ class Trainer {
       if (ch == 0) {  // heuristic: ~2048 root work items per job batch
         const uint64_t work = static_cast<uint64_t>(N) * J.T;
         ch = 1024;
-        while (ch < 65536 && work / ch > 2048) ch <<= 1;
+        while (ch < kMaxHistChunk && work / ch > 2048) ch <<= 1;
       }
+      ch = std::min(ch, kMaxHistChunk);  // packed cells count < 2^16 rows per item
       J.hist_chunk = ch;
       float* y = alloc<float>(N);
       if (S.y_host) TC_HIP(hipMemcpyAsync(y, S.y_host, 4ull * N, hipMemcpyHostToDevice, s));
@@ -382,7 +397,7 @@ class Trainer {
       J.ng = alloc<float>(nheap);
       J.nh = alloc<float>(nheap);
       J.nc = alloc<uint32_t>(nheap);
-      J.hist = alloc<float2>(((1ull << D_) - 1) * J.F * 256);
+      J.hist = alloc<HistCell>(((1ull << D_) - 1) * J.F * 256);
       J.st = alloc<TreeState>(1);
       hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1), 0, s, J.st);
       J.part_cap = ceil_div(N, kPartChunk) + (1u << D_);
@@ -475,6 +490,14 @@ class Trainer {
   std::vector<void*> allocs_;
   hipGraphExec_t graph_exec_ = nullptr;
   std::vector<Sampler> samplers_;
+  bool forced_ = false;
+
+  // Packed MSE histogram cells need h == 2 on every row of every job.
+  bool packed_hist() const {
+    if (forced_) return false;
+    for (const auto& S : specs_) if (S.p.loss != TC_LOSS_MSE) return false;
+    return true;
+  }
 };
 
 }  // namespace tc
@@ -688,13 +711,25 @@ int32_t tc_hist_build(tc_dataset* ds, const float* g, const float* h, const uint
       TC_HIP(hipMemcpyAsync(J.gh, ghh.data(), 8ull * N, hipMemcpyHostToDevice, s));
       J.rows = static_cast<uint32_t*>(al(4ull * std::max(1u, nr)));
       if (nr) TC_HIP(hipMemcpyAsync(J.rows, r.data(), 4ull * nr, hipMemcpyHostToDevice, s));
-      J.hist = static_cast<float2*>(al(8ull * F * 256));
-      TC_HIP(hipMemsetAsync(J.hist, 0, 8ull * F * 256, s));
+      J.hist = static_cast<HistCell*>(al(sizeof(HistCell) * F * 256));
+      TC_HIP(hipMemsetAsync(J.hist, 0, sizeof(HistCell) * F * 256, s));
+      float2* fh = static_cast<float2*>(al(sizeof(float2) * F * 256));
+      // fixed-point scales from the selected rows, as k_tree_begin sets them
+      float gmax = 0.f, hmax = 0.f;
+      for (uint32_t i = 0; i < nr; ++i) {
+        gmax = std::max(gmax, std::fabs(g[r[i]]));
+        hmax = std::max(hmax, std::fabs(h[r[i]]));
+      }
+      const uint32_t ch = 2048;
+      const int eg = fixed_exponent(gmax, nr, ch, false), eh = fixed_exponent(hmax, nr, ch, false);
       TreeState st{};
       st.active = 1;
+      st.gscale = std::ldexp(1.0, eg);
+      st.ginv = std::ldexp(1.0, -eg);
+      st.hscale = std::ldexp(1.0, eh);
+      st.hinv = std::ldexp(1.0, -eh);
       J.st = static_cast<TreeState*>(al(sizeof(TreeState)));
       TC_HIP(hipMemcpyAsync(J.st, &st, sizeof(st), hipMemcpyHostToDevice, s));
-      const uint32_t ch = 2048;
       std::vector<HistWork> wl;
       for (uint32_t s0 = 0; s0 < nr; s0 += ch)
         for (uint32_t t = 0; t < J.T; ++t) {
@@ -710,9 +745,12 @@ int32_t tc_hist_build(tc_dataset* ds, const float* g, const float* h, const uint
       JobDev* dj = static_cast<JobDev*>(al(sizeof(JobDev)));
       TC_HIP(hipMemcpyAsync(dj, &J, sizeof(J), hipMemcpyHostToDevice, s));
       if (cnt)
-        hipLaunchKernelGGL(k_hist<4>, dim3(round8(cnt)), dim3(kHistThreads), 0, s, dj, dw, dc, 0u);
+        hipLaunchKernelGGL((k_hist<4, false>), dim3(round8(cnt)), dim3(kHistThreads), 0, s, dj, dw,
+                           dc, 0u);
+      hipLaunchKernelGGL(k_hist_to_float, dim3(ceil_div(F * 256ull, 256)), dim3(256), 0, s, J.hist,
+                         J.st, F * 256u, fh);
       TC_HIP(hipGetLastError());
-      TC_HIP(hipMemcpyAsync(hist_out, J.hist, 8ull * F * 256, hipMemcpyDeviceToHost, s));
+      TC_HIP(hipMemcpyAsync(hist_out, fh, sizeof(float2) * F * 256, hipMemcpyDeviceToHost, s));
       TC_HIP(hipStreamSynchronize(s));
     } catch (...) {
       for (void* p : owned) dfree(p);
@@ -767,8 +805,8 @@ int32_t tc_split_scan(tc_ctx* ctx, const float* hist, const float* node_g, const
       JobDev J;
       std::memset(&J, 0, sizeof(J));
       J.F = F; J.max_bins = max_bins; J.lambda_l2 = lambda_l2; J.D = level + 1;
-      J.hist = static_cast<float2*>(al(hb * nheap));
-      TC_HIP(hipMemcpyAsync(J.hist + ((1ull << level) - 1) * F * 256, hist, hb * n_nodes,
+      float2* fh = static_cast<float2*>(al(hb * nheap));
+      TC_HIP(hipMemcpyAsync(fh + ((1ull << level) - 1) * F * 256, hist, hb * n_nodes,
                             hipMemcpyHostToDevice, s));
       J.ng = static_cast<float*>(al(4 * nheap));
       J.nh = static_cast<float*>(al(4 * nheap));
@@ -783,7 +821,8 @@ int32_t tc_split_scan(tc_ctx* ctx, const float* hist, const float* node_g, const
       TC_HIP(hipMemcpyAsync(J.st, &st, sizeof(st), hipMemcpyHostToDevice, s));
       JobDev* dj = static_cast<JobDev*>(al(sizeof(JobDev)));
       TC_HIP(hipMemcpyAsync(dj, &J, sizeof(J), hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(k_split_scan, dim3(F, 1), dim3(256), 0, s, dj, level);
+      hipLaunchKernelGGL(k_split_scan<false>, dim3(F, 1), dim3(256), 0, s, dj,
+                         static_cast<const float2*>(fh), level);
       TC_HIP(hipGetLastError());
       TC_HIP(hipMemcpyAsync(&st, J.st, sizeof(st), hipMemcpyDeviceToHost, s));
       TC_HIP(hipStreamSynchronize(s));

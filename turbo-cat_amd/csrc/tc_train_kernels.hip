@@ -14,6 +14,14 @@
 //     split_scan(d), finalize(d)
 //     part_count(d), part_plan(d), part_scatter(d)
 //   leaves, grad_update(apply)
+//
+// Histograms are exact fixed-point sums: a row's g and h enter as int64
+// round(v * 2^e), with e chosen per tree from max |g|, max |h| and the row
+// count so that no sum overflows.  They are accumulated in LDS with integer
+// atomics (on gfx950 about 20x the rate of LDS float atomics) and merged into
+// HBM with int64 atomics.  Integer addition is order-independent, so every
+// histogram, and every split, is the same run to run and the same for a job
+// trained alone or in a batch.
 #include "tc_device.h"
 
 namespace tc {
@@ -25,31 +33,83 @@ struct Lists {
   uint32_t* part_count[kMaxDepth];
 };
 
-// ---------------------------------------------------------------------------
-// Work-list helpers (called by one workgroup per job)
-
-// Append the histogram work for rows [start, start+len) of `node` (all
-// feature tiles of each row chunk, contiguous so the tiles share an XCD).
-__device__ inline void emit_hist(const JobDev& J, uint32_t job, HistWork* list, uint32_t* count,
-                                 uint32_t node, uint32_t start, uint32_t len) {
-  if (len == 0) return;
-  if (J.hist_rows_total) atomicAdd(J.hist_rows_total, static_cast<unsigned long long>(len));
-  const uint32_t chunks = (len + J.hist_chunk - 1) / J.hist_chunk;
-  const uint32_t base = atomicAdd(count, chunks * J.T);
-  for (uint32_t c = 0; c < chunks; ++c) {
-    const uint32_t s = start + c * J.hist_chunk;
-    const uint32_t l = min(J.hist_chunk, start + len - s);
-    for (uint32_t t = 0; t < J.T; ++t) {
-      HistWork w;
-      w.job = job; w.node = node; w.start = s; w.len = l; w.tile = t;
-      w.pad0 = w.pad1 = w.pad2 = 0;
-      list[base + c * J.T + t] = w;
-    }
-  }
-}
-
 __device__ inline uint32_t part_slots(uint32_t len) {
   return (len + kPartChunk - 1) / kPartChunk;
+}
+
+// In-place exclusive scan of a[0..n) (LDS, n <= 1024) by a 256-thread
+// workgroup; returns the total to every thread.  tmp: 256 words of LDS.
+__device__ inline uint32_t block_scan_excl(uint32_t* a, uint32_t n, uint32_t* tmp) {
+  const uint32_t tid = threadIdx.x, k0 = tid * 4;
+  uint32_t loc[4], sum = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t v = (k0 + i < n) ? a[k0 + i] : 0u;
+    loc[i] = sum;
+    sum += v;
+  }
+  tmp[tid] = sum;
+  __syncthreads();
+  for (uint32_t off = 1; off < 256; off <<= 1) {
+    const uint32_t v = tid >= off ? tmp[tid - off] : 0u;
+    __syncthreads();
+    tmp[tid] += v;
+    __syncthreads();
+  }
+  const uint32_t excl = tmp[tid] - sum, total = tmp[255];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (k0 + i < n) a[k0 + i] = excl + loc[i];
+  __syncthreads();
+  return total;
+}
+
+// Item owning element e of a scanned list: the last i in [0, n) with
+// off[i] <= e (off non-decreasing, e < total).
+__device__ inline uint32_t find_item(const uint32_t* off, uint32_t n, uint32_t e) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= e) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ inline void put_hist_work(HistWork* dst, uint32_t job, uint32_t node, uint32_t start,
+                                     uint32_t len, uint32_t tile) {
+  HistWork w;
+  w.job = job; w.node = node; w.start = start; w.len = len; w.tile = tile;
+  w.pad0 = w.pad1 = w.pad2 = 0;
+  *dst = w;
+}
+
+__device__ inline void put_part_work(PartWork* dst, uint32_t job, uint32_t node, uint32_t start,
+                                     uint32_t len, uint32_t slot) {
+  PartWork w;
+  w.job = job; w.node = node; w.start = start; w.len = len; w.slot = slot;
+  w.pad0 = w.pad1 = w.pad2 = 0;
+  *dst = w;
+}
+
+// Workgroup max of two non-negative floats, merged into st (float bits of
+// non-negative floats order like the floats).
+__device__ inline void block_max2_to(TreeState* st, float a, float b) {
+  __shared__ uint32_t s_m[2];
+  if (threadIdx.x < 2) s_m[threadIdx.x] = 0u;
+  __syncthreads();
+  for (int o = 32; o > 0; o >>= 1) {
+    a = fmaxf(a, __shfl_down(a, o, 64));
+    b = fmaxf(b, __shfl_down(b, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&s_m[0], __float_as_uint(a));
+    atomicMax(&s_m[1], __float_as_uint(b));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_m[0]) atomicMax(&st->gmax_next, s_m[0]);
+    if (s_m[1]) atomicMax(&st->hmax_next, s_m[1]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -62,7 +122,8 @@ __device__ inline uint32_t part_slots(uint32_t len) {
 //         preds += lr * leaf as one fused multiply-add (the reference
 //         compiler's contraction of booster.cpp:460-463), then g/h.
 // Rows in natural order, 1024 per workgroup, every row of the dataset (also
-// when the tree was grown on a row sample).
+// when the tree was grown on a row sample).  Also records max |g|, max |h|
+// for the next tree's histogram scales.
 constexpr int kGradTile = 1024;
 
 __device__ inline uint32_t job_of_block(const uint32_t* job_wg_off, uint32_t njobs) {
@@ -99,6 +160,7 @@ __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ 
     __syncthreads();
   }
   const float* leaves = J.m_leaves + static_cast<size_t>(t < 0 ? 0 : t) * (1u << J.D);
+  float gmax = 0.f, hmax = 0.f;
   for (int k = 0; k < kGradTile / 256; ++k) {
     const uint32_t row = lw * kGradTile + k * 256 + tid;
     if (row >= N) break;
@@ -114,8 +176,12 @@ __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ 
       pred = J.base;
     }
     J.preds[row] = pred;
-    J.gh[row] = row_gradient(J.loss, pred, J.y[row], row < vecN);
+    const float2 gh = row_gradient(J.loss, pred, J.y[row], row < vecN);
+    J.gh[row] = gh;
+    gmax = fmaxf(gmax, fabsf(gh.x));
+    hmax = fmaxf(hmax, fabsf(gh.y));
   }
+  block_max2_to(J.st, gmax, hmax);
 }
 
 // Root totals: (sum g, sum h) over 1024-position tiles of the root list
@@ -136,7 +202,7 @@ __global__ void __launch_bounds__(256) k_root_partials(const JobDev* __restrict_
     for (int k = 0; k < kGradTile / 256; ++k) {
       const uint32_t p = lw * kGradTile + k * 256 + tid;
       if (p >= J.n_root) break;
-      const float2 v = J.gh[J.rows[p]];
+      const float2 v = J.gh[J.root_identity ? p : J.rows[p]];
       sg += v.x;
       sh += v.y;
     }
@@ -162,19 +228,28 @@ __global__ void k_reset_counters(Lists L) {
   }
 }
 
-// Root of a new tree (symmetric_tree.cpp:97-124): totals from the gradient
-// partials (in workgroup order, double), level-0 stop test, root work.
-__global__ void __launch_bounds__(256) k_tree_begin(const JobDev* __restrict__ jobs, Lists L) {
+// Root of a new tree (symmetric_tree.cpp:97-124): totals from the root
+// partials (in workgroup order, double), level-0 stop test, the tree's
+// histogram scales, zeroed root histogram, root work lists.
+// packed: histograms use the packed MSE cell (h == 2 for every row).
+__global__ void __launch_bounds__(256) k_tree_begin(const JobDev* __restrict__ jobs, Lists L,
+                                                    int packed) {
   const uint32_t job = blockIdx.y;
   const JobDev& J = jobs[job];
   const uint32_t tid = threadIdx.x;
   __shared__ int s_go;
+  __shared__ uint32_t s_hbase, s_pbase;
+  const uint32_t n = J.n_root;
+  const uint32_t hcnt = ((n + J.hist_chunk - 1) / J.hist_chunk) * J.T, pcnt = part_slots(n);
   if (tid == 0) {
     TreeState* st = J.st;
     const int t = st->tree + 1;
     st->tree = t;
     st->depth = 0;
     for (int d = 0; d < kMaxDepth; ++d) st->best[d] = kEmptyKey;
+    const float gmax = __uint_as_float(st->gmax_next), hmax = __uint_as_float(st->hmax_next);
+    st->gmax_next = 0u;
+    st->hmax_next = 0u;
     int go = 0;
     if (t < static_cast<int>(J.n_trees)) {
       double G = 0.0, H = 0.0;
@@ -184,64 +259,85 @@ __global__ void __launch_bounds__(256) k_tree_begin(const JobDev* __restrict__ j
       }
       J.ng[0] = static_cast<float>(G);
       J.nh[0] = static_cast<float>(H);
-      J.nc[0] = J.n_root;
+      J.nc[0] = n;
       J.off[0] = 0;
-      J.off[1] = J.n_root;
-      go = (J.n_root >= 2u * J.min_samples_leaf && J.nh[0] >= J.min_child_weight * 2.0f) ? 1 : 0;
-      if (J.n_root == 0) go = 0;  // empty sample: depth-0 tree, leaf 0 (:39-43)
+      J.off[1] = n;
+      go = (n >= 2u * J.min_samples_leaf && J.nh[0] >= J.min_child_weight * 2.0f) ? 1 : 0;
+      if (n == 0) go = 0;  // empty sample: depth-0 tree, leaf 0 (:39-43)
+      const int eg = fixed_exponent(gmax, n, J.hist_chunk, packed != 0);
+      const int eh = fixed_exponent(hmax, n, J.hist_chunk, false);
+      st->gscale = ldexp(1.0, eg);
+      st->ginv = ldexp(1.0, -eg);
+      st->hscale = ldexp(1.0, eh);
+      st->hinv = ldexp(1.0, -eh);
+      if (go) {
+        s_hbase = atomicAdd(L.hist_count[0], hcnt);
+        s_pbase = atomicAdd(L.part_count[0], pcnt);
+        if (J.hist_rows_total) atomicAdd(J.hist_rows_total, static_cast<unsigned long long>(n));
+      }
     }
     st->active = go;
     s_go = go;
   }
   __syncthreads();
   if (!s_go) return;
-  // zero the root histogram
-  float4* h4 = reinterpret_cast<float4*>(J.hist);
-  const uint32_t n4 = J.F * 256u / 2u;
-  for (uint32_t i = tid; i < n4; i += blockDim.x) h4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (tid == 0) {
-    emit_hist(J, job, L.hist[0], L.hist_count[0], 0, 0, J.n_root);
-    const uint32_t ns = part_slots(J.n_root);
-    const uint32_t base = atomicAdd(L.part_count[0], ns);
-    for (uint32_t s = 0; s < ns; ++s) {
-      PartWork w;
-      w.job = job; w.node = 0; w.start = s * kPartChunk;
-      w.len = min(static_cast<uint32_t>(kPartChunk), J.n_root - w.start);
-      w.slot = s; w.pad0 = w.pad1 = w.pad2 = 0;
-      L.part[0][base + s] = w;
-    }
+  HistCell* h = J.hist;
+  for (uint32_t i = tid; i < J.F * 256u; i += blockDim.x) h[i] = HistCell{0, 0};
+  // chunk-major: the tiles of one row chunk are adjacent (and share an XCD)
+  for (uint32_t e = tid; e < hcnt; e += blockDim.x) {
+    const uint32_t start = (e / J.T) * J.hist_chunk;
+    put_hist_work(L.hist[0] + s_hbase + e, job, 0, start, min(J.hist_chunk, n - start), e % J.T);
+  }
+  for (uint32_t e = tid; e < pcnt; e += blockDim.x) {
+    const uint32_t start = e * kPartChunk;
+    put_part_work(L.part[0] + s_pbase + e, job, 0, start,
+                  min(static_cast<uint32_t>(kPartChunk), n - start), e);
   }
 }
 
 // ---------------------------------------------------------------------------
 // A9: per-level histogram.  One workgroup = one work item (row chunk of one
 // node) x one 16-feature tile.  Each lane loads the tile's 16 bins of a row
-// with one 16-byte load from the row-major bins, its (g, h) pair, and adds
-// them into a workgroup histogram in LDS (g plane and h plane, 257-float
-// feature stride).  Lane l starts at feature l % 16 so that the 64 lanes of
-// one LDS atomic instruction spread over 16 feature histograms.  At the end
-// the non-empty bins are added to the node's histogram in HBM.
-template <int U>
+// with one 16-byte load from the row-major bins and the row's (g, h), turns
+// them into fixed point once, and adds them into the workgroup's LDS
+// histogram with 64-bit integer atomics (feature stride 257 cells).  Lane l
+// starts at feature l % 16, so the 64 lanes of one atomic instruction spread
+// over the 16 feature histograms.
+//
+// PACKED (MSE, h == 2 for every row): one atomic per value on a cell holding
+// g * 2^16 + 1 -- a signed 48-bit g sum above a 16-bit row count (a work
+// item has fewer than 2^16 rows, so the count never carries); h = 2 * count.
+// Otherwise one atomic on the g plane and one on the h plane.
+//
+// The non-empty cells are added into the node's HBM histogram with int64
+// atomics.
+template <int U, bool PACKED>
 __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict__ jobs,
                                                        const HistWork* __restrict__ work,
                                                        const uint32_t* __restrict__ count,
                                                        uint32_t level) {
-  __shared__ float s_g[kTileF * 257];
-  __shared__ float s_h[kTileF * 257];
+  constexpr int kCells = kTileF * 257;
+  __shared__ unsigned long long s_a[kCells];
+  __shared__ unsigned long long s_b[PACKED ? 1 : kCells];
   uint32_t li;
   if (!xcd_rank(blockIdx.x, *count, &li)) return;
   const HistWork w = work[li];
   const JobDev& J = jobs[w.job];
   if (!J.st->active) return;
   const uint32_t tid = threadIdx.x;
-  for (uint32_t i = tid; i < kTileF * 257; i += kHistThreads) { s_g[i] = 0.f; s_h[i] = 0.f; }
+  for (uint32_t i = tid; i < kCells; i += kHistThreads) {
+    s_a[i] = 0ull;
+    if (!PACKED) s_b[i] = 0ull;
+  }
+  const double gsc = J.st->gscale, hsc = J.st->hscale;
   __syncthreads();
 
   const uint8_t* __restrict__ brm = J.bins_rm + w.tile * kTileF;
   const uint32_t* __restrict__ rows = J.rows + static_cast<size_t>(level) * J.N + w.start;
+  const bool direct = level == 0 && J.root_identity;  // row = position, no list read
   const float2* __restrict__ gh = J.gh;
   const uint32_t Fp = J.Fp;
-  const uint32_t s = tid & 15u;        // this lane's starting feature
+  const uint32_t s = tid & 15u;  // this lane's starting feature
   const uint32_t q = s >> 2, r = s & 3u;
 
   for (uint32_t j0 = 0; j0 < w.len; j0 += kHistThreads * U) {
@@ -252,7 +348,7 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
     for (int u = 0; u < U; ++u) {
       const uint32_t j = j0 + u * kHistThreads + tid;
       ok[u] = j < w.len;
-      const uint32_t row = ok[u] ? rows[j] : 0u;
+      const uint32_t row = ok[u] ? (direct ? w.start + j : rows[j]) : 0u;
       v[u] = ok[u] ? *reinterpret_cast<const uint4*>(brm + static_cast<size_t>(row) * Fp)
                    : make_uint4(0, 0, 0, 0);
       g[u] = ok[u] ? gh[row] : make_float2(0.f, 0.f);
@@ -260,6 +356,12 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (!ok[u]) continue;
+      const long long gq = __double2ll_rn(static_cast<double>(g[u].x) * gsc);
+      const unsigned long long va = PACKED ? (static_cast<unsigned long long>(gq) << 16) + 1ull
+                                           : static_cast<unsigned long long>(gq);
+      const unsigned long long vb =
+          PACKED ? 0ull
+                 : static_cast<unsigned long long>(__double2ll_rn(static_cast<double>(g[u].y) * hsc));
       // rotate the 16 bytes by s: byte k of wr = feature (k + s) % 16
       const uint32_t a0 = v[u].x, a1 = v[u].y, a2 = v[u].z, a3 = v[u].w;
       const uint32_t d0 = q == 0 ? a0 : q == 1 ? a1 : q == 2 ? a2 : a3;
@@ -275,21 +377,30 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
       for (int k = 0; k < 16; ++k) {
         const uint32_t f = (static_cast<uint32_t>(k) + s) & 15u;
         const uint32_t b = (wr[k >> 2] >> (8 * (k & 3))) & 0xFFu;
-        atomicAdd(&s_g[f * 257 + b], g[u].x);
-        atomicAdd(&s_h[f * 257 + b], g[u].y);
+        atomicAdd(&s_a[f * 257 + b], va);
+        if (!PACKED) atomicAdd(&s_b[f * 257 + b], vb);
       }
     }
   }
   __syncthreads();
   const uint32_t f0 = w.tile * kTileF;
-  float2* out = J.hist + (static_cast<size_t>(heap_index(level, w.node)) * J.F + f0) * 256u;
+  HistCell* out = J.hist + (static_cast<size_t>(heap_index(level, w.node)) * J.F + f0) * 256u;
   const uint32_t nf = min(static_cast<uint32_t>(kTileF), J.F - f0);
+  const long long h2 = PACKED ? __double2ll_rn(2.0 * hsc) : 0;
   for (uint32_t i = tid; i < nf * 256u; i += kHistThreads) {
     const uint32_t f = i >> 8, b = i & 255u;
-    const float hg = s_g[f * 257 + b], hh = s_h[f * 257 + b];
-    if (hh != 0.f || hg != 0.f) {
-      unsafeAtomicAdd(&out[i].x, hg);
-      unsafeAtomicAdd(&out[i].y, hh);
+    const unsigned long long a = s_a[f * 257 + b];
+    long long gs, hs;
+    if (PACKED) {
+      gs = static_cast<long long>(a) >> 16;
+      hs = static_cast<long long>(a & 0xFFFFull) * h2;
+    } else {
+      gs = static_cast<long long>(a);
+      hs = static_cast<long long>(s_b[f * 257 + b]);
+    }
+    if (gs != 0 || hs != 0) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(&out[i].g), static_cast<unsigned long long>(gs));
+      atomicAdd(reinterpret_cast<unsigned long long*>(&out[i].h), static_cast<unsigned long long>(hs));
     }
   }
 }
@@ -304,12 +415,12 @@ __global__ void __launch_bounds__(256) k_zero_smaller(const JobDev* __restrict__
   const uint32_t lc = 2 * pair, rc = lc + 1;
   const uint32_t nl = J.nc[heap_index(level, lc)], nr = J.nc[heap_index(level, rc)];
   const uint32_t sm = nl <= nr ? lc : rc;
-  float2* h = J.hist + (static_cast<size_t>(heap_index(level, sm)) * J.F + f) * 256u;
-  h[threadIdx.x] = make_float2(0.f, 0.f);
+  J.hist[(static_cast<size_t>(heap_index(level, sm)) * J.F + f) * 256u + threadIdx.x] = HistCell{0, 0};
 }
 
 // A13: larger child = parent - smaller child, over every (feature, slot)
-// (Histogram::subtract_from, histogram.cpp:39-42 via symmetric_tree.cpp:346).
+// (Histogram::subtract_from, histogram.cpp:39-42 via symmetric_tree.cpp:346),
+// exact in fixed point.
 __global__ void __launch_bounds__(256) k_subtract(const JobDev* __restrict__ jobs,
                                                   uint32_t level, uint32_t fmax) {
   const JobDev& J = jobs[blockIdx.y];
@@ -320,20 +431,24 @@ __global__ void __launch_bounds__(256) k_subtract(const JobDev* __restrict__ job
   const uint32_t nl = J.nc[heap_index(level, lc)], nr = J.nc[heap_index(level, rc)];
   const uint32_t sm = nl <= nr ? lc : rc, lg = nl <= nr ? rc : lc;
   const size_t F = J.F;
-  const float2 p = J.hist[(heap_index(level - 1, pair) * F + f) * 256u + threadIdx.x];
-  const float2 c = J.hist[(heap_index(level, sm) * F + f) * 256u + threadIdx.x];
-  J.hist[(heap_index(level, lg) * F + f) * 256u + threadIdx.x] =
-      make_float2(__fsub_rn(p.x, c.x), __fsub_rn(p.y, c.y));
+  const HistCell p = J.hist[(heap_index(level - 1, pair) * F + f) * 256u + threadIdx.x];
+  const HistCell c = J.hist[(heap_index(level, sm) * F + f) * 256u + threadIdx.x];
+  J.hist[(heap_index(level, lg) * F + f) * 256u + threadIdx.x] = HistCell{p.g - c.g, p.h - c.h};
 }
 
 // ---------------------------------------------------------------------------
 // A10: level split scan.  Workgroup = (feature, job), thread = candidate bin.
-// Per node the left sums are the sequential prefix over bins (one lane per
-// node, in LDS), so for a given histogram every candidate's left sum, gain
-// term and node-order total is the reference's float value
-// (symmetric_tree.cpp:153-193, :214-262, with the deeper levels' total +=
-// 0.5 * (...) as the fused multiply-add its compiler emits).
+// The bins of up to 32 nodes are staged in LDS as floats (all lanes), then
+// one lane per node forms the sequential float prefix over bins, so for a
+// given float histogram every candidate's left sum, gain term and node-order
+// total is the reference's float value (symmetric_tree.cpp:153-193,
+// :214-262, with the deeper levels' total += 0.5 * (...) as the fused
+// multiply-add its compiler emits).  FIXED: the histogram is fixed-point
+// cells (bin value = float of the exact sum); otherwise float pairs (the
+// tc_split_scan entry point).
+template <bool FIXED>
 __global__ void __launch_bounds__(256) k_split_scan(const JobDev* __restrict__ jobs,
+                                                    const float2* __restrict__ fhist,
                                                     uint32_t level) {
   __shared__ float2 s_pref[kScanNodesPerPass][256];
   __shared__ unsigned long long s_red[4];
@@ -343,15 +458,28 @@ __global__ void __launch_bounds__(256) k_split_scan(const JobDev* __restrict__ j
   const uint32_t nn = 1u << level;
   const uint32_t mb = J.max_bins & 0xFFu;
   const float lambda = J.lambda_l2;
+  const double gi = FIXED ? J.st->ginv : 1.0, hi = FIXED ? J.st->hinv : 1.0;
   float total = 0.f;
   bool any = false;
   for (uint32_t c0 = 0; c0 < nn; c0 += kScanNodesPerPass) {
     const uint32_t cs = min(static_cast<uint32_t>(kScanNodesPerPass), nn - c0);
+    for (uint32_t k = 0; k < cs; ++k) {
+      const size_t cell = (static_cast<size_t>(heap_index(level, c0 + k)) * J.F + f) * 256u + tid;
+      float2 v;
+      if (FIXED) {
+        const HistCell hc = J.hist[cell];
+        v = make_float2(static_cast<float>(static_cast<double>(hc.g) * gi),
+                        static_cast<float>(static_cast<double>(hc.h) * hi));
+      } else {
+        v = fhist[cell];
+      }
+      s_pref[k][tid] = v;
+    }
+    __syncthreads();
     if (tid < cs) {
-      const float2* hp = J.hist + (static_cast<size_t>(heap_index(level, c0 + tid)) * J.F + f) * 256u;
       float lg = 0.f, lh = 0.f;
       for (uint32_t b = 0; b + 1 < mb; ++b) {
-        const float2 v = hp[b];
+        const float2 v = s_pref[tid][b];
         lg = __fadd_rn(lg, v.x);
         lh = __fadd_rn(lh, v.y);
         s_pref[tid][b] = make_float2(lg, lh);
@@ -422,6 +550,10 @@ __device__ inline bool part_active(const JobDev& J, uint32_t level) {
   return J.st->depth == static_cast<int32_t>(level) + 1 && J.st->tree < static_cast<int>(J.n_trees);
 }
 
+__device__ inline uint32_t level_row(const JobDev& J, uint32_t level, uint32_t pos) {
+  return (level == 0 && J.root_identity) ? pos : J.rows[static_cast<size_t>(level) * J.N + pos];
+}
+
 __global__ void __launch_bounds__(256) k_part_count(const JobDev* __restrict__ jobs,
                                                     const PartWork* __restrict__ work,
                                                     const uint32_t* __restrict__ count,
@@ -433,12 +565,11 @@ __global__ void __launch_bounds__(256) k_part_count(const JobDev* __restrict__ j
   const size_t o = static_cast<size_t>(J.st->tree) * J.D + level;
   const uint8_t* col = J.bins_cm + static_cast<size_t>(J.m_feat[o]) * J.N;
   const uint8_t thr = J.m_bin[o];
-  const uint32_t* rows = J.rows + static_cast<size_t>(level) * J.N + w.start;
   const uint32_t tid = threadIdx.x;
   uint32_t nl = 0;
   double gl = 0.0, hl = 0.0, gr = 0.0, hr = 0.0;
   for (uint32_t j = tid; j < w.len; j += 256) {
-    const uint32_t row = rows[j];
+    const uint32_t row = level_row(J, level, w.start + j);
     const uint8_t b = col[row];
     const float2 v = J.gh[row];
     if (b == 255 || b <= thr) { ++nl; gl += v.x; hl += v.y; }
@@ -463,31 +594,33 @@ __global__ void __launch_bounds__(256) k_part_count(const JobDev* __restrict__ j
   }
 }
 
-// Per job: child offsets and totals, level stop test, next level's work.
+// Per job: child offsets and totals, level stop test, next level's work
+// lists (block-wide scans over nodes, then one lane per work item).
 __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jobs, Lists L,
                                                    uint32_t level) {
   const uint32_t job = blockIdx.y, tid = threadIdx.x;
   const JobDev& J = jobs[job];
   if (!part_active(J, level)) return;
-  const uint32_t nn = 1u << level;
+  const uint32_t nn = 1u << level;  // <= 512
   const uint32_t stride = (1u << J.D) + 1u;
   const uint32_t* offd = J.off + static_cast<size_t>(level) * stride;
   uint32_t* offn = J.off + static_cast<size_t>(level + 1) * stride;
-  __shared__ uint32_t s_base[(1 << kMaxDepth) + 1];
+  __shared__ uint32_t s_slot[1 << (kMaxDepth - 1)];  // per node: partition slots -> first slot
+  __shared__ uint32_t s_hw[1 << (kMaxDepth - 1)];    // per pair: hist work items -> first item
+  __shared__ uint32_t s_pw[1 << kMaxDepth];          // per child: partition slots -> first slot
+  __shared__ uint32_t s_off[(1 << kMaxDepth) + 1];   // child offsets
+  __shared__ uint32_t s_tmp[256];
+  __shared__ uint32_t s_hbase, s_pbase;
   __shared__ int s_any;
-  if (tid == 0) {  // slot base per node: node order, ceil(len / chunk) slots each
-    uint32_t acc = 0;
-    for (uint32_t n = 0; n < nn; ++n) {
-      s_base[n] = acc;
-      acc += part_slots(offd[n + 1] - offd[n]);
-    }
-    s_base[nn] = acc;
-    s_any = 0;
-  }
+  __shared__ unsigned long long s_rows;
+  if (tid == 0) { s_any = 0; s_rows = 0ull; }
+  for (uint32_t n = tid; n < nn; n += 256) s_slot[n] = part_slots(offd[n + 1] - offd[n]);
   __syncthreads();
+  block_scan_excl(s_slot, nn, s_tmp);
   const float mcw2 = J.min_child_weight * 2.0f;
   for (uint32_t n = tid; n < nn; n += 256) {
-    const uint32_t b0 = s_base[n], b1 = s_base[n + 1];
+    const uint32_t len = offd[n + 1] - offd[n];
+    const uint32_t b0 = s_slot[n], b1 = b0 + part_slots(len);
     uint32_t acc = 0;
     double gl = 0, hl = 0, gr = 0, hr = 0;
     for (uint32_t s = b0; s < b1; ++s) {
@@ -498,51 +631,63 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
       gr += J.part_stats[4 * static_cast<size_t>(s) + 2];
       hr += J.part_stats[4 * static_cast<size_t>(s) + 3];
     }
-    const uint32_t len = offd[n + 1] - offd[n];
     const uint32_t hl_i = heap_index(level + 1, 2 * n), hr_i = hl_i + 1;
+    const float fhl = static_cast<float>(hl), fhr = static_cast<float>(hr);
     J.nc[hl_i] = acc;
     J.nc[hr_i] = len - acc;
     J.ng[hl_i] = static_cast<float>(gl);
-    J.nh[hl_i] = static_cast<float>(hl);
+    J.nh[hl_i] = fhl;
     J.ng[hr_i] = static_cast<float>(gr);
-    J.nh[hr_i] = static_cast<float>(hr);
+    J.nh[hr_i] = fhr;
+    s_off[2 * n] = offd[n];
+    s_off[2 * n + 1] = offd[n] + acc;
     offn[2 * n] = offd[n];
     offn[2 * n + 1] = offd[n] + acc;
     // stop test of the next level (symmetric_tree.cpp:115-124)
-    if ((acc >= 2u * J.min_samples_leaf && J.nh[hl_i] >= mcw2) ||
-        (len - acc >= 2u * J.min_samples_leaf && J.nh[hr_i] >= mcw2))
+    if ((acc >= 2u * J.min_samples_leaf && fhl >= mcw2) ||
+        (len - acc >= 2u * J.min_samples_leaf && fhr >= mcw2))
       s_any = 1;
+    // next level: histogram of the smaller child of the pair (ties: left),
+    // partition slots of both children
+    const uint32_t sm_len = min(acc, len - acc);
+    s_hw[n] = ((sm_len + J.hist_chunk - 1) / J.hist_chunk) * J.T;
+    s_pw[2 * n] = part_slots(acc);
+    s_pw[2 * n + 1] = part_slots(len - acc);
+    if (sm_len) atomicAdd(&s_rows, static_cast<unsigned long long>(sm_len));
   }
-  if (tid == 0) offn[2 * nn] = offd[nn];
+  if (tid == 0) {
+    offn[2 * nn] = offd[nn];
+    s_off[2 * nn] = offd[nn];
+  }
   __syncthreads();
   if (level + 1 >= J.D || !s_any) {
     if (tid == 0) J.st->active = 0;
     return;
   }
+  const uint32_t htot = block_scan_excl(s_hw, nn, s_tmp);
+  const uint32_t ptot = block_scan_excl(s_pw, 2 * nn, s_tmp);
   if (tid == 0) {
-    // next level: histograms of the smaller child of every pair (ties: left)
-    for (uint32_t n = 0; n < nn; ++n) {
-      const uint32_t nl = J.nc[heap_index(level + 1, 2 * n)];
-      const uint32_t nr = J.nc[heap_index(level + 1, 2 * n + 1)];
-      const uint32_t sm = nl <= nr ? 2 * n : 2 * n + 1;
-      emit_hist(J, job, L.hist[level + 1], L.hist_count[level + 1], sm, offn[sm],
-                offn[sm + 1] - offn[sm]);
-    }
-    // next level: partition work over every child, slots in node order
-    uint32_t total = 0;
-    for (uint32_t c = 0; c < 2 * nn; ++c) total += part_slots(offn[c + 1] - offn[c]);
-    const uint32_t base = atomicAdd(L.part_count[level + 1], total);
-    uint32_t slot = 0;
-    for (uint32_t c = 0; c < 2 * nn; ++c) {
-      const uint32_t len = offn[c + 1] - offn[c];
-      for (uint32_t s = 0; s < part_slots(len); ++s, ++slot) {
-        PartWork w;
-        w.job = job; w.node = c; w.start = offn[c] + s * kPartChunk;
-        w.len = min(static_cast<uint32_t>(kPartChunk), len - s * kPartChunk);
-        w.slot = slot; w.pad0 = w.pad1 = w.pad2 = 0;
-        L.part[level + 1][base + slot] = w;
-      }
-    }
+    s_hbase = atomicAdd(L.hist_count[level + 1], htot);
+    s_pbase = atomicAdd(L.part_count[level + 1], ptot);
+    if (J.hist_rows_total) atomicAdd(J.hist_rows_total, s_rows);
+  }
+  __syncthreads();
+  HistWork* hout = L.hist[level + 1] + s_hbase;
+  for (uint32_t e = tid; e < htot; e += 256) {
+    const uint32_t n = find_item(s_hw, nn, e);
+    const uint32_t lc = 2 * n, rc = lc + 1;
+    const uint32_t nl = s_off[lc + 1] - s_off[lc], nr = s_off[rc + 1] - s_off[rc];
+    const uint32_t sm = nl <= nr ? lc : rc;
+    const uint32_t le = e - s_hw[n];
+    const uint32_t start = s_off[sm] + (le / J.T) * J.hist_chunk;
+    put_hist_work(hout + e, job, sm, start, min(J.hist_chunk, s_off[sm + 1] - start), le % J.T);
+  }
+  PartWork* pout = L.part[level + 1] + s_pbase;
+  for (uint32_t e = tid; e < ptot; e += 256) {
+    const uint32_t c = find_item(s_pw, 2 * nn, e);
+    const uint32_t start = s_off[c] + (e - s_pw[c]) * kPartChunk;
+    put_part_work(pout + e, job, c, start,
+                  min(static_cast<uint32_t>(kPartChunk), s_off[c + 1] - start), e);
   }
 }
 
@@ -560,7 +705,6 @@ __global__ void __launch_bounds__(256) k_part_scatter(const JobDev* __restrict__
   const uint32_t stride = (1u << J.D) + 1u;
   const uint32_t* offd = J.off + static_cast<size_t>(level) * stride;
   const uint32_t* offn = J.off + static_cast<size_t>(level + 1) * stride;
-  const uint32_t* rows = J.rows + static_cast<size_t>(level) * J.N;
   uint32_t* rows_n = J.rows + static_cast<size_t>(level + 1) * J.N;
   const uint32_t loff = J.part_loff[w.slot];
   const uint32_t before = w.start - offd[w.node];
@@ -572,7 +716,7 @@ __global__ void __launch_bounds__(256) k_part_scatter(const JobDev* __restrict__
   for (uint32_t j0 = 0; j0 < w.len; j0 += 256) {
     const uint32_t j = j0 + tid;
     const bool valid = j < w.len;
-    const uint32_t row = valid ? rows[w.start + j] : 0u;
+    const uint32_t row = valid ? level_row(J, level, w.start + j) : 0u;
     const uint8_t b = valid ? col[row] : 0;
     const bool left = valid && (b == 255 || b <= thr);
     const bool right = valid && !left;
@@ -623,12 +767,29 @@ __global__ void __launch_bounds__(256) k_leaves(const JobDev* __restrict__ jobs)
   if (threadIdx.x == 0) J.m_depth[t] = static_cast<uint16_t>(depth);
 }
 
-// Copy externally supplied gradients (teacher forcing) into gh.
+// Copy externally supplied gradients (teacher forcing) into gh, recording
+// max |g| and max |h| for the histogram scales.
 __global__ void __launch_bounds__(256) k_set_gh(const JobDev* __restrict__ jobs, const float* g,
                                                 const float* h) {
   const JobDev& J = jobs[0];
   const uint32_t p = blockIdx.x * 256 + threadIdx.x;
-  if (p < J.N) J.gh[p] = make_float2(g[p], h[p]);
+  float a = 0.f, b = 0.f;
+  if (p < J.N) {
+    J.gh[p] = make_float2(g[p], h[p]);
+    a = fabsf(g[p]);
+    b = fabsf(h[p]);
+  }
+  block_max2_to(J.st, a, b);
+}
+
+// Fixed-point histogram cells -> float pairs (tc_hist_build).
+__global__ void __launch_bounds__(256) k_hist_to_float(const HistCell* __restrict__ h,
+                                                       const TreeState* __restrict__ st, uint32_t n,
+                                                       float2* __restrict__ out) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n)
+    out[i] = make_float2(static_cast<float>(static_cast<double>(h[i].g) * st->ginv),
+                         static_cast<float>(static_cast<double>(h[i].h) * st->hinv));
 }
 
 }  // namespace tc
