@@ -5,8 +5,9 @@
 //                                 16-byte load = one 16-feature tile of a row)
 //   bins_cm   uint8  [F][N]      column-major (split column reads)
 //   gh        float2 [N]         gradient / hessian per row
-//   rows      uint32 [D+1][N]    per level, row ids grouped by node, each
-//                                 node's rows in ascending order (stable)
+//   rows      uint32 [D+1][N]    per level, row ids grouped by node (order
+//                                 within a node is irrelevant: every sum
+//                                 over a node is an exact integer sum)
 //   off       uint32 [D+1][2^D+1] node segment offsets into rows[level]
 //   hist      HistCell{int64 g, int64 h} [2^D-1][F][256]  fixed-point per
 //                                 node (heap order: level d node n at
@@ -20,7 +21,7 @@ namespace tc {
 
 constexpr int kMaxDepth = 10;          // 2^10 leaves; u16 leaf index range
 constexpr int kTileF = 16;             // features per histogram tile (16 B)
-constexpr int kHistThreads = 256;
+constexpr int kHistThreads = 512;
 constexpr int kPartChunk = 4096;       // rows per partition work item
 constexpr int kScanNodesPerPass = 32;  // node prefixes staged in LDS at a time
 constexpr uint32_t kEmptyKey = 0u;
@@ -87,11 +88,10 @@ struct JobDev {
   uint32_t* nc;
   HistCell* hist;           // heap [2^D - 1][F][256]
   TreeState* st;
-  // partition scratch, per slot (capacity part_cap)
-  uint32_t* part_left;      // left count
-  uint32_t* part_loff;      // exclusive scan of left counts within the node
-  double* part_stats;       // [slot][4] = GL, HL, GR, HR
-  uint32_t part_cap;
+  // partition cursors of the level being split: [node][2] = next free
+  // position of the left child (counts up), end of the right child (counts
+  // down); capacity 2^D
+  uint32_t* cursor;
   // root-sum partials (sum g, sum h over 1024-row tiles of the root list)
   double* grad_partials;    // [wg][2]
   uint32_t grad_wg_off;     // first workgroup of this job in the row grid

@@ -19,7 +19,7 @@ namespace tc {
 namespace {
 
 constexpr int kHistUnroll = 4;
-constexpr uint32_t kMaxHistChunk = 32768;
+constexpr uint32_t kMaxHistChunk = 16384;
 
 uint32_t ceil_div(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
 uint32_t round8(uint32_t x) { return std::max(8u, (x + 7u) & ~7u); }
@@ -156,17 +156,12 @@ class Trainer {
   // histogram launch for live timing.
   void enqueue_tree(std::vector<std::pair<hipEvent_t, hipEvent_t>>* hist_events) {
     hipStream_t s = ctx_->stream;
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, lists_);
     hipLaunchKernelGGL(k_root_partials, dim3(total_grad_wg_), dim3(256), 0, s, d_jobs_,
-                       d_job_wg_off_, njobs_);
+                       d_job_wg_off_, njobs_, lists_);
     const bool packed = packed_hist();
     hipLaunchKernelGGL(k_tree_begin, dim3(1, njobs_), dim3(256), 0, s, d_jobs_, lists_,
                        packed ? 1 : 0);
     for (uint32_t d = 0; d < D_; ++d) {
-      const uint32_t pairs = d > 0 ? (1u << (d - 1)) : 0u;
-      if (d > 0)
-        hipLaunchKernelGGL(k_zero_smaller, dim3(pairs * fmax_, njobs_), dim3(256), 0, s, d_jobs_, d,
-                           fmax_);
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (hist_events) {
         TC_HIP(hipEventCreate(&e0));
@@ -184,17 +179,17 @@ class Trainer {
         hist_events->emplace_back(e0, e1);
       }
       if (d > 0)
-        hipLaunchKernelGGL(k_subtract, dim3(pairs * fmax_, njobs_), dim3(256), 0, s, d_jobs_, d, fmax_);
+        hipLaunchKernelGGL(k_subtract, dim3((1u << (d - 1)) * fmax_, njobs_), dim3(256), 0, s,
+                           d_jobs_, d, fmax_);
       hipLaunchKernelGGL(k_split_scan<true>, dim3(fmax_, njobs_), dim3(256), 0, s, d_jobs_,
                          static_cast<const float2*>(nullptr), d);
-      hipLaunchKernelGGL(k_finalize, dim3(1, njobs_), dim3(64), 0, s, d_jobs_, d);
-      hipLaunchKernelGGL(k_part_count, dim3(part_cap_[d]), dim3(256), 0, s, d_jobs_, lists_.part[d],
-                         lists_.part_count[d], d);
-      hipLaunchKernelGGL(k_part_plan, dim3(1, njobs_), dim3(256), 0, s, d_jobs_, lists_, d);
-      hipLaunchKernelGGL(k_part_scatter, dim3(part_cap_[d]), dim3(256), 0, s, d_jobs_,
-                         lists_.part[d], lists_.part_count[d], d);
+      hipLaunchKernelGGL(k_finalize, dim3(1u << d, njobs_), dim3(256), 0, s, d_jobs_, d);
+      if (d + 1 < D_ || keep_rows_) {  // training never reads the last level's rows
+        hipLaunchKernelGGL(k_partition, dim3(part_cap_[d]), dim3(256), 0, s, d_jobs_, lists_.part[d],
+                           lists_.part_count[d], d);
+        hipLaunchKernelGGL(k_part_plan, dim3(1, njobs_), dim3(256), 0, s, d_jobs_, lists_, d);
+      }
     }
-    hipLaunchKernelGGL(k_leaves, dim3(1, njobs_), dim3(256), 0, s, d_jobs_);
     hipLaunchKernelGGL(k_grad_update, dim3(total_grad_wg_), dim3(256), 0, s, d_jobs_, d_job_wg_off_,
                        njobs_, 1);
     TC_HIP(hipGetLastError());
@@ -307,7 +302,11 @@ class Trainer {
     m->base = J.base;
   }
 
-  // Per-row leaf index of tree 0 (teacher-forcing checks).
+  // Partition the last level as well, so that leaf_index can read it.
+  void keep_final_rows() { keep_rows_ = true; }
+
+  // Per-row leaf index of tree 0 from the final row lists (teacher-forcing
+  // checks of the partition; needs keep_final_rows before the tree).
   void leaf_index(uint32_t* out) {
     const JobDev& J = h_jobs_[0];
     uint16_t depth = 0;
@@ -377,10 +376,14 @@ class Trainer {
       J.min_split_gain = S.p.tree.min_split_gain;
       J.base = S.base;
       uint32_t ch = S.p.hist_chunk;
-      if (ch == 0) {  // heuristic: ~2048 root work items per job batch
+      if (ch == 0) {
+        // ~512 root work items (two per CU): every work item ends with an
+        // int64-atomic flush of its 16 x 256 cells, which costs about as much
+        // as accumulating a few thousand rows, so items are made as long as
+        // the root's parallelism allows (1M rows x 7 tiles -> 16384 rows)
         const uint64_t work = static_cast<uint64_t>(N) * J.T;
         ch = 1024;
-        while (ch < kMaxHistChunk && work / ch > 2048) ch <<= 1;
+        while (ch < kMaxHistChunk && work / ch > 512) ch <<= 1;
       }
       ch = std::min(ch, kMaxHistChunk);  // packed cells count < 2^16 rows per item
       J.hist_chunk = ch;
@@ -400,10 +403,7 @@ class Trainer {
       J.hist = alloc<HistCell>(((1ull << D_) - 1) * J.F * 256);
       J.st = alloc<TreeState>(1);
       hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1), 0, s, J.st);
-      J.part_cap = ceil_div(N, kPartChunk) + (1u << D_);
-      J.part_left = alloc<uint32_t>(J.part_cap);
-      J.part_loff = alloc<uint32_t>(J.part_cap);
-      J.part_stats = alloc<double>(4ull * J.part_cap);
+      J.cursor = alloc<uint32_t>(1u << D_);
       J.grad_wg_cnt = std::max(1u, ceil_div(N, kGradTile));
       J.grad_wg_off = wg_off[j];
       wg_off[j + 1] = wg_off[j] + J.grad_wg_cnt;
@@ -491,6 +491,7 @@ class Trainer {
   hipGraphExec_t graph_exec_ = nullptr;
   std::vector<Sampler> samplers_;
   bool forced_ = false;
+  bool keep_rows_ = false;  // partition the last level too (leaf_index)
 
   // Packed MSE histogram cells need h == 2 on every row of every job.
   bool packed_hist() const {
@@ -670,6 +671,7 @@ int32_t tc_tree_build(tc_dataset* ds, const float* g, const float* h, const tc_t
     JobSpec s{ds, nullptr, p, 0.f};
     Trainer tr(ds->ctx, {s});
     tr.set_gradients(g, h);
+    if (leaf_idx) tr.keep_final_rows();
     tr.enqueue_tree(nullptr);
     TC_HIP(hipStreamSynchronize(ds->ctx->stream));
     const uint32_t D = tp->max_depth;

@@ -7,13 +7,13 @@
 // data-dependent control (level stops, tree count) lives in TreeState and
 // makes later kernels of a stopped job return at once.
 //
-//   reset_counters, root_partials, tree_begin
+//   root_partials, tree_begin
 //   for level d < D:
-//     [d>0] zero_smaller(d), hist(d, smaller children), subtract(d)
 //     [d=0] hist(0, root)
+//     [d>0] hist(d, smaller children), subtract(d)
 //     split_scan(d), finalize(d)
-//     part_count(d), part_plan(d), part_scatter(d)
-//   leaves, grad_update(apply)
+//     [d<D-1] partition(d), part_plan(d)
+//   grad_update(apply)        (leaf values, prediction update, gradients)
 //
 // Histograms are exact fixed-point sums: a row's g and h enter as int64
 // round(v * 2^e), with e chosen per tree from max |g|, max |h| and the row
@@ -21,7 +21,9 @@
 // atomics (on gfx950 about 20x the rate of LDS float atomics) and merged into
 // HBM with int64 atomics.  Integer addition is order-independent, so every
 // histogram, and every split, is the same run to run and the same for a job
-// trained alone or in a batch.
+// trained alone or in a batch.  The same exactness makes the order of rows
+// inside a node irrelevant, so the partition is a single unstable pass, and
+// child totals come from the parent's histogram instead of a row pass.
 #include "tc_device.h"
 
 namespace tc {
@@ -113,10 +115,14 @@ __device__ inline void block_max2_to(TreeState* st, float a, float b) {
 }
 
 // ---------------------------------------------------------------------------
-// Gradients (A6/A7) fused with the training-prediction update (A15 + K9).
+// Gradients (A6/A7) fused with the leaf values (A14) and the training-
+// prediction update (A15 + K9).
 //
 // mode 0: preds = base for every row (before the first tree).
-// mode 1: apply tree st->tree to every row: leaf index from the binned
+// mode 1: leaf values of tree st->tree from its final level's totals
+//         (compute_leaf_value, symmetric_tree.cpp:753-775; every workgroup
+//         computes them, workgroup 0 stores them), then apply the tree to
+//         every row: leaf index from the binned
 //         compares go_right = bin != 255 && bin > threshold, MSB first
 //         (SymmetricTree::predict_batch, symmetric_tree.cpp:813-840), then
 //         preds += lr * leaf as one fused multiply-add (the reference
@@ -141,6 +147,7 @@ __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ 
   __shared__ uint32_t s_job;
   __shared__ uint16_t s_f[kMaxDepth];
   __shared__ uint8_t s_b[kMaxDepth];
+  __shared__ float s_leaf[1 << kMaxDepth];
   const uint32_t tid = threadIdx.x;
   if (tid == 0) s_job = job_of_block(job_wg_off, njobs);
   __syncthreads();
@@ -152,14 +159,33 @@ __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ 
   if (mode == 1) {
     t = J.st->tree;
     if (t < 0 || t >= static_cast<int>(J.n_trees)) return;
-    depth = J.m_depth[t];
+    depth = J.st->depth;
     if (tid < static_cast<uint32_t>(depth)) {
       s_f[tid] = J.m_feat[static_cast<size_t>(t) * J.D + tid];
       s_b[tid] = J.m_bin[static_cast<size_t>(t) * J.D + tid];
     }
+    float* out = J.m_leaves + static_cast<size_t>(t) * (1u << J.D);
+    for (uint32_t i = tid; i < (1u << J.D); i += 256) {
+      float v = 0.f;
+      if (i < (1u << depth)) {
+        const uint32_t hi = heap_index(depth, i);
+        const float G = J.ng[hi], H = J.nh[hi];
+        if (!(H < 1e-10f)) {
+          v = __fdiv_rn(-G, __fadd_rn(H, J.lambda_l2));
+          if (J.lambda_l1 > 0.f) {
+            if (v > J.lambda_l1) v = __fsub_rn(v, J.lambda_l1);
+            else if (v < -J.lambda_l1) v = __fadd_rn(v, J.lambda_l1);
+            else v = 0.f;
+          }
+          if (J.max_delta_step > 0.f) v = fmaxf(-J.max_delta_step, fminf(J.max_delta_step, v));
+        }
+      }
+      s_leaf[i] = v;
+      if (lw == 0) out[i] = v;
+    }
+    if (lw == 0 && tid == 0) J.m_depth[t] = static_cast<uint16_t>(depth);
     __syncthreads();
   }
-  const float* leaves = J.m_leaves + static_cast<size_t>(t < 0 ? 0 : t) * (1u << J.D);
   float gmax = 0.f, hmax = 0.f;
   for (int k = 0; k < kGradTile / 256; ++k) {
     const uint32_t row = lw * kGradTile + k * 256 + tid;
@@ -171,7 +197,7 @@ __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ 
         const uint8_t v = J.bins_cm[static_cast<size_t>(s_f[d]) * N + row];
         li = (li << 1) | ((v != 255 && v > s_b[d]) ? 1u : 0u);
       }
-      pred = __fmaf_rn(J.lr, leaves[li], J.preds[row]);
+      pred = __fmaf_rn(J.lr, s_leaf[li], J.preds[row]);
     } else {
       pred = J.base;
     }
@@ -184,21 +210,31 @@ __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ 
   block_max2_to(J.st, gmax, hmax);
 }
 
-// Root totals: (sum g, sum h) over 1024-position tiles of the root list
-// rows[0][0 .. n_root), double, fixed order within the tile; k_tree_begin
-// adds the tiles in order.
+// Start of a tree, over the row grid: root totals (sum g, sum h) over
+// 1024-position tiles of the root list rows[0][0 .. n_root), double, fixed
+// order within the tile (k_tree_begin adds the tiles in a fixed order);
+// zeroed histograms of levels 0 and 1 (deeper levels are zeroed by the
+// previous level's k_subtract); workgroup 0 resets the work-list counters.
 __global__ void __launch_bounds__(256) k_root_partials(const JobDev* __restrict__ jobs,
                                                        const uint32_t* __restrict__ job_wg_off,
-                                                       uint32_t njobs) {
+                                                       uint32_t njobs, Lists L) {
   __shared__ uint32_t s_job;
   __shared__ double s_red[2][4];
   const uint32_t tid = threadIdx.x;
   if (tid == 0) s_job = job_of_block(job_wg_off, njobs);
+  if (blockIdx.x == 0 && tid < kMaxDepth) {
+    *L.hist_count[tid] = 0;
+    *L.part_count[tid] = 0;
+  }
   __syncthreads();
   const JobDev& J = jobs[s_job];
   const uint32_t lw = blockIdx.x - J.grad_wg_off;
   double sg = 0.0, sh = 0.0;
   if (J.st->tree + 1 < static_cast<int>(J.n_trees)) {
+    const uint32_t cells = (J.D > 1 ? 3u : 1u) * J.F * 256u;  // heap nodes 0..2
+    const uint32_t per = (cells + J.grad_wg_cnt - 1) / J.grad_wg_cnt;
+    const uint32_t c1 = min(cells, (lw + 1) * per);
+    for (uint32_t c = lw * per + tid; c < c1; c += 256) J.hist[c] = HistCell{0, 0};
     for (int k = 0; k < kGradTile / 256; ++k) {
       const uint32_t p = lw * kGradTile + k * 256 + tid;
       if (p >= J.n_root) break;
@@ -220,17 +256,9 @@ __global__ void __launch_bounds__(256) k_root_partials(const JobDev* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-__global__ void k_reset_counters(Lists L) {
-  const uint32_t i = threadIdx.x;
-  if (i < kMaxDepth) {
-    *L.hist_count[i] = 0;
-    *L.part_count[i] = 0;
-  }
-}
-
 // Root of a new tree (symmetric_tree.cpp:97-124): totals from the root
-// partials (in workgroup order, double), level-0 stop test, the tree's
-// histogram scales, zeroed root histogram, root work lists.
+// partials (double, fixed order), level-0 stop test, the tree's histogram
+// scales, root work lists.
 // packed: histograms use the packed MSE cell (h == 2 for every row).
 __global__ void __launch_bounds__(256) k_tree_begin(const JobDev* __restrict__ jobs, Lists L,
                                                     int packed) {
@@ -239,8 +267,20 @@ __global__ void __launch_bounds__(256) k_tree_begin(const JobDev* __restrict__ j
   const uint32_t tid = threadIdx.x;
   __shared__ int s_go;
   __shared__ uint32_t s_hbase, s_pbase;
+  __shared__ double s_red[2][4];
   const uint32_t n = J.n_root;
   const uint32_t hcnt = ((n + J.hist_chunk - 1) / J.hist_chunk) * J.T, pcnt = part_slots(n);
+  {  // thread t adds tiles t, t + 256, ... in order; then a fixed tree
+    double a = 0.0, b = 0.0;
+    for (uint32_t w = tid; w < J.grad_wg_cnt; w += 256) {
+      a += J.grad_partials[2 * w];
+      b += J.grad_partials[2 * w + 1];
+    }
+    a = warp_sum_d(a);
+    b = warp_sum_d(b);
+    if ((tid & 63) == 0) { s_red[0][tid >> 6] = a; s_red[1][tid >> 6] = b; }
+    __syncthreads();
+  }
   if (tid == 0) {
     TreeState* st = J.st;
     const int t = st->tree + 1;
@@ -252,11 +292,8 @@ __global__ void __launch_bounds__(256) k_tree_begin(const JobDev* __restrict__ j
     st->hmax_next = 0u;
     int go = 0;
     if (t < static_cast<int>(J.n_trees)) {
-      double G = 0.0, H = 0.0;
-      for (uint32_t w = 0; w < J.grad_wg_cnt; ++w) {
-        G += J.grad_partials[2 * w];
-        H += J.grad_partials[2 * w + 1];
-      }
+      const double G = (s_red[0][0] + s_red[0][1]) + (s_red[0][2] + s_red[0][3]);
+      const double H = (s_red[1][0] + s_red[1][1]) + (s_red[1][2] + s_red[1][3]);
       J.ng[0] = static_cast<float>(G);
       J.nh[0] = static_cast<float>(H);
       J.nc[0] = n;
@@ -281,8 +318,6 @@ __global__ void __launch_bounds__(256) k_tree_begin(const JobDev* __restrict__ j
   }
   __syncthreads();
   if (!s_go) return;
-  HistCell* h = J.hist;
-  for (uint32_t i = tid; i < J.F * 256u; i += blockDim.x) h[i] = HistCell{0, 0};
   // chunk-major: the tiles of one row chunk are adjacent (and share an XCD)
   for (uint32_t e = tid; e < hcnt; e += blockDim.x) {
     const uint32_t start = (e / J.T) * J.hist_chunk;
@@ -339,6 +374,8 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
   const uint32_t Fp = J.Fp;
   const uint32_t s = tid & 15u;  // this lane's starting feature
   const uint32_t q = s >> 2, r = s & 3u;
+  const uint32_t f0 = w.tile * kTileF;
+  const uint32_t nf = min(static_cast<uint32_t>(kTileF), J.F - f0);  // real features of the tile
 
   for (uint32_t j0 = 0; j0 < w.len; j0 += kHistThreads * U) {
     uint4 v[U];
@@ -377,15 +414,15 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
       for (int k = 0; k < 16; ++k) {
         const uint32_t f = (static_cast<uint32_t>(k) + s) & 15u;
         const uint32_t b = (wr[k >> 2] >> (8 * (k & 3))) & 0xFFu;
-        atomicAdd(&s_a[f * 257 + b], va);
-        if (!PACKED) atomicAdd(&s_b[f * 257 + b], vb);
+        if (f < nf) {  // padding features of the last tile cost no atomics
+          atomicAdd(&s_a[f * 257 + b], va);
+          if (!PACKED) atomicAdd(&s_b[f * 257 + b], vb);
+        }
       }
     }
   }
   __syncthreads();
-  const uint32_t f0 = w.tile * kTileF;
   HistCell* out = J.hist + (static_cast<size_t>(heap_index(level, w.node)) * J.F + f0) * 256u;
-  const uint32_t nf = min(static_cast<uint32_t>(kTileF), J.F - f0);
   const long long h2 = PACKED ? __double2ll_rn(2.0 * hsc) : 0;
   for (uint32_t i = tid; i < nf * 256u; i += kHistThreads) {
     const uint32_t f = i >> 8, b = i & 255u;
@@ -405,22 +442,11 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
   }
 }
 
-// Zero the histograms of the smaller children at `level` (built by k_hist).
-__global__ void __launch_bounds__(256) k_zero_smaller(const JobDev* __restrict__ jobs,
-                                                      uint32_t level, uint32_t fmax) {
-  const JobDev& J = jobs[blockIdx.y];
-  if (!J.st->active) return;
-  const uint32_t pair = blockIdx.x / fmax, f = blockIdx.x % fmax;
-  if (f >= J.F) return;
-  const uint32_t lc = 2 * pair, rc = lc + 1;
-  const uint32_t nl = J.nc[heap_index(level, lc)], nr = J.nc[heap_index(level, rc)];
-  const uint32_t sm = nl <= nr ? lc : rc;
-  J.hist[(static_cast<size_t>(heap_index(level, sm)) * J.F + f) * 256u + threadIdx.x] = HistCell{0, 0};
-}
-
 // A13: larger child = parent - smaller child, over every (feature, slot)
 // (Histogram::subtract_from, histogram.cpp:39-42 via symmetric_tree.cpp:346),
-// exact in fixed point.
+// exact in fixed point.  Workgroup = (pair, feature), thread = slot.  Also
+// zeroes the four level+1 histograms below the pair (feature f), which the
+// next level's k_hist accumulates into.
 __global__ void __launch_bounds__(256) k_subtract(const JobDev* __restrict__ jobs,
                                                   uint32_t level, uint32_t fmax) {
   const JobDev& J = jobs[blockIdx.y];
@@ -431,21 +457,26 @@ __global__ void __launch_bounds__(256) k_subtract(const JobDev* __restrict__ job
   const uint32_t nl = J.nc[heap_index(level, lc)], nr = J.nc[heap_index(level, rc)];
   const uint32_t sm = nl <= nr ? lc : rc, lg = nl <= nr ? rc : lc;
   const size_t F = J.F;
-  const HistCell p = J.hist[(heap_index(level - 1, pair) * F + f) * 256u + threadIdx.x];
-  const HistCell c = J.hist[(heap_index(level, sm) * F + f) * 256u + threadIdx.x];
-  J.hist[(heap_index(level, lg) * F + f) * 256u + threadIdx.x] = HistCell{p.g - c.g, p.h - c.h};
+  const uint32_t b = threadIdx.x;
+  const HistCell p = J.hist[(heap_index(level - 1, pair) * F + f) * 256u + b];
+  const HistCell c = J.hist[(heap_index(level, sm) * F + f) * 256u + b];
+  J.hist[(heap_index(level, lg) * F + f) * 256u + b] = HistCell{p.g - c.g, p.h - c.h};
+  if (level + 1 < J.D) {
+    for (uint32_t k = 0; k < 4; ++k)
+      J.hist[(heap_index(level + 1, 4 * pair + k) * F + f) * 256u + b] = HistCell{0, 0};
+  }
 }
 
 // ---------------------------------------------------------------------------
 // A10: level split scan.  Workgroup = (feature, job), thread = candidate bin.
-// The bins of up to 32 nodes are staged in LDS as floats (all lanes), then
-// one lane per node forms the sequential float prefix over bins, so for a
-// given float histogram every candidate's left sum, gain term and node-order
-// total is the reference's float value (symmetric_tree.cpp:153-193,
-// :214-262, with the deeper levels' total += 0.5 * (...) as the fused
-// multiply-add its compiler emits).  FIXED: the histogram is fixed-point
-// cells (bin value = float of the exact sum); otherwise float pairs (the
-// tc_split_scan entry point).
+// Per node the left sums of every candidate are staged in LDS (up to 32
+// nodes at a time), then each candidate's gain terms are added over nodes in
+// node order (symmetric_tree.cpp:153-193, :214-262, with the deeper levels'
+// total += 0.5 * (...) as the fused multiply-add its compiler emits).
+// FIXED (training): the left sums are the exact fixed-point prefixes, each
+// rounded once to float.  Otherwise (the tc_split_scan entry point, float
+// bins): the reference's sequential float prefix, so for a given float
+// histogram every left sum, gain and total is the reference's float value.
 template <bool FIXED>
 __global__ void __launch_bounds__(256) k_split_scan(const JobDev* __restrict__ jobs,
                                                     const float2* __restrict__ fhist,
@@ -461,31 +492,52 @@ __global__ void __launch_bounds__(256) k_split_scan(const JobDev* __restrict__ j
   const double gi = FIXED ? J.st->ginv : 1.0, hi = FIXED ? J.st->hinv : 1.0;
   float total = 0.f;
   bool any = false;
+  const uint32_t lane = tid & 63u, wave = tid >> 6;
   for (uint32_t c0 = 0; c0 < nn; c0 += kScanNodesPerPass) {
     const uint32_t cs = min(static_cast<uint32_t>(kScanNodesPerPass), nn - c0);
-    for (uint32_t k = 0; k < cs; ++k) {
-      const size_t cell = (static_cast<size_t>(heap_index(level, c0 + k)) * J.F + f) * 256u + tid;
-      float2 v;
-      if (FIXED) {
-        const HistCell hc = J.hist[cell];
-        v = make_float2(static_cast<float>(static_cast<double>(hc.g) * gi),
-                        static_cast<float>(static_cast<double>(hc.h) * hi));
-      } else {
-        v = fhist[cell];
+    if (FIXED) {
+      // exact prefix: one wave per node, lane l holds bins 4l .. 4l+3
+      for (uint32_t k = wave; k < cs; k += 4) {
+        const HistCell* hp = J.hist + (static_cast<size_t>(heap_index(level, c0 + k)) * J.F + f) * 256u;
+        long long pg[4], ph[4], ag = 0, ah = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const HistCell hc = hp[4 * lane + i];
+          ag += hc.g;
+          ah += hc.h;
+          pg[i] = ag;
+          ph[i] = ah;
+        }
+        long long sg = ag, sh = ah;  // inclusive scan of lane totals
+        for (int o = 1; o < 64; o <<= 1) {
+          const long long ug = __shfl_up(sg, o, 64), uh = __shfl_up(sh, o, 64);
+          if (lane >= static_cast<uint32_t>(o)) { sg += ug; sh += uh; }
+        }
+        sg -= ag;
+        sh -= ah;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          s_pref[k][4 * lane + i] =
+              make_float2(static_cast<float>(static_cast<double>(sg + pg[i]) * gi),
+                          static_cast<float>(static_cast<double>(sh + ph[i]) * hi));
       }
-      s_pref[k][tid] = v;
-    }
-    __syncthreads();
-    if (tid < cs) {
-      float lg = 0.f, lh = 0.f;
-      for (uint32_t b = 0; b + 1 < mb; ++b) {
-        const float2 v = s_pref[tid][b];
-        lg = __fadd_rn(lg, v.x);
-        lh = __fadd_rn(lh, v.y);
-        s_pref[tid][b] = make_float2(lg, lh);
+      __syncthreads();
+    } else {
+      // the reference's sequential float prefix over the given float bins
+      for (uint32_t k = 0; k < cs; ++k)
+        s_pref[k][tid] = fhist[(static_cast<size_t>(heap_index(level, c0 + k)) * J.F + f) * 256u + tid];
+      __syncthreads();
+      if (tid < cs) {
+        float lg = 0.f, lh = 0.f;
+        for (uint32_t b = 0; b + 1 < mb; ++b) {
+          const float2 v = s_pref[tid][b];
+          lg = __fadd_rn(lg, v.x);
+          lh = __fadd_rn(lh, v.y);
+          s_pref[tid][b] = make_float2(lg, lh);
+        }
       }
+      __syncthreads();
     }
-    __syncthreads();
     if (tid + 1 < mb) {
       for (uint32_t k = 0; k < cs; ++k) {
         const uint32_t hn = heap_index(level, c0 + k);
@@ -518,34 +570,70 @@ __global__ void __launch_bounds__(256) k_split_scan(const JobDev* __restrict__ j
 }
 
 // A11: decode the level's argmax, threshold lookup, min_split_gain stop
-// (symmetric_tree.cpp:264-282).
-__global__ void k_finalize(const JobDev* __restrict__ jobs, uint32_t level) {
+// (symmetric_tree.cpp:264-282).  Workgroup (node n, job): every workgroup
+// decodes the key; workgroup 0 records the split.  Workgroup n then forms
+// node n's child totals from its histogram of the split feature -- left =
+// bins <= threshold plus the NaN slot 255, right = the other value bins,
+// exact -- and sets the node's partition cursors.
+__global__ void __launch_bounds__(256) k_finalize(const JobDev* __restrict__ jobs, uint32_t level) {
   const JobDev& J = jobs[blockIdx.y];
-  if (threadIdx.x != 0 || !J.st->active) return;
   TreeState* st = J.st;
+  if (!st->active) return;
+  const uint32_t tid = threadIdx.x, n = blockIdx.x;
   const unsigned long long key = st->best[level];
-  if (key == kEmptyKey) { st->active = 0; return; }
   const float gain = float_unorder(static_cast<uint32_t>(key >> 32));
+  const bool accept = key != kEmptyKey && gain > -1e29f && !(gain < J.min_split_gain);
   const uint32_t fb = 0xFFFFFFFFu - static_cast<uint32_t>(key & 0xFFFFFFFFu);
-  const uint32_t f = fb >> 8, b = fb & 255u;
-  if (!(gain > -1e29f) || gain < J.min_split_gain) { st->active = 0; return; }
-  float fthr = 0.f;
-  const uint32_t ne = J.n_edges[f];
-  if (b < ne) fthr = J.edges[static_cast<size_t>(f) * 255 + b];
-  else if (ne > 0) fthr = J.edges[static_cast<size_t>(f) * 255 + ne - 1];
-  const size_t o = static_cast<size_t>(st->tree) * J.D + level;
-  J.m_feat[o] = static_cast<uint16_t>(f);
-  J.m_bin[o] = static_cast<uint8_t>(b);
-  J.m_fthr[o] = fthr;
-  J.m_gain[o] = gain;
-  st->depth = static_cast<int32_t>(level) + 1;
+  const uint32_t f = fb >> 8, thr = fb & 255u;
+  if (n == 0 && tid == 0) {
+    if (!accept) {
+      st->active = 0;
+    } else {
+      float fthr = 0.f;
+      const uint32_t ne = J.n_edges[f];
+      if (thr < ne) fthr = J.edges[static_cast<size_t>(f) * 255 + thr];
+      else if (ne > 0) fthr = J.edges[static_cast<size_t>(f) * 255 + ne - 1];
+      const size_t o = static_cast<size_t>(st->tree) * J.D + level;
+      J.m_feat[o] = static_cast<uint16_t>(f);
+      J.m_bin[o] = static_cast<uint8_t>(thr);
+      J.m_fthr[o] = fthr;
+      J.m_gain[o] = gain;
+      st->depth = static_cast<int32_t>(level) + 1;
+    }
+  }
+  if (!accept || n >= (1u << level)) return;
+  const HistCell hc = J.hist[(static_cast<size_t>(heap_index(level, n)) * J.F + f) * 256u + tid];
+  const bool left = tid <= thr || tid == 255u;
+  long long v[4] = {left ? hc.g : 0, left ? hc.h : 0, left ? 0 : hc.g, left ? 0 : hc.h};
+  __shared__ long long s_v[4][4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_down(v[k], o, 64);
+    if ((tid & 63) == 0) s_v[k][tid >> 6] = v[k];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    long long t[4];
+    for (int k = 0; k < 4; ++k) t[k] = s_v[k][0] + s_v[k][1] + s_v[k][2] + s_v[k][3];
+    const double gi = st->ginv, hi = st->hinv;
+    const uint32_t cl = heap_index(level + 1, 2 * n);
+    J.ng[cl] = static_cast<float>(static_cast<double>(t[0]) * gi);
+    J.nh[cl] = static_cast<float>(static_cast<double>(t[1]) * hi);
+    J.ng[cl + 1] = static_cast<float>(static_cast<double>(t[2]) * gi);
+    J.nh[cl + 1] = static_cast<float>(static_cast<double>(t[3]) * hi);
+    const uint32_t* offd = J.off + static_cast<size_t>(level) * ((1u << J.D) + 1u);
+    J.cursor[2 * n] = offd[n];
+    J.cursor[2 * n + 1] = offd[n + 1];
+  }
 }
 
 // ---------------------------------------------------------------------------
-// A12: partition.  Pass 1 counts left rows and sums child statistics per work
-// item (double, fixed reduction order); the plan kernel scans them per node;
-// pass 3 writes each row to its child segment, stable.  Routing: NaN bin 255
-// or bin <= threshold goes left (symmetric_tree.cpp:309-311).
+// A12: partition, one pass.  Each work item (<= 4096 rows of one node)
+// routes its rows -- NaN bin 255 or bin <= threshold goes left
+// (symmetric_tree.cpp:309-311) -- keeping them in registers, reserves room
+// with one atomic per side on the node's cursors (left fills up from the
+// node's start, right fills down from its end) and writes the rows, placed
+// by wave ballots.
 __device__ inline bool part_active(const JobDev& J, uint32_t level) {
   return J.st->depth == static_cast<int32_t>(level) + 1 && J.st->tree < static_cast<int>(J.n_trees);
 }
@@ -554,10 +642,11 @@ __device__ inline uint32_t level_row(const JobDev& J, uint32_t level, uint32_t p
   return (level == 0 && J.root_identity) ? pos : J.rows[static_cast<size_t>(level) * J.N + pos];
 }
 
-__global__ void __launch_bounds__(256) k_part_count(const JobDev* __restrict__ jobs,
-                                                    const PartWork* __restrict__ work,
-                                                    const uint32_t* __restrict__ count,
-                                                    uint32_t level) {
+__global__ void __launch_bounds__(256) k_partition(const JobDev* __restrict__ jobs,
+                                                   const PartWork* __restrict__ work,
+                                                   const uint32_t* __restrict__ count,
+                                                   uint32_t level) {
+  constexpr int kIt = kPartChunk / 256;  // rows per lane
   if (blockIdx.x >= *count) return;
   const PartWork w = work[blockIdx.x];
   const JobDev& J = jobs[w.job];
@@ -565,37 +654,54 @@ __global__ void __launch_bounds__(256) k_part_count(const JobDev* __restrict__ j
   const size_t o = static_cast<size_t>(J.st->tree) * J.D + level;
   const uint8_t* col = J.bins_cm + static_cast<size_t>(J.m_feat[o]) * J.N;
   const uint8_t thr = J.m_bin[o];
-  const uint32_t tid = threadIdx.x;
-  uint32_t nl = 0;
-  double gl = 0.0, hl = 0.0, gr = 0.0, hr = 0.0;
-  for (uint32_t j = tid; j < w.len; j += 256) {
-    const uint32_t row = level_row(J, level, w.start + j);
-    const uint8_t b = col[row];
-    const float2 v = J.gh[row];
-    if (b == 255 || b <= thr) { ++nl; gl += v.x; hl += v.y; }
-    else { gr += v.x; hr += v.y; }
-  }
-  __shared__ double s_d[4][4];
-  __shared__ uint32_t s_u[4];
-  nl = warp_sum_u(nl);
-  gl = warp_sum_d(gl); hl = warp_sum_d(hl); gr = warp_sum_d(gr); hr = warp_sum_d(hr);
-  const uint32_t wv = tid >> 6;
-  if ((tid & 63) == 0) { s_u[wv] = nl; s_d[wv][0] = gl; s_d[wv][1] = hl; s_d[wv][2] = gr; s_d[wv][3] = hr; }
-  __syncthreads();
-  if (tid == 0) {
-    uint32_t a = 0;
-    double st[4] = {0, 0, 0, 0};
-    for (int i = 0; i < 4; ++i) {
-      a += s_u[i];
-      for (int k = 0; k < 4; ++k) st[k] += s_d[i][k];
+  uint32_t* rows_n = J.rows + static_cast<size_t>(level + 1) * J.N;
+  uint32_t* cur = J.cursor + 2 * w.node;
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  __shared__ uint32_t s_cnt[2][kIt * 4];  // per (step, wave): count -> offset in the item
+  __shared__ uint32_t s_base[2];
+  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  // route every row of the item, keeping row ids and directions in registers
+  uint32_t rv[kIt];
+  uint32_t lbits = 0, rbits = 0;
+#pragma unroll
+  for (int k = 0; k < kIt; ++k) {
+    const uint32_t j = k * 256 + tid;
+    const bool valid = j < w.len;
+    rv[k] = valid ? level_row(J, level, w.start + j) : 0u;
+    const uint8_t b = valid ? col[rv[k]] : 0;
+    const bool left = valid && (b == 255 || b <= thr);
+    const bool right = valid && !left;
+    lbits |= static_cast<uint32_t>(left) << k;
+    rbits |= static_cast<uint32_t>(right) << k;
+    const unsigned long long ml = __ballot(left), mr = __ballot(right);
+    if (lane == 0) {
+      s_cnt[0][k * 4 + wv] = __popcll(ml);
+      s_cnt[1][k * 4 + wv] = __popcll(mr);
     }
-    J.part_left[w.slot] = a;
-    for (int k = 0; k < 4; ++k) J.part_stats[4 * static_cast<size_t>(w.slot) + k] = st[k];
+  }
+  __syncthreads();
+  if (tid < 2) {  // offsets in (step, wave) order; one reservation per side
+    uint32_t acc = 0;
+    for (int i = 0; i < kIt * 4; ++i) {
+      const uint32_t v = s_cnt[tid][i];
+      s_cnt[tid][i] = acc;
+      acc += v;
+    }
+    s_base[tid] = acc == 0 ? 0u : (tid == 0 ? atomicAdd(&cur[0], acc) : atomicSub(&cur[1], acc) - acc);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kIt; ++k) {
+    const bool left = (lbits >> k) & 1u, right = (rbits >> k) & 1u;
+    const unsigned long long ml = __ballot(left), mr = __ballot(right);
+    if (left) rows_n[s_base[0] + s_cnt[0][k * 4 + wv] + __popcll(ml & lt)] = rv[k];
+    if (right) rows_n[s_base[1] + s_cnt[1][k * 4 + wv] + __popcll(mr & lt)] = rv[k];
   }
 }
 
-// Per job: child offsets and totals, level stop test, next level's work
-// lists (block-wide scans over nodes, then one lane per work item).
+// Per job, after the partition: child counts and offsets, the next level's
+// stop test, and the next level's work lists (block-wide scans over nodes,
+// then one lane per work item).
 __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jobs, Lists L,
                                                    uint32_t level) {
   const uint32_t job = blockIdx.y, tid = threadIdx.x;
@@ -605,7 +711,6 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
   const uint32_t stride = (1u << J.D) + 1u;
   const uint32_t* offd = J.off + static_cast<size_t>(level) * stride;
   uint32_t* offn = J.off + static_cast<size_t>(level + 1) * stride;
-  __shared__ uint32_t s_slot[1 << (kMaxDepth - 1)];  // per node: partition slots -> first slot
   __shared__ uint32_t s_hw[1 << (kMaxDepth - 1)];    // per pair: hist work items -> first item
   __shared__ uint32_t s_pw[1 << kMaxDepth];          // per child: partition slots -> first slot
   __shared__ uint32_t s_off[(1 << kMaxDepth) + 1];   // child offsets
@@ -614,38 +719,21 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
   __shared__ int s_any;
   __shared__ unsigned long long s_rows;
   if (tid == 0) { s_any = 0; s_rows = 0ull; }
-  for (uint32_t n = tid; n < nn; n += 256) s_slot[n] = part_slots(offd[n + 1] - offd[n]);
   __syncthreads();
-  block_scan_excl(s_slot, nn, s_tmp);
   const float mcw2 = J.min_child_weight * 2.0f;
   for (uint32_t n = tid; n < nn; n += 256) {
     const uint32_t len = offd[n + 1] - offd[n];
-    const uint32_t b0 = s_slot[n], b1 = b0 + part_slots(len);
-    uint32_t acc = 0;
-    double gl = 0, hl = 0, gr = 0, hr = 0;
-    for (uint32_t s = b0; s < b1; ++s) {
-      J.part_loff[s] = acc;
-      acc += J.part_left[s];
-      gl += J.part_stats[4 * static_cast<size_t>(s) + 0];
-      hl += J.part_stats[4 * static_cast<size_t>(s) + 1];
-      gr += J.part_stats[4 * static_cast<size_t>(s) + 2];
-      hr += J.part_stats[4 * static_cast<size_t>(s) + 3];
-    }
-    const uint32_t hl_i = heap_index(level + 1, 2 * n), hr_i = hl_i + 1;
-    const float fhl = static_cast<float>(hl), fhr = static_cast<float>(hr);
-    J.nc[hl_i] = acc;
-    J.nc[hr_i] = len - acc;
-    J.ng[hl_i] = static_cast<float>(gl);
-    J.nh[hl_i] = fhl;
-    J.ng[hr_i] = static_cast<float>(gr);
-    J.nh[hr_i] = fhr;
+    const uint32_t acc = J.cursor[2 * n] - offd[n];  // rows sent left
+    const uint32_t cl = heap_index(level + 1, 2 * n);
+    J.nc[cl] = acc;
+    J.nc[cl + 1] = len - acc;
     s_off[2 * n] = offd[n];
     s_off[2 * n + 1] = offd[n] + acc;
     offn[2 * n] = offd[n];
     offn[2 * n + 1] = offd[n] + acc;
     // stop test of the next level (symmetric_tree.cpp:115-124)
-    if ((acc >= 2u * J.min_samples_leaf && fhl >= mcw2) ||
-        (len - acc >= 2u * J.min_samples_leaf && fhr >= mcw2))
+    if ((acc >= 2u * J.min_samples_leaf && J.nh[cl] >= mcw2) ||
+        (len - acc >= 2u * J.min_samples_leaf && J.nh[cl + 1] >= mcw2))
       s_any = 1;
     // next level: histogram of the smaller child of the pair (ties: left),
     // partition slots of both children
@@ -689,82 +777,6 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
     put_part_work(pout + e, job, c, start,
                   min(static_cast<uint32_t>(kPartChunk), s_off[c + 1] - start), e);
   }
-}
-
-__global__ void __launch_bounds__(256) k_part_scatter(const JobDev* __restrict__ jobs,
-                                                      const PartWork* __restrict__ work,
-                                                      const uint32_t* __restrict__ count,
-                                                      uint32_t level) {
-  if (blockIdx.x >= *count) return;
-  const PartWork w = work[blockIdx.x];
-  const JobDev& J = jobs[w.job];
-  if (!part_active(J, level)) return;
-  const size_t o = static_cast<size_t>(J.st->tree) * J.D + level;
-  const uint8_t* col = J.bins_cm + static_cast<size_t>(J.m_feat[o]) * J.N;
-  const uint8_t thr = J.m_bin[o];
-  const uint32_t stride = (1u << J.D) + 1u;
-  const uint32_t* offd = J.off + static_cast<size_t>(level) * stride;
-  const uint32_t* offn = J.off + static_cast<size_t>(level + 1) * stride;
-  uint32_t* rows_n = J.rows + static_cast<size_t>(level + 1) * J.N;
-  const uint32_t loff = J.part_loff[w.slot];
-  const uint32_t before = w.start - offd[w.node];
-  uint32_t lpos = offn[2 * w.node] + loff;
-  uint32_t rpos = offn[2 * w.node + 1] + (before - loff);
-  const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  __shared__ uint32_t s_cnt[2][4];
-  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (uint32_t j0 = 0; j0 < w.len; j0 += 256) {
-    const uint32_t j = j0 + tid;
-    const bool valid = j < w.len;
-    const uint32_t row = valid ? level_row(J, level, w.start + j) : 0u;
-    const uint8_t b = valid ? col[row] : 0;
-    const bool left = valid && (b == 255 || b <= thr);
-    const bool right = valid && !left;
-    const unsigned long long ml = __ballot(left), mr = __ballot(right);
-    if (lane == 0) { s_cnt[0][wv] = __popcll(ml); s_cnt[1][wv] = __popcll(mr); }
-    __syncthreads();
-    uint32_t pl = 0, pr = 0, tl = 0, tr = 0;
-    for (uint32_t k = 0; k < 4; ++k) {
-      if (k < wv) { pl += s_cnt[0][k]; pr += s_cnt[1][k]; }
-      tl += s_cnt[0][k];
-      tr += s_cnt[1][k];
-    }
-    if (left) rows_n[lpos + pl + __popcll(ml & lt)] = row;
-    if (right) rows_n[rpos + pr + __popcll(mr & lt)] = row;
-    lpos += tl;
-    rpos += tr;
-    __syncthreads();
-  }
-}
-
-// ---------------------------------------------------------------------------
-// A14: leaf values of the finished tree (compute_leaf_value,
-// symmetric_tree.cpp:753-775).
-__global__ void __launch_bounds__(256) k_leaves(const JobDev* __restrict__ jobs) {
-  const JobDev& J = jobs[blockIdx.y];
-  const int t = J.st->tree;
-  if (t < 0 || t >= static_cast<int>(J.n_trees)) return;
-  const int depth = J.st->depth;
-  const uint32_t nl = 1u << depth;
-  float* out = J.m_leaves + static_cast<size_t>(t) * (1u << J.D);
-  for (uint32_t i = threadIdx.x; i < (1u << J.D); i += blockDim.x) {
-    float v = 0.f;
-    if (i < nl) {
-      const uint32_t hi = heap_index(depth, i);
-      const float G = J.ng[hi], H = J.nh[hi];
-      if (!(H < 1e-10f)) {
-        v = __fdiv_rn(-G, __fadd_rn(H, J.lambda_l2));
-        if (J.lambda_l1 > 0.f) {
-          if (v > J.lambda_l1) v = __fsub_rn(v, J.lambda_l1);
-          else if (v < -J.lambda_l1) v = __fadd_rn(v, J.lambda_l1);
-          else v = 0.f;
-        }
-        if (J.max_delta_step > 0.f) v = fmaxf(-J.max_delta_step, fminf(J.max_delta_step, v));
-      }
-    }
-    out[i] = v;
-  }
-  if (threadIdx.x == 0) J.m_depth[t] = static_cast<uint16_t>(depth);
 }
 
 // Copy externally supplied gradients (teacher forcing) into gh, recording
