@@ -23,6 +23,7 @@ constexpr int kMaxDepth = 10;          // 2^10 leaves; u16 leaf index range
 constexpr int kTileF = 16;             // features per histogram tile (16 B)
 constexpr int kHistThreads = 512;
 constexpr int kPartChunk = 4096;       // rows per partition work item
+constexpr int kPartThreads = 1024;
 constexpr int kScanNodesPerPass = 32;  // node prefixes staged in LDS at a time
 constexpr uint32_t kEmptyKey = 0u;
 
@@ -35,7 +36,6 @@ struct TreeState {
   int32_t tree;         // index of the tree being built
   uint32_t gmax_next;   // float bits of max |g| of the gradients being written
   uint32_t hmax_next;   // float bits of max |h| of the gradients being written
-  uint32_t pad;
   unsigned long long best[kMaxDepth];  // argmax key per level
   // Fixed-point scales of the current tree's histograms: a row's g enters as
   // round(g * gscale) (int64), h as round(h * hscale).  Powers of two chosen

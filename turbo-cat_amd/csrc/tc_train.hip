@@ -35,7 +35,6 @@ __global__ void k_init_state(TreeState* st) {
   st->tree = -1;
   st->gmax_next = 0u;
   st->hmax_next = 0u;
-  st->pad = 0u;
   for (int d = 0; d < kMaxDepth; ++d) st->best[d] = kEmptyKey;
   st->gscale = st->hscale = st->ginv = st->hinv = 1.0;
 }
@@ -133,7 +132,7 @@ class Trainer {
 
   void init_gradients() {
     hipLaunchKernelGGL(k_grad_update, dim3(total_grad_wg_), dim3(256), 0, ctx_->stream, d_jobs_,
-                       d_job_wg_off_, njobs_, 0);
+                       d_job_wg_off_, njobs_, 0, lists_);
     TC_HIP(hipGetLastError());
   }
 
@@ -156,9 +155,9 @@ class Trainer {
   // histogram launch for live timing.
   void enqueue_tree(std::vector<std::pair<hipEvent_t, hipEvent_t>>* hist_events) {
     hipStream_t s = ctx_->stream;
-    hipLaunchKernelGGL(k_root_partials, dim3(total_grad_wg_), dim3(256), 0, s, d_jobs_,
-                       d_job_wg_off_, njobs_, lists_);
     const bool packed = packed_hist();
+    hipLaunchKernelGGL(k_root_partials, dim3(total_grad_wg_), dim3(256), 0, s, d_jobs_,
+                       d_job_wg_off_, njobs_);
     hipLaunchKernelGGL(k_tree_begin, dim3(1, njobs_), dim3(256), 0, s, d_jobs_, lists_,
                        packed ? 1 : 0);
     for (uint32_t d = 0; d < D_; ++d) {
@@ -178,20 +177,17 @@ class Trainer {
         TC_HIP(hipEventRecord(e1, s));
         hist_events->emplace_back(e0, e1);
       }
-      if (d > 0)
-        hipLaunchKernelGGL(k_subtract, dim3((1u << (d - 1)) * fmax_, njobs_), dim3(256), 0, s,
-                           d_jobs_, d, fmax_);
-      hipLaunchKernelGGL(k_split_scan<true>, dim3(fmax_, njobs_), dim3(256), 0, s, d_jobs_,
+      hipLaunchKernelGGL(k_split_scan<true>, dim3(fmax_, njobs_), dim3(1024), 0, s, d_jobs_,
                          static_cast<const float2*>(nullptr), d);
-      hipLaunchKernelGGL(k_finalize, dim3(1u << d, njobs_), dim3(256), 0, s, d_jobs_, d);
+      hipLaunchKernelGGL(k_finalize, dim3(ceil_div(1u << d, 4), njobs_), dim3(256), 0, s, d_jobs_, d);
       if (d + 1 < D_ || keep_rows_) {  // training never reads the last level's rows
-        hipLaunchKernelGGL(k_partition, dim3(part_cap_[d]), dim3(256), 0, s, d_jobs_, lists_.part[d],
+        hipLaunchKernelGGL(k_partition, dim3(part_cap_[d]), dim3(kPartThreads), 0, s, d_jobs_, lists_.part[d],
                            lists_.part_count[d], d);
         hipLaunchKernelGGL(k_part_plan, dim3(1, njobs_), dim3(256), 0, s, d_jobs_, lists_, d);
       }
     }
     hipLaunchKernelGGL(k_grad_update, dim3(total_grad_wg_), dim3(256), 0, s, d_jobs_, d_job_wg_off_,
-                       njobs_, 1);
+                       njobs_, 1, lists_);
     TC_HIP(hipGetLastError());
   }
 

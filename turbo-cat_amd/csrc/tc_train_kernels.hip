@@ -129,7 +129,7 @@ __device__ inline void block_max2_to(TreeState* st, float a, float b) {
 //         compiler's contraction of booster.cpp:460-463), then g/h.
 // Rows in natural order, 1024 per workgroup, every row of the dataset (also
 // when the tree was grown on a row sample).  Also records max |g|, max |h|
-// for the next tree's histogram scales.
+// for the next tree's histogram scales, and empties the work lists.
 constexpr int kGradTile = 1024;
 
 __device__ inline uint32_t job_of_block(const uint32_t* job_wg_off, uint32_t njobs) {
@@ -143,12 +143,16 @@ __device__ inline uint32_t job_of_block(const uint32_t* job_wg_off, uint32_t njo
 
 __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ jobs,
                                                      const uint32_t* __restrict__ job_wg_off,
-                                                     uint32_t njobs, int mode) {
+                                                     uint32_t njobs, int mode, Lists L) {
   __shared__ uint32_t s_job;
   __shared__ uint16_t s_f[kMaxDepth];
   __shared__ uint8_t s_b[kMaxDepth];
   __shared__ float s_leaf[1 << kMaxDepth];
   const uint32_t tid = threadIdx.x;
+  if (blockIdx.x == 0 && tid < kMaxDepth) {  // work lists of the next tree start empty
+    *L.hist_count[tid] = 0;
+    *L.part_count[tid] = 0;
+  }
   if (tid == 0) s_job = job_of_block(job_wg_off, njobs);
   __syncthreads();
   const JobDev& J = jobs[s_job];
@@ -186,88 +190,56 @@ __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ 
     if (lw == 0 && tid == 0) J.m_depth[t] = static_cast<uint16_t>(depth);
     __syncthreads();
   }
-  float gmax = 0.f, hmax = 0.f;
-  for (int k = 0; k < kGradTile / 256; ++k) {
-    const uint32_t row = lw * kGradTile + k * 256 + tid;
-    if (row >= N) break;
-    float pred;
-    if (mode == 1) {
-      uint32_t li = 0;
-      for (int d = 0; d < depth; ++d) {
-        const uint8_t v = J.bins_cm[static_cast<size_t>(s_f[d]) * N + row];
-        li = (li << 1) | ((v != 255 && v > s_b[d]) ? 1u : 0u);
+  // each lane takes rows r + 256k (k < 4); all loads are issued before use
+  constexpr int kR = kGradTile / 256;
+  uint32_t row[kR], li[kR];
+  bool ok[kR];
+#pragma unroll
+  for (int k = 0; k < kR; ++k) {
+    row[k] = lw * kGradTile + k * 256 + tid;
+    ok[k] = row[k] < N;
+    li[k] = 0;
+  }
+  if (mode == 1) {
+    for (int d = 0; d < depth; ++d) {
+      const uint8_t* col = J.bins_cm + static_cast<size_t>(s_f[d]) * N;
+      const uint8_t thr = s_b[d];
+#pragma unroll
+      for (int k = 0; k < kR; ++k) {
+        const uint8_t v = ok[k] ? col[row[k]] : 0;
+        li[k] = (li[k] << 1) | ((v != 255 && v > thr) ? 1u : 0u);
       }
-      pred = __fmaf_rn(J.lr, s_leaf[li], J.preds[row]);
-    } else {
-      pred = J.base;
     }
-    J.preds[row] = pred;
-    const float2 gh = row_gradient(J.loss, pred, J.y[row], row < vecN);
-    J.gh[row] = gh;
+  }
+  float prev[kR], lab[kR];
+#pragma unroll
+  for (int k = 0; k < kR; ++k) {
+    prev[k] = (ok[k] && mode == 1) ? J.preds[row[k]] : 0.f;
+    lab[k] = ok[k] ? J.y[row[k]] : 0.f;
+  }
+  float gmax = 0.f, hmax = 0.f;
+#pragma unroll
+  for (int k = 0; k < kR; ++k) {
+    if (!ok[k]) continue;
+    const float pred = mode == 1 ? __fmaf_rn(J.lr, s_leaf[li[k]], prev[k]) : J.base;
+    J.preds[row[k]] = pred;
+    const float2 gh = row_gradient(J.loss, pred, lab[k], row[k] < vecN);
+    J.gh[row[k]] = gh;
     gmax = fmaxf(gmax, fabsf(gh.x));
     hmax = fmaxf(hmax, fabsf(gh.y));
   }
   block_max2_to(J.st, gmax, hmax);
 }
 
-// Start of a tree, over the row grid: root totals (sum g, sum h) over
-// 1024-position tiles of the root list rows[0][0 .. n_root), double, fixed
-// order within the tile (k_tree_begin adds the tiles in a fixed order);
-// zeroed histograms of levels 0 and 1 (deeper levels are zeroed by the
-// previous level's k_subtract); workgroup 0 resets the work-list counters.
-__global__ void __launch_bounds__(256) k_root_partials(const JobDev* __restrict__ jobs,
-                                                       const uint32_t* __restrict__ job_wg_off,
-                                                       uint32_t njobs, Lists L) {
-  __shared__ uint32_t s_job;
-  __shared__ double s_red[2][4];
-  const uint32_t tid = threadIdx.x;
-  if (tid == 0) s_job = job_of_block(job_wg_off, njobs);
-  if (blockIdx.x == 0 && tid < kMaxDepth) {
-    *L.hist_count[tid] = 0;
-    *L.part_count[tid] = 0;
-  }
-  __syncthreads();
-  const JobDev& J = jobs[s_job];
-  const uint32_t lw = blockIdx.x - J.grad_wg_off;
-  double sg = 0.0, sh = 0.0;
-  if (J.st->tree + 1 < static_cast<int>(J.n_trees)) {
-    const uint32_t cells = (J.D > 1 ? 3u : 1u) * J.F * 256u;  // heap nodes 0..2
-    const uint32_t per = (cells + J.grad_wg_cnt - 1) / J.grad_wg_cnt;
-    const uint32_t c1 = min(cells, (lw + 1) * per);
-    for (uint32_t c = lw * per + tid; c < c1; c += 256) J.hist[c] = HistCell{0, 0};
-    for (int k = 0; k < kGradTile / 256; ++k) {
-      const uint32_t p = lw * kGradTile + k * 256 + tid;
-      if (p >= J.n_root) break;
-      const float2 v = J.gh[J.root_identity ? p : J.rows[p]];
-      sg += v.x;
-      sh += v.y;
-    }
-  }
-  sg = warp_sum_d(sg);
-  sh = warp_sum_d(sh);
-  if ((tid & 63) == 0) { s_red[0][tid >> 6] = sg; s_red[1][tid >> 6] = sh; }
-  __syncthreads();
-  if (tid == 0) {
-    double a = 0.0, b = 0.0;
-    for (int w = 0; w < 4; ++w) { a += s_red[0][w]; b += s_red[1][w]; }
-    J.grad_partials[2 * lw + 0] = a;
-    J.grad_partials[2 * lw + 1] = b;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Root of a new tree (symmetric_tree.cpp:97-124): totals from the root
-// partials (double, fixed order), level-0 stop test, the tree's histogram
-// scales, root work lists.
+// Root of a new tree (symmetric_tree.cpp:97-124), one 256-thread workgroup
+// per job: totals from the root partials (double, fixed order), level-0 stop
+// test, the tree's histogram scales, root work lists.
 // packed: histograms use the packed MSE cell (h == 2 for every row).
-__global__ void __launch_bounds__(256) k_tree_begin(const JobDev* __restrict__ jobs, Lists L,
-                                                    int packed) {
-  const uint32_t job = blockIdx.y;
-  const JobDev& J = jobs[job];
+__device__ void tree_begin(const JobDev& J, uint32_t job, const Lists& L, int packed) {
   const uint32_t tid = threadIdx.x;
   __shared__ int s_go;
   __shared__ uint32_t s_hbase, s_pbase;
-  __shared__ double s_red[2][4];
+  __shared__ double s_tot[2][4];
   const uint32_t n = J.n_root;
   const uint32_t hcnt = ((n + J.hist_chunk - 1) / J.hist_chunk) * J.T, pcnt = part_slots(n);
   {  // thread t adds tiles t, t + 256, ... in order; then a fixed tree
@@ -278,7 +250,7 @@ __global__ void __launch_bounds__(256) k_tree_begin(const JobDev* __restrict__ j
     }
     a = warp_sum_d(a);
     b = warp_sum_d(b);
-    if ((tid & 63) == 0) { s_red[0][tid >> 6] = a; s_red[1][tid >> 6] = b; }
+    if ((tid & 63) == 0) { s_tot[0][tid >> 6] = a; s_tot[1][tid >> 6] = b; }
     __syncthreads();
   }
   if (tid == 0) {
@@ -292,8 +264,8 @@ __global__ void __launch_bounds__(256) k_tree_begin(const JobDev* __restrict__ j
     st->hmax_next = 0u;
     int go = 0;
     if (t < static_cast<int>(J.n_trees)) {
-      const double G = (s_red[0][0] + s_red[0][1]) + (s_red[0][2] + s_red[0][3]);
-      const double H = (s_red[1][0] + s_red[1][1]) + (s_red[1][2] + s_red[1][3]);
+      const double G = (s_tot[0][0] + s_tot[0][1]) + (s_tot[0][2] + s_tot[0][3]);
+      const double H = (s_tot[1][0] + s_tot[1][1]) + (s_tot[1][2] + s_tot[1][3]);
       J.ng[0] = static_cast<float>(G);
       J.nh[0] = static_cast<float>(H);
       J.nc[0] = n;
@@ -328,6 +300,52 @@ __global__ void __launch_bounds__(256) k_tree_begin(const JobDev* __restrict__ j
     put_part_work(L.part[0] + s_pbase + e, job, 0, start,
                   min(static_cast<uint32_t>(kPartChunk), n - start), e);
   }
+}
+
+// Start of a tree, over the row grid: root totals (sum g, sum h) over
+// 1024-position tiles of the root list rows[0][0 .. n_root), double, fixed
+// order within the tile (k_tree_begin adds the tiles in a fixed order); the
+// root's histogram zeroed (deeper levels are zeroed by the split scan of the
+// level above).
+__global__ void __launch_bounds__(256) k_root_partials(const JobDev* __restrict__ jobs,
+                                                       const uint32_t* __restrict__ job_wg_off,
+                                                       uint32_t njobs) {
+  __shared__ uint32_t s_job;
+  __shared__ double s_red[2][4];
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) s_job = job_of_block(job_wg_off, njobs);
+  __syncthreads();
+  const JobDev& J = jobs[s_job];
+  const uint32_t lw = blockIdx.x - J.grad_wg_off;
+  double sg = 0.0, sh = 0.0;
+  if (J.st->tree + 1 < static_cast<int>(J.n_trees)) {
+    const uint32_t cells = J.F * 256u;  // the root's histogram
+    const uint32_t per = (cells + J.grad_wg_cnt - 1) / J.grad_wg_cnt;
+    const uint32_t c1 = min(cells, (lw + 1) * per);
+    for (uint32_t c = lw * per + tid; c < c1; c += 256) J.hist[c] = HistCell{0, 0};
+    for (int k = 0; k < kGradTile / 256; ++k) {
+      const uint32_t p = lw * kGradTile + k * 256 + tid;
+      if (p >= J.n_root) break;
+      const float2 v = J.gh[J.root_identity ? p : J.rows[p]];
+      sg += v.x;
+      sh += v.y;
+    }
+  }
+  sg = warp_sum_d(sg);
+  sh = warp_sum_d(sh);
+  if ((tid & 63) == 0) { s_red[0][tid >> 6] = sg; s_red[1][tid >> 6] = sh; }
+  __syncthreads();
+  if (tid == 0) {
+    double a = 0.0, b = 0.0;
+    for (int w = 0; w < 4; ++w) { a += s_red[0][w]; b += s_red[1][w]; }
+    J.grad_partials[2 * lw + 0] = a;
+    J.grad_partials[2 * lw + 1] = b;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_tree_begin(const JobDev* __restrict__ jobs, Lists L,
+                                                    int packed) {
+  tree_begin(jobs[blockIdx.y], blockIdx.y, L, packed);
 }
 
 // ---------------------------------------------------------------------------
@@ -442,31 +460,6 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
   }
 }
 
-// A13: larger child = parent - smaller child, over every (feature, slot)
-// (Histogram::subtract_from, histogram.cpp:39-42 via symmetric_tree.cpp:346),
-// exact in fixed point.  Workgroup = (pair, feature), thread = slot.  Also
-// zeroes the four level+1 histograms below the pair (feature f), which the
-// next level's k_hist accumulates into.
-__global__ void __launch_bounds__(256) k_subtract(const JobDev* __restrict__ jobs,
-                                                  uint32_t level, uint32_t fmax) {
-  const JobDev& J = jobs[blockIdx.y];
-  if (!J.st->active) return;
-  const uint32_t pair = blockIdx.x / fmax, f = blockIdx.x % fmax;
-  if (f >= J.F) return;
-  const uint32_t lc = 2 * pair, rc = lc + 1;
-  const uint32_t nl = J.nc[heap_index(level, lc)], nr = J.nc[heap_index(level, rc)];
-  const uint32_t sm = nl <= nr ? lc : rc, lg = nl <= nr ? rc : lc;
-  const size_t F = J.F;
-  const uint32_t b = threadIdx.x;
-  const HistCell p = J.hist[(heap_index(level - 1, pair) * F + f) * 256u + b];
-  const HistCell c = J.hist[(heap_index(level, sm) * F + f) * 256u + b];
-  J.hist[(heap_index(level, lg) * F + f) * 256u + b] = HistCell{p.g - c.g, p.h - c.h};
-  if (level + 1 < J.D) {
-    for (uint32_t k = 0; k < 4; ++k)
-      J.hist[(heap_index(level + 1, 4 * pair + k) * F + f) * 256u + b] = HistCell{0, 0};
-  }
-}
-
 // ---------------------------------------------------------------------------
 // A10: level split scan.  Workgroup = (feature, job), thread = candidate bin.
 // Per node the left sums of every candidate are staged in LDS (up to 32
@@ -478,10 +471,15 @@ __global__ void __launch_bounds__(256) k_subtract(const JobDev* __restrict__ job
 // bins): the reference's sequential float prefix, so for a given float
 // histogram every left sum, gain and total is the reference's float value.
 template <bool FIXED>
-__global__ void __launch_bounds__(256) k_split_scan(const JobDev* __restrict__ jobs,
-                                                    const float2* __restrict__ fhist,
-                                                    uint32_t level) {
+__global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev* __restrict__ jobs,
+                                                                   const float2* __restrict__ fhist,
+                                                                   uint32_t level) {
+  constexpr uint32_t TH = FIXED ? 1024u : 256u;  // threads
+  constexpr uint32_t NW = TH / 64u;              // waves
+  constexpr uint32_t Q = TH / 256u;              // candidate lanes per bin
   __shared__ float2 s_pref[kScanNodesPerPass][256];
+  __shared__ float s_tot[Q][256];
+  __shared__ uint8_t s_any[Q][256];
   __shared__ unsigned long long s_red[4];
   const uint32_t job = blockIdx.y, f = blockIdx.x, tid = threadIdx.x;
   const JobDev& J = jobs[job];
@@ -490,21 +488,58 @@ __global__ void __launch_bounds__(256) k_split_scan(const JobDev* __restrict__ j
   const uint32_t mb = J.max_bins & 0xFFu;
   const float lambda = J.lambda_l2;
   const double gi = FIXED ? J.st->ginv : 1.0, hi = FIXED ? J.st->hinv : 1.0;
+  const uint32_t lane = tid & 63u, wave = tid >> 6;
+  const uint32_t bin = tid & 255u, part = tid >> 8;  // candidate, node subset
   float total = 0.f;
   bool any = false;
-  const uint32_t lane = tid & 63u, wave = tid >> 6;
   for (uint32_t c0 = 0; c0 < nn; c0 += kScanNodesPerPass) {
     const uint32_t cs = min(static_cast<uint32_t>(kScanNodesPerPass), nn - c0);
     if (FIXED) {
-      // exact prefix: one wave per node, lane l holds bins 4l .. 4l+3
-      for (uint32_t k = wave; k < cs; k += 4) {
-        const HistCell* hp = J.hist + (static_cast<size_t>(heap_index(level, c0 + k)) * J.F + f) * 256u;
+      // exact prefix: one wave per node, lane l holds bins 4l .. 4l+3.  At
+      // level > 0 the larger child of each pair is formed here, parent -
+      // smaller (A13: Histogram::subtract_from, histogram.cpp:39-42 via
+      // symmetric_tree.cpp:346), and stored; the node's two level+1
+      // histograms are zeroed for the next level's k_hist.
+      for (uint32_t k = wave; k < cs; k += NW) {
+        const uint32_t node = c0 + k;
+        const size_t F = J.F;
+        HistCell* hp = J.hist + (static_cast<size_t>(heap_index(level, node)) * F + f) * 256u;
+        HistCell cell[4];
+        bool larger = false;
+        uint32_t sm = node;
+        if (level > 0) {
+          const uint32_t pr = node >> 1;
+          const uint32_t nl = J.nc[heap_index(level, 2 * pr)], nr = J.nc[heap_index(level, 2 * pr + 1)];
+          sm = nl <= nr ? 2 * pr : 2 * pr + 1;
+          larger = node != sm;
+        }
+        if (larger) {
+          const HistCell* pp = J.hist + (static_cast<size_t>(heap_index(level - 1, node >> 1)) * F + f) * 256u;
+          const HistCell* sp = J.hist + (static_cast<size_t>(heap_index(level, sm)) * F + f) * 256u;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const HistCell a = pp[4 * lane + i], b = sp[4 * lane + i];
+            cell[i] = HistCell{a.g - b.g, a.h - b.h};
+            hp[4 * lane + i] = cell[i];
+          }
+        } else {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) cell[i] = hp[4 * lane + i];
+        }
+        if (level + 1 < J.D) {
+          HistCell* z0 = J.hist + (static_cast<size_t>(heap_index(level + 1, 2 * node)) * F + f) * 256u;
+          HistCell* z1 = J.hist + (static_cast<size_t>(heap_index(level + 1, 2 * node + 1)) * F + f) * 256u;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            z0[4 * lane + i] = HistCell{0, 0};
+            z1[4 * lane + i] = HistCell{0, 0};
+          }
+        }
         long long pg[4], ph[4], ag = 0, ah = 0;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const HistCell hc = hp[4 * lane + i];
-          ag += hc.g;
-          ah += hc.h;
+          ag += cell[i].g;
+          ah += cell[i].h;
           pg[i] = ag;
           ph[i] = ah;
         }
@@ -538,12 +573,14 @@ __global__ void __launch_bounds__(256) k_split_scan(const JobDev* __restrict__ j
       }
       __syncthreads();
     }
-    if (tid + 1 < mb) {
-      for (uint32_t k = 0; k < cs; ++k) {
+    // gain of candidate `bin`: node terms in node order (part p takes the
+    // nodes k = p, p + Q, ...; Q = 1 is the reference's single sequence)
+    if (bin + 1 < mb) {
+      for (uint32_t k = part; k < cs; k += Q) {
         const uint32_t hn = heap_index(level, c0 + k);
         if (level > 0 && J.nc[hn] == 0) continue;
         const float G = J.ng[hn], H = J.nh[hn];
-        const float2 Lf = s_pref[k][tid];
+        const float2 Lf = s_pref[k][bin];
         const float rg = __fsub_rn(G, Lf.x), rh = __fsub_rn(H, Lf.y);
         if (Lf.y < 1e-10f || rh < 1e-10f) continue;
         const float pg = H > 1e-10f ? __fdiv_rn(__fmul_rn(G, G), __fadd_rn(H, lambda)) : 0.f;
@@ -555,12 +592,25 @@ __global__ void __launch_bounds__(256) k_split_scan(const JobDev* __restrict__ j
     }
     __syncthreads();
   }
-  unsigned long long key = kEmptyKey;
-  if (tid + 1 < mb && (level > 0 || any) && !isnan(total)) {
-    key = split_key(total == 0.f ? 0.f : total, f, tid);
+  if (Q > 1) {  // the parts' sums, added in part order
+    s_tot[part][bin] = total;
+    s_any[part][bin] = any ? 1 : 0;
+    __syncthreads();
+    if (part == 0) {
+      for (uint32_t p = 1; p < Q; ++p) {
+        total = __fadd_rn(total, s_tot[p][bin]);
+        any = any || s_any[p][bin] != 0;
+      }
+    }
   }
-  key = warp_max_u64(key);
-  if ((tid & 63) == 0) s_red[tid >> 6] = key;
+  if (part == 0) {
+    unsigned long long key = kEmptyKey;
+    if (bin + 1 < mb && (level > 0 || any) && !isnan(total)) {
+      key = split_key(total == 0.f ? 0.f : total, f, bin);
+    }
+    key = warp_max_u64(key);
+    if (lane == 0) s_red[wave] = key;
+  }
   __syncthreads();
   if (tid == 0) {
     unsigned long long m = s_red[0];
@@ -570,22 +620,16 @@ __global__ void __launch_bounds__(256) k_split_scan(const JobDev* __restrict__ j
 }
 
 // A11: decode the level's argmax, threshold lookup, min_split_gain stop
-// (symmetric_tree.cpp:264-282).  Workgroup (node n, job): every workgroup
-// decodes the key; workgroup 0 records the split.  Workgroup n then forms
-// node n's child totals from its histogram of the split feature -- left =
-// bins <= threshold plus the NaN slot 255, right = the other value bins,
-// exact -- and sets the node's partition cursors.
-__global__ void __launch_bounds__(256) k_finalize(const JobDev* __restrict__ jobs, uint32_t level) {
-  const JobDev& J = jobs[blockIdx.y];
+// (symmetric_tree.cpp:264-282), recorded by one thread.
+__device__ inline bool decode_split(const JobDev& J, uint32_t level, bool record, uint32_t* f_out,
+                                    uint32_t* thr_out) {
   TreeState* st = J.st;
-  if (!st->active) return;
-  const uint32_t tid = threadIdx.x, n = blockIdx.x;
   const unsigned long long key = st->best[level];
   const float gain = float_unorder(static_cast<uint32_t>(key >> 32));
   const bool accept = key != kEmptyKey && gain > -1e29f && !(gain < J.min_split_gain);
   const uint32_t fb = 0xFFFFFFFFu - static_cast<uint32_t>(key & 0xFFFFFFFFu);
   const uint32_t f = fb >> 8, thr = fb & 255u;
-  if (n == 0 && tid == 0) {
+  if (record) {
     if (!accept) {
       st->active = 0;
     } else {
@@ -601,30 +645,52 @@ __global__ void __launch_bounds__(256) k_finalize(const JobDev* __restrict__ job
       st->depth = static_cast<int32_t>(level) + 1;
     }
   }
-  if (!accept || n >= (1u << level)) return;
-  const HistCell hc = J.hist[(static_cast<size_t>(heap_index(level, n)) * J.F + f) * 256u + tid];
-  const bool left = tid <= thr || tid == 255u;
-  long long v[4] = {left ? hc.g : 0, left ? hc.h : 0, left ? 0 : hc.g, left ? 0 : hc.h};
-  __shared__ long long s_v[4][4];
+  *f_out = f;
+  *thr_out = thr;
+  return accept;
+}
+
+// Child totals of node n under split (f, thr), by one wave: left = bins <=
+// thr plus the NaN slot 255, right = the other value bins, exact; lane 0
+// stores them (float) with the node's partition cursors.
+__device__ inline void child_totals(const JobDev& J, uint32_t level, uint32_t n, uint32_t f,
+                                    uint32_t thr) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const HistCell* hp = J.hist + (static_cast<size_t>(heap_index(level, n)) * J.F + f) * 256u;
+  long long v[4] = {0, 0, 0, 0};
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_down(v[k], o, 64);
-    if ((tid & 63) == 0) s_v[k][tid >> 6] = v[k];
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t b = 4 * lane + i;
+    const HistCell c = hp[b];
+    const int side = (b <= thr || b == 255u) ? 0 : 2;
+    v[side] += c.g;
+    v[side + 1] += c.h;
   }
-  __syncthreads();
-  if (tid == 0) {
-    long long t[4];
-    for (int k = 0; k < 4; ++k) t[k] = s_v[k][0] + s_v[k][1] + s_v[k][2] + s_v[k][3];
-    const double gi = st->ginv, hi = st->hinv;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_down(v[k], o, 64);
+  if (lane == 0) {
+    const double gi = J.st->ginv, hi = J.st->hinv;
     const uint32_t cl = heap_index(level + 1, 2 * n);
-    J.ng[cl] = static_cast<float>(static_cast<double>(t[0]) * gi);
-    J.nh[cl] = static_cast<float>(static_cast<double>(t[1]) * hi);
-    J.ng[cl + 1] = static_cast<float>(static_cast<double>(t[2]) * gi);
-    J.nh[cl + 1] = static_cast<float>(static_cast<double>(t[3]) * hi);
+    J.ng[cl] = static_cast<float>(static_cast<double>(v[0]) * gi);
+    J.nh[cl] = static_cast<float>(static_cast<double>(v[1]) * hi);
+    J.ng[cl + 1] = static_cast<float>(static_cast<double>(v[2]) * gi);
+    J.nh[cl + 1] = static_cast<float>(static_cast<double>(v[3]) * hi);
     const uint32_t* offd = J.off + static_cast<size_t>(level) * ((1u << J.D) + 1u);
     J.cursor[2 * n] = offd[n];
     J.cursor[2 * n + 1] = offd[n + 1];
   }
+}
+
+// Finalize launch: workgroup (group of 4 nodes, job), one wave per node;
+// workgroup 0 records the split.
+__global__ void __launch_bounds__(256) k_finalize(const JobDev* __restrict__ jobs, uint32_t level) {
+  const JobDev& J = jobs[blockIdx.y];
+  if (!J.st->active) return;
+  uint32_t f, thr;
+  const bool accept = decode_split(J, level, blockIdx.x == 0 && threadIdx.x == 0, &f, &thr);
+  const uint32_t n = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (accept && n < (1u << level)) child_totals(J, level, n, f, thr);
 }
 
 // ---------------------------------------------------------------------------
@@ -642,11 +708,12 @@ __device__ inline uint32_t level_row(const JobDev& J, uint32_t level, uint32_t p
   return (level == 0 && J.root_identity) ? pos : J.rows[static_cast<size_t>(level) * J.N + pos];
 }
 
-__global__ void __launch_bounds__(256) k_partition(const JobDev* __restrict__ jobs,
+__global__ void __launch_bounds__(kPartThreads) k_partition(const JobDev* __restrict__ jobs,
                                                    const PartWork* __restrict__ work,
                                                    const uint32_t* __restrict__ count,
                                                    uint32_t level) {
-  constexpr int kIt = kPartChunk / 256;  // rows per lane
+  constexpr int kIt = kPartChunk / kPartThreads;  // rows per lane
+  constexpr int kW = kPartThreads / 64;           // waves
   if (blockIdx.x >= *count) return;
   const PartWork w = work[blockIdx.x];
   const JobDev& J = jobs[w.job];
@@ -657,7 +724,7 @@ __global__ void __launch_bounds__(256) k_partition(const JobDev* __restrict__ jo
   uint32_t* rows_n = J.rows + static_cast<size_t>(level + 1) * J.N;
   uint32_t* cur = J.cursor + 2 * w.node;
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  __shared__ uint32_t s_cnt[2][kIt * 4];  // per (step, wave): count -> offset in the item
+  __shared__ uint32_t s_cnt[2][kIt * kW];  // per (step, wave): count -> offset in the item
   __shared__ uint32_t s_base[2];
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   // route every row of the item, keeping row ids and directions in registers
@@ -665,7 +732,7 @@ __global__ void __launch_bounds__(256) k_partition(const JobDev* __restrict__ jo
   uint32_t lbits = 0, rbits = 0;
 #pragma unroll
   for (int k = 0; k < kIt; ++k) {
-    const uint32_t j = k * 256 + tid;
+    const uint32_t j = k * kPartThreads + tid;
     const bool valid = j < w.len;
     rv[k] = valid ? level_row(J, level, w.start + j) : 0u;
     const uint8_t b = valid ? col[rv[k]] : 0;
@@ -675,14 +742,14 @@ __global__ void __launch_bounds__(256) k_partition(const JobDev* __restrict__ jo
     rbits |= static_cast<uint32_t>(right) << k;
     const unsigned long long ml = __ballot(left), mr = __ballot(right);
     if (lane == 0) {
-      s_cnt[0][k * 4 + wv] = __popcll(ml);
-      s_cnt[1][k * 4 + wv] = __popcll(mr);
+      s_cnt[0][k * kW + wv] = __popcll(ml);
+      s_cnt[1][k * kW + wv] = __popcll(mr);
     }
   }
   __syncthreads();
   if (tid < 2) {  // offsets in (step, wave) order; one reservation per side
     uint32_t acc = 0;
-    for (int i = 0; i < kIt * 4; ++i) {
+    for (int i = 0; i < kIt * kW; ++i) {
       const uint32_t v = s_cnt[tid][i];
       s_cnt[tid][i] = acc;
       acc += v;
@@ -694,8 +761,8 @@ __global__ void __launch_bounds__(256) k_partition(const JobDev* __restrict__ jo
   for (int k = 0; k < kIt; ++k) {
     const bool left = (lbits >> k) & 1u, right = (rbits >> k) & 1u;
     const unsigned long long ml = __ballot(left), mr = __ballot(right);
-    if (left) rows_n[s_base[0] + s_cnt[0][k * 4 + wv] + __popcll(ml & lt)] = rv[k];
-    if (right) rows_n[s_base[1] + s_cnt[1][k * 4 + wv] + __popcll(mr & lt)] = rv[k];
+    if (left) rows_n[s_base[0] + s_cnt[0][k * kW + wv] + __popcll(ml & lt)] = rv[k];
+    if (right) rows_n[s_base[1] + s_cnt[1][k * kW + wv] + __popcll(mr & lt)] = rv[k];
   }
 }
 
