@@ -32,6 +32,33 @@ sys.path.insert(0, ROOT)
 
 METRIC = "hist GB/s, trees/s train, rows/s predict + p50/p99 serve latency vs plain"
 HBM_PEAK_GBPS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_hist_train_1Mx100_d6_traffic.json")
+KERNEL_SOURCES = ["tc_common.h", "tc_train_kernels.hip", "tc_train.hip"]
+
+
+def kernel_sources_hash():
+    import hashlib
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "turbo-cat_amd", "csrc", name), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def measured_traffic(rows, features, depth, hist_chunk):
+    """HBM bytes per k_hist launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE measurement (copy-calibrated), used only when it was taken on
+    this workload with these exact kernel sources; otherwise None."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    same = (t.get("rows"), t.get("features"), t.get("depth"), t.get("hist_chunk")) == \
+        (rows, features, depth, hist_chunk)
+    if not same or t.get("sources_sha256") != kernel_sources_hash():
+        return None
+    return t["traffic_bytes_per_launch"]
 
 
 def log(*a):
@@ -183,7 +210,8 @@ def run_train(args):
                                f"lambda 1", "rows": n, "features": F, "depth": 6,
                    "trees_timed": T_timed},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": measured_traffic(n, F, 6, args.hist_chunk),
                      "kernel": "k_hist", "avg_launch_us": round(hist_ms / launches * 1e3, 2),
                      "algorithmic_bytes_per_launch": int(alg_bytes / launches)},
         "cpu_baseline": cpu,
@@ -341,12 +369,61 @@ def run_batched(args):
     print(json.dumps(res), flush=True)
 
 
+def run_copy(args):
+    """Calibration for HBM byte counters: a plain device copy of a known
+    size (256 MiB read, 256 MiB written per copy)."""
+    import torch
+    n = 64 << 20
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(max(1, args.warmup)):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    print(json.dumps({"metric": "device copy", "bytes_read_per_copy": 4 * n,
+                      "bytes_written_per_copy": 4 * n, "copies": args.steps + max(1, args.warmup),
+                      "GB/s": round(8 * n / dt / 1e9, 1)}), flush=True)
+
+
+def pmc_summary(root, kernel_regex):
+    """Average value per dispatch of every counter in the rocprofv3 counter
+    collection CSVs under `root`, for the kernels matching kernel_regex."""
+    import csv
+    import glob
+    import re
+    pat = re.compile(kernel_regex)
+    acc = {}
+    for path in glob.glob(os.path.join(root, "**", "*counter_collection.csv"), recursive=True):
+        with open(path, newline="") as fh:
+            for r in csv.DictReader(fh):
+                name = r.get("Kernel_Name", "")
+                if not pat.search(name):
+                    continue
+                key = (name.split("(")[0], r["Counter_Name"])
+                d = acc.setdefault(key, {})
+                disp = r.get("Dispatch_Id", str(len(d)))
+                d[disp] = d.get(disp, 0.0) + float(r["Counter_Value"])
+    out = []
+    for (name, ctr), d in sorted(acc.items()):
+        v = list(d.values())
+        out.append({"kernel": name, "counter": ctr, "dispatches": len(v),
+                    "mean_per_dispatch": sum(v) / len(v), "min": min(v), "max": max(v)})
+    print(json.dumps(out, indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="train", choices=["train", "serve", "batched"])
+    ap.add_argument("--workload", default="train", choices=["train", "serve", "batched", "copy"])
+    ap.add_argument("--pmc-summary", metavar="DIR", default=None,
+                    help="print per-dispatch averages of the rocprofv3 counter CSVs under DIR")
+    ap.add_argument("--kernel", default="k_hist", help="kernel name regex for --pmc-summary")
     ap.add_argument("--rows", type=int, default=1_000_000)
     ap.add_argument("--features", type=int, default=100)
     ap.add_argument("--hist-chunk", type=int, default=0)
@@ -361,11 +438,15 @@ def main():
     ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--max-job-rows", type=int, default=2_000_000)
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary, args.kernel)
+        return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU "
-                                   "sharding is described in DESIGN.md, not implemented)"}))
+        print(json.dumps({"error": "this library runs on one GPU; --gpus must be 1 (sharding "
+                                   "across GPUs is designed but not implemented)"}))
         sys.exit(2)
-    {"train": run_train, "serve": run_serve, "batched": run_batched}[args.workload](args)
+    {"train": run_train, "serve": run_serve, "batched": run_batched,
+     "copy": run_copy}[args.workload](args)
 
 
 if __name__ == "__main__":
