@@ -18,6 +18,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -204,6 +205,74 @@ struct Segment {
   uint32_t row0, count, slot_row;
 };
 
+// Host copies of one batch split over a few threads (the caller works too):
+// one thread's memcpy bandwidth, not PCIe, would otherwise bound the server.
+class CopyPool {
+ public:
+  struct Piece {
+    void* dst;
+    const void* src;
+    size_t bytes;
+  };
+  explicit CopyPool(int helpers) {
+    for (int i = 0; i < helpers; ++i) th_.emplace_back([this] { run(); });
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  // Copies every piece; returns when all are done.
+  void copy(const std::vector<Piece>& pieces) {
+    if (pieces.empty()) return;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      jobs_ = &pieces;
+      next_ = 0;
+      left_ = pieces.size();
+    }
+    cv_.notify_all();
+    work();
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [this] { return left_ == 0; });
+    jobs_ = nullptr;
+  }
+
+ private:
+  void work() {
+    while (true) {
+      const Piece* pc;
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (!jobs_ || next_ >= jobs_->size()) return;
+        pc = &(*jobs_)[next_++];
+      }
+      std::memcpy(pc->dst, pc->src, pc->bytes);
+      std::lock_guard<std::mutex> lk(mu_);
+      if (--left_ == 0) done_.notify_all();
+    }
+  }
+  void run() {
+    std::unique_lock<std::mutex> lk(mu_);
+    while (true) {
+      cv_.wait(lk, [this] { return stop_ || (jobs_ && next_ < jobs_->size()); });
+      if (stop_) return;
+      lk.unlock();
+      work();
+      lk.lock();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  const std::vector<Piece>* jobs_ = nullptr;
+  size_t next_ = 0, left_ = 0;
+  bool stop_ = false;
+};
+
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
@@ -231,6 +300,7 @@ struct tc_server {
   uint64_t n_batches = 0, n_rows = 0;
   bool stop = false;
   std::string error;
+  std::unique_ptr<tc::CopyPool> copier;
   std::thread worker;
 };
 
@@ -269,21 +339,34 @@ static void serve_loop(tc_server* S) {
       progressed = true;
       S->cv_done.notify_all();
     }
-    // fill free slots from the admitted queue
+    // fill free slots from the admitted queue; the rows are staged with the
+    // queue unlocked (submitters keep going), in parallel pieces
     for (auto& sl : S->slots) {
       if (sl.busy || S->admitted.empty()) continue;
       uint32_t rows = 0;
       while (!S->admitted.empty() && rows < S->p.max_batch_rows) {
         auto& rq = S->admitted.front();
         const uint32_t take = std::min(rq->n - rq->next, S->p.max_batch_rows - rows);
-        std::memcpy(sl.h_x + static_cast<size_t>(rows) * F, rq->X + static_cast<size_t>(rq->next) * F,
-                    4ull * take * F);
         sl.segs.push_back(Segment{rq, rq->next, take, rows});
         rq->next += take;
         rows += take;
         if (rq->next == rq->n) S->admitted.pop_front();
       }
       if (rows == 0) continue;
+      lk.unlock();
+      {
+        constexpr size_t kPiece = 1u << 20;
+        std::vector<CopyPool::Piece> pieces;
+        for (const auto& sg : sl.segs) {
+          const char* src = reinterpret_cast<const char*>(sg.req->X + static_cast<size_t>(sg.row0) * F);
+          char* dst = reinterpret_cast<char*>(sl.h_x + static_cast<size_t>(sg.slot_row) * F);
+          const size_t bytes = 4ull * sg.count * F;
+          for (size_t o = 0; o < bytes; o += kPiece)
+            pieces.push_back(CopyPool::Piece{dst + o, src + o, std::min(kPiece, bytes - o)});
+        }
+        S->copier->copy(pieces);
+      }
+      lk.lock();
       try {
         TC_HIP(hipMemcpyAsync(sl.d_x, sl.h_x, 4ull * rows * F, hipMemcpyHostToDevice, sl.stream));
         launch_predict(S->ens, sl.d_x, false, rows, F, 0, S->p.transform, sl.d_out, nullptr, sl.stream);
@@ -463,6 +546,7 @@ int32_t tc_serve_start(tc_ensemble* e, const tc_serve_params* p, tc_server** out
       delete S;
       throw;
     }
+    S->copier = std::make_unique<CopyPool>(7);
     S->worker = std::thread(serve_loop, S);
     *out = S;
   });
