@@ -97,6 +97,17 @@ def regression_workload(n, F, seed):
 
 
 # ------------------------------------------------------------------ CPU baseline
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline_train(X, y, loss, depth, lr, lam, budget_s):
     """Time the reference's own boosting loop (compiled from its sources) on
     the same matrix for a bounded number of trees."""
@@ -119,6 +130,7 @@ def cpu_baseline_train(X, y, loss, depth, lr, lam, budget_s):
     r = RefLib.isolated(body)
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return dict(value=r["trees"] / r["loop_s"], unit="trees/s", cores=cores, kind="reference",
+                host_cpu=host_cpu_model(),
                 sample=(f"reference hot path (oracle/_ref, -O3 -mavx2 -mfma, OpenMP) on the full "
                         f"matrix: compute_bins {r['quantize_s']:.2f} s, then {r['trees']} trees "
                         f"in {r['loop_s']:.2f} s"),

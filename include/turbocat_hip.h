@@ -45,6 +45,9 @@ extern "C" {
 
 #define TC_LOSS_MSE 0
 #define TC_LOSS_LOGLOSS 1
+/* K-class softmax (CrossEntropyLoss, loss.cpp:269-330) with K-channel
+ * symmetric trees: labels are class indices 0..K-1. */
+#define TC_LOSS_SOFTMAX 2
 
 typedef struct tc_ctx tc_ctx;
 typedef struct tc_dataset tc_dataset;
@@ -79,11 +82,14 @@ typedef struct {
   float subsample;           /* in (0, 1): row sample per tree; 0 or >= 1: every row */
   float colsample_bytree;    /* in (0, 1): consumes an RNG draw per tree */
   uint64_t seed;             /* Config::seed (config.hpp:235) */
+  uint32_t n_classes;        /* TC_LOSS_SOFTMAX: K in [2, 16], (2^max_depth) * K <= 8192;
+                                otherwise 0 or 1 */
 } tc_train_params;
 
 /* Host-side model arrays filled by training (caller-allocated).
  * feature/bin/fthr/gain: n_trees * max_depth; depth: n_trees;
- * leaves: n_trees << max_depth; train_preds: n rows (optional). */
+ * leaves: (n_trees << max_depth) * K, [tree][leaf][class];
+ * train_preds (optional): K x n, class-major.  K = 1 except TC_LOSS_SOFTMAX. */
 typedef struct {
   uint16_t* feature;
   uint8_t* bin;
@@ -125,6 +131,10 @@ int32_t tc_dataset_destroy(tc_dataset* ds);
 /* A6/A7: Loss::compute_gradients (loss.hpp:57-63; loss.cpp:123-211, :348-362). */
 int32_t tc_gradients(tc_ctx* ctx, int32_t loss, const float* y, const float* preds, float* g,
                      float* h, uint32_t n);
+/* A8: CrossEntropyLoss::compute_multiclass_gradients (loss.cpp:287-318):
+ * softmax of each row's K raw scores; preds, g, h row-major n x K. */
+int32_t tc_gradients_multi(tc_ctx* ctx, const float* y, const float* preds, float* g, float* h,
+                           uint32_t n, uint32_t K);
 /* Row sample of one tree: Dataset::random_subsample(ratio, seed)
  * (dataset.cpp:552-562).  out needs n slots; *n_out = uint32(n * ratio). */
 int32_t tc_random_subsample(uint32_t n, float ratio, uint64_t seed, uint32_t* out,
@@ -158,6 +168,15 @@ int32_t tc_split_scan(tc_ctx* ctx, const float* hist, const float* node_g, const
 int32_t tc_tree_build(tc_dataset* ds, const float* g, const float* h, const tc_tree_params* p,
                       uint16_t* feature, uint8_t* bin, float* fthr, float* gain, float* leaves,
                       uint32_t* leaf_idx, int32_t* depth);
+
+/* A18: K-channel symmetric tree on given gradients (g, h class-major K x n):
+ * one shared (feature, bin) per level whose gain is the sum over nodes and
+ * classes of the per-class gains; leaves 2^depth x K.  K = 1 is exactly
+ * tc_tree_build.  No reference counterpart (the reference's multiclass
+ * grows regular per-class trees, booster.cpp:1367-1395). */
+int32_t tc_tree_build_multi(tc_dataset* ds, const float* g, const float* h, uint32_t K,
+                            const tc_tree_params* p, uint16_t* feature, uint8_t* bin, float* fthr,
+                            float* gain, float* leaves, uint32_t* leaf_idx, int32_t* depth);
 
 /* ------------------------------------------------------------- boosting */
 /* A5-A15: Booster::train, symmetric branch without sampling
@@ -202,6 +221,15 @@ int32_t tc_ensemble_create(tc_ctx* ctx, uint32_t n_trees, uint32_t max_depth,
                            const uint16_t* feature, const float* fthr, const uint8_t* bin,
                            const uint16_t* depth, const float* leaves, const float* weights,
                            float base, uint32_t n_features, tc_ensemble** out);
+/* A18 inference: K-channel ensemble, leaves [tree][leaf][class].  tc_predict,
+ * tc_predict_dev and tc_predict_binned then write n x K (row-major) scores;
+ * transform 1 applies softmax (CrossEntropyLoss::transform_to_proba,
+ * loss.cpp:320-330).  No reference counterpart. */
+int32_t tc_ensemble_create_multi(tc_ctx* ctx, uint32_t n_trees, uint32_t max_depth,
+                                 uint32_t n_classes, const uint16_t* feature, const float* fthr,
+                                 const uint8_t* bin, const uint16_t* depth, const float* leaves,
+                                 const float* weights, float base, uint32_t n_features,
+                                 tc_ensemble** out);
 int32_t tc_ensemble_destroy(tc_ensemble* ens);
 /* A16: predict_batch_raw_fast + base (booster.cpp:899-925).  transform: 0 raw,
  * 1 sigmoid (LogLoss::transform_prediction, loss.cpp:213-215).  leaf_idx

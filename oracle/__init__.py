@@ -70,6 +70,9 @@ class Oracle:
         for fn in ("tco_gradients_f32", "tco_gradients_f64"):
             getattr(L, fn).argtypes = [c_int, V, V, V, V, c_uint32]
         L.tco_gradients_mc_f64.argtypes = [V, V, V, V, c_uint32, c_uint32]
+        L.tco_gradients_mc_f32.argtypes = [V, V, V, V, c_uint32, c_uint32]
+        L.tco_predict_mc_f64.argtypes = [V, V, V, V, V, c_uint32, c_uint32, c_uint32, c_double, V,
+                                         c_uint32, c_uint32, V, V]
         L.tco_hist_f32.argtypes = [V, c_uint32, c_uint32, V, V, V, c_uint32, V, V]
         L.tco_hist_f64.argtypes = [V, c_uint32, c_uint32, V, V, V, c_uint32, V, V, V]
         L.tco_split_f32.argtypes = [V, V, V, V, c_uint32, c_uint32, c_uint32, c_float, c_int, V, V, V]
@@ -125,6 +128,14 @@ class Oracle:
         y = np.ascontiguousarray(y, np.float32); p = np.ascontiguousarray(p, np.float32)
         g = np.zeros(len(y)); h = np.zeros(len(y))
         self.L.tco_gradients_f64(loss, _p(y), _p(p), _p(g), _p(h), len(y))
+        return g, h
+
+    def gradients_mc_f32(self, y, p):
+        """A8 in the reference's float order; p, g, h row-major n x K."""
+        y = np.ascontiguousarray(y, np.float32); p = np.ascontiguousarray(p, np.float32)
+        n, K = p.shape
+        g = np.zeros((n, K), np.float32); h = np.zeros((n, K), np.float32)
+        self.L.tco_gradients_mc_f32(_p(y), _p(p), _p(g), _p(h), n, K)
         return g, h
 
     def gradients_mc_f64(self, y, p):
@@ -246,6 +257,18 @@ class Oracle:
                                model["base"], _p(X), n, F, _p(out), _p(ab))
         return out, ab
 
+    def predict_mc_f64(self, model, X):
+        """K-channel ensemble: (n, K) raw scores and sum |terms|."""
+        X = np.ascontiguousarray(X, np.float32)
+        n, F = X.shape
+        K = model["K"]
+        out = np.zeros((n, K)); ab = np.zeros((n, K))
+        self.L.tco_predict_mc_f64(_p(model["feature"]), _p(model["fthr"]), _p(model["depth"]),
+                                  _p(np.ascontiguousarray(model["leaves"], np.float32)),
+                                  _p(model["weight"]), model["T"], model["D"], K, model["base"],
+                                  _p(X), n, F, _p(out), _p(ab))
+        return out, ab
+
     def fma_update(self, preds, lr, vals):
         out = np.array(preds, np.float32, copy=True)
         v = np.ascontiguousarray(vals, np.float32)
@@ -286,6 +309,7 @@ class RefLib:
         L.tcref_dataset_free.argtypes = [V]
         L.tcref_dataset_bins.argtypes = [V, V, V, V]
         L.tcref_gradients.argtypes = [c_int, V, V, V, V, c_uint32]
+        L.tcref_gradients_mc.argtypes = [V, V, V, V, c_uint32, c_uint32]
         L.tcref_init_prediction.argtypes = [c_int, V, c_uint32]
         L.tcref_init_prediction.restype = c_float
         L.tcref_hist.argtypes = [V, V, V, V, c_uint32, V, V]
@@ -343,6 +367,14 @@ class RefLib:
     def init_prediction(self, loss, y):
         y = np.ascontiguousarray(y, np.float32)
         return float(self.L.tcref_init_prediction(loss, _p(y), len(y)))
+
+    def gradients_mc(self, y, p):
+        """CrossEntropyLoss::compute_multiclass_gradients; p row-major n x K."""
+        y = np.ascontiguousarray(y, np.float32); p = np.ascontiguousarray(p, np.float32)
+        n, K = p.shape
+        g = np.zeros((n, K), np.float32); h = np.zeros((n, K), np.float32)
+        self.L.tcref_gradients_mc(_p(y), _p(p), _p(g), _p(h), n, K)
+        return g, h
 
     def predict(self, model, X):
         X = np.ascontiguousarray(X, np.float32)

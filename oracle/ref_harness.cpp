@@ -102,6 +102,14 @@ void tcref_gradients(int loss_id, const float* y, const float* p, float* g, floa
     loss->compute_gradients(y, p, g, h, n);
 }
 
+// CrossEntropyLoss::compute_multiclass_gradients (loss.cpp:287-318):
+// p, g, h row-major n x K.
+void tcref_gradients_mc(const float* y, const float* p, float* g, float* h, uint32_t n,
+                        uint32_t K) {
+    CrossEntropyLoss loss(K);
+    loss.compute_multiclass_gradients(y, p, g, h, n);
+}
+
 float tcref_init_prediction(int loss_id, const float* y, uint32_t n) {
     LossConfig lc;
     lc.loss_type = loss_id == 0 ? LossType::MSE : LossType::LogLoss;

@@ -154,6 +154,30 @@ void tco_gradients_f64(int loss, const float* y, const float* p, double* g, doub
     }
 }
 
+void tco_gradients_mc_f32(const float* y, const float* p, float* g, float* h, uint32_t n, uint32_t K) {
+    /* compute_multiclass_gradients + softmax in the reference's float order,
+     * loss.cpp:273-318: max shift, exp, sequential sum, divide; p and g/h
+     * row-major n x K */
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = p + (size_t)i * K;
+        float m = r[0];
+        for (uint32_t c = 1; c < K; ++c) m = r[c] > m ? r[c] : m;
+        float e[64];
+        float s = 0.0f;
+        for (uint32_t c = 0; c < K; ++c) {
+            e[c] = expf(r[c] - m);
+            s += e[c];
+        }
+        const uint32_t lab = (uint32_t)y[i];
+        for (uint32_t c = 0; c < K; ++c) {
+            const float pc = e[c] / s;
+            g[(size_t)i * K + c] = pc - (c == lab ? 1.0f : 0.0f);
+            const float hh = pc * (1.0f - pc);
+            h[(size_t)i * K + c] = hh > 1e-6f ? hh : 1e-6f;
+        }
+    }
+}
+
 void tco_gradients_mc_f64(const float* y, const float* p, double* g, double* h, uint32_t n, uint32_t K) {
     /* compute_multiclass_gradients + softmax, loss.cpp:273-318 */
     for (uint32_t i = 0; i < n; ++i) {
@@ -385,6 +409,30 @@ void tco_predict_f64(const uint16_t* feature, const float* fthr, const uint16_t*
         }
         out[i] = s + base;
         if (abs_sum) abs_sum[i] = a;
+    }
+}
+
+/* K-channel ensemble (A18): out[i][c] = base + sum_t w_t * leaf_t[idx_t][c],
+ * leaves [T][2^D][K], FP64; abs_sum[i][c] = |base| + sum_t |w_t * leaf|. */
+void tco_predict_mc_f64(const uint16_t* feature, const float* fthr, const uint16_t* depth,
+                        const float* leaves, const float* weight, uint32_t T, uint32_t D,
+                        uint32_t K, double base, const float* X, uint32_t n, uint32_t F,
+                        double* out, double* abs_sum) {
+    const uint32_t LP = 1u << D;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* row = X + (size_t)i * F;
+        for (uint32_t c = 0; c < K; ++c) {
+            out[(size_t)i * K + c] = base;
+            if (abs_sum) abs_sum[(size_t)i * K + c] = fabs(base);
+        }
+        for (uint32_t t = 0; t < T; ++t) {
+            const uint32_t l = float_leaf(row, feature + (size_t)t * D, fthr + (size_t)t * D, depth[t]);
+            for (uint32_t c = 0; c < K; ++c) {
+                const double v = (double)weight[t] * (double)leaves[((size_t)t * LP + l) * K + c];
+                out[(size_t)i * K + c] += v;
+                if (abs_sum) abs_sum[(size_t)i * K + c] += fabs(v);
+            }
+        }
     }
 }
 

@@ -44,7 +44,12 @@ void tco_gradients_f32(int loss, const float* y, const float* p, float* g, float
 /* A6/A7 in double precision with an exact exp (yardstick). */
 void tco_gradients_f64(int loss, const float* y, const float* p, double* g, double* h, uint32_t n);
 /* A8: softmax gradients, K channels, row-major n x K (src/boosting/loss.cpp:273-318). */
+void tco_gradients_mc_f32(const float* y, const float* p, float* g, float* h, uint32_t n, uint32_t K);
 void tco_gradients_mc_f64(const float* y, const float* p, double* g, double* h, uint32_t n, uint32_t K);
+void tco_predict_mc_f64(const uint16_t* feature, const float* fthr, const uint16_t* depth,
+                        const float* leaves, const float* weight, uint32_t T, uint32_t D,
+                        uint32_t K, double base, const float* X, uint32_t n, uint32_t F,
+                        double* out, double* abs_sum);
 
 /* A9: histogram of a row list (rows == NULL: all rows in order).
  * f32: sequential row-order sums (src/tree/histogram.cpp:93-133, :307-378).

@@ -42,6 +42,16 @@ def regression_data(n, F, seed, low_card_col=None):
     return X, y
 
 
+def multiclass_data(n, F, K, seed, sigma=0.6):
+    """Synthetic K-class set in the spirit of the 4M x 256 workload:
+    X ~ N(0,1), W K x F ~ N(0, sigma), label = argmax(W x + Gumbel(0, 1))."""
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((n, F)).astype(np.float32)
+    W = rng.normal(0.0, sigma, (K, F)).astype(np.float32)
+    z = X @ W.T + rng.gumbel(0.0, 1.0, (n, K))
+    return X, np.argmax(z, axis=1).astype(np.float32)
+
+
 def binary_data(n, F, seed):
     """Synthetic binary set: label ~ Bernoulli(sigmoid(w.x)), w with 10 nonzeros."""
     rng = np.random.default_rng(seed)

@@ -52,7 +52,21 @@ def make(name, kind, n, F, seed, loss, T, D, lr, lam):
         train_preds=tp, Xp=Xp, pred=pred)
 
 
+def make_ce(name, n, K, seed):
+    """CrossEntropyLoss::compute_multiclass_gradients on seeded raw scores
+    (uniform, saturated and random rows)."""
+    rng = np.random.default_rng(seed)
+    p = (rng.standard_normal((n, K)) * 4).astype(np.float32)
+    p[::13] = 0.0
+    p[1::17, 0] = 40.0
+    y = rng.integers(0, K, n).astype(np.float32)
+    g, h = RefLib().gradients_mc(y, p)
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), y=y, p=p, g=g, h=h)
+
+
 if __name__ == "__main__":
     for c in CASES:
         RefLib.isolated(make, *c)  # fresh thread: see RefLib.isolated
         print("wrote", c[0])
+    RefLib.isolated(make_ce, "ce_grad", 517, 7, 104)
+    print("wrote ce_grad")

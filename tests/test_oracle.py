@@ -21,7 +21,9 @@ from conftest import binary_data, regression_data
 
 from oracle import LOSS_LOGLOSS, LOSS_MSE, RefLib, tree_params
 
-GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+GOLDEN_DIR = os.path.join(os.path.dirname(__file__), "golden")
+GOLDEN = sorted(p for p in glob.glob(os.path.join(GOLDEN_DIR, "*.npz"))
+                if not os.path.basename(p).startswith("ce_"))
 HAVE_REF = RefLib.available()
 
 
@@ -218,6 +220,34 @@ def test_oracle_matches_golden(oracle, path):
     assert np.array_equal(oracle.predict_f32(m, z["Xp"]), z["pred"])
 
 
+def test_ce_gradients_match_golden(oracle):
+    """A8: the FP32 restatement equals CrossEntropyLoss::compute_multiclass_gradients
+    (loss.cpp:287-318) bit for bit on the reference's own outputs."""
+    z = np.load(os.path.join(GOLDEN_DIR, "ce_grad.npz"), allow_pickle=False)
+    g, h = oracle.gradients_mc_f32(z["y"], z["p"])
+    assert np.array_equal(g, z["g"]) and np.array_equal(h, z["h"])
+    g64, h64 = oracle.gradients_mc_f64(z["y"], z["p"])
+    assert np.abs(g64 - z["g"]).max() < 1e-6 and np.abs(h64 - z["h"]).max() < 1e-6
+
+
+def test_predict_mc_k1_reduction(oracle):
+    """The K-channel FP64 prediction with K = 1 is the single-channel one, and
+    with K > 1 each class is the single-channel prediction of its leaves."""
+    rng = np.random.default_rng(9)
+    T, D, K, F, n = 23, 5, 3, 6, 700
+    X = rng.standard_normal((n, F)).astype(np.float32)
+    m = dict(T=T, D=D, K=K, feature=rng.integers(0, F, T * D).astype(np.uint16),
+             fthr=rng.standard_normal(T * D).astype(np.float32),
+             depth=rng.integers(0, D + 1, T).astype(np.uint16),
+             leaves=rng.standard_normal((T << D) * K).astype(np.float32),
+             weight=np.full(T, 0.1, np.float32), base=0.25)
+    out, _ = oracle.predict_mc_f64(m, X)
+    for c in range(K):
+        mc = dict(m, K=1, leaves=np.ascontiguousarray(m["leaves"][c::K]))
+        ref, _ = oracle.predict_f64(mc, X)
+        assert np.allclose(out[:, c], ref, rtol=0, atol=1e-12)
+
+
 # ---------------------------------------------------------------- compiled reference
 
 @pytest.mark.skipif(not HAVE_REF, reason="oracle/_ref/libtcref.so not built (no reference tree)")
@@ -241,3 +271,14 @@ def test_oracle_matches_reference(oracle, n, F, seed, D):
             Xp = np.random.default_rng(seed).standard_normal((64 * 8, F)).astype(np.float32)
             assert np.array_equal(R.predict(rm, Xp), oracle.predict_f32(om, Xp))
     RefLib.isolated(body)
+
+
+@pytest.mark.skipif(not HAVE_REF, reason="oracle/_ref/libtcref.so not built (no reference tree)")
+@pytest.mark.parametrize("K", [2, 3, 10])
+def test_ce_gradients_match_reference(oracle, K):
+    rng = np.random.default_rng(K)
+    p = (rng.standard_normal((3001, K)) * 3).astype(np.float32)
+    y = rng.integers(0, K, 3001).astype(np.float32)
+    rg, rh = RefLib.isolated(lambda: RefLib().gradients_mc(y, p))
+    g, h = oracle.gradients_mc_f32(y, p)
+    assert np.array_equal(g, rg) and np.array_equal(h, rh)

@@ -4,15 +4,18 @@
 //   bins_rm   uint8  [N][Fp]     row-major, Fp = F rounded up to 16 (one
 //                                 16-byte load = one 16-feature tile of a row)
 //   bins_cm   uint8  [F][N]      column-major (split column reads)
-//   gh        float2 [N]         gradient / hessian per row
+//   gh        float2 [K][N]      gradient / hessian per channel and row (K = 1
+//                                 except K-class softmax; class-major)
+//   preds     float  [K][N]      raw training predictions, class-major
 //   rows      uint32 [D+1][N]    per level, row ids grouped by node (order
 //                                 within a node is irrelevant: every sum
 //                                 over a node is an exact integer sum)
 //   off       uint32 [D+1][2^D+1] node segment offsets into rows[level]
-//   hist      HistCell{int64 g, int64 h} [2^D-1][F][256]  fixed-point per
-//                                 node (heap order: level d node n at
-//                                 (2^d - 1 + n)); slot 255 = NaN bin
-//   ng/nh/nc  float/float/uint32 [2^(D+1)-1] node totals, heap order
+//   hist      HistCell{int64 g, int64 h} [2^D-1][K][F][256]  fixed-point
+//                                 per node and channel (heap order: level d
+//                                 node n at (2^d - 1 + n)); slot 255 = NaN bin
+//   ng/nh     float [2^(D+1)-1][K]  node totals per channel, heap order
+//   nc        uint32 [2^(D+1)-1]    node row counts
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -27,7 +30,9 @@ constexpr int kPartThreads = 1024;
 constexpr int kScanNodesPerPass = 32;  // node prefixes staged in LDS at a time
 constexpr uint32_t kEmptyKey = 0u;
 
-enum Loss : int32_t { kMSE = 0, kLogLoss = 1 };
+enum Loss : int32_t { kMSE = 0, kLogLoss = 1, kSoftmax = 2 };
+constexpr int kMaxClasses = 16;         // softmax channels
+constexpr int kMaxLeafValues = 8192;    // 2^depth * K (leaf values staged in LDS)
 
 // Per-tree state of one job, device resident.
 struct TreeState {
@@ -36,6 +41,8 @@ struct TreeState {
   int32_t tree;         // index of the tree being built
   uint32_t gmax_next;   // float bits of max |g| of the gradients being written
   uint32_t hmax_next;   // float bits of max |h| of the gradients being written
+  uint32_t fault;       // nonzero: a kernel found inconsistent state and skipped its writes
+  uint32_t pad;
   unsigned long long best[kMaxDepth];  // argmax key per level
   // Fixed-point scales of the current tree's histograms: a row's g enters as
   // round(g * gscale) (int64), h as round(h * hscale).  Powers of two chosen
@@ -50,9 +57,9 @@ struct HistCell {
 };
 
 // One unit of histogram work: rows [start, start+len) of rows[level] for
-// node `node` (level-local index), feature tile `tile`.
+// node `node` (level-local index), feature tile `tile`, channel `cls`.
 struct HistWork {
-  uint32_t job, node, start, len, tile, pad0, pad1, pad2;
+  uint32_t job, node, start, len, tile, cls, pad1, pad2;
 };
 
 // One unit of partition work: rows [start, start+len) of rows[level],
@@ -73,27 +80,28 @@ struct JobDev {
   uint32_t root_identity;   // 1: the root list is 0..N-1 (no sampling), rows[0] not read
   // hyper-parameters
   int32_t loss;
+  uint32_t K;               // gradient channels: classes for softmax, else 1
   uint32_t D, max_bins, min_samples_leaf;
   float lr, lambda_l2, lambda_l1, max_delta_step, min_child_weight, min_split_gain;
   float base;
   uint32_t hist_chunk;      // rows per histogram work item
   // training state
-  const float* y;
-  float* preds;
-  float2* gh;
+  const float* y;           // [N] (class index for softmax)
+  float* preds;             // [K][N]
+  float2* gh;               // [K][N]
   uint32_t* rows;           // [D+1][N]
   uint32_t* off;            // [D+1][2^D + 1]
-  float* ng;                // heap [2^(D+1) - 1]
+  float* ng;                // heap [2^(D+1) - 1][K]
   float* nh;
-  uint32_t* nc;
-  HistCell* hist;           // heap [2^D - 1][F][256]
+  uint32_t* nc;             // heap [2^(D+1) - 1]
+  HistCell* hist;           // heap [2^D - 1][K][F][256]
   TreeState* st;
   // partition cursors of the level being split: [node][2] = next free
   // position of the left child (counts up), end of the right child (counts
   // down); capacity 2^D
   uint32_t* cursor;
   // root-sum partials (sum g, sum h over 1024-row tiles of the root list)
-  double* grad_partials;    // [wg][2]
+  double* grad_partials;    // [wg][K][2]
   uint32_t grad_wg_off;     // first workgroup of this job in the row grid
   uint32_t grad_wg_cnt;
   // model, tree-major
@@ -103,7 +111,7 @@ struct JobDev {
   float* m_fthr;
   float* m_gain;
   uint16_t* m_depth;        // [n_trees]
-  float* m_leaves;          // [n_trees][2^D]
+  float* m_leaves;          // [n_trees][2^D][K]
   unsigned long long* hist_rows_total;  // rows fed to histogram kernels (all trees)
 };
 

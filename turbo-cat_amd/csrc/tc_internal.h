@@ -30,12 +30,13 @@ struct tc_dataset {
 struct tc_ensemble {
   tc_ctx* ctx = nullptr;
   uint32_t T = 0, D = 0, F = 0;
+  uint32_t K = 1;               // output channels (classes of a softmax ensemble)
   float base = 0.f;
   uint16_t* feat = nullptr;     // [T][D]
   float* thr = nullptr;         // [T][D]
   uint8_t* bin = nullptr;       // [T][D]
   uint16_t* depth = nullptr;    // [T]
-  float* leaves = nullptr;      // [T][2^D]
+  float* leaves = nullptr;      // [T][2^D][K]
   float* weight = nullptr;      // [T]
 };
 

@@ -35,7 +35,7 @@ struct EnsDev {
   const uint16_t* depth;
   const float* leaves;
   const float* weight;
-  uint32_t T, D;
+  uint32_t T, D, K;
   float base;
 };
 
@@ -134,19 +134,130 @@ __global__ void __launch_bounds__(256) k_predict(EnsDev E, const void* __restric
   }
 }
 
+// A18 inference: the layout of k_predict, with K outputs per row.  Tree
+// lane q of row r sums w_t * leaf_t[idx][c] over its trees t = q, q + Q, ...
+// (fused multiply-adds, tree order) into K partial sums; lane 0 then adds
+// base + the Q partials in lane order, and optionally applies softmax (max
+// shift, exp, class-order sum, divide: CrossEntropyLoss::softmax,
+// loss.cpp:273-285).  As in k_predict, a row's value depends only on the
+// row and the model.
+template <bool BINNED>
+__global__ void __launch_bounds__(256) k_predict_multi(EnsDev E, const void* __restrict__ Xv,
+                                                       uint32_t n, uint32_t F, uint32_t ldc,
+                                                       uint32_t R, int transform,
+                                                       float* __restrict__ out,
+                                                       uint32_t* __restrict__ leaf_idx) {
+  extern __shared__ unsigned char s_raw[];
+  const uint32_t Q = 256u / R, K = E.K;
+  const uint32_t S = BINNED ? (F | 3u) + 1u : (F | 1u);
+  float* s_acc = reinterpret_cast<float*>(s_raw);  // [Q][R][K] partial sums
+  unsigned char* s_x = s_raw + sizeof(float) * 256 * K;
+  const uint32_t tid = threadIdx.x, q = tid / R, r = tid % R;
+  const uint32_t row0 = blockIdx.x * R;
+  const uint32_t nr = min(R, n - row0);
+  if (BINNED) {
+    const uint8_t* B = static_cast<const uint8_t*>(Xv);  // column-major [F][ldc]
+    for (uint32_t i = tid; i < nr * F; i += 256) {
+      const uint32_t f = i / nr, rr = i % nr;
+      s_x[rr * S + f] = B[static_cast<size_t>(f) * ldc + row0 + rr];
+    }
+  } else {
+    const float* X = static_cast<const float*>(Xv);
+    float* sx = reinterpret_cast<float*>(s_x);
+    const float* src = X + static_cast<size_t>(row0) * F;
+    for (uint32_t i = tid; i < nr * F; i += 256) sx[(i / F) * S + (i % F)] = src[i];
+  }
+  __syncthreads();
+  const uint32_t LP = 1u << E.D;
+  float acc[kMaxClasses];
+#pragma unroll
+  for (int c = 0; c < kMaxClasses; ++c) acc[c] = 0.f;
+  if (r < nr) {
+    for (uint32_t t = q; t < E.T; t += Q) {
+      const uint32_t dep = E.depth[t];
+      uint32_t li = 0;
+      for (uint32_t d = 0; d < dep; ++d) {
+        const uint32_t f = E.feat[t * E.D + d];
+        uint32_t right;
+        if (BINNED) {
+          const uint8_t v = s_x[r * S + f];
+          right = (v != 255 && v > E.bin[t * E.D + d]) ? 1u : 0u;
+        } else {
+          const float v = reinterpret_cast<const float*>(s_x)[r * S + f];
+          right = (!isnan(v) && v >= E.thr[t * E.D + d]) ? 1u : 0u;
+        }
+        li = (li << 1) | right;
+      }
+      if (leaf_idx) leaf_idx[static_cast<size_t>(row0 + r) * E.T + t] = li;
+      const float w = E.weight[t];
+      const float* lv = E.leaves + (static_cast<size_t>(t) * LP + li) * K;
+#pragma unroll
+      for (uint32_t c = 0; c < kMaxClasses; ++c) {
+        if (c >= K) break;
+        acc[c] = __fmaf_rn(w, lv[c], acc[c]);
+      }
+    }
+  }
+#pragma unroll
+  for (uint32_t c = 0; c < kMaxClasses; ++c) {
+    if (c >= K) break;
+    s_acc[(q * R + r) * K + c] = acc[c];
+  }
+  __syncthreads();
+  if (q == 0 && r < nr) {
+    float v[kMaxClasses];
+#pragma unroll
+    for (uint32_t c = 0; c < kMaxClasses; ++c) v[c] = 0.f;
+    for (uint32_t qq = 0; qq < Q; ++qq) {  // lane partials in lane order
+#pragma unroll
+      for (uint32_t c = 0; c < kMaxClasses; ++c) {
+        if (c >= K) break;
+        v[c] = __fadd_rn(v[c], s_acc[(qq * R + r) * K + c]);
+      }
+    }
+    float m = -INFINITY;
+#pragma unroll
+    for (uint32_t c = 0; c < kMaxClasses; ++c) {
+      if (c >= K) break;
+      v[c] = __fadd_rn(E.base, v[c]);
+      m = fmaxf(m, v[c]);
+    }
+    if (transform == 1) {
+      float z = 0.f;
+#pragma unroll
+      for (uint32_t c = 0; c < kMaxClasses; ++c) {
+        if (c >= K) break;
+        v[c] = expf(__fsub_rn(v[c], m));
+        z = __fadd_rn(z, v[c]);
+      }
+#pragma unroll
+      for (uint32_t c = 0; c < kMaxClasses; ++c) {
+        if (c >= K) break;
+        v[c] = __fdiv_rn(v[c], z);
+      }
+    }
+    float* o = out + static_cast<size_t>(row0 + r) * K;
+#pragma unroll
+    for (uint32_t c = 0; c < kMaxClasses; ++c) {
+      if (c >= K) break;
+      o[c] = v[c];
+    }
+  }
+}
+
 EnsDev ens_dev(const tc_ensemble* e) {
   EnsDev E;
   E.feat = e->feat; E.thr = e->thr; E.bin = e->bin; E.depth = e->depth; E.leaves = e->leaves;
-  E.weight = e->weight; E.T = e->T; E.D = e->D; E.base = e->base;
+  E.weight = e->weight; E.T = e->T; E.D = e->D; E.K = e->K; E.base = e->base;
   return E;
 }
 
-// Rows per workgroup and LDS bytes for a feature count.
-void predict_shape(uint32_t F, bool binned, uint32_t* R, size_t* lds) {
+// Rows per workgroup and LDS bytes for a feature count and output channels.
+void predict_shape(uint32_t F, uint32_t K, bool binned, uint32_t* R, size_t* lds) {
   const uint32_t S = binned ? (F | 3u) + 1u : (F | 1u);
   const size_t elem = binned ? 1 : 4;
   for (uint32_t r : {64u, 32u, 16u, 8u, 4u, 2u, 1u}) {
-    const size_t bytes = sizeof(float) * 256 + static_cast<size_t>(r) * S * elem;
+    const size_t bytes = sizeof(float) * 256 * K + static_cast<size_t>(r) * S * elem;
     if (bytes <= 160 * 1024 || r == 1) {
       *R = r;
       *lds = bytes;
@@ -160,10 +271,17 @@ void launch_predict(const tc_ensemble* e, const void* X, bool binned, uint32_t n
   if (n == 0) return;
   uint32_t R;
   size_t lds;
-  predict_shape(F, binned, &R, &lds);
+  predict_shape(F, e->K, binned, &R, &lds);
   TC_REQUIRE(lds <= 160 * 1024, "too many features for one LDS row tile");
   const dim3 grid((n + R - 1) / R);
-  if (binned) {
+  if (e->K > 1) {
+    auto kern = binned ? &k_predict_multi<true> : &k_predict_multi<false>;
+    if (lds > 64 * 1024)
+      TC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, ens_dev(e), X, n, F, ldc, R, transform, out,
+                       leaf_idx);
+  } else if (binned) {
     if (lds > 64 * 1024)
       TC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_predict<true>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
@@ -406,13 +524,17 @@ using namespace tc;
 
 extern "C" {
 
-int32_t tc_ensemble_create(tc_ctx* ctx, uint32_t n_trees, uint32_t max_depth,
-                           const uint16_t* feature, const float* fthr, const uint8_t* bin,
-                           const uint16_t* depth, const float* leaves, const float* weights,
-                           float base, uint32_t n_features, tc_ensemble** out) {
+int32_t tc_ensemble_create_multi(tc_ctx* ctx, uint32_t n_trees, uint32_t max_depth,
+                                 uint32_t n_classes, const uint16_t* feature, const float* fthr,
+                                 const uint8_t* bin, const uint16_t* depth, const float* leaves,
+                                 const float* weights, float base, uint32_t n_features,
+                                 tc_ensemble** out) {
   return guard([&] {
     TC_REQUIRE(ctx && out && feature && fthr && depth && leaves && weights, "tc_ensemble_create: NULL argument");
     TC_REQUIRE(max_depth >= 1 && max_depth <= 16, "max_depth must be in [1, 16]");
+    TC_REQUIRE(n_classes >= 1 && n_classes <= static_cast<uint32_t>(kMaxClasses),
+               "n_classes must be in [1, 16]");
+    const uint32_t K = n_classes;
     *out = nullptr;
     for (uint32_t t = 0; t < n_trees; ++t) {
       TC_REQUIRE(depth[t] <= max_depth, "tree depth exceeds max_depth");
@@ -420,10 +542,10 @@ int32_t tc_ensemble_create(tc_ctx* ctx, uint32_t n_trees, uint32_t max_depth,
         TC_REQUIRE(feature[t * max_depth + d] < n_features, "split feature out of range");
     }
     auto* e = new tc_ensemble();
-    e->ctx = ctx; e->T = n_trees; e->D = max_depth; e->F = n_features; e->base = base;
+    e->ctx = ctx; e->T = n_trees; e->D = max_depth; e->F = n_features; e->K = K; e->base = base;
     try {
       const size_t TD = std::max<size_t>(1, static_cast<size_t>(n_trees) * max_depth);
-      const size_t TL = std::max<size_t>(1, static_cast<size_t>(n_trees) << max_depth);
+      const size_t TL = std::max<size_t>(1, (static_cast<size_t>(n_trees) << max_depth) * K);
       e->feat = dmalloc<uint16_t>(TD);
       e->thr = dmalloc<float>(TD);
       e->bin = dmalloc<uint8_t>(TD);
@@ -438,7 +560,7 @@ int32_t tc_ensemble_create(tc_ctx* ctx, uint32_t n_trees, uint32_t max_depth,
         TC_HIP(hipMemcpy(e->thr, fthr, td * 4, hipMemcpyHostToDevice));
         TC_HIP(hipMemcpy(e->bin, b.data(), td, hipMemcpyHostToDevice));
         TC_HIP(hipMemcpy(e->depth, depth, n_trees * 2ull, hipMemcpyHostToDevice));
-        TC_HIP(hipMemcpy(e->leaves, leaves, (static_cast<size_t>(n_trees) << max_depth) * 4,
+        TC_HIP(hipMemcpy(e->leaves, leaves, (static_cast<size_t>(n_trees) << max_depth) * K * 4,
                          hipMemcpyHostToDevice));
         TC_HIP(hipMemcpy(e->weight, weights, n_trees * 4ull, hipMemcpyHostToDevice));
       }
@@ -448,6 +570,14 @@ int32_t tc_ensemble_create(tc_ctx* ctx, uint32_t n_trees, uint32_t max_depth,
     }
     *out = e;
   });
+}
+
+int32_t tc_ensemble_create(tc_ctx* ctx, uint32_t n_trees, uint32_t max_depth,
+                           const uint16_t* feature, const float* fthr, const uint8_t* bin,
+                           const uint16_t* depth, const float* leaves, const float* weights,
+                           float base, uint32_t n_features, tc_ensemble** out) {
+  return tc_ensemble_create_multi(ctx, n_trees, max_depth, 1, feature, fthr, bin, depth, leaves,
+                                  weights, base, n_features, out);
 }
 
 int32_t tc_ensemble_destroy(tc_ensemble* e) {
@@ -474,13 +604,14 @@ int32_t tc_predict(tc_ensemble* e, const float* X, uint32_t n, uint32_t F, int32
     TC_REQUIRE(F == e->F, "feature count differs from the model's");
     if (n == 0) return;
     hipStream_t s = e->ctx->stream;
-    float* dx = dmalloc<float>(static_cast<size_t>(n) * F + n);
+    float* dx = dmalloc<float>(static_cast<size_t>(n) * F + static_cast<size_t>(n) * e->K);
     uint32_t* dl = nullptr;
     try {
       if (leaf_idx) dl = dmalloc<uint32_t>(static_cast<size_t>(n) * std::max(1u, e->T));
       TC_HIP(hipMemcpyAsync(dx, X, 4ull * n * F, hipMemcpyHostToDevice, s));
       launch_predict(e, dx, false, n, F, 0, transform, dx + static_cast<size_t>(n) * F, dl, s);
-      TC_HIP(hipMemcpyAsync(out, dx + static_cast<size_t>(n) * F, 4ull * n, hipMemcpyDeviceToHost, s));
+      TC_HIP(hipMemcpyAsync(out, dx + static_cast<size_t>(n) * F, 4ull * n * e->K,
+                            hipMemcpyDeviceToHost, s));
       if (leaf_idx && e->T)
         TC_HIP(hipMemcpyAsync(leaf_idx, dl, 4ull * n * e->T, hipMemcpyDeviceToHost, s));
       TC_HIP(hipStreamSynchronize(s));
@@ -500,10 +631,10 @@ int32_t tc_predict_binned(tc_ensemble* e, tc_dataset* ds, float* out) {
     TC_REQUIRE(ds->F == e->F, "feature count differs from the model's");
     if (ds->N == 0) return;
     hipStream_t s = e->ctx->stream;
-    float* d = dmalloc<float>(ds->N);
+    float* d = dmalloc<float>(static_cast<size_t>(ds->N) * e->K);
     try {
       launch_predict(e, ds->bins_cm, true, ds->N, ds->F, ds->N, 0, d, nullptr, s);
-      TC_HIP(hipMemcpyAsync(out, d, 4ull * ds->N, hipMemcpyDeviceToHost, s));
+      TC_HIP(hipMemcpyAsync(out, d, 4ull * ds->N * e->K, hipMemcpyDeviceToHost, s));
       TC_HIP(hipStreamSynchronize(s));
     } catch (...) {
       dfree(d);
@@ -516,6 +647,7 @@ int32_t tc_predict_binned(tc_ensemble* e, tc_dataset* ds, float* out) {
 int32_t tc_serve_start(tc_ensemble* e, const tc_serve_params* p, tc_server** out) {
   return guard([&] {
     TC_REQUIRE(e && p && out, "tc_serve_start: NULL argument");
+    TC_REQUIRE(e->K == 1, "tc_serve_start: serving takes single-output ensembles");
     TC_REQUIRE(p->max_batch_rows > 0 && p->n_streams > 0 && p->n_streams <= 8,
                "tc_serve_start: bad parameters");
     *out = nullptr;

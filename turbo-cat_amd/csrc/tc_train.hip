@@ -5,6 +5,7 @@
 // tc_gradients).
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -35,6 +36,8 @@ __global__ void k_init_state(TreeState* st) {
   st->tree = -1;
   st->gmax_next = 0u;
   st->hmax_next = 0u;
+  st->fault = 0u;
+  st->pad = 0u;
   for (int d = 0; d < kMaxDepth; ++d) st->best[d] = kEmptyKey;
   st->gscale = st->hscale = st->ginv = st->hinv = 1.0;
 }
@@ -42,6 +45,39 @@ __global__ void k_init_state(TreeState* st) {
 __global__ void k_sub_elementwise(const float* a, const float* b, float* c, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) c[i] = __fsub_rn(a[i], b[i]);
+}
+
+// A8 over row-major n x K scores (CrossEntropyLoss::compute_multiclass_gradients,
+// loss.cpp:287-318), the same arithmetic as the trainer's softmax rows.
+__global__ void k_gradients_multi(const float* y, const float* p, float* g, float* h, uint32_t n,
+                                  uint32_t K) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* r = p + static_cast<size_t>(i) * K;
+  float e[kMaxClasses];
+  float m = -INFINITY;
+#pragma unroll
+  for (uint32_t c = 0; c < kMaxClasses; ++c) {
+    if (c >= K) break;
+    e[c] = r[c];
+    m = fmaxf(m, e[c]);
+  }
+  float sum = 0.f;
+#pragma unroll
+  for (uint32_t c = 0; c < kMaxClasses; ++c) {
+    if (c >= K) break;
+    e[c] = expf(__fsub_rn(e[c], m));
+    sum = __fadd_rn(sum, e[c]);
+  }
+  const uint32_t lab = static_cast<uint32_t>(y[i]);
+#pragma unroll
+  for (uint32_t c = 0; c < kMaxClasses; ++c) {
+    if (c >= K) break;
+    const float pc = __fdiv_rn(e[c], sum);
+    g[static_cast<size_t>(i) * K + c] = __fsub_rn(pc, c == lab ? 1.0f : 0.0f);
+    const float hh = __fmul_rn(pc, __fsub_rn(1.0f, pc));
+    h[static_cast<size_t>(i) * K + c] = hh > 1e-6f ? hh : 1e-6f;
+  }
 }
 
 __global__ void k_gradients(int loss, const float* y, const float* p, float* g, float* h, uint32_t n) {
@@ -81,6 +117,11 @@ struct JobSpec {
   float base;
 };
 
+// Gradient channels of a job: K classes for softmax, else 1.
+uint32_t channels(const tc_train_params& p) {
+  return p.loss == TC_LOSS_SOFTMAX ? p.n_classes : 1u;
+}
+
 class Trainer {
  public:
   Trainer(tc_ctx* ctx, const std::vector<JobSpec>& specs) : ctx_(ctx), specs_(specs) {
@@ -92,9 +133,11 @@ class Trainer {
     for (const auto& s : specs) {
       TC_REQUIRE(s.p.tree.max_depth == D_, "all jobs of a batch share max_depth");
       TC_REQUIRE(s.p.n_trees == n_trees_, "all jobs of a batch share n_trees");
-      TC_REQUIRE(s.p.loss == TC_LOSS_MSE || s.p.loss == TC_LOSS_LOGLOSS, "unsupported loss");
+      TC_REQUIRE(s.p.loss == TC_LOSS_MSE || s.p.loss == TC_LOSS_LOGLOSS ||
+                     s.p.loss == TC_LOSS_SOFTMAX, "unsupported loss");
       TC_REQUIRE((s.p.tree.max_bins & 0xFFu) >= 2, "max_bins must be in [2, 255]");
       fmax_ = std::max(fmax_, s.ds->F);
+      kmax_ = std::max(kmax_, channels(s.p));
     }
     try {
       setup();
@@ -131,13 +174,14 @@ class Trainer {
   }
 
   void init_gradients() {
-    hipLaunchKernelGGL(k_grad_update, dim3(total_grad_wg_), dim3(256), 0, ctx_->stream, d_jobs_,
-                       d_job_wg_off_, njobs_, 0, lists_);
+    hipLaunchKernelGGL(k_grad_update, dim3(total_grad_wg_), dim3(256), leaf_lds(), ctx_->stream,
+                       d_jobs_, d_job_wg_off_, njobs_, 0, lists_);
     TC_HIP(hipGetLastError());
   }
 
+  // Teacher forcing: job 0's gradients, class-major K x N.
   void set_gradients(const float* g, const float* h) {
-    const uint32_t N = h_jobs_[0].N;
+    const uint32_t N = h_jobs_[0].N * h_jobs_[0].K;
     float* dg = dmalloc<float>(N);
     float* dh = dmalloc<float>(N);
     TC_HIP(hipMemcpyAsync(dg, g, sizeof(float) * N, hipMemcpyHostToDevice, ctx_->stream));
@@ -179,15 +223,17 @@ class Trainer {
       }
       hipLaunchKernelGGL(k_split_scan<true>, dim3(fmax_, njobs_), dim3(1024), 0, s, d_jobs_,
                          static_cast<const float2*>(nullptr), d);
-      hipLaunchKernelGGL(k_finalize, dim3(ceil_div(1u << d, 4), njobs_), dim3(256), 0, s, d_jobs_, d);
+      // one wave per (node, channel) unit of the widest job
+      hipLaunchKernelGGL(k_finalize, dim3(ceil_div(static_cast<uint64_t>(kmax_) << d, 4), njobs_),
+                         dim3(256), 0, s, d_jobs_, d);
       if (d + 1 < D_ || keep_rows_) {  // training never reads the last level's rows
         hipLaunchKernelGGL(k_partition, dim3(part_cap_[d]), dim3(kPartThreads), 0, s, d_jobs_, lists_.part[d],
                            lists_.part_count[d], d);
         hipLaunchKernelGGL(k_part_plan, dim3(1, njobs_), dim3(256), 0, s, d_jobs_, lists_, d);
       }
     }
-    hipLaunchKernelGGL(k_grad_update, dim3(total_grad_wg_), dim3(256), 0, s, d_jobs_, d_job_wg_off_,
-                       njobs_, 1, lists_);
+    hipLaunchKernelGGL(k_grad_update, dim3(total_grad_wg_), dim3(256), leaf_lds(), s, d_jobs_,
+                       d_job_wg_off_, njobs_, 1, lists_);
     TC_HIP(hipGetLastError());
   }
 
@@ -280,7 +326,19 @@ class Trainer {
     TC_HIP(hipGetLastError());
   }
 
+  // Device-side consistency flags (TreeState::fault) of every job.
+  void check_faults() {
+    for (uint32_t j = 0; j < njobs_; ++j) {
+      uint32_t f = 0;
+      TC_HIP(hipMemcpy(&f, reinterpret_cast<const char*>(h_jobs_[j].st) + offsetof(TreeState, fault),
+                       4, hipMemcpyDeviceToHost));
+      if (f) throw Error(TC_EDEVICE, "internal consistency check failed in the partition of job " +
+                                         std::to_string(j));
+    }
+  }
+
   void download(uint32_t j, tc_model* m) {
+    check_faults();
     const JobDev& J = h_jobs_[j];
     const size_t TD = static_cast<size_t>(n_trees_) * D_;
     hipStream_t s = ctx_->stream;
@@ -290,10 +348,11 @@ class Trainer {
     if (m->gain) TC_HIP(hipMemcpyAsync(m->gain, J.m_gain, TD * 4, hipMemcpyDeviceToHost, s));
     if (m->depth) TC_HIP(hipMemcpyAsync(m->depth, J.m_depth, n_trees_ * 2, hipMemcpyDeviceToHost, s));
     if (m->leaves)
-      TC_HIP(hipMemcpyAsync(m->leaves, J.m_leaves, (static_cast<size_t>(n_trees_) << D_) * 4,
+      TC_HIP(hipMemcpyAsync(m->leaves, J.m_leaves, (static_cast<size_t>(n_trees_) << D_) * J.K * 4,
                             hipMemcpyDeviceToHost, s));
     if (m->train_preds)
-      TC_HIP(hipMemcpyAsync(m->train_preds, J.preds, sizeof(float) * J.N, hipMemcpyDeviceToHost, s));
+      TC_HIP(hipMemcpyAsync(m->train_preds, J.preds, sizeof(float) * J.N * J.K,
+                            hipMemcpyDeviceToHost, s));
     TC_HIP(hipStreamSynchronize(s));
     m->base = J.base;
   }
@@ -365,7 +424,8 @@ class Trainer {
           TC_HIP(hipEventCreateWithFlags(&SM.ev[b], hipEventDisableTiming));
         }
       }
-      J.loss = S.p.loss; J.D = D_; J.max_bins = S.p.tree.max_bins;
+      const uint32_t K = channels(S.p);
+      J.loss = S.p.loss; J.K = K; J.D = D_; J.max_bins = S.p.tree.max_bins;
       J.min_samples_leaf = S.p.tree.min_samples_leaf;
       J.lr = S.p.learning_rate; J.lambda_l2 = S.p.tree.lambda_l2; J.lambda_l1 = S.p.tree.lambda_l1;
       J.max_delta_step = S.p.tree.max_delta_step; J.min_child_weight = S.p.tree.min_child_weight;
@@ -387,23 +447,23 @@ class Trainer {
       if (S.y_host) TC_HIP(hipMemcpyAsync(y, S.y_host, 4ull * N, hipMemcpyHostToDevice, s));
       else TC_HIP(hipMemsetAsync(y, 0, 4ull * N, s));
       J.y = y;
-      J.preds = alloc<float>(N);
-      J.gh = alloc<float2>(N);
+      J.preds = alloc<float>(static_cast<size_t>(N) * K);
+      J.gh = alloc<float2>(static_cast<size_t>(N) * K);
       J.rows = alloc<uint32_t>(static_cast<size_t>(D_ + 1) * N);
       hipLaunchKernelGGL(k_iota, dim3(std::max(1u, ceil_div(N, 256))), dim3(256), 0, s, J.rows, N);
       J.off = alloc<uint32_t>(static_cast<size_t>(D_ + 1) * ((1u << D_) + 1));
       const size_t nheap = (2ull << D_) - 1;
-      J.ng = alloc<float>(nheap);
-      J.nh = alloc<float>(nheap);
+      J.ng = alloc<float>(nheap * K);
+      J.nh = alloc<float>(nheap * K);
       J.nc = alloc<uint32_t>(nheap);
-      J.hist = alloc<HistCell>(((1ull << D_) - 1) * J.F * 256);
+      J.hist = alloc<HistCell>(((1ull << D_) - 1) * K * J.F * 256);
       J.st = alloc<TreeState>(1);
       hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1), 0, s, J.st);
       J.cursor = alloc<uint32_t>(1u << D_);
       J.grad_wg_cnt = std::max(1u, ceil_div(N, kGradTile));
       J.grad_wg_off = wg_off[j];
       wg_off[j + 1] = wg_off[j] + J.grad_wg_cnt;
-      J.grad_partials = alloc<double>(2ull * J.grad_wg_cnt);
+      J.grad_partials = alloc<double>(2ull * J.grad_wg_cnt * K);
       J.n_trees = n_trees_;
       const size_t TD = static_cast<size_t>(n_trees_) * D_;
       J.m_feat = alloc<uint16_t>(TD);
@@ -411,18 +471,20 @@ class Trainer {
       J.m_fthr = alloc<float>(TD);
       J.m_gain = alloc<float>(TD);
       J.m_depth = alloc<uint16_t>(n_trees_);
-      J.m_leaves = alloc<float>(static_cast<size_t>(n_trees_) << D_);
+      J.m_leaves = alloc<float>((static_cast<size_t>(n_trees_) << D_) * K);
       TC_HIP(hipMemsetAsync(J.m_feat, 0, TD * 2, s));
       TC_HIP(hipMemsetAsync(J.m_bin, 0, TD, s));
       TC_HIP(hipMemsetAsync(J.m_fthr, 0, TD * 4, s));
       TC_HIP(hipMemsetAsync(J.m_gain, 0, TD * 4, s));
+      TC_HIP(hipMemsetAsync(J.m_depth, 0, 2ull * n_trees_, s));
+      TC_HIP(hipMemsetAsync(J.m_leaves, 0, ((static_cast<size_t>(n_trees_) << D_) * K) * 4, s));
       hist_rows_[j] = alloc<unsigned long long>(1);
       TC_HIP(hipMemsetAsync(hist_rows_[j], 0, 8, s));
       J.hist_rows_total = hist_rows_[j];
       // work-list capacities (see emit_hist / k_part_plan)
       for (uint32_t d = 0; d < D_; ++d) {
         const uint32_t hc = d == 0 ? ceil_div(N, ch) : ceil_div(N / 2, ch) + (1u << (d - 1));
-        hist_cap_[d] += hc * J.T;
+        hist_cap_[d] += hc * J.T * K;
         part_cap_[d] += ceil_div(N, kPartChunk) + (1u << d);
       }
     }
@@ -475,7 +537,7 @@ class Trainer {
 
   tc_ctx* ctx_;
   std::vector<JobSpec> specs_;
-  uint32_t njobs_ = 0, D_ = 0, n_trees_ = 0, fmax_ = 0;
+  uint32_t njobs_ = 0, D_ = 0, n_trees_ = 0, fmax_ = 0, kmax_ = 1;
   std::vector<JobDev> h_jobs_;
   std::vector<unsigned long long*> hist_rows_;
   JobDev* d_jobs_ = nullptr;
@@ -488,6 +550,9 @@ class Trainer {
   std::vector<Sampler> samplers_;
   bool forced_ = false;
   bool keep_rows_ = false;  // partition the last level too (leaf_index)
+
+  // Dynamic LDS of k_grad_update: the leaf values of one tree, [2^D][K].
+  size_t leaf_lds() const { return 4ull * (static_cast<size_t>(kmax_) << D_); }
 
   // Packed MSE histogram cells need h == 2 on every row of every job.
   bool packed_hist() const {
@@ -507,7 +572,9 @@ struct tc_trainer {
 namespace tc {
 
 float init_prediction_host(int32_t loss, const float* y, uint32_t n) {
-  // loss.cpp:217-226 (LogLoss) and :364-370 (MSE): sequential float sum
+  // loss.cpp:217-226 (LogLoss) and :364-370 (MSE): sequential float sum;
+  // softmax starts every class at 0 (loss.cpp:269-271)
+  if (loss == TC_LOSS_SOFTMAX) return 0.f;
   float s = 0.f;
   for (uint32_t i = 0; i < n; ++i) s += y[i];
   if (loss == TC_LOSS_MSE) return s / static_cast<float>(n);
@@ -521,6 +588,12 @@ void check_params(const tc_train_params* p) {
   TC_REQUIRE(p->tree.max_depth >= 1 && p->tree.max_depth <= static_cast<uint32_t>(kMaxDepth),
              "max_depth must be in [1, 10]");
   TC_REQUIRE(p->learning_rate == p->learning_rate, "learning_rate is NaN");
+  if (p->loss == TC_LOSS_SOFTMAX) {
+    TC_REQUIRE(p->n_classes >= 2 && p->n_classes <= static_cast<uint32_t>(kMaxClasses),
+               "softmax needs n_classes in [2, 16]");
+    TC_REQUIRE((p->n_classes << p->tree.max_depth) <= static_cast<uint32_t>(kMaxLeafValues),
+               "softmax: n_classes * 2^max_depth must be <= 8192");
+  }
 }
 
 }  // namespace tc
@@ -618,6 +691,7 @@ int32_t tc_trainer_sync(tc_trainer* t) {
   return guard([&] {
     TC_REQUIRE(t, "NULL trainer");
     TC_HIP(hipStreamSynchronize(t->ctx->stream));
+    t->tr->check_faults();
   });
 }
 
@@ -649,37 +723,52 @@ int32_t tc_last_train_timing(tc_ctx* ctx, double* loop_ms, double* hist_ms) {
   });
 }
 
+// Teacher-forced tree of K channels (class-major g, h).
+static void tree_build_k(tc_dataset* ds, const float* g, const float* h, uint32_t K,
+                         const tc_tree_params* tp, uint16_t* feature, uint8_t* bin, float* fthr,
+                         float* gain, float* leaves, uint32_t* leaf_idx, int32_t* depth) {
+  TC_REQUIRE(ds && g && h && tp && feature && bin && fthr && gain && leaves && depth,
+             "tc_tree_build: NULL argument");
+  tc_train_params p{};
+  p.loss = K > 1 ? TC_LOSS_SOFTMAX : TC_LOSS_MSE;  // only the channel count matters here
+  p.n_classes = K;
+  p.n_trees = 1;
+  p.learning_rate = 0.f;
+  p.tree = *tp;
+  p.subsample = 1.0f;
+  p.colsample_bytree = 1.0f;
+  check_params(&p);
+  TC_REQUIRE(ds->N > 0, "tc_tree_build: empty dataset");
+  JobSpec s{ds, nullptr, p, 0.f};
+  Trainer tr(ds->ctx, {s});
+  tr.set_gradients(g, h);
+  if (leaf_idx) tr.keep_final_rows();
+  tr.enqueue_tree(nullptr);
+  TC_HIP(hipStreamSynchronize(ds->ctx->stream));
+  const uint32_t D = tp->max_depth;
+  std::vector<uint16_t> dep(1);
+  tc_model m{};
+  m.feature = feature; m.bin = bin; m.fthr = fthr; m.gain = gain; m.depth = dep.data();
+  std::vector<float> lv((static_cast<size_t>(1) << D) * K);
+  m.leaves = lv.data();
+  tr.download(0, &m);
+  *depth = dep[0];
+  std::memcpy(leaves, lv.data(), sizeof(float) * (static_cast<size_t>(1) << dep[0]) * K);
+  if (leaf_idx) tr.leaf_index(leaf_idx);
+}
+
 int32_t tc_tree_build(tc_dataset* ds, const float* g, const float* h, const tc_tree_params* tp,
                       uint16_t* feature, uint8_t* bin, float* fthr, float* gain, float* leaves,
                       uint32_t* leaf_idx, int32_t* depth) {
+  return guard([&] { tree_build_k(ds, g, h, 1, tp, feature, bin, fthr, gain, leaves, leaf_idx, depth); });
+}
+
+int32_t tc_tree_build_multi(tc_dataset* ds, const float* g, const float* h, uint32_t K,
+                            const tc_tree_params* tp, uint16_t* feature, uint8_t* bin, float* fthr,
+                            float* gain, float* leaves, uint32_t* leaf_idx, int32_t* depth) {
   return guard([&] {
-    TC_REQUIRE(ds && g && h && tp && feature && bin && fthr && gain && leaves && depth,
-               "tc_tree_build: NULL argument");
-    tc_train_params p{};
-    p.loss = TC_LOSS_MSE;
-    p.n_trees = 1;
-    p.learning_rate = 0.f;
-    p.tree = *tp;
-    p.subsample = 1.0f;
-    p.colsample_bytree = 1.0f;
-    check_params(&p);
-    TC_REQUIRE(ds->N > 0, "tc_tree_build: empty dataset");
-    JobSpec s{ds, nullptr, p, 0.f};
-    Trainer tr(ds->ctx, {s});
-    tr.set_gradients(g, h);
-    if (leaf_idx) tr.keep_final_rows();
-    tr.enqueue_tree(nullptr);
-    TC_HIP(hipStreamSynchronize(ds->ctx->stream));
-    const uint32_t D = tp->max_depth;
-    std::vector<uint16_t> dep(1);
-    tc_model m{};
-    m.feature = feature; m.bin = bin; m.fthr = fthr; m.gain = gain; m.depth = dep.data();
-    std::vector<float> lv(1u << D);
-    m.leaves = lv.data();
-    tr.download(0, &m);
-    *depth = dep[0];
-    std::memcpy(leaves, lv.data(), sizeof(float) * (1u << dep[0]));
-    if (leaf_idx) tr.leaf_index(leaf_idx);
+    TC_REQUIRE(K >= 1 && K <= static_cast<uint32_t>(kMaxClasses), "K must be in [1, 16]");
+    tree_build_k(ds, g, h, K, tp, feature, bin, fthr, gain, leaves, leaf_idx, depth);
   });
 }
 
@@ -703,6 +792,7 @@ int32_t tc_hist_build(tc_dataset* ds, const float* g, const float* h, const uint
       std::memset(&J, 0, sizeof(J));
       J.bins_rm = ds->bins_rm; J.bins_cm = ds->bins_cm; J.N = N; J.F = F; J.Fp = ds->Fp;
       J.T = ds->Fp / kTileF;
+      J.K = 1;
       std::vector<float2> ghh(N);
       for (uint32_t i = 0; i < N; ++i) ghh[i] = make_float2(g[i], h[i]);
       J.gh = static_cast<float2*>(al(8ull * N));
@@ -802,7 +892,7 @@ int32_t tc_split_scan(tc_ctx* ctx, const float* hist, const float* node_g, const
       const size_t hb = static_cast<size_t>(F) * 256 * 8;
       JobDev J;
       std::memset(&J, 0, sizeof(J));
-      J.F = F; J.max_bins = max_bins; J.lambda_l2 = lambda_l2; J.D = level + 1;
+      J.F = F; J.K = 1; J.max_bins = max_bins; J.lambda_l2 = lambda_l2; J.D = level + 1;
       float2* fh = static_cast<float2*>(al(hb * nheap));
       TC_HIP(hipMemcpyAsync(fh + ((1ull << level) - 1) * F * 256, hist, hb * n_nodes,
                             hipMemcpyHostToDevice, s));
@@ -842,6 +932,32 @@ int32_t tc_split_scan(tc_ctx* ctx, const float* hist, const float* node_g, const
       throw;
     }
     for (void* p : owned) dfree(p);
+  });
+}
+
+int32_t tc_gradients_multi(tc_ctx* ctx, const float* y, const float* preds, float* g, float* h,
+                           uint32_t n, uint32_t K) {
+  return guard([&] {
+    TC_REQUIRE(ctx && y && preds && g && h, "tc_gradients_multi: NULL argument");
+    TC_REQUIRE(K >= 1 && K <= static_cast<uint32_t>(kMaxClasses), "K must be in [1, 16]");
+    if (n == 0) return;
+    const size_t nk = static_cast<size_t>(n) * K;
+    float* buf = dmalloc<float>(n + 3 * nk);
+    try {
+      float *dy = buf, *dp = buf + n, *dg = dp + nk, *dh = dg + nk;
+      TC_HIP(hipMemcpy(dy, y, 4ull * n, hipMemcpyHostToDevice));
+      TC_HIP(hipMemcpy(dp, preds, 4 * nk, hipMemcpyHostToDevice));
+      hipLaunchKernelGGL(k_gradients_multi, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, dy,
+                         dp, dg, dh, n, K);
+      TC_HIP(hipGetLastError());
+      TC_HIP(hipStreamSynchronize(ctx->stream));
+      TC_HIP(hipMemcpy(g, dg, 4 * nk, hipMemcpyDeviceToHost));
+      TC_HIP(hipMemcpy(h, dh, 4 * nk, hipMemcpyDeviceToHost));
+    } catch (...) {
+      dfree(buf);
+      throw;
+    }
+    dfree(buf);
   });
 }
 

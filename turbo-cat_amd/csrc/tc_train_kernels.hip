@@ -78,10 +78,10 @@ __device__ inline uint32_t find_item(const uint32_t* off, uint32_t n, uint32_t e
 }
 
 __device__ inline void put_hist_work(HistWork* dst, uint32_t job, uint32_t node, uint32_t start,
-                                     uint32_t len, uint32_t tile) {
+                                     uint32_t len, uint32_t tile, uint32_t cls) {
   HistWork w;
-  w.job = job; w.node = node; w.start = start; w.len = len; w.tile = tile;
-  w.pad0 = w.pad1 = w.pad2 = 0;
+  w.job = job; w.node = node; w.start = start; w.len = len; w.tile = tile; w.cls = cls;
+  w.pad1 = w.pad2 = 0;
   *dst = w;
 }
 
@@ -141,13 +141,65 @@ __device__ inline uint32_t job_of_block(const uint32_t* job_wg_off, uint32_t njo
   return lo;
 }
 
+// K-class rows of k_grad_update: every channel's prediction += lr * leaf
+// (fused), then A8 (CrossEntropyLoss::compute_multiclass_gradients +
+// softmax, loss.cpp:273-318): max shift, exp, sum in class order, divide,
+// g = p - [c == y], h = max(p (1 - p), 1e-6).
+__device__ void grad_update_softmax(const JobDev& J, uint32_t lw, int mode, int depth,
+                                    const uint16_t* s_f, const uint8_t* s_b, const float* s_leaf) {
+  const uint32_t N = J.N, K = J.K, tid = threadIdx.x;
+  float gmax = 0.f, hmax = 0.f;
+  for (int k = 0; k < kGradTile / 256; ++k) {
+    const uint32_t row = lw * kGradTile + k * 256 + tid;
+    if (row < N) {
+      uint32_t li = 0;
+      if (mode == 1)
+        for (int d = 0; d < depth; ++d) {
+          const uint8_t v = J.bins_cm[static_cast<size_t>(s_f[d]) * N + row];
+          li = (li << 1) | ((v != 255 && v > s_b[d]) ? 1u : 0u);
+        }
+      float e[kMaxClasses];  // static indices only (unrolled), so it stays in registers
+      float m = -INFINITY;
+#pragma unroll
+      for (uint32_t c = 0; c < kMaxClasses; ++c) {
+        if (c >= K) break;
+        float* pp = J.preds + static_cast<size_t>(c) * N + row;
+        const float pr = mode == 1 ? __fmaf_rn(J.lr, s_leaf[li * K + c], *pp) : J.base;
+        *pp = pr;
+        e[c] = pr;
+        m = fmaxf(m, pr);
+      }
+      float sum = 0.f;
+#pragma unroll
+      for (uint32_t c = 0; c < kMaxClasses; ++c) {
+        if (c >= K) break;
+        e[c] = expf(__fsub_rn(e[c], m));
+        sum = __fadd_rn(sum, e[c]);
+      }
+      const uint32_t lab = static_cast<uint32_t>(J.y[row]);
+#pragma unroll
+      for (uint32_t c = 0; c < kMaxClasses; ++c) {
+        if (c >= K) break;
+        const float p = __fdiv_rn(e[c], sum);
+        const float g = __fsub_rn(p, c == lab ? 1.0f : 0.0f);
+        const float hh = __fmul_rn(p, __fsub_rn(1.0f, p));
+        const float h = hh > 1e-6f ? hh : 1e-6f;
+        J.gh[static_cast<size_t>(c) * N + row] = make_float2(g, h);
+        gmax = fmaxf(gmax, fabsf(g));
+        hmax = fmaxf(hmax, h);
+      }
+    }
+  }
+  block_max2_to(J.st, gmax, hmax);
+}
+
 __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ jobs,
                                                      const uint32_t* __restrict__ job_wg_off,
                                                      uint32_t njobs, int mode, Lists L) {
   __shared__ uint32_t s_job;
   __shared__ uint16_t s_f[kMaxDepth];
   __shared__ uint8_t s_b[kMaxDepth];
-  __shared__ float s_leaf[1 << kMaxDepth];
+  extern __shared__ float s_leaf[];  // [2^D][K], sized by the launch
   const uint32_t tid = threadIdx.x;
   if (blockIdx.x == 0 && tid < kMaxDepth) {  // work lists of the next tree start empty
     *L.hist_count[tid] = 0;
@@ -168,11 +220,12 @@ __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ 
       s_f[tid] = J.m_feat[static_cast<size_t>(t) * J.D + tid];
       s_b[tid] = J.m_bin[static_cast<size_t>(t) * J.D + tid];
     }
-    float* out = J.m_leaves + static_cast<size_t>(t) * (1u << J.D);
-    for (uint32_t i = tid; i < (1u << J.D); i += 256) {
+    const uint32_t K = J.K;
+    float* out = J.m_leaves + (static_cast<size_t>(t) << J.D) * K;
+    for (uint32_t i = tid; i < (K << J.D); i += 256) {  // i = leaf * K + channel
       float v = 0.f;
-      if (i < (1u << depth)) {
-        const uint32_t hi = heap_index(depth, i);
+      if (i < (K << depth)) {
+        const uint32_t hi = heap_index(depth, i / K) * K + i % K;
         const float G = J.ng[hi], H = J.nh[hi];
         if (!(H < 1e-10f)) {
           v = __fdiv_rn(-G, __fadd_rn(H, J.lambda_l2));
@@ -189,6 +242,10 @@ __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ 
     }
     if (lw == 0 && tid == 0) J.m_depth[t] = static_cast<uint16_t>(depth);
     __syncthreads();
+  }
+  if (J.K > 1) {  // softmax channels
+    grad_update_softmax(J, lw, mode, depth, s_f, s_b, s_leaf);
+    return;
   }
   // each lane takes rows r + 256k (k < 4); all loads are issued before use
   constexpr int kR = kGradTile / 256;
@@ -232,25 +289,31 @@ __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ 
 }
 
 // Root of a new tree (symmetric_tree.cpp:97-124), one 256-thread workgroup
-// per job: totals from the root partials (double, fixed order), level-0 stop
-// test, the tree's histogram scales, root work lists.
+// per job: per-channel totals from the root partials (double, fixed order),
+// level-0 stop test, the tree's histogram scales, root work lists.
 // packed: histograms use the packed MSE cell (h == 2 for every row).
 __device__ void tree_begin(const JobDev& J, uint32_t job, const Lists& L, int packed) {
-  const uint32_t tid = threadIdx.x;
+  const uint32_t tid = threadIdx.x, K = J.K;
   __shared__ int s_go;
   __shared__ uint32_t s_hbase, s_pbase;
   __shared__ double s_tot[2][4];
+  __shared__ double s_gh[2 * kMaxClasses];
   const uint32_t n = J.n_root;
-  const uint32_t hcnt = ((n + J.hist_chunk - 1) / J.hist_chunk) * J.T, pcnt = part_slots(n);
-  {  // thread t adds tiles t, t + 256, ... in order; then a fixed tree
+  const uint32_t hcnt = ((n + J.hist_chunk - 1) / J.hist_chunk) * J.T * K, pcnt = part_slots(n);
+  for (uint32_t c = 0; c < K; ++c) {  // thread t adds tiles t, t + 256, ... in order; then a fixed tree
     double a = 0.0, b = 0.0;
     for (uint32_t w = tid; w < J.grad_wg_cnt; w += 256) {
-      a += J.grad_partials[2 * w];
-      b += J.grad_partials[2 * w + 1];
+      a += J.grad_partials[2 * (w * K + c)];
+      b += J.grad_partials[2 * (w * K + c) + 1];
     }
     a = warp_sum_d(a);
     b = warp_sum_d(b);
     if ((tid & 63) == 0) { s_tot[0][tid >> 6] = a; s_tot[1][tid >> 6] = b; }
+    __syncthreads();
+    if (tid == 0) {
+      s_gh[2 * c] = (s_tot[0][0] + s_tot[0][1]) + (s_tot[0][2] + s_tot[0][3]);
+      s_gh[2 * c + 1] = (s_tot[1][0] + s_tot[1][1]) + (s_tot[1][2] + s_tot[1][3]);
+    }
     __syncthreads();
   }
   if (tid == 0) {
@@ -264,14 +327,16 @@ __device__ void tree_begin(const JobDev& J, uint32_t job, const Lists& L, int pa
     st->hmax_next = 0u;
     int go = 0;
     if (t < static_cast<int>(J.n_trees)) {
-      const double G = (s_tot[0][0] + s_tot[0][1]) + (s_tot[0][2] + s_tot[0][3]);
-      const double H = (s_tot[1][0] + s_tot[1][1]) + (s_tot[1][2] + s_tot[1][3]);
-      J.ng[0] = static_cast<float>(G);
-      J.nh[0] = static_cast<float>(H);
+      float Hsum = 0.f;  // the stop test's H sums the channels in class order
+      for (uint32_t c = 0; c < K; ++c) {
+        J.ng[c] = static_cast<float>(s_gh[2 * c]);
+        J.nh[c] = static_cast<float>(s_gh[2 * c + 1]);
+        Hsum = __fadd_rn(Hsum, J.nh[c]);
+      }
       J.nc[0] = n;
       J.off[0] = 0;
       J.off[1] = n;
-      go = (n >= 2u * J.min_samples_leaf && J.nh[0] >= J.min_child_weight * 2.0f) ? 1 : 0;
+      go = (n >= 2u * J.min_samples_leaf && Hsum >= J.min_child_weight * 2.0f) ? 1 : 0;
       if (n == 0) go = 0;  // empty sample: depth-0 tree, leaf 0 (:39-43)
       const int eg = fixed_exponent(gmax, n, J.hist_chunk, packed != 0);
       const int eh = fixed_exponent(hmax, n, J.hist_chunk, false);
@@ -290,10 +355,12 @@ __device__ void tree_begin(const JobDev& J, uint32_t job, const Lists& L, int pa
   }
   __syncthreads();
   if (!s_go) return;
-  // chunk-major: the tiles of one row chunk are adjacent (and share an XCD)
+  // chunk-major, then tile, then channel: the items reading the same row
+  // segments are adjacent (and share an XCD)
+  const uint32_t TK = J.T * K;
   for (uint32_t e = tid; e < hcnt; e += blockDim.x) {
-    const uint32_t start = (e / J.T) * J.hist_chunk;
-    put_hist_work(L.hist[0] + s_hbase + e, job, 0, start, min(J.hist_chunk, n - start), e % J.T);
+    const uint32_t start = (e / TK) * J.hist_chunk, r = e % TK;
+    put_hist_work(L.hist[0] + s_hbase + e, job, 0, start, min(J.hist_chunk, n - start), r / K, r % K);
   }
   for (uint32_t e = tid; e < pcnt; e += blockDim.x) {
     const uint32_t start = e * kPartChunk;
@@ -302,11 +369,11 @@ __device__ void tree_begin(const JobDev& J, uint32_t job, const Lists& L, int pa
   }
 }
 
-// Start of a tree, over the row grid: root totals (sum g, sum h) over
-// 1024-position tiles of the root list rows[0][0 .. n_root), double, fixed
-// order within the tile (k_tree_begin adds the tiles in a fixed order); the
-// root's histogram zeroed (deeper levels are zeroed by the split scan of the
-// level above).
+// Start of a tree, over the row grid: per-channel root totals (sum g, sum h)
+// over 1024-position tiles of the root list rows[0][0 .. n_root), double,
+// fixed order within the tile (k_tree_begin adds the tiles in a fixed
+// order); the root's histograms zeroed (deeper levels are zeroed by the
+// split scan of the level above).
 __global__ void __launch_bounds__(256) k_root_partials(const JobDev* __restrict__ jobs,
                                                        const uint32_t* __restrict__ job_wg_off,
                                                        uint32_t njobs) {
@@ -316,30 +383,37 @@ __global__ void __launch_bounds__(256) k_root_partials(const JobDev* __restrict_
   if (tid == 0) s_job = job_of_block(job_wg_off, njobs);
   __syncthreads();
   const JobDev& J = jobs[s_job];
-  const uint32_t lw = blockIdx.x - J.grad_wg_off;
-  double sg = 0.0, sh = 0.0;
-  if (J.st->tree + 1 < static_cast<int>(J.n_trees)) {
-    const uint32_t cells = J.F * 256u;  // the root's histogram
+  const uint32_t lw = blockIdx.x - J.grad_wg_off, K = J.K;
+  const bool live = J.st->tree + 1 < static_cast<int>(J.n_trees);
+  if (live) {
+    const uint32_t cells = K * J.F * 256u;  // the root's histograms
     const uint32_t per = (cells + J.grad_wg_cnt - 1) / J.grad_wg_cnt;
     const uint32_t c1 = min(cells, (lw + 1) * per);
     for (uint32_t c = lw * per + tid; c < c1; c += 256) J.hist[c] = HistCell{0, 0};
-    for (int k = 0; k < kGradTile / 256; ++k) {
-      const uint32_t p = lw * kGradTile + k * 256 + tid;
-      if (p >= J.n_root) break;
-      const float2 v = J.gh[J.root_identity ? p : J.rows[p]];
-      sg += v.x;
-      sh += v.y;
-    }
   }
-  sg = warp_sum_d(sg);
-  sh = warp_sum_d(sh);
-  if ((tid & 63) == 0) { s_red[0][tid >> 6] = sg; s_red[1][tid >> 6] = sh; }
-  __syncthreads();
-  if (tid == 0) {
-    double a = 0.0, b = 0.0;
-    for (int w = 0; w < 4; ++w) { a += s_red[0][w]; b += s_red[1][w]; }
-    J.grad_partials[2 * lw + 0] = a;
-    J.grad_partials[2 * lw + 1] = b;
+  for (uint32_t c = 0; c < K; ++c) {
+    double sg = 0.0, sh = 0.0;
+    if (live) {
+      const float2* gh = J.gh + static_cast<size_t>(c) * J.N;
+      for (int k = 0; k < kGradTile / 256; ++k) {
+        const uint32_t p = lw * kGradTile + k * 256 + tid;
+        if (p >= J.n_root) break;
+        const float2 v = gh[J.root_identity ? p : J.rows[p]];
+        sg += v.x;
+        sh += v.y;
+      }
+    }
+    sg = warp_sum_d(sg);
+    sh = warp_sum_d(sh);
+    if ((tid & 63) == 0) { s_red[0][tid >> 6] = sg; s_red[1][tid >> 6] = sh; }
+    __syncthreads();
+    if (tid == 0) {
+      double a = 0.0, b = 0.0;
+      for (int w = 0; w < 4; ++w) { a += s_red[0][w]; b += s_red[1][w]; }
+      J.grad_partials[2 * (lw * K + c)] = a;
+      J.grad_partials[2 * (lw * K + c) + 1] = b;
+    }
+    __syncthreads();
   }
 }
 
@@ -350,7 +424,7 @@ __global__ void __launch_bounds__(256) k_tree_begin(const JobDev* __restrict__ j
 
 // ---------------------------------------------------------------------------
 // A9: per-level histogram.  One workgroup = one work item (row chunk of one
-// node) x one 16-feature tile.  Each lane loads the tile's 16 bins of a row
+// node, one gradient channel) x one 16-feature tile.  Each lane loads the tile's 16 bins of a row
 // with one 16-byte load from the row-major bins and the row's (g, h), turns
 // them into fixed point once, and adds them into the workgroup's LDS
 // histogram with 64-bit integer atomics (feature stride 257 cells).  Lane l
@@ -388,7 +462,7 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
   const uint8_t* __restrict__ brm = J.bins_rm + w.tile * kTileF;
   const uint32_t* __restrict__ rows = J.rows + static_cast<size_t>(level) * J.N + w.start;
   const bool direct = level == 0 && J.root_identity;  // row = position, no list read
-  const float2* __restrict__ gh = J.gh;
+  const float2* __restrict__ gh = J.gh + static_cast<size_t>(w.cls) * J.N;
   const uint32_t Fp = J.Fp;
   const uint32_t s = tid & 15u;  // this lane's starting feature
   const uint32_t q = s >> 2, r = s & 3u;
@@ -440,7 +514,8 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
     }
   }
   __syncthreads();
-  HistCell* out = J.hist + (static_cast<size_t>(heap_index(level, w.node)) * J.F + f0) * 256u;
+  HistCell* out =
+      J.hist + ((static_cast<size_t>(heap_index(level, w.node)) * J.K + w.cls) * J.F + f0) * 256u;
   const long long h2 = PACKED ? __double2ll_rn(2.0 * hsc) : 0;
   for (uint32_t i = tid; i < nf * 256u; i += kHistThreads) {
     const uint32_t f = i >> 8, b = i & 255u;
@@ -484,26 +559,30 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
   const uint32_t job = blockIdx.y, f = blockIdx.x, tid = threadIdx.x;
   const JobDev& J = jobs[job];
   if (!J.st->active || f >= J.F) return;
-  const uint32_t nn = 1u << level;
+  const uint32_t K = J.K;
+  const uint32_t nu = K << level;  // units (node, channel), node-major
   const uint32_t mb = J.max_bins & 0xFFu;
   const float lambda = J.lambda_l2;
   const double gi = FIXED ? J.st->ginv : 1.0, hi = FIXED ? J.st->hinv : 1.0;
   const uint32_t lane = tid & 63u, wave = tid >> 6;
-  const uint32_t bin = tid & 255u, part = tid >> 8;  // candidate, node subset
+  const uint32_t bin = tid & 255u, part = tid >> 8;  // candidate, unit subset
+  const size_t F = J.F;
   float total = 0.f;
   bool any = false;
-  for (uint32_t c0 = 0; c0 < nn; c0 += kScanNodesPerPass) {
-    const uint32_t cs = min(static_cast<uint32_t>(kScanNodesPerPass), nn - c0);
+  for (uint32_t c0 = 0; c0 < nu; c0 += kScanNodesPerPass) {
+    const uint32_t cs = min(static_cast<uint32_t>(kScanNodesPerPass), nu - c0);
     if (FIXED) {
-      // exact prefix: one wave per node, lane l holds bins 4l .. 4l+3.  At
+      // exact prefix: one wave per unit, lane l holds bins 4l .. 4l+3.  At
       // level > 0 the larger child of each pair is formed here, parent -
       // smaller (A13: Histogram::subtract_from, histogram.cpp:39-42 via
       // symmetric_tree.cpp:346), and stored; the node's two level+1
       // histograms are zeroed for the next level's k_hist.
       for (uint32_t k = wave; k < cs; k += NW) {
-        const uint32_t node = c0 + k;
-        const size_t F = J.F;
-        HistCell* hp = J.hist + (static_cast<size_t>(heap_index(level, node)) * F + f) * 256u;
+        const uint32_t node = (c0 + k) / K, ch = (c0 + k) % K;
+        auto cells = [&](uint32_t lvl, uint32_t nd) {
+          return J.hist + ((static_cast<size_t>(heap_index(lvl, nd)) * K + ch) * F + f) * 256u;
+        };
+        HistCell* hp = cells(level, node);
         HistCell cell[4];
         bool larger = false;
         uint32_t sm = node;
@@ -514,8 +593,8 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
           larger = node != sm;
         }
         if (larger) {
-          const HistCell* pp = J.hist + (static_cast<size_t>(heap_index(level - 1, node >> 1)) * F + f) * 256u;
-          const HistCell* sp = J.hist + (static_cast<size_t>(heap_index(level, sm)) * F + f) * 256u;
+          const HistCell* pp = cells(level - 1, node >> 1);
+          const HistCell* sp = cells(level, sm);
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const HistCell a = pp[4 * lane + i], b = sp[4 * lane + i];
@@ -527,8 +606,8 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
           for (int i = 0; i < 4; ++i) cell[i] = hp[4 * lane + i];
         }
         if (level + 1 < J.D) {
-          HistCell* z0 = J.hist + (static_cast<size_t>(heap_index(level + 1, 2 * node)) * F + f) * 256u;
-          HistCell* z1 = J.hist + (static_cast<size_t>(heap_index(level + 1, 2 * node + 1)) * F + f) * 256u;
+          HistCell* z0 = cells(level + 1, 2 * node);
+          HistCell* z1 = cells(level + 1, 2 * node + 1);
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             z0[4 * lane + i] = HistCell{0, 0};
@@ -559,8 +638,8 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
       __syncthreads();
     } else {
       // the reference's sequential float prefix over the given float bins
-      for (uint32_t k = 0; k < cs; ++k)
-        s_pref[k][tid] = fhist[(static_cast<size_t>(heap_index(level, c0 + k)) * J.F + f) * 256u + tid];
+      for (uint32_t k = 0; k < cs; ++k)  // K == 1 here
+        s_pref[k][tid] = fhist[(static_cast<size_t>(heap_index(level, c0 + k)) * F + f) * 256u + tid];
       __syncthreads();
       if (tid < cs) {
         float lg = 0.f, lh = 0.f;
@@ -573,13 +652,15 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
       }
       __syncthreads();
     }
-    // gain of candidate `bin`: node terms in node order (part p takes the
-    // nodes k = p, p + Q, ...; Q = 1 is the reference's single sequence)
+    // gain of candidate `bin`: unit terms in (node, channel) order (part p
+    // takes the units k = p, p + Q, ...; Q = 1 is the reference's single
+    // sequence)
     if (bin + 1 < mb) {
       for (uint32_t k = part; k < cs; k += Q) {
-        const uint32_t hn = heap_index(level, c0 + k);
+        const uint32_t hn = heap_index(level, (c0 + k) / K);
         if (level > 0 && J.nc[hn] == 0) continue;
-        const float G = J.ng[hn], H = J.nh[hn];
+        const uint32_t hu = hn * K + (c0 + k) % K;
+        const float G = J.ng[hu], H = J.nh[hu];
         const float2 Lf = s_pref[k][bin];
         const float rg = __fsub_rn(G, Lf.x), rh = __fsub_rn(H, Lf.y);
         if (Lf.y < 1e-10f || rh < 1e-10f) continue;
@@ -650,13 +731,14 @@ __device__ inline bool decode_split(const JobDev& J, uint32_t level, bool record
   return accept;
 }
 
-// Child totals of node n under split (f, thr), by one wave: left = bins <=
-// thr plus the NaN slot 255, right = the other value bins, exact; lane 0
-// stores them (float) with the node's partition cursors.
-__device__ inline void child_totals(const JobDev& J, uint32_t level, uint32_t n, uint32_t f,
-                                    uint32_t thr) {
-  const uint32_t lane = threadIdx.x & 63u;
-  const HistCell* hp = J.hist + (static_cast<size_t>(heap_index(level, n)) * J.F + f) * 256u;
+// Child totals of node n, channel c under split (f, thr), by one wave: left
+// = bins <= thr plus the NaN slot 255, right = the other value bins, exact;
+// lane 0 stores them (float), and for channel 0 the node's partition cursors.
+__device__ inline void child_totals(const JobDev& J, uint32_t level, uint32_t n, uint32_t c,
+                                    uint32_t f, uint32_t thr) {
+  const uint32_t lane = threadIdx.x & 63u, K = J.K;
+  const HistCell* hp =
+      J.hist + ((static_cast<size_t>(heap_index(level, n)) * K + c) * J.F + f) * 256u;
   long long v[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -672,25 +754,27 @@ __device__ inline void child_totals(const JobDev& J, uint32_t level, uint32_t n,
   if (lane == 0) {
     const double gi = J.st->ginv, hi = J.st->hinv;
     const uint32_t cl = heap_index(level + 1, 2 * n);
-    J.ng[cl] = static_cast<float>(static_cast<double>(v[0]) * gi);
-    J.nh[cl] = static_cast<float>(static_cast<double>(v[1]) * hi);
-    J.ng[cl + 1] = static_cast<float>(static_cast<double>(v[2]) * gi);
-    J.nh[cl + 1] = static_cast<float>(static_cast<double>(v[3]) * hi);
-    const uint32_t* offd = J.off + static_cast<size_t>(level) * ((1u << J.D) + 1u);
-    J.cursor[2 * n] = offd[n];
-    J.cursor[2 * n + 1] = offd[n + 1];
+    J.ng[cl * K + c] = static_cast<float>(static_cast<double>(v[0]) * gi);
+    J.nh[cl * K + c] = static_cast<float>(static_cast<double>(v[1]) * hi);
+    J.ng[(cl + 1) * K + c] = static_cast<float>(static_cast<double>(v[2]) * gi);
+    J.nh[(cl + 1) * K + c] = static_cast<float>(static_cast<double>(v[3]) * hi);
+    if (c == 0) {
+      const uint32_t* offd = J.off + static_cast<size_t>(level) * ((1u << J.D) + 1u);
+      J.cursor[2 * n] = offd[n];
+      J.cursor[2 * n + 1] = offd[n + 1];
+    }
   }
 }
 
-// Finalize launch: workgroup (group of 4 nodes, job), one wave per node;
-// workgroup 0 records the split.
+// Finalize launch: workgroup (group of 4 units, job), one wave per unit
+// (node, channel); workgroup 0 records the split.
 __global__ void __launch_bounds__(256) k_finalize(const JobDev* __restrict__ jobs, uint32_t level) {
   const JobDev& J = jobs[blockIdx.y];
   if (!J.st->active) return;
   uint32_t f, thr;
   const bool accept = decode_split(J, level, blockIdx.x == 0 && threadIdx.x == 0, &f, &thr);
-  const uint32_t n = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (accept && n < (1u << level)) child_totals(J, level, n, f, thr);
+  const uint32_t u = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (accept && u < (J.K << level)) child_totals(J, level, u / J.K, u % J.K, f, thr);
 }
 
 // ---------------------------------------------------------------------------
@@ -747,6 +831,7 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(const JobDev* __rest
     }
   }
   __syncthreads();
+  __shared__ uint32_t s_tot[2];
   if (tid < 2) {  // offsets in (step, wave) order; one reservation per side
     uint32_t acc = 0;
     for (int i = 0; i < kIt * kW; ++i) {
@@ -754,9 +839,20 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(const JobDev* __rest
       s_cnt[tid][i] = acc;
       acc += v;
     }
+    s_tot[tid] = acc;
     s_base[tid] = acc == 0 ? 0u : (tid == 0 ? atomicAdd(&cur[0], acc) : atomicSub(&cur[1], acc) - acc);
   }
   __syncthreads();
+  {  // both reservations must lie inside the node's segment; otherwise write nothing
+    const uint32_t* offd = J.off + static_cast<size_t>(level) * ((1u << J.D) + 1u);
+    const uint32_t lo = offd[w.node], hi = offd[w.node + 1];
+    const bool bad = (s_tot[0] && (s_base[0] < lo || s_base[0] > hi || s_tot[0] > hi - s_base[0])) ||
+                     (s_tot[1] && (s_base[1] < lo || s_base[1] > hi || s_tot[1] > hi - s_base[1]));
+    if (bad) {
+      if (tid == 0) atomicOr(&J.st->fault, 1u);
+      return;
+    }
+  }
 #pragma unroll
   for (int k = 0; k < kIt; ++k) {
     const bool left = (lbits >> k) & 1u, right = (rbits >> k) & 1u;
@@ -798,14 +894,20 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
     s_off[2 * n + 1] = offd[n] + acc;
     offn[2 * n] = offd[n];
     offn[2 * n + 1] = offd[n] + acc;
-    // stop test of the next level (symmetric_tree.cpp:115-124)
-    if ((acc >= 2u * J.min_samples_leaf && J.nh[cl] >= mcw2) ||
-        (len - acc >= 2u * J.min_samples_leaf && J.nh[cl + 1] >= mcw2))
+    // stop test of the next level (symmetric_tree.cpp:115-124; H sums the
+    // channels in class order)
+    float hl = 0.f, hr = 0.f;
+    for (uint32_t c = 0; c < J.K; ++c) {
+      hl = __fadd_rn(hl, J.nh[cl * J.K + c]);
+      hr = __fadd_rn(hr, J.nh[(cl + 1) * J.K + c]);
+    }
+    if ((acc >= 2u * J.min_samples_leaf && hl >= mcw2) ||
+        (len - acc >= 2u * J.min_samples_leaf && hr >= mcw2))
       s_any = 1;
     // next level: histogram of the smaller child of the pair (ties: left),
     // partition slots of both children
     const uint32_t sm_len = min(acc, len - acc);
-    s_hw[n] = ((sm_len + J.hist_chunk - 1) / J.hist_chunk) * J.T;
+    s_hw[n] = ((sm_len + J.hist_chunk - 1) / J.hist_chunk) * J.T * J.K;
     s_pw[2 * n] = part_slots(acc);
     s_pw[2 * n + 1] = part_slots(len - acc);
     if (sm_len) atomicAdd(&s_rows, static_cast<unsigned long long>(sm_len));
@@ -833,9 +935,10 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
     const uint32_t lc = 2 * n, rc = lc + 1;
     const uint32_t nl = s_off[lc + 1] - s_off[lc], nr = s_off[rc + 1] - s_off[rc];
     const uint32_t sm = nl <= nr ? lc : rc;
-    const uint32_t le = e - s_hw[n];
-    const uint32_t start = s_off[sm] + (le / J.T) * J.hist_chunk;
-    put_hist_work(hout + e, job, sm, start, min(J.hist_chunk, s_off[sm + 1] - start), le % J.T);
+    const uint32_t le = e - s_hw[n], TK = J.T * J.K, r = le % TK;
+    const uint32_t start = s_off[sm] + (le / TK) * J.hist_chunk;
+    put_hist_work(hout + e, job, sm, start, min(J.hist_chunk, s_off[sm + 1] - start), r / J.K,
+                  r % J.K);
   }
   PartWork* pout = L.part[level + 1] + s_pbase;
   for (uint32_t e = tid; e < ptot; e += 256) {
@@ -846,14 +949,14 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
   }
 }
 
-// Copy externally supplied gradients (teacher forcing) into gh, recording
-// max |g| and max |h| for the histogram scales.
+// Copy externally supplied gradients (teacher forcing, [K][N]) into gh,
+// recording max |g| and max |h| for the histogram scales.
 __global__ void __launch_bounds__(256) k_set_gh(const JobDev* __restrict__ jobs, const float* g,
                                                 const float* h) {
   const JobDev& J = jobs[0];
-  const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t p = blockIdx.x * 256 + threadIdx.x;  // over K * N, class-major
   float a = 0.f, b = 0.f;
-  if (p < J.N) {
+  if (p < J.N * J.K) {
     J.gh[p] = make_float2(g[p], h[p]);
     a = fabsf(g[p]);
     b = fabsf(h[p]);

@@ -19,7 +19,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libturbocat_hip.so")
 
-TC_LOSS_MSE, TC_LOSS_LOGLOSS = 0, 1
+TC_LOSS_MSE, TC_LOSS_LOGLOSS, TC_LOSS_SOFTMAX = 0, 1, 2
 TC_EBUSY, TC_ETIMEOUT = -5, -6
 
 # Every exported symbol of include/turbocat_hip.h (checked by the tests).
@@ -27,12 +27,13 @@ EXPORTS = [
     "tc_ctx_create", "tc_ctx_destroy", "tc_ctx_synchronize", "tc_last_error",
     "tc_device_alloc", "tc_device_free", "tc_memcpy_h2d", "tc_memcpy_d2h",
     "tc_quantize_fit", "tc_dataset_create", "tc_dataset_bins", "tc_dataset_shape",
-    "tc_dataset_destroy", "tc_gradients", "tc_random_subsample", "tc_init_prediction",
-    "tc_hist_build",
-    "tc_hist_subtract", "tc_split_scan", "tc_tree_build", "tc_train", "tc_train_batch",
+    "tc_dataset_destroy", "tc_gradients", "tc_gradients_multi", "tc_random_subsample",
+    "tc_init_prediction", "tc_hist_build",
+    "tc_hist_subtract", "tc_split_scan", "tc_tree_build", "tc_tree_build_multi", "tc_train",
+    "tc_train_batch",
     "tc_last_train_timing", "tc_trainer_create", "tc_trainer_step", "tc_trainer_step_timed",
     "tc_trainer_sync", "tc_trainer_model", "tc_trainer_hist_rows", "tc_trainer_destroy",
-    "tc_ensemble_create", "tc_ensemble_destroy", "tc_predict",
+    "tc_ensemble_create", "tc_ensemble_create_multi", "tc_ensemble_destroy", "tc_predict",
     "tc_predict_dev", "tc_predict_binned", "tc_serve_start", "tc_serve_submit",
     "tc_serve_wait", "tc_serve_stats", "tc_serve_stop",
 ]
@@ -54,7 +55,8 @@ class TreeParams(Structure):
 class TrainParams(Structure):
     _fields_ = [("loss", c_int32), ("n_trees", c_uint32), ("learning_rate", c_float),
                 ("tree", TreeParams), ("hist_chunk", c_uint32), ("use_graph", c_int32),
-                ("subsample", c_float), ("colsample_bytree", c_float), ("seed", c_uint64)]
+                ("subsample", c_float), ("colsample_bytree", c_float), ("seed", c_uint64),
+                ("n_classes", c_uint32)]
 
 
 class Model(Structure):
@@ -75,9 +77,15 @@ def tree_params(max_depth=6, max_bins=255, lambda_l2=1.0, lambda_l1=0.0, max_del
 
 
 def train_params(loss=TC_LOSS_MSE, n_trees=100, learning_rate=0.1, hist_chunk=0, use_graph=1,
-                 subsample=1.0, colsample_bytree=1.0, seed=42, **tree) -> TrainParams:
+                 subsample=1.0, colsample_bytree=1.0, seed=42, n_classes=0,
+                 **tree) -> TrainParams:
     return TrainParams(loss, n_trees, learning_rate, tree_params(**tree), hist_chunk, use_graph,
-                       subsample, colsample_bytree, seed)
+                       subsample, colsample_bytree, seed, n_classes)
+
+
+def channels(p: TrainParams):
+    """Gradient channels of a fit: K classes for softmax, else 1."""
+    return p.n_classes if p.loss == TC_LOSS_SOFTMAX else 1
 
 
 def random_subsample(n, ratio, seed):
@@ -117,10 +125,12 @@ def load(path: str = LIB_PATH):
         "tc_dataset_create": [V, V, U, U, V, V, U, V],
         "tc_dataset_bins": [V, V], "tc_dataset_shape": [V, V, V], "tc_dataset_destroy": [V],
         "tc_gradients": [V, I, V, V, V, V, U], "tc_init_prediction": [I, V, U, V],
+        "tc_gradients_multi": [V, V, V, V, V, U, U],
         "tc_random_subsample": [U, c_float, c_uint64, V, V],
         "tc_hist_build": [V, V, V, V, U, V], "tc_hist_subtract": [V, V, V, V, U],
         "tc_split_scan": [V, V, V, V, V, U, U, U, c_float, I, V, V, V],
         "tc_tree_build": [V, V, V, POINTER(TreeParams), V, V, V, V, V, V, V],
+        "tc_tree_build_multi": [V, V, V, U, POINTER(TreeParams), V, V, V, V, V, V, V],
         "tc_train": [V, V, POINTER(TrainParams), POINTER(Model)],
         "tc_train_batch": [V, U, V, V, V, V],
         "tc_last_train_timing": [V, V, V],
@@ -129,6 +139,7 @@ def load(path: str = LIB_PATH):
         "tc_trainer_model": [V, U, POINTER(Model)], "tc_trainer_hist_rows": [V, U, V],
         "tc_trainer_destroy": [V],
         "tc_ensemble_create": [V, U, U, V, V, V, V, V, V, c_float, U, V],
+        "tc_ensemble_create_multi": [V, U, U, U, V, V, V, V, V, V, c_float, U, V],
         "tc_ensemble_destroy": [V], "tc_predict": [V, V, U, U, I, V, V],
         "tc_predict_dev": [V, V, U, U, I, V], "tc_predict_binned": [V, V, V],
         "tc_serve_start": [V, POINTER(ServeParams), V], "tc_serve_submit": [V, V, U, V, V],
@@ -207,6 +218,16 @@ class Context:
         check(self.L.tc_gradients(self.h, loss, _p(y), _p(p), _p(g), _p(h), len(y)))
         return g, h
 
+    def gradients_multi(self, y, preds):
+        """A8 softmax gradients; preds (n, K) row-major -> g, h (n, K)."""
+        y = np.ascontiguousarray(y, np.float32)
+        p = np.ascontiguousarray(preds, np.float32)
+        n, K = p.shape
+        g = np.zeros_like(p)
+        h = np.zeros_like(p)
+        check(self.L.tc_gradients_multi(self.h, _p(y), _p(p), _p(g), _p(h), n, K))
+        return g, h
+
     def hist_subtract(self, parent, child):
         parent = np.ascontiguousarray(parent, np.float32)
         child = np.ascontiguousarray(child, np.float32)
@@ -239,7 +260,7 @@ class Context:
         Y = (c_void_p * n)(*[y.ctypes.data for y in ys])
         models, arrs = [], []
         for d, p in zip(datasets, params):
-            m, a = _new_model(p.n_trees, p.tree.max_depth, d.n)
+            m, a = _new_model(p.n_trees, p.tree.max_depth, d.n, channels(p))
             models.append(m)
             arrs.append(a)
         M = (Model * n)(*models)
@@ -261,11 +282,13 @@ class Context:
         return Ensemble(self, model, n_features)
 
 
-def _new_model(T, D, n):
-    a = dict(T=T, D=D, feature=np.zeros(T * D, np.uint16), bin=np.zeros(T * D, np.uint8),
+def _new_model(T, D, n, K=1):
+    """Host arrays of a model; with K channels leaves are [tree][leaf][class]
+    and train_preds class-major (K, n)."""
+    a = dict(T=T, D=D, K=K, feature=np.zeros(T * D, np.uint16), bin=np.zeros(T * D, np.uint8),
              fthr=np.zeros(T * D, np.float32), gain=np.zeros(T * D, np.float32),
-             depth=np.zeros(T, np.uint16), leaves=np.zeros(T << D, np.float32),
-             train_preds=np.zeros(n, np.float32))
+             depth=np.zeros(T, np.uint16), leaves=np.zeros((T << D) * K, np.float32),
+             train_preds=np.zeros(n * K, np.float32) if K == 1 else np.zeros((K, n), np.float32))
     m = Model(_p(a["feature"]), _p(a["bin"]), _p(a["fthr"]), _p(a["gain"]), _p(a["depth"]),
               _p(a["leaves"]), _p(a["train_preds"]), 0.0)
     return m, a
@@ -313,8 +336,23 @@ class Dataset:
         return dict(depth=d, feature=feat[:d], bin=b[:d], fthr=ft[:d], gain=gn[:d],
                     leaves=lv[:1 << d], leaf_idx=li)
 
+    def tree_multi(self, g, h, params: TreeParams, want_leaf_idx=True):
+        """A18 teacher-forced K-channel tree; g, h class-major (K, n)."""
+        g = np.ascontiguousarray(g, np.float32); h = np.ascontiguousarray(h, np.float32)
+        K = g.shape[0]
+        D = params.max_depth
+        feat = np.zeros(D, np.uint16); b = np.zeros(D, np.uint8); ft = np.zeros(D, np.float32)
+        gn = np.zeros(D, np.float32); lv = np.zeros((1 << D) * K, np.float32)
+        li = np.zeros(self.n, np.uint32) if want_leaf_idx else None
+        dep = c_int32()
+        check(self.ctx.L.tc_tree_build_multi(self.h, _p(g), _p(h), K, byref(params), _p(feat),
+                                             _p(b), _p(ft), _p(gn), _p(lv), _p(li), byref(dep)))
+        d = dep.value
+        return dict(depth=d, feature=feat[:d], bin=b[:d], fthr=ft[:d], gain=gn[:d],
+                    leaves=lv[:(1 << d) * K].reshape(1 << d, K), leaf_idx=li)
+
     def train(self, y, params: TrainParams):
-        m, a = _new_model(params.n_trees, params.tree.max_depth, self.n)
+        m, a = _new_model(params.n_trees, params.tree.max_depth, self.n, channels(params))
         y = np.ascontiguousarray(y, np.float32)
         check(self.ctx.L.tc_train(self.h, _p(y), byref(params), byref(m)))
         a["base"] = float(m.base)
@@ -355,7 +393,7 @@ class Trainer:
 
     def model(self, job=0):
         p = self.params[job]
-        m, a = _new_model(p.n_trees, p.tree.max_depth, self.datasets[job].n)
+        m, a = _new_model(p.n_trees, p.tree.max_depth, self.datasets[job].n, channels(p))
         check(self.ctx.L.tc_trainer_model(self.h, job, byref(m)))
         a["base"] = float(m.base)
         a["weight"] = np.full(p.n_trees, p.learning_rate, np.float32)
@@ -377,14 +415,17 @@ class Ensemble:
     def __init__(self, ctx: Context, model: dict, n_features: int):
         self.ctx = ctx
         self.T, self.D, self.F = model["T"], model["D"], n_features
+        self.K = model.get("K", 1)
         self.h = c_void_p()
         w = model.get("weight")
         if w is None:
             w = np.ones(self.T, np.float32)
-        check(ctx.L.tc_ensemble_create(ctx.h, self.T, self.D, _p(model["feature"]),
-                                       _p(model["fthr"]), _p(model.get("bin")), _p(model["depth"]),
-                                       _p(model["leaves"]), _p(np.ascontiguousarray(w, np.float32)),
-                                       float(model["base"]), n_features, byref(self.h)))
+        check(ctx.L.tc_ensemble_create_multi(ctx.h, self.T, self.D, self.K, _p(model["feature"]),
+                                             _p(model["fthr"]), _p(model.get("bin")),
+                                             _p(model["depth"]),
+                                             _p(np.ascontiguousarray(model["leaves"], np.float32)),
+                                             _p(np.ascontiguousarray(w, np.float32)),
+                                             float(model["base"]), n_features, byref(self.h)))
 
     def close(self):
         if self.h:
@@ -400,7 +441,7 @@ class Ensemble:
     def predict(self, X, transform=0, want_leaf_idx=False):
         X = np.ascontiguousarray(X, np.float32)
         n = X.shape[0]
-        out = np.zeros(n, np.float32)
+        out = np.zeros(n, np.float32) if self.K == 1 else np.zeros((n, self.K), np.float32)
         li = np.zeros((n, self.T), np.uint32) if want_leaf_idx else None
         check(self.ctx.L.tc_predict(self.h, _p(X), n, self.F, transform, _p(out), _p(li)))
         return (out, li) if want_leaf_idx else out
@@ -409,7 +450,7 @@ class Ensemble:
         check(self.ctx.L.tc_predict_dev(self.h, x_dev, n, self.F, transform, out_dev))
 
     def predict_binned(self, ds: Dataset):
-        out = np.zeros(ds.n, np.float32)
+        out = np.zeros(ds.n, np.float32) if self.K == 1 else np.zeros((ds.n, self.K), np.float32)
         check(self.ctx.L.tc_predict_binned(self.h, ds.h, _p(out)))
         return out
 
