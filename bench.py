@@ -400,6 +400,108 @@ def run_batched(args):
     print(json.dumps(res), flush=True)
 
 
+def multiclass_workload(n, F, K, seed, sigma=0.2):
+    """X ~ N(0,1); W K x F ~ N(0, sigma); label = argmax(W x + Gumbel(0,1))."""
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((n, F), dtype=np.float32)
+    W = rng.normal(0.0, sigma, (K, F)).astype(np.float32)
+    y = np.empty(n, np.float32)
+    for s0 in range(0, n, 1 << 18):  # bounded temporaries
+        z = X[s0:s0 + (1 << 18)] @ W.T + rng.gumbel(0.0, 1.0, (min(1 << 18, n - s0), K))
+        y[s0:s0 + len(z)] = np.argmax(z, axis=1)
+    return X, y
+
+
+def run_multiclass(args):
+    """K-channel symmetric multiclass: training trees/s and histogram GB/s
+    (algorithmic bytes rows x (F + 8K)), batched inference over every row
+    with a full-size ensemble of random weights, and the FP64 K-channel CPU
+    restatement timed on a bounded row sample."""
+    from turbocat import capi
+    n, F, K, D = args.mc_rows, args.mc_features, args.classes, 8
+    log(f"[bench] generating {n} x {F} multiclass workload, {K} classes")
+    X, y = multiclass_workload(n, F, K, seed=1004)
+    ctx = capi.Context(0)
+    t0 = time.perf_counter()
+    ds, edges, ne = ctx.quantize_fit(X)
+    quant_s = time.perf_counter() - t0
+    log(f"[bench] quantize_fit {quant_s:.2f} s")
+    R, T_timed = 2, args.steps
+    prm = capi.train_params(capi.TC_LOSS_SOFTMAX, n_trees=args.warmup + T_timed + R,
+                            learning_rate=0.05, max_depth=D, lambda_l2=1.0, n_classes=K)
+    tr = capi.Trainer(ctx, [ds], [y], prm)
+    tr.step(args.warmup)
+    tr.sync()
+    t0 = time.perf_counter()
+    tr.step(T_timed)
+    tr.sync()
+    dt = time.perf_counter() - t0
+    rows0 = tr.hist_rows()
+    hist_ms, launches = tr.step_timed(R)
+    rows1 = tr.hist_rows()
+    alg = (rows1 - rows0) * (F + 8 * K)
+    achieved = alg / (hist_ms * 1e-3) / 1e9
+    log(f"[bench] {T_timed / dt:.2f} trees/s ({1e3 * dt / T_timed:.1f} ms/tree); histogram "
+        f"{hist_ms / launches * 1e3:.0f} us avg per launch -> {achieved:.0f} GB/s")
+    tr.close()
+    # inference: a 1000-tree depth-8 ensemble of this shape, random weights
+    rng = np.random.default_rng(7)
+    Tm = 1000
+    model = dict(T=Tm, D=D, K=K, feature=rng.integers(0, F, Tm * D).astype(np.uint16),
+                 fthr=rng.standard_normal(Tm * D).astype(np.float32),
+                 depth=np.full(Tm, D, np.uint16),
+                 leaves=(rng.standard_normal((Tm << D) * K) * 0.05).astype(np.float32),
+                 weight=np.full(Tm, 0.05, np.float32), base=0.0)
+    ens = ctx.ensemble(model, F)
+    import ctypes
+    dx, do = ctypes.c_void_p(), ctypes.c_void_p()
+    capi.check(ctx.L.tc_device_alloc(ctx.h, X.nbytes, ctypes.byref(dx)))
+    capi.check(ctx.L.tc_device_alloc(ctx.h, 4 * n * K, ctypes.byref(do)))
+    capi.check(ctx.L.tc_memcpy_h2d(ctx.h, dx, X.ctypes.data_as(ctypes.c_void_p), X.nbytes))
+    ens.predict_dev(dx, n, do, transform=1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ens.predict_dev(dx, n, do, transform=1)
+    ctx.sync()
+    pred_s = time.perf_counter() - t0
+    capi.check(ctx.L.tc_device_free(ctx.h, dx))
+    capi.check(ctx.L.tc_device_free(ctx.h, do))
+    log(f"[bench] inference {n / pred_s / 1e6:.1f} M rows/s ({Tm} trees depth {D}, {K} classes)")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import Oracle, tree_params
+        o = Oracle()
+        ns = min(n, args.mc_cpu_rows)
+        bins = ds.bins()[:, :ns].copy() if ns < n else ds.bins()
+        g, h = o.gradients_mc_f64(y[:ns], np.zeros((ns, K), np.float32))
+        t0 = time.perf_counter()
+        o.tree_mc_f64(bins, g, h, edges, ne, tree_params(D, lambda_l2=1.0))
+        sec = time.perf_counter() - t0
+        cpu = dict(value=1.0 / sec, unit="trees/s", cores=1, kind="port", host_cpu=host_cpu_model(),
+                   sample=(f"FP64 K-channel restatement (oracle/, single thread) building one "
+                           f"depth-{D} tree on the first {ns} rows: {sec:.2f} s; for all {n} rows "
+                           f"expect about {n / ns:.0f}x longer (extrapolated, not measured)"))
+        log(f"[bench] CPU K-channel restatement {sec:.2f} s per tree on {ns} rows")
+    out = {
+        "metric": METRIC, "value": round(T_timed / dt, 3), "unit": "trees/s", "n_gpus": 1,
+        "steps": T_timed, "warmup": args.warmup, "ms_per_step": round(1e3 * dt / T_timed, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"multiclass softmax {n} rows x {F} float32 N(0,1) features, "
+                               f"{K} classes (argmax(Wx + Gumbel)), K-channel symmetric trees "
+                               f"depth {D}, lr 0.05, lambda 1", "rows": n, "features": F,
+                   "classes": K, "depth": D},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "kernel": "k_hist", "avg_launch_us": round(hist_ms / launches * 1e3, 2),
+                     "algorithmic_bytes_per_launch": int(alg / launches)},
+        "cpu_baseline": cpu,
+        "extra": {"quantize_s": round(quant_s, 3), "inference_rows_per_s": round(n / pred_s, 1),
+                  "inference_trees": Tm},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_copy(args):
     """Calibration for HBM byte counters: a plain device copy of a known
     size (256 MiB read, 256 MiB written per copy)."""
@@ -451,7 +553,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="train", choices=["train", "serve", "batched", "copy"])
+    ap.add_argument("--workload", default="train",
+                    choices=["train", "serve", "batched", "multiclass", "copy"])
     ap.add_argument("--pmc-summary", metavar="DIR", default=None,
                     help="print per-dispatch averages of the rocprofv3 counter CSVs under DIR")
     ap.add_argument("--kernel", default="k_hist", help="kernel name regex for --pmc-summary")
@@ -469,6 +572,10 @@ def main():
     ap.add_argument("--max-batch-rows", type=int, default=32768)
     ap.add_argument("--streams", type=int, default=4)
     ap.add_argument("--jobs", type=int, default=64)
+    ap.add_argument("--mc-rows", type=int, default=4_000_000)
+    ap.add_argument("--mc-features", type=int, default=256)
+    ap.add_argument("--classes", type=int, default=10)
+    ap.add_argument("--mc-cpu-rows", type=int, default=200_000)
     ap.add_argument("--max-job-rows", type=int, default=2_000_000)
     args = ap.parse_args()
     if args.pmc_summary:
@@ -479,7 +586,7 @@ def main():
                                    "across GPUs is designed but not implemented)"}))
         sys.exit(2)
     {"train": run_train, "serve": run_serve, "batched": run_batched,
-     "copy": run_copy}[args.workload](args)
+     "multiclass": run_multiclass, "copy": run_copy}[args.workload](args)
 
 
 if __name__ == "__main__":
