@@ -20,7 +20,7 @@ namespace tc {
 namespace {
 
 constexpr int kHistUnroll = 4;
-constexpr uint32_t kMaxHistChunk = 16384;
+constexpr uint32_t kMaxHistChunk = 32768;
 
 uint32_t ceil_div(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
 uint32_t round8(uint32_t x) { return std::max(8u, (x + 7u) & ~7u); }
@@ -433,13 +433,14 @@ class Trainer {
       J.base = S.base;
       uint32_t ch = S.p.hist_chunk;
       if (ch == 0) {
-        // ~512 root work items (two per CU): every work item ends with an
+        // ~1024 root work items (four per CU): every work item ends with an
         // int64-atomic flush of its 16 x 256 cells, which costs about as much
-        // as accumulating a few thousand rows, so items are made as long as
-        // the root's parallelism allows (1M rows x 7 tiles -> 16384 rows)
-        const uint64_t work = static_cast<uint64_t>(N) * J.T;
+        // as accumulating a few thousand rows, so items are as long as the
+        // root's parallelism allows (1M rows x 7 tiles -> 8192 rows; measured
+        // 1,918 trees/s against 1,756 at 16384 and 1,771 at 4096)
+        const uint64_t work = static_cast<uint64_t>(N) * J.T * K;
         ch = 1024;
-        while (ch < kMaxHistChunk && work / ch > 512) ch <<= 1;
+        while (ch < 16384 && work / ch > 1024) ch <<= 1;
       }
       ch = std::min(ch, kMaxHistChunk);  // packed cells count < 2^16 rows per item
       J.hist_chunk = ch;

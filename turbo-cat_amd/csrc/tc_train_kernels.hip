@@ -427,9 +427,12 @@ __global__ void __launch_bounds__(256) k_tree_begin(const JobDev* __restrict__ j
 // node, one gradient channel) x one 16-feature tile.  Each lane loads the tile's 16 bins of a row
 // with one 16-byte load from the row-major bins and the row's (g, h), turns
 // them into fixed point once, and adds them into the workgroup's LDS
-// histogram with 64-bit integer atomics (feature stride 257 cells).  Lane l
-// starts at feature l % 16, so the 64 lanes of one atomic instruction spread
-// over the 16 feature histograms.
+// histogram with 64-bit integer atomics.  The LDS cells are bin-major,
+// feature-minor (cell b * 16 + f), so a cell's bank is set by its feature
+// alone; lane l starts at feature l % 16, so every 16 lanes of one atomic
+// instruction hit 16 different banks whatever their bins (the feature-major
+// layout put the bank at (f + b) mod 16 and measured 63% of LDS cycles in
+// bank-conflict stalls).
 //
 // PACKED (MSE, h == 2 for every row): one atomic per value on a cell holding
 // g * 2^16 + 1 -- a signed 48-bit g sum above a 16-bit row count (a work
@@ -443,7 +446,7 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
                                                        const HistWork* __restrict__ work,
                                                        const uint32_t* __restrict__ count,
                                                        uint32_t level) {
-  constexpr int kCells = kTileF * 257;
+  constexpr int kCells = kTileF * 256;
   __shared__ unsigned long long s_a[kCells];
   __shared__ unsigned long long s_b[PACKED ? 1 : kCells];
   uint32_t li;
@@ -507,8 +510,8 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
         const uint32_t f = (static_cast<uint32_t>(k) + s) & 15u;
         const uint32_t b = (wr[k >> 2] >> (8 * (k & 3))) & 0xFFu;
         if (f < nf) {  // padding features of the last tile cost no atomics
-          atomicAdd(&s_a[f * 257 + b], va);
-          if (!PACKED) atomicAdd(&s_b[f * 257 + b], vb);
+          atomicAdd(&s_a[b * 16 + f], va);
+          if (!PACKED) atomicAdd(&s_b[b * 16 + f], vb);
         }
       }
     }
@@ -517,20 +520,22 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
   HistCell* out =
       J.hist + ((static_cast<size_t>(heap_index(level, w.node)) * J.K + w.cls) * J.F + f0) * 256u;
   const long long h2 = PACKED ? __double2ll_rn(2.0 * hsc) : 0;
-  for (uint32_t i = tid; i < nf * 256u; i += kHistThreads) {
-    const uint32_t f = i >> 8, b = i & 255u;
-    const unsigned long long a = s_a[f * 257 + b];
+  for (uint32_t i = tid; i < static_cast<uint32_t>(kCells); i += kHistThreads) {
+    const uint32_t f = i & 15u, b = i >> 4;  // LDS order: conflict-free reads
+    if (f >= nf) continue;
+    const unsigned long long a = s_a[i];
     long long gs, hs;
     if (PACKED) {
       gs = static_cast<long long>(a) >> 16;
       hs = static_cast<long long>(a & 0xFFFFull) * h2;
     } else {
       gs = static_cast<long long>(a);
-      hs = static_cast<long long>(s_b[f * 257 + b]);
+      hs = static_cast<long long>(s_b[i]);
     }
     if (gs != 0 || hs != 0) {
-      atomicAdd(reinterpret_cast<unsigned long long*>(&out[i].g), static_cast<unsigned long long>(gs));
-      atomicAdd(reinterpret_cast<unsigned long long*>(&out[i].h), static_cast<unsigned long long>(hs));
+      HistCell* o = out + f * 256u + b;
+      atomicAdd(reinterpret_cast<unsigned long long*>(&o->g), static_cast<unsigned long long>(gs));
+      atomicAdd(reinterpret_cast<unsigned long long*>(&o->h), static_cast<unsigned long long>(hs));
     }
   }
 }
