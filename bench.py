@@ -168,7 +168,9 @@ def run_train(args):
     tr = capi.Trainer(ctx, [ds], [y], prm)
     tr.step(args.warmup)
     tr.sync()
-    log(f"[bench] warmup {args.warmup} trees done; timing {T_timed}")
+    root, deep = tr.hist_chunks()
+    log(f"[bench] warmup {args.warmup} trees done (histogram work items: {root} rows at the "
+        f"root, {deep} below); timing {T_timed}")
     t0 = time.perf_counter()
     tr.step(T_timed)
     tr.sync()

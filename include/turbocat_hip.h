@@ -77,7 +77,7 @@ typedef struct {
   uint32_t n_trees;
   float learning_rate;
   tc_tree_params tree;
-  uint32_t hist_chunk;       /* rows per histogram work item; 0 = autotuned */
+  uint32_t hist_chunk;       /* rows per histogram work item; 0 = autotuned (tc_trainer_hist_chunks) */
   int32_t use_graph;         /* 1: replay one captured hipGraph per tree */
   float subsample;           /* in (0, 1): row sample per tree; 0 or >= 1: every row */
   float colsample_bytree;    /* in (0, 1): consumes an RNG draw per tree */
@@ -212,6 +212,13 @@ int32_t tc_trainer_model(tc_trainer* tr, uint32_t job, tc_model* model);
 /* Rows fed to histogram kernels for job j since creation (for algorithmic
  * bytes: rows x (F + 8)). */
 int32_t tc_trainer_hist_rows(tc_trainer* tr, uint32_t job, uint64_t* rows);
+/* Rows per histogram work item of job j, at the root and at deeper levels.
+ * With hist_chunk = 0 they are picked by timing candidate shapes on the
+ * fit's first trees before the per-tree graph is captured (every histogram
+ * is an exact integer sum, so the shape never changes the model); otherwise
+ * both equal hist_chunk.  No reference counterpart. */
+int32_t tc_trainer_hist_chunks(tc_trainer* tr, uint32_t job, uint32_t* root_rows,
+                               uint32_t* deep_rows);
 int32_t tc_trainer_destroy(tc_trainer* tr);
 
 /* ------------------------------------------------------------- inference */

@@ -217,3 +217,31 @@ def test_fit_sampled_teacher_forced(ctx, oracle, loss, subsample, colsample):
         preds = fma_update(oracle, preds, np.float32(0.1), gpu["leaves"][li_all])
     assert ties <= 1
     assert np.array_equal(m["train_preds"], preds)
+
+
+@pytest.mark.parametrize("loss", [0, 1])
+def test_work_item_shape_never_changes_the_model(ctx, loss):
+    """Histograms are exact integer sums with a fixed-point scale that does not
+    depend on the work-item size, so an autotuned fit (shapes timed on its own
+    first trees) and fits with fixed work-item sizes give identical models."""
+    from turbocat import capi
+    X, y = regression_data(60000, 12, 41) if loss == 0 else binary_data(60000, 12, 41)
+    ds, _, _ = ctx.quantize_fit(X)
+    lam = 0.0 if loss == 0 else 1.0
+    models = []
+    for hc in (0, 1024, 4096, 32768):
+        prm = capi.train_params(loss, n_trees=16, learning_rate=0.1, max_depth=6, lambda_l2=lam,
+                                hist_chunk=hc)
+        tr = capi.Trainer(ctx, [ds], [y], prm)
+        tr.step(16)
+        tr.sync()
+        root, deep = tr.hist_chunks()
+        if hc:
+            assert root == deep == hc
+        else:
+            assert 1024 <= deep <= root <= 16384
+        models.append(tr.model())
+        tr.close()
+    for m in models[1:]:
+        for k in ("feature", "bin", "fthr", "gain", "depth", "leaves", "train_preds"):
+            assert np.array_equal(m[k], models[0][k]), k

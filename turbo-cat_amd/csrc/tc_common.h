@@ -27,6 +27,7 @@ constexpr int kTileF = 16;             // features per histogram tile (16 B)
 constexpr int kHistThreads = 512;
 constexpr int kPartChunk = 4096;       // rows per partition work item
 constexpr int kPartThreads = 1024;
+constexpr uint32_t kMaxHistChunk = 32768;  // rows per histogram work item (packed count < 2^16)
 constexpr int kScanNodesPerPass = 32;  // node prefixes staged in LDS at a time
 constexpr uint32_t kEmptyKey = 0u;
 
@@ -84,7 +85,8 @@ struct JobDev {
   uint32_t D, max_bins, min_samples_leaf;
   float lr, lambda_l2, lambda_l1, max_delta_step, min_child_weight, min_split_gain;
   float base;
-  uint32_t hist_chunk;      // rows per histogram work item
+  uint32_t hist_chunk;      // rows per histogram work item at the root
+  uint32_t hist_chunk_deep; // rows per histogram work item at levels >= 1
   // training state
   const float* y;           // [N] (class index for softmax)
   float* preds;             // [K][N]

@@ -20,7 +20,6 @@ namespace tc {
 namespace {
 
 constexpr int kHistUnroll = 4;
-constexpr uint32_t kMaxHistChunk = 32768;
 
 uint32_t ceil_div(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
 uint32_t round8(uint32_t x) { return std::max(8u, (x + 7u) & ~7u); }
@@ -240,7 +239,9 @@ class Trainer {
   // Enqueue n more trees as graph replays (captured on first use).
   void step_graph(uint32_t n) {
     hipStream_t s = ctx_->stream;
+    uint32_t built = 0;
     if (!graph_exec_) {
+      built = autotune(n);
       hipGraph_t g;
       TC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       enqueue_tree(nullptr);
@@ -250,7 +251,7 @@ class Trainer {
       TC_HIP(e);
     }
     const bool smp = sampling();
-    for (uint32_t t = 0; t < n; ++t) {
+    for (uint32_t t = built; t < n; ++t) {
       if (smp) prepare_samples();
       TC_HIP(hipGraphLaunch(graph_exec_, s));
     }
@@ -284,6 +285,8 @@ class Trainer {
     TC_HIP(hipEventCreate(&b));
     std::vector<std::pair<hipEvent_t, hipEvent_t>> hev;
     if (use_graph && !time_hist) {
+      TC_HIP(hipEventRecord(a, s));
+      const uint32_t built = autotune(n_trees_);
       hipGraph_t g;
       TC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       enqueue_tree(nullptr);
@@ -291,9 +294,8 @@ class Trainer {
       hipGraphExec_t ge;
       TC_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
       TC_HIP(hipGraphDestroy(g));
-      TC_HIP(hipEventRecord(a, s));
       const bool smp = sampling();
-      for (uint32_t t = 0; t < n_trees_; ++t) {
+      for (uint32_t t = built; t < n_trees_; ++t) {
         if (smp) prepare_samples();
         TC_HIP(hipGraphLaunch(ge, s));
       }
@@ -324,6 +326,100 @@ class Trainer {
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     TC_HIP(hipGetLastError());
+  }
+
+  // ---- histogram work-item shapes and their autotuner.
+  // Candidate shapes: root items per job (each of N x tiles x channels rows
+  // of work), and the deeper levels' item size as a fraction of the root's.
+  // The rows of a work item never change a result (every histogram is an
+  // exact integer sum and the fixed-point scale does not depend on the item
+  // size), so the tuner times the fit's own first trees.
+  struct TuneChoice {
+    uint32_t root_items, deep_div;
+  };
+  static constexpr TuneChoice kTune[5] = {{1024, 1}, {1024, 2}, {512, 1}, {2048, 1}, {512, 2}};
+
+  static uint32_t chunk_for(uint64_t work, uint32_t items) {
+    uint32_t ch = 1024;
+    while (ch < 16384 && work / ch > items) ch <<= 1;
+    return ch;
+  }
+
+  static uint32_t hist_items(uint32_t N, uint32_t d, uint32_t root, uint32_t deep) {
+    return d == 0 ? ceil_div(N, root) : ceil_div(N / 2, deep) + (1u << (d - 1));
+  }
+
+  void set_chunks(uint32_t j, const TuneChoice& c) {
+    JobDev& J = h_jobs_[j];
+    if (fixed_chunk_[j]) {
+      J.hist_chunk = J.hist_chunk_deep = fixed_chunk_[j];
+      return;
+    }
+    J.hist_chunk = chunk_for(work_[j], c.root_items);
+    J.hist_chunk_deep = std::max(1024u, J.hist_chunk / c.deep_div);
+  }
+
+  // Launch grids of k_hist for the current shapes (<= the allocated lists).
+  void recompute_caps() {
+    for (uint32_t d = 0; d < D_; ++d) {
+      uint32_t c = 0;
+      for (uint32_t j = 0; j < njobs_; ++j) {
+        const JobDev& J = h_jobs_[j];
+        c += hist_items(J.N, d, J.hist_chunk, J.hist_chunk_deep) * J.T * J.K;
+      }
+      hist_cap_[d] = c;
+    }
+  }
+
+  void choose(uint32_t c) {
+    for (uint32_t j = 0; j < njobs_; ++j) set_chunks(j, kTune[c]);
+    recompute_caps();
+    TC_HIP(hipMemcpyAsync(d_jobs_, h_jobs_.data(), sizeof(JobDev) * njobs_, hipMemcpyHostToDevice,
+                          ctx_->stream));
+    choice_ = c;
+  }
+
+  // Times one tree per candidate shape (two each when the budget allows),
+  // after one warm-up tree, and keeps the fastest.  Returns the trees built.
+  uint32_t autotune(uint32_t budget) {
+    constexpr uint32_t kN = sizeof(kTune) / sizeof(kTune[0]);
+    if (!tune_ || tuned_) return 0;
+    const uint32_t reps = budget >= 2 * kN + 1 ? 2u : 1u;
+    if (budget < kN * reps + 1) return 0;  // too few trees: keep the default shape
+    hipStream_t s = ctx_->stream;
+    std::vector<hipEvent_t> ev(2 * kN * reps);
+    for (auto& e : ev) TC_HIP(hipEventCreate(&e));
+    if (sampling()) prepare_samples();
+    enqueue_tree(nullptr);
+    for (uint32_t r = 0, k = 0; r < reps; ++r)
+      for (uint32_t c = 0; c < kN; ++c, k += 2) {
+        choose(c);
+        if (sampling()) prepare_samples();
+        TC_HIP(hipEventRecord(ev[k], s));
+        enqueue_tree(nullptr);
+        TC_HIP(hipEventRecord(ev[k + 1], s));
+      }
+    TC_HIP(hipStreamSynchronize(s));
+    uint32_t best = 0;
+    for (uint32_t c = 0; c < kN; ++c) {
+      float t = 0.f;
+      for (uint32_t r = 0; r < reps; ++r) {
+        float x = 0.f;
+        TC_HIP(hipEventElapsedTime(&x, ev[2 * (r * kN + c)], ev[2 * (r * kN + c) + 1]));
+        t += x;
+      }
+      tune_ms_[c] = t / reps;
+      if (tune_ms_[c] < tune_ms_[best]) best = c;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    choose(best);
+    tuned_ = true;
+    return 1 + kN * reps;
+  }
+
+  void hist_chunks(uint32_t j, uint32_t* root, uint32_t* deep) const {
+    *root = h_jobs_[j].hist_chunk;
+    *deep = h_jobs_[j].hist_chunk_deep;
   }
 
   // Device-side consistency flags (TreeState::fault) of every job.
@@ -395,6 +491,9 @@ class Trainer {
   void setup() {
     hipStream_t s = ctx_->stream;
     h_jobs_.resize(njobs_);
+    work_.assign(njobs_, 0);
+    fixed_chunk_.assign(njobs_, 0);
+    std::fill(list_cap_, list_cap_ + kMaxDepth, 0u);
     hist_rows_.resize(njobs_);
     samplers_.resize(njobs_);
     std::vector<uint32_t> wg_off(njobs_ + 1, 0);
@@ -431,19 +530,10 @@ class Trainer {
       J.max_delta_step = S.p.tree.max_delta_step; J.min_child_weight = S.p.tree.min_child_weight;
       J.min_split_gain = S.p.tree.min_split_gain;
       J.base = S.base;
-      uint32_t ch = S.p.hist_chunk;
-      if (ch == 0) {
-        // ~1024 root work items (four per CU): every work item ends with an
-        // int64-atomic flush of its 16 x 256 cells, which costs about as much
-        // as accumulating a few thousand rows, so items are as long as the
-        // root's parallelism allows (1M rows x 7 tiles -> 8192 rows; measured
-        // 1,918 trees/s against 1,756 at 16384 and 1,771 at 4096)
-        const uint64_t work = static_cast<uint64_t>(N) * J.T * K;
-        ch = 1024;
-        while (ch < 16384 && work / ch > 1024) ch <<= 1;
-      }
-      ch = std::min(ch, kMaxHistChunk);  // packed cells count < 2^16 rows per item
-      J.hist_chunk = ch;
+      work_[j] = static_cast<uint64_t>(N) * J.T * K;
+      fixed_chunk_[j] = S.p.hist_chunk ? std::min(S.p.hist_chunk, kMaxHistChunk) : 0u;
+      if (!fixed_chunk_[j]) tune_ = true;
+      set_chunks(j, kTune[0]);
       float* y = alloc<float>(N);
       if (S.y_host) TC_HIP(hipMemcpyAsync(y, S.y_host, 4ull * N, hipMemcpyHostToDevice, s));
       else TC_HIP(hipMemsetAsync(y, 0, 4ull * N, s));
@@ -482,15 +572,23 @@ class Trainer {
       hist_rows_[j] = alloc<unsigned long long>(1);
       TC_HIP(hipMemsetAsync(hist_rows_[j], 0, 8, s));
       J.hist_rows_total = hist_rows_[j];
-      // work-list capacities (see emit_hist / k_part_plan)
+      // work-list capacities: room for the smallest work items any choice
+      // uses (tree_begin / k_part_plan emit at most this many)
+      uint32_t min_root = kMaxHistChunk, min_deep = kMaxHistChunk;
+      for (const TuneChoice& c : kTune) {
+        set_chunks(j, c);
+        min_root = std::min(min_root, J.hist_chunk);
+        min_deep = std::min(min_deep, J.hist_chunk_deep);
+      }
+      set_chunks(j, kTune[0]);
       for (uint32_t d = 0; d < D_; ++d) {
-        const uint32_t hc = d == 0 ? ceil_div(N, ch) : ceil_div(N / 2, ch) + (1u << (d - 1));
-        hist_cap_[d] += hc * J.T * K;
+        list_cap_[d] += hist_items(N, d, min_root, min_deep) * J.T * K;
         part_cap_[d] += ceil_div(N, kPartChunk) + (1u << d);
       }
     }
+    recompute_caps();
     for (uint32_t d = 0; d < D_; ++d) {
-      lists_.hist[d] = alloc<HistWork>(std::max(1u, hist_cap_[d]));
+      lists_.hist[d] = alloc<HistWork>(std::max(1u, list_cap_[d]));
       lists_.part[d] = alloc<PartWork>(std::max(1u, part_cap_[d]));
       part_cap_[d] = std::max(1u, part_cap_[d]);
     }
@@ -551,6 +649,13 @@ class Trainer {
   std::vector<Sampler> samplers_;
   bool forced_ = false;
   bool keep_rows_ = false;  // partition the last level too (leaf_index)
+  // histogram work-item shapes (rows per item at the root and below)
+  std::vector<uint64_t> work_;          // per job: rows x tiles x channels
+  std::vector<uint32_t> fixed_chunk_;   // per job: caller's hist_chunk, 0 = tuned
+  uint32_t list_cap_[kMaxDepth] = {};   // allocated work-list entries per level
+  bool tune_ = false, tuned_ = false;
+  uint32_t choice_ = 0;
+  float tune_ms_[8] = {};
 
   // Dynamic LDS of k_grad_update: the leaf values of one tree, [2^D][K].
   size_t leaf_lds() const { return 4ull * (static_cast<size_t>(kmax_) << D_); }
@@ -709,6 +814,15 @@ int32_t tc_trainer_hist_rows(tc_trainer* t, uint32_t job, uint64_t* rows) {
     TC_REQUIRE(t && rows, "tc_trainer_hist_rows: NULL argument");
     TC_REQUIRE(job < t->tr->njobs(), "job index out of range");
     *rows = t->tr->hist_rows(job);
+  });
+}
+
+int32_t tc_trainer_hist_chunks(tc_trainer* t, uint32_t job, uint32_t* root_rows,
+                               uint32_t* deep_rows) {
+  return guard([&] {
+    TC_REQUIRE(t && root_rows && deep_rows, "tc_trainer_hist_chunks: NULL argument");
+    TC_REQUIRE(job < t->tr->njobs(), "job index out of range");
+    t->tr->hist_chunks(job, root_rows, deep_rows);
   });
 }
 

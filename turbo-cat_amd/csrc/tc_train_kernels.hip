@@ -338,8 +338,10 @@ __device__ void tree_begin(const JobDev& J, uint32_t job, const Lists& L, int pa
       J.off[1] = n;
       go = (n >= 2u * J.min_samples_leaf && Hsum >= J.min_child_weight * 2.0f) ? 1 : 0;
       if (n == 0) go = 0;  // empty sample: depth-0 tree, leaf 0 (:39-43)
-      const int eg = fixed_exponent(gmax, n, J.hist_chunk, packed != 0);
-      const int eh = fixed_exponent(hmax, n, J.hist_chunk, false);
+      // the packed bound uses the largest work item any chunking may use, so
+      // the scales -- and every histogram -- do not depend on the chunking
+      const int eg = fixed_exponent(gmax, n, kMaxHistChunk, packed != 0);
+      const int eh = fixed_exponent(hmax, n, kMaxHistChunk, false);
       st->gscale = ldexp(1.0, eg);
       st->ginv = ldexp(1.0, -eg);
       st->hscale = ldexp(1.0, eh);
@@ -912,7 +914,7 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
     // next level: histogram of the smaller child of the pair (ties: left),
     // partition slots of both children
     const uint32_t sm_len = min(acc, len - acc);
-    s_hw[n] = ((sm_len + J.hist_chunk - 1) / J.hist_chunk) * J.T * J.K;
+    s_hw[n] = ((sm_len + J.hist_chunk_deep - 1) / J.hist_chunk_deep) * J.T * J.K;
     s_pw[2 * n] = part_slots(acc);
     s_pw[2 * n + 1] = part_slots(len - acc);
     if (sm_len) atomicAdd(&s_rows, static_cast<unsigned long long>(sm_len));
@@ -941,8 +943,8 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
     const uint32_t nl = s_off[lc + 1] - s_off[lc], nr = s_off[rc + 1] - s_off[rc];
     const uint32_t sm = nl <= nr ? lc : rc;
     const uint32_t le = e - s_hw[n], TK = J.T * J.K, r = le % TK;
-    const uint32_t start = s_off[sm] + (le / TK) * J.hist_chunk;
-    put_hist_work(hout + e, job, sm, start, min(J.hist_chunk, s_off[sm + 1] - start), r / J.K,
+    const uint32_t start = s_off[sm] + (le / TK) * J.hist_chunk_deep;
+    put_hist_work(hout + e, job, sm, start, min(J.hist_chunk_deep, s_off[sm + 1] - start), r / J.K,
                   r % J.K);
   }
   PartWork* pout = L.part[level + 1] + s_pbase;

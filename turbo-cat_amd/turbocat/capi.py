@@ -32,7 +32,8 @@ EXPORTS = [
     "tc_hist_subtract", "tc_split_scan", "tc_tree_build", "tc_tree_build_multi", "tc_train",
     "tc_train_batch",
     "tc_last_train_timing", "tc_trainer_create", "tc_trainer_step", "tc_trainer_step_timed",
-    "tc_trainer_sync", "tc_trainer_model", "tc_trainer_hist_rows", "tc_trainer_destroy",
+    "tc_trainer_sync", "tc_trainer_model", "tc_trainer_hist_rows", "tc_trainer_hist_chunks",
+    "tc_trainer_destroy",
     "tc_ensemble_create", "tc_ensemble_create_multi", "tc_ensemble_destroy", "tc_predict",
     "tc_predict_dev", "tc_predict_binned", "tc_serve_start", "tc_serve_submit",
     "tc_serve_wait", "tc_serve_stats", "tc_serve_stop",
@@ -137,6 +138,7 @@ def load(path: str = LIB_PATH):
         "tc_trainer_create": [V, U, V, V, V, V], "tc_trainer_step": [V, U],
         "tc_trainer_step_timed": [V, U, V, V], "tc_trainer_sync": [V],
         "tc_trainer_model": [V, U, POINTER(Model)], "tc_trainer_hist_rows": [V, U, V],
+        "tc_trainer_hist_chunks": [V, U, V, V],
         "tc_trainer_destroy": [V],
         "tc_ensemble_create": [V, U, U, V, V, V, V, V, V, c_float, U, V],
         "tc_ensemble_create_multi": [V, U, U, U, V, V, V, V, V, V, c_float, U, V],
@@ -385,6 +387,12 @@ class Trainer:
 
     def sync(self):
         check(self.ctx.L.tc_trainer_sync(self.h))
+
+    def hist_chunks(self, job=0):
+        """Rows per histogram work item (root, deeper levels) in use for job."""
+        r, d = c_uint32(), c_uint32()
+        check(self.ctx.L.tc_trainer_hist_chunks(self.h, job, byref(r), byref(d)))
+        return r.value, d.value
 
     def hist_rows(self, job=0):
         v = c_uint64()
