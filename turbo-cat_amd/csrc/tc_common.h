@@ -1,8 +1,9 @@
 // Shared device/host definitions for the TurboCat MI355X (gfx950) hot path.
 //
 // Data layout in HBM (one "job" = one boosting run over one dataset):
-//   bins_rm   uint8  [N][Fp]     row-major, Fp = F rounded up to 16 (one
-//                                 16-byte load = one 16-feature tile of a row)
+//   bins_rm   uint8  [N][Fp]     row-major (one 16-byte load = one 16-feature
+//                                 tile of a row); the row stride Fp keeps a
+//                                 row inside whole 128-byte lines (rm_stride)
 //   bins_cm   uint8  [F][N]      column-major (split column reads)
 //   gh        float2 [K][N]      gradient / hessian per channel and row (K = 1
 //                                 except K-class softmax; class-major)
@@ -76,7 +77,7 @@ struct JobDev {
   const uint8_t* bins_cm;
   const float* edges;       // [F][255]
   const uint32_t* n_edges;  // [F]
-  uint32_t N, F, Fp, T;     // T = Fp / 16
+  uint32_t N, F, Fp, T;     // Fp = rm_stride(F), T = feature_tiles(F)
   uint32_t n_root;          // rows of the tree's root list (N, or the row sample)
   uint32_t root_identity;   // 1: the root list is 0..N-1 (no sampling), rows[0] not read
   // hyper-parameters
@@ -116,6 +117,21 @@ struct JobDev {
   float* m_leaves;          // [n_trees][2^D][K]
   unsigned long long* hist_rows_total;  // rows fed to histogram kernels (all trees)
 };
+
+// 16-feature tiles of a row.
+__host__ __device__ inline uint32_t feature_tiles(uint32_t F) { return (F + kTileF - 1) / kTileF; }
+
+// Row stride of the row-major bins: the tiles, padded so that no row
+// straddles a 128-byte line (16, 32, 64, 128, then multiples of 128).  The
+// histogram gathers the rows of a node; a 112-byte row would touch two lines
+// about 7 times in 8.  Padding bytes are never written or read.
+__host__ __device__ inline uint32_t rm_stride(uint32_t F) {
+  const uint32_t t = feature_tiles(F) * kTileF;
+  if (t > 128) return (t + 127) / 128 * 128;
+  uint32_t p = kTileF;
+  while (p < t) p <<= 1;
+  return p;
+}
 
 __host__ __device__ inline uint32_t heap_index(uint32_t level, uint32_t node) {
   return (1u << level) - 1u + node;

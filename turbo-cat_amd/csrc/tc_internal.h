@@ -19,7 +19,7 @@ struct tc_ctx {
 struct tc_dataset {
   tc_ctx* ctx = nullptr;
   uint32_t N = 0, F = 0, Fp = 0, max_bins = 255;
-  uint8_t* bins_rm = nullptr;   // [N][Fp]
+  uint8_t* bins_rm = nullptr;   // [N][Fp], Fp = rm_stride(F)
   uint8_t* bins_cm = nullptr;   // [F][N]
   float* edges = nullptr;       // [F][255]
   uint32_t* n_edges = nullptr;  // [F]

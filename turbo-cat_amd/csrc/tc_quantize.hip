@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(256) k_bin(const float* __restrict__ X, uint32
 void bin_on_device(tc_dataset* ds, const float* X) {
   const uint32_t N = ds->N, F = ds->F;
   hipStream_t s = ds->ctx->stream;
-  ds->Fp = ((F + kTileF - 1) / kTileF) * kTileF;
+  ds->Fp = rm_stride(F);
   ds->bins_rm = dmalloc<uint8_t>(static_cast<size_t>(N) * ds->Fp);
   ds->bins_cm = dmalloc<uint8_t>(static_cast<size_t>(N) * F);
   ds->edges = dmalloc<float>(static_cast<size_t>(F) * 255);
@@ -124,7 +124,7 @@ void bin_on_device(tc_dataset* ds, const float* X) {
       TC_HIP(hipMemcpyAsync(xd, X + static_cast<size_t>(r0) * F, sizeof(float) * nr * F,
                             hipMemcpyHostToDevice, s));
       // rows r0.. of this block: offset the output pointers, keep full N stride
-      dim3 grid((nr + 255) / 256, ds->Fp / kTileF);
+      dim3 grid((nr + 255) / 256, feature_tiles(F));
       hipLaunchKernelGGL(k_bin, grid, dim3(256), 0, s, xd, nr, F, ds->Fp, ds->edges, ds->n_edges,
                          static_cast<uint8_t>(ds->max_bins), ds->bins_rm + static_cast<size_t>(r0) * ds->Fp,
                          ds->bins_cm + r0, N);

@@ -506,7 +506,7 @@ class Trainer {
       std::memset(&J, 0, sizeof(J));
       const uint32_t N = ds->N;
       J.bins_rm = ds->bins_rm; J.bins_cm = ds->bins_cm; J.edges = ds->edges; J.n_edges = ds->n_edges;
-      J.N = N; J.F = ds->F; J.Fp = ds->Fp; J.T = ds->Fp / kTileF;
+      J.N = N; J.F = ds->F; J.Fp = ds->Fp; J.T = feature_tiles(ds->F);
       Sampler& SM = samplers_[j];
       SM.state = S.p.seed;
       SM.ratio = S.p.subsample;
@@ -906,7 +906,7 @@ int32_t tc_hist_build(tc_dataset* ds, const float* g, const float* h, const uint
       JobDev J;
       std::memset(&J, 0, sizeof(J));
       J.bins_rm = ds->bins_rm; J.bins_cm = ds->bins_cm; J.N = N; J.F = F; J.Fp = ds->Fp;
-      J.T = ds->Fp / kTileF;
+      J.T = feature_tiles(F);
       J.K = 1;
       std::vector<float2> ghh(N);
       for (uint32_t i = 0; i < N; ++i) ghh[i] = make_float2(g[i], h[i]);
