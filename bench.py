@@ -168,9 +168,9 @@ def run_train(args):
     tr = capi.Trainer(ctx, [ds], [y], prm)
     tr.step(args.warmup)
     tr.sync()
-    root, deep = tr.hist_chunks()
+    root, deep_items = tr.hist_chunks()
     log(f"[bench] warmup {args.warmup} trees done (histogram work items: {root} rows at the "
-        f"root, {deep} below); timing {T_timed}")
+        f"root, ~{deep_items} items per deeper level); timing {T_timed}")
     t0 = time.perf_counter()
     tr.step(T_timed)
     tr.sync()

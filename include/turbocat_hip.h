@@ -212,13 +212,16 @@ int32_t tc_trainer_model(tc_trainer* tr, uint32_t job, tc_model* model);
 /* Rows fed to histogram kernels for job j since creation (for algorithmic
  * bytes: rows x (F + 8)). */
 int32_t tc_trainer_hist_rows(tc_trainer* tr, uint32_t job, uint64_t* rows);
-/* Rows per histogram work item of job j, at the root and at deeper levels.
- * With hist_chunk = 0 they are picked by timing candidate shapes on the
- * fit's first trees before the per-tree graph is captured (every histogram
- * is an exact integer sum, so the shape never changes the model); otherwise
- * both equal hist_chunk.  No reference counterpart. */
+/* Histogram work-item shape of job j: rows per work item at the root, and
+ * the number of work items each deeper level is cut into (the rows per item
+ * of a level then follow that level's rows; 0 = hist_chunk rows at every
+ * level).  With hist_chunk = 0 the shape is picked by timing candidate
+ * shapes on the fit's first trees before the per-tree graph is captured
+ * (every histogram is an exact integer sum, so the shape never changes the
+ * model); otherwise root_rows = hist_chunk and deep_items = 0.  No reference
+ * counterpart. */
 int32_t tc_trainer_hist_chunks(tc_trainer* tr, uint32_t job, uint32_t* root_rows,
-                               uint32_t* deep_rows);
+                               uint32_t* deep_items);
 int32_t tc_trainer_destroy(tc_trainer* tr);
 
 /* ------------------------------------------------------------- inference */

@@ -17,6 +17,10 @@
 //                                 node n at (2^d - 1 + n)); slot 255 = NaN bin
 //   ng/nh     float [2^(D+1)-1][K]  node totals per channel, heap order
 //   nc        uint32 [2^(D+1)-1]    node row counts
+//   partials  HistCell [item][16][256]  the level's work items but each
+//                                 node's last chunk: each item's tile
+//                                 histogram (plain stores), added to the
+//                                 node's histogram by the split scan
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -60,8 +64,9 @@ struct HistCell {
 
 // One unit of histogram work: rows [start, start+len) of rows[level] for
 // node `node` (level-local index), feature tile `tile`, channel `cls`.
+// `last`: the node's last chunk (its cells go into the node's histogram).
 struct HistWork {
-  uint32_t job, node, start, len, tile, cls, pad1, pad2;
+  uint32_t job, node, start, len, tile, cls, last, pad;
 };
 
 // One unit of partition work: rows [start, start+len) of rows[level],
@@ -87,7 +92,12 @@ struct JobDev {
   float lr, lambda_l2, lambda_l1, max_delta_step, min_child_weight, min_split_gain;
   float base;
   uint32_t hist_chunk;      // rows per histogram work item at the root
-  uint32_t hist_chunk_deep; // rows per histogram work item at levels >= 1
+  uint32_t deep_items;      // below the root: work items a level is cut into (about; the
+                            // rows per item follow the level's rows); 0: hist_chunk rows
+  uint32_t* lvl_chunk;      // [D] rows per histogram work item at each level of the tree
+  uint32_t store_hist;      // 1: each node's last chunk stores the node's histogram and the
+                            // other chunks store partial slots (no zeroing, no atomics);
+                            // 0: every chunk adds into the zeroed node histogram
   // training state
   const float* y;           // [N] (class index for softmax)
   float* preds;             // [K][N]
@@ -98,6 +108,7 @@ struct JobDev {
   float* nh;
   uint32_t* nc;             // heap [2^(D+1) - 1]
   HistCell* hist;           // heap [2^D - 1][K][F][256]
+  uint32_t* hfirst;         // heap [2^D - 1]: first work-list item of a histogrammed node
   TreeState* st;
   // partition cursors of the level being split: [node][2] = next free
   // position of the left child (counts up), end of the right child (counts
@@ -131,6 +142,15 @@ __host__ __device__ inline uint32_t rm_stride(uint32_t F) {
   uint32_t p = kTileF;
   while (p < t) p <<= 1;
   return p;
+}
+
+// Rows per histogram work item for `work` row-tiles cut into about `items`
+// work items: the smallest power of two in [1024, 16384] that does not give
+// more items.
+__host__ __device__ inline uint32_t chunk_for(uint64_t work, uint32_t items) {
+  uint32_t ch = 1024;
+  while (ch < 16384u && work / ch > items) ch <<= 1;
+  return ch;
 }
 
 __host__ __device__ inline uint32_t heap_index(uint32_t level, uint32_t node) {

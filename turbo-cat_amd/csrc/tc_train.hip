@@ -212,16 +212,16 @@ class Trainer {
       }
       if (packed)
         hipLaunchKernelGGL((k_hist<kHistUnroll, true>), dim3(round8(hist_cap_[d])), dim3(kHistThreads),
-                           0, s, d_jobs_, lists_.hist[d], lists_.hist_count[d], d);
+                           0, s, d_jobs_, lists_.hist[d], lists_.hist_count[d], d, partials_);
       else
         hipLaunchKernelGGL((k_hist<kHistUnroll, false>), dim3(round8(hist_cap_[d])), dim3(kHistThreads),
-                           0, s, d_jobs_, lists_.hist[d], lists_.hist_count[d], d);
+                           0, s, d_jobs_, lists_.hist[d], lists_.hist_count[d], d, partials_);
       if (hist_events) {
         TC_HIP(hipEventRecord(e1, s));
         hist_events->emplace_back(e0, e1);
       }
       hipLaunchKernelGGL(k_split_scan<true>, dim3(fmax_, njobs_), dim3(1024), 0, s, d_jobs_,
-                         static_cast<const float2*>(nullptr), d);
+                         static_cast<const float2*>(nullptr), d, partials_);
       // one wave per (node, channel) unit of the widest job
       hipLaunchKernelGGL(k_finalize, dim3(ceil_div(static_cast<uint64_t>(kmax_) << d, 4), njobs_),
                          dim3(256), 0, s, d_jobs_, d);
@@ -330,43 +330,47 @@ class Trainer {
 
   // ---- histogram work-item shapes and their autotuner.
   // Candidate shapes: root items per job (each of N x tiles x channels rows
-  // of work), and the deeper levels' item size as a fraction of the root's.
-  // The rows of a work item never change a result (every histogram is an
-  // exact integer sum and the fixed-point scale does not depend on the item
-  // size), so the tuner times the fit's own first trees.
+  // of work), and the items each deeper level is cut into (k_part_plan sizes
+  // that level's items from its rows).  The rows of a work item never change
+  // a result (every histogram is an exact integer sum and the fixed-point
+  // scale does not depend on the item size), so the tuner times the fit's
+  // own first trees.
   struct TuneChoice {
-    uint32_t root_items, deep_div;
+    uint32_t root_items, deep_items;
   };
-  static constexpr TuneChoice kTune[5] = {{1024, 1}, {1024, 2}, {512, 1}, {2048, 1}, {512, 2}};
+  static constexpr TuneChoice kTune[6] = {{1024, 512}, {512, 512},  {512, 1024},
+                                          {1024, 1024}, {512, 256}, {2048, 1024}};
 
-  static uint32_t chunk_for(uint64_t work, uint32_t items) {
-    uint32_t ch = 1024;
-    while (ch < 16384 && work / ch > items) ch <<= 1;
-    return ch;
-  }
-
-  static uint32_t hist_items(uint32_t N, uint32_t d, uint32_t root, uint32_t deep) {
-    return d == 0 ? ceil_div(N, root) : ceil_div(N / 2, deep) + (1u << (d - 1));
+  // Upper bound of job J's histogram work items at level d.  Below the root
+  // the histogrammed (smaller) children hold at most N/2 rows over 2^(d-1)
+  // pairs, and each pair adds at most one partial chunk per tile-channel.
+  static uint32_t hist_items(const JobDev& J, uint32_t d) {
+    const uint32_t TK = J.T * J.K;
+    if (d == 0) return ceil_div(J.N, J.hist_chunk) * TK;
+    const uint32_t pairs = 1u << (d - 1);
+    if (!J.deep_items) return (ceil_div(J.N / 2, J.hist_chunk) + pairs) * TK;
+    // chunk_for: floor(rows x TK / chunk) <= deep_items unless the chunk is
+    // 16384; the items of a pair are at most its rows / chunk + 1
+    const uint32_t full = std::max((J.deep_items + 1) / TK + 2, ceil_div(J.N / 2, 16384) + 1);
+    return (full + pairs) * TK;
   }
 
   void set_chunks(uint32_t j, const TuneChoice& c) {
     JobDev& J = h_jobs_[j];
     if (fixed_chunk_[j]) {
-      J.hist_chunk = J.hist_chunk_deep = fixed_chunk_[j];
+      J.hist_chunk = fixed_chunk_[j];
+      J.deep_items = 0;
       return;
     }
     J.hist_chunk = chunk_for(work_[j], c.root_items);
-    J.hist_chunk_deep = std::max(1024u, J.hist_chunk / c.deep_div);
+    J.deep_items = c.deep_items;
   }
 
   // Launch grids of k_hist for the current shapes (<= the allocated lists).
   void recompute_caps() {
     for (uint32_t d = 0; d < D_; ++d) {
       uint32_t c = 0;
-      for (uint32_t j = 0; j < njobs_; ++j) {
-        const JobDev& J = h_jobs_[j];
-        c += hist_items(J.N, d, J.hist_chunk, J.hist_chunk_deep) * J.T * J.K;
-      }
+      for (uint32_t j = 0; j < njobs_; ++j) c += hist_items(h_jobs_[j], d);
       hist_cap_[d] = c;
     }
   }
@@ -419,7 +423,7 @@ class Trainer {
 
   void hist_chunks(uint32_t j, uint32_t* root, uint32_t* deep) const {
     *root = h_jobs_[j].hist_chunk;
-    *deep = h_jobs_[j].hist_chunk_deep;
+    *deep = h_jobs_[j].deep_items;
   }
 
   // Device-side consistency flags (TreeState::fault) of every job.
@@ -548,6 +552,8 @@ class Trainer {
       J.nh = alloc<float>(nheap * K);
       J.nc = alloc<uint32_t>(nheap);
       J.hist = alloc<HistCell>(((1ull << D_) - 1) * K * J.F * 256);
+      J.hfirst = alloc<uint32_t>((1ull << D_) - 1);
+      J.lvl_chunk = alloc<uint32_t>(D_);
       J.st = alloc<TreeState>(1);
       hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1), 0, s, J.st);
       J.cursor = alloc<uint32_t>(1u << D_);
@@ -572,21 +578,33 @@ class Trainer {
       hist_rows_[j] = alloc<unsigned long long>(1);
       TC_HIP(hipMemsetAsync(hist_rows_[j], 0, 8, s));
       J.hist_rows_total = hist_rows_[j];
-      // work-list capacities: room for the smallest work items any choice
-      // uses (tree_begin / k_part_plan emit at most this many)
-      uint32_t min_root = kMaxHistChunk, min_deep = kMaxHistChunk;
-      for (const TuneChoice& c : kTune) {
-        set_chunks(j, c);
-        min_root = std::min(min_root, J.hist_chunk);
-        min_deep = std::min(min_deep, J.hist_chunk_deep);
-      }
-      set_chunks(j, kTune[0]);
+      // work-list capacities: room for the most work items any choice
+      // gives (tree_begin / k_part_plan emit at most this many)
       for (uint32_t d = 0; d < D_; ++d) {
-        list_cap_[d] += hist_items(N, d, min_root, min_deep) * J.T * K;
+        uint32_t most = 0;
+        for (const TuneChoice& c : kTune) {
+          set_chunks(j, c);
+          most = std::max(most, hist_items(J, d));
+        }
+        list_cap_[d] += most;
         part_cap_[d] += ceil_div(N, kPartChunk) + (1u << d);
       }
+      set_chunks(j, kTune[0]);
     }
     recompute_caps();
+    // partial-histogram slots: one per work item of the longest list (reused
+    // level after level), unless that would take more than a quarter of the
+    // free device memory -- then every item adds into zeroed node histograms
+    // with atomics
+    {
+      uint32_t cap = 1;
+      for (uint32_t d = 0; d < D_; ++d) cap = std::max(cap, list_cap_[d]);
+      const size_t bytes = static_cast<size_t>(cap) * kTileF * 256 * sizeof(HistCell);
+      size_t free_b = 0, total_b = 0;
+      TC_HIP(hipMemGetInfo(&free_b, &total_b));
+      if (bytes <= free_b / 4) partials_ = alloc<HistCell>(static_cast<size_t>(cap) * kTileF * 256);
+      for (JobDev& J : h_jobs_) J.store_hist = partials_ ? 1u : 0u;
+    }
     for (uint32_t d = 0; d < D_; ++d) {
       lists_.hist[d] = alloc<HistWork>(std::max(1u, list_cap_[d]));
       lists_.part[d] = alloc<PartWork>(std::max(1u, part_cap_[d]));
@@ -643,6 +661,7 @@ class Trainer {
   uint32_t* d_job_wg_off_ = nullptr;
   uint32_t total_grad_wg_ = 0;
   Lists lists_{};
+  HistCell* partials_ = nullptr;        // [item][16][256] of the level being histogrammed
   uint32_t hist_cap_[kMaxDepth] = {}, part_cap_[kMaxDepth] = {};
   std::vector<void*> allocs_;
   hipGraphExec_t graph_exec_ = nullptr;
@@ -949,7 +968,7 @@ int32_t tc_hist_build(tc_dataset* ds, const float* g, const float* h, const uint
       TC_HIP(hipMemcpyAsync(dj, &J, sizeof(J), hipMemcpyHostToDevice, s));
       if (cnt)
         hipLaunchKernelGGL((k_hist<4, false>), dim3(round8(cnt)), dim3(kHistThreads), 0, s, dj, dw,
-                           dc, 0u);
+                           dc, 0u, static_cast<HistCell*>(nullptr));
       hipLaunchKernelGGL(k_hist_to_float, dim3(ceil_div(F * 256ull, 256)), dim3(256), 0, s, J.hist,
                          J.st, F * 256u, fh);
       TC_HIP(hipGetLastError());
@@ -1025,7 +1044,7 @@ int32_t tc_split_scan(tc_ctx* ctx, const float* hist, const float* node_g, const
       JobDev* dj = static_cast<JobDev*>(al(sizeof(JobDev)));
       TC_HIP(hipMemcpyAsync(dj, &J, sizeof(J), hipMemcpyHostToDevice, s));
       hipLaunchKernelGGL(k_split_scan<false>, dim3(F, 1), dim3(256), 0, s, dj,
-                         static_cast<const float2*>(fh), level);
+                         static_cast<const float2*>(fh), level, static_cast<const HistCell*>(nullptr));
       TC_HIP(hipGetLastError());
       TC_HIP(hipMemcpyAsync(&st, J.st, sizeof(st), hipMemcpyDeviceToHost, s));
       TC_HIP(hipStreamSynchronize(s));

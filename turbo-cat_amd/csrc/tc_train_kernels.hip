@@ -78,10 +78,11 @@ __device__ inline uint32_t find_item(const uint32_t* off, uint32_t n, uint32_t e
 }
 
 __device__ inline void put_hist_work(HistWork* dst, uint32_t job, uint32_t node, uint32_t start,
-                                     uint32_t len, uint32_t tile, uint32_t cls) {
+                                     uint32_t len, uint32_t tile, uint32_t cls, bool last) {
   HistWork w;
   w.job = job; w.node = node; w.start = start; w.len = len; w.tile = tile; w.cls = cls;
-  w.pad1 = w.pad2 = 0;
+  w.last = last ? 1u : 0u;
+  w.pad = 0;
   *dst = w;
 }
 
@@ -349,6 +350,8 @@ __device__ void tree_begin(const JobDev& J, uint32_t job, const Lists& L, int pa
       if (go) {
         s_hbase = atomicAdd(L.hist_count[0], hcnt);
         s_pbase = atomicAdd(L.part_count[0], pcnt);
+        J.hfirst[0] = s_hbase;
+        J.lvl_chunk[0] = J.hist_chunk;
         if (J.hist_rows_total) atomicAdd(J.hist_rows_total, static_cast<unsigned long long>(n));
       }
     }
@@ -359,10 +362,11 @@ __device__ void tree_begin(const JobDev& J, uint32_t job, const Lists& L, int pa
   if (!s_go) return;
   // chunk-major, then tile, then channel: the items reading the same row
   // segments are adjacent (and share an XCD)
-  const uint32_t TK = J.T * K;
+  const uint32_t TK = J.T * K, nch = hcnt / TK;
   for (uint32_t e = tid; e < hcnt; e += blockDim.x) {
     const uint32_t start = (e / TK) * J.hist_chunk, r = e % TK;
-    put_hist_work(L.hist[0] + s_hbase + e, job, 0, start, min(J.hist_chunk, n - start), r / K, r % K);
+    put_hist_work(L.hist[0] + s_hbase + e, job, 0, start, min(J.hist_chunk, n - start), r / K, r % K,
+                  e / TK + 1 == nch);
   }
   for (uint32_t e = tid; e < pcnt; e += blockDim.x) {
     const uint32_t start = e * kPartChunk;
@@ -374,8 +378,8 @@ __device__ void tree_begin(const JobDev& J, uint32_t job, const Lists& L, int pa
 // Start of a tree, over the row grid: per-channel root totals (sum g, sum h)
 // over 1024-position tiles of the root list rows[0][0 .. n_root), double,
 // fixed order within the tile (k_tree_begin adds the tiles in a fixed
-// order); the root's histograms zeroed (deeper levels are zeroed by the
-// split scan of the level above).
+// order); in the atomic histogram mode the root's histograms zeroed (deeper
+// levels are zeroed by the split scan of the level above).
 __global__ void __launch_bounds__(256) k_root_partials(const JobDev* __restrict__ jobs,
                                                        const uint32_t* __restrict__ job_wg_off,
                                                        uint32_t njobs) {
@@ -387,7 +391,7 @@ __global__ void __launch_bounds__(256) k_root_partials(const JobDev* __restrict_
   const JobDev& J = jobs[s_job];
   const uint32_t lw = blockIdx.x - J.grad_wg_off, K = J.K;
   const bool live = J.st->tree + 1 < static_cast<int>(J.n_trees);
-  if (live) {
+  if (live && !J.store_hist) {
     const uint32_t cells = K * J.F * 256u;  // the root's histograms
     const uint32_t per = (cells + J.grad_wg_cnt - 1) / J.grad_wg_cnt;
     const uint32_t c1 = min(cells, (lw + 1) * per);
@@ -441,13 +445,21 @@ __global__ void __launch_bounds__(256) k_tree_begin(const JobDev* __restrict__ j
 // item has fewer than 2^16 rows, so the count never carries); h = 2 * count.
 // Otherwise one atomic on the g plane and one on the h plane.
 //
-// The non-empty cells are added into the node's HBM histogram with int64
+// Training (`partials` given, J.store_hist): a node's last chunk stores its
+// cells into the node's HBM histogram, every other chunk into its slot of
+// `partials` (slot = list index), all with plain stores; the split scan adds
+// a node's slots to its histogram.  Nothing has to be zeroed beforehand and
+// no global atomics are issued: int64 atomics from every workgroup (2 x 4096
+// per item) are served by the memory-side atomic units at ~1.5e11 per second
+// for the whole chip, several times the cost of the stores.  Otherwise the
+// non-empty cells are added into the zeroed node histogram with int64
 // atomics.
 template <int U, bool PACKED>
 __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict__ jobs,
                                                        const HistWork* __restrict__ work,
                                                        const uint32_t* __restrict__ count,
-                                                       uint32_t level) {
+                                                       uint32_t level,
+                                                       HistCell* __restrict__ partials) {
   constexpr int kCells = kTileF * 256;
   __shared__ unsigned long long s_a[kCells];
   __shared__ unsigned long long s_b[PACKED ? 1 : kCells];
@@ -474,22 +486,38 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
   const uint32_t f0 = w.tile * kTileF;
   const uint32_t nf = min(static_cast<uint32_t>(kTileF), J.F - f0);  // real features of the tile
 
-  for (uint32_t j0 = 0; j0 < w.len; j0 += kHistThreads * U) {
-    uint4 v[U];
-    float2 g[U];
-    bool ok[U];
+  // Software pipeline over batches of U x 512 rows: while batch i is added
+  // into LDS, the bins and (g, h) of batch i+1 and the row ids of batch i+2
+  // are in flight.
+  constexpr uint32_t STEP = kHistThreads * U;
+  const uint32_t len = w.len;
+  auto row_at = [&](uint32_t j) { return j < len ? (direct ? w.start + j : rows[j]) : 0u; };
+  uint32_t r1[U], r2[U];
+  uint4 v[U];
+  float2 g[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const uint32_t j = u * kHistThreads + tid;
+    const uint32_t row = row_at(j);
+    v[u] = j < len ? *reinterpret_cast<const uint4*>(brm + static_cast<size_t>(row) * Fp)
+                   : make_uint4(0, 0, 0, 0);
+    g[u] = j < len ? gh[row] : make_float2(0.f, 0.f);
+    r1[u] = row_at(STEP + j);
+  }
+  for (uint32_t j0 = 0; j0 < len; j0 += STEP) {
+    uint4 vn[U];
+    float2 gn[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint32_t j = j0 + u * kHistThreads + tid;
-      ok[u] = j < w.len;
-      const uint32_t row = ok[u] ? (direct ? w.start + j : rows[j]) : 0u;
-      v[u] = ok[u] ? *reinterpret_cast<const uint4*>(brm + static_cast<size_t>(row) * Fp)
-                   : make_uint4(0, 0, 0, 0);
-      g[u] = ok[u] ? gh[row] : make_float2(0.f, 0.f);
+      const uint32_t j = j0 + STEP + u * kHistThreads + tid;
+      vn[u] = j < len ? *reinterpret_cast<const uint4*>(brm + static_cast<size_t>(r1[u]) * Fp)
+                      : make_uint4(0, 0, 0, 0);
+      gn[u] = j < len ? gh[r1[u]] : make_float2(0.f, 0.f);
+      r2[u] = row_at(j + STEP);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      if (!ok[u]) continue;
+      if (j0 + u * kHistThreads + tid >= len) continue;
       const long long gq = __double2ll_rn(static_cast<double>(g[u].x) * gsc);
       const unsigned long long va = PACKED ? (static_cast<unsigned long long>(gq) << 16) + 1ull
                                            : static_cast<unsigned long long>(gq);
@@ -517,10 +545,19 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
         }
       }
     }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      v[u] = vn[u];
+      g[u] = gn[u];
+      r1[u] = r2[u];
+    }
   }
   __syncthreads();
+  const bool store = partials != nullptr;
   HistCell* out =
-      J.hist + ((static_cast<size_t>(heap_index(level, w.node)) * J.K + w.cls) * J.F + f0) * 256u;
+      store && !w.last
+          ? partials + static_cast<size_t>(li) * kCells
+          : J.hist + ((static_cast<size_t>(heap_index(level, w.node)) * J.K + w.cls) * J.F + f0) * 256u;
   const long long h2 = PACKED ? __double2ll_rn(2.0 * hsc) : 0;
   for (uint32_t i = tid; i < static_cast<uint32_t>(kCells); i += kHistThreads) {
     const uint32_t f = i & 15u, b = i >> 4;  // LDS order: conflict-free reads
@@ -534,7 +571,9 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
       gs = static_cast<long long>(a);
       hs = static_cast<long long>(s_b[i]);
     }
-    if (gs != 0 || hs != 0) {
+    if (store) {
+      out[f * 256u + b] = HistCell{gs, hs};
+    } else if (gs != 0 || hs != 0) {
       HistCell* o = out + f * 256u + b;
       atomicAdd(reinterpret_cast<unsigned long long*>(&o->g), static_cast<unsigned long long>(gs));
       atomicAdd(reinterpret_cast<unsigned long long*>(&o->h), static_cast<unsigned long long>(hs));
@@ -552,17 +591,33 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
 // rounded once to float.  Otherwise (the tc_split_scan entry point, float
 // bins): the reference's sequential float prefix, so for a given float
 // histogram every left sum, gain and total is the reference's float value.
+//
+// FIXED, per pass: one task per histogrammed node and channel (the root's
+// channels at level 0; the smaller child of each pair below).  The task's
+// cells start from the node's HBM histogram (its last chunk's cells, or in
+// the atomic mode the whole sum; zero for an empty node) and every wave adds
+// partial-histogram slots of the tasks into LDS (int64 atomics: exact in any
+// order), two slots per wave in flight.  Then one wave per task stores the
+// node's histogram, forms the larger sibling as parent - smaller (A13:
+// Histogram::subtract_from, histogram.cpp:39-42 via symmetric_tree.cpp:346)
+// and writes both prefixes; in the atomic mode it also zeroes the children's
+// next-level histograms.
 template <bool FIXED>
 __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev* __restrict__ jobs,
                                                                    const float2* __restrict__ fhist,
-                                                                   uint32_t level) {
+                                                                   uint32_t level,
+                                                                   const HistCell* __restrict__ partials) {
   constexpr uint32_t TH = FIXED ? 1024u : 256u;  // threads
   constexpr uint32_t NW = TH / 64u;              // waves
   constexpr uint32_t Q = TH / 256u;              // candidate lanes per bin
+  constexpr uint32_t kTasks = kScanNodesPerPass / 2;  // histogrammed units per pass
+  static_assert(!FIXED || kTasks <= NW, "one wave per task");
   __shared__ float2 s_pref[kScanNodesPerPass][256];
   __shared__ float s_tot[Q][256];
   __shared__ uint8_t s_any[Q][256];
   __shared__ unsigned long long s_red[4];
+  __shared__ unsigned long long s_acc[FIXED ? kTasks : 1][256][2];  // smaller child's cells (g, h)
+  __shared__ uint32_t s_sm[kTasks], s_first[kTasks], s_cpre[kTasks + 1], s_empty[kTasks];
   const uint32_t job = blockIdx.y, f = blockIdx.x, tid = threadIdx.x;
   const JobDev& J = jobs[job];
   if (!J.st->active || f >= J.F) return;
@@ -574,53 +629,87 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
   const uint32_t lane = tid & 63u, wave = tid >> 6;
   const uint32_t bin = tid & 255u, part = tid >> 8;  // candidate, unit subset
   const size_t F = J.F;
+  // units per pass: whole pairs of nodes (all channels) below the root
+  const uint32_t P = !FIXED ? static_cast<uint32_t>(kScanNodesPerPass)
+                   : level == 0 ? K : (kScanNodesPerPass / (2 * K)) * (2 * K);
+  const bool store = FIXED && partials != nullptr && J.store_hist;
   float total = 0.f;
   bool any = false;
-  for (uint32_t c0 = 0; c0 < nu; c0 += kScanNodesPerPass) {
-    const uint32_t cs = min(static_cast<uint32_t>(kScanNodesPerPass), nu - c0);
+  for (uint32_t c0 = 0; c0 < nu; c0 += P) {
+    const uint32_t cs = min(P, nu - c0);
     if (FIXED) {
-      // exact prefix: one wave per unit, lane l holds bins 4l .. 4l+3.  At
-      // level > 0 the larger child of each pair is formed here, parent -
-      // smaller (A13: Histogram::subtract_from, histogram.cpp:39-42 via
-      // symmetric_tree.cpp:346), and stored; the node's two level+1
-      // histograms are zeroed for the next level's k_hist.
-      for (uint32_t k = wave; k < cs; k += NW) {
-        const uint32_t node = (c0 + k) / K, ch = (c0 + k) % K;
-        auto cells = [&](uint32_t lvl, uint32_t nd) {
-          return J.hist + ((static_cast<size_t>(heap_index(lvl, nd)) * K + ch) * F + f) * 256u;
-        };
-        HistCell* hp = cells(level, node);
-        HistCell cell[4];
-        bool larger = false;
-        uint32_t sm = node;
-        if (level > 0) {
-          const uint32_t pr = node >> 1;
-          const uint32_t nl = J.nc[heap_index(level, 2 * pr)], nr = J.nc[heap_index(level, 2 * pr + 1)];
-          sm = nl <= nr ? 2 * pr : 2 * pr + 1;
-          larger = node != sm;
-        }
-        if (larger) {
-          const HistCell* pp = cells(level - 1, node >> 1);
-          const HistCell* sp = cells(level, sm);
+      auto cells = [&](uint32_t lvl, uint32_t nd, uint32_t ch) {
+        return J.hist + ((static_cast<size_t>(heap_index(lvl, nd)) * K + ch) * F + f) * 256u;
+      };
+      const uint32_t nt = level == 0 ? cs : cs / 2;  // tasks of this pass
+      // task t: channel t % K of (level 0) the root / (deeper) pair t / K
+      auto task_left = [&](uint32_t t) { return level == 0 ? 0u : c0 / K + 2u * (t / K); };
+      if (tid < nt) {
+        const uint32_t nl = task_left(tid);
+        uint32_t sm = nl;
+        if (level > 0 && J.nc[heap_index(level, nl)] > J.nc[heap_index(level, nl + 1)]) sm = nl + 1;
+        const uint32_t hn = heap_index(level, sm);
+        const uint32_t rows = J.nc[hn], chunk = J.lvl_chunk[level];
+        // sources: the node's HBM histogram, then (store mode) one partial
+        // slot per chunk but the last; none for an empty node
+        const bool empty = store && rows == 0;
+        s_sm[tid] = sm;
+        s_first[tid] = J.hfirst[hn];
+        s_cpre[tid + 1] = empty ? 0u : 1u + (store ? (rows - 1) / chunk : 0u);
+        s_empty[tid] = empty ? 1u : 0u;
+      }
+      for (uint32_t i = tid; i < nt * 512u; i += TH) (&s_acc[0][0][0])[i] = 0ull;
+      __syncthreads();
+      if (tid == 0) {
+        s_cpre[0] = 0;
+        for (uint32_t t = 0; t < nt; ++t) s_cpre[t + 1] += s_cpre[t];
+      }
+      // the wave that finishes task `wave` (nt <= NW) loads its parent now
+      HistCell par[4];
+      if (level > 0 && wave < nt) {
+        const HistCell* pp = cells(level - 1, task_left(wave) >> 1, wave % K) + 4u * lane;
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const HistCell a = pp[4 * lane + i], b = sp[4 * lane + i];
-            cell[i] = HistCell{a.g - b.g, a.h - b.h};
-            hp[4 * lane + i] = cell[i];
+        for (int i = 0; i < 4; ++i) par[i] = pp[i];
+      }
+      __syncthreads();
+      const uint32_t ne = s_cpre[nt];
+      const uint32_t tile = f / kTileF, fi = f % kTileF, TK = J.T * K;
+      auto source = [&](uint32_t e, uint32_t* t_out) {  // lane's 4 cells of source e
+        uint32_t t = 0;
+        while (s_cpre[t + 1] <= e) ++t;
+        *t_out = t;
+        const uint32_t j = e - s_cpre[t];
+        if (j == 0) return static_cast<const HistCell*>(cells(level, s_sm[t], t % K) + 4u * lane);
+        const uint32_t item = s_first[t] + (j - 1) * TK + tile * K + t % K;
+        return partials + (static_cast<size_t>(item) * kTileF + fi) * 256u + 4u * lane;
+      };
+      constexpr uint32_t kInFlight = 2;  // sources loaded per wave before adding
+      for (uint32_t e0 = wave; e0 < ne; e0 += kInFlight * NW) {
+        HistCell v[kInFlight][4];
+        uint32_t tt[kInFlight];
+#pragma unroll
+        for (uint32_t d = 0; d < kInFlight; ++d) {
+          if (e0 + d * NW < ne) {
+            const HistCell* p = source(e0 + d * NW, &tt[d]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[d][i] = p[i];
           }
-        } else {
-#pragma unroll
-          for (int i = 0; i < 4; ++i) cell[i] = hp[4 * lane + i];
         }
-        if (level + 1 < J.D) {
-          HistCell* z0 = cells(level + 1, 2 * node);
-          HistCell* z1 = cells(level + 1, 2 * node + 1);
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            z0[4 * lane + i] = HistCell{0, 0};
-            z1[4 * lane + i] = HistCell{0, 0};
+        for (uint32_t d = 0; d < kInFlight; ++d) {
+          if (e0 + d * NW < ne) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              atomicAdd(&s_acc[tt[d]][4 * lane + i][0], static_cast<unsigned long long>(v[d][i].g));
+              atomicAdd(&s_acc[tt[d]][4 * lane + i][1], static_cast<unsigned long long>(v[d][i].h));
+            }
           }
         }
+      }
+      __syncthreads();
+      // exact prefix of one unit's cells (lane l holds bins 4l .. 4l+3),
+      // rounded once to float, into s_pref[k]
+      auto prefix = [&](const HistCell* cell, uint32_t k) {
         long long pg[4], ph[4], ag = 0, ah = 0;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -641,6 +730,44 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
           s_pref[k][4 * lane + i] =
               make_float2(static_cast<float>(static_cast<double>(sg + pg[i]) * gi),
                           static_cast<float>(static_cast<double>(sh + ph[i]) * hi));
+      };
+      for (uint32_t t = wave; t < nt; t += NW) {
+        const uint32_t ch = t % K, nl = task_left(t), sm = s_sm[t];
+        HistCell cs_[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          cs_[i] = HistCell{static_cast<long long>(s_acc[t][4 * lane + i][0]),
+                            static_cast<long long>(s_acc[t][4 * lane + i][1])};
+        HistCell* hs = cells(level, sm, ch);
+        if (s_cpre[t + 1] - s_cpre[t] > 1u || s_empty[t]) {  // a parent at the next level
+#pragma unroll
+          for (int i = 0; i < 4; ++i) hs[4 * lane + i] = cs_[i];
+        }
+        prefix(cs_, sm * K + ch - c0);
+        if (level > 0) {
+          const uint32_t lg = sm == nl ? nl + 1 : nl;
+          HistCell* hl = cells(level, lg, ch);
+          HistCell cl[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const HistCell a = par[i];
+            cl[i] = HistCell{a.g - cs_[i].g, a.h - cs_[i].h};
+            hl[4 * lane + i] = cl[i];
+          }
+          prefix(cl, lg * K + ch - c0);
+        }
+        if (!store && level + 1 < J.D) {  // the next level's histograms of this task's nodes
+          const uint32_t n1 = level == 0 ? nl : nl + 1;  // the root, or the pair
+          for (uint32_t nd = nl; nd <= n1; ++nd) {
+            HistCell* z0 = cells(level + 1, 2 * nd, ch);
+            HistCell* z1 = cells(level + 1, 2 * nd + 1, ch);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              z0[4 * lane + i] = HistCell{0, 0};
+              z1[4 * lane + i] = HistCell{0, 0};
+            }
+          }
+        }
       }
       __syncthreads();
     } else {
@@ -885,7 +1012,7 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
   __shared__ uint32_t s_pw[1 << kMaxDepth];          // per child: partition slots -> first slot
   __shared__ uint32_t s_off[(1 << kMaxDepth) + 1];   // child offsets
   __shared__ uint32_t s_tmp[256];
-  __shared__ uint32_t s_hbase, s_pbase;
+  __shared__ uint32_t s_hbase, s_pbase, s_chunk;
   __shared__ int s_any;
   __shared__ unsigned long long s_rows;
   if (tid == 0) { s_any = 0; s_rows = 0ull; }
@@ -914,7 +1041,7 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
     // next level: histogram of the smaller child of the pair (ties: left),
     // partition slots of both children
     const uint32_t sm_len = min(acc, len - acc);
-    s_hw[n] = ((sm_len + J.hist_chunk_deep - 1) / J.hist_chunk_deep) * J.T * J.K;
+    s_hw[n] = sm_len;
     s_pw[2 * n] = part_slots(acc);
     s_pw[2 * n + 1] = part_slots(len - acc);
     if (sm_len) atomicAdd(&s_rows, static_cast<unsigned long long>(sm_len));
@@ -928,6 +1055,17 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
     if (tid == 0) J.st->active = 0;
     return;
   }
+  // rows per work item of the next level: about J.deep_items items over its
+  // rows, or the fixed chunk
+  const uint32_t TK = J.T * J.K;
+  if (tid == 0) {
+    s_chunk = J.deep_items ? chunk_for(s_rows * TK, J.deep_items) : J.hist_chunk;
+    J.lvl_chunk[level + 1] = s_chunk;
+  }
+  __syncthreads();
+  const uint32_t chunk = s_chunk;
+  for (uint32_t n = tid; n < nn; n += 256) s_hw[n] = ((s_hw[n] + chunk - 1) / chunk) * TK;
+  __syncthreads();
   const uint32_t htot = block_scan_excl(s_hw, nn, s_tmp);
   const uint32_t ptot = block_scan_excl(s_pw, 2 * nn, s_tmp);
   if (tid == 0) {
@@ -936,16 +1074,22 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
     if (J.hist_rows_total) atomicAdd(J.hist_rows_total, s_rows);
   }
   __syncthreads();
+  for (uint32_t n = tid; n < nn; n += 256) {  // first item of each pair's smaller child
+    const uint32_t lc = 2 * n, rc = lc + 1;
+    const uint32_t sm = s_off[lc + 1] - s_off[lc] <= s_off[rc + 1] - s_off[rc] ? lc : rc;
+    J.hfirst[heap_index(level + 1, sm)] = s_hbase + s_hw[n];
+  }
   HistWork* hout = L.hist[level + 1] + s_hbase;
   for (uint32_t e = tid; e < htot; e += 256) {
     const uint32_t n = find_item(s_hw, nn, e);
     const uint32_t lc = 2 * n, rc = lc + 1;
     const uint32_t nl = s_off[lc + 1] - s_off[lc], nr = s_off[rc + 1] - s_off[rc];
     const uint32_t sm = nl <= nr ? lc : rc;
-    const uint32_t le = e - s_hw[n], TK = J.T * J.K, r = le % TK;
-    const uint32_t start = s_off[sm] + (le / TK) * J.hist_chunk_deep;
-    put_hist_work(hout + e, job, sm, start, min(J.hist_chunk_deep, s_off[sm + 1] - start), r / J.K,
-                  r % J.K);
+    const uint32_t le = e - s_hw[n], r = le % TK;
+    const uint32_t start = s_off[sm] + (le / TK) * chunk;
+    const uint32_t end = s_off[sm + 1];
+    put_hist_work(hout + e, job, sm, start, min(chunk, end - start), r / J.K, r % J.K,
+                  start + chunk >= end);
   }
   PartWork* pout = L.part[level + 1] + s_pbase;
   for (uint32_t e = tid; e < ptot; e += 256) {

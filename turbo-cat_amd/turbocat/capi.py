@@ -389,7 +389,9 @@ class Trainer:
         check(self.ctx.L.tc_trainer_sync(self.h))
 
     def hist_chunks(self, job=0):
-        """Rows per histogram work item (root, deeper levels) in use for job."""
+        """Histogram work-item shape of job: (rows per item at the root, items
+        each deeper level is cut into; 0 when a fixed hist_chunk is used at
+        every level)."""
         r, d = c_uint32(), c_uint32()
         check(self.ctx.L.tc_trainer_hist_chunks(self.h, job, byref(r), byref(d)))
         return r.value, d.value
