@@ -222,13 +222,15 @@ class Trainer {
       }
       hipLaunchKernelGGL(k_split_scan<true>, dim3(fmax_, njobs_), dim3(1024), 0, s, d_jobs_,
                          static_cast<const float2*>(nullptr), d, partials_);
-      // one wave per (node, channel) unit of the widest job
-      hipLaunchKernelGGL(k_finalize, dim3(ceil_div(static_cast<uint64_t>(kmax_) << d, 4), njobs_),
-                         dim3(256), 0, s, d_jobs_, d);
       if (d + 1 < D_ || keep_rows_) {  // training never reads the last level's rows
+        // the partition also finalizes the level (split record, child totals)
         hipLaunchKernelGGL(k_partition, dim3(part_cap_[d]), dim3(kPartThreads), 0, s, d_jobs_, lists_.part[d],
                            lists_.part_count[d], d);
         hipLaunchKernelGGL(k_part_plan, dim3(1, njobs_), dim3(256), 0, s, d_jobs_, lists_, d);
+      } else {
+        // one wave per (node, channel) unit of the widest job
+        hipLaunchKernelGGL(k_finalize, dim3(ceil_div(static_cast<uint64_t>(kmax_) << d, 4), njobs_),
+                           dim3(256), 0, s, d_jobs_, d);
       }
     }
     hipLaunchKernelGGL(k_grad_update, dim3(total_grad_wg_), dim3(256), leaf_lds(), s, d_jobs_,

@@ -352,6 +352,8 @@ __device__ void tree_begin(const JobDev& J, uint32_t job, const Lists& L, int pa
         s_pbase = atomicAdd(L.part_count[0], pcnt);
         J.hfirst[0] = s_hbase;
         J.lvl_chunk[0] = J.hist_chunk;
+        J.cursor[0] = 0;
+        J.cursor[1] = n;
         if (J.hist_rows_total) atomicAdd(J.hist_rows_total, static_cast<unsigned long long>(n));
       }
     }
@@ -609,12 +611,11 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
                                                                    const HistCell* __restrict__ partials) {
   constexpr uint32_t TH = FIXED ? 1024u : 256u;  // threads
   constexpr uint32_t NW = TH / 64u;              // waves
-  constexpr uint32_t Q = TH / 256u;              // candidate lanes per bin
+  constexpr uint32_t Q = TH / 256u;              // threads per candidate bin
   constexpr uint32_t kTasks = kScanNodesPerPass / 2;  // histogrammed units per pass
   static_assert(!FIXED || kTasks <= NW, "one wave per task");
+  static_assert(!FIXED || kTasks * 256 * 16 >= kScanNodesPerPass * 256 * 5, "gain terms fit s_acc");
   __shared__ float2 s_pref[kScanNodesPerPass][256];
-  __shared__ float s_tot[Q][256];
-  __shared__ uint8_t s_any[Q][256];
   __shared__ unsigned long long s_red[4];
   __shared__ unsigned long long s_acc[FIXED ? kTasks : 1][256][2];  // smaller child's cells (g, h)
   __shared__ uint32_t s_sm[kTasks], s_first[kTasks], s_cpre[kTasks + 1], s_empty[kTasks];
@@ -786,10 +787,41 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
       }
       __syncthreads();
     }
-    // gain of candidate `bin`: unit terms in (node, channel) order (part p
-    // takes the units k = p, p + Q, ...; Q = 1 is the reference's single
-    // sequence)
-    if (bin + 1 < mb) {
+    // gain of candidate `bin`: the unit terms added in (node, channel) order,
+    // one sequence per candidate as in the reference.  FIXED: all threads
+    // form the terms (three divisions each) into LDS, then the candidate's
+    // thread adds them.
+    if (FIXED) {
+      float* s_term = reinterpret_cast<float*>(&s_acc[0][0][0]);               // [32][256]
+      uint8_t* s_use = reinterpret_cast<uint8_t*>(s_term + kScanNodesPerPass * 256);  // [32][256]
+      for (uint32_t i = tid; i < cs * 256u; i += TH) {
+        const uint32_t k = i >> 8, b = i & 255u;
+        const uint32_t hn = heap_index(level, (c0 + k) / K);
+        const uint32_t hu = hn * K + (c0 + k) % K;
+        bool use = b + 1 < mb && !(level > 0 && J.nc[hn] == 0);
+        float term = 0.f;
+        if (use) {
+          const float G = J.ng[hu], H = J.nh[hu];
+          const float2 Lf = s_pref[k][b];
+          const float rg = __fsub_rn(G, Lf.x), rh = __fsub_rn(H, Lf.y);
+          use = !(Lf.y < 1e-10f || rh < 1e-10f);
+          const float pg = H > 1e-10f ? __fdiv_rn(__fmul_rn(G, G), __fadd_rn(H, lambda)) : 0.f;
+          const float lgn = __fdiv_rn(__fmul_rn(Lf.x, Lf.x), __fadd_rn(Lf.y, lambda));
+          const float rgn = __fdiv_rn(__fmul_rn(rg, rg), __fadd_rn(rh, lambda));
+          term = __fsub_rn(__fadd_rn(lgn, rgn), pg);
+        }
+        s_term[i] = term;
+        s_use[i] = use ? 1 : 0;
+      }
+      __syncthreads();
+      if (part == 0) {
+        for (uint32_t k = 0; k < cs; ++k) {
+          if (!s_use[k * 256u + bin]) continue;
+          total = __fmaf_rn(0.5f, s_term[k * 256u + bin], total);
+          any = true;
+        }
+      }
+    } else if (bin + 1 < mb) {
       for (uint32_t k = part; k < cs; k += Q) {
         const uint32_t hn = heap_index(level, (c0 + k) / K);
         if (level > 0 && J.nc[hn] == 0) continue;
@@ -806,17 +838,6 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
       }
     }
     __syncthreads();
-  }
-  if (Q > 1) {  // the parts' sums, added in part order
-    s_tot[part][bin] = total;
-    s_any[part][bin] = any ? 1 : 0;
-    __syncthreads();
-    if (part == 0) {
-      for (uint32_t p = 1; p < Q; ++p) {
-        total = __fadd_rn(total, s_tot[p][bin]);
-        any = any || s_any[p][bin] != 0;
-      }
-    }
   }
   if (part == 0) {
     unsigned long long key = kEmptyKey;
@@ -867,7 +888,7 @@ __device__ inline bool decode_split(const JobDev& J, uint32_t level, bool record
 
 // Child totals of node n, channel c under split (f, thr), by one wave: left
 // = bins <= thr plus the NaN slot 255, right = the other value bins, exact;
-// lane 0 stores them (float), and for channel 0 the node's partition cursors.
+// lane 0 stores them (float).
 __device__ inline void child_totals(const JobDev& J, uint32_t level, uint32_t n, uint32_t c,
                                     uint32_t f, uint32_t thr) {
   const uint32_t lane = threadIdx.x & 63u, K = J.K;
@@ -892,16 +913,12 @@ __device__ inline void child_totals(const JobDev& J, uint32_t level, uint32_t n,
     J.nh[cl * K + c] = static_cast<float>(static_cast<double>(v[1]) * hi);
     J.ng[(cl + 1) * K + c] = static_cast<float>(static_cast<double>(v[2]) * gi);
     J.nh[(cl + 1) * K + c] = static_cast<float>(static_cast<double>(v[3]) * hi);
-    if (c == 0) {
-      const uint32_t* offd = J.off + static_cast<size_t>(level) * ((1u << J.D) + 1u);
-      J.cursor[2 * n] = offd[n];
-      J.cursor[2 * n + 1] = offd[n + 1];
-    }
   }
 }
 
-// Finalize launch: workgroup (group of 4 units, job), one wave per unit
-// (node, channel); workgroup 0 records the split.
+// Finalize of the last level (the levels above are finalized by their
+// partition): workgroup (group of 4 units, job), one wave per unit (node,
+// channel); workgroup 0 records the split.
 __global__ void __launch_bounds__(256) k_finalize(const JobDev* __restrict__ jobs, uint32_t level) {
   const JobDev& J = jobs[blockIdx.y];
   if (!J.st->active) return;
@@ -912,12 +929,15 @@ __global__ void __launch_bounds__(256) k_finalize(const JobDev* __restrict__ job
 }
 
 // ---------------------------------------------------------------------------
-// A12: partition, one pass.  Each work item (<= 4096 rows of one node)
-// routes its rows -- NaN bin 255 or bin <= threshold goes left
+// A12: partition, one pass.  Every workgroup decodes the level's split
+// itself; the job's first work item records it (the level's finalize), the
+// first item of each node computes the node's child totals (one wave per
+// channel) while its rows are routed.  Each work item (<= 4096 rows of one
+// node) routes its rows -- NaN bin 255 or bin <= threshold goes left
 // (symmetric_tree.cpp:309-311) -- keeping them in registers, reserves room
 // with one atomic per side on the node's cursors (left fills up from the
-// node's start, right fills down from its end) and writes the rows, placed
-// by wave ballots.
+// node's start, right fills down from its end; set by the previous level's
+// k_part_plan) and writes the rows, placed by wave ballots.
 __device__ inline bool part_active(const JobDev& J, uint32_t level) {
   return J.st->depth == static_cast<int32_t>(level) + 1 && J.st->tree < static_cast<int>(J.n_trees);
 }
@@ -935,13 +955,16 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(const JobDev* __rest
   if (blockIdx.x >= *count) return;
   const PartWork w = work[blockIdx.x];
   const JobDev& J = jobs[w.job];
-  if (!part_active(J, level)) return;
-  const size_t o = static_cast<size_t>(J.st->tree) * J.D + level;
-  const uint8_t* col = J.bins_cm + static_cast<size_t>(J.m_feat[o]) * J.N;
-  const uint8_t thr = J.m_bin[o];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (!J.st->active || J.st->tree >= static_cast<int>(J.n_trees)) return;
+  uint32_t feat, split_bin;
+  if (!decode_split(J, level, w.slot == 0 && tid == 0, &feat, &split_bin)) return;
+  const uint32_t* offd = J.off + static_cast<size_t>(level) * ((1u << J.D) + 1u);
+  if (wv < J.K && w.start == offd[w.node]) child_totals(J, level, w.node, wv, feat, split_bin);
+  const uint8_t* col = J.bins_cm + static_cast<size_t>(feat) * J.N;
+  const uint8_t thr = static_cast<uint8_t>(split_bin);
   uint32_t* rows_n = J.rows + static_cast<size_t>(level + 1) * J.N;
   uint32_t* cur = J.cursor + 2 * w.node;
-  const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   __shared__ uint32_t s_cnt[2][kIt * kW];  // per (step, wave): count -> offset in the item
   __shared__ uint32_t s_base[2];
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -978,7 +1001,6 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(const JobDev* __rest
   }
   __syncthreads();
   {  // both reservations must lie inside the node's segment; otherwise write nothing
-    const uint32_t* offd = J.off + static_cast<size_t>(level) * ((1u << J.D) + 1u);
     const uint32_t lo = offd[w.node], hi = offd[w.node + 1];
     const bool bad = (s_tot[0] && (s_base[0] < lo || s_base[0] > hi || s_tot[0] > hi - s_base[0])) ||
                      (s_tot[1] && (s_base[1] < lo || s_base[1] > hi || s_tot[1] > hi - s_base[1]));
@@ -1024,6 +1046,14 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
     const uint32_t cl = heap_index(level + 1, 2 * n);
     J.nc[cl] = acc;
     J.nc[cl + 1] = len - acc;
+    if (len == 0) {  // an empty node's children (no partition item computed them)
+      for (uint32_t c = 0; c < J.K; ++c) {
+        J.ng[cl * J.K + c] = 0.f;
+        J.nh[cl * J.K + c] = 0.f;
+        J.ng[(cl + 1) * J.K + c] = 0.f;
+        J.nh[(cl + 1) * J.K + c] = 0.f;
+      }
+    }
     s_off[2 * n] = offd[n];
     s_off[2 * n + 1] = offd[n] + acc;
     offn[2 * n] = offd[n];
@@ -1074,10 +1104,16 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
     if (J.hist_rows_total) atomicAdd(J.hist_rows_total, s_rows);
   }
   __syncthreads();
-  for (uint32_t n = tid; n < nn; n += 256) {  // first item of each pair's smaller child
+  for (uint32_t n = tid; n < nn; n += 256) {
+    // first item of each pair's smaller child
     const uint32_t lc = 2 * n, rc = lc + 1;
     const uint32_t sm = s_off[lc + 1] - s_off[lc] <= s_off[rc + 1] - s_off[rc] ? lc : rc;
     J.hfirst[heap_index(level + 1, sm)] = s_hbase + s_hw[n];
+    // partition cursors of the next level (every cursor was read above)
+    J.cursor[2 * lc] = s_off[lc];
+    J.cursor[2 * lc + 1] = s_off[lc + 1];
+    J.cursor[2 * rc] = s_off[rc];
+    J.cursor[2 * rc + 1] = s_off[rc + 1];
   }
   HistWork* hout = L.hist[level + 1] + s_hbase;
   for (uint32_t e = tid; e < htot; e += 256) {
