@@ -619,6 +619,8 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
   __shared__ unsigned long long s_red[4];
   __shared__ unsigned long long s_acc[FIXED ? kTasks : 1][256][2];  // smaller child's cells (g, h)
   __shared__ uint32_t s_sm[kTasks], s_first[kTasks], s_cpre[kTasks + 1], s_empty[kTasks];
+  __shared__ float s_G[kScanNodesPerPass], s_H[kScanNodesPerPass];
+  __shared__ uint32_t s_N[kScanNodesPerPass];
   const uint32_t job = blockIdx.y, f = blockIdx.x, tid = threadIdx.x;
   const JobDev& J = jobs[job];
   if (!J.st->active || f >= J.F) return;
@@ -645,17 +647,25 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
       const uint32_t nt = level == 0 ? cs : cs / 2;  // tasks of this pass
       // task t: channel t % K of (level 0) the root / (deeper) pair t / K
       auto task_left = [&](uint32_t t) { return level == 0 ? 0u : c0 / K + 2u * (t / K); };
+      if (tid < cs) {  // the pass's unit totals, for the gain terms
+        const uint32_t hn = heap_index(level, (c0 + tid) / K);
+        s_G[tid] = J.ng[hn * K + (c0 + tid) % K];
+        s_H[tid] = J.nh[hn * K + (c0 + tid) % K];
+        s_N[tid] = J.nc[hn];
+      }
       if (tid < nt) {
-        const uint32_t nl = task_left(tid);
-        uint32_t sm = nl;
-        if (level > 0 && J.nc[heap_index(level, nl)] > J.nc[heap_index(level, nl + 1)]) sm = nl + 1;
-        const uint32_t hn = heap_index(level, sm);
-        const uint32_t rows = J.nc[hn], chunk = J.lvl_chunk[level];
+        const uint32_t nl = task_left(tid), hl = heap_index(level, nl);
+        // both siblings' counts and first items in one round trip
+        const uint32_t cl = J.nc[hl], fl = J.hfirst[hl];
+        const uint32_t cr = level > 0 ? J.nc[hl + 1] : 0u, fr = level > 0 ? J.hfirst[hl + 1] : 0u;
+        const bool right = level > 0 && cl > cr;
+        const uint32_t sm = right ? nl + 1 : nl;
+        const uint32_t rows = right ? cr : cl, chunk = J.lvl_chunk[level];
         // sources: the node's HBM histogram, then (store mode) one partial
         // slot per chunk but the last; none for an empty node
         const bool empty = store && rows == 0;
         s_sm[tid] = sm;
-        s_first[tid] = J.hfirst[hn];
+        s_first[tid] = right ? fr : fl;
         s_cpre[tid + 1] = empty ? 0u : 1u + (store ? (rows - 1) / chunk : 0u);
         s_empty[tid] = empty ? 1u : 0u;
       }
@@ -796,12 +806,10 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
       uint8_t* s_use = reinterpret_cast<uint8_t*>(s_term + kScanNodesPerPass * 256);  // [32][256]
       for (uint32_t i = tid; i < cs * 256u; i += TH) {
         const uint32_t k = i >> 8, b = i & 255u;
-        const uint32_t hn = heap_index(level, (c0 + k) / K);
-        const uint32_t hu = hn * K + (c0 + k) % K;
-        bool use = b + 1 < mb && !(level > 0 && J.nc[hn] == 0);
+        bool use = b + 1 < mb && !(level > 0 && s_N[k] == 0);
         float term = 0.f;
         if (use) {
-          const float G = J.ng[hu], H = J.nh[hu];
+          const float G = s_G[k], H = s_H[k];
           const float2 Lf = s_pref[k][b];
           const float rg = __fsub_rn(G, Lf.x), rh = __fsub_rn(H, Lf.y);
           use = !(Lf.y < 1e-10f || rh < 1e-10f);
