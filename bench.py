@@ -45,17 +45,19 @@ def kernel_sources_hash():
     return h.hexdigest()
 
 
-def measured_traffic(rows, features, depth, hist_chunk):
+def measured_traffic(rows, features, depth, shape):
     """HBM bytes per k_hist launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE measurement (copy-calibrated), used only when it was taken on
-    this workload with these exact kernel sources; otherwise None."""
+    this workload, with the same histogram work-item shape (rows per root
+    item, items per deeper level) and these exact kernel sources; otherwise
+    None."""
     try:
         with open(TRAFFIC_FILE) as fh:
             t = json.load(fh)
     except (OSError, ValueError):
         return None
-    same = (t.get("rows"), t.get("features"), t.get("depth"), t.get("hist_chunk")) == \
-        (rows, features, depth, hist_chunk)
+    same = (t.get("rows"), t.get("features"), t.get("depth"), t.get("shape")) == \
+        (rows, features, depth, list(shape))
     if not same or t.get("sources_sha256") != kernel_sources_hash():
         return None
     return t["traffic_bytes_per_launch"]
@@ -164,7 +166,7 @@ def run_train(args):
     R = 5  # trees timed kernel by kernel for the roofline
     prm = capi.train_params(loss_id, n_trees=args.warmup + T_timed + R,
                             learning_rate=0.1, max_depth=6, lambda_l2=lam,
-                            hist_chunk=args.hist_chunk)
+                            hist_chunk=args.hist_chunk, hist_deep_items=args.deep_items)
     tr = capi.Trainer(ctx, [ds], [y], prm)
     tr.step(args.warmup)
     tr.sync()
@@ -230,11 +232,12 @@ def run_train(args):
                    "trees_timed": T_timed},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": None if mse else measured_traffic(n, F, 6, args.hist_chunk),
+                     "traffic": None if mse else measured_traffic(n, F, 6, (root, deep_items)),
                      "kernel": "k_hist", "avg_launch_us": round(hist_ms / launches * 1e3, 2),
                      "algorithmic_bytes_per_launch": int(alg_bytes / launches)},
         "cpu_baseline": cpu,
         "extra": {"quantize_s": round(quant_s, 3),
+                  "hist_work_items": {"root_rows": root, "deep_items": deep_items},
                   "predict_rows_per_s_resident": round(pred_kernel, 1),
                   "predict_rows_per_s_with_copies": round(pred_e2e, 1),
                   "predict_trees": model["T"]},
@@ -562,7 +565,10 @@ def main():
     ap.add_argument("--kernel", default="k_hist", help="kernel name regex for --pmc-summary")
     ap.add_argument("--rows", type=int, default=1_000_000)
     ap.add_argument("--features", type=int, default=100)
-    ap.add_argument("--hist-chunk", type=int, default=0)
+    ap.add_argument("--hist-chunk", type=int, default=0,
+                    help="rows per histogram work item at the root (0: autotuned)")
+    ap.add_argument("--deep-items", type=int, default=0,
+                    help="with --hist-chunk: work items per deeper level (0: --hist-chunk rows)")
     ap.add_argument("--loss", default="logloss", choices=["logloss", "mse"],
                     help="train workload: binary logloss (default) or the MSE regression set")
     ap.add_argument("--no-cpu", action="store_true")

@@ -84,6 +84,10 @@ typedef struct {
   uint64_t seed;             /* Config::seed (config.hpp:235) */
   uint32_t n_classes;        /* TC_LOSS_SOFTMAX: K in [2, 16], (2^max_depth) * K <= 8192;
                                 otherwise 0 or 1 */
+  uint32_t hist_deep_items;  /* with hist_chunk != 0: 0 = hist_chunk rows per work item at
+                                every level, else hist_chunk rows at the root and each deeper
+                                level cut into about this many items (<= 65536); ignored
+                                when autotuned.  Never changes the model. */
 } tc_train_params;
 
 /* Host-side model arrays filled by training (caller-allocated).

@@ -146,3 +146,38 @@ def test_tcat_v3_round_trip(tmp_path):
         bad = tmp_path / "bad.tcat"
         bad.write_bytes(b"NOPE" + b"\0" * 16)
         turbocat.TurboCatRegressor().load(str(bad))
+
+
+def test_ctypes_structs_match_the_c_header(tmp_path):
+    """Sizes and field offsets of the parameter / model structs as a C
+    compiler lays them out from include/turbocat_hip.h equal the ctypes
+    mirrors the Python layer passes across the boundary."""
+    import ctypes
+    import shutil
+    import subprocess
+    from turbocat import capi
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {"tc_tree_params": capi.TreeParams, "tc_train_params": capi.TrainParams,
+               "tc_model": capi.Model, "tc_serve_params": capi.ServeParams}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "turbocat_hip.h"',
+             'int main(void) {']
+    for cname, cls in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ['  return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        parts = line.split()
+        got[tuple(parts[:-1])] = int(parts[-1])
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)

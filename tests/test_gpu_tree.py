@@ -229,15 +229,15 @@ def test_work_item_shape_never_changes_the_model(ctx, loss):
     ds, _, _ = ctx.quantize_fit(X)
     lam = 0.0 if loss == 0 else 1.0
     models = []
-    for hc in (0, 1024, 4096, 32768):
+    for hc, di in ((0, 0), (1024, 0), (4096, 0), (32768, 0), (2048, 64), (16384, 4096)):
         prm = capi.train_params(loss, n_trees=16, learning_rate=0.1, max_depth=6, lambda_l2=lam,
-                                hist_chunk=hc)
+                                hist_chunk=hc, hist_deep_items=di)
         tr = capi.Trainer(ctx, [ds], [y], prm)
         tr.step(16)
         tr.sync()
         root, deep_items = tr.hist_chunks()
         if hc:
-            assert (root, deep_items) == (hc, 0)
+            assert (root, deep_items) == (hc, di)
         else:
             assert 1024 <= root <= 16384 and deep_items in (256, 512, 1024)
         models.append(tr.model())

@@ -361,7 +361,7 @@ class Trainer {
     JobDev& J = h_jobs_[j];
     if (fixed_chunk_[j]) {
       J.hist_chunk = fixed_chunk_[j];
-      J.deep_items = 0;
+      J.deep_items = specs_[j].p.hist_deep_items;
       return;
     }
     J.hist_chunk = chunk_for(work_[j], c.root_items);
@@ -715,6 +715,7 @@ void check_params(const tc_train_params* p) {
   TC_REQUIRE(p->tree.max_depth >= 1 && p->tree.max_depth <= static_cast<uint32_t>(kMaxDepth),
              "max_depth must be in [1, 10]");
   TC_REQUIRE(p->learning_rate == p->learning_rate, "learning_rate is NaN");
+  TC_REQUIRE(p->hist_deep_items <= 65536u, "hist_deep_items must be <= 65536");
   if (p->loss == TC_LOSS_SOFTMAX) {
     TC_REQUIRE(p->n_classes >= 2 && p->n_classes <= static_cast<uint32_t>(kMaxClasses),
                "softmax needs n_classes in [2, 16]");

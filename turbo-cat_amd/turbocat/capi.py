@@ -57,7 +57,7 @@ class TrainParams(Structure):
     _fields_ = [("loss", c_int32), ("n_trees", c_uint32), ("learning_rate", c_float),
                 ("tree", TreeParams), ("hist_chunk", c_uint32), ("use_graph", c_int32),
                 ("subsample", c_float), ("colsample_bytree", c_float), ("seed", c_uint64),
-                ("n_classes", c_uint32)]
+                ("n_classes", c_uint32), ("hist_deep_items", c_uint32)]
 
 
 class Model(Structure):
@@ -78,10 +78,10 @@ def tree_params(max_depth=6, max_bins=255, lambda_l2=1.0, lambda_l1=0.0, max_del
 
 
 def train_params(loss=TC_LOSS_MSE, n_trees=100, learning_rate=0.1, hist_chunk=0, use_graph=1,
-                 subsample=1.0, colsample_bytree=1.0, seed=42, n_classes=0,
+                 subsample=1.0, colsample_bytree=1.0, seed=42, n_classes=0, hist_deep_items=0,
                  **tree) -> TrainParams:
     return TrainParams(loss, n_trees, learning_rate, tree_params(**tree), hist_chunk, use_graph,
-                       subsample, colsample_bytree, seed, n_classes)
+                       subsample, colsample_bytree, seed, n_classes, hist_deep_items)
 
 
 def channels(p: TrainParams):
