@@ -614,16 +614,25 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
   constexpr uint32_t Q = TH / 256u;              // threads per candidate bin
   constexpr uint32_t kTasks = kScanNodesPerPass / 2;  // histogrammed units per pass
   static_assert(!FIXED || kTasks <= NW, "one wave per task");
-  static_assert(!FIXED || kTasks * 256 * 16 >= kScanNodesPerPass * 256 * 5, "gain terms fit s_acc");
-  __shared__ float2 s_pref[kScanNodesPerPass][256];
+  // s_pref: the pass's left sums per unit and candidate.  FIXED: s_acc (the
+  // tasks' summed cells) shares its 64 KB; each task's cells are in
+  // registers before the first prefix is written.
+  __shared__ __align__(16) unsigned char s_raw[kScanNodesPerPass * 256 * sizeof(float2)];
+  float2(*s_pref)[256] = reinterpret_cast<float2(*)[256]>(s_raw);
+  unsigned long long(*s_acc)[256][2] = reinterpret_cast<unsigned long long(*)[256][2]>(s_raw);
+  static_assert(kTasks * 256 * 2 * sizeof(unsigned long long) <= sizeof(s_raw), "task cells fit");
   __shared__ unsigned long long s_red[4];
-  __shared__ unsigned long long s_acc[FIXED ? kTasks : 1][256][2];  // smaller child's cells (g, h)
+  __shared__ float s_chain[256];
+  __shared__ uint8_t s_chain_any[256];
   __shared__ uint32_t s_sm[kTasks], s_first[kTasks], s_cpre[kTasks + 1], s_empty[kTasks];
   __shared__ float s_G[kScanNodesPerPass], s_H[kScanNodesPerPass];
   __shared__ uint32_t s_N[kScanNodesPerPass];
   const uint32_t job = blockIdx.y, f = blockIdx.x, tid = threadIdx.x;
   const JobDev& J = jobs[job];
-  if (!J.st->active || f >= J.F) return;
+  if (f >= J.F) return;
+  // FIXED waits for the tree state only after its first loads are issued
+  const int32_t active = J.st->active;
+  if (!FIXED && !active) return;
   const uint32_t K = J.K;
   const uint32_t nu = K << level;  // units (node, channel), node-major
   const uint32_t mb = J.max_bins & 0xFFu;
@@ -671,6 +680,7 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
       }
       for (uint32_t i = tid; i < nt * 512u; i += TH) (&s_acc[0][0][0])[i] = 0ull;
       __syncthreads();
+      if (!active) return;  // the job's tree is done (uniform)
       if (tid == 0) {
         s_cpre[0] = 0;
         for (uint32_t t = 0; t < nt; ++t) s_cpre[t + 1] += s_cpre[t];
@@ -742,13 +752,18 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
               make_float2(static_cast<float>(static_cast<double>(sg + pg[i]) * gi),
                           static_cast<float>(static_cast<double>(sh + ph[i]) * hi));
       };
-      for (uint32_t t = wave; t < nt; t += NW) {
-        const uint32_t ch = t % K, nl = task_left(t), sm = s_sm[t];
-        HistCell cs_[4];
+      const bool has_task = wave < nt;  // nt <= NW: at most one task per wave
+      const uint32_t t = wave;
+      HistCell cs_[4];
+      if (has_task) {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
           cs_[i] = HistCell{static_cast<long long>(s_acc[t][4 * lane + i][0]),
                             static_cast<long long>(s_acc[t][4 * lane + i][1])};
+      }
+      __syncthreads();  // s_acc's space becomes s_pref
+      if (has_task) {
+        const uint32_t ch = t % K, nl = task_left(t), sm = s_sm[t];
         HistCell* hs = cells(level, sm, ch);
         if (s_cpre[t + 1] - s_cpre[t] > 1u || s_empty[t]) {  // a parent at the next level
 #pragma unroll
@@ -802,32 +817,47 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
     // form the terms (three divisions each) into LDS, then the candidate's
     // thread adds them.
     if (FIXED) {
-      float* s_term = reinterpret_cast<float*>(&s_acc[0][0][0]);               // [32][256]
-      uint8_t* s_use = reinterpret_cast<uint8_t*>(s_term + kScanNodesPerPass * 256);  // [32][256]
-      for (uint32_t i = tid; i < cs * 256u; i += TH) {
-        const uint32_t k = i >> 8, b = i & 255u;
-        bool use = b + 1 < mb && !(level > 0 && s_N[k] == 0);
-        float term = 0.f;
-        if (use) {
+      // part p forms the terms of units [8p, 8p + 8) in registers, then the
+      // parts extend the candidate's sequence in order
+      constexpr uint32_t kPer = kScanNodesPerPass / Q;
+      float term[kPer];
+      uint32_t use = 0;
+#pragma unroll
+      for (uint32_t u = 0; u < kPer; ++u) {
+        const uint32_t k = part * kPer + u;
+        term[u] = 0.f;
+        if (k < cs && bin + 1 < mb && !(level > 0 && s_N[k] == 0)) {
           const float G = s_G[k], H = s_H[k];
-          const float2 Lf = s_pref[k][b];
+          const float2 Lf = s_pref[k][bin];
           const float rg = __fsub_rn(G, Lf.x), rh = __fsub_rn(H, Lf.y);
-          use = !(Lf.y < 1e-10f || rh < 1e-10f);
-          const float pg = H > 1e-10f ? __fdiv_rn(__fmul_rn(G, G), __fadd_rn(H, lambda)) : 0.f;
-          const float lgn = __fdiv_rn(__fmul_rn(Lf.x, Lf.x), __fadd_rn(Lf.y, lambda));
-          const float rgn = __fdiv_rn(__fmul_rn(rg, rg), __fadd_rn(rh, lambda));
-          term = __fsub_rn(__fadd_rn(lgn, rgn), pg);
+          if (!(Lf.y < 1e-10f || rh < 1e-10f)) {
+            const float pg = H > 1e-10f ? __fdiv_rn(__fmul_rn(G, G), __fadd_rn(H, lambda)) : 0.f;
+            const float lgn = __fdiv_rn(__fmul_rn(Lf.x, Lf.x), __fadd_rn(Lf.y, lambda));
+            const float rgn = __fdiv_rn(__fmul_rn(rg, rg), __fadd_rn(rh, lambda));
+            term[u] = __fsub_rn(__fadd_rn(lgn, rgn), pg);
+            use |= 1u << u;
+          }
         }
-        s_term[i] = term;
-        s_use[i] = use ? 1 : 0;
       }
-      __syncthreads();
-      if (part == 0) {
-        for (uint32_t k = 0; k < cs; ++k) {
-          if (!s_use[k * 256u + bin]) continue;
-          total = __fmaf_rn(0.5f, s_term[k * 256u + bin], total);
-          any = true;
+      for (uint32_t p = 0; p < Q; ++p) {
+        if (part == p && p * kPer < cs) {
+          float tt = p == 0 ? total : s_chain[bin];
+          bool aa = p == 0 ? any : s_chain_any[bin] != 0;
+#pragma unroll
+          for (uint32_t u = 0; u < kPer; ++u) {
+            if ((use >> u) & 1u) {
+              tt = __fmaf_rn(0.5f, term[u], tt);
+              aa = true;
+            }
+          }
+          s_chain[bin] = tt;
+          s_chain_any[bin] = aa ? 1 : 0;
         }
+        __syncthreads();
+      }
+      if (part == 0) {
+        total = s_chain[bin];
+        any = s_chain_any[bin] != 0;
       }
     } else if (bin + 1 < mb) {
       for (uint32_t k = part; k < cs; k += Q) {
