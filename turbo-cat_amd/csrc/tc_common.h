@@ -85,6 +85,8 @@ struct JobDev {
   uint32_t N, F, Fp, T;     // Fp = rm_stride(F), T = feature_tiles(F)
   uint32_t n_root;          // rows of the tree's root list (N, or the row sample)
   uint32_t root_identity;   // 1: the root list is 0..N-1 (no sampling), rows[0] not read
+  uint32_t fused_root;      // 1: the gradient update writes the root partials (K = 1, no
+                            // sampling, stored histograms); k_root_partials skips the job
   // hyper-parameters
   int32_t loss;
   uint32_t K;               // gradient channels: classes for softmax, else 1

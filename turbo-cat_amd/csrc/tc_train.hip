@@ -188,6 +188,11 @@ class Trainer {
     hipLaunchKernelGGL(k_set_gh, dim3(std::max(1u, ceil_div(N, 256))), dim3(256), 0,
                        ctx_->stream, d_jobs_, dg, dh);
     forced_ = true;  // arbitrary h: no packed histogram cells
+    // the root partials now come from these gradients, not from a gradient update
+    h_jobs_[0].fused_root = 0;
+    root_partials_ = true;
+    TC_HIP(hipMemcpyAsync(d_jobs_, h_jobs_.data(), sizeof(JobDev) * njobs_, hipMemcpyHostToDevice,
+                          ctx_->stream));
     TC_HIP(hipGetLastError());
     TC_HIP(hipStreamSynchronize(ctx_->stream));
     dfree(dg);
@@ -199,8 +204,9 @@ class Trainer {
   void enqueue_tree(std::vector<std::pair<hipEvent_t, hipEvent_t>>* hist_events) {
     hipStream_t s = ctx_->stream;
     const bool packed = packed_hist();
-    hipLaunchKernelGGL(k_root_partials, dim3(total_grad_wg_), dim3(256), 0, s, d_jobs_,
-                       d_job_wg_off_, njobs_);
+    if (root_partials_)  // otherwise the gradient update wrote every job's root partials
+      hipLaunchKernelGGL(k_root_partials, dim3(total_grad_wg_), dim3(256), 0, s, d_jobs_,
+                         d_job_wg_off_, njobs_);
     hipLaunchKernelGGL(k_tree_begin, dim3(1, njobs_), dim3(256), 0, s, d_jobs_, lists_,
                        packed ? 1 : 0);
     for (uint32_t d = 0; d < D_; ++d) {
@@ -605,7 +611,11 @@ class Trainer {
       size_t free_b = 0, total_b = 0;
       TC_HIP(hipMemGetInfo(&free_b, &total_b));
       if (bytes <= free_b / 4) partials_ = alloc<HistCell>(static_cast<size_t>(cap) * kTileF * 256);
-      for (JobDev& J : h_jobs_) J.store_hist = partials_ ? 1u : 0u;
+      for (JobDev& J : h_jobs_) {
+        J.store_hist = partials_ ? 1u : 0u;
+        J.fused_root = J.store_hist && J.root_identity && J.K == 1 ? 1u : 0u;
+        if (!J.fused_root) root_partials_ = true;
+      }
     }
     for (uint32_t d = 0; d < D_; ++d) {
       lists_.hist[d] = alloc<HistWork>(std::max(1u, list_cap_[d]));
@@ -664,6 +674,7 @@ class Trainer {
   uint32_t total_grad_wg_ = 0;
   Lists lists_{};
   HistCell* partials_ = nullptr;        // [item][16][256] of the level being histogrammed
+  bool root_partials_ = false;          // some job needs k_root_partials
   uint32_t hist_cap_[kMaxDepth] = {}, part_cap_[kMaxDepth] = {};
   std::vector<void*> allocs_;
   hipGraphExec_t graph_exec_ = nullptr;

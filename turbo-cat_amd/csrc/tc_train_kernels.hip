@@ -133,6 +133,26 @@ __device__ inline void block_max2_to(TreeState* st, float a, float b) {
 // for the next tree's histogram scales, and empties the work lists.
 constexpr int kGradTile = 1024;
 
+// One 256-thread workgroup's root partial (sum g, sum h of its 1024-row tile,
+// channel c): every thread's double sum of its rows in order, a fixed
+// shuffle tree per wave, the 4 waves added in order.
+__device__ inline void root_partial_store(const JobDev& J, uint32_t lw, uint32_t c, double sg,
+                                          double sh) {
+  __shared__ double s_red[2][4];
+  const uint32_t tid = threadIdx.x;
+  sg = warp_sum_d(sg);
+  sh = warp_sum_d(sh);
+  if ((tid & 63) == 0) { s_red[0][tid >> 6] = sg; s_red[1][tid >> 6] = sh; }
+  __syncthreads();
+  if (tid == 0) {
+    double a = 0.0, b = 0.0;
+    for (int w = 0; w < 4; ++w) { a += s_red[0][w]; b += s_red[1][w]; }
+    J.grad_partials[2 * (lw * J.K + c)] = a;
+    J.grad_partials[2 * (lw * J.K + c) + 1] = b;
+  }
+  __syncthreads();
+}
+
 __device__ inline uint32_t job_of_block(const uint32_t* job_wg_off, uint32_t njobs) {
   uint32_t lo = 0, hi = njobs;  // last job with job_wg_off[j] <= blockIdx.x
   while (hi - lo > 1) {
@@ -212,6 +232,22 @@ __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ 
   const uint32_t lw = blockIdx.x - J.grad_wg_off;
   const uint32_t N = J.N;
   const uint32_t vecN = (N / 8u) * 8u;
+  // each lane takes rows r + 256k (k < 4); every load -- predictions,
+  // labels, and the split column of each level -- is issued before any use
+  constexpr int kR = kGradTile / 256;
+  uint32_t row[kR], li[kR];
+  bool ok[kR];
+  float prev[kR], lab[kR];
+  if (J.K == 1) {
+#pragma unroll
+    for (int k = 0; k < kR; ++k) {
+      row[k] = lw * kGradTile + k * 256 + tid;
+      ok[k] = row[k] < N;
+      li[k] = 0;
+      prev[k] = (ok[k] && mode == 1) ? J.preds[row[k]] : 0.f;
+      lab[k] = ok[k] ? J.y[row[k]] : 0.f;
+    }
+  }
   int t = -1, depth = 0;
   if (mode == 1) {
     t = J.st->tree;
@@ -248,34 +284,27 @@ __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ 
     grad_update_softmax(J, lw, mode, depth, s_f, s_b, s_leaf);
     return;
   }
-  // each lane takes rows r + 256k (k < 4); all loads are issued before use
-  constexpr int kR = kGradTile / 256;
-  uint32_t row[kR], li[kR];
-  bool ok[kR];
-#pragma unroll
-  for (int k = 0; k < kR; ++k) {
-    row[k] = lw * kGradTile + k * 256 + tid;
-    ok[k] = row[k] < N;
-    li[k] = 0;
-  }
   if (mode == 1) {
-    for (int d = 0; d < depth; ++d) {
-      const uint8_t* col = J.bins_cm + static_cast<size_t>(s_f[d]) * N;
-      const uint8_t thr = s_b[d];
+    uint32_t v[kMaxDepth][kR];
 #pragma unroll
-      for (int k = 0; k < kR; ++k) {
-        const uint8_t v = ok[k] ? col[row[k]] : 0;
-        li[k] = (li[k] << 1) | ((v != 255 && v > thr) ? 1u : 0u);
+    for (int d = 0; d < kMaxDepth; ++d) {
+      if (d < depth) {
+        const uint8_t* col = J.bins_cm + static_cast<size_t>(s_f[d]) * N;
+#pragma unroll
+        for (int k = 0; k < kR; ++k) v[d][k] = ok[k] ? col[row[k]] : 0u;
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < kMaxDepth; ++d) {
+      if (d < depth) {
+        const uint32_t thr = s_b[d];
+#pragma unroll
+        for (int k = 0; k < kR; ++k) li[k] = (li[k] << 1) | ((v[d][k] != 255u && v[d][k] > thr) ? 1u : 0u);
       }
     }
   }
-  float prev[kR], lab[kR];
-#pragma unroll
-  for (int k = 0; k < kR; ++k) {
-    prev[k] = (ok[k] && mode == 1) ? J.preds[row[k]] : 0.f;
-    lab[k] = ok[k] ? J.y[row[k]] : 0.f;
-  }
   float gmax = 0.f, hmax = 0.f;
+  double sg = 0.0, sh = 0.0;  // the root partials of this tile (k_root_partials' order)
 #pragma unroll
   for (int k = 0; k < kR; ++k) {
     if (!ok[k]) continue;
@@ -285,7 +314,10 @@ __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ 
     J.gh[row[k]] = gh;
     gmax = fmaxf(gmax, fabsf(gh.x));
     hmax = fmaxf(hmax, fabsf(gh.y));
+    sg += gh.x;
+    sh += gh.y;
   }
+  if (J.fused_root) root_partial_store(J, lw, 0, sg, sh);
   block_max2_to(J.st, gmax, hmax);
 }
 
@@ -386,11 +418,11 @@ __global__ void __launch_bounds__(256) k_root_partials(const JobDev* __restrict_
                                                        const uint32_t* __restrict__ job_wg_off,
                                                        uint32_t njobs) {
   __shared__ uint32_t s_job;
-  __shared__ double s_red[2][4];
   const uint32_t tid = threadIdx.x;
   if (tid == 0) s_job = job_of_block(job_wg_off, njobs);
   __syncthreads();
   const JobDev& J = jobs[s_job];
+  if (J.fused_root) return;  // written by the gradient update
   const uint32_t lw = blockIdx.x - J.grad_wg_off, K = J.K;
   const bool live = J.st->tree + 1 < static_cast<int>(J.n_trees);
   if (live && !J.store_hist) {
@@ -411,17 +443,7 @@ __global__ void __launch_bounds__(256) k_root_partials(const JobDev* __restrict_
         sh += v.y;
       }
     }
-    sg = warp_sum_d(sg);
-    sh = warp_sum_d(sh);
-    if ((tid & 63) == 0) { s_red[0][tid >> 6] = sg; s_red[1][tid >> 6] = sh; }
-    __syncthreads();
-    if (tid == 0) {
-      double a = 0.0, b = 0.0;
-      for (int w = 0; w < 4; ++w) { a += s_red[0][w]; b += s_red[1][w]; }
-      J.grad_partials[2 * (lw * K + c)] = a;
-      J.grad_partials[2 * (lw * K + c) + 1] = b;
-    }
-    __syncthreads();
+    root_partial_store(J, lw, c, sg, sh);
   }
 }
 
