@@ -490,8 +490,7 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
   uint32_t li;
   if (!xcd_rank(blockIdx.x, *count, &li)) return;
   const HistWork w = work[li];
-  const JobDev& J = jobs[w.job];
-  if (!J.st->active) return;
+  const JobDev& J = jobs[w.job];  // items exist only while the job's tree grows
   const uint32_t tid = threadIdx.x;
   for (uint32_t i = tid; i < kCells; i += kHistThreads) {
     s_a[i] = 0ull;
@@ -1016,6 +1015,13 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(const JobDev* __rest
   const PartWork w = work[blockIdx.x];
   const JobDev& J = jobs[w.job];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  // the item's row ids do not depend on the split: in flight while it is decoded
+  uint32_t rv[kIt];
+#pragma unroll
+  for (int k = 0; k < kIt; ++k) {
+    const uint32_t j = k * kPartThreads + tid;
+    rv[k] = j < w.len ? level_row(J, level, w.start + j) : 0u;
+  }
   if (!J.st->active || J.st->tree >= static_cast<int>(J.n_trees)) return;
   uint32_t feat, split_bin;
   if (!decode_split(J, level, w.slot == 0 && tid == 0, &feat, &split_bin)) return;
@@ -1029,13 +1035,11 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(const JobDev* __rest
   __shared__ uint32_t s_base[2];
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   // route every row of the item, keeping row ids and directions in registers
-  uint32_t rv[kIt];
   uint32_t lbits = 0, rbits = 0;
 #pragma unroll
   for (int k = 0; k < kIt; ++k) {
     const uint32_t j = k * kPartThreads + tid;
     const bool valid = j < w.len;
-    rv[k] = valid ? level_row(J, level, w.start + j) : 0u;
     const uint8_t b = valid ? col[rv[k]] : 0;
     const bool left = valid && (b == 255 || b <= thr);
     const bool right = valid && !left;
@@ -1085,11 +1089,23 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
                                                    uint32_t level) {
   const uint32_t job = blockIdx.y, tid = threadIdx.x;
   const JobDev& J = jobs[job];
-  if (!part_active(J, level)) return;
-  const uint32_t nn = 1u << level;  // <= 512
+  const uint32_t nn = 1u << level;  // <= 512: two nodes per thread at most
   const uint32_t stride = (1u << J.D) + 1u;
   const uint32_t* offd = J.off + static_cast<size_t>(level) * stride;
   uint32_t* offn = J.off + static_cast<size_t>(level + 1) * stride;
+  // the nodes' segments and left cursors, in flight while the state is checked
+  uint32_t p_lo[2] = {0, 0}, p_hi[2] = {0, 0}, p_cur[2] = {0, 0};
+#pragma unroll
+  for (uint32_t i = 0; i < 2; ++i) {
+    const uint32_t n = tid + 256u * i;
+    if (n < nn) {
+      p_lo[i] = offd[n];
+      p_hi[i] = offd[n + 1];
+      p_cur[i] = J.cursor[2 * n];
+    }
+  }
+  const uint32_t p_end = tid == 0 ? offd[nn] : 0u;
+  if (!part_active(J, level)) return;
   __shared__ uint32_t s_hw[1 << (kMaxDepth - 1)];    // per pair: hist work items -> first item
   __shared__ uint32_t s_pw[1 << kMaxDepth];          // per child: partition slots -> first slot
   __shared__ uint32_t s_off[(1 << kMaxDepth) + 1];   // child offsets
@@ -1100,9 +1116,12 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
   if (tid == 0) { s_any = 0; s_rows = 0ull; }
   __syncthreads();
   const float mcw2 = J.min_child_weight * 2.0f;
-  for (uint32_t n = tid; n < nn; n += 256) {
-    const uint32_t len = offd[n + 1] - offd[n];
-    const uint32_t acc = J.cursor[2 * n] - offd[n];  // rows sent left
+#pragma unroll
+  for (uint32_t i = 0; i < 2; ++i) {
+    const uint32_t n = tid + 256u * i;
+    if (n >= nn) break;
+    const uint32_t len = p_hi[i] - p_lo[i];
+    const uint32_t acc = p_cur[i] - p_lo[i];  // rows sent left
     const uint32_t cl = heap_index(level + 1, 2 * n);
     J.nc[cl] = acc;
     J.nc[cl + 1] = len - acc;
@@ -1114,10 +1133,10 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
         J.nh[(cl + 1) * J.K + c] = 0.f;
       }
     }
-    s_off[2 * n] = offd[n];
-    s_off[2 * n + 1] = offd[n] + acc;
-    offn[2 * n] = offd[n];
-    offn[2 * n + 1] = offd[n] + acc;
+    s_off[2 * n] = p_lo[i];
+    s_off[2 * n + 1] = p_lo[i] + acc;
+    offn[2 * n] = p_lo[i];
+    offn[2 * n + 1] = p_lo[i] + acc;
     // stop test of the next level (symmetric_tree.cpp:115-124; H sums the
     // channels in class order)
     float hl = 0.f, hr = 0.f;
@@ -1137,8 +1156,8 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
     if (sm_len) atomicAdd(&s_rows, static_cast<unsigned long long>(sm_len));
   }
   if (tid == 0) {
-    offn[2 * nn] = offd[nn];
-    s_off[2 * nn] = offd[nn];
+    offn[2 * nn] = p_end;
+    s_off[2 * nn] = p_end;
   }
   __syncthreads();
   if (level + 1 >= J.D || !s_any) {
