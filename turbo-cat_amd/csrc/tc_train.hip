@@ -631,6 +631,7 @@ class Trainer {
     for (int d = 0; d < kMaxDepth; ++d) {
       lists_.hist_count[d] = counters + d;
       lists_.part_count[d] = counters + kMaxDepth + d;
+      lists_.single = njobs_ == 1 ? 1u : 0u;
     }
     total_grad_wg_ = wg_off[njobs_];
     d_job_wg_off_ = alloc<uint32_t>(njobs_ + 1);

@@ -33,7 +33,18 @@ struct Lists {
   uint32_t* hist_count[kMaxDepth];
   PartWork* part[kMaxDepth];
   uint32_t* part_count[kMaxDepth];
+  uint32_t single;  // one job: its items start at 0 and the counts are stored, not reserved
 };
+
+// Room for n items in a level list: with one job the list is the job's
+// (store the count, no round trip); otherwise reserve with an atomic.
+__device__ inline uint32_t list_reserve(const Lists& L, uint32_t* count, uint32_t n) {
+  if (L.single) {
+    *count = n;
+    return 0u;
+  }
+  return atomicAdd(count, n);
+}
 
 __device__ inline uint32_t part_slots(uint32_t len) {
   return (len + kPartChunk - 1) / kPartChunk;
@@ -333,6 +344,14 @@ __device__ void tree_begin(const JobDev& J, uint32_t job, const Lists& L, int pa
   __shared__ double s_gh[2 * kMaxClasses];
   const uint32_t n = J.n_root;
   const uint32_t hcnt = ((n + J.hist_chunk - 1) / J.hist_chunk) * J.T * K, pcnt = part_slots(n);
+  // the tree state, in flight while the partials are summed
+  int prev_tree = 0;
+  uint32_t gbits = 0, hbits = 0;
+  if (tid == 0) {
+    prev_tree = J.st->tree;
+    gbits = J.st->gmax_next;
+    hbits = J.st->hmax_next;
+  }
   for (uint32_t c = 0; c < K; ++c) {  // thread t adds tiles t, t + 256, ... in order; then a fixed tree
     double a = 0.0, b = 0.0;
     for (uint32_t w = tid; w < J.grad_wg_cnt; w += 256) {
@@ -351,11 +370,11 @@ __device__ void tree_begin(const JobDev& J, uint32_t job, const Lists& L, int pa
   }
   if (tid == 0) {
     TreeState* st = J.st;
-    const int t = st->tree + 1;
+    const int t = prev_tree + 1;
     st->tree = t;
     st->depth = 0;
     for (int d = 0; d < kMaxDepth; ++d) st->best[d] = kEmptyKey;
-    const float gmax = __uint_as_float(st->gmax_next), hmax = __uint_as_float(st->hmax_next);
+    const float gmax = __uint_as_float(gbits), hmax = __uint_as_float(hbits);
     st->gmax_next = 0u;
     st->hmax_next = 0u;
     int go = 0;
@@ -380,8 +399,8 @@ __device__ void tree_begin(const JobDev& J, uint32_t job, const Lists& L, int pa
       st->hscale = ldexp(1.0, eh);
       st->hinv = ldexp(1.0, -eh);
       if (go) {
-        s_hbase = atomicAdd(L.hist_count[0], hcnt);
-        s_pbase = atomicAdd(L.part_count[0], pcnt);
+        s_hbase = list_reserve(L, L.hist_count[0], hcnt);
+        s_pbase = list_reserve(L, L.part_count[0], pcnt);
         J.hfirst[0] = s_hbase;
         J.lvl_chunk[0] = J.hist_chunk;
         J.cursor[0] = 0;
@@ -1178,8 +1197,8 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
   const uint32_t htot = block_scan_excl(s_hw, nn, s_tmp);
   const uint32_t ptot = block_scan_excl(s_pw, 2 * nn, s_tmp);
   if (tid == 0) {
-    s_hbase = atomicAdd(L.hist_count[level + 1], htot);
-    s_pbase = atomicAdd(L.part_count[level + 1], ptot);
+    s_hbase = list_reserve(L, L.hist_count[level + 1], htot);
+    s_pbase = list_reserve(L, L.part_count[level + 1], ptot);
     if (J.hist_rows_total) atomicAdd(J.hist_rows_total, s_rows);
   }
   __syncthreads();
