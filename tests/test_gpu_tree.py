@@ -245,3 +245,33 @@ def test_work_item_shape_never_changes_the_model(ctx, loss):
     for m in models[1:]:
         for k in ("feature", "bin", "fthr", "gain", "depth", "leaves", "train_preds"):
             assert np.array_equal(m[k], models[0][k]), k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("loss,K", [(0, 1), (1, 1), (2, 3)])
+def test_atomic_histogram_mode_gives_the_same_model(ctx, loss, K, monkeypatch):
+    """The two histogram flushes -- stored cells with partial slots summed by
+    the split scan (default), and int64 atomics into zeroed node histograms
+    (TURBOCAT_HIST_ATOMIC=1, the fallback when the slots would not fit) --
+    produce bit-identical models: every histogram is an exact integer sum."""
+    from turbocat import capi
+    if K > 1:
+        rng = np.random.default_rng(7)
+        X = rng.standard_normal((40000, 20)).astype(np.float32)
+        y = (np.argmax(X[:, :K] + 0.3 * rng.standard_normal((40000, K)), axis=1)).astype(np.float32)
+    else:
+        X, y = regression_data(40000, 20, 9) if loss == 0 else binary_data(40000, 20, 9)
+    ds, _, _ = ctx.quantize_fit(X)
+    models = []
+    for mode in ("0", "1"):
+        monkeypatch.setenv("TURBOCAT_HIST_ATOMIC", mode)
+        prm = capi.train_params(loss, n_trees=12, learning_rate=0.1, max_depth=6,
+                                lambda_l2=1.0, hist_chunk=2048, hist_deep_items=64,
+                                n_classes=K if K > 1 else 0)
+        tr = capi.Trainer(ctx, [ds], [y], prm)
+        tr.step(12)
+        tr.sync()
+        models.append(tr.model())
+        tr.close()
+    for k in ("feature", "bin", "fthr", "gain", "depth", "leaves", "train_preds"):
+        assert np.array_equal(models[0][k], models[1][k]), k

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstddef>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -610,7 +611,11 @@ class Trainer {
       const size_t bytes = static_cast<size_t>(cap) * kTileF * 256 * sizeof(HistCell);
       size_t free_b = 0, total_b = 0;
       TC_HIP(hipMemGetInfo(&free_b, &total_b));
-      if (bytes <= free_b / 4) partials_ = alloc<HistCell>(static_cast<size_t>(cap) * kTileF * 256);
+      // TURBOCAT_HIST_ATOMIC=1 selects the atomic mode (same results; tests use it)
+      const char* force = std::getenv("TURBOCAT_HIST_ATOMIC");
+      const bool atomic = force != nullptr && force[0] == '1';
+      if (!atomic && bytes <= free_b / 4)
+        partials_ = alloc<HistCell>(static_cast<size_t>(cap) * kTileF * 256);
       for (JobDev& J : h_jobs_) {
         J.store_hist = partials_ ? 1u : 0u;
         J.fused_root = J.store_hist && J.root_identity && J.K == 1 ? 1u : 0u;

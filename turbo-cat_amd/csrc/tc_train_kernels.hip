@@ -1030,8 +1030,10 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(const JobDev* __rest
                                                    uint32_t level) {
   constexpr int kIt = kPartChunk / kPartThreads;  // rows per lane
   constexpr int kW = kPartThreads / 64;           // waves
-  if (blockIdx.x >= *count) return;
+  // the list is read with its count (the grid never exceeds the list's capacity)
+  const uint32_t cnt = *count;
   const PartWork w = work[blockIdx.x];
+  if (blockIdx.x >= cnt) return;
   const JobDev& J = jobs[w.job];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   // the item's row ids do not depend on the split: in flight while it is decoded
