@@ -92,8 +92,39 @@ class Oracle:
                                   POINTER(TreeParams), V, V, V, V, V, V, V]
         L.tco_fit_f32.restype = c_float
         L.tco_fma_update.argtypes = [V, V, c_float, c_uint32]
+        L.tco_compute_loss_f32.argtypes = [c_int, V, V, c_uint32]
+        L.tco_compute_loss_f32.restype = c_float
+        L.tco_compute_loss_f64.argtypes = [c_int, V, V, c_uint32]
+        L.tco_compute_loss_f64.restype = c_double
+        L.tco_compute_loss_mc_f32.argtypes = [V, V, c_uint32, c_uint32]
+        L.tco_compute_loss_mc_f32.restype = c_float
+        L.tco_early_stop.argtypes = [V, c_uint32, c_uint32, c_uint32, c_int, V, V]
+        L.tco_early_stop.restype = c_uint32
         for fn in ("tco_tree_build_f32", "tco_tree_build_f64", "tco_tree_build_mc_f64"):
             getattr(L, fn).restype = c_int
+
+    # ---------------------------------------------------------------- A19
+    def compute_loss_f32(self, loss, y, p):
+        """Loss::compute_loss (mean per-row loss, float, row order)."""
+        y = np.ascontiguousarray(y, np.float32); p = np.ascontiguousarray(p, np.float32)
+        return float(self.L.tco_compute_loss_f32(loss, _p(y), _p(p), len(y)))
+
+    def compute_loss_f64(self, loss, y, p):
+        y = np.ascontiguousarray(y, np.float32); p = np.ascontiguousarray(p, np.float32)
+        return float(self.L.tco_compute_loss_f64(loss, _p(y), _p(p), len(y)))
+
+    def compute_loss_mc_f32(self, y, p):
+        """CrossEntropyLoss::compute_loss; p row-major n x K."""
+        y = np.ascontiguousarray(y, np.float32); p = np.ascontiguousarray(p, np.float32)
+        return float(self.L.tco_compute_loss_mc_f32(_p(y), _p(p), p.shape[0], p.shape[1]))
+
+    def early_stop(self, metric, n_estimators, rounds, higher_is_better=False):
+        """Trees kept, best iteration and best metric under the early-stopping rule."""
+        m = np.ascontiguousarray(metric, np.float32)
+        bi = np.zeros(1, np.uint32); b = np.zeros(1, np.float32)
+        kept = self.L.tco_early_stop(_p(m), len(m), n_estimators, rounds, int(higher_is_better),
+                                     _p(bi), _p(b))
+        return int(kept), int(bi[0]), float(b[0])
 
     # ---------------------------------------------------------------- A2/A3
     def quantize(self, X, max_bins=255, wrap=True):
@@ -312,6 +343,10 @@ class RefLib:
         L.tcref_gradients_mc.argtypes = [V, V, V, V, c_uint32, c_uint32]
         L.tcref_init_prediction.argtypes = [c_int, V, c_uint32]
         L.tcref_init_prediction.restype = c_float
+        L.tcref_compute_loss.argtypes = [c_int, V, V, c_uint32]
+        L.tcref_compute_loss.restype = c_float
+        L.tcref_compute_loss_mc.argtypes = [V, V, c_uint32, c_uint32]
+        L.tcref_compute_loss_mc.restype = c_float
         L.tcref_hist.argtypes = [V, V, V, V, c_uint32, V, V]
         L.tcref_tree_build.argtypes = [V, V, V, c_uint32, c_float, c_float, V, V, V, V, V, V]
         L.tcref_tree_build.restype = c_int
@@ -367,6 +402,14 @@ class RefLib:
     def init_prediction(self, loss, y):
         y = np.ascontiguousarray(y, np.float32)
         return float(self.L.tcref_init_prediction(loss, _p(y), len(y)))
+
+    def compute_loss(self, loss, y, p):
+        y = np.ascontiguousarray(y, np.float32); p = np.ascontiguousarray(p, np.float32)
+        return float(self.L.tcref_compute_loss(loss, _p(y), _p(p), len(y)))
+
+    def compute_loss_mc(self, y, p):
+        y = np.ascontiguousarray(y, np.float32); p = np.ascontiguousarray(p, np.float32)
+        return float(self.L.tcref_compute_loss_mc(_p(y), _p(p), p.shape[0], p.shape[1]))
 
     def gradients_mc(self, y, p):
         """CrossEntropyLoss::compute_multiclass_gradients; p row-major n x K."""

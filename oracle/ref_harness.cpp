@@ -24,6 +24,7 @@
 #include <chrono>
 #include <cstdint>
 #include <cstring>
+#include <omp.h>
 #include <memory>
 #include <numeric>
 #include <vector>
@@ -108,6 +109,24 @@ void tcref_gradients_mc(const float* y, const float* p, float* g, float* h, uint
                         uint32_t K) {
     CrossEntropyLoss loss(K);
     loss.compute_multiclass_gradients(y, p, g, h, n);
+}
+
+// Loss::compute_loss (loss.cpp:110-121, :232-244, :336-346) on one OpenMP
+// thread, so the float reduction runs in row order.
+float tcref_compute_loss(int loss_id, const float* y, const float* p, uint32_t n) {
+    LossConfig lc;
+    lc.loss_type = loss_id == 0 ? LossType::MSE : LossType::LogLoss;
+    auto loss = Loss::create(lc, loss_id == 0 ? TaskType::Regression : TaskType::BinaryClassification, 2);
+    const int threads = omp_get_max_threads();
+    omp_set_num_threads(1);
+    const float v = loss->compute_loss(y, p, n);
+    omp_set_num_threads(threads);
+    return v;
+}
+
+float tcref_compute_loss_mc(const float* y, const float* p, uint32_t n, uint32_t K) {
+    CrossEntropyLoss loss(K);
+    return loss.compute_loss(y, p, n);
 }
 
 float tcref_init_prediction(int loss_id, const float* y, uint32_t n) {

@@ -467,3 +467,83 @@ float tco_fit_f32(const uint8_t* bins, uint32_t n, uint32_t F, const float* edge
     free(g); free(h); free(li);
     return base;
 }
+
+/* ---------------------------------------------------------------- A19 */
+
+/* Loss::compute_loss, the mean per-row loss after each tree (booster.cpp:252
+ * for the training rows, :283 for the validation rows), float accumulation in
+ * row order (the reference's OpenMP reduction on one thread):
+ * MSELoss (loss.cpp:336-346): (p - y)^2; LogLoss (:110-121):
+ * -y log q - (1 - y) log(1 - q), q = clip(1 / (1 + exp(-p)), 1e-7)
+ * (loss.hpp:99-105). */
+float tco_compute_loss_f32(int loss, const float* y, const float* p, uint32_t n) {
+    float s = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (loss == 0) {
+            const float d = p[i] - y[i];
+            s += d * d;
+        } else {
+            float q = 1.0f / (1.0f + expf(-p[i]));
+            q = q < 1e-7f ? 1e-7f : (q > 1.0f - 1e-7f ? 1.0f - 1e-7f : q);
+            s += -y[i] * logf(q) - (1 - y[i]) * logf(1 - q);
+        }
+    }
+    return s / (float)n;
+}
+
+double tco_compute_loss_f64(int loss, const float* y, const float* p, uint32_t n) {
+    double s = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (loss == 0) {
+            const double d = (double)p[i] - (double)y[i];
+            s += d * d;
+        } else {
+            double q = 1.0 / (1.0 + exp(-(double)p[i]));
+            q = q < 1e-7 ? 1e-7 : (q > 1.0 - 1e-7 ? 1.0 - 1e-7 : q);
+            s += -(double)y[i] * log(q) - (1.0 - (double)y[i]) * log(1.0 - q);
+        }
+    }
+    return s / (double)n;
+}
+
+/* CrossEntropyLoss::compute_loss (loss.cpp:232-244): p row-major n x K,
+ * softmax with the max shift, -log(max(q[label], 1e-7)), float, row order. */
+float tco_compute_loss_mc_f32(const float* y, const float* p, uint32_t n, uint32_t K) {
+    float s = 0.0f;
+    float e[64];
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = p + (size_t)i * K;
+        float m = r[0];
+        for (uint32_t c = 1; c < K; ++c) m = r[c] > m ? r[c] : m;
+        float sum = 0.0f;
+        for (uint32_t c = 0; c < K; ++c) { e[c] = expf(r[c] - m); sum += e[c]; }
+        const uint32_t lab = (uint32_t)y[i];
+        const float q = e[lab] / sum;
+        s += -logf(q > 1e-7f ? q : 1e-7f);
+    }
+    return s / (float)n;
+}
+
+/* Early stopping over a sequence of per-tree validation metrics
+ * (booster.cpp:115-121, :326-352): improved = metric < best - 1e-10 (or
+ * > best + 1e-10 when higher is better), best starting at +-1e30; after an
+ * improvement the count resets, else it grows; the loop breaks after tree
+ * iter when iter >= max(50, n_estimators / 10) and the count has reached
+ * `rounds`.  Returns the trees kept (iter + 1 at the break, else n_iter);
+ * best_iter / best get the last improvement. */
+uint32_t tco_early_stop(const float* metric, uint32_t n_iter, uint32_t n_estimators,
+                        uint32_t rounds, int higher_is_better, uint32_t* best_iter, float* best) {
+    float b = higher_is_better ? -1e30f : 1e30f;
+    uint32_t bi = 0, count = 0;
+    const uint32_t min_it = n_estimators / 10 > 50u ? n_estimators / 10 : 50u;
+    uint32_t kept = n_iter;
+    for (uint32_t it = 0; it < n_iter; ++it) {
+        const float v = metric[it];
+        const int improved = higher_is_better ? (v > b + 1e-10f) : (v < b - 1e-10f);
+        if (improved) { b = v; bi = it; count = 0; } else { ++count; }
+        if (it >= min_it && count >= rounds) { kept = it + 1; break; }
+    }
+    if (best_iter) *best_iter = bi;
+    if (best) *best = b;
+    return kept;
+}

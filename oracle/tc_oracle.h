@@ -142,6 +142,13 @@ float tco_fit_f32(const uint8_t* bins, uint32_t n, uint32_t F, const float* edge
                   float lr, const tco_tree_params* p, uint16_t* feature, uint8_t* bin,
                   float* fthr, float* gain, uint16_t* depth, float* leaves, float* train_preds);
 
+/* A19: per-tree losses and the early-stopping rule (booster.cpp:252-352). */
+float tco_compute_loss_f32(int loss, const float* y, const float* p, uint32_t n);
+double tco_compute_loss_f64(int loss, const float* y, const float* p, uint32_t n);
+float tco_compute_loss_mc_f32(const float* y, const float* p, uint32_t n, uint32_t K);
+uint32_t tco_early_stop(const float* metric, uint32_t n_iter, uint32_t n_estimators,
+                        uint32_t rounds, int higher_is_better, uint32_t* best_iter, float* best);
+
 #ifdef __cplusplus
 }
 #endif

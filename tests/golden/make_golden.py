@@ -64,9 +64,28 @@ def make_ce(name, n, K, seed):
     np.savez_compressed(os.path.join(HERE, f"{name}.npz"), y=y, p=p, g=g, h=h)
 
 
+def make_loss(name, n, seed):
+    """Loss::compute_loss of MSE, LogLoss (saturated scores included) and
+    CrossEntropyLoss (K = 3) on seeded predictions, one OpenMP thread."""
+    rng = np.random.default_rng(seed)
+    R = RefLib()
+    p = (rng.standard_normal(n) * 4).astype(np.float32)
+    p[:4] = [0.0, 45.0, -45.0, 1e-9]
+    yr = rng.standard_normal(n).astype(np.float32)
+    yb = (rng.random(n) < 0.3).astype(np.float32)
+    pk = (rng.standard_normal((n, 3)) * 3).astype(np.float32)
+    yk = rng.integers(0, 3, n).astype(np.float32)
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), p=p, yr=yr, yb=yb, pk=pk, yk=yk,
+                        mse=np.float32(R.compute_loss(0, yr, p)),
+                        logloss=np.float32(R.compute_loss(1, yb, p)),
+                        ce=np.float32(R.compute_loss_mc(yk, pk)))
+
+
 if __name__ == "__main__":
     for c in CASES:
         RefLib.isolated(make, *c)  # fresh thread: see RefLib.isolated
         print("wrote", c[0])
     RefLib.isolated(make_ce, "ce_grad", 517, 7, 104)
     print("wrote ce_grad")
+    RefLib.isolated(make_loss, "loss_values", 1501, 105)
+    print("wrote loss_values")

@@ -23,7 +23,7 @@ from oracle import LOSS_LOGLOSS, LOSS_MSE, RefLib, tree_params
 
 GOLDEN_DIR = os.path.join(os.path.dirname(__file__), "golden")
 GOLDEN = sorted(p for p in glob.glob(os.path.join(GOLDEN_DIR, "*.npz"))
-                if not os.path.basename(p).startswith("ce_"))
+                if not os.path.basename(p).startswith(("ce_", "loss_")))
 HAVE_REF = RefLib.available()
 
 
@@ -282,3 +282,77 @@ def test_ce_gradients_match_reference(oracle, K):
     rg, rh = RefLib.isolated(lambda: RefLib().gradients_mc(y, p))
     g, h = oracle.gradients_mc_f32(y, p)
     assert np.array_equal(g, rg) and np.array_equal(h, rh)
+
+
+# --------------------------------------------------------------- A19: losses, early stopping
+@pytest.mark.skipif(not HAVE_REF, reason="oracle/_ref/libtcref.so not built (no reference tree)")
+@pytest.mark.parametrize("loss", [0, 1])
+def test_compute_loss_matches_reference(oracle, loss):
+    """Loss::compute_loss restated (mean per-row loss, float, row order) equals the
+    compiled reference's on one OpenMP thread, bit for bit; the FP64 form is
+    within float rounding."""
+    rng = np.random.default_rng(31 + loss)
+    n = 5003
+    p = (rng.standard_normal(n) * 3).astype(np.float32)
+    p[:5] = [0.0, 40.0, -40.0, 1e-8, -17.0]  # clip range of the sigmoid
+    y = (rng.random(n) < 0.4).astype(np.float32) if loss else rng.standard_normal(n).astype(np.float32)
+    a = oracle.compute_loss_f32(loss, y, p)
+    b = RefLib.isolated(lambda: RefLib().compute_loss(loss, y, p))
+    assert a == b
+    assert abs(oracle.compute_loss_f64(loss, y, p) - a) <= 1e-5 * abs(a)
+
+
+@pytest.mark.skipif(not HAVE_REF, reason="oracle/_ref/libtcref.so not built (no reference tree)")
+def test_compute_loss_mc_matches_reference(oracle):
+    rng = np.random.default_rng(5)
+    for K in (2, 3, 10):
+        n = 2001
+        p = (rng.standard_normal((n, K)) * 2).astype(np.float32)
+        y = rng.integers(0, K, n).astype(np.float32)
+        assert oracle.compute_loss_mc_f32(y, p) == RefLib.isolated(lambda: RefLib().compute_loss_mc(y, p))
+
+
+def test_compute_loss_matches_golden(oracle):
+    """The compiled reference's Loss::compute_loss values (tests/golden/
+    make_golden.py), for hosts without the reference tree."""
+    z = np.load(os.path.join(GOLDEN_DIR, "loss_values.npz"), allow_pickle=False)
+    assert np.float32(oracle.compute_loss_f32(0, z["yr"], z["p"])) == z["mse"]
+    assert np.float32(oracle.compute_loss_f32(1, z["yb"], z["p"])) == z["logloss"]
+    assert np.float32(oracle.compute_loss_mc_f32(z["yk"], z["pk"])) == z["ce"]
+
+
+def test_compute_loss_closed_forms(oracle):
+    """MSE of a constant offset d is d^2; LogLoss at p = 0 is ln 2 for any labels;
+    a confident wrong prediction is capped by the 1e-7 clip at -ln(1e-7)."""
+    y = np.linspace(-1, 1, 64).astype(np.float32)
+    assert oracle.compute_loss_f32(0, y, y + np.float32(0.5)) == pytest.approx(0.25, rel=1e-6)
+    yb = (np.arange(64) % 2).astype(np.float32)
+    assert oracle.compute_loss_f32(1, yb, np.zeros(64, np.float32)) == pytest.approx(np.log(2), rel=1e-6)
+    worst = oracle.compute_loss_f32(1, np.ones(4, np.float32), np.full(4, -60.0, np.float32))
+    assert worst == pytest.approx(-np.log(np.float32(1e-7)), rel=1e-5)
+
+
+def test_early_stop_rule_known_answers(oracle):
+    """booster.cpp:326-352: any improvement beyond 1e-10 resets the count; the
+    loop breaks after tree `iter` once iter >= max(50, n_estimators / 10) and
+    `rounds` trees in a row failed to improve."""
+    # strictly improving: never stops, best is the last tree
+    m = np.linspace(1.0, 0.5, 80).astype(np.float32)
+    assert oracle.early_stop(m, 80, 5) == (80, 79, pytest.approx(0.5))
+    # improves until tree 9, flat afterwards: the count reaches 5 at tree 14, but the
+    # break waits for tree 50 (min iterations)
+    m = np.concatenate([np.linspace(1.0, 0.9, 10), np.full(70, 0.9)]).astype(np.float32)
+    kept, best_it, best = oracle.early_stop(m, 80, 5)
+    assert (kept, best_it) == (51, 9) and best == pytest.approx(0.9)
+    # n_estimators = 1000 moves the minimum to tree 100
+    m = np.concatenate([np.linspace(1.0, 0.9, 10), np.full(190, 0.9)]).astype(np.float32)
+    assert oracle.early_stop(m, 1000, 5)[0] == 101
+    # improving through tree 55, four flat trees, an improvement at 60 resets the
+    # count, which then reaches 5 at tree 65
+    m = (1.0 - 0.001 * np.minimum(np.arange(100), 55)).astype(np.float32)
+    m[60:] = m[55] - np.float32(0.01)
+    assert oracle.early_stop(m, 100, 5)[:2] == (66, 60)
+    # an improvement smaller than 1e-10 does not count; higher-is-better flips it
+    m = np.full(70, 0.5, np.float32)
+    assert oracle.early_stop(m, 70, 3)[:2] == (51, 0)
+    assert oracle.early_stop(-m, 70, 3, higher_is_better=True)[:2] == (51, 0)
