@@ -186,6 +186,28 @@ int32_t tc_tree_build_multi(tc_dataset* ds, const float* g, const float* h, uint
 /* A5-A15: Booster::train, symmetric branch without sampling
  * (booster.cpp:47-418, build_tree :438-465). */
 int32_t tc_train(tc_dataset* ds, const float* y, const tc_train_params* p, tc_model* model);
+/* A19: validation set and early stopping (booster.cpp:252-371), the metric
+ * being the loss.  After every tree the validation rows (binned with the
+ * training edges, tc_dataset_create) get preds += lr * tree value and the mean
+ * loss (Loss::compute_loss: MSE or LogLoss) is taken; the fit stops after the
+ * tree `iter` at which iter >= max(50, n_trees / 10) and early_stopping_rounds
+ * trees in a row did not improve the best loss by more than 1e-10.  Trees
+ * after the stop stay zero in the model; the reference keeps every tree up to
+ * and including the stopping one, as here.  valid may be NULL (then only the
+ * training losses are recorded, if asked for).  history: outputs; its
+ * train_loss / valid_loss arrays (n_trees floats, optional) receive the mean
+ * losses after each tree. */
+typedef struct {
+  float* train_loss;          /* in: n_trees floats or NULL */
+  float* valid_loss;          /* in: n_trees floats or NULL */
+  uint32_t n_trees_built;     /* out: trees built (the stopping tree included) */
+  int32_t stopped_early;      /* out: 1 when early stopping ended the fit */
+  uint32_t best_iteration;    /* out: tree of the best validation loss */
+  float best_valid_loss;      /* out */
+} tc_train_history;
+int32_t tc_train_valid(tc_dataset* ds, const float* y, const tc_train_params* p, tc_dataset* valid,
+                       const float* y_valid, uint32_t early_stopping_rounds, tc_model* model,
+                       tc_train_history* history);
 /* Batched form: n_jobs independent fits in lockstep, one launch per kernel
  * for all jobs.  All jobs share max_depth and n_trees; y[j], models[j] per
  * job.  No reference counterpart (the reference fits one model per call). */

@@ -38,6 +38,17 @@ void reject(bool cond, const std::string& what) {
   if (cond) throw std::invalid_argument(what + " is not supported by the MI355X symmetric-tree backend");
 }
 
+// The validation set of fit (an empty X_val means none; bindings.cpp:261-279).
+Matrix validation(const FArray& X_val, const FArray& y_val, uint32_t F) {
+  if (X_val.size() == 0) return Matrix{nullptr, 0, F};
+  const Matrix v = as_matrix(X_val);
+  if (v.F != F) throw std::invalid_argument("X_val has " + std::to_string(v.F) + " features, X has " +
+                                            std::to_string(F));
+  if (y_val.ndim() != 1 || static_cast<uint32_t>(y_val.shape(0)) != v.n)
+    throw std::invalid_argument("X_val and y_val differ in rows");
+  return v;
+}
+
 py::dict importance_dict(const std::vector<float>& imp) {
   double s = 0;
   for (float v : imp) s += v;
@@ -82,9 +93,10 @@ class TurboCatClassifier {
     use_gradtree_ = use_gradtree;
     use_ordered_ = use_ordered_boosting;
     auto_interactions_ = auto_interactions;
+    p_.early_stopping_rounds = static_cast<uint32_t>(std::max(0, early_stopping_rounds));
+    es_metric_ = early_stopping_metric;
     (void)goss_top_rate; (void)goss_other_rate; (void)use_symmetric; (void)grow_policy;
-    (void)early_stopping_rounds; (void)early_stopping_metric; (void)cat_encoding;
-    (void)max_interactions; (void)interaction_method; (void)n_threads;
+    (void)cat_encoding; (void)max_interactions; (void)interaction_method; (void)n_threads;
   }
 
   void fit(FArray X, FArray y, FArray X_val, FArray y_val, std::vector<int> cat_features) {
@@ -97,7 +109,11 @@ class TurboCatClassifier {
     reject(use_ordered_, "use_ordered_boosting");
     reject(auto_interactions_, "auto_interactions");
     reject(!cat_features.empty(), "cat_features");
-    reject(X_val.size() > 0, "validation data (X_val/y_val)");
+    const Matrix v = validation(X_val, y_val, m.F);
+    // the loss is the metric; the reference's other metrics rank probabilities
+    reject(v.n > 0 && (es_metric_ == "roc_auc" || es_metric_ == "auc" || es_metric_ == "pr_auc" ||
+                       es_metric_ == "f1"),
+           "early_stopping_metric='" + es_metric_ + "'");
     reject(!(loss_ == "auto" || loss_ == "logloss" || loss_ == "binary_crossentropy"),
            "loss='" + loss_ + "'");
     float max_label = 0.f;
@@ -110,7 +126,7 @@ class TurboCatClassifier {
       std::printf("[TurboCat/MI355X] fit: n_samples=%u, n_features=%u, n_estimators=%u\n", m.n, m.F,
                   p_.n_estimators);
     py::gil_scoped_release nogil;
-    booster_->fit(m.data, y.data(), m.n, m.F);
+    booster_->fit(m.data, y.data(), m.n, m.F, v.n ? v.data : nullptr, v.n ? y_val.data() : nullptr, v.n);
   }
 
   py::array_t<float> predict_proba(FArray X) { return predict_proba_nobinning_fast(X); }
@@ -184,7 +200,7 @@ class TurboCatClassifier {
     if (!booster_ || !booster_->fitted()) throw std::runtime_error("Model not fitted. Call fit() first.");
   }
   Params p_;
-  std::string mode_, loss_;
+  std::string mode_, loss_, es_metric_ = "loss";
   bool use_goss_ = false, use_gradtree_ = false, use_ordered_ = false, auto_interactions_ = false;
   uint32_t n_classes_ = 2;
   std::unique_ptr<Booster> booster_;
@@ -213,7 +229,8 @@ class TurboCatRegressor {
     p_.loss = TC_LOSS_MSE;
     p_.task = tcamd::kRegression;
     use_goss_ = use_goss;
-    (void)goss_top_rate; (void)goss_other_rate; (void)early_stopping_rounds; (void)max_leaves;
+    p_.early_stopping_rounds = static_cast<uint32_t>(std::max(0, early_stopping_rounds));
+    (void)goss_top_rate; (void)goss_other_rate; (void)max_leaves;
     (void)grow_policy; (void)n_threads;
   }
 
@@ -223,14 +240,14 @@ class TurboCatRegressor {
     if (static_cast<uint32_t>(y.shape(0)) != m.n) throw std::invalid_argument("X and y differ in rows");
     reject(mode_ == "small", "mode='small' (regular trees)");
     reject(use_goss_, "use_goss");
-    reject(X_val.size() > 0, "validation data (X_val/y_val)");
+    const Matrix v = validation(X_val, y_val, m.F);
     reject(!(loss_ == "mse" || loss_ == "l2"), "loss='" + loss_ + "'");
     booster_ = std::make_unique<Booster>(p_);
     if (p_.verbosity > 0)
       std::printf("[TurboCat/MI355X] fit: n_samples=%u, n_features=%u, n_estimators=%u\n", m.n, m.F,
                   p_.n_estimators);
     py::gil_scoped_release nogil;
-    booster_->fit(m.data, y.data(), m.n, m.F);
+    booster_->fit(m.data, y.data(), m.n, m.F, v.n ? v.data : nullptr, v.n ? y_val.data() : nullptr, v.n);
   }
 
   py::array_t<float> predict_raw(FArray X, bool timing) {
@@ -291,7 +308,7 @@ class TurboCatRegressor {
     if (!booster_ || !booster_->fitted()) throw std::runtime_error("Model not fitted. Call fit() first.");
   }
   Params p_;
-  std::string mode_, loss_;
+  std::string mode_, loss_, es_metric_ = "loss";
   bool use_goss_ = false;
   std::unique_ptr<Booster> booster_;
 };

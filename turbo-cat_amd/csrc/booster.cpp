@@ -37,7 +37,8 @@ void Booster::drop_ensemble() {
   }
 }
 
-void Booster::fit(const float* X, const float* y, uint32_t n, uint32_t F) {
+void Booster::fit(const float* X, const float* y, uint32_t n, uint32_t F, const float* Xv,
+                  const float* yv, uint32_t nv) {
   if (n == 0) throw std::invalid_argument("X has no rows");
   if (F == 0) throw std::invalid_argument("X has no features");
   if (p_.max_depth < 1 || p_.max_depth > 10)
@@ -85,14 +86,37 @@ void Booster::fit(const float* X, const float* y, uint32_t n, uint32_t F) {
   m.gain = gain_.data();
   m.depth = depth_.data();
   m.leaves = leaves_.data();
-  const int32_t rc = tc_train(ds, y, &tp, &m);
+  uint32_t built = T;
+  int32_t rc;
+  if (Xv != nullptr && nv > 0) {
+    // Dataset::apply_bins with the training edges (bindings.cpp:261-279)
+    tc_dataset* vds = nullptr;
+    rc = tc_dataset_create(ctx_, Xv, nv, F, edges.data(), ne.data(), p_.max_bins, &vds);
+    if (rc == 0) {
+      tc_train_history h{};
+      rc = tc_train_valid(ds, y, &tp, vds, yv, p_.early_stopping_rounds, &m, &h);
+      built = h.n_trees_built;
+      tc_dataset_destroy(vds);
+    }
+  } else {
+    rc = tc_train(ds, y, &tp, &m);
+  }
   tc_dataset_destroy(ds);
   check(rc);
-  T_ = T;
+  if (built < T) {  // the trees after the stopping one were never built
+    feat_.resize(static_cast<size_t>(built) * D);
+    bin_.resize(static_cast<size_t>(built) * D);
+    fthr_.resize(static_cast<size_t>(built) * D);
+    gain_.resize(static_cast<size_t>(built) * D);
+    depth_.resize(built);
+    leaves_.resize(static_cast<size_t>(built) << D);
+  }
+  const uint32_t T_built = built;
+  T_ = T_built;
   D_ = D;
   F_ = F;
   base_ = m.base;
-  weights_.assign(T, p_.learning_rate);  // add_tree(tree, lr), booster.cpp:465
+  weights_.assign(T_built, p_.learning_rate);  // add_tree(tree, lr), booster.cpp:465
   edges_.assign(F, {});
   for (uint32_t f = 0; f < F; ++f)
     edges_[f].assign(edges.begin() + static_cast<size_t>(f) * 255,

@@ -24,6 +24,7 @@ struct Params {
   float learning_rate = 0.1f;
   float subsample = 1.0f;
   float colsample_bytree = 1.0f;
+  uint32_t early_stopping_rounds = 50;  // with a validation set (booster.cpp:344-346)
   // tree (TreeConfig, config.hpp:26-59)
   uint32_t max_depth = 6;
   uint32_t max_bins = 255;
@@ -53,8 +54,11 @@ class Booster {
   Booster(const Booster&) = delete;
   Booster& operator=(const Booster&) = delete;
 
-  // Quantize X (row-major n x F), train n_estimators symmetric trees.
-  void fit(const float* X, const float* y, uint32_t n, uint32_t F);
+  // Quantize X (row-major n x F), train n_estimators symmetric trees.  With a
+  // validation set (Xv: nv x F, binned with the training edges) the fit early-
+  // stops on the validation loss (booster.cpp:255-371) and keeps the trees built.
+  void fit(const float* X, const float* y, uint32_t n, uint32_t F, const float* Xv = nullptr,
+           const float* yv = nullptr, uint32_t nv = 0);
   // Raw scores + base (predict_raw_nobinning_fast, booster.cpp:899-925).
   void predict_raw(const float* X, uint32_t n, uint32_t F, float* out);
   // sigmoid(raw) (predict_proba_nobinning_fast, booster.cpp:927-945).

@@ -48,6 +48,10 @@ struct TreeState {
   uint32_t gmax_next;   // float bits of max |g| of the gradients being written
   uint32_t hmax_next;   // float bits of max |h| of the gradients being written
   uint32_t fault;       // nonzero: a kernel found inconsistent state and skipped its writes
+  uint32_t stopped;     // early stopping ended the fit after tree `tree`
+  float es_best;        // best validation loss so far (early stopping)
+  uint32_t es_best_iter;
+  uint32_t es_count;    // trees since the last improvement
   uint32_t pad;
   unsigned long long best[kMaxDepth];  // argmax key per level
   // Fixed-point scales of the current tree's histograms: a row's g enters as
@@ -129,6 +133,16 @@ struct JobDev {
   uint16_t* m_depth;        // [n_trees]
   float* m_leaves;          // [n_trees][2^D][K]
   unsigned long long* hist_rows_total;  // rows fed to histogram kernels (all trees)
+  // validation set and per-tree losses (K = 1); Nv = 0: no validation set
+  const uint8_t* v_bins_cm; // [F][Nv], binned with the training edges
+  const float* v_y;         // [Nv]
+  float* v_preds;           // [Nv] raw validation predictions
+  uint32_t Nv, v_wg_cnt;    // validation rows; their gradient-update workgroups
+  double* loss_partials;    // [grad_wg_cnt + v_wg_cnt] loss sums per 1024-row tile
+  float* train_loss;        // [n_trees] mean training loss after each tree (null: not kept)
+  float* valid_loss;        // [n_trees] mean validation loss after each tree
+  uint32_t es_rounds;       // early stopping: trees without improvement
+  uint32_t es_min_iter;     // first tree after which the fit may stop
 };
 
 // 16-feature tiles of a row.

@@ -37,6 +37,10 @@ __global__ void k_init_state(TreeState* st) {
   st->gmax_next = 0u;
   st->hmax_next = 0u;
   st->fault = 0u;
+  st->stopped = 0u;
+  st->es_best = 1e30f;  // booster.cpp:119: lower is better for the loss
+  st->es_best_iter = 0u;
+  st->es_count = 0u;
   st->pad = 0u;
   for (int d = 0; d < kMaxDepth; ++d) st->best[d] = kEmptyKey;
   st->gscale = st->hscale = st->ginv = st->hinv = 1.0;
@@ -115,6 +119,10 @@ struct JobSpec {
   const float* y_host;   // labels (may be null for teacher forcing)
   tc_train_params p;
   float base;
+  const tc_dataset* valid = nullptr;  // validation rows, binned with the training edges
+  const float* y_valid = nullptr;     // their labels (host)
+  uint32_t es_rounds = 0;             // early stopping rounds (with a validation set)
+  bool history = false;               // keep the training loss after every tree
 };
 
 // Gradient channels of a job: K classes for softmax, else 1.
@@ -242,6 +250,7 @@ class Trainer {
     }
     hipLaunchKernelGGL(k_grad_update, dim3(total_grad_wg_), dim3(256), leaf_lds(), s, d_jobs_,
                        d_job_wg_off_, njobs_, 1, lists_);
+    if (eval_) hipLaunchKernelGGL(k_eval, dim3(1, njobs_), dim3(256), 0, s, d_jobs_);
     TC_HIP(hipGetLastError());
   }
 
@@ -287,8 +296,52 @@ class Trainer {
   }
 
   // Run all trees; returns loop time (ms) and histogram time (ms).
+  // Early stopping: every 16 trees the jobs' stop flags are copied to pinned
+  // memory; once a copy shows every job stopped, no more trees are enqueued
+  // (the trees enqueued meanwhile do nothing on the device).
+  struct StopPoll {
+    uint32_t* flags = nullptr;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+    StopPoll() = default;
+    StopPoll(const StopPoll&) = delete;
+    StopPoll& operator=(const StopPoll&) = delete;
+    ~StopPoll() {
+      if (ev) (void)hipEventSynchronize(ev);
+      if (flags) (void)hipHostFree(flags);
+      if (ev) (void)hipEventDestroy(ev);
+    }
+  };
+  bool validating() const {
+    for (const JobDev& J : h_jobs_) if (J.Nv == 0) return false;
+    return !h_jobs_.empty();
+  }
+  bool poll_stopped(StopPoll& P, uint32_t t) {
+    if (!validating() || t % 16 != 0) return false;
+    hipStream_t s = ctx_->stream;
+    if (!P.flags) {
+      TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&P.flags), 4ull * njobs_, hipHostMallocDefault));
+      TC_HIP(hipEventCreateWithFlags(&P.ev, hipEventDisableTiming));
+    }
+    if (P.pending && hipEventQuery(P.ev) == hipSuccess) {
+      P.pending = false;
+      bool all = true;
+      for (uint32_t j = 0; j < njobs_; ++j) all = all && P.flags[j] != 0;
+      if (all) return true;
+    }
+    if (!P.pending) {
+      for (uint32_t j = 0; j < njobs_; ++j)
+        TC_HIP(hipMemcpyAsync(P.flags + j,
+                              reinterpret_cast<const char*>(h_jobs_[j].st) + offsetof(TreeState, stopped),
+                              4, hipMemcpyDeviceToHost, s));
+      TC_HIP(hipEventRecord(P.ev, s));
+      P.pending = true;
+    }
+    return false;
+  }
   void run(bool use_graph, bool time_hist, double* loop_ms, double* hist_ms) {
     hipStream_t s = ctx_->stream;
+    StopPoll poll;
     hipEvent_t a, b;
     TC_HIP(hipEventCreate(&a));
     TC_HIP(hipEventCreate(&b));
@@ -305,6 +358,7 @@ class Trainer {
       TC_HIP(hipGraphDestroy(g));
       const bool smp = sampling();
       for (uint32_t t = built; t < n_trees_; ++t) {
+        if (poll_stopped(poll, t)) break;
         if (smp) prepare_samples();
         TC_HIP(hipGraphLaunch(ge, s));
       }
@@ -314,6 +368,7 @@ class Trainer {
     } else {
       TC_HIP(hipEventRecord(a, s));
       for (uint32_t t = 0; t < n_trees_; ++t) {
+        if (poll_stopped(poll, t)) break;
         if (sampling()) prepare_samples();
         enqueue_tree(time_hist ? &hev : nullptr);
       }
@@ -446,6 +501,26 @@ class Trainer {
     }
   }
 
+  // Per-tree losses and early-stopping outcome of job j.
+  void history(uint32_t j, tc_train_history* h) {
+    const JobDev& J = h_jobs_[j];
+    TreeState st{};
+    TC_HIP(hipMemcpy(&st, J.st, sizeof(st), hipMemcpyDeviceToHost));
+    const uint32_t built = st.tree < 0 ? 0u : std::min(n_trees_, static_cast<uint32_t>(st.tree) + 1u);
+    h->n_trees_built = built;
+    h->stopped_early = st.stopped ? 1 : 0;
+    h->best_iteration = st.es_best_iter;
+    h->best_valid_loss = J.Nv ? st.es_best : 0.0f;
+    if (h->train_loss) {
+      std::fill(h->train_loss, h->train_loss + n_trees_, 0.0f);
+      if (J.train_loss) TC_HIP(hipMemcpy(h->train_loss, J.train_loss, 4ull * built, hipMemcpyDeviceToHost));
+    }
+    if (h->valid_loss) {
+      std::fill(h->valid_loss, h->valid_loss + n_trees_, 0.0f);
+      if (J.valid_loss) TC_HIP(hipMemcpy(h->valid_loss, J.valid_loss, 4ull * built, hipMemcpyDeviceToHost));
+    }
+  }
+
   void download(uint32_t j, tc_model* m) {
     check_faults();
     const JobDev& J = h_jobs_[j];
@@ -568,7 +643,32 @@ class Trainer {
       J.cursor = alloc<uint32_t>(1u << D_);
       J.grad_wg_cnt = std::max(1u, ceil_div(N, kGradTile));
       J.grad_wg_off = wg_off[j];
-      wg_off[j + 1] = wg_off[j] + J.grad_wg_cnt;
+      if (S.valid) {  // validation rows ride on the gradient update's grid
+        TC_REQUIRE(S.valid->F == ds->F, "validation set: feature count differs from training");
+        TC_REQUIRE(K == 1, "validation sets need a single-output loss (MSE or LogLoss)");
+        TC_REQUIRE(S.y_valid != nullptr, "validation set without labels");
+        const uint32_t Nv = S.valid->N;
+        J.Nv = Nv;
+        J.v_bins_cm = S.valid->bins_cm;
+        J.v_wg_cnt = ceil_div(Nv, kGradTile);
+        float* vy = alloc<float>(std::max(1u, Nv));
+        if (Nv) TC_HIP(hipMemcpyAsync(vy, S.y_valid, 4ull * Nv, hipMemcpyHostToDevice, s));
+        J.v_y = vy;
+        J.v_preds = alloc<float>(std::max(1u, Nv));
+        J.valid_loss = alloc<float>(std::max(1u, n_trees_));
+        TC_HIP(hipMemsetAsync(J.valid_loss, 0, 4ull * std::max(1u, n_trees_), s));
+        J.es_rounds = S.es_rounds;
+        J.es_min_iter = std::max(50u, n_trees_ / 10);  // booster.cpp:345
+      }
+      if (S.history) {
+        J.train_loss = alloc<float>(std::max(1u, n_trees_));
+        TC_HIP(hipMemsetAsync(J.train_loss, 0, 4ull * std::max(1u, n_trees_), s));
+      }
+      if (J.Nv || J.train_loss) {
+        J.loss_partials = alloc<double>(J.grad_wg_cnt + J.v_wg_cnt);
+        eval_ = true;
+      }
+      wg_off[j + 1] = wg_off[j] + J.grad_wg_cnt + J.v_wg_cnt;
       J.grad_partials = alloc<double>(2ull * J.grad_wg_cnt * K);
       J.n_trees = n_trees_;
       const size_t TD = static_cast<size_t>(n_trees_) * D_;
@@ -687,6 +787,7 @@ class Trainer {
   std::vector<Sampler> samplers_;
   bool forced_ = false;
   bool keep_rows_ = false;  // partition the last level too (leaf_index)
+  bool eval_ = false;       // some job records losses / early-stops (k_eval per tree)
   // histogram work-item shapes (rows per item at the root and below)
   std::vector<uint64_t> work_;          // per job: rows x tiles x channels
   std::vector<uint32_t> fixed_chunk_;   // per job: caller's hist_chunk, 0 = tuned
@@ -775,6 +876,33 @@ int32_t tc_train(tc_dataset* ds, const float* y, const tc_train_params* p, tc_mo
     ds->ctx->last_loop_ms = loop_ms;
     ds->ctx->last_hist_ms = 0.0;
     tr.download(0, model);
+  });
+}
+
+int32_t tc_train_valid(tc_dataset* ds, const float* y, const tc_train_params* p, tc_dataset* valid,
+                       const float* y_valid, uint32_t early_stopping_rounds, tc_model* model,
+                       tc_train_history* history) {
+  return guard([&] {
+    TC_REQUIRE(ds && y && model, "tc_train_valid: NULL argument");
+    check_params(p);
+    TC_REQUIRE(ds->N > 0, "tc_train_valid: empty dataset");
+    TC_REQUIRE(p->loss == TC_LOSS_MSE || p->loss == TC_LOSS_LOGLOSS,
+               "tc_train_valid: MSE or LogLoss");
+    TC_REQUIRE(!valid || valid->ctx == ds->ctx, "tc_train_valid: datasets on different contexts");
+    TC_REQUIRE(!valid || y_valid, "tc_train_valid: validation labels missing");
+    JobSpec s{ds, y, *p, init_prediction_host(p->loss, y, ds->N)};
+    s.valid = valid;
+    s.y_valid = y_valid;
+    s.es_rounds = early_stopping_rounds;
+    s.history = history != nullptr && history->train_loss != nullptr;
+    Trainer tr(ds->ctx, {s});
+    tr.init_gradients();
+    double loop_ms = 0, hist_ms = 0;
+    tr.run(p->use_graph != 0, false, &loop_ms, &hist_ms);
+    ds->ctx->last_loop_ms = loop_ms;
+    ds->ctx->last_hist_ms = 0.0;
+    tr.download(0, model);
+    if (history) tr.history(0, history);
   });
 }
 

@@ -144,6 +144,29 @@ __device__ inline void block_max2_to(TreeState* st, float a, float b) {
 // for the next tree's histogram scales, and empties the work lists.
 constexpr int kGradTile = 1024;
 
+// Mean per-row loss terms of Loss::compute_loss (A19): MSE (loss.cpp:336-346)
+// and LogLoss (loss.cpp:110-121, sigmoid and 1e-7 clip of loss.hpp:99-105).
+__device__ inline float row_loss(int loss, float pred, float label) {
+  if (loss == kMSE) {
+    const float d = __fsub_rn(pred, label);
+    return __fmul_rn(d, d);
+  }
+  float q = __fdiv_rn(1.0f, __fadd_rn(1.0f, expf(-pred)));
+  q = fmaxf(1e-7f, fminf(1.0f - 1e-7f, q));
+  return __fsub_rn(__fmul_rn(-label, logf(q)), __fmul_rn(__fsub_rn(1.0f, label), logf(__fsub_rn(1.0f, q))));
+}
+
+// A 256-thread workgroup's sum of one double per thread, in a fixed order
+// (shuffle tree per wave, waves in order), stored by thread 0.
+__device__ inline void tile_sum_store(double v, double* dst) {
+  __shared__ double s_w[4];
+  v = warp_sum_d(v);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) *dst = ((s_w[0] + s_w[1]) + s_w[2]) + s_w[3];
+  __syncthreads();
+}
+
 // One 256-thread workgroup's root partial (sum g, sum h of its 1024-row tile,
 // channel c): every thread's double sum of its rows in order, a fixed
 // shuffle tree per wave, the 4 waves added in order.
@@ -262,7 +285,7 @@ __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ 
   int t = -1, depth = 0;
   if (mode == 1) {
     t = J.st->tree;
-    if (t < 0 || t >= static_cast<int>(J.n_trees)) return;
+    if (t < 0 || t >= static_cast<int>(J.n_trees) || J.st->stopped) return;
     depth = J.st->depth;
     if (tid < static_cast<uint32_t>(depth)) {
       s_f[tid] = J.m_feat[static_cast<size_t>(t) * J.D + tid];
@@ -295,6 +318,50 @@ __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ 
     grad_update_softmax(J, lw, mode, depth, s_f, s_b, s_leaf);
     return;
   }
+  if (lw >= J.grad_wg_cnt) {  // validation rows: apply the tree, then the loss (booster.cpp:259-283)
+    const uint32_t vw = lw - J.grad_wg_cnt, Nv = J.Nv;
+    uint32_t vr[kR], vli[kR];
+    bool vok[kR];
+    float vp[kR], vy[kR];
+#pragma unroll
+    for (int k = 0; k < kR; ++k) {
+      vr[k] = vw * kGradTile + k * 256 + tid;
+      vok[k] = vr[k] < Nv;
+      vli[k] = 0;
+      vp[k] = (vok[k] && mode == 1) ? J.v_preds[vr[k]] : 0.f;
+      vy[k] = vok[k] ? J.v_y[vr[k]] : 0.f;
+    }
+    if (mode == 1) {
+      uint32_t v[kMaxDepth][kR];
+#pragma unroll
+      for (int d = 0; d < kMaxDepth; ++d) {
+        if (d < depth) {
+          const uint8_t* col = J.v_bins_cm + static_cast<size_t>(s_f[d]) * Nv;
+#pragma unroll
+          for (int k = 0; k < kR; ++k) v[d][k] = vok[k] ? col[vr[k]] : 0u;
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < kMaxDepth; ++d) {
+        if (d < depth) {
+          const uint32_t thr = s_b[d];
+#pragma unroll
+          for (int k = 0; k < kR; ++k) vli[k] = (vli[k] << 1) | ((v[d][k] != 255u && v[d][k] > thr) ? 1u : 0u);
+        }
+      }
+    }
+    double sl = 0.0;
+#pragma unroll
+    for (int k = 0; k < kR; ++k) {
+      if (!vok[k]) continue;
+      // valid_preds[i] += lr * tree value, one fused multiply-add
+      const float pred = mode == 1 ? __fmaf_rn(J.lr, s_leaf[vli[k]], vp[k]) : J.base;
+      J.v_preds[vr[k]] = pred;
+      sl += row_loss(J.loss, pred, vy[k]);
+    }
+    if (mode == 1) tile_sum_store(sl, J.loss_partials + lw);
+    return;
+  }
   if (mode == 1) {
     uint32_t v[kMaxDepth][kR];
 #pragma unroll
@@ -316,6 +383,7 @@ __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ 
   }
   float gmax = 0.f, hmax = 0.f;
   double sg = 0.0, sh = 0.0;  // the root partials of this tile (k_root_partials' order)
+  double tl = 0.0;            // the training loss after this tree (A19)
 #pragma unroll
   for (int k = 0; k < kR; ++k) {
     if (!ok[k]) continue;
@@ -327,9 +395,54 @@ __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ 
     hmax = fmaxf(hmax, fabsf(gh.y));
     sg += gh.x;
     sh += gh.y;
+    tl += J.train_loss ? row_loss(J.loss, pred, lab[k]) : 0.0f;
   }
+  if (mode == 1 && J.train_loss) tile_sum_store(tl, J.loss_partials + lw);
   if (J.fused_root) root_partial_store(J, lw, 0, sg, sh);
   block_max2_to(J.st, gmax, hmax);
+}
+
+// A19 and early stopping (booster.cpp:252-352), one workgroup per job after
+// the gradient update: the tiles' loss sums added in a fixed order, the mean
+// losses of the tree just added recorded, then the stopping rule on the
+// validation loss (improved = loss < best - 1e-10; stop once the tree index
+// reaches es_min_iter and es_rounds trees in a row did not improve).
+__global__ void __launch_bounds__(256) k_eval(const JobDev* __restrict__ jobs) {
+  const JobDev& J = jobs[blockIdx.y];
+  if (!J.loss_partials) return;
+  TreeState* st = J.st;
+  const int t = st->tree;
+  if (st->stopped || t < 0 || t >= static_cast<int>(J.n_trees)) return;
+  const uint32_t tid = threadIdx.x;
+  __shared__ double s_w[2][4];
+  double a = 0.0, b = 0.0;
+  if (J.train_loss)
+    for (uint32_t w = tid; w < J.grad_wg_cnt; w += 256) a += J.loss_partials[w];
+  for (uint32_t w = tid; w < J.v_wg_cnt; w += 256) b += J.loss_partials[J.grad_wg_cnt + w];
+  a = warp_sum_d(a);
+  b = warp_sum_d(b);
+  if ((tid & 63) == 0) {
+    s_w[0][tid >> 6] = a;
+    s_w[1][tid >> 6] = b;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const double ta = ((s_w[0][0] + s_w[0][1]) + s_w[0][2]) + s_w[0][3];
+    const double va = ((s_w[1][0] + s_w[1][1]) + s_w[1][2]) + s_w[1][3];
+    if (J.train_loss) J.train_loss[t] = static_cast<float>(ta / J.N);
+    if (J.Nv) {
+      const float vl = static_cast<float>(va / J.Nv);
+      J.valid_loss[t] = vl;
+      if (vl < __fsub_rn(st->es_best, 1e-10f)) {
+        st->es_best = vl;
+        st->es_best_iter = static_cast<uint32_t>(t);
+        st->es_count = 0;
+      } else {
+        st->es_count += 1;
+      }
+      if (static_cast<uint32_t>(t) >= J.es_min_iter && st->es_count >= J.es_rounds) st->stopped = 1;
+    }
+  }
 }
 
 // Root of a new tree (symmetric_tree.cpp:97-124), one 256-thread workgroup
@@ -346,9 +459,10 @@ __device__ void tree_begin(const JobDev& J, uint32_t job, const Lists& L, int pa
   const uint32_t hcnt = ((n + J.hist_chunk - 1) / J.hist_chunk) * J.T * K, pcnt = part_slots(n);
   // the tree state, in flight while the partials are summed
   int prev_tree = 0;
-  uint32_t gbits = 0, hbits = 0;
+  uint32_t gbits = 0, hbits = 0, stopped = 0;
   if (tid == 0) {
     prev_tree = J.st->tree;
+    stopped = J.st->stopped;
     gbits = J.st->gmax_next;
     hbits = J.st->hmax_next;
   }
@@ -368,7 +482,10 @@ __device__ void tree_begin(const JobDev& J, uint32_t job, const Lists& L, int pa
     }
     __syncthreads();
   }
-  if (tid == 0) {
+  if (tid == 0 && stopped) {  // early stopping ended the fit: no more trees
+    J.st->active = 0;
+    s_go = 0;
+  } else if (tid == 0) {
     TreeState* st = J.st;
     const int t = prev_tree + 1;
     st->tree = t;
@@ -441,8 +558,8 @@ __global__ void __launch_bounds__(256) k_root_partials(const JobDev* __restrict_
   if (tid == 0) s_job = job_of_block(job_wg_off, njobs);
   __syncthreads();
   const JobDev& J = jobs[s_job];
-  if (J.fused_root) return;  // written by the gradient update
   const uint32_t lw = blockIdx.x - J.grad_wg_off, K = J.K;
+  if (J.fused_root || lw >= J.grad_wg_cnt) return;  // written by the gradient update / validation rows
   const bool live = J.st->tree + 1 < static_cast<int>(J.n_trees);
   if (live && !J.store_hist) {
     const uint32_t cells = K * J.F * 256u;  // the root's histograms

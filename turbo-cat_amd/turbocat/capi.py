@@ -30,7 +30,7 @@ EXPORTS = [
     "tc_dataset_destroy", "tc_gradients", "tc_gradients_multi", "tc_random_subsample",
     "tc_init_prediction", "tc_hist_build",
     "tc_hist_subtract", "tc_split_scan", "tc_tree_build", "tc_tree_build_multi", "tc_train",
-    "tc_train_batch",
+    "tc_train_valid", "tc_train_batch",
     "tc_last_train_timing", "tc_trainer_create", "tc_trainer_step", "tc_trainer_step_timed",
     "tc_trainer_sync", "tc_trainer_model", "tc_trainer_hist_rows", "tc_trainer_hist_chunks",
     "tc_trainer_destroy",
@@ -64,6 +64,12 @@ class Model(Structure):
     _fields_ = [("feature", c_void_p), ("bin", c_void_p), ("fthr", c_void_p), ("gain", c_void_p),
                 ("depth", c_void_p), ("leaves", c_void_p), ("train_preds", c_void_p),
                 ("base", c_float)]
+
+
+class TrainHistory(Structure):
+    _fields_ = [("train_loss", c_void_p), ("valid_loss", c_void_p), ("n_trees_built", c_uint32),
+                ("stopped_early", c_int32), ("best_iteration", c_uint32),
+                ("best_valid_loss", c_float)]
 
 
 class ServeParams(Structure):
@@ -133,6 +139,8 @@ def load(path: str = LIB_PATH):
         "tc_tree_build": [V, V, V, POINTER(TreeParams), V, V, V, V, V, V, V],
         "tc_tree_build_multi": [V, V, V, U, POINTER(TreeParams), V, V, V, V, V, V, V],
         "tc_train": [V, V, POINTER(TrainParams), POINTER(Model)],
+        "tc_train_valid": [V, V, POINTER(TrainParams), V, V, U, POINTER(Model),
+                           POINTER(TrainHistory)],
         "tc_train_batch": [V, U, V, V, V, V],
         "tc_last_train_timing": [V, V, V],
         "tc_trainer_create": [V, U, V, V, V, V], "tc_trainer_step": [V, U],
@@ -360,6 +368,29 @@ class Dataset:
         a["base"] = float(m.base)
         a["weight"] = np.full(params.n_trees, params.learning_rate, np.float32)
         return a
+
+    def train_valid(self, y, params: TrainParams, valid=None, y_valid=None,
+                    early_stopping_rounds=50):
+        """tc_train_valid: returns (model, history); history holds the mean
+        training / validation loss after each tree, the trees built and the
+        best iteration."""
+        T = params.n_trees
+        m, a = _new_model(T, params.tree.max_depth, self.n, channels(params))
+        y = np.ascontiguousarray(y, np.float32)
+        tl = np.zeros(max(1, T), np.float32)
+        vl = np.zeros(max(1, T), np.float32)
+        h = TrainHistory(tl.ctypes.data, vl.ctypes.data, 0, 0, 0, 0.0)
+        yv = None if y_valid is None else np.ascontiguousarray(y_valid, np.float32)
+        check(self.ctx.L.tc_train_valid(self.h, _p(y), byref(params),
+                                        None if valid is None else valid.h,
+                                        None if yv is None else _p(yv),
+                                        early_stopping_rounds, byref(m), byref(h)))
+        a["base"] = float(m.base)
+        a["weight"] = np.full(T, params.learning_rate, np.float32)
+        hist = {"train_loss": tl[:T].copy(), "valid_loss": vl[:T].copy(),
+                "n_trees_built": int(h.n_trees_built), "stopped_early": bool(h.stopped_early),
+                "best_iteration": int(h.best_iteration), "best_valid_loss": float(h.best_valid_loss)}
+        return a, hist
 
 
 class Trainer:
