@@ -4,9 +4,10 @@
 // types and error behaviour (RuntimeError "Model not fitted. Call fit()
 // first." before fit; ValueError for invalid arguments).  Options whose
 // reference code path is not the symmetric-tree hot path (mode="small",
-// GOSS, ordered boosting, categorical features, multiclass,
-// validation-set early stopping, non-MSE/LogLoss losses) raise ValueError
-// naming the option instead of silently training something else.
+// GOSS, ordered boosting, categorical features, multiclass, early-stopping
+// metrics other than the loss, non-MSE/LogLoss losses) raise ValueError
+// naming the option instead of silently training something else.  A
+// validation set early-stops on its loss (booster.cpp:255-371).
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
