@@ -90,6 +90,29 @@ def test_tree_teacher_forced(ctx, oracle, n, F, depth, lam, seed):
     assert not tie
 
 
+@pytest.mark.parametrize("n,F,depth,seed", [
+    (3000, 1, 10, 5),      # one feature, the deepest tree: 512 pairs at the last level
+    (2500, 300, 3, 6),     # 19 tiles, a 384-byte row stride
+    (17, 5, 4, 7),         # fewer rows than any work item
+    (70000, 129, 7, 8),    # 9 tiles, the last with one feature
+])
+def test_tree_extreme_shapes(ctx, oracle, n, F, depth, seed):
+    """Teacher-forced trees at the edges of the histogram layout (tile and
+    stride rounding, tiny nodes and empty children, maximum depth) against
+    the oracle; a split may differ only at an FP64 near-tie."""
+    from turbocat.capi import tree_params
+    from oracle import tree_params as otp
+    X, y = regression_data(n, F, seed)
+    ds, edges, ne = ctx.quantize_fit(X)
+    bins = ds.bins()
+    rng = np.random.default_rng(seed + 200)
+    g = (rng.standard_normal(n) + 0.5 * X[:, 0]).astype(np.float32)
+    h = (rng.random(n) * 0.3 + 0.05).astype(np.float32)
+    gpu = ds.tree(g, h, tree_params(max_depth=depth, lambda_l2=1.0, min_child_weight=0.0))
+    compare_tree(gpu, bins, g, h, edges, ne, otp(max_depth=depth, lambda_l2=1.0,
+                                                  min_child_weight=0.0), oracle)
+
+
 def test_tree_stops_early(ctx, oracle):
     """min_child_weight large enough that the level stop test fires."""
     from turbocat.capi import tree_params
