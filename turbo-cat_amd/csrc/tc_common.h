@@ -122,6 +122,7 @@ struct JobDev {
   uint32_t* cursor;
   // root-sum partials (sum g, sum h over 1024-row tiles of the root list)
   double* grad_partials;    // [wg][K][2]
+  uint32_t* ghmax_partials; // [wg][2] float bits of each tile's max |g|, max |h| (all channels)
   uint32_t grad_wg_off;     // first workgroup of this job in the row grid
   uint32_t grad_wg_cnt;
   // model, tree-major

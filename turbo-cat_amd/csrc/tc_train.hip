@@ -194,6 +194,8 @@ class Trainer {
     float* dh = dmalloc<float>(N);
     TC_HIP(hipMemcpyAsync(dg, g, sizeof(float) * N, hipMemcpyHostToDevice, ctx_->stream));
     TC_HIP(hipMemcpyAsync(dh, h, sizeof(float) * N, hipMemcpyHostToDevice, ctx_->stream));
+    // the scales come from these gradients alone: clear the gradient update's tile maxima
+    TC_HIP(hipMemsetAsync(h_jobs_[0].ghmax_partials, 0, 8ull * h_jobs_[0].grad_wg_cnt, ctx_->stream));
     hipLaunchKernelGGL(k_set_gh, dim3(std::max(1u, ceil_div(N, 256))), dim3(256), 0,
                        ctx_->stream, d_jobs_, dg, dh);
     forced_ = true;  // arbitrary h: no packed histogram cells
@@ -670,6 +672,8 @@ class Trainer {
       }
       wg_off[j + 1] = wg_off[j] + J.grad_wg_cnt + J.v_wg_cnt;
       J.grad_partials = alloc<double>(2ull * J.grad_wg_cnt * K);
+      J.ghmax_partials = alloc<uint32_t>(2ull * J.grad_wg_cnt);
+      TC_HIP(hipMemsetAsync(J.ghmax_partials, 0, 8ull * J.grad_wg_cnt, s));
       J.n_trees = n_trees_;
       const size_t TD = static_cast<size_t>(n_trees_) * D_;
       J.m_feat = alloc<uint16_t>(TD);

@@ -105,6 +105,31 @@ __device__ inline void put_part_work(PartWork* dst, uint32_t job, uint32_t node,
   *dst = w;
 }
 
+// Workgroup max of two non-negative floats, stored by thread 0 as float bits
+// into dst[0..1] (the gradient update's per-tile maxima; the next tree start
+// takes their max -- order-free, and no two workgroups touch one address).
+__device__ inline void block_max2_store(float a, float b, uint32_t* dst) {
+  __shared__ uint32_t s_m[2][4];
+  for (int o = 32; o > 0; o >>= 1) {
+    a = fmaxf(a, __shfl_down(a, o, 64));
+    b = fmaxf(b, __shfl_down(b, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s_m[0][threadIdx.x >> 6] = __float_as_uint(a);
+    s_m[1][threadIdx.x >> 6] = __float_as_uint(b);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t x = s_m[0][0], y = s_m[1][0];
+    for (int w = 1; w < 4; ++w) {
+      x = s_m[0][w] > x ? s_m[0][w] : x;
+      y = s_m[1][w] > y ? s_m[1][w] : y;
+    }
+    dst[0] = x;
+    dst[1] = y;
+  }
+}
+
 // Workgroup max of two non-negative floats, merged into st (float bits of
 // non-negative floats order like the floats).
 __device__ inline void block_max2_to(TreeState* st, float a, float b) {
@@ -245,7 +270,7 @@ __device__ void grad_update_softmax(const JobDev& J, uint32_t lw, int mode, int 
       }
     }
   }
-  block_max2_to(J.st, gmax, hmax);
+  block_max2_store(gmax, hmax, J.ghmax_partials + 2 * lw);
 }
 
 __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ jobs,
@@ -399,7 +424,7 @@ __global__ void __launch_bounds__(256) k_grad_update(const JobDev* __restrict__ 
   }
   if (mode == 1 && J.train_loss) tile_sum_store(tl, J.loss_partials + lw);
   if (J.fused_root) root_partial_store(J, lw, 0, sg, sh);
-  block_max2_to(J.st, gmax, hmax);
+  block_max2_store(gmax, hmax, J.ghmax_partials + 2 * lw);
 }
 
 // A19 and early stopping (booster.cpp:252-352), one workgroup per job after
@@ -463,8 +488,28 @@ __device__ void tree_begin(const JobDev& J, uint32_t job, const Lists& L, int pa
   if (tid == 0) {
     prev_tree = J.st->tree;
     stopped = J.st->stopped;
-    gbits = J.st->gmax_next;
-    hbits = J.st->hmax_next;
+  }
+  {  // max |g|, max |h| of the gradients: the gradient update's tile maxima, and
+     // what teacher forcing merged into the state
+    __shared__ uint32_t s_mx[2][4];
+    uint32_t x = tid == 0 ? J.st->gmax_next : 0u, y = tid == 0 ? J.st->hmax_next : 0u;
+    for (uint32_t w = tid; w < J.grad_wg_cnt; w += 256) {
+      x = max(x, J.ghmax_partials[2 * w]);
+      y = max(y, J.ghmax_partials[2 * w + 1]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      x = max(x, static_cast<uint32_t>(__shfl_down(static_cast<int>(x), o, 64)));
+      y = max(y, static_cast<uint32_t>(__shfl_down(static_cast<int>(y), o, 64)));
+    }
+    if ((tid & 63) == 0) {
+      s_mx[0][tid >> 6] = x;
+      s_mx[1][tid >> 6] = y;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      gbits = max(max(s_mx[0][0], s_mx[0][1]), max(s_mx[0][2], s_mx[0][3]));
+      hbits = max(max(s_mx[1][0], s_mx[1][1]), max(s_mx[1][2], s_mx[1][3]));
+    }
   }
   for (uint32_t c = 0; c < K; ++c) {  // thread t adds tiles t, t + 256, ... in order; then a fixed tree
     double a = 0.0, b = 0.0;
