@@ -116,9 +116,12 @@ struct JobDev {
   HistCell* hist;           // heap [2^D - 1][K][F][256]
   uint32_t* hfirst;         // heap [2^D - 1]: first work-list item of a histogrammed node
   TreeState* st;
-  // partition cursors of the level being split: [node][2] = next free
-  // position of the left child (counts up), end of the right child (counts
-  // down); capacity 2^D
+  unsigned long long* best_keys;  // [D][F] each feature's best split key of the level
+                                  // (null: merged with atomics into TreeState::best)
+  // partition cursors of the level being split, one 64-bit word per node:
+  // [node][0] next free position of the left child (counts up from the
+  // node's start), [node][1] rows taken by the right child (filled down from
+  // the node's end); capacity 2^D
   uint32_t* cursor;
   // root-sum partials (sum g, sum h over 1024-row tiles of the root list)
   double* grad_partials;    // [wg][K][2]

@@ -639,6 +639,7 @@ class Trainer {
       J.nc = alloc<uint32_t>(nheap);
       J.hist = alloc<HistCell>(((1ull << D_) - 1) * K * J.F * 256);
       J.hfirst = alloc<uint32_t>((1ull << D_) - 1);
+      J.best_keys = alloc<unsigned long long>(static_cast<size_t>(D_) * ds->F);
       J.lvl_chunk = alloc<uint32_t>(D_);
       J.st = alloc<TreeState>(1);
       hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1), 0, s, J.st);

@@ -565,8 +565,8 @@ __device__ void tree_begin(const JobDev& J, uint32_t job, const Lists& L, int pa
         s_pbase = list_reserve(L, L.part_count[0], pcnt);
         J.hfirst[0] = s_hbase;
         J.lvl_chunk[0] = J.hist_chunk;
-        J.cursor[0] = 0;
-        J.cursor[1] = n;
+        J.cursor[0] = 0;  // left position
+        J.cursor[1] = 0;  // right rows taken
         if (J.hist_rows_total) atomicAdd(J.hist_rows_total, static_cast<unsigned long long>(n));
       }
     }
@@ -1091,7 +1091,10 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
   if (tid == 0) {
     unsigned long long m = s_red[0];
     for (int i = 1; i < 4; ++i) m = s_red[i] > m ? s_red[i] : m;
-    if (m != kEmptyKey) atomicMax(&J.st->best[level], m);
+    if (J.best_keys)  // every feature's key, read back by the level's decode
+      J.best_keys[static_cast<size_t>(level) * F + f] = m;
+    else if (m != kEmptyKey)
+      atomicMax(&J.st->best[level], m);
   }
 }
 
@@ -1100,7 +1103,16 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
 __device__ inline bool decode_split(const JobDev& J, uint32_t level, bool record, uint32_t* f_out,
                                     uint32_t* thr_out) {
   TreeState* st = J.st;
-  const unsigned long long key = st->best[level];
+  unsigned long long key;
+  if (J.best_keys) {  // the max over the features' keys, each wave for itself
+    const unsigned long long* keys = J.best_keys + static_cast<size_t>(level) * J.F;
+    unsigned long long m = kEmptyKey;
+    for (uint32_t f = threadIdx.x & 63u; f < J.F; f += 64) m = keys[f] > m ? keys[f] : m;
+    m = warp_max_u64(m);
+    key = __shfl(m, 0, 64);
+  } else {
+    key = st->best[level];
+  }
   const float gain = float_unorder(static_cast<uint32_t>(key >> 32));
   const bool accept = key != kEmptyKey && gain > -1e29f && !(gain < J.min_split_gain);
   const uint32_t fb = 0xFFFFFFFFu - static_cast<uint32_t>(key & 0xFFFFFFFFu);
@@ -1175,7 +1187,7 @@ __global__ void __launch_bounds__(256) k_finalize(const JobDev* __restrict__ job
 // channel) while its rows are routed.  Each work item (<= 4096 rows of one
 // node) routes its rows -- NaN bin 255 or bin <= threshold goes left
 // (symmetric_tree.cpp:309-311) -- keeping them in registers, reserves room
-// with one atomic per side on the node's cursors (left fills up from the
+// with one 64-bit atomic on the node's cursor pair (left fills up from the
 // node's start, right fills down from its end; set by the previous level's
 // k_part_plan) and writes the rows, placed by wave ballots.
 __device__ inline bool part_active(const JobDev& J, uint32_t level) {
@@ -1236,7 +1248,7 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(const JobDev* __rest
   }
   __syncthreads();
   __shared__ uint32_t s_tot[2];
-  if (tid < 2) {  // offsets in (step, wave) order; one reservation per side
+  if (tid < 2) {  // offsets in (step, wave) order
     uint32_t acc = 0;
     for (int i = 0; i < kIt * kW; ++i) {
       const uint32_t v = s_cnt[tid][i];
@@ -1244,7 +1256,15 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(const JobDev* __rest
       acc += v;
     }
     s_tot[tid] = acc;
-    s_base[tid] = acc == 0 ? 0u : (tid == 0 ? atomicAdd(&cur[0], acc) : atomicSub(&cur[1], acc) - acc);
+  }
+  __syncthreads();
+  if (tid == 0) {  // one 64-bit reservation for both sides: left position, right rows taken
+    const unsigned long long add =
+        static_cast<unsigned long long>(s_tot[0]) | (static_cast<unsigned long long>(s_tot[1]) << 32);
+    const unsigned long long old =
+        add == 0 ? 0ull : atomicAdd(reinterpret_cast<unsigned long long*>(cur), add);
+    s_base[0] = static_cast<uint32_t>(old);
+    s_base[1] = offd[w.node + 1] - static_cast<uint32_t>(old >> 32) - s_tot[1];
   }
   __syncthreads();
   {  // both reservations must lie inside the node's segment; otherwise write nothing
@@ -1373,9 +1393,9 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
     J.hfirst[heap_index(level + 1, sm)] = s_hbase + s_hw[n];
     // partition cursors of the next level (every cursor was read above)
     J.cursor[2 * lc] = s_off[lc];
-    J.cursor[2 * lc + 1] = s_off[lc + 1];
+    J.cursor[2 * lc + 1] = 0;
     J.cursor[2 * rc] = s_off[rc];
-    J.cursor[2 * rc + 1] = s_off[rc + 1];
+    J.cursor[2 * rc + 1] = 0;
   }
   HistWork* hout = L.hist[level + 1] + s_hbase;
   for (uint32_t e = tid; e < htot; e += 256) {
