@@ -53,7 +53,9 @@ __device__ inline uint32_t part_slots(uint32_t len) {
 // In-place exclusive scan of a[0..n) (LDS, n <= 1024) by a 256-thread
 // workgroup; returns the total to every thread.  tmp: 256 words of LDS.
 __device__ inline uint32_t block_scan_excl(uint32_t* a, uint32_t n, uint32_t* tmp) {
-  const uint32_t tid = threadIdx.x, k0 = tid * 4;
+  // four elements per thread, an inclusive shuffle scan of the thread sums
+  // per wave, then the 4 wave totals through LDS: two barriers in all
+  const uint32_t tid = threadIdx.x, k0 = tid * 4, lane = tid & 63u, wave = tid >> 6;
   uint32_t loc[4], sum = 0;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -61,15 +63,22 @@ __device__ inline uint32_t block_scan_excl(uint32_t* a, uint32_t n, uint32_t* tm
     loc[i] = sum;
     sum += v;
   }
-  tmp[tid] = sum;
-  __syncthreads();
-  for (uint32_t off = 1; off < 256; off <<= 1) {
-    const uint32_t v = tid >= off ? tmp[tid - off] : 0u;
-    __syncthreads();
-    tmp[tid] += v;
-    __syncthreads();
+  uint32_t incl = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(incl, o, 64);
+    if (lane >= static_cast<uint32_t>(o)) incl += u;
   }
-  const uint32_t excl = tmp[tid] - sum, total = tmp[255];
+  if (lane == 63u) tmp[wave] = incl;
+  __syncthreads();
+  uint32_t wbase = 0, total = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < 4; ++w) {
+    const uint32_t t = tmp[w];
+    wbase += w < wave ? t : 0u;
+    total += t;
+  }
+  const uint32_t excl = wbase + incl - sum;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
     if (k0 + i < n) a[k0 + i] = excl + loc[i];
@@ -1298,6 +1307,7 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
   uint32_t* offn = J.off + static_cast<size_t>(level + 1) * stride;
   // the nodes' segments and left cursors, in flight while the state is checked
   uint32_t p_lo[2] = {0, 0}, p_hi[2] = {0, 0}, p_cur[2] = {0, 0};
+  float p_hl[2] = {0.f, 0.f}, p_hr[2] = {0.f, 0.f};  // single channel: the children's H
 #pragma unroll
   for (uint32_t i = 0; i < 2; ++i) {
     const uint32_t n = tid + 256u * i;
@@ -1305,6 +1315,10 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
       p_lo[i] = offd[n];
       p_hi[i] = offd[n + 1];
       p_cur[i] = J.cursor[2 * n];
+      if (J.K == 1) {
+        p_hl[i] = J.nh[heap_index(level + 1, 2 * n)];
+        p_hr[i] = J.nh[heap_index(level + 1, 2 * n) + 1];
+      }
     }
   }
   const uint32_t p_end = tid == 0 ? offd[nn] : 0u;
@@ -1343,9 +1357,16 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
     // stop test of the next level (symmetric_tree.cpp:115-124; H sums the
     // channels in class order)
     float hl = 0.f, hr = 0.f;
-    for (uint32_t c = 0; c < J.K; ++c) {
-      hl = __fadd_rn(hl, J.nh[cl * J.K + c]);
-      hr = __fadd_rn(hr, J.nh[(cl + 1) * J.K + c]);
+    if (len == 0) {
+      // an empty node's children: zero totals (set above)
+    } else if (J.K == 1) {
+      hl = p_hl[i];
+      hr = p_hr[i];
+    } else {
+      for (uint32_t c = 0; c < J.K; ++c) {
+        hl = __fadd_rn(hl, J.nh[cl * J.K + c]);
+        hr = __fadd_rn(hr, J.nh[(cl + 1) * J.K + c]);
+      }
     }
     if ((acc >= 2u * J.min_samples_leaf && hl >= mcw2) ||
         (len - acc >= 2u * J.min_samples_leaf && hr >= mcw2))
