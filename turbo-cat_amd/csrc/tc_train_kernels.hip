@@ -1257,14 +1257,17 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(const JobDev* __rest
   }
   __syncthreads();
   __shared__ uint32_t s_tot[2];
-  if (tid < 2) {  // offsets in (step, wave) order
-    uint32_t acc = 0;
-    for (int i = 0; i < kIt * kW; ++i) {
-      const uint32_t v = s_cnt[tid][i];
-      s_cnt[tid][i] = acc;
-      acc += v;
+  static_assert(kIt * kW == 64, "one wave scans one side's (step, wave) counts");
+  if (wv < 2) {  // offsets in (step, wave) order: wave s scans side s, lane = entry
+    const uint32_t v = s_cnt[wv][lane];
+    uint32_t incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t u = __shfl_up(incl, o, 64);
+      if (lane >= static_cast<uint32_t>(o)) incl += u;
     }
-    s_tot[tid] = acc;
+    s_cnt[wv][lane] = incl - v;
+    if (lane == 63) s_tot[wv] = incl;
   }
   __syncthreads();
   if (tid == 0) {  // one 64-bit reservation for both sides: left position, right rows taken
