@@ -262,7 +262,7 @@ def test_work_item_shape_never_changes_the_model(ctx, loss):
         if hc:
             assert (root, deep_items) == (hc, di)
         else:
-            assert 1024 <= root <= 16384 and deep_items in (256, 512, 1024)
+            assert 1024 <= root <= 16384 and deep_items in (128, 256, 512, 1024)
         models.append(tr.model())
         tr.close()
     for m in models[1:]:

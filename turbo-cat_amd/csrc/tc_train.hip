@@ -404,8 +404,8 @@ class Trainer {
   struct TuneChoice {
     uint32_t root_items, deep_items;
   };
-  static constexpr TuneChoice kTune[6] = {{1024, 512}, {512, 512},  {512, 1024},
-                                          {1024, 1024}, {512, 256}, {2048, 1024}};
+  static constexpr TuneChoice kTune[8] = {{1024, 512}, {512, 512},  {512, 1024}, {1024, 1024},
+                                          {512, 256},  {2048, 1024}, {512, 128}, {1024, 256}};
 
   // Upper bound of job J's histogram work items at level d.  Below the root
   // the histogrammed (smaller) children hold at most N/2 rows over 2^(d-1)
