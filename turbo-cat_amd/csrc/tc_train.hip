@@ -227,12 +227,14 @@ class Trainer {
         TC_HIP(hipEventCreate(&e1));
         TC_HIP(hipEventRecord(e0, s));
       }
-      if (packed)
-        hipLaunchKernelGGL((k_hist<kHistUnroll, true>), dim3(round8(hist_cap_[d])), dim3(kHistThreads),
-                           0, s, d_jobs_, lists_.hist[d], lists_.hist_count[d], d, partials_);
-      else
-        hipLaunchKernelGGL((k_hist<kHistUnroll, false>), dim3(round8(hist_cap_[d])), dim3(kHistThreads),
-                           0, s, d_jobs_, lists_.hist[d], lists_.hist_count[d], d, partials_);
+      {
+        const dim3 grid(round8(hist_cap_[d]));
+        const bool one = njobs_ == 1;
+        auto* k = packed ? (one ? k_hist<kHistUnroll, true, true> : k_hist<kHistUnroll, true, false>)
+                         : (one ? k_hist<kHistUnroll, false, true> : k_hist<kHistUnroll, false, false>);
+        hipLaunchKernelGGL(k, grid, dim3(kHistThreads), 0, s, d_jobs_, lists_.hist[d],
+                           lists_.hist_count[d], d, partials_);
+      }
       if (hist_events) {
         TC_HIP(hipEventRecord(e1, s));
         hist_events->emplace_back(e0, e1);
@@ -241,8 +243,8 @@ class Trainer {
                          static_cast<const float2*>(nullptr), d, partials_);
       if (d + 1 < D_ || keep_rows_) {  // training never reads the last level's rows
         // the partition also finalizes the level (split record, child totals)
-        hipLaunchKernelGGL(k_partition, dim3(part_cap_[d]), dim3(kPartThreads), 0, s, d_jobs_, lists_.part[d],
-                           lists_.part_count[d], d);
+        hipLaunchKernelGGL(njobs_ == 1 ? k_partition<true> : k_partition<false>, dim3(part_cap_[d]),
+                           dim3(kPartThreads), 0, s, d_jobs_, lists_.part[d], lists_.part_count[d], d);
         hipLaunchKernelGGL(k_part_plan, dim3(1, njobs_), dim3(256), 0, s, d_jobs_, lists_, d);
       } else {
         // one wave per (node, channel) unit of the widest job
@@ -1120,7 +1122,7 @@ int32_t tc_hist_build(tc_dataset* ds, const float* g, const float* h, const uint
       JobDev* dj = static_cast<JobDev*>(al(sizeof(JobDev)));
       TC_HIP(hipMemcpyAsync(dj, &J, sizeof(J), hipMemcpyHostToDevice, s));
       if (cnt)
-        hipLaunchKernelGGL((k_hist<4, false>), dim3(round8(cnt)), dim3(kHistThreads), 0, s, dj, dw,
+        hipLaunchKernelGGL((k_hist<4, false, true>), dim3(round8(cnt)), dim3(kHistThreads), 0, s, dj, dw,
                            dc, 0u, static_cast<HistCell*>(nullptr));
       hipLaunchKernelGGL(k_hist_to_float, dim3(ceil_div(F * 256ull, 256)), dim3(256), 0, s, J.hist,
                          J.st, F * 256u, fh);

@@ -668,7 +668,8 @@ __global__ void __launch_bounds__(256) k_tree_begin(const JobDev* __restrict__ j
 // for the whole chip, several times the cost of the stores.  Otherwise the
 // non-empty cells are added into the zeroed node histogram with int64
 // atomics.
-template <int U, bool PACKED>
+// SINGLE: one job, so its table entry is read without waiting for the item.
+template <int U, bool PACKED, bool SINGLE>
 __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict__ jobs,
                                                        const HistWork* __restrict__ work,
                                                        const uint32_t* __restrict__ count,
@@ -680,7 +681,7 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
   uint32_t li;
   if (!xcd_rank(blockIdx.x, *count, &li)) return;
   const HistWork w = work[li];
-  const JobDev& J = jobs[w.job];  // items exist only while the job's tree grows
+  const JobDev& J = jobs[SINGLE ? 0u : w.job];  // items exist only while the job's tree grows
   const uint32_t tid = threadIdx.x;
   for (uint32_t i = tid; i < kCells; i += kHistThreads) {
     s_a[i] = 0ull;
@@ -1207,6 +1208,7 @@ __device__ inline uint32_t level_row(const JobDev& J, uint32_t level, uint32_t p
   return (level == 0 && J.root_identity) ? pos : J.rows[static_cast<size_t>(level) * J.N + pos];
 }
 
+template <bool SINGLE>
 __global__ void __launch_bounds__(kPartThreads) k_partition(const JobDev* __restrict__ jobs,
                                                    const PartWork* __restrict__ work,
                                                    const uint32_t* __restrict__ count,
@@ -1217,7 +1219,7 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(const JobDev* __rest
   const uint32_t cnt = *count;
   const PartWork w = work[blockIdx.x];
   if (blockIdx.x >= cnt) return;
-  const JobDev& J = jobs[w.job];
+  const JobDev& J = jobs[SINGLE ? 0u : w.job];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   // the item's row ids do not depend on the split: in flight while it is decoded
   uint32_t rv[kIt];
