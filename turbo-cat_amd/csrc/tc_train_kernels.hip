@@ -7,19 +7,25 @@
 // data-dependent control (level stops, tree count) lives in TreeState and
 // makes later kernels of a stopped job return at once.
 //
-//   root_partials, tree_begin
+//   [root_partials]           (only for jobs whose gradient update cannot write them)
+//   tree_begin
 //   for level d < D:
 //     [d=0] hist(0, root)
-//     [d>0] hist(d, smaller children), subtract(d)
-//     split_scan(d), finalize(d)
-//     [d<D-1] partition(d), part_plan(d)
-//   grad_update(apply)        (leaf values, prediction update, gradients)
+//     [d>0] hist(d, smaller children)
+//     split_scan(d)           (partial slots summed, sibling = parent - smaller)
+//     [d<D-1] partition(d)    (finalizes the level: split record, child totals)
+//             part_plan(d)
+//     [d=D-1] finalize(d)
+//   grad_update(apply)        (leaf values, prediction update, gradients,
+//                              validation rows, next root partials)
+//   [eval]                    (losses and early stopping)
 //
 // Histograms are exact fixed-point sums: a row's g and h enter as int64
 // round(v * 2^e), with e chosen per tree from max |g|, max |h| and the row
 // count so that no sum overflows.  They are accumulated in LDS with integer
-// atomics (on gfx950 about 20x the rate of LDS float atomics) and merged into
-// HBM with int64 atomics.  Integer addition is order-independent, so every
+// atomics (on gfx950 about 20x the rate of LDS float atomics); each work item
+// stores its cells (a node's last chunk into the node's histogram, the others
+// into partial slots the split scan adds).  Integer addition is order-independent, so every
 // histogram, and every split, is the same run to run and the same for a job
 // trained alone or in a batch.  The same exactness makes the order of rows
 // inside a node irrelevant, so the partition is a single unstable pass, and
