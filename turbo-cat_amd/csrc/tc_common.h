@@ -29,7 +29,7 @@ namespace tc {
 
 constexpr int kMaxDepth = 10;          // 2^10 leaves; u16 leaf index range
 constexpr int kTileF = 16;             // features per histogram tile (16 B)
-constexpr int kHistThreads = 512;
+constexpr int kHistThreads = 1024;     // 16 waves; two workgroups per CU (64 KB LDS, <= 64 VGPRs)
 constexpr int kPartChunk = 4096;       // rows per partition work item
 constexpr int kPartThreads = 1024;
 constexpr uint32_t kMaxHistChunk = 32768;  // rows per histogram work item (packed count < 2^16)
@@ -164,13 +164,26 @@ __host__ __device__ inline uint32_t rm_stride(uint32_t F) {
   return p;
 }
 
-// Rows per histogram work item for `work` row-tiles cut into about `items`
-// work items: the smallest power of two in [1024, 16384] that does not give
-// more items.
+// Rows per histogram work item below the root, for `work` row-tiles cut into
+// about `items` work items: the smallest power of two in [1024, 16384] that
+// does not give more items.  (Cutting each level into exactly `items` equal
+// pieces measured slower: more items, each flushing a full 64 KB tile.)
 __host__ __device__ inline uint32_t chunk_for(uint64_t work, uint32_t items) {
   uint32_t ch = 1024;
   while (ch < 16384u && work / ch > items) ch <<= 1;
   return ch;
+}
+
+// Rows per root work item when the root's `rows` rows of `tk` tile-channels
+// are cut into at most `items` work items: rows / floor(items / tk), rounded
+// up, in [1024, 16384].  All root items start at once when `items` is twice
+// the CU count: 511 items of 13,699 rows for 1M rows x 7 tiles and 512 items
+// run two per CU on 256 CUs, where 16,384-row items (434 of them) leave some
+// CUs one item and most two.
+__host__ __device__ inline uint32_t root_chunk(uint64_t rows, uint32_t tk, uint32_t items) {
+  const uint64_t per = items / tk > 0 ? items / tk : 1u;
+  const uint64_t ch = (rows + per - 1) / per;
+  return ch < 1024u ? 1024u : (ch > 16384u ? 16384u : static_cast<uint32_t>(ch));
 }
 
 __host__ __device__ inline uint32_t heap_index(uint32_t level, uint32_t node) {

@@ -20,7 +20,7 @@ namespace tc {
 
 namespace {
 
-constexpr int kHistUnroll = 4;
+constexpr int kHistUnroll = 2;  // rows per lane in flight in k_hist (U = 3 needs more than 64 VGPRs)
 
 uint32_t ceil_div(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
 uint32_t round8(uint32_t x) { return std::max(8u, (x + 7u) & ~7u); }
@@ -430,7 +430,7 @@ class Trainer {
       J.deep_items = specs_[j].p.hist_deep_items;
       return;
     }
-    J.hist_chunk = chunk_for(work_[j], c.root_items);
+    J.hist_chunk = root_chunk(J.N, J.T * J.K, c.root_items);
     J.deep_items = c.deep_items;
   }
 
@@ -583,7 +583,6 @@ class Trainer {
   void setup() {
     hipStream_t s = ctx_->stream;
     h_jobs_.resize(njobs_);
-    work_.assign(njobs_, 0);
     fixed_chunk_.assign(njobs_, 0);
     std::fill(list_cap_, list_cap_ + kMaxDepth, 0u);
     hist_rows_.resize(njobs_);
@@ -622,7 +621,6 @@ class Trainer {
       J.max_delta_step = S.p.tree.max_delta_step; J.min_child_weight = S.p.tree.min_child_weight;
       J.min_split_gain = S.p.tree.min_split_gain;
       J.base = S.base;
-      work_[j] = static_cast<uint64_t>(N) * J.T * K;
       fixed_chunk_[j] = S.p.hist_chunk ? std::min(S.p.hist_chunk, kMaxHistChunk) : 0u;
       if (!fixed_chunk_[j]) tune_ = true;
       set_chunks(j, kTune[0]);
@@ -796,7 +794,6 @@ class Trainer {
   bool keep_rows_ = false;  // partition the last level too (leaf_index)
   bool eval_ = false;       // some job records losses / early-stops (k_eval per tree)
   // histogram work-item shapes (rows per item at the root and below)
-  std::vector<uint64_t> work_;          // per job: rows x tiles x channels
   std::vector<uint32_t> fixed_chunk_;   // per job: caller's hist_chunk, 0 = tuned
   uint32_t list_cap_[kMaxDepth] = {};   // allocated work-list entries per level
   bool tune_ = false, tuned_ = false;
@@ -1122,8 +1119,8 @@ int32_t tc_hist_build(tc_dataset* ds, const float* g, const float* h, const uint
       JobDev* dj = static_cast<JobDev*>(al(sizeof(JobDev)));
       TC_HIP(hipMemcpyAsync(dj, &J, sizeof(J), hipMemcpyHostToDevice, s));
       if (cnt)
-        hipLaunchKernelGGL((k_hist<4, false, true>), dim3(round8(cnt)), dim3(kHistThreads), 0, s, dj, dw,
-                           dc, 0u, static_cast<HistCell*>(nullptr));
+        hipLaunchKernelGGL((k_hist<kHistUnroll, false, true>), dim3(round8(cnt)), dim3(kHistThreads),
+                           0, s, dj, dw, dc, 0u, static_cast<HistCell*>(nullptr));
       hipLaunchKernelGGL(k_hist_to_float, dim3(ceil_div(F * 256ull, 256)), dim3(256), 0, s, J.hist,
                          J.st, F * 256u, fh);
       TC_HIP(hipGetLastError());

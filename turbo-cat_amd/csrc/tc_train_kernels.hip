@@ -706,7 +706,7 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
   const uint32_t f0 = w.tile * kTileF;
   const uint32_t nf = min(static_cast<uint32_t>(kTileF), J.F - f0);  // real features of the tile
 
-  // Software pipeline over batches of U x 512 rows: while batch i is added
+  // Software pipeline over batches of U x 1024 rows: while batch i is added
   // into LDS, the bins and (g, h) of batch i+1 and the row ids of batch i+2
   // are in flight.
   constexpr uint32_t STEP = kHistThreads * U;
