@@ -164,16 +164,6 @@ __host__ __device__ inline uint32_t rm_stride(uint32_t F) {
   return p;
 }
 
-// Rows per histogram work item below the root, for `work` row-tiles cut into
-// about `items` work items: the smallest power of two in [1024, 16384] that
-// does not give more items.  (Cutting each level into exactly `items` equal
-// pieces measured slower: more items, each flushing a full 64 KB tile.)
-__host__ __device__ inline uint32_t chunk_for(uint64_t work, uint32_t items) {
-  uint32_t ch = 1024;
-  while (ch < 16384u && work / ch > items) ch <<= 1;
-  return ch;
-}
-
 // Rows per root work item when the root's `rows` rows of `tk` tile-channels
 // are cut into at most `items` work items: rows / floor(items / tk), rounded
 // up, in [1024, 16384].  All root items start at once when `items` is twice
@@ -184,6 +174,24 @@ __host__ __device__ inline uint32_t root_chunk(uint64_t rows, uint32_t tk, uint3
   const uint64_t per = items / tk > 0 ? items / tk : 1u;
   const uint64_t ch = (rows + per - 1) / per;
   return ch < 1024u ? 1024u : (ch > 16384u ? 16384u : static_cast<uint32_t>(ch));
+}
+
+// Below the root a level's chunk is the smallest of kChunkCands candidates,
+// base x (1 + k/4) with base = root_chunk(level rows, 1, budget), whose
+// pieces (ceil(rows / chunk) per histogrammed node) fit the budget.  Each
+// piece costs about its rows on one CU, so the level takes about its largest
+// piece; powers of two overshot the fitted size by 1.4x on average.
+constexpr uint32_t kChunkCands = 8;
+__host__ __device__ inline uint32_t chunk_cand(uint32_t base, uint32_t k) {
+  const uint32_t c = base + base * k / 4u;
+  return c < 16384u ? c : 16384u;
+}
+
+// A node of n rows is cut into ceil(n / chunk) pieces of equal length (the
+// last up to one piece shorter; none empty); the length of each but the last.
+__host__ __device__ inline uint32_t piece_rows(uint32_t n, uint32_t chunk) {
+  const uint32_t p = (n + chunk - 1) / chunk;
+  return p ? (n + p - 1) / p : 0u;
 }
 
 __host__ __device__ inline uint32_t heap_index(uint32_t level, uint32_t node) {

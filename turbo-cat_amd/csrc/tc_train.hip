@@ -417,8 +417,9 @@ class Trainer {
     if (d == 0) return ceil_div(J.N, J.hist_chunk) * TK;
     const uint32_t pairs = 1u << (d - 1);
     if (!J.deep_items) return (ceil_div(J.N / 2, J.hist_chunk) + pairs) * TK;
-    // chunk_for: floor(rows x TK / chunk) <= deep_items unless the chunk is
-    // 16384; the items of a pair are at most its rows / chunk + 1
+    // k_part_plan's chunk: at most deep_items / TK pieces when a candidate
+    // fits; otherwise a chunk >= rows / (deep_items / TK) or 16384 (rows <=
+    // N/2), with at most rows / chunk + 1 pieces per pair
     const uint32_t full = std::max((J.deep_items + 1) / TK + 2, ceil_div(J.N / 2, 16384) + 1);
     return (full + pairs) * TK;
   }

@@ -1335,13 +1335,16 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
   const uint32_t p_end = tid == 0 ? offd[nn] : 0u;
   if (!part_active(J, level)) return;
   __shared__ uint32_t s_hw[1 << (kMaxDepth - 1)];    // per pair: hist work items -> first item
+  __shared__ uint32_t s_pl[1 << (kMaxDepth - 1)];    // per pair: rows per piece
   __shared__ uint32_t s_pw[1 << kMaxDepth];          // per child: partition slots -> first slot
   __shared__ uint32_t s_off[(1 << kMaxDepth) + 1];   // child offsets
   __shared__ uint32_t s_tmp[256];
-  __shared__ uint32_t s_hbase, s_pbase, s_chunk;
+  __shared__ uint32_t s_hbase, s_pbase;
+  __shared__ uint32_t s_cnt[kChunkCands];
   __shared__ int s_any;
   __shared__ unsigned long long s_rows;
   if (tid == 0) { s_any = 0; s_rows = 0ull; }
+  if (tid < kChunkCands) s_cnt[tid] = 0;
   __syncthreads();
   const float mcw2 = J.min_child_weight * 2.0f;
 #pragma unroll
@@ -1399,16 +1402,30 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
     if (tid == 0) J.st->active = 0;
     return;
   }
-  // rows per work item of the next level: about J.deep_items items over its
-  // rows, or the fixed chunk
+  // rows per work item of the next level: the smallest candidate whose
+  // pieces (ceil(rows / chunk) per histogrammed child) fit J.deep_items
+  // items, so that each CU gets one item of about the same size; or the
+  // fixed chunk
   const uint32_t TK = J.T * J.K;
-  if (tid == 0) {
-    s_chunk = J.deep_items ? chunk_for(s_rows * TK, J.deep_items) : J.hist_chunk;
-    J.lvl_chunk[level + 1] = s_chunk;
+  uint32_t chunk = J.hist_chunk;
+  if (J.deep_items) {
+    // thread: candidate tid % 8 over pairs tid / 8, tid / 8 + 32, ...
+    const uint32_t budget = max(1u, J.deep_items / TK);  // pieces
+    const uint32_t base = root_chunk(s_rows, 1u, budget);
+    const uint32_t k = tid % kChunkCands, c = chunk_cand(base, k);
+    uint32_t cnt = 0;
+    for (uint32_t n = tid / kChunkCands; n < nn; n += 256u / kChunkCands) cnt += (s_hw[n] + c - 1) / c;
+    if (cnt) atomicAdd(&s_cnt[k], cnt);
+    __syncthreads();
+    chunk = chunk_cand(base, kChunkCands - 1);
+    for (int q = kChunkCands - 1; q >= 0; --q)
+      if (s_cnt[q] <= budget) chunk = chunk_cand(base, q);
   }
-  __syncthreads();
-  const uint32_t chunk = s_chunk;
-  for (uint32_t n = tid; n < nn; n += 256) s_hw[n] = ((s_hw[n] + chunk - 1) / chunk) * TK;
+  if (tid == 0) J.lvl_chunk[level + 1] = chunk;
+  for (uint32_t n = tid; n < nn; n += 256) {
+    s_pl[n] = piece_rows(s_hw[n], chunk);
+    s_hw[n] = ((s_hw[n] + chunk - 1) / chunk) * TK;
+  }
   __syncthreads();
   const uint32_t htot = block_scan_excl(s_hw, nn, s_tmp);
   const uint32_t ptot = block_scan_excl(s_pw, 2 * nn, s_tmp);
@@ -1436,10 +1453,10 @@ __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jo
     const uint32_t nl = s_off[lc + 1] - s_off[lc], nr = s_off[rc + 1] - s_off[rc];
     const uint32_t sm = nl <= nr ? lc : rc;
     const uint32_t le = e - s_hw[n], r = le % TK;
-    const uint32_t start = s_off[sm] + (le / TK) * chunk;
-    const uint32_t end = s_off[sm + 1];
-    put_hist_work(hout + e, job, sm, start, min(chunk, end - start), r / J.K, r % J.K,
-                  start + chunk >= end);
+    const uint32_t end = s_off[sm + 1], pl = s_pl[n];
+    const uint32_t start = s_off[sm] + (le / TK) * pl;
+    put_hist_work(hout + e, job, sm, start, min(pl, end - start), r / J.K, r % J.K,
+                  start + pl >= end);
   }
   PartWork* pout = L.part[level + 1] + s_pbase;
   for (uint32_t e = tid; e < ptot; e += 256) {
