@@ -86,8 +86,8 @@ typedef struct {
                                 otherwise 0 or 1 */
   uint32_t hist_deep_items;  /* with hist_chunk != 0: 0 = hist_chunk rows per work item at
                                 every level, else hist_chunk rows at the root and each deeper
-                                level cut into about this many items (<= 65536); ignored
-                                when autotuned.  Never changes the model. */
+                                level cut into at most about this many items (<= 65536);
+                                ignored when autotuned.  Never changes the model. */
 } tc_train_params;
 
 /* Host-side model arrays filled by training (caller-allocated).
@@ -244,8 +244,8 @@ int32_t tc_trainer_hist_rows(tc_trainer* tr, uint32_t job, uint64_t* rows);
  * level).  With hist_chunk = 0 the shape is picked by timing candidate
  * shapes on the fit's first trees before the per-tree graph is captured
  * (every histogram is an exact integer sum, so the shape never changes the
- * model); otherwise root_rows = hist_chunk and deep_items = 0.  No reference
- * counterpart. */
+ * model); otherwise root_rows = hist_chunk and deep_items = hist_deep_items.
+ * No reference counterpart. */
 int32_t tc_trainer_hist_chunks(tc_trainer* tr, uint32_t job, uint32_t* root_rows,
                                uint32_t* deep_items);
 int32_t tc_trainer_destroy(tc_trainer* tr);
