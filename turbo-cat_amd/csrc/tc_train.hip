@@ -228,12 +228,17 @@ class Trainer {
         TC_HIP(hipEventRecord(e0, s));
       }
       {
+        // one job: below the root the histogram launch plans its own level
         const dim3 grid(round8(hist_cap_[d]));
-        const bool one = njobs_ == 1;
-        auto* k = packed ? (one ? k_hist<kHistUnroll, true, true> : k_hist<kHistUnroll, true, false>)
-                         : (one ? k_hist<kHistUnroll, false, true> : k_hist<kHistUnroll, false, false>);
+        const bool one = njobs_ == 1, plan = one && d > 0;
+        auto* k = packed ? (plan ? k_hist<kHistUnroll, true, true, true>
+                                 : one ? k_hist<kHistUnroll, true, true, false>
+                                       : k_hist<kHistUnroll, true, false, false>)
+                         : (plan ? k_hist<kHistUnroll, false, true, true>
+                                 : one ? k_hist<kHistUnroll, false, true, false>
+                                       : k_hist<kHistUnroll, false, false, false>);
         hipLaunchKernelGGL(k, grid, dim3(kHistThreads), 0, s, d_jobs_, lists_.hist[d],
-                           lists_.hist_count[d], d, partials_);
+                           lists_.hist_count[d], d, partials_, lists_);
       }
       if (hist_events) {
         TC_HIP(hipEventRecord(e1, s));
@@ -245,7 +250,8 @@ class Trainer {
         // the partition also finalizes the level (split record, child totals)
         hipLaunchKernelGGL(njobs_ == 1 ? k_partition<true> : k_partition<false>, dim3(part_cap_[d]),
                            dim3(kPartThreads), 0, s, d_jobs_, lists_.part[d], lists_.part_count[d], d);
-        hipLaunchKernelGGL(k_part_plan, dim3(1, njobs_), dim3(256), 0, s, d_jobs_, lists_, d);
+        if (njobs_ > 1 || d + 1 == D_)  // otherwise the next histogram launch plans the level
+          hipLaunchKernelGGL(k_part_plan, dim3(1, njobs_), dim3(256), 0, s, d_jobs_, lists_, d);
       } else {
         // one wave per (node, channel) unit of the widest job
         hipLaunchKernelGGL(k_finalize, dim3(ceil_div(static_cast<uint64_t>(kmax_) << d, 4), njobs_),
@@ -644,7 +650,7 @@ class Trainer {
       J.lvl_chunk = alloc<uint32_t>(D_);
       J.st = alloc<TreeState>(1);
       hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1), 0, s, J.st);
-      J.cursor = alloc<uint32_t>(1u << D_);
+      J.cursor = alloc<uint32_t>(2u << D_);  // two sets (level parity) of 2^(D-1) nodes x 2 words
       J.grad_wg_cnt = std::max(1u, ceil_div(N, kGradTile));
       J.grad_wg_off = wg_off[j];
       if (S.valid) {  // validation rows ride on the gradient update's grid
@@ -1120,8 +1126,9 @@ int32_t tc_hist_build(tc_dataset* ds, const float* g, const float* h, const uint
       JobDev* dj = static_cast<JobDev*>(al(sizeof(JobDev)));
       TC_HIP(hipMemcpyAsync(dj, &J, sizeof(J), hipMemcpyHostToDevice, s));
       if (cnt)
-        hipLaunchKernelGGL((k_hist<kHistUnroll, false, true>), dim3(round8(cnt)), dim3(kHistThreads),
-                           0, s, dj, dw, dc, 0u, static_cast<HistCell*>(nullptr));
+        hipLaunchKernelGGL((k_hist<kHistUnroll, false, true, false>), dim3(round8(cnt)),
+                           dim3(kHistThreads), 0, s, dj, dw, dc, 0u, static_cast<HistCell*>(nullptr),
+                           Lists{});
       hipLaunchKernelGGL(k_hist_to_float, dim3(ceil_div(F * 256ull, 256)), dim3(256), 0, s, J.hist,
                          J.st, F * 256u, fh);
       TC_HIP(hipGetLastError());

@@ -649,6 +649,218 @@ __global__ void __launch_bounds__(256) k_tree_begin(const JobDev* __restrict__ j
 }
 
 // ---------------------------------------------------------------------------
+// Level plan: what the partition of a level leaves for the level below.
+__device__ inline bool part_active(const JobDev& J, uint32_t level) {
+  return J.st->depth == static_cast<int32_t>(level) + 1 && J.st->tree < static_cast<int>(J.n_trees);
+}
+
+// Partition cursors of a level: two sets, by the level's parity, so that the
+// plan of the next level can reset one set while the other is still read.
+__device__ inline uint32_t* cursor_set(const JobDev& J, uint32_t level) {
+  return J.cursor + ((level & 1u) << J.D);
+}
+
+// LDS of a level plan (about 12 KB).
+struct PlanShared {
+  uint32_t hw[1 << (kMaxDepth - 1)];   // per pair: smaller child's rows -> items -> first item
+  uint32_t pl[1 << (kMaxDepth - 1)];   // per pair: rows per piece
+  uint32_t pw[1 << kMaxDepth];         // per child: partition slots -> first slot
+  uint32_t off[(1 << kMaxDepth) + 1];  // child offsets
+  uint32_t tmp[16];
+  uint32_t cnt[kChunkCands];
+  uint32_t chunk, htot, hbase, pbase;
+  int any;
+  unsigned long long rows;
+};
+
+// Histogram work item e of a planned level: pair n = the last with first
+// item <= e, its smaller child (ties: left), piece (e - first) / (tiles x K),
+// then tile and channel.  nn: nodes of the partitioned level.
+__device__ inline HistWork plan_item(const JobDev& J, const PlanShared& S, uint32_t nn, uint32_t e,
+                                     uint32_t job) {
+  const uint32_t TK = J.T * J.K;
+  const uint32_t n = find_item(S.hw, nn, e);
+  const uint32_t lc = 2 * n, rc = lc + 1;
+  const uint32_t nl = S.off[lc + 1] - S.off[lc], nr = S.off[rc + 1] - S.off[rc];
+  const uint32_t sm = nl <= nr ? lc : rc;
+  const uint32_t le = e - S.hw[n], r = le % TK;
+  const uint32_t end = S.off[sm + 1], pl = S.pl[n];
+  const uint32_t start = S.off[sm] + (le / TK) * pl;
+  HistWork w;
+  w.job = job; w.node = sm; w.start = start; w.len = min(pl, end - start);
+  w.tile = r / J.K; w.cls = r % J.K; w.last = start + pl >= end ? 1u : 0u; w.pad = 0;
+  return w;
+}
+
+// Plan of the level below `level` for job J, by one workgroup of NT threads,
+// from level `level`'s partition (its offsets and left cursors): the
+// children's counts and offsets (children of an empty node get zero
+// totals), the next level's stop test (H summed over channels), its rows
+// per histogram work item, and its work items (block scans over pairs).
+// Returns whether the next level is histogrammed; then S.hw holds each
+// pair's first item, S.pl its rows per piece, S.off the child offsets,
+// S.chunk and S.htot.  emit 0: the plan only (a histogram workgroup's own
+// item); 1: also store what later launches read (counts, offsets, stop flag,
+// chunk, first items, the next partition's cursors and work list); 2: also
+// the histogram work list.
+template <uint32_t NT>
+__device__ bool level_plan(const JobDev& J, const Lists& L, uint32_t job, uint32_t level,
+                           PlanShared& S, int emit) {
+  constexpr uint32_t PER = (512u + NT - 1) / NT;  // pairs per thread (nn <= 512)
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nn = 1u << level;
+  const uint32_t stride = (1u << J.D) + 1u;
+  const uint32_t* offd = J.off + static_cast<size_t>(level) * stride;
+  uint32_t* offn = J.off + static_cast<size_t>(level + 1) * stride;
+  const uint32_t* curd = cursor_set(J, level);
+  // the nodes' segments and left cursors, in flight while the state is checked
+  uint32_t p_lo[PER], p_hi[PER], p_cur[PER];
+  float p_hl[PER], p_hr[PER];  // single channel: the children's H
+#pragma unroll
+  for (uint32_t i = 0; i < PER; ++i) {
+    const uint32_t n = tid + NT * i;
+    p_lo[i] = p_hi[i] = p_cur[i] = 0;
+    p_hl[i] = p_hr[i] = 0.f;
+    if (n < nn) {
+      p_lo[i] = offd[n];
+      p_hi[i] = offd[n + 1];
+      p_cur[i] = curd[2 * n];
+      if (J.K == 1) {
+        p_hl[i] = J.nh[heap_index(level + 1, 2 * n)];
+        p_hr[i] = J.nh[heap_index(level + 1, 2 * n) + 1];
+      }
+    }
+  }
+  const uint32_t p_end = tid == 0 ? offd[nn] : 0u;
+  if (!part_active(J, level)) return false;
+  if (tid == 0) { S.any = 0; S.rows = 0ull; }
+  if (tid < kChunkCands) S.cnt[tid] = 0;
+  __syncthreads();
+  const float mcw2 = J.min_child_weight * 2.0f;
+#pragma unroll
+  for (uint32_t i = 0; i < PER; ++i) {
+    const uint32_t n = tid + NT * i;
+    if (n >= nn) break;
+    const uint32_t len = p_hi[i] - p_lo[i];
+    const uint32_t acc = p_cur[i] - p_lo[i];  // rows sent left
+    const uint32_t cl = heap_index(level + 1, 2 * n);
+    if (emit) {
+      J.nc[cl] = acc;
+      J.nc[cl + 1] = len - acc;
+      if (len == 0) {  // an empty node's children (no partition item computed them)
+        for (uint32_t c = 0; c < J.K; ++c) {
+          J.ng[cl * J.K + c] = 0.f;
+          J.nh[cl * J.K + c] = 0.f;
+          J.ng[(cl + 1) * J.K + c] = 0.f;
+          J.nh[(cl + 1) * J.K + c] = 0.f;
+        }
+      }
+      offn[2 * n] = p_lo[i];
+      offn[2 * n + 1] = p_lo[i] + acc;
+    }
+    S.off[2 * n] = p_lo[i];
+    S.off[2 * n + 1] = p_lo[i] + acc;
+    // stop test of the next level (symmetric_tree.cpp:115-124; H sums the
+    // channels in class order)
+    float hl = 0.f, hr = 0.f;
+    if (len == 0) {
+      // an empty node's children: zero totals
+    } else if (J.K == 1) {
+      hl = p_hl[i];
+      hr = p_hr[i];
+    } else {
+      for (uint32_t c = 0; c < J.K; ++c) {
+        hl = __fadd_rn(hl, J.nh[cl * J.K + c]);
+        hr = __fadd_rn(hr, J.nh[(cl + 1) * J.K + c]);
+      }
+    }
+    if ((acc >= 2u * J.min_samples_leaf && hl >= mcw2) ||
+        (len - acc >= 2u * J.min_samples_leaf && hr >= mcw2))
+      S.any = 1;
+    // next level: histogram of the smaller child of the pair (ties: left),
+    // partition slots of both children
+    const uint32_t sm_len = min(acc, len - acc);
+    S.hw[n] = sm_len;
+    S.pw[2 * n] = part_slots(acc);
+    S.pw[2 * n + 1] = part_slots(len - acc);
+    if (sm_len) atomicAdd(&S.rows, static_cast<unsigned long long>(sm_len));
+  }
+  if (tid == 0) {
+    if (emit) offn[2 * nn] = p_end;
+    S.off[2 * nn] = p_end;
+  }
+  __syncthreads();
+  if (level + 1 >= J.D || !S.any) {
+    if (emit && tid == 0) J.st->active = 0;
+    return false;
+  }
+  // rows per work item of the next level: the smallest candidate whose
+  // pieces (ceil(rows / chunk) per histogrammed child) fit J.deep_items
+  // items, so that each CU gets one item of about the same size; or the
+  // fixed chunk
+  const uint32_t TK = J.T * J.K;
+  uint32_t chunk = J.hist_chunk;
+  if (J.deep_items) {
+    // thread: candidate tid % 8 over pairs tid / 8, tid / 8 + NT / 8, ...
+    const uint32_t budget = max(1u, J.deep_items / TK);  // pieces
+    const uint32_t base = root_chunk(S.rows, 1u, budget);
+    const uint32_t k = tid % kChunkCands, c = chunk_cand(base, k);
+    uint32_t cnt = 0;
+    for (uint32_t n = tid / kChunkCands; n < nn; n += NT / kChunkCands) cnt += (S.hw[n] + c - 1) / c;
+    if (cnt) atomicAdd(&S.cnt[k], cnt);
+    __syncthreads();
+    chunk = chunk_cand(base, kChunkCands - 1);
+    for (int q = kChunkCands - 1; q >= 0; --q)
+      if (S.cnt[q] <= budget) chunk = chunk_cand(base, q);
+  }
+  if (emit && tid == 0) J.lvl_chunk[level + 1] = chunk;
+  for (uint32_t n = tid; n < nn; n += NT) {
+    S.pl[n] = piece_rows(S.hw[n], chunk);
+    S.hw[n] = ((S.hw[n] + chunk - 1) / chunk) * TK;
+  }
+  __syncthreads();
+  const uint32_t htot = block_scan_excl(S.hw, nn, S.tmp);
+  const uint32_t ptot = emit ? block_scan_excl(S.pw, 2 * nn, S.tmp) : 0u;  // emit is uniform
+  if (tid == 0) {
+    S.chunk = chunk;
+    S.htot = htot;
+    S.hbase = 0;
+    S.pbase = 0;
+    if (emit) {
+      if (emit >= 2) S.hbase = list_reserve(L, L.hist_count[level + 1], htot);
+      S.pbase = list_reserve(L, L.part_count[level + 1], ptot);
+      if (J.hist_rows_total) atomicAdd(J.hist_rows_total, S.rows);
+    }
+  }
+  __syncthreads();
+  if (!emit) return true;
+  uint32_t* curn = cursor_set(J, level + 1);  // the other set: every cursor of this level was read
+  for (uint32_t n = tid; n < nn; n += NT) {
+    // first item of each pair's smaller child
+    const uint32_t lc = 2 * n, rc = lc + 1;
+    const uint32_t sm = S.off[lc + 1] - S.off[lc] <= S.off[rc + 1] - S.off[rc] ? lc : rc;
+    J.hfirst[heap_index(level + 1, sm)] = S.hbase + S.hw[n];
+    // partition cursors of the next level
+    curn[2 * lc] = S.off[lc];
+    curn[2 * lc + 1] = 0;
+    curn[2 * rc] = S.off[rc];
+    curn[2 * rc + 1] = 0;
+  }
+  if (emit >= 2) {
+    HistWork* hout = L.hist[level + 1] + S.hbase;
+    for (uint32_t e = tid; e < htot; e += NT) hout[e] = plan_item(J, S, nn, e, job);
+  }
+  PartWork* pout = L.part[level + 1] + S.pbase;
+  for (uint32_t e = tid; e < ptot; e += NT) {
+    const uint32_t c = find_item(S.pw, 2 * nn, e);
+    const uint32_t start = S.off[c] + (e - S.pw[c]) * kPartChunk;
+    put_part_work(pout + e, job, c, start,
+                  min(static_cast<uint32_t>(kPartChunk), S.off[c + 1] - start), e);
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
 // A9: per-level histogram.  One workgroup = one work item (row chunk of one
 // node, one gradient channel) x one 16-feature tile.  Each lane loads the tile's 16 bins of a row
 // with one 16-byte load from the row-major bins and the row's (g, h), turns
@@ -675,18 +887,33 @@ __global__ void __launch_bounds__(256) k_tree_begin(const JobDev* __restrict__ j
 // non-empty cells are added into the zeroed node histogram with int64
 // atomics.
 // SINGLE: one job, so its table entry is read without waiting for the item.
-template <int U, bool PACKED, bool SINGLE>
+// PLAN (one job, below the root): no work list; every workgroup plans the
+// level from the partition above (level_plan, in the LDS of its cells) and
+// takes its own item, and workgroup 0 stores the plan's outputs for the
+// later launches (the plan costs less than the launch it replaces).
+template <int U, bool PACKED, bool SINGLE, bool PLAN>
 __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict__ jobs,
                                                        const HistWork* __restrict__ work,
                                                        const uint32_t* __restrict__ count,
                                                        uint32_t level,
-                                                       HistCell* __restrict__ partials) {
+                                                       HistCell* __restrict__ partials, Lists L) {
   constexpr int kCells = kTileF * 256;
   __shared__ unsigned long long s_a[kCells];
   __shared__ unsigned long long s_b[PACKED ? 1 : kCells];
+  static_assert(!PLAN || SINGLE, "the in-kernel plan serves one job");
+  static_assert(sizeof(PlanShared) <= sizeof(s_a), "the plan fits the cells' LDS");
   uint32_t li;
-  if (!xcd_rank(blockIdx.x, *count, &li)) return;
-  const HistWork w = work[li];
+  HistWork w;
+  if (PLAN && level > 0) {
+    PlanShared& S = *reinterpret_cast<PlanShared*>(s_a);
+    if (!level_plan<kHistThreads>(jobs[0], L, 0u, level - 1, S, blockIdx.x == 0 ? 1 : 0)) return;
+    if (!xcd_rank(blockIdx.x, S.htot, &li)) return;
+    w = plan_item(jobs[0], S, 1u << (level - 1), li, 0u);
+    __syncthreads();  // the plan's LDS becomes the cells
+  } else {
+    if (!xcd_rank(blockIdx.x, *count, &li)) return;
+    w = work[li];
+  }
   const JobDev& J = jobs[SINGLE ? 0u : w.job];  // items exist only while the job's tree grows
   const uint32_t tid = threadIdx.x;
   for (uint32_t i = tid; i < kCells; i += kHistThreads) {
@@ -1204,12 +1431,8 @@ __global__ void __launch_bounds__(256) k_finalize(const JobDev* __restrict__ job
 // node) routes its rows -- NaN bin 255 or bin <= threshold goes left
 // (symmetric_tree.cpp:309-311) -- keeping them in registers, reserves room
 // with one 64-bit atomic on the node's cursor pair (left fills up from the
-// node's start, right fills down from its end; set by the previous level's
-// k_part_plan) and writes the rows, placed by wave ballots.
-__device__ inline bool part_active(const JobDev& J, uint32_t level) {
-  return J.st->depth == static_cast<int32_t>(level) + 1 && J.st->tree < static_cast<int>(J.n_trees);
-}
-
+// node's start, right fills down from its end; set by the level's plan) and
+// writes the rows, placed by wave ballots.
 __device__ inline uint32_t level_row(const JobDev& J, uint32_t level, uint32_t pos) {
   return (level == 0 && J.root_identity) ? pos : J.rows[static_cast<size_t>(level) * J.N + pos];
 }
@@ -1242,7 +1465,7 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(const JobDev* __rest
   const uint8_t* col = J.bins_cm + static_cast<size_t>(feat) * J.N;
   const uint8_t thr = static_cast<uint8_t>(split_bin);
   uint32_t* rows_n = J.rows + static_cast<size_t>(level + 1) * J.N;
-  uint32_t* cur = J.cursor + 2 * w.node;
+  uint32_t* cur = cursor_set(J, level) + 2 * w.node;
   __shared__ uint32_t s_cnt[2][kIt * kW];  // per (step, wave): count -> offset in the item
   __shared__ uint32_t s_base[2];
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -1305,166 +1528,13 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(const JobDev* __rest
   }
 }
 
-// Per job, after the partition: child counts and offsets, the next level's
-// stop test, and the next level's work lists (block-wide scans over nodes,
-// then one lane per work item).
+// Per job, after the partition: the plan of the next level with its stored
+// outputs and both work lists (several jobs, or a partition of the last
+// level; a one-job fit plans inside its next histogram launch).
 __global__ void __launch_bounds__(256) k_part_plan(const JobDev* __restrict__ jobs, Lists L,
                                                    uint32_t level) {
-  const uint32_t job = blockIdx.y, tid = threadIdx.x;
-  const JobDev& J = jobs[job];
-  const uint32_t nn = 1u << level;  // <= 512: two nodes per thread at most
-  const uint32_t stride = (1u << J.D) + 1u;
-  const uint32_t* offd = J.off + static_cast<size_t>(level) * stride;
-  uint32_t* offn = J.off + static_cast<size_t>(level + 1) * stride;
-  // the nodes' segments and left cursors, in flight while the state is checked
-  uint32_t p_lo[2] = {0, 0}, p_hi[2] = {0, 0}, p_cur[2] = {0, 0};
-  float p_hl[2] = {0.f, 0.f}, p_hr[2] = {0.f, 0.f};  // single channel: the children's H
-#pragma unroll
-  for (uint32_t i = 0; i < 2; ++i) {
-    const uint32_t n = tid + 256u * i;
-    if (n < nn) {
-      p_lo[i] = offd[n];
-      p_hi[i] = offd[n + 1];
-      p_cur[i] = J.cursor[2 * n];
-      if (J.K == 1) {
-        p_hl[i] = J.nh[heap_index(level + 1, 2 * n)];
-        p_hr[i] = J.nh[heap_index(level + 1, 2 * n) + 1];
-      }
-    }
-  }
-  const uint32_t p_end = tid == 0 ? offd[nn] : 0u;
-  if (!part_active(J, level)) return;
-  __shared__ uint32_t s_hw[1 << (kMaxDepth - 1)];    // per pair: hist work items -> first item
-  __shared__ uint32_t s_pl[1 << (kMaxDepth - 1)];    // per pair: rows per piece
-  __shared__ uint32_t s_pw[1 << kMaxDepth];          // per child: partition slots -> first slot
-  __shared__ uint32_t s_off[(1 << kMaxDepth) + 1];   // child offsets
-  __shared__ uint32_t s_tmp[256];
-  __shared__ uint32_t s_hbase, s_pbase;
-  __shared__ uint32_t s_cnt[kChunkCands];
-  __shared__ int s_any;
-  __shared__ unsigned long long s_rows;
-  if (tid == 0) { s_any = 0; s_rows = 0ull; }
-  if (tid < kChunkCands) s_cnt[tid] = 0;
-  __syncthreads();
-  const float mcw2 = J.min_child_weight * 2.0f;
-#pragma unroll
-  for (uint32_t i = 0; i < 2; ++i) {
-    const uint32_t n = tid + 256u * i;
-    if (n >= nn) break;
-    const uint32_t len = p_hi[i] - p_lo[i];
-    const uint32_t acc = p_cur[i] - p_lo[i];  // rows sent left
-    const uint32_t cl = heap_index(level + 1, 2 * n);
-    J.nc[cl] = acc;
-    J.nc[cl + 1] = len - acc;
-    if (len == 0) {  // an empty node's children (no partition item computed them)
-      for (uint32_t c = 0; c < J.K; ++c) {
-        J.ng[cl * J.K + c] = 0.f;
-        J.nh[cl * J.K + c] = 0.f;
-        J.ng[(cl + 1) * J.K + c] = 0.f;
-        J.nh[(cl + 1) * J.K + c] = 0.f;
-      }
-    }
-    s_off[2 * n] = p_lo[i];
-    s_off[2 * n + 1] = p_lo[i] + acc;
-    offn[2 * n] = p_lo[i];
-    offn[2 * n + 1] = p_lo[i] + acc;
-    // stop test of the next level (symmetric_tree.cpp:115-124; H sums the
-    // channels in class order)
-    float hl = 0.f, hr = 0.f;
-    if (len == 0) {
-      // an empty node's children: zero totals (set above)
-    } else if (J.K == 1) {
-      hl = p_hl[i];
-      hr = p_hr[i];
-    } else {
-      for (uint32_t c = 0; c < J.K; ++c) {
-        hl = __fadd_rn(hl, J.nh[cl * J.K + c]);
-        hr = __fadd_rn(hr, J.nh[(cl + 1) * J.K + c]);
-      }
-    }
-    if ((acc >= 2u * J.min_samples_leaf && hl >= mcw2) ||
-        (len - acc >= 2u * J.min_samples_leaf && hr >= mcw2))
-      s_any = 1;
-    // next level: histogram of the smaller child of the pair (ties: left),
-    // partition slots of both children
-    const uint32_t sm_len = min(acc, len - acc);
-    s_hw[n] = sm_len;
-    s_pw[2 * n] = part_slots(acc);
-    s_pw[2 * n + 1] = part_slots(len - acc);
-    if (sm_len) atomicAdd(&s_rows, static_cast<unsigned long long>(sm_len));
-  }
-  if (tid == 0) {
-    offn[2 * nn] = p_end;
-    s_off[2 * nn] = p_end;
-  }
-  __syncthreads();
-  if (level + 1 >= J.D || !s_any) {
-    if (tid == 0) J.st->active = 0;
-    return;
-  }
-  // rows per work item of the next level: the smallest candidate whose
-  // pieces (ceil(rows / chunk) per histogrammed child) fit J.deep_items
-  // items, so that each CU gets one item of about the same size; or the
-  // fixed chunk
-  const uint32_t TK = J.T * J.K;
-  uint32_t chunk = J.hist_chunk;
-  if (J.deep_items) {
-    // thread: candidate tid % 8 over pairs tid / 8, tid / 8 + 32, ...
-    const uint32_t budget = max(1u, J.deep_items / TK);  // pieces
-    const uint32_t base = root_chunk(s_rows, 1u, budget);
-    const uint32_t k = tid % kChunkCands, c = chunk_cand(base, k);
-    uint32_t cnt = 0;
-    for (uint32_t n = tid / kChunkCands; n < nn; n += 256u / kChunkCands) cnt += (s_hw[n] + c - 1) / c;
-    if (cnt) atomicAdd(&s_cnt[k], cnt);
-    __syncthreads();
-    chunk = chunk_cand(base, kChunkCands - 1);
-    for (int q = kChunkCands - 1; q >= 0; --q)
-      if (s_cnt[q] <= budget) chunk = chunk_cand(base, q);
-  }
-  if (tid == 0) J.lvl_chunk[level + 1] = chunk;
-  for (uint32_t n = tid; n < nn; n += 256) {
-    s_pl[n] = piece_rows(s_hw[n], chunk);
-    s_hw[n] = ((s_hw[n] + chunk - 1) / chunk) * TK;
-  }
-  __syncthreads();
-  const uint32_t htot = block_scan_excl(s_hw, nn, s_tmp);
-  const uint32_t ptot = block_scan_excl(s_pw, 2 * nn, s_tmp);
-  if (tid == 0) {
-    s_hbase = list_reserve(L, L.hist_count[level + 1], htot);
-    s_pbase = list_reserve(L, L.part_count[level + 1], ptot);
-    if (J.hist_rows_total) atomicAdd(J.hist_rows_total, s_rows);
-  }
-  __syncthreads();
-  for (uint32_t n = tid; n < nn; n += 256) {
-    // first item of each pair's smaller child
-    const uint32_t lc = 2 * n, rc = lc + 1;
-    const uint32_t sm = s_off[lc + 1] - s_off[lc] <= s_off[rc + 1] - s_off[rc] ? lc : rc;
-    J.hfirst[heap_index(level + 1, sm)] = s_hbase + s_hw[n];
-    // partition cursors of the next level (every cursor was read above)
-    J.cursor[2 * lc] = s_off[lc];
-    J.cursor[2 * lc + 1] = 0;
-    J.cursor[2 * rc] = s_off[rc];
-    J.cursor[2 * rc + 1] = 0;
-  }
-  HistWork* hout = L.hist[level + 1] + s_hbase;
-  for (uint32_t e = tid; e < htot; e += 256) {
-    const uint32_t n = find_item(s_hw, nn, e);
-    const uint32_t lc = 2 * n, rc = lc + 1;
-    const uint32_t nl = s_off[lc + 1] - s_off[lc], nr = s_off[rc + 1] - s_off[rc];
-    const uint32_t sm = nl <= nr ? lc : rc;
-    const uint32_t le = e - s_hw[n], r = le % TK;
-    const uint32_t end = s_off[sm + 1], pl = s_pl[n];
-    const uint32_t start = s_off[sm] + (le / TK) * pl;
-    put_hist_work(hout + e, job, sm, start, min(pl, end - start), r / J.K, r % J.K,
-                  start + pl >= end);
-  }
-  PartWork* pout = L.part[level + 1] + s_pbase;
-  for (uint32_t e = tid; e < ptot; e += 256) {
-    const uint32_t c = find_item(s_pw, 2 * nn, e);
-    const uint32_t start = s_off[c] + (e - s_pw[c]) * kPartChunk;
-    put_part_work(pout + e, job, c, start,
-                  min(static_cast<uint32_t>(kPartChunk), s_off[c + 1] - start), e);
-  }
+  __shared__ PlanShared S;
+  (void)level_plan<256>(jobs[blockIdx.y], L, blockIdx.y, level, S, 2);
 }
 
 // Copy externally supplied gradients (teacher forcing, [K][N]) into gh,
