@@ -11,11 +11,11 @@
 //   tree_begin
 //   for level d < D:
 //     [d=0] hist(0, root)
-//     [d>0] hist(d, smaller children)
+//     [d>0] hist(d, smaller children)  (one job: plans level d itself)
 //     split_scan(d)           (partial slots summed, sibling = parent - smaller)
 //     [d<D-1] partition(d)    (finalizes the level: split record, child totals)
-//             part_plan(d)
-//     [d=D-1] finalize(d)
+//             [several jobs] part_plan(d)
+//     [d=D-1] finalize(d)     (or, keeping the last rows, partition + part_plan)
 //   grad_update(apply)        (leaf values, prediction update, gradients,
 //                              validation rows, next root partials)
 //   [eval]                    (losses and early stopping)
