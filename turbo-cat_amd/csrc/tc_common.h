@@ -7,6 +7,8 @@
 //   bins_cm   uint8  [F][N]      column-major (split column reads)
 //   gh        float2 [K][N]      gradient / hessian per channel and row (K = 1
 //                                 except K-class softmax; class-major)
+//   ghl       float2 [2][K][N]   K > 1: the same pairs in the order of a level's
+//                                 row list (the histogram reads them by position)
 //   preds     float  [K][N]      raw training predictions, class-major
 //   rows      uint32 [D+1][N]    per level, row ids grouped by node (order
 //                                 within a node is irrelevant: every sum
@@ -108,6 +110,8 @@ struct JobDev {
   const float* y;           // [N] (class index for softmax)
   float* preds;             // [K][N]
   float2* gh;               // [K][N]
+  float2* ghl;              // K > 1: [2][K][N] each level's (g, h) in the order of its row list
+                            // (by level parity, written by the partition); null: read gh by row
   uint32_t* rows;           // [D+1][N]
   uint32_t* off;            // [D+1][2^D + 1]
   float* ng;                // heap [2^(D+1) - 1][K]
@@ -118,10 +122,10 @@ struct JobDev {
   TreeState* st;
   unsigned long long* best_keys;  // [D][F] each feature's best split key of the level
                                   // (null: merged with atomics into TreeState::best)
-  // partition cursors of the level being split, one 64-bit word per node:
+  // partition cursors, one 64-bit word per node of the level being split:
   // [node][0] next free position of the left child (counts up from the
   // node's start), [node][1] rows taken by the right child (filled down from
-  // the node's end); capacity 2^D
+  // the node's end); two sets of 2^(D-1) nodes, by level parity
   uint32_t* cursor;
   // root-sum partials (sum g, sum h over 1024-row tiles of the root list)
   double* grad_partials;    // [wg][K][2]

@@ -637,6 +637,7 @@ class Trainer {
       J.y = y;
       J.preds = alloc<float>(static_cast<size_t>(N) * K);
       J.gh = alloc<float2>(static_cast<size_t>(N) * K);
+      J.ghl = K > 1 ? alloc<float2>(2ull * N * K) : nullptr;
       J.rows = alloc<uint32_t>(static_cast<size_t>(D_ + 1) * N);
       hipLaunchKernelGGL(k_iota, dim3(std::max(1u, ceil_div(N, 256))), dim3(256), 0, s, J.rows, N);
       J.off = alloc<uint32_t>(static_cast<size_t>(D_ + 1) * ((1u << D_) + 1));

@@ -926,7 +926,11 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
   const uint8_t* __restrict__ brm = J.bins_rm + w.tile * kTileF;
   const uint32_t* __restrict__ rows = J.rows + static_cast<size_t>(level) * J.N + w.start;
   const bool direct = level == 0 && J.root_identity;  // row = position, no list read
-  const float2* __restrict__ gh = J.gh + static_cast<size_t>(w.cls) * J.N;
+  // several channels below the root: the level's (g, h) in the order of its row list
+  const bool gpos = J.ghl != nullptr && level > 0;
+  const float2* __restrict__ gh =
+      gpos ? J.ghl + (static_cast<size_t>(level & 1u) * J.K + w.cls) * J.N + w.start
+           : J.gh + static_cast<size_t>(w.cls) * J.N;
   const uint32_t Fp = J.Fp;
   const uint32_t s = tid & 15u;  // this lane's starting feature
   const uint32_t q = s >> 2, r = s & 3u;
@@ -948,7 +952,7 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
     const uint32_t row = row_at(j);
     v[u] = j < len ? *reinterpret_cast<const uint4*>(brm + static_cast<size_t>(row) * Fp)
                    : make_uint4(0, 0, 0, 0);
-    g[u] = j < len ? gh[row] : make_float2(0.f, 0.f);
+    g[u] = j < len ? gh[gpos ? j : row] : make_float2(0.f, 0.f);
     r1[u] = row_at(STEP + j);
   }
   for (uint32_t j0 = 0; j0 < len; j0 += STEP) {
@@ -959,7 +963,7 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
       const uint32_t j = j0 + STEP + u * kHistThreads + tid;
       vn[u] = j < len ? *reinterpret_cast<const uint4*>(brm + static_cast<size_t>(r1[u]) * Fp)
                       : make_uint4(0, 0, 0, 0);
-      gn[u] = j < len ? gh[r1[u]] : make_float2(0.f, 0.f);
+      gn[u] = j < len ? gh[gpos ? j : r1[u]] : make_float2(0.f, 0.f);
       r2[u] = row_at(j + STEP);
     }
 #pragma unroll
@@ -1519,12 +1523,31 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(const JobDev* __rest
       return;
     }
   }
+  uint32_t dst[kIt];
 #pragma unroll
   for (int k = 0; k < kIt; ++k) {
     const bool left = (lbits >> k) & 1u, right = (rbits >> k) & 1u;
     const unsigned long long ml = __ballot(left), mr = __ballot(right);
-    if (left) rows_n[s_base[0] + s_cnt[0][k * kW + wv] + __popcll(ml & lt)] = rv[k];
-    if (right) rows_n[s_base[1] + s_cnt[1][k * kW + wv] + __popcll(mr & lt)] = rv[k];
+    dst[k] = left ? s_base[0] + s_cnt[0][k * kW + wv] + __popcll(ml & lt)
+                  : s_base[1] + s_cnt[1][k * kW + wv] + __popcll(mr & lt);
+    if (left || right) rows_n[dst[k]] = rv[k];
+  }
+  if (J.ghl) {  // several channels: carry the rows' (g, h) into the next level's row order
+    const size_t N = J.N;
+    const float2* src = level == 0 ? J.gh : J.ghl + static_cast<size_t>(level & 1u) * J.K * N;
+    float2* out = J.ghl + static_cast<size_t>((level + 1) & 1u) * J.K * N;
+    const uint32_t moved = lbits | rbits;
+    for (uint32_t c = 0; c < J.K; ++c) {
+      float2 v[kIt];
+#pragma unroll
+      for (int k = 0; k < kIt; ++k) {
+        const uint32_t at = level == 0 ? rv[k] : w.start + k * kPartThreads + tid;
+        v[k] = ((moved >> k) & 1u) ? src[c * N + at] : make_float2(0.f, 0.f);
+      }
+#pragma unroll
+      for (int k = 0; k < kIt; ++k)
+        if ((moved >> k) & 1u) out[c * N + dst[k]] = v[k];
+    }
   }
 }
 
