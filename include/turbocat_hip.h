@@ -71,7 +71,10 @@ typedef struct {
  * subsample < 1 an xorshift64* draw seeds Dataset::random_subsample
  * (dataset.cpp:552-562: std::mt19937_64 + std::shuffle of 0..N-1, first
  * uint32(N * subsample) kept, in shuffled order); when colsample_bytree < 1
- * one more draw is consumed (symmetric trees ignore the feature subset). */
+ * one more draw is consumed (symmetric trees ignore the feature subset).
+ * With use_goss (booster.cpp:207-212, single channel only) the draw seeds
+ * Dataset::goss_subsample on the current gradients instead (tc_goss_subsample)
+ * and subsample is not used. */
 typedef struct {
   int32_t loss;              /* TC_LOSS_MSE or TC_LOSS_LOGLOSS */
   uint32_t n_trees;
@@ -88,6 +91,9 @@ typedef struct {
                                 every level, else hist_chunk rows at the root and each deeper
                                 level cut into at most about this many items (<= 65536);
                                 ignored when autotuned.  Never changes the model. */
+  int32_t use_goss;          /* BoostingConfig::use_goss (config.hpp:83) */
+  float goss_top_rate;       /* in [0, 1]: rows kept by largest |g| (config.hpp:84) */
+  float goss_other_rate;     /* in [0, 1]: share of rows drawn from the rest (config.hpp:85) */
 } tc_train_params;
 
 /* Host-side model arrays filled by training (caller-allocated).
@@ -143,6 +149,13 @@ int32_t tc_gradients_multi(tc_ctx* ctx, const float* y, const float* preds, floa
  * (dataset.cpp:552-562).  out needs n slots; *n_out = uint32(n * ratio). */
 int32_t tc_random_subsample(uint32_t n, float ratio, uint64_t seed, uint32_t* out,
                             uint32_t* n_out);
+/* Row sample of one tree under GOSS: Dataset::goss_subsample(top_rate,
+ * other_rate, seed) (dataset.cpp:494-550) on gradients g[n]: the
+ * uint32(n * top_rate) rows of largest |g| (std::nth_element order), then
+ * min(uint32(n * other_rate), n - top) rows of the rest by a partial
+ * Fisher-Yates draw (std::mt19937_64).  out needs n slots. */
+int32_t tc_goss_subsample(const float* g, uint32_t n, float top_rate, float other_rate,
+                          uint64_t seed, uint32_t* out, uint32_t* n_out);
 /* A5: Loss::init_prediction (loss.cpp:217-226, :364-370). */
 int32_t tc_init_prediction(int32_t loss, const float* y, uint32_t n, float* base);
 

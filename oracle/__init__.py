@@ -354,6 +354,12 @@ class RefLib:
                                 c_float, c_float, ctypes.c_uint64, V, V, V, V, V, V, V, V]
         L.tcref_random_subsample.argtypes = [V, c_float, ctypes.c_uint64, V]
         L.tcref_random_subsample.restype = c_uint32
+        L.tcref_goss_subsample.argtypes = [V, V, c_float, c_float, ctypes.c_uint64, V]
+        L.tcref_goss_subsample.restype = c_uint32
+        L.tcref_fit_goss.argtypes = [V, c_int, c_uint32, c_uint32, c_float, c_float, c_float,
+                                     c_float, c_float, c_float, ctypes.c_uint64, V, V, V, V, V, V,
+                                     V]
+        L.tcref_fit_goss.restype = c_float
         L.tcref_fit.restype = c_float
         L.tcref_predict.argtypes = [V, V, V, V, V, c_uint32, c_uint32, c_float, c_float, V,
                                     c_uint32, c_uint32, V]
@@ -470,6 +476,26 @@ class RefDataset:
         out = np.zeros(self.n, np.uint32)
         k = self.lib.L.tcref_random_subsample(self.h, ratio, seed, _p(out))
         return out[:k]
+
+    def goss_subsample(self, g, top_rate, other_rate, seed):
+        """Dataset::goss_subsample on gradients g of the handle's rows."""
+        out = np.zeros(self.n, np.uint32)
+        k = self.lib.L.tcref_goss_subsample(self.h, _p(np.ascontiguousarray(g, np.float32)),
+                                            top_rate, other_rate, seed, _p(out))
+        return out[:k]
+
+    def fit_goss(self, loss, n_trees, max_depth=6, lr=0.1, lambda_l2=1.0, min_child_weight=1.0,
+                 top_rate=0.2, other_rate=0.1, colsample=1.0, seed=42):
+        """The reference's fit loop with GOSS row sampling (booster.cpp:207-212)."""
+        m = new_model(n_trees, max_depth)
+        tp = np.zeros(self.n, np.float32)
+        base = self.lib.L.tcref_fit_goss(self.h, loss, n_trees, max_depth, lr, lambda_l2,
+                                         min_child_weight, top_rate, other_rate, colsample, seed,
+                                         _p(m["feature"]), _p(m["bin"]), _p(m["fthr"]),
+                                         _p(m["gain"]), _p(m["depth"]), _p(m["leaves"]), _p(tp))
+        m["base"] = float(base)
+        m["weight"][:] = lr
+        return m, tp
 
     def fit(self, loss, n_trees, max_depth=6, lr=0.1, lambda_l2=1.0, min_child_weight=1.0,
             subsample=1.0, colsample=1.0, seed=42):

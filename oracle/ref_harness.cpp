@@ -208,6 +208,19 @@ uint32_t tcref_random_subsample(void* h, float ratio, uint64_t seed, uint32_t* o
     return static_cast<uint32_t>(idx.size());
 }
 
+// Dataset::goss_subsample (src/core/dataset.cpp:494-550) on gradients g[n]
+// of the handle's rows; returns the number of rows written to out.
+uint32_t tcref_goss_subsample(void* h, const float* g, float top_rate, float other_rate,
+                              uint64_t seed, uint32_t* out) {
+    auto* r = static_cast<RefData*>(h);
+    const uint32_t n = r->ds->n_samples();
+    std::vector<float> ones(n, 1.0f);
+    r->ds->set_gradients_copy(g, ones.data(), n);
+    auto s = r->ds->goss_subsample(top_rate, other_rate, seed);
+    std::memcpy(out, s.row_indices.data(), s.row_indices.size() * sizeof(uint32_t));
+    return static_cast<uint32_t>(s.row_indices.size());
+}
+
 // Full fit on the dataset (symmetric trees), mirroring
 // booster.cpp's loop, row sampling included.  Model arrays: feature/bin/fthr/gain [T*D], depth [T],
 // leaves [T*2^D].  train_preds [n] gets the final training predictions.
@@ -220,11 +233,11 @@ static uint64_t xorshift64s(uint64_t& s) {  // Booster::next_random, booster.cpp
     return s * 0x2545F4914F6CDD1DULL;
 }
 
-float tcref_fit(void* h, int loss_id, uint32_t n_trees, uint32_t max_depth, float lr,
-                float lambda_l2, float min_child_weight, float subsample, float colsample,
-                uint64_t seed,
-                uint16_t* feat, uint8_t* bin, float* fthr, float* gain, uint16_t* depths,
-                float* leaves, float* train_preds, double* seconds) {
+static float fit_impl(void* h, int loss_id, uint32_t n_trees, uint32_t max_depth, float lr,
+                      float lambda_l2, float min_child_weight, float subsample, float colsample,
+                      bool goss, float top_rate, float other_rate, uint64_t seed,
+                      uint16_t* feat, uint8_t* bin, float* fthr, float* gain, uint16_t* depths,
+                      float* leaves, float* train_preds, double* seconds) {
     auto* r = static_cast<RefData*>(h);
     Dataset& ds = *r->ds;
     const uint32_t n = ds.n_samples();
@@ -246,7 +259,8 @@ float tcref_fit(void* h, int loss_id, uint32_t n_trees, uint32_t max_depth, floa
         loss->compute_gradients(ds.labels().data(), preds.data(), ds.gradients(), ds.hessians(), n);
         // sample selection and feature draw (booster.cpp:198-229)
         std::vector<Index> rows = all;
-        if (subsample < 1.0f) rows = ds.random_subsample(subsample, xorshift64s(rng));
+        if (goss) rows = ds.goss_subsample(top_rate, other_rate, xorshift64s(rng)).row_indices;
+        else if (subsample < 1.0f) rows = ds.random_subsample(subsample, xorshift64s(rng));
         if (colsample < 1.0f) (void)ds.random_feature_subsample(colsample, xorshift64s(rng));
         SymmetricTree t(c.tree);
         double a = now_s();
@@ -268,6 +282,27 @@ float tcref_fit(void* h, int loss_id, uint32_t n_trees, uint32_t max_depth, floa
     if (train_preds) std::memcpy(train_preds, preds.data(), n * sizeof(float));
     if (seconds) { seconds[0] = t1 - t0; seconds[1] = t2 - t1; seconds[2] = tb; }
     return base;
+}
+
+float tcref_fit(void* h, int loss_id, uint32_t n_trees, uint32_t max_depth, float lr,
+                float lambda_l2, float min_child_weight, float subsample, float colsample,
+                uint64_t seed,
+                uint16_t* feat, uint8_t* bin, float* fthr, float* gain, uint16_t* depths,
+                float* leaves, float* train_preds, double* seconds) {
+    return fit_impl(h, loss_id, n_trees, max_depth, lr, lambda_l2, min_child_weight, subsample,
+                    colsample, false, 0.f, 0.f, seed, feat, bin, fthr, gain, depths, leaves,
+                    train_preds, seconds);
+}
+
+// The same fit with GOSS row sampling (booster.cpp:207-212).
+float tcref_fit_goss(void* h, int loss_id, uint32_t n_trees, uint32_t max_depth, float lr,
+                     float lambda_l2, float min_child_weight, float top_rate, float other_rate,
+                     float colsample, uint64_t seed,
+                     uint16_t* feat, uint8_t* bin, float* fthr, float* gain, uint16_t* depths,
+                     float* leaves, float* train_preds) {
+    return fit_impl(h, loss_id, n_trees, max_depth, lr, lambda_l2, min_child_weight, 1.0f,
+                    colsample, true, top_rate, other_rate, seed, feat, bin, fthr, gain, depths,
+                    leaves, train_preds, nullptr);
 }
 
 // predict_raw_nobinning_fast without the Booster: flat model -> raw scores

@@ -95,6 +95,8 @@ def test_not_fitted_and_unsupported_options():
         turbocat.TurboCatClassifier(subsample=1.0).fit(X, np.arange(50) % 3)
     with pytest.raises(ValueError, match="loss='huber'"):
         turbocat.TurboCatRegressor(loss="huber").fit(X, y)
+    with pytest.raises(ValueError, match="goss_top_rate"):
+        turbocat.TurboCatRegressor(use_goss=True, goss_top_rate=1.5).fit(X, y)
     assert clf.n_trees == 0
 
 

@@ -48,7 +48,8 @@ def test_multiple_trees_and_sampling(random_data):
     from turbocat import TurboCatClassifier
     X, y = random_data
     for kw in (dict(n_estimators=10), dict(n_estimators=10, subsample=0.7),
-               dict(n_estimators=10, colsample_bytree=0.5)):
+               dict(n_estimators=10, colsample_bytree=0.5),
+               dict(n_estimators=10, use_goss=True, goss_top_rate=0.3, goss_other_rate=0.3)):
         clf = TurboCatClassifier(learning_rate=0.3, verbosity=0, **kw).fit(X, y)
         assert clf.n_trees == 10
         assert clf.predict_proba(X).shape == (100, 2)

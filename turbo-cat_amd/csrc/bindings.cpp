@@ -4,10 +4,11 @@
 // types and error behaviour (RuntimeError "Model not fitted. Call fit()
 // first." before fit; ValueError for invalid arguments).  Options whose
 // reference code path is not the symmetric-tree hot path (mode="small",
-// GOSS, ordered boosting, categorical features, multiclass, early-stopping
+// ordered boosting, categorical features, multiclass, early-stopping
 // metrics other than the loss, non-MSE/LogLoss losses) raise ValueError
 // naming the option instead of silently training something else.  A
-// validation set early-stops on its loss (booster.cpp:255-371).
+// validation set early-stops on its loss (booster.cpp:255-371); use_goss
+// samples each tree's rows by GOSS (booster.cpp:207-212).
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
@@ -90,13 +91,15 @@ class TurboCatClassifier {
     p_.verbosity = verbosity;
     p_.loss = TC_LOSS_LOGLOSS;
     p_.task = tcamd::kBinary;
-    use_goss_ = use_goss;
+    p_.use_goss = use_goss;
+    p_.goss_top_rate = goss_top_rate;
+    p_.goss_other_rate = goss_other_rate;
     use_gradtree_ = use_gradtree;
     use_ordered_ = use_ordered_boosting;
     auto_interactions_ = auto_interactions;
     p_.early_stopping_rounds = static_cast<uint32_t>(std::max(0, early_stopping_rounds));
     es_metric_ = early_stopping_metric;
-    (void)goss_top_rate; (void)goss_other_rate; (void)use_symmetric; (void)grow_policy;
+    (void)use_symmetric; (void)grow_policy;
     (void)cat_encoding; (void)max_interactions; (void)interaction_method; (void)n_threads;
   }
 
@@ -105,7 +108,6 @@ class TurboCatClassifier {
     if (y.ndim() != 1) throw std::runtime_error("y must be 1-dimensional");
     if (static_cast<uint32_t>(y.shape(0)) != m.n) throw std::invalid_argument("X and y differ in rows");
     reject(mode_ == "small", "mode='small' (regular trees)");
-    reject(use_goss_, "use_goss");
     reject(use_gradtree_, "use_gradtree");
     reject(use_ordered_, "use_ordered_boosting");
     reject(auto_interactions_, "auto_interactions");
@@ -184,6 +186,7 @@ class TurboCatClassifier {
     d["max_bins"] = p_.max_bins;
     d["subsample"] = p_.subsample;
     d["colsample_bytree"] = p_.colsample_bytree;
+    d["use_goss"] = p_.use_goss;
     d["min_child_weight"] = p_.min_child_weight;
     d["lambda_l2"] = p_.lambda_l2;
     d["seed"] = p_.seed;
@@ -202,7 +205,7 @@ class TurboCatClassifier {
   }
   Params p_;
   std::string mode_, loss_, es_metric_ = "loss";
-  bool use_goss_ = false, use_gradtree_ = false, use_ordered_ = false, auto_interactions_ = false;
+  bool use_gradtree_ = false, use_ordered_ = false, auto_interactions_ = false;
   uint32_t n_classes_ = 2;
   std::unique_ptr<Booster> booster_;
 };
@@ -229,9 +232,11 @@ class TurboCatRegressor {
     p_.verbosity = verbosity;
     p_.loss = TC_LOSS_MSE;
     p_.task = tcamd::kRegression;
-    use_goss_ = use_goss;
+    p_.use_goss = use_goss;
+    p_.goss_top_rate = goss_top_rate;
+    p_.goss_other_rate = goss_other_rate;
     p_.early_stopping_rounds = static_cast<uint32_t>(std::max(0, early_stopping_rounds));
-    (void)goss_top_rate; (void)goss_other_rate; (void)max_leaves;
+    (void)max_leaves;
     (void)grow_policy; (void)n_threads;
   }
 
@@ -240,7 +245,6 @@ class TurboCatRegressor {
     if (y.ndim() != 1) throw std::runtime_error("y must be 1-dimensional");
     if (static_cast<uint32_t>(y.shape(0)) != m.n) throw std::invalid_argument("X and y differ in rows");
     reject(mode_ == "small", "mode='small' (regular trees)");
-    reject(use_goss_, "use_goss");
     const Matrix v = validation(X_val, y_val, m.F);
     reject(!(loss_ == "mse" || loss_ == "l2"), "loss='" + loss_ + "'");
     booster_ = std::make_unique<Booster>(p_);
@@ -310,7 +314,6 @@ class TurboCatRegressor {
   }
   Params p_;
   std::string mode_, loss_, es_metric_ = "loss";
-  bool use_goss_ = false;
   std::unique_ptr<Booster> booster_;
 };
 

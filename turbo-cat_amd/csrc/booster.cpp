@@ -46,6 +46,9 @@ void Booster::fit(const float* X, const float* y, uint32_t n, uint32_t F, const 
   if (p_.max_bins < 2 || p_.max_bins > 255) throw std::invalid_argument("max_bins must be in [2, 255]");
   if (!(p_.subsample > 0.0f && p_.subsample <= 1.0f))
     throw std::invalid_argument("subsample must be in (0, 1]");
+  if (p_.use_goss && !(p_.goss_top_rate >= 0.0f && p_.goss_top_rate <= 1.0f &&
+                       p_.goss_other_rate >= 0.0f && p_.goss_other_rate <= 1.0f))
+    throw std::invalid_argument("goss_top_rate and goss_other_rate must be in [0, 1]");
   const auto t0 = std::chrono::steady_clock::now();
   ctx_ = device_context(p_.device);
   drop_ensemble();
@@ -71,6 +74,9 @@ void Booster::fit(const float* X, const float* y, uint32_t n, uint32_t F, const 
   tp.use_graph = p_.use_graph ? 1 : 0;
   tp.subsample = p_.subsample;
   tp.colsample_bytree = p_.colsample_bytree;
+  tp.use_goss = p_.use_goss ? 1 : 0;
+  tp.goss_top_rate = p_.goss_top_rate;
+  tp.goss_other_rate = p_.goss_other_rate;
   tp.seed = p_.seed;
   const uint32_t T = p_.n_estimators, D = p_.max_depth;
   feat_.assign(static_cast<size_t>(T) * D, 0);

@@ -24,6 +24,9 @@ struct Params {
   float learning_rate = 0.1f;
   float subsample = 1.0f;
   float colsample_bytree = 1.0f;
+  bool use_goss = false;                // GOSS row sampling (config.hpp:83-85)
+  float goss_top_rate = 0.2f;
+  float goss_other_rate = 0.1f;
   uint32_t early_stopping_rounds = 50;  // with a validation set (booster.cpp:344-346)
   // tree (TreeConfig, config.hpp:26-59)
   uint32_t max_depth = 6;

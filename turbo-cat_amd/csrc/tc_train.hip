@@ -114,6 +114,48 @@ uint32_t random_subsample(uint32_t n, float ratio, uint64_t seed, uint32_t* out)
   return std::min(k, n);
 }
 
+// Rows Dataset::goss_subsample (dataset.cpp:494-550) keeps for one tree: the
+// uint32(n * top_rate) rows of largest |g| -- std::nth_element over (|g|, row)
+// pairs ordered by |g| alone, so equal |g| fall where the library puts them --
+// then min(uint32(n * other_rate), n - top) rows of the rest by a partial
+// Fisher-Yates draw from std::mt19937_64.  The same library calls on the same
+// array in the same order as the reference: the same rows in the same order.
+uint32_t goss_subsample(const float* g, uint32_t n, float top_rate, float other_rate,
+                        uint64_t seed, uint32_t* out) {
+  const uint32_t top_n = static_cast<uint32_t>(static_cast<float>(n) * top_rate);
+  uint32_t other_n = static_cast<uint32_t>(static_cast<float>(n) * other_rate);
+  if (top_n == 0 && other_n == 0) return 0;
+  std::vector<std::pair<float, uint32_t>> gi(n);
+  for (uint32_t i = 0; i < n; ++i) gi[i] = {std::abs(g[i]), i};
+  if (top_n > 0 && top_n < n)
+    std::nth_element(gi.begin(), gi.begin() + top_n, gi.end(),
+                     [](const std::pair<float, uint32_t>& a, const std::pair<float, uint32_t>& b) {
+                       return a.first > b.first;
+                     });
+  uint32_t k = 0;
+  for (uint32_t i = 0; i < top_n; ++i) out[k++] = gi[i].second;
+  if (other_n > 0 && top_n < n) {
+    const uint32_t rest = n - top_n;
+    other_n = std::min(other_n, rest);
+    std::mt19937_64 rng(seed);
+    for (uint32_t i = 0; i < other_n; ++i) {
+      std::uniform_int_distribution<uint32_t> dist(i, rest - 1);
+      const uint32_t j = dist(rng);
+      if (i != j) std::swap(gi[top_n + i], gi[top_n + j]);
+      out[k++] = gi[top_n + i].second;
+    }
+  }
+  return k;
+}
+
+// Rows goss_subsample returns for n rows (independent of the gradients).
+uint32_t goss_rows(uint32_t n, float top_rate, float other_rate) {
+  const uint32_t top_n = static_cast<uint32_t>(static_cast<float>(n) * top_rate);
+  const uint32_t other_n = static_cast<uint32_t>(static_cast<float>(n) * other_rate);
+  if (top_n == 0 && other_n == 0) return 0;
+  return top_n + (top_n < n ? std::min(other_n, n - top_n) : 0u);
+}
+
 struct JobSpec {
   const tc_dataset* ds;
   const float* y_host;   // labels (may be null for teacher forcing)
@@ -169,7 +211,16 @@ class Trainer {
       const int b = S.next;
       S.next ^= 1;
       TC_HIP(hipEventSynchronize(S.ev[b]));
-      random_subsample(h_jobs_[j].N, S.ratio, seed, S.host[b]);
+      const uint32_t N = h_jobs_[j].N;
+      if (S.goss) {  // the gradients the next tree is grown on: every queued tree first
+        TC_HIP(hipMemcpyAsync(S.gh_host, h_jobs_[j].gh, sizeof(float2) * N, hipMemcpyDeviceToHost,
+                              ctx_->stream));
+        TC_HIP(hipStreamSynchronize(ctx_->stream));
+        for (uint32_t i = 0; i < N; ++i) S.g[i] = S.gh_host[i].x;
+        goss_subsample(S.g.data(), N, S.top_rate, S.other_rate, seed, S.host[b]);
+      } else {
+        random_subsample(N, S.ratio, seed, S.host[b]);
+      }
       TC_HIP(hipMemcpyAsync(h_jobs_[j].rows, S.host[b], 4ull * h_jobs_[j].n_root,
                             hipMemcpyHostToDevice, ctx_->stream));
       TC_HIP(hipEventRecord(S.ev[b], ctx_->stream));
@@ -613,6 +664,15 @@ class Trainer {
       SM.colsample = S.p.colsample_bytree > 0.0f && S.p.colsample_bytree < 1.0f;
       J.n_root = SM.active ? std::min(N, static_cast<uint32_t>(static_cast<float>(N) * S.p.subsample))
                            : N;
+      if (S.p.use_goss) {  // booster.cpp:207-212: GOSS takes the place of the row sample
+        SM.goss = SM.active = true;
+        SM.top_rate = S.p.goss_top_rate;
+        SM.other_rate = S.p.goss_other_rate;
+        J.n_root = goss_rows(N, SM.top_rate, SM.other_rate);
+        TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&SM.gh_host), sizeof(float2) * std::max(1u, N),
+                             hipHostMallocDefault));
+        SM.g.resize(N);
+      }
       J.root_identity = SM.active ? 0u : 1u;
       if (SM.active) {
         for (int b = 0; b < 2; ++b) {
@@ -765,19 +825,26 @@ class Trainer {
     graph_exec_ = nullptr;
     for (void* p : allocs_) dfree(p);
     allocs_.clear();
-    for (auto& S : samplers_)
+    for (auto& S : samplers_) {
       for (int b = 0; b < 2; ++b) {
         if (S.host[b]) (void)hipHostFree(S.host[b]);
         if (S.ev[b]) (void)hipEventDestroy(S.ev[b]);
         S.host[b] = nullptr;
         S.ev[b] = nullptr;
       }
+      if (S.gh_host) (void)hipHostFree(S.gh_host);
+      S.gh_host = nullptr;
+    }
   }
 
   struct Sampler {
     uint64_t state = 42;
     float ratio = 1.f;
     bool active = false, colsample = false;
+    bool goss = false;                 // GOSS on the current gradients instead of a shuffle
+    float top_rate = 0.f, other_rate = 0.f;
+    float2* gh_host = nullptr;         // pinned copy of the gradients (GOSS)
+    std::vector<float> g;              // their g
     uint32_t* host[2] = {nullptr, nullptr};
     hipEvent_t ev[2] = {nullptr, nullptr};
     int next = 0;
@@ -846,6 +913,12 @@ void check_params(const tc_train_params* p) {
              "max_depth must be in [1, 10]");
   TC_REQUIRE(p->learning_rate == p->learning_rate, "learning_rate is NaN");
   TC_REQUIRE(p->hist_deep_items <= 65536u, "hist_deep_items must be <= 65536");
+  if (p->use_goss) {
+    TC_REQUIRE(p->goss_top_rate >= 0.f && p->goss_top_rate <= 1.f &&
+                   p->goss_other_rate >= 0.f && p->goss_other_rate <= 1.f,
+               "goss_top_rate and goss_other_rate must be in [0, 1]");
+    TC_REQUIRE(p->loss != TC_LOSS_SOFTMAX, "use_goss needs a single-channel loss");
+  }
   if (p->loss == TC_LOSS_SOFTMAX) {
     TC_REQUIRE(p->n_classes >= 2 && p->n_classes <= static_cast<uint32_t>(kMaxClasses),
                "softmax needs n_classes in [2, 16]");
@@ -864,6 +937,16 @@ int32_t tc_random_subsample(uint32_t n, float ratio, uint64_t seed, uint32_t* ou
   return guard([&] {
     TC_REQUIRE(out && n_out, "tc_random_subsample: NULL argument");
     *n_out = random_subsample(n, ratio, seed, out);
+  });
+}
+
+int32_t tc_goss_subsample(const float* g, uint32_t n, float top_rate, float other_rate,
+                          uint64_t seed, uint32_t* out, uint32_t* n_out) {
+  return guard([&] {
+    TC_REQUIRE(g && out && n_out, "tc_goss_subsample: NULL argument");
+    TC_REQUIRE(top_rate >= 0.f && top_rate <= 1.f && other_rate >= 0.f && other_rate <= 1.f,
+               "tc_goss_subsample: rates must be in [0, 1]");
+    *n_out = goss_subsample(g, n, top_rate, other_rate, seed, out);
   });
 }
 
