@@ -299,8 +299,10 @@ class Trainer {
                          static_cast<const float2*>(nullptr), d, partials_);
       if (d + 1 < D_ || keep_rows_) {  // training never reads the last level's rows
         // the partition also finalizes the level (split record, child totals)
-        hipLaunchKernelGGL(njobs_ == 1 ? k_partition<true> : k_partition<false>, dim3(part_cap_[d]),
-                           dim3(kPartThreads), 0, s, d_jobs_, lists_.part[d], lists_.part_count[d], d);
+        // the work items, then one workgroup per (job, node) for the child totals
+        hipLaunchKernelGGL(njobs_ == 1 ? k_partition<true> : k_partition<false>,
+                           dim3(part_cap_[d] + (njobs_ << d)), dim3(kPartThreads), 0, s, d_jobs_,
+                           lists_.part[d], lists_.part_count[d], d, part_cap_[d]);
         if (njobs_ > 1 || d + 1 == D_)  // otherwise the next histogram launch plans the level
           hipLaunchKernelGGL(k_part_plan, dim3(1, njobs_), dim3(256), 0, s, d_jobs_, lists_, d);
       } else {

@@ -1429,9 +1429,10 @@ __global__ void __launch_bounds__(256) k_finalize(const JobDev* __restrict__ job
 
 // ---------------------------------------------------------------------------
 // A12: partition, one pass.  Every workgroup decodes the level's split
-// itself; the job's first work item records it (the level's finalize), the
-// first item of each node computes the node's child totals (one wave per
-// channel) while its rows are routed.  Each work item (<= 4096 rows of one
+// itself; the job's first work item records it (the level's finalize).
+// Workgroups past the work list's capacity, one per (job, node), compute the
+// node's child totals (one wave per channel) beside the row routing; inside
+// a routing workgroup they lengthened the launch by 2-4 us.  Each work item (<= 4096 rows of one
 // node) routes its rows -- NaN bin 255 or bin <= threshold goes left
 // (symmetric_tree.cpp:309-311) -- keeping them in registers, reserves room
 // with one 64-bit atomic on the node's cursor pair (left fills up from the
@@ -1445,9 +1446,21 @@ template <bool SINGLE>
 __global__ void __launch_bounds__(kPartThreads) k_partition(const JobDev* __restrict__ jobs,
                                                    const PartWork* __restrict__ work,
                                                    const uint32_t* __restrict__ count,
-                                                   uint32_t level) {
+                                                   uint32_t level, uint32_t items_cap) {
   constexpr int kIt = kPartChunk / kPartThreads;  // rows per lane
   constexpr int kW = kPartThreads / 64;           // waves
+  if (blockIdx.x >= items_cap) {  // one workgroup per (job, node): the node's child totals
+    const uint32_t b = blockIdx.x - items_cap, node = b & ((1u << level) - 1u);
+    const JobDev& J = jobs[SINGLE ? 0u : b >> level];
+    if (!J.st->active || J.st->tree >= static_cast<int>(J.n_trees)) return;
+    uint32_t feat, split_bin;
+    if (!decode_split(J, level, false, &feat, &split_bin)) return;
+    const uint32_t* offd = J.off + static_cast<size_t>(level) * ((1u << J.D) + 1u);
+    // an empty node's children get zero totals from the plan
+    if ((threadIdx.x >> 6) < J.K && offd[node + 1] > offd[node])
+      child_totals(J, level, node, threadIdx.x >> 6, feat, split_bin);
+    return;
+  }
   // the list is read with its count (the grid never exceeds the list's capacity)
   const uint32_t cnt = *count;
   const PartWork w = work[blockIdx.x];
@@ -1465,7 +1478,6 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(const JobDev* __rest
   uint32_t feat, split_bin;
   if (!decode_split(J, level, w.slot == 0 && tid == 0, &feat, &split_bin)) return;
   const uint32_t* offd = J.off + static_cast<size_t>(level) * ((1u << J.D) + 1u);
-  if (wv < J.K && w.start == offd[w.node]) child_totals(J, level, w.node, wv, feat, split_bin);
   const uint8_t* col = J.bins_cm + static_cast<size_t>(feat) * J.N;
   const uint8_t thr = static_cast<uint8_t>(split_bin);
   uint32_t* rows_n = J.rows + static_cast<size_t>(level + 1) * J.N;
