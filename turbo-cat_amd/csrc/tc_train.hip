@@ -295,8 +295,14 @@ class Trainer {
         TC_HIP(hipEventRecord(e1, s));
         hist_events->emplace_back(e0, e1);
       }
-      hipLaunchKernelGGL(k_split_scan<true>, dim3(fmax_, njobs_), dim3(1024), 0, s, d_jobs_,
-                         static_cast<const float2*>(nullptr), d, partials_);
+      // a batch: 512 threads, two workgroups per CU (a pass then holds 16 units: at most 8
+      // channels)
+      if (njobs_ == 1 || kmax_ > 8u)
+        hipLaunchKernelGGL((k_split_scan<true, 1024>), dim3(fmax_, njobs_), dim3(1024), 0, s,
+                           d_jobs_, static_cast<const float2*>(nullptr), d, partials_);
+      else
+        hipLaunchKernelGGL((k_split_scan<true, 512>), dim3(fmax_, njobs_), dim3(512), 0, s,
+                           d_jobs_, static_cast<const float2*>(nullptr), d, partials_);
       if (d + 1 < D_ || keep_rows_) {  // training never reads the last level's rows
         // the partition also finalizes the level (split record, child totals)
         // the work items, then one workgroup per (job, node) for the child totals
@@ -1289,7 +1295,7 @@ int32_t tc_split_scan(tc_ctx* ctx, const float* hist, const float* node_g, const
       TC_HIP(hipMemcpyAsync(J.st, &st, sizeof(st), hipMemcpyHostToDevice, s));
       JobDev* dj = static_cast<JobDev*>(al(sizeof(JobDev)));
       TC_HIP(hipMemcpyAsync(dj, &J, sizeof(J), hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(k_split_scan<false>, dim3(F, 1), dim3(256), 0, s, dj,
+      hipLaunchKernelGGL((k_split_scan<false, 256>), dim3(F, 1), dim3(256), 0, s, dj,
                          static_cast<const float2*>(fh), level, static_cast<const HistCell*>(nullptr));
       TC_HIP(hipGetLastError());
       TC_HIP(hipMemcpyAsync(&st, J.st, sizeof(st), hipMemcpyDeviceToHost, s));

@@ -1053,20 +1053,26 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
 // Histogram::subtract_from, histogram.cpp:39-42 via symmetric_tree.cpp:346)
 // and writes both prefixes; in the atomic mode it also zeroes the children's
 // next-level histograms.
-template <bool FIXED>
-__global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev* __restrict__ jobs,
-                                                                   const float2* __restrict__ fhist,
-                                                                   uint32_t level,
-                                                                   const HistCell* __restrict__ partials) {
-  constexpr uint32_t TH = FIXED ? 1024u : 256u;  // threads
+// TH threads: FIXED 1024 (one workgroup per CU: a lone job's few features
+// finish sooner) or 512 (two per CU: a batch's thousands of workgroups
+// overlap their memory waits); 256 otherwise.
+template <bool FIXED, uint32_t TH>
+__global__ void __launch_bounds__(TH) k_split_scan(const JobDev* __restrict__ jobs,
+                                                   const float2* __restrict__ fhist,
+                                                   uint32_t level,
+                                                   const HistCell* __restrict__ partials) {
   constexpr uint32_t NW = TH / 64u;              // waves
   constexpr uint32_t Q = TH / 256u;              // threads per candidate bin
-  constexpr uint32_t kTasks = kScanNodesPerPass / 2;  // histogrammed units per pass
+  // units staged per pass: two per wave (one task per wave: the smaller
+  // child of a pair, and its sibling), 32 without tasks
+  constexpr uint32_t NP = FIXED ? 2u * NW : static_cast<uint32_t>(kScanNodesPerPass);
+  constexpr uint32_t kTasks = NP / 2;  // histogrammed units per pass
   static_assert(!FIXED || kTasks <= NW, "one wave per task");
+  static_assert(FIXED ? (TH == 1024u || TH == 512u) : TH == 256u, "thread counts");
   // s_pref: the pass's left sums per unit and candidate.  FIXED: s_acc (the
   // tasks' summed cells) shares its 64 KB; each task's cells are in
   // registers before the first prefix is written.
-  __shared__ __align__(16) unsigned char s_raw[kScanNodesPerPass * 256 * sizeof(float2)];
+  __shared__ __align__(16) unsigned char s_raw[NP * 256 * sizeof(float2)];
   float2(*s_pref)[256] = reinterpret_cast<float2(*)[256]>(s_raw);
   unsigned long long(*s_acc)[256][2] = reinterpret_cast<unsigned long long(*)[256][2]>(s_raw);
   static_assert(kTasks * 256 * 2 * sizeof(unsigned long long) <= sizeof(s_raw), "task cells fit");
@@ -1074,8 +1080,8 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
   __shared__ float s_chain[256];
   __shared__ uint8_t s_chain_any[256];
   __shared__ uint32_t s_sm[kTasks], s_first[kTasks], s_cpre[kTasks + 1], s_empty[kTasks];
-  __shared__ float s_G[kScanNodesPerPass], s_H[kScanNodesPerPass];
-  __shared__ uint32_t s_N[kScanNodesPerPass];
+  __shared__ float s_G[NP], s_H[NP];
+  __shared__ uint32_t s_N[NP];
   const uint32_t job = blockIdx.y, f = blockIdx.x, tid = threadIdx.x;
   const JobDev& J = jobs[job];
   if (f >= J.F) return;
@@ -1091,8 +1097,11 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
   const uint32_t bin = tid & 255u, part = tid >> 8;  // candidate, unit subset
   const size_t F = J.F;
   // units per pass: whole pairs of nodes (all channels) below the root
-  const uint32_t P = !FIXED ? static_cast<uint32_t>(kScanNodesPerPass)
-                   : level == 0 ? K : (kScanNodesPerPass / (2 * K)) * (2 * K);
+  const uint32_t P = !FIXED ? NP : level == 0 ? K : (NP / (2 * K)) * (2 * K);
+  if (FIXED && (P == 0 || (level == 0 && K > kTasks))) {  // more channels than a pass holds
+    if (tid == 0) atomicOr(&J.st->fault, 2u);
+    return;
+  }
   const bool store = FIXED && partials != nullptr && J.store_hist;
   float total = 0.f;
   bool any = false;
@@ -1268,7 +1277,7 @@ __global__ void __launch_bounds__(FIXED ? 1024 : 256) k_split_scan(const JobDev*
     if (FIXED) {
       // part p forms the terms of units [8p, 8p + 8) in registers, then the
       // parts extend the candidate's sequence in order
-      constexpr uint32_t kPer = kScanNodesPerPass / Q;
+      constexpr uint32_t kPer = NP / Q;
       float term[kPer];
       uint32_t use = 0;
 #pragma unroll
