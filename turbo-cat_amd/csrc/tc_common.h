@@ -50,6 +50,9 @@ struct TreeState {
   uint32_t gmax_next;   // float bits of max |g| of the gradients being written
   uint32_t hmax_next;   // float bits of max |h| of the gradients being written
   uint32_t fault;       // nonzero: a kernel found inconsistent state and skipped its writes
+                        // (1: a partition reservation outside its node; 2: a split-scan pass
+                        // too small for the job's channels; 4: a level planned with more
+                        // histogram items than its launch has workgroups)
   uint32_t stopped;     // early stopping ended the fit after tree `tree`
   float es_best;        // best validation loss so far (early stopping)
   uint32_t es_best_iter;

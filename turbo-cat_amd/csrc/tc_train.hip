@@ -566,6 +566,12 @@ class Trainer {
       uint32_t f = 0;
       TC_HIP(hipMemcpy(&f, reinterpret_cast<const char*>(h_jobs_[j].st) + offsetof(TreeState, fault),
                        4, hipMemcpyDeviceToHost));
+      if (f & 2u)
+        throw Error(TC_EDEVICE, "internal consistency check failed in the split scan of job " +
+                                    std::to_string(j) + " (more channels than a pass holds)");
+      if (f & 4u)
+        throw Error(TC_EDEVICE, "internal consistency check failed in the histogram of job " +
+                                    std::to_string(j) + " (more work items than the grid)");
       if (f) throw Error(TC_EDEVICE, "internal consistency check failed in the partition of job " +
                                          std::to_string(j));
     }

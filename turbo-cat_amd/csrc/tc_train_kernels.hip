@@ -907,6 +907,8 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
   if (PLAN && level > 0) {
     PlanShared& S = *reinterpret_cast<PlanShared*>(s_a);
     if (!level_plan<kHistThreads>(jobs[0], L, 0u, level - 1, S, blockIdx.x == 0 ? 1 : 0)) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && S.htot > gridDim.x)  // the grid's bound broke
+      atomicOr(&jobs[0].st->fault, 4u);
     if (!xcd_rank(blockIdx.x, S.htot, &li)) return;
     w = plan_item(jobs[0], S, 1u << (level - 1), li, 0u);
     __syncthreads();  // the plan's LDS becomes the cells
