@@ -1082,7 +1082,7 @@ __global__ void __launch_bounds__(TH) k_split_scan(const JobDev* __restrict__ jo
   __shared__ float s_chain[256];
   __shared__ uint8_t s_chain_any[256];
   __shared__ uint32_t s_sm[kTasks], s_first[kTasks], s_cpre[kTasks + 1], s_empty[kTasks];
-  __shared__ float s_G[NP], s_H[NP];
+  __shared__ float s_G[NP], s_H[NP], s_P[NP];  // unit totals; s_P: the unit's parent term
   __shared__ uint32_t s_N[NP];
   const uint32_t job = blockIdx.y, f = blockIdx.x, tid = threadIdx.x;
   const JobDev& J = jobs[job];
@@ -1118,8 +1118,11 @@ __global__ void __launch_bounds__(TH) k_split_scan(const JobDev* __restrict__ jo
       auto task_left = [&](uint32_t t) { return level == 0 ? 0u : c0 / K + 2u * (t / K); };
       if (tid < cs) {  // the pass's unit totals, for the gain terms
         const uint32_t hn = heap_index(level, (c0 + tid) / K);
-        s_G[tid] = J.ng[hn * K + (c0 + tid) % K];
-        s_H[tid] = J.nh[hn * K + (c0 + tid) % K];
+        const float G = J.ng[hn * K + (c0 + tid) % K], H = J.nh[hn * K + (c0 + tid) % K];
+        s_G[tid] = G;
+        s_H[tid] = H;
+        // the parent term G^2 / (H + lambda), the same for every candidate
+        s_P[tid] = H > 1e-10f ? __fdiv_rn(__fmul_rn(G, G), __fadd_rn(H, lambda)) : 0.f;
         s_N[tid] = J.nc[hn];
       }
       if (tid < nt) {
@@ -1274,8 +1277,8 @@ __global__ void __launch_bounds__(TH) k_split_scan(const JobDev* __restrict__ jo
     }
     // gain of candidate `bin`: the unit terms added in (node, channel) order,
     // one sequence per candidate as in the reference.  FIXED: all threads
-    // form the terms (three divisions each) into LDS, then the candidate's
-    // thread adds them.
+    // form the terms (two divisions each; the unit's parent term is staged
+    // once), then the candidate's sequence runs through the parts in order.
     if (FIXED) {
       // part p forms the terms of units [8p, 8p + 8) in registers, then the
       // parts extend the candidate's sequence in order
@@ -1291,10 +1294,9 @@ __global__ void __launch_bounds__(TH) k_split_scan(const JobDev* __restrict__ jo
           const float2 Lf = s_pref[k][bin];
           const float rg = __fsub_rn(G, Lf.x), rh = __fsub_rn(H, Lf.y);
           if (!(Lf.y < 1e-10f || rh < 1e-10f)) {
-            const float pg = H > 1e-10f ? __fdiv_rn(__fmul_rn(G, G), __fadd_rn(H, lambda)) : 0.f;
             const float lgn = __fdiv_rn(__fmul_rn(Lf.x, Lf.x), __fadd_rn(Lf.y, lambda));
             const float rgn = __fdiv_rn(__fmul_rn(rg, rg), __fadd_rn(rh, lambda));
-            term[u] = __fsub_rn(__fadd_rn(lgn, rgn), pg);
+            term[u] = __fsub_rn(__fadd_rn(lgn, rgn), s_P[k]);
             use |= 1u << u;
           }
         }
