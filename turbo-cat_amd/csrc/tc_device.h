@@ -70,6 +70,32 @@ __device__ inline uint32_t warp_sum_u(uint32_t v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   return v;
 }
+// One step of a wave64 scan on the DPP crossbar: v + (v of the lane CTRL
+// selects), both 32-bit halves moved by update_dpp; lanes the control leaves
+// without a source (or rows outside ROW_MASK) add 0.  No LDS traffic.
+template <int CTRL, int ROW_MASK, bool BOUND>
+__device__ inline long long dpp_step_add(long long v) {
+  const unsigned long long u = static_cast<unsigned long long>(v);
+  const uint32_t lo = __builtin_amdgcn_update_dpp(0u, static_cast<uint32_t>(u), CTRL, ROW_MASK, 0xf, BOUND);
+  const uint32_t hi =
+      __builtin_amdgcn_update_dpp(0u, static_cast<uint32_t>(u >> 32), CTRL, ROW_MASK, 0xf, BOUND);
+  return static_cast<long long>(u + ((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+// Inclusive prefix sum of an int64 over the 64 lanes of a wave (all lanes
+// active): row_shr 1, 2, 4, 8 inside each 16-lane row, then the row_bcast:15
+// and row_bcast:31 broadcasts of the GFX9 family carry the row totals on.
+// Integer addition, so the sums are exact whatever the order.
+__device__ inline long long wave_incl_scan_i64(long long v) {
+  v = dpp_step_add<0x111, 0xf, true>(v);
+  v = dpp_step_add<0x112, 0xf, true>(v);
+  v = dpp_step_add<0x114, 0xf, true>(v);
+  v = dpp_step_add<0x118, 0xf, true>(v);
+  v = dpp_step_add<0x142, 0xa, false>(v);
+  v = dpp_step_add<0x143, 0xc, false>(v);
+  return v;
+}
+
 __device__ inline unsigned long long warp_max_u64(unsigned long long v) {
   for (int o = 32; o > 0; o >>= 1) {
     const unsigned long long w = __shfl_down(v, o, 64);

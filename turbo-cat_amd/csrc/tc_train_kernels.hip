@@ -1202,13 +1202,8 @@ __global__ void __launch_bounds__(TH) k_split_scan(const JobDev* __restrict__ jo
           pg[i] = ag;
           ph[i] = ah;
         }
-        long long sg = ag, sh = ah;  // inclusive scan of lane totals
-        for (int o = 1; o < 64; o <<= 1) {
-          const long long ug = __shfl_up(sg, o, 64), uh = __shfl_up(sh, o, 64);
-          if (lane >= static_cast<uint32_t>(o)) { sg += ug; sh += uh; }
-        }
-        sg -= ag;
-        sh -= ah;
+        // inclusive scan of the lane totals, then this lane's start
+        long long sg = wave_incl_scan_i64(ag) - ag, sh = wave_incl_scan_i64(ah) - ah;
 #pragma unroll
         for (int i = 0; i < 4; ++i)
           s_pref[k][4 * lane + i] =
