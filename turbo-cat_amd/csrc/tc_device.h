@@ -66,42 +66,81 @@ __device__ inline double warp_sum_d(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   return v;
 }
-__device__ inline uint32_t warp_sum_u(uint32_t v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return v;
-}
-// One step of a wave64 scan on the DPP crossbar: v + (v of the lane CTRL
-// selects), both 32-bit halves moved by update_dpp; lanes the control leaves
-// without a source (or rows outside ROW_MASK) add 0.  No LDS traffic.
+// One DPP move of a wave64 scan step: the value of the lane CTRL selects
+// (0 where the control leaves a lane without a source, or outside ROW_MASK).
 template <int CTRL, int ROW_MASK, bool BOUND>
-__device__ inline long long dpp_step_add(long long v) {
-  const unsigned long long u = static_cast<unsigned long long>(v);
-  const uint32_t lo = __builtin_amdgcn_update_dpp(0u, static_cast<uint32_t>(u), CTRL, ROW_MASK, 0xf, BOUND);
-  const uint32_t hi =
-      __builtin_amdgcn_update_dpp(0u, static_cast<uint32_t>(u >> 32), CTRL, ROW_MASK, 0xf, BOUND);
-  return static_cast<long long>(u + ((static_cast<unsigned long long>(hi) << 32) | lo));
+__device__ inline uint32_t dpp_mov(uint32_t v) {
+  return __builtin_amdgcn_update_dpp(0u, v, CTRL, ROW_MASK, 0xf, BOUND);
 }
 
-// Inclusive prefix sum of an int64 over the 64 lanes of a wave (all lanes
-// active): row_shr 1, 2, 4, 8 inside each 16-lane row, then the row_bcast:15
-// and row_bcast:31 broadcasts of the GFX9 family carry the row totals on.
-// Integer addition, so the sums are exact whatever the order.
+// Wave64 inclusive scan on the DPP crossbar (no LDS traffic; every lane of
+// the wave active): row_shr 1, 2, 4, 8 inside each 16-lane row, then the
+// row_bcast:15 and row_bcast:31 broadcasts of the GFX9 family carry the row
+// results on.  OP must be order-free with identity 0: an integer sum, or the
+// max of unsigned or non-negative values.
+template <typename Op>
+__device__ inline uint32_t wave_scan_u32(uint32_t v, Op op) {
+  v = op(v, dpp_mov<0x111, 0xf, true>(v));
+  v = op(v, dpp_mov<0x112, 0xf, true>(v));
+  v = op(v, dpp_mov<0x114, 0xf, true>(v));
+  v = op(v, dpp_mov<0x118, 0xf, true>(v));
+  v = op(v, dpp_mov<0x142, 0xa, false>(v));
+  v = op(v, dpp_mov<0x143, 0xc, false>(v));
+  return v;
+}
+__device__ inline uint32_t lane63(uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+}
+__device__ inline uint32_t wave_incl_scan_u32(uint32_t v) {
+  return wave_scan_u32(v, [](uint32_t a, uint32_t b) { return a + b; });
+}
+// Max over the wave, in every lane.
+__device__ inline uint32_t wave_max_u32(uint32_t v) {
+  return lane63(wave_scan_u32(v, [](uint32_t a, uint32_t b) { return a > b ? a : b; }));
+}
+// Max of non-negative floats over the wave (fmaxf), in every lane.
+__device__ inline float wave_max_f32(float v) {
+  return __uint_as_float(lane63(wave_scan_u32(__float_as_uint(v), [](uint32_t a, uint32_t b) {
+    return __float_as_uint(fmaxf(__uint_as_float(a), __uint_as_float(b)));
+  })));
+}
+
+// 64-bit steps: both halves moved by the same control.
+template <int CTRL, int ROW_MASK, bool BOUND>
+__device__ inline unsigned long long dpp_mov64(unsigned long long u) {
+  const uint32_t lo = dpp_mov<CTRL, ROW_MASK, BOUND>(static_cast<uint32_t>(u));
+  const uint32_t hi = dpp_mov<CTRL, ROW_MASK, BOUND>(static_cast<uint32_t>(u >> 32));
+  return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+template <typename Op>
+__device__ inline unsigned long long wave_scan_u64(unsigned long long v, Op op) {
+  v = op(v, dpp_mov64<0x111, 0xf, true>(v));
+  v = op(v, dpp_mov64<0x112, 0xf, true>(v));
+  v = op(v, dpp_mov64<0x114, 0xf, true>(v));
+  v = op(v, dpp_mov64<0x118, 0xf, true>(v));
+  v = op(v, dpp_mov64<0x142, 0xa, false>(v));
+  v = op(v, dpp_mov64<0x143, 0xc, false>(v));
+  return v;
+}
+__device__ inline unsigned long long lane63_u64(unsigned long long v) {
+  return (static_cast<unsigned long long>(lane63(static_cast<uint32_t>(v >> 32))) << 32) |
+         lane63(static_cast<uint32_t>(v));
+}
+// Inclusive prefix sum of an int64 over the wave: exact in any order.
 __device__ inline long long wave_incl_scan_i64(long long v) {
-  v = dpp_step_add<0x111, 0xf, true>(v);
-  v = dpp_step_add<0x112, 0xf, true>(v);
-  v = dpp_step_add<0x114, 0xf, true>(v);
-  v = dpp_step_add<0x118, 0xf, true>(v);
-  v = dpp_step_add<0x142, 0xa, false>(v);
-  v = dpp_step_add<0x143, 0xc, false>(v);
-  return v;
+  return static_cast<long long>(wave_scan_u64(
+      static_cast<unsigned long long>(v),
+      [](unsigned long long a, unsigned long long b) { return a + b; }));
 }
-
-__device__ inline unsigned long long warp_max_u64(unsigned long long v) {
-  for (int o = 32; o > 0; o >>= 1) {
-    const unsigned long long w = __shfl_down(v, o, 64);
-    v = w > v ? w : v;
-  }
-  return v;
+// Sum of an int64 over the wave, in every lane.
+__device__ inline long long wave_sum_i64(long long v) {
+  return static_cast<long long>(lane63_u64(static_cast<unsigned long long>(wave_incl_scan_i64(v))));
+}
+// Max of a u64 over the wave, in every lane.
+__device__ inline unsigned long long wave_max_u64(unsigned long long v) {
+  return lane63_u64(wave_scan_u64(v, [](unsigned long long a, unsigned long long b) {
+    return a > b ? a : b;
+  }));
 }
 
 // Rank a logical work index so that consecutive indices (the feature tiles

@@ -69,12 +69,7 @@ __device__ inline uint32_t block_scan_excl(uint32_t* a, uint32_t n, uint32_t* tm
     loc[i] = sum;
     sum += v;
   }
-  uint32_t incl = sum;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t u = __shfl_up(incl, o, 64);
-    if (lane >= static_cast<uint32_t>(o)) incl += u;
-  }
+  const uint32_t incl = wave_incl_scan_u32(sum);
   if (lane == 63u) tmp[wave] = incl;
   __syncthreads();
   uint32_t wbase = 0, total = 0;
@@ -125,10 +120,8 @@ __device__ inline void put_part_work(PartWork* dst, uint32_t job, uint32_t node,
 // takes their max -- order-free, and no two workgroups touch one address).
 __device__ inline void block_max2_store(float a, float b, uint32_t* dst) {
   __shared__ uint32_t s_m[2][4];
-  for (int o = 32; o > 0; o >>= 1) {
-    a = fmaxf(a, __shfl_down(a, o, 64));
-    b = fmaxf(b, __shfl_down(b, o, 64));
-  }
+  a = wave_max_f32(a);
+  b = wave_max_f32(b);
   if ((threadIdx.x & 63) == 0) {
     s_m[0][threadIdx.x >> 6] = __float_as_uint(a);
     s_m[1][threadIdx.x >> 6] = __float_as_uint(b);
@@ -151,10 +144,8 @@ __device__ inline void block_max2_to(TreeState* st, float a, float b) {
   __shared__ uint32_t s_m[2];
   if (threadIdx.x < 2) s_m[threadIdx.x] = 0u;
   __syncthreads();
-  for (int o = 32; o > 0; o >>= 1) {
-    a = fmaxf(a, __shfl_down(a, o, 64));
-    b = fmaxf(b, __shfl_down(b, o, 64));
-  }
+  a = wave_max_f32(a);
+  b = wave_max_f32(b);
   if ((threadIdx.x & 63) == 0) {
     atomicMax(&s_m[0], __float_as_uint(a));
     atomicMax(&s_m[1], __float_as_uint(b));
@@ -512,10 +503,8 @@ __device__ void tree_begin(const JobDev& J, uint32_t job, const Lists& L, int pa
       x = max(x, J.ghmax_partials[2 * w]);
       y = max(y, J.ghmax_partials[2 * w + 1]);
     }
-    for (int o = 32; o > 0; o >>= 1) {
-      x = max(x, static_cast<uint32_t>(__shfl_down(static_cast<int>(x), o, 64)));
-      y = max(y, static_cast<uint32_t>(__shfl_down(static_cast<int>(y), o, 64)));
-    }
+    x = wave_max_u32(x);
+    y = wave_max_u32(y);
     if ((tid & 63) == 0) {
       s_mx[0][tid >> 6] = x;
       s_mx[1][tid >> 6] = y;
@@ -1339,7 +1328,7 @@ __global__ void __launch_bounds__(TH) k_split_scan(const JobDev* __restrict__ jo
     if (bin + 1 < mb && (level > 0 || any) && !isnan(total)) {
       key = split_key(total == 0.f ? 0.f : total, f, bin);
     }
-    key = warp_max_u64(key);
+    key = wave_max_u64(key);
     if (lane == 0) s_red[wave] = key;
   }
   __syncthreads();
@@ -1363,8 +1352,7 @@ __device__ inline bool decode_split(const JobDev& J, uint32_t level, bool record
     const unsigned long long* keys = J.best_keys + static_cast<size_t>(level) * J.F;
     unsigned long long m = kEmptyKey;
     for (uint32_t f = threadIdx.x & 63u; f < J.F; f += 64) m = keys[f] > m ? keys[f] : m;
-    m = warp_max_u64(m);
-    key = __shfl(m, 0, 64);
+    key = wave_max_u64(m);
   } else {
     key = st->best[level];
   }
@@ -1411,8 +1399,7 @@ __device__ inline void child_totals(const JobDev& J, uint32_t level, uint32_t n,
     v[side + 1] += c.h;
   }
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
-    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_down(v[k], o, 64);
+  for (int k = 0; k < 4; ++k) v[k] = wave_sum_i64(v[k]);
   if (lane == 0) {
     const double gi = J.st->ginv, hi = J.st->hinv;
     const uint32_t cl = heap_index(level + 1, 2 * n);
@@ -1515,12 +1502,7 @@ __global__ void __launch_bounds__(kPartThreads) k_partition(const JobDev* __rest
   static_assert(kIt * kW == 64, "one wave scans one side's (step, wave) counts");
   if (wv < 2) {  // offsets in (step, wave) order: wave s scans side s, lane = entry
     const uint32_t v = s_cnt[wv][lane];
-    uint32_t incl = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t u = __shfl_up(incl, o, 64);
-      if (lane >= static_cast<uint32_t>(o)) incl += u;
-    }
+    const uint32_t incl = wave_incl_scan_u32(v);
     s_cnt[wv][lane] = incl - v;
     if (lane == 63) s_tot[wv] = incl;
   }
