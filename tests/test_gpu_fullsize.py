@@ -1,0 +1,131 @@
+"""Parity at the full sizes of the headline workloads.
+
+Where the C oracle finishes in seconds it is compared directly; elsewhere
+the checks are properties that hold at any size:
+
+- training, 1,000,000 rows x 100 features, binary LogLoss, depth 6: the
+  bins of a column sample (Gaussian columns and Uniform columns with 5%
+  duplicated values) bit-exact against the oracle's quantizer; the first
+  tree teacher-forced against the FP64 oracle tree; a 12-tree fit identical
+  under three work-item shapes (every histogram is an exact integer sum),
+  its first four trees replayed by the oracle and its training predictions equal
+  to the oracle's fused updates with the GPU's leaves;
+- multiclass, 4,000,000 rows x 256 features, 10-class softmax, depth 8: a
+  fit identical under two work-item shapes, predictions of all rows equal
+  to predictions of two row halves and to predictions from the training
+  bins, class probabilities summing to one.
+
+The data are the same seeded synthetic workloads the benchmark times."""
+import numpy as np
+import pytest
+
+import bench
+from test_gpu_tree import compare_tree, fma_update
+
+pytestmark = pytest.mark.gpu
+
+
+def fit(ctx, ds, y, loss, n_trees, depth, lr, shape, n_classes=0):
+    from turbocat import capi
+    prm = capi.train_params(loss, n_trees=n_trees, learning_rate=lr, max_depth=depth,
+                            lambda_l2=1.0, hist_chunk=shape[0], hist_deep_items=shape[1],
+                            n_classes=n_classes)
+    tr = capi.Trainer(ctx, [ds], [y], prm)
+    tr.step(n_trees)
+    tr.sync()
+    m = tr.model()
+    tr.close()
+    return m
+
+
+def same_model(a, b):
+    for k in ("feature", "bin", "fthr", "gain", "depth", "leaves", "train_preds"):
+        assert np.array_equal(a[k], b[k]), k
+    assert a["base"] == b["base"]
+
+
+@pytest.fixture(scope="module")
+def train_set(ctx):
+    n, F = 1_000_000, 100
+    print(f"\n[fullsize] training set {n} x {F}", flush=True)
+    X, y = bench.binary_workload(n, F, seed=1001)
+    ds, edges, ne = ctx.quantize_fit(X)
+    yield X, y, ds, edges, ne
+    ds.close()
+
+
+def test_full_size_bins_of_a_column_sample(train_set, oracle):
+    X, y, ds, edges, ne = train_set
+    cols = [0, 1, 41, 79, 80, 81, 90, 99]  # Gaussian 0-79, Uniform with duplicates 80-99
+    ob, oe, on = oracle.quantize(np.ascontiguousarray(X[:, cols]))
+    assert np.array_equal(ne[cols], on)
+    for i, f in enumerate(cols):
+        assert np.array_equal(edges[f, :ne[f]], oe[i, :on[i]]), f
+    bins = ds.bins()
+    assert np.array_equal(bins[cols], ob)
+
+
+def test_full_size_first_tree_teacher_forced(train_set, oracle):
+    from turbocat.capi import tree_params
+    from oracle import tree_params as otp
+    X, y, ds, edges, ne = train_set
+    base = oracle.init_prediction(1, y)
+    g, h = oracle.gradients_f32(1, y, np.full(len(y), np.float32(base), np.float32))
+    gpu = ds.tree(g, h, tree_params(max_depth=6, lambda_l2=1.0))
+    levels, tie = compare_tree(gpu, ds.bins(), g, h, edges, ne, otp(max_depth=6, lambda_l2=1.0),
+                               oracle)
+    assert levels == 6 and not tie
+
+
+def test_full_size_fit_is_shape_free_and_replays(ctx, train_set, oracle):
+    from oracle import tree_params as otp
+    X, y, ds, edges, ne = train_set
+    T, D, lr = 12, 6, np.float32(0.1)
+    models = [fit(ctx, ds, y, 1, T, D, 0.1, s) for s in ((0, 0), (13699, 256), (4096, 0))]
+    for m in models[1:]:
+        same_model(m, models[0])
+    m = models[0]
+    assert m["base"] == oracle.init_prediction(1, y)
+    bins = ds.bins()
+    preds = np.full(len(y), np.float32(m["base"]), np.float32)
+    ties = 0
+    for t in range(T):
+        d = int(m["depth"][t])
+        gpu = dict(depth=d, feature=m["feature"][t * D:t * D + d], bin=m["bin"][t * D:t * D + d],
+                   fthr=m["fthr"][t * D:t * D + d], gain=m["gain"][t * D:t * D + d],
+                   leaves=m["leaves"][t << D:(t << D) + (1 << d)])
+        gpu["leaf_idx"] = oracle.leaf_index_binned(bins, gpu["feature"], gpu["bin"])
+        if t < 4:  # the oracle's FP64 tree on the same gradients
+            g, h = oracle.gradients_f32(1, y, preds)
+            _, tie = compare_tree(gpu, bins, g, h, edges, ne, otp(max_depth=D, lambda_l2=1.0),
+                                  oracle)
+            ties += tie
+        preds = fma_update(oracle, preds, lr, gpu["leaves"][gpu["leaf_idx"]])
+    assert ties == 0
+    assert np.array_equal(m["train_preds"], preds)
+
+
+def test_full_size_multiclass_fit_and_predict(ctx):
+    from turbocat import capi
+    n, F, K = 4_000_000, 256, 10
+    print(f"\n[fullsize] multiclass set {n} x {F}, {K} classes", flush=True)
+    X, y = bench.multiclass_workload(n, F, K, seed=1004)
+    ds, edges, ne = ctx.quantize_fit(X)
+    try:
+        a = fit(ctx, ds, y, capi.TC_LOSS_SOFTMAX, 2, 8, 0.05, (0, 0), n_classes=K)
+        b = fit(ctx, ds, y, capi.TC_LOSS_SOFTMAX, 2, 8, 0.05, (4096, 512), n_classes=K)
+        same_model(a, b)
+        assert (a["depth"] > 0).all()
+        ens = ctx.ensemble(a, F)
+        try:
+            raw = ens.predict(X)
+            half = n // 2 + 7
+            assert np.array_equal(np.concatenate([ens.predict(X[:half]), ens.predict(X[half:])]),
+                                  raw)
+            assert np.array_equal(ens.predict_binned(ds), raw)
+            p = ens.predict(X[:200_000], transform=1)
+            assert np.abs(p.astype(np.float64).sum(axis=1) - 1.0).max() < 1e-5
+        finally:
+            ens.close()
+    finally:
+        ds.close()
