@@ -10,6 +10,10 @@ the checks are properties that hold at any size:
   under three work-item shapes (every histogram is an exact integer sum),
   its first four trees replayed by the oracle and its training predictions equal
   to the oracle's fused updates with the GPU's leaves;
+- batched training, 64 regression jobs of 10,000 to 2,000,000 rows
+  (log-uniform) and 16 to 256 features: every job's model from the batched
+  launches identical to the job trained alone, and the largest job's first
+  tree teacher-forced against the FP64 oracle;
 - multiclass, 4,000,000 rows x 256 features, 10-class softmax, depth 8: a
   fit identical under two work-item shapes, predictions of all rows equal
   to predictions of two row halves and to predictions from the training
@@ -129,3 +133,50 @@ def test_full_size_multiclass_fit_and_predict(ctx):
             ens.close()
     finally:
         ds.close()
+
+
+def test_full_size_batched_jobs_equal_each_job_alone(ctx, oracle):
+    from turbocat import capi
+    from oracle import tree_params as otp
+    rng = np.random.default_rng(2002)
+    J, T, D = 64, 4, 6
+    rows = np.floor(np.exp(rng.uniform(np.log(1e4), np.log(2e6), J))).astype(int)
+    feats = rng.integers(16, 257, J)
+    big = int(np.argmax(rows))
+    print(f"\n[fullsize] {J} jobs, {rows.sum()} rows", flush=True)
+    dss, ys = [], []
+    try:
+        for j in range(J):
+            X, y = bench.regression_workload(int(rows[j]), int(feats[j]), seed=5000 + j)
+            ds, edges, ne = ctx.quantize_fit(X)
+            if j == big:
+                big_edges, big_ne = edges, ne
+            dss.append(ds)
+            ys.append(y)
+            del X
+        prm = capi.train_params(capi.TC_LOSS_MSE, n_trees=T, learning_rate=0.1, max_depth=D,
+                                lambda_l2=0.0)
+        tr = capi.Trainer(ctx, dss, ys, prm)
+        tr.step(T)
+        tr.sync()
+        batched = [tr.model(j) for j in range(J)]
+        tr.close()
+        for j in range(J):
+            tj = capi.Trainer(ctx, [dss[j]], [ys[j]], prm)
+            tj.step(T)
+            tj.sync()
+            same_model(tj.model(), batched[j])
+            tj.close()
+        m, y, bins = batched[big], ys[big], dss[big].bins()
+        assert m["base"] == oracle.init_prediction(0, y)
+        g, h = oracle.gradients_f32(0, y, np.full(len(y), np.float32(m["base"]), np.float32))
+        d = int(m["depth"][0])
+        gpu = dict(depth=d, feature=m["feature"][:d], bin=m["bin"][:d], fthr=m["fthr"][:d],
+                   gain=m["gain"][:d], leaves=m["leaves"][:1 << d])
+        gpu["leaf_idx"] = oracle.leaf_index_binned(bins, gpu["feature"], gpu["bin"])
+        _, tie = compare_tree(gpu, bins, g, h, big_edges, big_ne,
+                              otp(max_depth=D, lambda_l2=0.0), oracle)
+        assert not tie
+    finally:
+        for ds in dss:
+            ds.close()
