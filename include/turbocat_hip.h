@@ -74,7 +74,10 @@ typedef struct {
  * one more draw is consumed (symmetric trees ignore the feature subset).
  * With use_goss (booster.cpp:207-212, single channel only) the draw seeds
  * Dataset::goss_subsample on the current gradients instead (tc_goss_subsample)
- * and subsample is not used. */
+ * and subsample is not used.  use_mvs (booster.cpp:201-205, single channel
+ * only) takes precedence over both: the draw seeds Dataset::mvs_subsample on
+ * the current gradients (tc_mvs_subsample), a sample whose size varies from
+ * tree to tree. */
 typedef struct {
   int32_t loss;              /* TC_LOSS_MSE or TC_LOSS_LOGLOSS */
   uint32_t n_trees;
@@ -94,6 +97,8 @@ typedef struct {
   int32_t use_goss;          /* BoostingConfig::use_goss (config.hpp:83) */
   float goss_top_rate;       /* in [0, 1]: rows kept by largest |g| (config.hpp:84) */
   float goss_other_rate;     /* in [0, 1]: share of rows drawn from the rest (config.hpp:85) */
+  int32_t use_mvs;           /* BoostingConfig::use_mvs (config.hpp:88) */
+  float mvs_subsample;       /* in (0, 1]: MVS target share of rows (config.hpp:89) */
 } tc_train_params;
 
 /* Host-side model arrays filled by training (caller-allocated).
@@ -156,6 +161,15 @@ int32_t tc_random_subsample(uint32_t n, float ratio, uint64_t seed, uint32_t* ou
  * Fisher-Yates draw (std::mt19937_64).  out needs n slots. */
 int32_t tc_goss_subsample(const float* g, uint32_t n, float top_rate, float other_rate,
                           uint64_t seed, uint32_t* out, uint32_t* n_out);
+/* Row sample of one tree under MVS: Dataset::mvs_subsample(ratio, seed)
+ * (dataset.cpp:576-673) on gradients g[n]: each row kept with probability
+ * min(1, s * max(|g|, t) / S * n) by a std::mt19937_64 uniform float draw,
+ * t a floor at the (1 - ratio) quantile of |g|, topped up by a shuffled draw
+ * of the rest when fewer than 80% of uint32(n * ratio) rows were kept.  The
+ * float sum S is taken in row order (the reference's OpenMP reduction on one
+ * thread).  out needs n slots; *n_out varies with the gradients. */
+int32_t tc_mvs_subsample(const float* g, uint32_t n, float ratio, uint64_t seed, uint32_t* out,
+                         uint32_t* n_out);
 /* A5: Loss::init_prediction (loss.cpp:217-226, :364-370). */
 int32_t tc_init_prediction(int32_t loss, const float* y, uint32_t n, float* base);
 

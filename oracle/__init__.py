@@ -360,6 +360,11 @@ class RefLib:
                                      c_float, c_float, c_float, ctypes.c_uint64, V, V, V, V, V, V,
                                      V]
         L.tcref_fit_goss.restype = c_float
+        L.tcref_mvs_subsample.argtypes = [V, V, c_float, ctypes.c_uint64, V]
+        L.tcref_mvs_subsample.restype = c_uint32
+        L.tcref_fit_mvs.argtypes = [V, c_int, c_uint32, c_uint32, c_float, c_float, c_float,
+                                    c_float, c_float, ctypes.c_uint64, V, V, V, V, V, V, V]
+        L.tcref_fit_mvs.restype = c_float
         L.tcref_fit.restype = c_float
         L.tcref_predict.argtypes = [V, V, V, V, V, c_uint32, c_uint32, c_float, c_float, V,
                                     c_uint32, c_uint32, V]
@@ -483,6 +488,27 @@ class RefDataset:
         k = self.lib.L.tcref_goss_subsample(self.h, _p(np.ascontiguousarray(g, np.float32)),
                                             top_rate, other_rate, seed, _p(out))
         return out[:k]
+
+    def mvs_subsample(self, g, ratio, seed):
+        """Dataset::mvs_subsample on gradients g of the handle's rows (one
+        OpenMP thread)."""
+        out = np.zeros(self.n, np.uint32)
+        k = self.lib.L.tcref_mvs_subsample(self.h, _p(np.ascontiguousarray(g, np.float32)),
+                                           ratio, seed, _p(out))
+        return out[:k]
+
+    def fit_mvs(self, loss, n_trees, max_depth=6, lr=0.1, lambda_l2=1.0, min_child_weight=1.0,
+                ratio=0.5, colsample=1.0, seed=42):
+        """The reference's fit loop with MVS row sampling (booster.cpp:201-205)."""
+        m = new_model(n_trees, max_depth)
+        tp = np.zeros(self.n, np.float32)
+        base = self.lib.L.tcref_fit_mvs(self.h, loss, n_trees, max_depth, lr, lambda_l2,
+                                        min_child_weight, ratio, colsample, seed,
+                                        _p(m["feature"]), _p(m["bin"]), _p(m["fthr"]),
+                                        _p(m["gain"]), _p(m["depth"]), _p(m["leaves"]), _p(tp))
+        m["base"] = float(base)
+        m["weight"][:] = lr
+        return m, tp
 
     def fit_goss(self, loss, n_trees, max_depth=6, lr=0.1, lambda_l2=1.0, min_child_weight=1.0,
                  top_rate=0.2, other_rate=0.1, colsample=1.0, seed=42):

@@ -221,6 +221,28 @@ uint32_t tcref_goss_subsample(void* h, const float* g, float top_rate, float oth
     return static_cast<uint32_t>(s.row_indices.size());
 }
 
+// Dataset::mvs_subsample (src/core/dataset.cpp:576-673) on gradients g[n] of
+// the handle's rows, on one OpenMP thread (its float score sum is an OpenMP
+// reduction: one thread adds in row order); returns the number of rows
+// written to out.
+static std::vector<Index> mvs_one_thread(const Dataset& ds, float ratio, uint64_t seed) {
+    const int threads = omp_get_max_threads();
+    omp_set_num_threads(1);
+    auto s = ds.mvs_subsample(ratio, seed);
+    omp_set_num_threads(threads);
+    return s.row_indices;
+}
+
+uint32_t tcref_mvs_subsample(void* h, const float* g, float ratio, uint64_t seed, uint32_t* out) {
+    auto* r = static_cast<RefData*>(h);
+    const uint32_t n = r->ds->n_samples();
+    std::vector<float> ones(n, 1.0f);
+    r->ds->set_gradients_copy(g, ones.data(), n);
+    auto rows = mvs_one_thread(*r->ds, ratio, seed);
+    std::memcpy(out, rows.data(), rows.size() * sizeof(uint32_t));
+    return static_cast<uint32_t>(rows.size());
+}
+
 // Full fit on the dataset (symmetric trees), mirroring
 // booster.cpp's loop, row sampling included.  Model arrays: feature/bin/fthr/gain [T*D], depth [T],
 // leaves [T*2^D].  train_preds [n] gets the final training predictions.
@@ -235,7 +257,7 @@ static uint64_t xorshift64s(uint64_t& s) {  // Booster::next_random, booster.cpp
 
 static float fit_impl(void* h, int loss_id, uint32_t n_trees, uint32_t max_depth, float lr,
                       float lambda_l2, float min_child_weight, float subsample, float colsample,
-                      bool goss, float top_rate, float other_rate, uint64_t seed,
+                      bool goss, float top_rate, float other_rate, float mvs_ratio, uint64_t seed,
                       uint16_t* feat, uint8_t* bin, float* fthr, float* gain, uint16_t* depths,
                       float* leaves, float* train_preds, double* seconds) {
     auto* r = static_cast<RefData*>(h);
@@ -259,7 +281,8 @@ static float fit_impl(void* h, int loss_id, uint32_t n_trees, uint32_t max_depth
         loss->compute_gradients(ds.labels().data(), preds.data(), ds.gradients(), ds.hessians(), n);
         // sample selection and feature draw (booster.cpp:198-229)
         std::vector<Index> rows = all;
-        if (goss) rows = ds.goss_subsample(top_rate, other_rate, xorshift64s(rng)).row_indices;
+        if (mvs_ratio > 0.0f) rows = mvs_one_thread(ds, mvs_ratio, xorshift64s(rng));
+        else if (goss) rows = ds.goss_subsample(top_rate, other_rate, xorshift64s(rng)).row_indices;
         else if (subsample < 1.0f) rows = ds.random_subsample(subsample, xorshift64s(rng));
         if (colsample < 1.0f) (void)ds.random_feature_subsample(colsample, xorshift64s(rng));
         SymmetricTree t(c.tree);
@@ -290,7 +313,7 @@ float tcref_fit(void* h, int loss_id, uint32_t n_trees, uint32_t max_depth, floa
                 uint16_t* feat, uint8_t* bin, float* fthr, float* gain, uint16_t* depths,
                 float* leaves, float* train_preds, double* seconds) {
     return fit_impl(h, loss_id, n_trees, max_depth, lr, lambda_l2, min_child_weight, subsample,
-                    colsample, false, 0.f, 0.f, seed, feat, bin, fthr, gain, depths, leaves,
+                    colsample, false, 0.f, 0.f, 0.f, seed, feat, bin, fthr, gain, depths, leaves,
                     train_preds, seconds);
 }
 
@@ -301,7 +324,18 @@ float tcref_fit_goss(void* h, int loss_id, uint32_t n_trees, uint32_t max_depth,
                      uint16_t* feat, uint8_t* bin, float* fthr, float* gain, uint16_t* depths,
                      float* leaves, float* train_preds) {
     return fit_impl(h, loss_id, n_trees, max_depth, lr, lambda_l2, min_child_weight, 1.0f,
-                    colsample, true, top_rate, other_rate, seed, feat, bin, fthr, gain, depths,
+                    colsample, true, top_rate, other_rate, 0.f, seed, feat, bin, fthr, gain, depths,
+                    leaves, train_preds, nullptr);
+}
+
+// The same fit with MVS row sampling (booster.cpp:201-205).
+float tcref_fit_mvs(void* h, int loss_id, uint32_t n_trees, uint32_t max_depth, float lr,
+                    float lambda_l2, float min_child_weight, float ratio, float colsample,
+                    uint64_t seed,
+                    uint16_t* feat, uint8_t* bin, float* fthr, float* gain, uint16_t* depths,
+                    float* leaves, float* train_preds) {
+    return fit_impl(h, loss_id, n_trees, max_depth, lr, lambda_l2, min_child_weight, 1.0f,
+                    colsample, false, 0.f, 0.f, ratio, seed, feat, bin, fthr, gain, depths,
                     leaves, train_preds, nullptr);
 }
 

@@ -289,6 +289,56 @@ def test_fit_goss_teacher_forced(ctx, oracle, loss, colsample, top, other):
     assert np.array_equal(m["train_preds"], preds)
 
 
+@pytest.mark.parametrize("loss,ratio,colsample", [(1, 0.5, 1.0), (0, 0.3, 0.8)])
+def test_fit_mvs_teacher_forced(ctx, oracle, loss, ratio, colsample):
+    """MVS on the device's gradients: each tree is grown on the rows the
+    reference's Dataset::mvs_subsample draws from the same gradients (a
+    sample of its own size each tree), and every row's prediction is
+    updated."""
+    from turbocat import capi
+    from oracle import tree_params as otp
+    n, F, T, D, seed = 15000, 9, 10, 6, 91
+    lam = 0.0 if loss == 0 else 1.0
+    X, y = regression_data(n, F, 8) if loss == 0 else binary_data(n, F, 8)
+    ds, edges, ne = ctx.quantize_fit(X)
+    bins = ds.bins()
+    m = ds.train(y, capi.train_params(loss, n_trees=T, learning_rate=0.1, max_depth=D,
+                                      lambda_l2=lam, colsample_bytree=colsample, seed=seed,
+                                      subsample=0.9, use_mvs=True, mvs_subsample=ratio))
+    preds = np.full(n, np.float32(m["base"]), np.float32)
+    state, ties, sizes = seed, 0, set()
+    for t in range(T):
+        g, h = oracle.gradients_f32(loss, y, preds)
+        s, state = capi.next_random(state)
+        rows = capi.mvs_subsample(g, ratio, s)
+        sizes.add(len(rows))
+        if colsample < 1.0:
+            _, state = capi.next_random(state)
+        d = int(m["depth"][t])
+        gpu = dict(depth=d, feature=m["feature"][t * D:t * D + d], bin=m["bin"][t * D:t * D + d],
+                   fthr=m["fthr"][t * D:t * D + d], gain=m["gain"][t * D:t * D + d],
+                   leaves=m["leaves"][t << D:(t << D) + (1 << d)])
+        ref = oracle.tree_f64(bins, g, h, edges, ne, otp(max_depth=D, lambda_l2=lam), rows=rows)
+        same = d == ref["depth"] and np.array_equal(gpu["feature"], ref["feature"]) and \
+            np.array_equal(gpu["bin"], ref["bin"])
+        if not same:
+            k = next(i for i in range(min(d, ref["depth"]))
+                     if (gpu["feature"][i], gpu["bin"][i]) != (ref["feature"][i], ref["bin"][i]))
+            assert ref["gap"][k] <= NEAR_TIE * max(1.0, abs(ref["gain"][k]))
+            ties += 1
+        else:
+            li = ref["leaf_idx"][rows]
+            L = 1 << d
+            absg = np.bincount(li, weights=np.abs(g[rows].astype(np.float64)), minlength=L)
+            hs = np.bincount(li, weights=h[rows].astype(np.float64), minlength=L)
+            assert rel_err(gpu["leaves"], ref["leaves"], absg / (hs + lam + 1e-30)).max() < TOL
+        li_all = oracle.leaf_index_binned(bins, gpu["feature"], gpu["bin"])
+        preds = fma_update(oracle, preds, np.float32(0.1), gpu["leaves"][li_all])
+    assert ties <= 1
+    assert len(sizes) > 1  # the sample size followed the gradients
+    assert np.array_equal(m["train_preds"], preds)
+
+
 @pytest.mark.parametrize("loss", [0, 1])
 def test_work_item_shape_never_changes_the_model(ctx, loss):
     """Histograms are exact integer sums with a fixed-point scale that does not

@@ -125,3 +125,69 @@ def test_goss_fit_replays_reference(oracle, loss, colsample, seed):
         li = oracle.leaf_index_binned(bins, tr["feature"], tr["bin"])
         preds = oracle.fma_update(preds, np.float32(lr), tr["leaves"][li])
     assert np.array_equal(preds, rtp)
+
+
+@pytest.mark.parametrize("n,ratio,seed,kind", [
+    (1000, 0.5, 7, "normal"), (5003, 0.3, 2 ** 63 + 9, "ties"), (20000, 0.5, 123, "two"),
+    (4000, 0.5, 17, "spiky"), (3000, 0.3, 5, "zeros"), (10, 0.5, 1, "normal"),
+    (1, 0.5, 3, "normal"), (777, 1.0, 11, "normal"), (2048, 0.05, 13, "spiky")])
+def test_mvs_subsample_matches_reference(n, ratio, seed, kind):
+    """MVS rows equal the compiled reference's Dataset::mvs_subsample on one
+    OpenMP thread: the probability pass, and ("spiky": a few huge |g| take
+    most of the score sum, so under 80% of the target is kept) the shuffled
+    top-up."""
+    from turbocat import capi
+    rng = np.random.default_rng(n + seed % 1000)
+    if kind == "normal":
+        g = rng.standard_normal(n).astype(np.float32)
+    elif kind == "ties":
+        g = (rng.integers(-4, 5, n) * 0.25).astype(np.float32)
+    elif kind == "two":  # LogLoss at the base score: two values of |p - y|
+        y = (rng.random(n) < 0.3).astype(np.float32)
+        g = (np.float32(0.3) - y).astype(np.float32)
+    elif kind == "spiky":
+        g = (rng.standard_normal(n) * 1e-3).astype(np.float32)
+        g[rng.choice(n, max(1, n // 100), replace=False)] = 1000.0
+    else:  # every |g| below the 1e-10 floor
+        g = np.zeros(n, np.float32)
+
+    def body():
+        ds = RefLib().dataset(np.zeros((n, 1), np.float32))
+        return ds.mvs_subsample(g, ratio, seed)
+    ref = RefLib.isolated(body)
+    got = capi.mvs_subsample(g, ratio, seed)
+    assert np.array_equal(got, ref)
+    assert len(np.unique(got)) == len(got)
+    if kind == "spiky":  # the top-up ran: exactly the target
+        assert len(got) == max(1, int(np.float32(n) * np.float32(ratio)))
+
+
+@pytest.mark.parametrize("loss,seed", [(LOSS_LOGLOSS, 42), (LOSS_MSE, 321)])
+def test_mvs_fit_replays_reference(oracle, loss, seed):
+    """The reference's MVS fit, replayed by the oracle on the library's
+    per-tree MVS rows (drawn from the oracle's own gradients)."""
+    from turbocat import capi
+    n, F, T, D, lr, ratio = 4000, 6, 8, 5, 0.1, 0.5
+    lam = 0.0 if loss == LOSS_MSE else 1.0
+    X, y = regression_data(n, F, 79) if loss == LOSS_MSE else binary_data(n, F, 79)
+
+    def body():
+        ds = RefLib().dataset(X, y)
+        return ds.fit_mvs(loss, T, D, lr, lam, 1.0, ratio=ratio, seed=seed)
+    rm, rtp = RefLib.isolated(body)
+    bins, edges, ne = oracle.quantize(X)
+    preds = np.full(n, np.float32(oracle.init_prediction(loss, y)), np.float32)
+    state = seed
+    for t in range(T):
+        g, h = oracle.gradients_f32(loss, y, preds)
+        s, state = capi.next_random(state)
+        rows = capi.mvs_subsample(g, ratio, s)
+        tr = oracle.tree_f32(bins, g, h, edges, ne, tree_params(D, lambda_l2=lam), rows=rows)
+        d = tr["depth"]
+        assert d == rm["depth"][t]
+        assert np.array_equal(tr["feature"], rm["feature"][t * D:t * D + d])
+        assert np.array_equal(tr["bin"], rm["bin"][t * D:t * D + d])
+        assert np.array_equal(tr["leaves"], rm["leaves"][t << D:(t << D) + (1 << d)])
+        li = oracle.leaf_index_binned(bins, tr["feature"], tr["bin"])
+        preds = oracle.fma_update(preds, np.float32(lr), tr["leaves"][li])
+    assert np.array_equal(preds, rtp)

@@ -49,6 +49,8 @@ void Booster::fit(const float* X, const float* y, uint32_t n, uint32_t F, const 
   if (p_.use_goss && !(p_.goss_top_rate >= 0.0f && p_.goss_top_rate <= 1.0f &&
                        p_.goss_other_rate >= 0.0f && p_.goss_other_rate <= 1.0f))
     throw std::invalid_argument("goss_top_rate and goss_other_rate must be in [0, 1]");
+  if (p_.use_mvs && !(p_.mvs_subsample > 0.0f && p_.mvs_subsample <= 1.0f))
+    throw std::invalid_argument("mvs_subsample must be in (0, 1]");
   const auto t0 = std::chrono::steady_clock::now();
   ctx_ = device_context(p_.device);
   drop_ensemble();
@@ -77,6 +79,8 @@ void Booster::fit(const float* X, const float* y, uint32_t n, uint32_t F, const 
   tp.use_goss = p_.use_goss ? 1 : 0;
   tp.goss_top_rate = p_.goss_top_rate;
   tp.goss_other_rate = p_.goss_other_rate;
+  tp.use_mvs = p_.use_mvs ? 1 : 0;
+  tp.mvs_subsample = p_.mvs_subsample;
   tp.seed = p_.seed;
   const uint32_t T = p_.n_estimators, D = p_.max_depth;
   feat_.assign(static_cast<size_t>(T) * D, 0);

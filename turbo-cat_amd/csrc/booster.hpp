@@ -27,6 +27,8 @@ struct Params {
   bool use_goss = false;                // GOSS row sampling (config.hpp:83-85)
   float goss_top_rate = 0.2f;
   float goss_other_rate = 0.1f;
+  bool use_mvs = false;                 // MVS row sampling, before GOSS (config.hpp:88-89)
+  float mvs_subsample = 0.5f;
   uint32_t early_stopping_rounds = 50;  // with a validation set (booster.cpp:344-346)
   // tree (TreeConfig, config.hpp:26-59)
   uint32_t max_depth = 6;

@@ -156,6 +156,61 @@ uint32_t goss_rows(uint32_t n, float top_rate, float other_rate) {
   return top_n + (top_n < n ? std::min(other_n, n - top_n) : 0u);
 }
 
+// Rows Dataset::mvs_subsample (dataset.cpp:576-673) keeps for one tree:
+// target = max(1, uint32(n * ratio)); a floor t = max(the |g| of rank
+// uint32((1 - ratio) * n) in ascending order, max |g| / 100, 1e-10); row i
+// kept when a uniform float draw is below min(1, s * max(|g_i|, t) / S * n),
+// S the float sum of the floored scores and s = target / S * S / n, in the
+// reference's float operation order; if fewer than uint32(target * 0.8f)
+// rows were kept, the rest are shuffled with the same engine and the first
+// ones appended up to target.  The same library calls (std::mt19937_64,
+// std::uniform_real_distribution<float>, std::shuffle) on the same values in
+// the same order give the same rows in the same order.  S is summed in row
+// order: the reference's own OpenMP reduction adds per-thread partial sums,
+// which equals this order on one thread.
+uint32_t mvs_subsample(const float* g, uint32_t n, float ratio, uint64_t seed, uint32_t* out) {
+  if (n == 0 || !(ratio > 0.0f)) return 0;
+  const uint32_t target = std::max(static_cast<uint32_t>(static_cast<float>(n) * ratio), 1u);
+  std::vector<float> a(n);
+  float gmax = 0.0f;
+  for (uint32_t i = 0; i < n; ++i) {
+    a[i] = std::abs(g[i]);
+    gmax = std::max(gmax, a[i]);
+  }
+  std::vector<float> sc(a);
+  const uint32_t ti =
+      std::min(static_cast<uint32_t>((1.0f - ratio) * static_cast<float>(n)), n - 1);
+  std::nth_element(sc.begin(), sc.begin() + ti, sc.end());
+  float thr = std::max(sc[ti], gmax * 0.01f);
+  if (thr < 1e-10f) thr = 1e-10f;
+  float ssum = 0.0f;
+  for (uint32_t i = 0; i < n; ++i) {
+    sc[i] = std::max(a[i], thr);
+    ssum += sc[i];
+  }
+  const float fn = static_cast<float>(n);
+  const float scale = static_cast<float>(target) / ssum * ssum / fn;
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<float> uniform(0.0f, 1.0f);
+  uint32_t k = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const float prob = std::min(1.0f, scale * sc[i] / ssum * fn);
+    if (uniform(rng) < prob) out[k++] = i;
+  }
+  if (k < static_cast<uint32_t>(static_cast<float>(target) * 0.8f)) {
+    std::vector<bool> kept(n, false);
+    for (uint32_t i = 0; i < k; ++i) kept[out[i]] = true;
+    std::vector<uint32_t> rest;
+    rest.reserve(n - k);
+    for (uint32_t i = 0; i < n; ++i)
+      if (!kept[i]) rest.push_back(i);
+    std::shuffle(rest.begin(), rest.end(), rng);
+    const uint32_t need = target - k;
+    for (uint32_t i = 0; i < need && i < rest.size(); ++i) out[k++] = rest[i];
+  }
+  return k;
+}
+
 struct JobSpec {
   const tc_dataset* ds;
   const float* y_host;   // labels (may be null for teacher forcing)
@@ -212,12 +267,19 @@ class Trainer {
       S.next ^= 1;
       TC_HIP(hipEventSynchronize(S.ev[b]));
       const uint32_t N = h_jobs_[j].N;
-      if (S.goss) {  // the gradients the next tree is grown on: every queued tree first
+      if (S.goss || S.mvs) {  // the gradients the next tree is grown on: every queued tree first
         TC_HIP(hipMemcpyAsync(S.gh_host, h_jobs_[j].gh, sizeof(float2) * N, hipMemcpyDeviceToHost,
                               ctx_->stream));
         TC_HIP(hipStreamSynchronize(ctx_->stream));
         for (uint32_t i = 0; i < N; ++i) S.g[i] = S.gh_host[i].x;
-        goss_subsample(S.g.data(), N, S.top_rate, S.other_rate, seed, S.host[b]);
+        if (S.goss) {
+          goss_subsample(S.g.data(), N, S.top_rate, S.other_rate, seed, S.host[b]);
+        } else {  // a sample of its own size each tree: the device reads n_root from the job table
+          h_jobs_[j].n_root = S.n_root_host[b] = mvs_subsample(S.g.data(), N, S.ratio, seed, S.host[b]);
+          TC_HIP(hipMemcpyAsync(reinterpret_cast<char*>(d_jobs_ + j) + offsetof(JobDev, n_root),
+                                &S.n_root_host[b], sizeof(uint32_t), hipMemcpyHostToDevice,
+                                ctx_->stream));
+        }
       } else {
         random_subsample(N, S.ratio, seed, S.host[b]);
       }
@@ -678,11 +740,18 @@ class Trainer {
       SM.colsample = S.p.colsample_bytree > 0.0f && S.p.colsample_bytree < 1.0f;
       J.n_root = SM.active ? std::min(N, static_cast<uint32_t>(static_cast<float>(N) * S.p.subsample))
                            : N;
-      if (S.p.use_goss) {  // booster.cpp:207-212: GOSS takes the place of the row sample
+      // booster.cpp:201-212: MVS, else GOSS, takes the place of the row sample
+      if (S.p.use_mvs) {
+        SM.mvs = SM.active = true;
+        SM.ratio = S.p.mvs_subsample;
+        J.n_root = N;  // until the first tree's sample is drawn
+      } else if (S.p.use_goss) {
         SM.goss = SM.active = true;
         SM.top_rate = S.p.goss_top_rate;
         SM.other_rate = S.p.goss_other_rate;
         J.n_root = goss_rows(N, SM.top_rate, SM.other_rate);
+      }
+      if (SM.goss || SM.mvs) {
         TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&SM.gh_host), sizeof(float2) * std::max(1u, N),
                              hipHostMallocDefault));
         SM.g.resize(N);
@@ -856,8 +925,10 @@ class Trainer {
     float ratio = 1.f;
     bool active = false, colsample = false;
     bool goss = false;                 // GOSS on the current gradients instead of a shuffle
+    bool mvs = false;                  // MVS on the current gradients (ratio: mvs_subsample)
     float top_rate = 0.f, other_rate = 0.f;
-    float2* gh_host = nullptr;         // pinned copy of the gradients (GOSS)
+    float2* gh_host = nullptr;         // pinned copy of the gradients (GOSS, MVS)
+    uint32_t n_root_host[2] = {0, 0};  // MVS: each buffer's row count, the source of its copy
     std::vector<float> g;              // their g
     uint32_t* host[2] = {nullptr, nullptr};
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -933,6 +1004,10 @@ void check_params(const tc_train_params* p) {
                "goss_top_rate and goss_other_rate must be in [0, 1]");
     TC_REQUIRE(p->loss != TC_LOSS_SOFTMAX, "use_goss needs a single-channel loss");
   }
+  if (p->use_mvs) {
+    TC_REQUIRE(p->mvs_subsample > 0.f && p->mvs_subsample <= 1.f, "mvs_subsample must be in (0, 1]");
+    TC_REQUIRE(p->loss != TC_LOSS_SOFTMAX, "use_mvs needs a single-channel loss");
+  }
   if (p->loss == TC_LOSS_SOFTMAX) {
     TC_REQUIRE(p->n_classes >= 2 && p->n_classes <= static_cast<uint32_t>(kMaxClasses),
                "softmax needs n_classes in [2, 16]");
@@ -961,6 +1036,15 @@ int32_t tc_goss_subsample(const float* g, uint32_t n, float top_rate, float othe
     TC_REQUIRE(top_rate >= 0.f && top_rate <= 1.f && other_rate >= 0.f && other_rate <= 1.f,
                "tc_goss_subsample: rates must be in [0, 1]");
     *n_out = goss_subsample(g, n, top_rate, other_rate, seed, out);
+  });
+}
+
+int32_t tc_mvs_subsample(const float* g, uint32_t n, float ratio, uint64_t seed, uint32_t* out,
+                         uint32_t* n_out) {
+  return guard([&] {
+    TC_REQUIRE(g && out && n_out, "tc_mvs_subsample: NULL argument");
+    TC_REQUIRE(ratio <= 1.f, "tc_mvs_subsample: ratio must be <= 1");
+    *n_out = mvs_subsample(g, n, ratio, seed, out);
   });
 }
 

@@ -29,6 +29,7 @@ EXPORTS = [
     "tc_quantize_fit", "tc_dataset_create", "tc_dataset_bins", "tc_dataset_shape",
     "tc_dataset_destroy", "tc_gradients", "tc_gradients_multi", "tc_random_subsample",
     "tc_goss_subsample",
+    "tc_mvs_subsample",
     "tc_init_prediction", "tc_hist_build",
     "tc_hist_subtract", "tc_split_scan", "tc_tree_build", "tc_tree_build_multi", "tc_train",
     "tc_train_valid", "tc_train_batch",
@@ -59,7 +60,8 @@ class TrainParams(Structure):
                 ("tree", TreeParams), ("hist_chunk", c_uint32), ("use_graph", c_int32),
                 ("subsample", c_float), ("colsample_bytree", c_float), ("seed", c_uint64),
                 ("n_classes", c_uint32), ("hist_deep_items", c_uint32),
-                ("use_goss", c_int32), ("goss_top_rate", c_float), ("goss_other_rate", c_float)]
+                ("use_goss", c_int32), ("goss_top_rate", c_float), ("goss_other_rate", c_float),
+                ("use_mvs", c_int32), ("mvs_subsample", c_float)]
 
 
 class Model(Structure):
@@ -87,10 +89,22 @@ def tree_params(max_depth=6, max_bins=255, lambda_l2=1.0, lambda_l1=0.0, max_del
 
 def train_params(loss=TC_LOSS_MSE, n_trees=100, learning_rate=0.1, hist_chunk=0, use_graph=1,
                  subsample=1.0, colsample_bytree=1.0, seed=42, n_classes=0, hist_deep_items=0,
-                 use_goss=False, goss_top_rate=0.2, goss_other_rate=0.1, **tree) -> TrainParams:
+                 use_goss=False, goss_top_rate=0.2, goss_other_rate=0.1, use_mvs=False,
+                 mvs_subsample=0.5, **tree) -> TrainParams:
     return TrainParams(loss, n_trees, learning_rate, tree_params(**tree), hist_chunk, use_graph,
                        subsample, colsample_bytree, seed, n_classes, hist_deep_items,
-                       1 if use_goss else 0, goss_top_rate, goss_other_rate)
+                       1 if use_goss else 0, goss_top_rate, goss_other_rate,
+                       1 if use_mvs else 0, mvs_subsample)
+
+
+def mvs_subsample(g, ratio, seed):
+    """The row sample the trainer draws for one tree under MVS, on gradients g
+    (tc_mvs_subsample); its size depends on the gradients."""
+    g = np.ascontiguousarray(g, np.float32)
+    out = np.zeros(max(len(g), 1), np.uint32)
+    k = c_uint32()
+    check(load().tc_mvs_subsample(_p(g), len(g), ratio, seed, _p(out), byref(k)))
+    return out[:k.value]
 
 
 def channels(p: TrainParams):
@@ -148,6 +162,7 @@ def load(path: str = LIB_PATH):
         "tc_gradients_multi": [V, V, V, V, V, U, U],
         "tc_random_subsample": [U, c_float, c_uint64, V, V],
         "tc_goss_subsample": [V, U, c_float, c_float, c_uint64, V, V],
+        "tc_mvs_subsample": [V, U, c_float, c_uint64, V, V],
         "tc_hist_build": [V, V, V, V, U, V], "tc_hist_subtract": [V, V, V, V, U],
         "tc_split_scan": [V, V, V, V, V, U, U, U, c_float, I, V, V, V],
         "tc_tree_build": [V, V, V, POINTER(TreeParams), V, V, V, V, V, V, V],
