@@ -175,9 +175,12 @@ int32_t tc_init_prediction(int32_t loss, const float* y, uint32_t n, float* base
 
 /* ------------------------------------------------------------ histograms */
 /* A9: HistogramBuilder::build(dataset, rows, {}, out) (histogram.hpp:78-83;
- * histogram.cpp:62-205).  rows == NULL: all rows.  hist_out: F x 256 x 2. */
+ * histogram.cpp:62-205).  rows == NULL: all rows.  hist_out: F x 256 x 2
+ * (grad, hess); count_out (optional): F x 256 row counts, the
+ * GradientPair::count the reference's SplitFinder reads (types.hpp:123-146,
+ * histogram.cpp:705-758), exact. */
 int32_t tc_hist_build(tc_dataset* ds, const float* g, const float* h, const uint32_t* rows,
-                      uint32_t n_rows, float* hist_out);
+                      uint32_t n_rows, float* hist_out, uint32_t* count_out);
 /* A13: Histogram::subtract_from(parent, other) (histogram.cpp:39-42). */
 int32_t tc_hist_subtract(tc_ctx* ctx, const float* parent, const float* child, float* out,
                          uint32_t F);

@@ -88,6 +88,8 @@ class Oracle:
                                       c_uint32, V, V]
         L.tco_predict_f64.argtypes = [V, V, V, V, V, c_uint32, c_uint32, c_double, V, c_uint32,
                                       c_uint32, V, V]
+        L.tco_predict_binned_f32.argtypes = [V, V, V, V, V, c_uint32, c_uint32, c_float, V,
+                                             c_uint32, c_uint32, V]
         L.tco_fit_f32.argtypes = [V, c_uint32, c_uint32, V, V, V, c_int, c_uint32, c_float,
                                   POINTER(TreeParams), V, V, V, V, V, V, V]
         L.tco_fit_f32.restype = c_float
@@ -279,6 +281,17 @@ class Oracle:
                                _p(X), n, F, _p(out), _p(li))
         return (out, li) if want_leaf_idx else out
 
+    def predict_binned_f32(self, model, bins):
+        """A17: binned prediction over column-major bins (F x n) in the
+        reference's float order, + base."""
+        bins = np.ascontiguousarray(bins, np.uint8)
+        F, n = bins.shape
+        out = np.zeros(n, np.float32)
+        self.L.tco_predict_binned_f32(_p(model["feature"]), _p(model["bin"]), _p(model["depth"]),
+                                      _p(model["leaves"]), _p(model["weight"]), model["T"],
+                                      model["D"], model["base"], _p(bins), n, F, _p(out))
+        return out
+
     def predict_f64(self, model, X):
         X = np.ascontiguousarray(X, np.float32)
         n, F = X.shape
@@ -368,6 +381,8 @@ class RefLib:
         L.tcref_fit.restype = c_float
         L.tcref_predict.argtypes = [V, V, V, V, V, c_uint32, c_uint32, c_float, c_float, V,
                                     c_uint32, c_uint32, V]
+        L.tcref_predict_binned.argtypes = [V, V, V, V, V, V, c_uint32, c_uint32, c_float, c_float,
+                                           V]
 
     @staticmethod
     def available(path: str = LIB_REF) -> bool:
@@ -476,6 +491,17 @@ class RefDataset:
                                         _p(li))
         return dict(depth=d, feature=feat[:d], bin=b[:d], fthr=ft[:d], gain=gn[:d],
                     leaves=lv[:1 << d], leaf_idx=li)
+
+    def predict_binned(self, model):
+        """Booster::predict_raw over this handle's binned rows (one weight,
+        the learning rate, for every tree)."""
+        out = np.zeros(self.n, np.float32)
+        self.lib.L.tcref_predict_binned(self.h, _p(model["feature"]), _p(model["fthr"]),
+                                        _p(model["bin"]), _p(model["depth"]), _p(model["leaves"]),
+                                        model["T"], model["D"],
+                                        float(model["weight"][0]) if model["T"] else 0.0,
+                                        model["base"], _p(out))
+        return out
 
     def random_subsample(self, ratio, seed):
         out = np.zeros(self.n, np.uint32)

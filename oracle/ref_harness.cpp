@@ -341,10 +341,9 @@ float tcref_fit_mvs(void* h, int loss_id, uint32_t n_trees, uint32_t max_depth, 
 
 // predict_raw_nobinning_fast without the Booster: flat model -> raw scores
 // (base added, booster.cpp:921-924).
-void tcref_predict(const uint16_t* feat, const float* fthr, const uint8_t* bin, const uint16_t* depths,
-                   const float* leaves, uint32_t n_trees, uint32_t max_depth, float lr, float base,
-                   const float* X, uint32_t n, uint32_t F, float* out) {
-    SymmetricEnsemble ens;
+static void make_ensemble(SymmetricEnsemble& ens, const uint16_t* feat, const float* fthr,
+                          const uint8_t* bin, const uint16_t* depths, const float* leaves,
+                          uint32_t n_trees, uint32_t max_depth, float lr) {
     const uint32_t L = 1u << max_depth;
     for (uint32_t t = 0; t < n_trees; ++t) {
         auto tree = std::make_unique<SymmetricTree>();
@@ -358,8 +357,29 @@ void tcref_predict(const uint16_t* feat, const float* fthr, const uint8_t* bin, 
         tree->set_data(depths[t], std::move(sp), std::move(lv));
         ens.add_tree(std::move(tree), lr);
     }
+}
+
+// Booster::predict_raw_nobinning_fast (booster.cpp:899-925) over the
+// ensemble: SymmetricEnsemble::predict_batch_raw_fast, then + base.
+void tcref_predict(const uint16_t* feat, const float* fthr, const uint8_t* bin, const uint16_t* depths,
+                   const float* leaves, uint32_t n_trees, uint32_t max_depth, float lr, float base,
+                   const float* X, uint32_t n, uint32_t F, float* out) {
+    SymmetricEnsemble ens;
+    make_ensemble(ens, feat, fthr, bin, depths, leaves, n_trees, max_depth, lr);
     ens.predict_batch_raw_fast(X, n, static_cast<FeatureIndex>(F), out);
     for (uint32_t i = 0; i < n; ++i) out[i] += base;
+}
+
+// Booster::predict_raw (booster.cpp:586-605) over the handle's binned rows:
+// SymmetricEnsemble::predict_batch (symmetric_tree.cpp:934-962), then + base.
+void tcref_predict_binned(void* h, const uint16_t* feat, const float* fthr, const uint8_t* bin,
+                          const uint16_t* depths, const float* leaves, uint32_t n_trees,
+                          uint32_t max_depth, float lr, float base, float* out) {
+    auto* r = static_cast<RefData*>(h);
+    SymmetricEnsemble ens;
+    make_ensemble(ens, feat, fthr, bin, depths, leaves, n_trees, max_depth, lr);
+    ens.predict_batch(*r->ds, out);
+    for (uint32_t i = 0; i < r->ds->n_samples(); ++i) out[i] += base;
 }
 
 }  // extern "C"

@@ -356,41 +356,86 @@ static uint32_t float_leaf(const float* row, const uint16_t* feat, const float* 
     return li;
 }
 
+/* Sum of one row's tree values in the reference's float order.  lv[t] is
+ * tree t's leaf value.  Rows of a full 8-row block (grouped != 0) take the
+ * AVX2 block order (fast_float_ensemble.hpp:388-541 for raw floats,
+ * fast_ensemble.hpp:256-426 for bins): groups of four trees, each
+ * w0*l0 + w1*l1 + w2*l2 + w3*l3 as the reference compiler contracts it (w1*l1
+ * rounded, the other three products fused onto it in order), the groups added
+ * with a plain add, the T mod 4 leftover trees fused in one by one (the
+ * compiler forwards the stored product into the vector add).  Other rows take
+ * the scalar order sum += w_t * l_t, one fused multiply-add per tree in tree
+ * order: the rows past the last full 8-row block (fast_float_ensemble.hpp:
+ * 279-287, :543-560; fast_ensemble.hpp:428-446), every row of a binned call
+ * under 16 rows (symmetric_tree.cpp:936-962) and every row of an ensemble
+ * whose trees all have depth 0 (the fast ensembles are then empty and the
+ * fallbacks add tree by tree, symmetric_tree.cpp:1144-1148, :1010-1012). */
+static float ensemble_sum(const float* lv, const float* weight, uint32_t T, int grouped) {
+    float sum = 0.0f;
+    if (!grouped) {
+        for (uint32_t t = 0; t < T; ++t) sum = fmaf(weight[t], lv[t], sum);
+        return sum;
+    }
+    const uint32_t TB = T / 4;
+    for (uint32_t tb = 0; tb < TB; ++tb) {
+        const uint32_t t0 = tb * 4;
+        float grp = weight[t0 + 1] * lv[t0 + 1];
+        grp = fmaf(weight[t0], lv[t0], grp);
+        grp = fmaf(weight[t0 + 2], lv[t0 + 2], grp);
+        grp = fmaf(weight[t0 + 3], lv[t0 + 3], grp);
+        sum = sum + grp;
+    }
+    for (uint32_t t = TB * 4; t < T; ++t) sum = fmaf(weight[t], lv[t], sum);
+    return sum;
+}
+
+static int any_split(const uint16_t* depth, uint32_t T) {
+    for (uint32_t t = 0; t < T; ++t)
+        if (depth[t] > 0) return 1;
+    return 0;
+}
+
 void tco_predict_f32(const uint16_t* feature, const float* fthr, const uint16_t* depth,
                      const float* leaves, const float* weight, uint32_t T, uint32_t D,
                      float base, const float* X, uint32_t n, uint32_t F, float* out,
                      uint32_t* leaf_idx) {
     const uint32_t LP = 1u << D;
-    const uint32_t TB = T / 4;
+    const uint32_t nblk = (n / 8) * 8;
+    const int split = any_split(depth, T);
+    float* lv = (float*)malloc(sizeof(float) * (T ? T : 1));
     for (uint32_t i = 0; i < n; ++i) {
         const float* row = X + (size_t)i * F;
-        float sum = 0.0f;
-        uint32_t li[4];
-        for (uint32_t tb = 0; tb < TB; ++tb) {
-            const uint32_t t0 = tb * 4;
-            for (uint32_t k = 0; k < 4; ++k) {
-                const uint32_t t = t0 + k;
-                li[k] = float_leaf(row, feature + (size_t)t * D, fthr + (size_t)t * D, depth[t]);
-                if (leaf_idx) leaf_idx[(size_t)i * T + t] = li[k];
-            }
-            /* w0*l0 + w1*l1 + w2*l2 + w3*l3 as the reference's compiler
-             * contracts it: w1*l1 rounded, the other products fused onto it
-             * in order (checked against the compiled reference) */
-            float grp = weight[t0 + 1] * leaves[(size_t)(t0 + 1) * LP + li[1]];
-            grp = fmaf(weight[t0], leaves[(size_t)t0 * LP + li[0]], grp);
-            grp = fmaf(weight[t0 + 2], leaves[(size_t)(t0 + 2) * LP + li[2]], grp);
-            grp = fmaf(weight[t0 + 3], leaves[(size_t)(t0 + 3) * LP + li[3]], grp);
-            sum = sum + grp;
-        }
-        for (uint32_t t = TB * 4; t < T; ++t) {
+        for (uint32_t t = 0; t < T; ++t) {
             const uint32_t l = float_leaf(row, feature + (size_t)t * D, fthr + (size_t)t * D, depth[t]);
             if (leaf_idx) leaf_idx[(size_t)i * T + t] = l;
-            /* leftover trees: the reference compiler forwards the stored
-             * product into the vector add and fuses the two */
-            sum = fmaf(weight[t], leaves[(size_t)t * LP + l], sum);
+            lv[t] = leaves[(size_t)t * LP + l];
         }
-        out[i] = sum + base;
+        out[i] = ensemble_sum(lv, weight, T, split && i < nblk) + base; /* booster.cpp:918-924 */
     }
+    free(lv);
+}
+
+void tco_predict_binned_f32(const uint16_t* feature, const uint8_t* bin, const uint16_t* depth,
+                            const float* leaves, const float* weight, uint32_t T, uint32_t D,
+                            float base, const uint8_t* bins, uint32_t n, uint32_t F, float* out) {
+    const uint32_t LP = 1u << D;
+    const uint32_t nblk = n >= 16 ? (n / 8) * 8 : 0;
+    const int split = any_split(depth, T);
+    float* lv = (float*)malloc(sizeof(float) * (T ? T : 1));
+    (void)F;
+    for (uint32_t i = 0; i < n; ++i) {
+        for (uint32_t t = 0; t < T; ++t) {
+            /* go right = bin != 255 && bin > threshold (fast_ensemble.hpp:293-301) */
+            uint32_t l = 0;
+            for (uint32_t d = 0; d < depth[t]; ++d) {
+                const uint8_t v = bins[(size_t)feature[(size_t)t * D + d] * n + i];
+                l = (l << 1) | ((v != 255 && v > bin[(size_t)t * D + d]) ? 1u : 0u);
+            }
+            lv[t] = leaves[(size_t)t * LP + l];
+        }
+        out[i] = ensemble_sum(lv, weight, T, split && i < nblk) + base; /* booster.cpp:599-604 */
+    }
+    free(lv);
 }
 
 void tco_predict_f64(const uint16_t* feature, const float* fthr, const uint16_t* depth,

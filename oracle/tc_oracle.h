@@ -118,14 +118,26 @@ void tco_leaf_index_binned(const uint8_t* bins, uint32_t n, uint32_t depth,
 
 /* A16: raw-float ensemble prediction.  Model arrays: feature/fthr [T*D],
  * depth [T], leaves [T * 2^D], weight [T].  out = sum_t w_t * leaf + base.
- * f32: the reference's 8-row-block order -- groups of four trees summed
- * w0*l0 + w1*l1 + w2*l2 + w3*l3 then added, remaining trees fused in one by one
- * (include/turbocat/fast_float_ensemble.hpp:388-541).  f64: double sum.
+ * f32: the reference's order for a call of n rows -- rows of the full 8-row
+ * blocks sum groups of four trees w0*l0 + w1*l1 + w2*l2 + w3*l3, then add
+ * them, remaining trees fused in one by one
+ * (include/turbocat/fast_float_ensemble.hpp:388-541); the last n mod 8 rows
+ * (and every row when no tree has a split) fuse w_t * l_t into the sum tree
+ * by tree (:279-287, :543-560).  f64: double sum.
  * leaf_idx (optional): n x T indices. */
 void tco_predict_f32(const uint16_t* feature, const float* fthr, const uint16_t* depth,
                      const float* leaves, const float* weight, uint32_t T, uint32_t D,
                      float base, const float* X, uint32_t n, uint32_t F, float* out,
                      uint32_t* leaf_idx);
+/* A17: binned ensemble prediction over column-major bins (F x n),
+ * SymmetricEnsemble::predict_batch -> FastEnsemble::predict_batch_column_major
+ * (src/tree/symmetric_tree.cpp:934-962, include/turbocat/fast_ensemble.hpp:
+ * 256-446) + base (booster.cpp:599-604).  Same summation orders as
+ * tco_predict_f32, except that a call of fewer than 16 rows takes the
+ * tree-by-tree order for every row (symmetric_tree.cpp:954-961). */
+void tco_predict_binned_f32(const uint16_t* feature, const uint8_t* bin, const uint16_t* depth,
+                            const float* leaves, const float* weight, uint32_t T, uint32_t D,
+                            float base, const uint8_t* bins, uint32_t n, uint32_t F, float* out);
 void tco_predict_f64(const uint16_t* feature, const float* fthr, const uint16_t* depth,
                      const float* leaves, const float* weight, uint32_t T, uint32_t D,
                      double base, const float* X, uint32_t n, uint32_t F, double* out,

@@ -81,6 +81,30 @@ def make_loss(name, n, seed):
                         ce=np.float32(R.compute_loss_mc(yk, pk)))
 
 
+def make_pred(name, n, F, seed, T, D, lr):
+    """Raw-float and binned predictions of a reference-trained model in calls
+    of several row counts: full 8-row blocks, a 5-row tail, a call under 16
+    rows (the binned path's tree-by-tree fallback) and under 8 rows."""
+    X, y = regression_data(n, F, seed, low_card_col=1)
+    R = RefLib()
+    ds = R.dataset(X, y)
+    m, _, _ = ds.fit(LOSS_MSE, T, D, lr, 0.0, 1.0)
+    rng = np.random.default_rng(seed + 1)
+    out = dict(meta=np.array([n, F, seed, T, D], np.int64), lr=np.float32(lr),
+               m_feature=m["feature"], m_bin=m["bin"], m_fthr=m["fthr"], m_depth=m["depth"],
+               m_leaves=m["leaves"], m_base=np.float32(m["base"]))
+    for r in (517, 13, 7):
+        Xp = rng.standard_normal((r, F)).astype(np.float32)
+        Xp[::5, 0] = np.nan  # NaN rows go left
+        out[f"X{r}"] = Xp
+        out[f"pred{r}"] = R.predict(m, Xp)
+        dr = R.dataset(Xp, np.zeros(r, np.float32))
+        bins, _, _ = dr.bins()
+        out[f"bins{r}"] = bins
+        out[f"pbin{r}"] = dr.predict_binned(m)
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+
+
 if __name__ == "__main__":
     for c in CASES:
         RefLib.isolated(make, *c)  # fresh thread: see RefLib.isolated
@@ -89,3 +113,5 @@ if __name__ == "__main__":
     print("wrote ce_grad")
     RefLib.isolated(make_loss, "loss_values", 1501, 105)
     print("wrote loss_values")
+    RefLib.isolated(make_pred, "pred_tail", 3000, 6, 106, 23, 5, 0.1)
+    print("wrote pred_tail")

@@ -79,6 +79,23 @@ def test_python_api_surface_matches_reference_defaults():
         assert hasattr(turbocat.TurboCatClassifier, m)
     for m in ("fit", "predict", "save", "load", "feature_importance"):
         assert hasattr(turbocat.TurboCatRegressor, m)
+    # the pybind11 classes' own defaults (python/bindings.cpp:1048-1073, :1099-1118)
+    from turbocat import _turbocat
+
+    def defaults(cls):
+        doc = cls.__init__.__doc__
+        return {k: v.strip() for k, v in re.findall(r"(\w+): [^=,]+ = ('[^']*'|[^,)]+)", doc)}
+
+    c = defaults(_turbocat.TurboCatClassifier)
+    for k, v in dict(n_estimators=500, max_depth=6, max_bins=255, subsample=1.0,
+                     colsample_bytree=1.0, lambda_l2=1.0, seed=42, verbosity=1).items():
+        assert float(c[k]) == v, k
+    assert float(np.float32(float(c["learning_rate"]))) == float(np.float32(0.2))
+    for k, v in dict(loss="logloss", mode="auto", early_stopping_metric="loss",
+                     grow_policy="depthwise", cat_encoding="ordered").items():
+        assert c[k] == repr(v), k
+    r = defaults(_turbocat.TurboCatRegressor)
+    assert r["loss"] == "'mse'" and float(r["lambda_l2"]) == 3.0 and int(r["max_leaves"]) == 64
 
 
 def test_not_fitted_and_unsupported_options():

@@ -105,3 +105,49 @@ def test_regressor_and_save_load(tmp_path):
     clf.save(path)
     clf2 = TurboCatClassifier().load(path)
     assert np.array_equal(clf2.predict_proba(X), clf.predict_proba(X))
+
+
+@pytest.mark.parametrize("name", ["reg_small", "bin_small", "reg_deep", "pred_tail"])
+def test_reference_model_file_predicts_like_the_reference(tmp_path, name):
+    """(f)1 TCAT v3 interoperability on the device path: each golden model the
+    reference trained (tests/golden/*.npz), written in the reference's file
+    layout (booster.cpp:1049-1109), loaded through TurboCatRegressor.load and
+    predicted on the GPU equals the reference's own predictions bit for bit
+    (booster.cpp:899-925), tail rows included; saving it again reproduces the
+    file byte for byte."""
+    import os
+    import sys
+    import turbocat
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_capi_host import write_tcat_v3
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", f"{name}.npz"), allow_pickle=False)
+    if name == "pred_tail":
+        n, F, seed, T, D = (int(v) for v in z["meta"])
+        gain = np.zeros(T * D, np.float32)
+        cases = [(z[f"X{r}"], z[f"pred{r}"]) for r in (517, 13, 7)]
+        edges = np.zeros((F, 255), np.float32)
+        ne = np.zeros(F, np.uint32)
+        task, lam = 2, 0.0
+    else:
+        n, F, seed, loss, T, D = (int(v) for v in z["meta"])
+        gain = z["m_gain"]
+        cases = [(z["Xp"], z["pred"]), (z["Xp"][:509], None)]
+        edges, ne = z["edges"], z["n_edges"]
+        task, lam = (2 if loss == 0 else 0), float(z["lam"])
+    model = dict(T=T, D=D, feature=z["m_feature"], bin=z["m_bin"], fthr=z["m_fthr"], gain=gain,
+                 depth=z["m_depth"], leaves=z["m_leaves"], base=float(z["m_base"]))
+    path = tmp_path / f"{name}.tcat"
+    write_tcat_v3(path, model, edges, ne, task=task, lr=float(z["lr"]), max_depth=D, lam=lam)
+    reg = turbocat.TurboCatRegressor().load(str(path))
+    assert reg.n_trees == T
+    for X, ref in cases:
+        out = reg.predict(X)
+        if ref is not None:
+            assert np.array_equal(out, ref)
+    if name != "pred_tail":       # 509 rows: the last 5 take the tree-by-tree order
+        from oracle import Oracle
+        m = dict(model, weight=np.full(T, z["lr"], np.float32))
+        assert np.array_equal(reg.predict(z["Xp"][:509]), Oracle().predict_f32(m, z["Xp"][:509]))
+    again = tmp_path / "again.tcat"
+    reg.save(str(again))
+    assert open(path, "rb").read() == open(again, "rb").read()

@@ -152,13 +152,14 @@ def test_softmax_fit(ctx, oracle):
     g0, h0 = oracle.gradients_mc_f64(y, np.zeros((n, K), np.float32))
     ref = oracle.tree_mc_f64(bins, g0, h0, edges, ne, otp(max_depth=D, lambda_l2=1.0))
     d0 = int(m["depth"][0])
-    tie_free = ref["gap"][:d0].min() > NEAR_TIE * max(1.0, abs(ref["gain"][0]))
-    if tie_free:
-        assert d0 == ref["depth"]
-        assert np.array_equal(m["feature"][:d0], ref["feature"])
-        assert np.array_equal(m["bin"][:d0], ref["bin"])
-        lv = m["leaves"][:(1 << D) * K].reshape(1 << D, K)[:1 << d0]
-        assert rel_err(lv, ref["leaves"], np.abs(ref["leaves"]) + 1e-3).max() < 1e-4
+    # this seed's first tree has no near-tie at any level (best - second best
+    # is 1.7e-4 of the gain at its closest), so it is always compared
+    assert ref["gap"][:ref["depth"]].min() > NEAR_TIE * max(1.0, abs(ref["gain"][0]))
+    assert d0 == ref["depth"]
+    assert np.array_equal(m["feature"][:d0], ref["feature"])
+    assert np.array_equal(m["bin"][:d0], ref["bin"])
+    lv = m["leaves"][:(1 << D) * K].reshape(1 << D, K)[:1 << d0]
+    assert rel_err(lv, ref["leaves"], np.abs(ref["leaves"]) + 1e-3).max() < 1e-4
     ens = ctx.ensemble(m, F)
     raw = ens.predict(X)
     tp = m["train_preds"].T                              # (n, K)

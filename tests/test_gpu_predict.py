@@ -46,6 +46,62 @@ def test_predict_bitexact(ctx, oracle, T, D, F, n):
     assert np.allclose(p, 1.0 / (1.0 + np.exp(-r64)), rtol=1e-6, atol=1e-7)
 
 
+@pytest.mark.parametrize("n", [1, 7, 8, 13, 16, 23, 517, 3001])
+def test_predict_tail_rows_bitexact(ctx, oracle, n):
+    """The rows past a call's last full 8-row block take the reference's
+    tree-by-tree order (fast_float_ensemble.hpp:543-560): bit for bit against
+    the oracle, which is pinned to the compiled reference on these rows."""
+    from turbocat.capi import train_params
+    X, y = regression_data(6000, 7, 44)
+    ds, _, _ = ctx.quantize_fit(X)
+    m = ds.train(y, train_params(0, n_trees=23, learning_rate=0.1, max_depth=5, lambda_l2=0.0))
+    m["weight"] = np.full(23, 0.1, np.float32)
+    ens = ctx.ensemble(m, 7)
+    Xp, _ = regression_data(n, 7, 45)
+    Xp[::3, 2] = np.nan
+    assert np.array_equal(ens.predict(Xp), oracle.predict_f32(m, Xp))
+    mr = random_model(101, 6, 7, seed=n)
+    assert np.array_equal(ctx.ensemble(mr, 7).predict(Xp), oracle.predict_f32(mr, Xp))
+
+
+def test_predict_matches_reference_golden(ctx):
+    """The reference's own raw-float and binned predictions of a reference-
+    trained model (tests/golden/pred_tail.npz), in calls of 517, 13 and 7
+    rows with NaN rows, reproduced bit for bit on the GPU."""
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "pred_tail.npz"),
+                allow_pickle=False)
+    n, F, seed, T, D = (int(v) for v in z["meta"])
+    m = dict(T=T, D=D, feature=z["m_feature"], bin=z["m_bin"], fthr=z["m_fthr"],
+             depth=z["m_depth"], leaves=z["m_leaves"], weight=np.full(T, z["lr"], np.float32),
+             base=float(z["m_base"]))
+    ens = ctx.ensemble(m, F)
+    for r in (517, 13, 7):
+        assert np.array_equal(ens.predict(z[f"X{r}"]), z[f"pred{r}"]), r
+        ds, _, _ = ctx.quantize_fit(z[f"X{r}"])      # the GPU quantizer: the reference's bins
+        assert np.array_equal(ds.bins(), z[f"bins{r}"]), r
+        assert np.array_equal(ens.predict_binned(ds), z[f"pbin{r}"]), r
+
+
+@pytest.mark.parametrize("n", [9, 15, 16, 517, 20000])
+def test_predict_binned_matches_oracle(ctx, oracle, n):
+    """A17: tc_predict_binned against the oracle's restatement of
+    SymmetricEnsemble::predict_batch (FastEnsemble column-major blocks; the
+    tree-by-tree fallback under 16 rows; tail rows), NaN rows in bin 255."""
+    from turbocat.capi import train_params
+    X, y = regression_data(20000, 9, 5)
+    X[::17, 4] = np.nan
+    ds, edges, ne = ctx.quantize_fit(X)
+    m = ds.train(y, train_params(0, n_trees=30, learning_rate=0.1, max_depth=6, lambda_l2=0.0))
+    m["weight"] = np.full(30, 0.1, np.float32)
+    ens = ctx.ensemble(m, 9)
+    Xp = X[:n]
+    dp = ctx.dataset(Xp, edges, ne)
+    bins = dp.bins()
+    assert (bins[4, ::17] == 255).all()
+    assert np.array_equal(ens.predict_binned(dp), oracle.predict_binned_f32(m, bins))
+
+
 def test_predict_binned_matches_float(ctx, oracle):
     """A.3: with thresholds taken from the training edges, bin > t <=> v >= e[t],
     so binned and raw-float predictions agree for non-NaN rows."""

@@ -79,6 +79,29 @@ def test_hist_all_rows(ctx, oracle, n, F, seed):
     assert np.allclose(gh[..., 1].sum(1, dtype=np.float64), h.sum(dtype=np.float64), rtol=1e-5)
 
 
+@pytest.mark.parametrize("with_rows", [False, True])
+def test_hist_count_plane(ctx, oracle, with_rows):
+    """tc_hist_build's per-bin row counts (GradientPair::count, the field the
+    reference's SplitFinder reads) equal the oracle's counts exactly, NaN rows
+    in slot 255 included."""
+    n, F = 30011, 19
+    X, _ = binary_data(n, F, 4)
+    X[::9, 3] = np.nan
+    ds, _, _ = ctx.quantize_fit(X)
+    bins = ds.bins()
+    rng = np.random.default_rng(5)
+    g = rng.standard_normal(n).astype(np.float32)
+    h = (rng.random(n) * 0.25 + 1e-6).astype(np.float32)
+    rows = np.sort(rng.choice(n, 7777, replace=False)).astype(np.uint32) if with_rows else None
+    gh, cnt = ds.hist(g, h, rows, counts=True)
+    o64, absg, ocnt = oracle.hist_f64(bins, g, h, rows)
+    assert np.array_equal(cnt, ocnt)
+    assert (cnt.sum(1) == (n if rows is None else len(rows))).all()
+    assert cnt[3, 255] == (len(range(0, n, 9)) if rows is None else np.isnan(X[rows, 3]).sum())
+    assert rel_err(gh[..., 0], o64[..., 0], absg).max() < TOL
+    assert np.array_equal(ds.hist(g, h, rows), gh)
+
+
 def test_hist_row_subset_and_subtraction(ctx, oracle):
     n, F = 40000, 11
     X, _ = regression_data(n, F, 7)

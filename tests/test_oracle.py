@@ -23,7 +23,7 @@ from oracle import LOSS_LOGLOSS, LOSS_MSE, RefLib, tree_params
 
 GOLDEN_DIR = os.path.join(os.path.dirname(__file__), "golden")
 GOLDEN = sorted(p for p in glob.glob(os.path.join(GOLDEN_DIR, "*.npz"))
-                if not os.path.basename(p).startswith(("ce_", "loss_")))
+                if not os.path.basename(p).startswith(("ce_", "loss_", "pred_")))
 HAVE_REF = RefLib.available()
 
 
@@ -230,6 +230,46 @@ def test_ce_gradients_match_golden(oracle):
     assert np.abs(g64 - z["g"]).max() < 1e-6 and np.abs(h64 - z["h"]).max() < 1e-6
 
 
+def golden_pred_model():
+    z = np.load(os.path.join(GOLDEN_DIR, "pred_tail.npz"), allow_pickle=False)
+    n, F, seed, T, D = (int(v) for v in z["meta"])
+    m = dict(T=T, D=D, feature=z["m_feature"], bin=z["m_bin"], fthr=z["m_fthr"],
+             depth=z["m_depth"], leaves=z["m_leaves"], weight=np.full(T, z["lr"], np.float32),
+             base=float(z["m_base"]))
+    return z, m
+
+
+def test_predict_tail_rows_match_golden(oracle):
+    """A16/A17 in calls of 517, 13 and 7 rows: the reference's raw-float and
+    binned outputs (tests/golden/make_golden.py) bit for bit, including the
+    rows past the last full 8-row block and NaN rows; the tail order differs
+    from the block order, so a restatement that used the block order for every
+    row fails here."""
+    z, m = golden_pred_model()
+    for r in (517, 13, 7):
+        assert np.array_equal(oracle.predict_f32(m, z[f"X{r}"]), z[f"pred{r}"]), r
+        assert np.array_equal(oracle.predict_binned_f32(m, z[f"bins{r}"]), z[f"pbin{r}"]), r
+    # the orders do differ on these rows
+    full = oracle.predict_f32(m, np.concatenate([z["X517"], z["X517"][:3]]))
+    assert not np.array_equal(full[512:517], z["pred517"][512:])
+
+
+def test_predict_binned_equals_float_on_training_bins(oracle):
+    """bin > t <=> v >= edges[t] and NaN -> bin 255 (both left): over the bins
+    of X the binned prediction is the float one, bit for bit, for every call
+    of 16 rows or more (below 16 the binned path adds tree by tree)."""
+    X, y = regression_data(3000, 6, 12)
+    X[::7, 2] = np.nan
+    bins, edges, ne = oracle.quantize(X)
+    m, _ = oracle.fit_f32(bins, edges, ne, y, LOSS_MSE, 19, 0.1, tree_params(5, lambda_l2=0.0))
+    for r in (3000, 517, 16, 21):
+        b = oracle.apply_bins(X[:r], edges, ne)
+        assert np.array_equal(oracle.predict_binned_f32(m, b), oracle.predict_f32(m, X[:r])), r
+    b = oracle.apply_bins(X[:13], edges, ne)
+    seq = oracle.predict_binned_f32(m, b)
+    assert np.allclose(seq, oracle.predict_f32(m, X[:13]), rtol=1e-6)
+
+
 def test_predict_mc_k1_reduction(oracle):
     """The K-channel FP64 prediction with K = 1 is the single-channel one, and
     with K > 1 each class is the single-channel prediction of its leaves."""
@@ -270,6 +310,34 @@ def test_oracle_matches_reference(oracle, n, F, seed, D):
             assert np.array_equal(rtp, otp)
             Xp = np.random.default_rng(seed).standard_normal((64 * 8, F)).astype(np.float32)
             assert np.array_equal(R.predict(rm, Xp), oracle.predict_f32(om, Xp))
+    RefLib.isolated(body)
+
+
+@pytest.mark.skipif(not HAVE_REF, reason="oracle/_ref/libtcref.so not built (no reference tree)")
+def test_predict_matches_reference_every_call_size(oracle):
+    """Raw-float (predict_batch_raw_fast) and binned (SymmetricEnsemble::
+    predict_batch) predictions of the compiled reference, bit for bit, for call
+    sizes around the 8-row blocks, the 16-row binned fallback and the 256-row
+    transpose threshold, and for an ensemble whose trees all have depth 0."""
+    def body():
+        R = RefLib()
+        X, y = regression_data(2400, 6, 21)
+        X[::11, 1] = np.nan
+        ds = R.dataset(X, y)
+        for T in (23, 8, 1):
+            m, _, _ = ds.fit(LOSS_MSE, T, 5, 0.1, 0.0, 1.0)
+            for r in (1, 7, 8, 13, 15, 16, 17, 255, 256, 263, 517):
+                Xp = X[:r]
+                assert np.array_equal(R.predict(m, Xp), oracle.predict_f32(m, Xp)), (T, r)
+                dr = R.dataset(Xp, y[:r])
+                bins, _, _ = dr.bins()
+                assert np.array_equal(dr.predict_binned(m), oracle.predict_binned_f32(m, bins)), (T, r)
+        flat = dict(T=5, D=3, feature=np.zeros(15, np.uint16), fthr=np.zeros(15, np.float32),
+                    bin=np.zeros(15, np.uint8), depth=np.zeros(5, np.uint16),
+                    leaves=np.random.default_rng(0).standard_normal(40).astype(np.float32),
+                    weight=np.full(5, 0.1, np.float32), base=0.3)
+        for r in (300, 17, 5):
+            assert np.array_equal(R.predict(flat, X[:r]), oracle.predict_f32(flat, X[:r]))
     RefLib.isolated(body)
 
 

@@ -340,7 +340,7 @@ PYBIND11_MODULE(_turbocat, m) {
            py::arg("n_estimators") = 500, py::arg("learning_rate") = 0.2f, py::arg("max_depth") = 6,
            py::arg("max_bins") = 255, py::arg("subsample") = 1.0f,
            py::arg("colsample_bytree") = 1.0f, py::arg("min_child_weight") = 1.0f,
-           py::arg("lambda_l2") = 1.0f, py::arg("loss") = "auto", py::arg("use_goss") = false,
+           py::arg("lambda_l2") = 1.0f, py::arg("loss") = "logloss", py::arg("use_goss") = false,
            py::arg("goss_top_rate") = 0.2f, py::arg("goss_other_rate") = 0.1f,
            py::arg("use_gradtree") = false, py::arg("use_symmetric") = false,
            py::arg("use_ordered_boosting") = false, py::arg("grow_policy") = "depthwise",

@@ -32,6 +32,7 @@ struct tc_ensemble {
   uint32_t T = 0, D = 0, F = 0;
   uint32_t K = 1;               // output channels (classes of a softmax ensemble)
   float base = 0.f;
+  bool any_split = false;       // some tree has depth > 0
   uint16_t* feat = nullptr;     // [T][D]
   float* thr = nullptr;         // [T][D]
   uint8_t* bin = nullptr;       // [T][D]

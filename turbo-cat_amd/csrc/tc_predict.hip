@@ -10,8 +10,18 @@
 // compiler contracts it: w1*l1 rounded, the other three fused onto it).  Then lane 0 of the row adds
 // the groups to the row's sum in tree order, exactly the reference's
 // 8-row-block order (fast_float_ensemble.hpp:480-541): groups added one by
-// one, leftover trees fused in one by one (sum = fma(w, leaf, sum)).  A row's value
-// depends only on the row and the model, never on the batch it came in.
+// one, leftover trees fused in one by one (sum = fma(w, leaf, sum)).
+//
+// The reference sums some rows tree by tree instead, sum = fma(w_t, l_t, sum)
+// for every tree in order: the rows of a call past its last full 8-row block
+// (fast_float_ensemble.hpp:279-287, :543-560; fast_ensemble.hpp:428-446),
+// every row of a binned call under 16 rows (symmetric_tree.cpp:954-961) and
+// every row when no tree has a split (the fast ensembles are then empty,
+// symmetric_tree.cpp:1144-1148).  Such a row is marked by seq_flag[row] (a
+// served batch, whose rows come from several requests) or by row >= seq_from
+// (one call); lane 0 then fuses the group's four leaf values one by one.  So a
+// row's value depends only on the row, the model and its place in its own
+// request, never on the batch the request was coalesced into.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -43,12 +53,14 @@ template <bool BINNED>
 __global__ void __launch_bounds__(256) k_predict(EnsDev E, const void* __restrict__ Xv, uint32_t n,
                                                  uint32_t F, uint32_t ldc, uint32_t R,
                                                  int transform, float* __restrict__ out,
-                                                 uint32_t* __restrict__ leaf_idx) {
+                                                 uint32_t* __restrict__ leaf_idx, uint32_t seq_from,
+                                                 const uint8_t* __restrict__ seq_flag) {
   extern __shared__ unsigned char s_raw[];
   const uint32_t Q = 256u / R;                 // tree lanes per row
   const uint32_t S = BINNED ? (F | 3u) + 1u : (F | 1u);  // row stride in LDS elements
   float* s_gv = reinterpret_cast<float*>(s_raw);           // [Q][R] group values
-  unsigned char* s_x = s_raw + sizeof(float) * 256;
+  float* s_lv = s_gv + 256;                                // [Q][4][R] leaf values (tree-by-tree rows)
+  unsigned char* s_x = s_raw + sizeof(float) * 256 * 5;
   const uint32_t tid = threadIdx.x, q = tid / R, r = tid % R;
   const uint32_t row0 = blockIdx.x * R;
   const uint32_t nr = min(R, n - row0);
@@ -68,6 +80,7 @@ __global__ void __launch_bounds__(256) k_predict(EnsDev E, const void* __restric
   __syncthreads();
   const uint32_t LP = 1u << E.D;
   const uint32_t groups = E.T / 4u;
+  const bool seq = r < nr && (seq_flag ? seq_flag[row0 + r] != 0 : row0 + r >= seq_from);
   float sum = 0.f;
   for (uint32_t g0 = 0; g0 < groups; g0 += Q) {
     const uint32_t g = g0 + q;
@@ -93,19 +106,30 @@ __global__ void __launch_bounds__(256) k_predict(EnsDev E, const void* __restric
         if (leaf_idx) leaf_idx[static_cast<size_t>(row0 + r) * E.T + t] = li;
         lv[k] = E.leaves[static_cast<size_t>(t) * LP + li];
       }
-      // the reference compiler's contraction of w0*l0 + w1*l1 + w2*l2 + w3*l3:
-      // w1*l1 rounded, then w0*l0, w2*l2, w3*l3 fused onto it in that order
-      const float* w = E.weight + 4 * g;
-      float gv = __fmul_rn(w[1], lv[1]);
-      gv = __fmaf_rn(w[0], lv[0], gv);
-      gv = __fmaf_rn(w[2], lv[2], gv);
-      gv = __fmaf_rn(w[3], lv[3], gv);
-      s_gv[q * R + r] = gv;
+      if (seq) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s_lv[(q * 4 + k) * R + r] = lv[k];
+      } else {
+        // the reference compiler's contraction of w0*l0 + w1*l1 + w2*l2 + w3*l3:
+        // w1*l1 rounded, then w0*l0, w2*l2, w3*l3 fused onto it in that order
+        const float* w = E.weight + 4 * g;
+        float gv = __fmul_rn(w[1], lv[1]);
+        gv = __fmaf_rn(w[0], lv[0], gv);
+        gv = __fmaf_rn(w[2], lv[2], gv);
+        gv = __fmaf_rn(w[3], lv[3], gv);
+        s_gv[q * R + r] = gv;
+      }
     }
     __syncthreads();
     if (q == 0 && r < nr) {
       const uint32_t ng = min(Q, groups - g0);
-      for (uint32_t k = 0; k < ng; ++k) sum = __fadd_rn(sum, s_gv[k * R + r]);
+      if (seq) {
+        for (uint32_t k = 0; k < ng; ++k)
+          for (uint32_t j = 0; j < 4; ++j)
+            sum = __fmaf_rn(E.weight[4 * (g0 + k) + j], s_lv[(k * 4 + j) * R + r], sum);
+      } else {
+        for (uint32_t k = 0; k < ng; ++k) sum = __fadd_rn(sum, s_gv[k * R + r]);
+      }
     }
     __syncthreads();
   }
@@ -256,8 +280,10 @@ EnsDev ens_dev(const tc_ensemble* e) {
 void predict_shape(uint32_t F, uint32_t K, bool binned, uint32_t* R, size_t* lds) {
   const uint32_t S = binned ? (F | 3u) + 1u : (F | 1u);
   const size_t elem = binned ? 1 : 4;
+  // K = 1: group values and the tree-by-tree rows' leaf values (5 x 256 floats)
+  const size_t head = sizeof(float) * 256 * (K > 1 ? K : 5u);
   for (uint32_t r : {64u, 32u, 16u, 8u, 4u, 2u, 1u}) {
-    const size_t bytes = sizeof(float) * 256 * K + static_cast<size_t>(r) * S * elem;
+    const size_t bytes = head + static_cast<size_t>(r) * S * elem;
     if (bytes <= 160 * 1024 || r == 1) {
       *R = r;
       *lds = bytes;
@@ -266,9 +292,19 @@ void predict_shape(uint32_t F, uint32_t K, bool binned, uint32_t* R, size_t* lds
   }
 }
 
+// First row of a call of n rows that the reference sums tree by tree (see
+// k_predict): past the last full 8-row block; every row of a binned call
+// under 16 rows, or of an ensemble without splits.
+uint32_t seq_rows_from(const tc_ensemble* e, bool binned, uint32_t n) {
+  if (!e->any_split || (binned && n < 16)) return 0u;
+  return n / 8u * 8u;
+}
+
 void launch_predict(const tc_ensemble* e, const void* X, bool binned, uint32_t n, uint32_t F,
-                    uint32_t ldc, int transform, float* out, uint32_t* leaf_idx, hipStream_t s) {
+                    uint32_t ldc, int transform, float* out, uint32_t* leaf_idx, hipStream_t s,
+                    const uint8_t* seq_flag = nullptr) {
   if (n == 0) return;
+  const uint32_t seq_from = seq_rows_from(e, binned, n);
   uint32_t R;
   size_t lds;
   predict_shape(F, e->K, binned, &R, &lds);
@@ -286,13 +322,13 @@ void launch_predict(const tc_ensemble* e, const void* X, bool binned, uint32_t n
       TC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_predict<true>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     hipLaunchKernelGGL(k_predict<true>, grid, dim3(256), lds, s, ens_dev(e), X, n, F, ldc, R,
-                       transform, out, leaf_idx);
+                       transform, out, leaf_idx, seq_from, seq_flag);
   } else {
     if (lds > 64 * 1024)
       TC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_predict<false>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     hipLaunchKernelGGL(k_predict<false>, grid, dim3(256), lds, s, ens_dev(e), X, n, F, ldc, R,
-                       transform, out, leaf_idx);
+                       transform, out, leaf_idx, seq_from, seq_flag);
   }
   TC_HIP(hipGetLastError());
 }
@@ -396,8 +432,10 @@ struct Slot {
   hipEvent_t done = nullptr;
   float* h_x = nullptr;    // pinned
   float* h_out = nullptr;  // pinned
+  uint8_t* h_flag = nullptr;  // pinned: rows the reference sums tree by tree
   float* d_x = nullptr;
   float* d_out = nullptr;
+  uint8_t* d_flag = nullptr;
   bool busy = false;
   std::vector<Segment> segs;
 };
@@ -476,6 +514,9 @@ static void serve_loop(tc_server* S) {
         constexpr size_t kPiece = 1u << 20;
         std::vector<CopyPool::Piece> pieces;
         for (const auto& sg : sl.segs) {
+          // a request's own tree-by-tree rows, wherever it lands in the batch
+          const uint32_t from = seq_rows_from(S->ens, false, sg.req->n);
+          for (uint32_t k = 0; k < sg.count; ++k) sl.h_flag[sg.slot_row + k] = sg.row0 + k >= from ? 1 : 0;
           const char* src = reinterpret_cast<const char*>(sg.req->X + static_cast<size_t>(sg.row0) * F);
           char* dst = reinterpret_cast<char*>(sl.h_x + static_cast<size_t>(sg.slot_row) * F);
           const size_t bytes = 4ull * sg.count * F;
@@ -487,7 +528,9 @@ static void serve_loop(tc_server* S) {
       lk.lock();
       try {
         TC_HIP(hipMemcpyAsync(sl.d_x, sl.h_x, 4ull * rows * F, hipMemcpyHostToDevice, sl.stream));
-        launch_predict(S->ens, sl.d_x, false, rows, F, 0, S->p.transform, sl.d_out, nullptr, sl.stream);
+        TC_HIP(hipMemcpyAsync(sl.d_flag, sl.h_flag, rows, hipMemcpyHostToDevice, sl.stream));
+        launch_predict(S->ens, sl.d_x, false, rows, F, 0, S->p.transform, sl.d_out, nullptr, sl.stream,
+                       sl.d_flag);
         TC_HIP(hipMemcpyAsync(sl.h_out, sl.d_out, 4ull * rows, hipMemcpyDeviceToHost, sl.stream));
         TC_HIP(hipEventRecord(sl.done, sl.stream));
       } catch (const std::exception& e) {
@@ -543,6 +586,7 @@ int32_t tc_ensemble_create_multi(tc_ctx* ctx, uint32_t n_trees, uint32_t max_dep
     }
     auto* e = new tc_ensemble();
     e->ctx = ctx; e->T = n_trees; e->D = max_depth; e->F = n_features; e->K = K; e->base = base;
+    for (uint32_t t = 0; t < n_trees; ++t) e->any_split = e->any_split || depth[t] > 0;
     try {
       const size_t TD = std::max<size_t>(1, static_cast<size_t>(n_trees) * max_depth);
       const size_t TL = std::max<size_t>(1, (static_cast<size_t>(n_trees) << max_depth) * K);
@@ -663,8 +707,11 @@ int32_t tc_serve_start(tc_ensemble* e, const tc_serve_params* p, tc_server** out
                              hipHostMallocDefault));
         TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_out), 4ull * p->max_batch_rows,
                              hipHostMallocDefault));
+        TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_flag), p->max_batch_rows,
+                             hipHostMallocDefault));
         sl.d_x = dmalloc<float>(static_cast<size_t>(p->max_batch_rows) * e->F);
         sl.d_out = dmalloc<float>(p->max_batch_rows);
+        sl.d_flag = dmalloc<uint8_t>(p->max_batch_rows);
       }
     } catch (...) {
       for (auto& sl : S->slots) {
@@ -672,8 +719,10 @@ int32_t tc_serve_start(tc_ensemble* e, const tc_serve_params* p, tc_server** out
         if (sl.done) (void)hipEventDestroy(sl.done);
         if (sl.h_x) (void)hipHostFree(sl.h_x);
         if (sl.h_out) (void)hipHostFree(sl.h_out);
+        if (sl.h_flag) (void)hipHostFree(sl.h_flag);
         dfree(sl.d_x);
         dfree(sl.d_out);
+        dfree(sl.d_flag);
       }
       delete S;
       throw;
@@ -750,8 +799,10 @@ int32_t tc_serve_stop(tc_server* S) {
       (void)hipEventDestroy(sl.done);
       (void)hipHostFree(sl.h_x);
       (void)hipHostFree(sl.h_out);
+      (void)hipHostFree(sl.h_flag);
       dfree(sl.d_x);
       dfree(sl.d_out);
+      dfree(sl.d_flag);
     }
     delete S;
   });

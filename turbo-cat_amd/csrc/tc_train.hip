@@ -530,11 +530,15 @@ class Trainer {
   // a result (every histogram is an exact integer sum and the fixed-point
   // scale does not depend on the item size), so the tuner times the fit's
   // own first trees.
+  // The first candidate is the shape used when a fit has too few trees to
+  // tune: two root items per CU and one item per CU below the root (256 CUs),
+  // the fastest shape measured on the 1M x 100 LogLoss workload (a level waits
+  // for its CUs that take a second item).
   struct TuneChoice {
     uint32_t root_items, deep_items;
   };
-  static constexpr TuneChoice kTune[8] = {{1024, 512}, {512, 512},  {512, 1024}, {1024, 1024},
-                                          {512, 256},  {2048, 1024}, {512, 128}, {1024, 256}};
+  static constexpr TuneChoice kTune[8] = {{512, 256},  {512, 512},  {1024, 512}, {512, 128},
+                                          {1024, 256}, {512, 1024}, {1024, 1024}, {2048, 1024}};
 
   // Upper bound of job J's histogram work items at level d.  Below the root
   // the histogrammed (smaller) children hold at most N/2 rows over 2^(d-1)
@@ -580,7 +584,9 @@ class Trainer {
   }
 
   // Times one tree per candidate shape (two each when the budget allows),
-  // after one warm-up tree, and keeps the fastest.  Returns the trees built.
+  // after one warm-up tree, and keeps the fastest per histogrammed row (trees
+  // grown on samples of different sizes, as under MVS, compare fairly).
+  // Returns the trees built.
   uint32_t autotune(uint32_t budget) {
     constexpr uint32_t kN = sizeof(kTune) / sizeof(kTune[0]);
     if (!tune_ || tuned_) return 0;
@@ -589,30 +595,54 @@ class Trainer {
     hipStream_t s = ctx_->stream;
     std::vector<hipEvent_t> ev(2 * kN * reps);
     for (auto& e : ev) TC_HIP(hipEventCreate(&e));
-    if (sampling()) prepare_samples();
-    enqueue_tree(nullptr);
-    for (uint32_t r = 0, k = 0; r < reps; ++r)
-      for (uint32_t c = 0; c < kN; ++c, k += 2) {
-        choose(c);
-        if (sampling()) prepare_samples();
-        TC_HIP(hipEventRecord(ev[k], s));
-        enqueue_tree(nullptr);
-        TC_HIP(hipEventRecord(ev[k + 1], s));
+    // every job's histogrammed-row counter after each timed tree (and before the first)
+    const uint32_t nsnap = kN * reps + 1;
+    unsigned long long* snap = dmalloc<unsigned long long>(static_cast<size_t>(nsnap) * njobs_);
+    auto snapshot = [&](uint32_t k) {
+      for (uint32_t j = 0; j < njobs_; ++j)
+        TC_HIP(hipMemcpyAsync(snap + static_cast<size_t>(k) * njobs_ + j, hist_rows_[j], 8,
+                              hipMemcpyDeviceToDevice, s));
+    };
+    try {
+      if (sampling()) prepare_samples();
+      enqueue_tree(nullptr);
+      snapshot(0);
+      for (uint32_t r = 0, k = 0; r < reps; ++r)
+        for (uint32_t c = 0; c < kN; ++c, k += 2) {
+          choose(c);
+          if (sampling()) prepare_samples();
+          TC_HIP(hipEventRecord(ev[k], s));
+          enqueue_tree(nullptr);
+          TC_HIP(hipEventRecord(ev[k + 1], s));
+          snapshot(r * kN + c + 1);
+        }
+      TC_HIP(hipStreamSynchronize(s));
+      std::vector<unsigned long long> rows(static_cast<size_t>(nsnap) * njobs_);
+      TC_HIP(hipMemcpy(rows.data(), snap, 8 * rows.size(), hipMemcpyDeviceToHost));
+      uint32_t best = 0;
+      for (uint32_t c = 0; c < kN; ++c) {
+        double t = 0.0, nr = 0.0;
+        for (uint32_t r = 0; r < reps; ++r) {
+          const uint32_t k = r * kN + c;
+          float x = 0.f;
+          TC_HIP(hipEventElapsedTime(&x, ev[2 * k], ev[2 * k + 1]));
+          t += x;
+          for (uint32_t j = 0; j < njobs_; ++j)
+            nr += static_cast<double>(rows[static_cast<size_t>(k + 1) * njobs_ + j] -
+                                      rows[static_cast<size_t>(k) * njobs_ + j]);
+        }
+        tune_ms_[c] = static_cast<float>(t / reps);
+        tune_cost_[c] = t / std::max(1.0, nr);
+        if (tune_cost_[c] < tune_cost_[best]) best = c;
       }
-    TC_HIP(hipStreamSynchronize(s));
-    uint32_t best = 0;
-    for (uint32_t c = 0; c < kN; ++c) {
-      float t = 0.f;
-      for (uint32_t r = 0; r < reps; ++r) {
-        float x = 0.f;
-        TC_HIP(hipEventElapsedTime(&x, ev[2 * (r * kN + c)], ev[2 * (r * kN + c) + 1]));
-        t += x;
-      }
-      tune_ms_[c] = t / reps;
-      if (tune_ms_[c] < tune_ms_[best]) best = c;
+      choose(best);
+    } catch (...) {
+      dfree(snap);
+      for (auto& e : ev) (void)hipEventDestroy(e);
+      throw;
     }
+    dfree(snap);
     for (auto& e : ev) (void)hipEventDestroy(e);
-    choose(best);
     tuned_ = true;
     return 1 + kN * reps;
   }
@@ -959,6 +989,7 @@ class Trainer {
   bool tune_ = false, tuned_ = false;
   uint32_t choice_ = 0;
   float tune_ms_[8] = {};
+  double tune_cost_[8] = {};  // ms per histogrammed row of each candidate
 
   // Dynamic LDS of k_grad_update: the leaf values of one tree, [2^D][K].
   size_t leaf_lds() const { return 4ull * (static_cast<size_t>(kmax_) << D_); }
@@ -1248,7 +1279,7 @@ int32_t tc_tree_build_multi(tc_dataset* ds, const float* g, const float* h, uint
 }
 
 int32_t tc_hist_build(tc_dataset* ds, const float* g, const float* h, const uint32_t* rows,
-                      uint32_t n_rows, float* hist_out) {
+                      uint32_t n_rows, float* hist_out, uint32_t* count_out) {
   return guard([&] {
     TC_REQUIRE(ds && g && h && hist_out, "tc_hist_build: NULL argument");
     tc_ctx* ctx = ds->ctx;
@@ -1315,6 +1346,27 @@ int32_t tc_hist_build(tc_dataset* ds, const float* g, const float* h, const uint
                          J.st, F * 256u, fh);
       TC_HIP(hipGetLastError());
       TC_HIP(hipMemcpyAsync(hist_out, fh, sizeof(float2) * F * 256, hipMemcpyDeviceToHost, s));
+      if (count_out) {
+        // GradientPair::count (types.hpp:123-146): the same rows once more with
+        // every (g, h) = (1, 1) at scale 1, so each cell's g sum is its row count
+        std::fill(ghh.begin(), ghh.end(), make_float2(1.f, 1.f));
+        TC_HIP(hipMemcpyAsync(J.gh, ghh.data(), 8ull * N, hipMemcpyHostToDevice, s));
+        TC_HIP(hipMemsetAsync(J.hist, 0, sizeof(HistCell) * F * 256, s));
+        TreeState one{};
+        one.active = 1;
+        one.gscale = one.ginv = one.hscale = one.hinv = 1.0;
+        TC_HIP(hipMemcpyAsync(J.st, &one, sizeof(one), hipMemcpyHostToDevice, s));
+        if (cnt)
+          hipLaunchKernelGGL((k_hist<kHistUnroll, false, true, false>), dim3(round8(cnt)),
+                             dim3(kHistThreads), 0, s, dj, dw, dc, 0u,
+                             static_cast<HistCell*>(nullptr), Lists{});
+        TC_HIP(hipGetLastError());
+        std::vector<HistCell> cells(static_cast<size_t>(F) * 256);
+        TC_HIP(hipMemcpyAsync(cells.data(), J.hist, sizeof(HistCell) * cells.size(),
+                              hipMemcpyDeviceToHost, s));
+        TC_HIP(hipStreamSynchronize(s));
+        for (size_t i = 0; i < cells.size(); ++i) count_out[i] = static_cast<uint32_t>(cells[i].g);
+      }
       TC_HIP(hipStreamSynchronize(s));
     } catch (...) {
       for (void* p : owned) dfree(p);

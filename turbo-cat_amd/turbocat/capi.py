@@ -163,7 +163,7 @@ def load(path: str = LIB_PATH):
         "tc_random_subsample": [U, c_float, c_uint64, V, V],
         "tc_goss_subsample": [V, U, c_float, c_float, c_uint64, V, V],
         "tc_mvs_subsample": [V, U, c_float, c_uint64, V, V],
-        "tc_hist_build": [V, V, V, V, U, V], "tc_hist_subtract": [V, V, V, V, U],
+        "tc_hist_build": [V, V, V, V, U, V, V], "tc_hist_subtract": [V, V, V, V, U],
         "tc_split_scan": [V, V, V, V, V, U, U, U, c_float, I, V, V, V],
         "tc_tree_build": [V, V, V, POINTER(TreeParams), V, V, V, V, V, V, V],
         "tc_tree_build_multi": [V, V, V, U, POINTER(TreeParams), V, V, V, V, V, V, V],
@@ -353,13 +353,16 @@ class Dataset:
         check(self.ctx.L.tc_dataset_bins(self.h, _p(out)))
         return out
 
-    def hist(self, g, h, rows=None):
+    def hist(self, g, h, rows=None, counts=False):
+        """F x 256 x (grad, hess) float32; with counts=True also the F x 256
+        row counts (GradientPair::count)."""
         out = np.zeros((self.F, 256, 2), np.float32)
+        cnt = np.zeros((self.F, 256), np.uint32) if counts else None
         r = None if rows is None else np.ascontiguousarray(rows, np.uint32)
         check(self.ctx.L.tc_hist_build(self.h, _p(np.ascontiguousarray(g, np.float32)),
                                        _p(np.ascontiguousarray(h, np.float32)), _p(r),
-                                       0 if r is None else len(r), _p(out)))
-        return out
+                                       0 if r is None else len(r), _p(out), _p(cnt)))
+        return (out, cnt) if counts else out
 
     def tree(self, g, h, params: TreeParams, want_leaf_idx=True):
         D = params.max_depth
