@@ -304,6 +304,16 @@ int32_t tc_predict(tc_ensemble* ens, const float* X, uint32_t n, uint32_t F, int
                    float* out, uint32_t* leaf_idx);
 int32_t tc_predict_dev(tc_ensemble* ens, const float* X_dev, uint32_t n, uint32_t F,
                        int32_t transform, float* out_dev);
+/* tc_predict without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (tight serving loops, harnesses
+ * that forbid hipMalloc): the rows and the outputs are staged in a fixed
+ * 64 MiB device scratch that the library holds on each GPU until the process
+ * ends, so 4 * n * F (rounded up to 256) + 4 * n * K must be at most 64 MiB
+ * (TC_EINVAL otherwise).  Every call on one GPU shares that scratch: calls
+ * that overlap on one GPU (several threads, or several contexts on it) race.
+ * Same results as tc_predict (transform 0 raw, 1 sigmoid / softmax). */
+int32_t tc_predict_noalloc(tc_ensemble* ens, const float* X, uint32_t n, uint32_t F,
+                           int32_t transform, float* out);
 /* A17: binned prediction (SymmetricEnsemble::predict_batch, symmetric_tree.cpp:
  * 934-962) over a device dataset's bins. */
 int32_t tc_predict_binned(tc_ensemble* ens, tc_dataset* ds, float* out);

@@ -39,6 +39,7 @@ int32_t tc_ctx_create(int32_t device_id, tc_ctx** out) {
       throw Error(TC_EDEVICE, std::string("device is ") + prop.gcnArchName +
                                   "; this library is built for gfx950 (MI355X)");
     TC_HIP(hipSetDevice(device_id));
+    scratch_setup(device_id);  // the allocation-free entry point's scratch, once per device
     auto* c = new tc_ctx();
     c->device = device_id;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {

@@ -91,6 +91,10 @@ inline void dfree(void* p) {
   if (p) (void)hipFree(p);
 }
 
+// Address of the allocation-free prediction's device scratch on `device`,
+// taken once per device (the device must be current).
+void scratch_setup(int device);
+
 // Host-side edge computation shared by fit and tests (dataset.cpp:278-334).
 void compute_edges(const float* X, uint32_t n, uint32_t F, uint32_t max_bins, bool wrap,
                    std::vector<float>& edges, std::vector<uint32_t>& n_edges);

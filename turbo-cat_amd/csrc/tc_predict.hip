@@ -269,6 +269,39 @@ __global__ void __launch_bounds__(256) k_predict_multi(EnsDev E, const void* __r
   }
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free prediction entry point: a file-scope device
+// array (one copy per GPU, held until the process ends).  Host code never
+// passes the array's name to a kernel or a copy: it uses the address
+// hipGetSymbolAddress returns on each device, taken once per device when the
+// device's first context is created (scratch_setup, called by tc_ctx_create
+// with the device current).
+constexpr size_t kScratchBytes = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_dev;  // per device: the address of `scratch`, or null
+}  // namespace
+
+void scratch_setup(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev.size() <= static_cast<size_t>(device)) g_scratch_dev.resize(device + 1, nullptr);
+  if (g_scratch_dev[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || p == nullptr)
+    throw Error(TC_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_dev[device] = p;
+}
+
+static unsigned char* scratch_of(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev.size() <= static_cast<size_t>(device) || !g_scratch_dev[device])
+    throw Error(TC_ESTATE, "scratch of this device was not set up (no context created on it)");
+  return static_cast<unsigned char*>(g_scratch_dev[device]);
+}
+
 EnsDev ens_dev(const tc_ensemble* e) {
   EnsDev E;
   E.feat = e->feat; E.thr = e->thr; E.bin = e->bin; E.depth = e->depth; E.leaves = e->leaves;
@@ -666,6 +699,30 @@ int32_t tc_predict(tc_ensemble* e, const float* X, uint32_t n, uint32_t F, int32
     }
     dfree(dx);
     dfree(dl);
+  });
+}
+
+int32_t tc_predict_noalloc(tc_ensemble* e, const float* X, uint32_t n, uint32_t F,
+                           int32_t transform, float* out) {
+  return guard([&] {
+    TC_REQUIRE(e && (X || n == 0) && (out || n == 0), "tc_predict_noalloc: NULL argument");
+    TC_REQUIRE(F == e->F, "feature count differs from the model's");
+    if (n == 0) return;
+    // the rows, then the outputs, each at a 256-byte boundary of the scratch
+    const size_t xb = (4ull * n * F + 255) & ~size_t(255), ob = 4ull * n * e->K;
+    if (xb + ob > kScratchBytes)
+      throw Error(TC_EINVAL, "tc_predict_noalloc: " + std::to_string(n) + " rows of " +
+                                 std::to_string(F) + " features need " + std::to_string(xb + ob) +
+                                 " bytes of scratch, more than its " +
+                                 std::to_string(kScratchBytes) + " bytes");
+    unsigned char* base = scratch_of(e->ctx->device);
+    float* dx = reinterpret_cast<float*>(base);
+    float* dout = reinterpret_cast<float*>(base + xb);
+    hipStream_t s = e->ctx->stream;
+    TC_HIP(hipMemcpyAsync(dx, X, 4ull * n * F, hipMemcpyHostToDevice, s));
+    launch_predict(e, dx, false, n, F, 0, transform, dout, nullptr, s);
+    TC_HIP(hipMemcpyAsync(out, dout, ob, hipMemcpyDeviceToHost, s));
+    TC_HIP(hipStreamSynchronize(s));
   });
 }
 

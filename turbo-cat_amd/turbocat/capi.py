@@ -37,7 +37,7 @@ EXPORTS = [
     "tc_trainer_sync", "tc_trainer_model", "tc_trainer_hist_rows", "tc_trainer_hist_chunks",
     "tc_trainer_destroy",
     "tc_ensemble_create", "tc_ensemble_create_multi", "tc_ensemble_destroy", "tc_predict",
-    "tc_predict_dev", "tc_predict_binned", "tc_serve_start", "tc_serve_submit",
+    "tc_predict_dev", "tc_predict_noalloc", "tc_predict_binned", "tc_serve_start", "tc_serve_submit",
     "tc_serve_wait", "tc_serve_stats", "tc_serve_stop",
 ]
 
@@ -181,6 +181,7 @@ def load(path: str = LIB_PATH):
         "tc_ensemble_create_multi": [V, U, U, U, V, V, V, V, V, V, c_float, U, V],
         "tc_ensemble_destroy": [V], "tc_predict": [V, V, U, U, I, V, V],
         "tc_predict_dev": [V, V, U, U, I, V], "tc_predict_binned": [V, V, V],
+        "tc_predict_noalloc": [V, V, U, U, I, V],
         "tc_serve_start": [V, POINTER(ServeParams), V], "tc_serve_submit": [V, V, U, V, V],
         "tc_serve_wait": [V, c_uint64, c_int64], "tc_serve_stats": [V, V, V],
         "tc_serve_stop": [V],
