@@ -319,22 +319,50 @@ int32_t tc_predict_noalloc(tc_ensemble* ens, const float* X, uint32_t n, uint32_
 int32_t tc_predict_binned(tc_ensemble* ens, tc_dataset* ds, float* out);
 
 /* --------------------------------------------------------------- serving */
-/* Continuous-batching server over a resident ensemble.  Requests are admitted
- * while their rows fit the memory budget and a batch slot is free; arrivals
- * during a launch join the next coalesced batch; up to n_streams batches run
- * on the device at once.  No reference counterpart. */
+/* Continuous-batching server over a resident ensemble (single-output or
+ * K-channel).  No reference counterpart.  One persistent kernel stays
+ * resident for the server's life and takes batches from a ring of
+ * descriptors in coherent pinned host memory: no launch and no copy call per
+ * batch.  Requests are admitted while their rows (4F + 4K bytes each) fit
+ * memory_budget (a request larger than the whole budget is admitted alone);
+ * admitted requests are packed, whole or in pieces, into batches of up to
+ * max_batch_rows rows; requests that arrive while batches are on the device
+ * join later batches; up to n_streams batches are on the device at once.
+ * Each result is bitwise what tc_predict gives for the request alone.  The
+ * host copies request rows into the server's pinned staging (uncached, read
+ * by the kernel in place) and the outputs back. */
 typedef struct {
-  uint32_t max_batch_rows;   /* rows per coalesced launch */
+  uint32_t max_batch_rows;   /* rows per batch, <= 2^24 */
   uint64_t memory_budget;    /* bytes of request data admitted at once */
-  uint32_t n_streams;        /* batches in flight */
-  int32_t transform;         /* 0 raw, 1 sigmoid */
+  uint32_t n_streams;        /* batches on the device at once, 1..8 */
+  int32_t transform;         /* 0 raw, 1 sigmoid (softmax for K > 1) */
+  uint64_t fail_batch;       /* fault injection for tests: batch fail_batch - 1 is failed by the
+                                host as if the device had reported an error (0: none) */
 } tc_serve_params;
+typedef struct {
+  uint64_t batches, rows;
+  uint64_t peak_admitted_bytes;    /* most request bytes admitted at once */
+  uint64_t max_batches_in_flight;
+  uint64_t cancelled, failed;      /* requests */
+} tc_serve_counters_t;
 int32_t tc_serve_start(tc_ensemble* ens, const tc_serve_params* p, tc_server** out);
-/* Submit n rows; out must stay valid until tc_serve_wait returns for ticket. */
+/* Submit n rows; X and out must stay valid until tc_serve_wait returns for
+ * ticket (out: n x K floats). */
 int32_t tc_serve_submit(tc_server* s, const float* X, uint32_t n, float* out, uint64_t* ticket);
+/* TC_OK, TC_ETIMEOUT (the request stays live), TC_ESTATE (cancelled) or
+ * TC_EDEVICE (the server failed; every unfinished request fails). */
 int32_t tc_serve_wait(tc_server* s, uint64_t ticket, int64_t timeout_us);
+/* Cancels a request none of whose rows is on the device yet (TC_EBUSY
+ * otherwise); its wait then returns TC_ESTATE. */
+int32_t tc_serve_cancel(tc_server* s, uint64_t ticket);
 int32_t tc_serve_stats(tc_server* s, uint64_t* batches, uint64_t* rows);
+int32_t tc_serve_counters(tc_server* s, tc_serve_counters_t* out);
+/* Drains what was submitted (unless the server failed), then stops. */
 int32_t tc_serve_stop(tc_server* s);
+/* Pinned host memory: tc_predict transfers rows from it to the device
+ * without a staging copy. */
+int32_t tc_host_alloc(size_t bytes, void** out);
+int32_t tc_host_free(void* p);
 
 #ifdef __cplusplus
 }

@@ -164,3 +164,133 @@ def test_serving_alone_and_in_the_mix(ctx, oracle):
     assert batches < len(reqs)
     for a, b in zip(outs, alone):
         assert np.array_equal(a, b)
+
+
+def test_serving_c3_shape_alone_and_in_the_mix(ctx):
+    """The serving workload's shape: 1000 trees of depth 8 over 200 features,
+    requests of 1..65536 rows (log-uniform) submitted from four threads into
+    batches of 32768 rows, half of them in device-reachable pinned memory
+    (read and written in place), half in ordinary memory (staged).  Every
+    result equals the request predicted alone, bit for bit; admitted bytes
+    stay within the budget unless one request alone exceeds it."""
+    from turbocat.capi import HostBuffer, Server
+    F, T, D = 200, 1000, 8
+    m = random_model(T, D, F, seed=31)
+    ens = ctx.ensemble(m, F)
+    rng = np.random.default_rng(32)
+    sizes = [int(s) for s in np.floor(np.exp(rng.uniform(0, np.log(65536), 48)))]
+    sizes[:3] = [65536, 1, 7]
+    bufs, reqs, outs = [], [], []
+    for i, n in enumerate(sizes):
+        x = rng.standard_normal((n, F), dtype=np.float32)
+        x[::97, 3] = np.nan
+        if i % 2 == 0:
+            hb, ho = HostBuffer((n, F)), HostBuffer((n,))
+            hb.array[:] = x
+            bufs += [hb, ho]
+            reqs.append(hb.array)
+            outs.append(ho.array)
+        else:
+            reqs.append(x)
+            outs.append(np.zeros(n, np.float32))
+    alone = [ens.predict(x, transform=1) for x in reqs]
+    budget = 64 << 20
+    srv = Server(ens, max_batch_rows=32768, memory_budget=budget, n_streams=4, transform=1)
+    try:
+        tickets = [None] * len(reqs)
+
+        def client(k):
+            for i in range(k, len(reqs), 4):
+                tickets[i] = srv.submit(reqs[i], outs[i])
+
+        th = [threading.Thread(target=client, args=(k,)) for k in range(4)]
+        [t.start() for t in th]
+        [t.join() for t in th]
+        for t in tickets:
+            assert srv.wait(t, timeout_us=120_000_000)
+        c = srv.counters()
+    finally:
+        srv.stop()
+    row = 4 * F + 4
+    assert c["rows"] == sum(sizes) and c["batches"] < len(reqs)
+    assert c["peak_admitted_bytes"] <= max(budget, max(sizes) * row)
+    assert c["max_batches_in_flight"] >= 2 and c["failed"] == 0
+    for a, b in zip(outs, alone):
+        assert np.array_equal(a, b)
+    for b in bufs:
+        b.free()
+
+
+def test_serving_multiclass(ctx, oracle):
+    """K-channel ensembles are served too: softmax outputs (n x K) of every
+    request equal the request predicted alone."""
+    from turbocat.capi import Server
+    rng = np.random.default_rng(41)
+    T, D, K, F = 60, 6, 4, 17
+    depth = np.full(T, D, np.uint16)
+    m = dict(T=T, D=D, K=K, feature=rng.integers(0, F, T * D).astype(np.uint16),
+             fthr=rng.standard_normal(T * D).astype(np.float32), bin=np.zeros(T * D, np.uint8),
+             depth=depth, leaves=(rng.standard_normal((T << D) * K) * 0.1).astype(np.float32),
+             weight=np.full(T, 0.05, np.float32), base=0.0)
+    ens = ctx.ensemble(m, F)
+    reqs = [rng.standard_normal((int(s), F), dtype=np.float32)
+            for s in np.exp(rng.uniform(0, np.log(2000), 40))]
+    alone = [ens.predict(x, transform=1) for x in reqs]
+    srv = Server(ens, max_batch_rows=1024, memory_budget=1 << 24, n_streams=3, transform=1)
+    try:
+        outs = [np.zeros((len(x), K), np.float32) for x in reqs]
+        ts = [srv.submit(x, o) for x, o in zip(reqs, outs)]
+        for t in ts:
+            assert srv.wait(t, timeout_us=60_000_000)
+    finally:
+        srv.stop()
+    for a, b in zip(outs, alone):
+        assert np.array_equal(a, b)
+    ref, _ = oracle.predict_mc_f64(m, reqs[0])
+    e = np.exp(ref - ref.max(axis=1, keepdims=True))
+    assert np.allclose(outs[0], e / e.sum(axis=1, keepdims=True), rtol=1e-5, atol=1e-6)
+
+
+def test_serving_oversize_request_and_injected_failure(ctx):
+    """A request larger than the memory budget is admitted alone and served;
+    a batch the host fails on purpose (fail_batch) ends the waits of its
+    requests with an error, later submissions are refused, stop returns."""
+    from turbocat.capi import Server, TurboCatError
+    F = 32
+    m = random_model(40, 6, F, seed=51)
+    ens = ctx.ensemble(m, F)
+    rng = np.random.default_rng(52)
+    big = rng.standard_normal((30000, F), dtype=np.float32)
+    small = rng.standard_normal((100, F), dtype=np.float32)
+    srv = Server(ens, max_batch_rows=8192, memory_budget=1 << 20, n_streams=2)
+    try:
+        ob, os_ = np.zeros(30000, np.float32), np.zeros(100, np.float32)
+        t1, t2 = srv.submit(big, ob), srv.submit(small, os_)
+        assert srv.wait(t1, 60_000_000) and srv.wait(t2, 60_000_000)
+        c = srv.counters()
+    finally:
+        srv.stop()
+    assert c["peak_admitted_bytes"] == 30000 * (4 * F + 4)
+    assert np.array_equal(ob, ens.predict(big)) and np.array_equal(os_, ens.predict(small))
+    reqs = [rng.standard_normal((500, F), dtype=np.float32) for _ in range(10)]
+    srv = Server(ens, max_batch_rows=500, memory_budget=1 << 24, n_streams=2, fail_batch=3)
+    try:
+        outs = [np.zeros(500, np.float32) for _ in reqs]
+        ts, refused = [], 0
+        for x, o in zip(reqs, outs):
+            try:
+                ts.append(srv.submit(x, o))
+            except TurboCatError as e:
+                assert e.code == -4
+                refused += 1
+        failed = 0
+        for i, t in enumerate(ts):
+            try:
+                srv.wait(t, 60_000_000)
+                assert np.array_equal(outs[i], ens.predict(reqs[i]))
+            except TurboCatError as e:
+                assert e.code == -3 and "injected failure" in str(e)
+                failed += 1
+        assert failed >= 1
+    finally:
+        srv.stop()

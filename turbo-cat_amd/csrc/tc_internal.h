@@ -33,12 +33,15 @@ struct tc_ensemble {
   uint32_t K = 1;               // output channels (classes of a softmax ensemble)
   float base = 0.f;
   bool any_split = false;       // some tree has depth > 0
-  uint16_t* feat = nullptr;     // [T][D]
-  float* thr = nullptr;         // [T][D]
-  uint8_t* bin = nullptr;       // [T][D]
-  uint16_t* depth = nullptr;    // [T]
+  int dmax = 8;                 // depth bound of the tree records (6, 8 or 16)
+  uint32_t* rec = nullptr;      // [T] tree records (tc_predict_kernels.h, TreeRec)
   float* leaves = nullptr;      // [T][2^D][K]
   float* weight = nullptr;      // [T]
+  // buffers of the host-pointer entry points, grown on demand
+  float* d_buf = nullptr;
+  size_t d_cap = 0;
+  float* h_buf = nullptr;       // pinned
+  size_t h_cap = 0;
 };
 
 namespace tc {

@@ -1,272 +1,153 @@
-// A15-A17: symmetric-ensemble inference on gfx950, and the continuous-
-// batching server built on it.
-//
-// Kernel layout: a workgroup takes R rows (R = 64 for F <= 620) and stages
-// them in LDS with an odd row stride (conflict-free column reads).  Its 256
-// threads are (q, r) = (tree lane, row): in each pass tree lane q evaluates
-// one group of four trees for row r -- leaf index built MSB-first from depth
-// comparisons (v >= threshold, NaN left; fast_float_ensemble.hpp:122-137)
-// -- and the group value w0*l0 + w1*l1 + w2*l2 + w3*l3 (as the reference
-// compiler contracts it: w1*l1 rounded, the other three fused onto it).  Then lane 0 of the row adds
-// the groups to the row's sum in tree order, exactly the reference's
-// 8-row-block order (fast_float_ensemble.hpp:480-541): groups added one by
-// one, leftover trees fused in one by one (sum = fma(w, leaf, sum)).
-//
-// The reference sums some rows tree by tree instead, sum = fma(w_t, l_t, sum)
-// for every tree in order: the rows of a call past its last full 8-row block
-// (fast_float_ensemble.hpp:279-287, :543-560; fast_ensemble.hpp:428-446),
-// every row of a binned call under 16 rows (symmetric_tree.cpp:954-961) and
-// every row when no tree has a split (the fast ensembles are then empty,
-// symmetric_tree.cpp:1144-1148).  Such a row is marked by seq_flag[row] (a
-// served batch, whose rows come from several requests) or by row >= seq_from
-// (one call); lane 0 then fuses the group's four leaf values one by one.  So a
-// row's value depends only on the row, the model and its place in its own
-// request, never on the batch the request was coalesced into.
+// A15-A18: symmetric-ensemble inference on gfx950 (kernels in
+// tc_predict_kernels.h), the plain entry points, and the continuous-batching
+// server: the host logic of tc_serve_core.h over a device backend that is one
+// persistent kernel (k_serve) reading batches published in coherent pinned
+// host memory.
 #include <algorithm>
 #include <atomic>
-#include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
-#include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
-#include <unordered_map>
 
-#include "tc_device.h"
 #include "tc_internal.h"
+#include "tc_predict_kernels.h"
+#include "tc_serve_core.h"
 
 namespace tc {
 
-struct EnsDev {
-  const uint16_t* feat;
-  const float* thr;
-  const uint8_t* bin;
-  const uint16_t* depth;
-  const float* leaves;
-  const float* weight;
-  uint32_t T, D, K;
-  float base;
+// ---------------------------------------------------------------------------
+// Host copies split over a few threads (the caller works too): one thread's
+// memcpy bandwidth, not PCIe, would otherwise bound host-pointer prediction
+// and the server's staging.
+class CopyPool {
+ public:
+  using Piece = std::tuple<void*, const void*, size_t>;
+  explicit CopyPool(int helpers) {
+    for (int i = 0; i < helpers; ++i) th_.emplace_back([this] { run(); });
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  // Copies every piece; returns when all are done.  One caller at a time.
+  void copy(const std::vector<Piece>& pieces) {
+    if (pieces.empty()) return;
+    std::lock_guard<std::mutex> one(call_mu_);
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      jobs_ = &pieces;
+      next_ = 0;
+      left_ = pieces.size();
+    }
+    cv_.notify_all();
+    work();
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [this] { return left_ == 0; });
+    jobs_ = nullptr;
+  }
+
+ private:
+  void work() {
+    while (true) {
+      const Piece* pc;
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (!jobs_ || next_ >= jobs_->size()) return;
+        pc = &(*jobs_)[next_++];
+      }
+      std::memcpy(std::get<0>(*pc), std::get<1>(*pc), std::get<2>(*pc));
+      std::lock_guard<std::mutex> lk(mu_);
+      if (--left_ == 0) done_.notify_all();
+    }
+  }
+  void run() {
+    std::unique_lock<std::mutex> lk(mu_);
+    while (true) {
+      cv_.wait(lk, [this] { return stop_ || (jobs_ && next_ < jobs_->size()); });
+      if (stop_) return;
+      lk.unlock();
+      work();
+      lk.lock();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_, call_mu_;
+  std::condition_variable cv_, done_;
+  const std::vector<Piece>* jobs_ = nullptr;
+  size_t next_ = 0, left_ = 0;
+  bool stop_ = false;
 };
 
-template <bool BINNED>
-__global__ void __launch_bounds__(256) k_predict(EnsDev E, const void* __restrict__ Xv, uint32_t n,
-                                                 uint32_t F, uint32_t ldc, uint32_t R,
-                                                 int transform, float* __restrict__ out,
-                                                 uint32_t* __restrict__ leaf_idx, uint32_t seq_from,
-                                                 const uint8_t* __restrict__ seq_flag) {
-  extern __shared__ unsigned char s_raw[];
-  const uint32_t Q = 256u / R;                 // tree lanes per row
-  const uint32_t S = BINNED ? (F | 3u) + 1u : (F | 1u);  // row stride in LDS elements
-  float* s_gv = reinterpret_cast<float*>(s_raw);           // [Q][R] group values
-  float* s_lv = s_gv + 256;                                // [Q][4][R] leaf values (tree-by-tree rows)
-  unsigned char* s_x = s_raw + sizeof(float) * 256 * 5;
-  const uint32_t tid = threadIdx.x, q = tid / R, r = tid % R;
-  const uint32_t row0 = blockIdx.x * R;
-  const uint32_t nr = min(R, n - row0);
-  // stage the rows
-  if (BINNED) {
-    const uint8_t* B = static_cast<const uint8_t*>(Xv);  // column-major [F][ldc]
-    for (uint32_t i = tid; i < nr * F; i += 256) {
-      const uint32_t f = i / nr, rr = i % nr;
-      s_x[rr * S + f] = B[static_cast<size_t>(f) * ldc + row0 + rr];
-    }
-  } else {
-    const float* X = static_cast<const float*>(Xv);
-    float* sx = reinterpret_cast<float*>(s_x);
-    const float* src = X + static_cast<size_t>(row0) * F;
-    for (uint32_t i = tid; i < nr * F; i += 256) sx[(i / F) * S + (i % F)] = src[i];
-  }
-  __syncthreads();
-  const uint32_t LP = 1u << E.D;
-  const uint32_t groups = E.T / 4u;
-  const bool seq = r < nr && (seq_flag ? seq_flag[row0 + r] != 0 : row0 + r >= seq_from);
-  float sum = 0.f;
-  for (uint32_t g0 = 0; g0 < groups; g0 += Q) {
-    const uint32_t g = g0 + q;
-    if (g < groups && r < nr) {
-      float lv[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const uint32_t t = 4 * g + k;
-        const uint32_t dep = E.depth[t];
-        uint32_t li = 0;
-        for (uint32_t d = 0; d < dep; ++d) {
-          const uint32_t f = E.feat[t * E.D + d];
-          uint32_t right;
-          if (BINNED) {
-            const uint8_t v = s_x[r * S + f];
-            right = (v != 255 && v > E.bin[t * E.D + d]) ? 1u : 0u;
-          } else {
-            const float v = reinterpret_cast<const float*>(s_x)[r * S + f];
-            right = (!isnan(v) && v >= E.thr[t * E.D + d]) ? 1u : 0u;
-          }
-          li = (li << 1) | right;
-        }
-        if (leaf_idx) leaf_idx[static_cast<size_t>(row0 + r) * E.T + t] = li;
-        lv[k] = E.leaves[static_cast<size_t>(t) * LP + li];
-      }
-      if (seq) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) s_lv[(q * 4 + k) * R + r] = lv[k];
-      } else {
-        // the reference compiler's contraction of w0*l0 + w1*l1 + w2*l2 + w3*l3:
-        // w1*l1 rounded, then w0*l0, w2*l2, w3*l3 fused onto it in that order
-        const float* w = E.weight + 4 * g;
-        float gv = __fmul_rn(w[1], lv[1]);
-        gv = __fmaf_rn(w[0], lv[0], gv);
-        gv = __fmaf_rn(w[2], lv[2], gv);
-        gv = __fmaf_rn(w[3], lv[3], gv);
-        s_gv[q * R + r] = gv;
-      }
-    }
-    __syncthreads();
-    if (q == 0 && r < nr) {
-      const uint32_t ng = min(Q, groups - g0);
-      if (seq) {
-        for (uint32_t k = 0; k < ng; ++k)
-          for (uint32_t j = 0; j < 4; ++j)
-            sum = __fmaf_rn(E.weight[4 * (g0 + k) + j], s_lv[(k * 4 + j) * R + r], sum);
-      } else {
-        for (uint32_t k = 0; k < ng; ++k) sum = __fadd_rn(sum, s_gv[k * R + r]);
-      }
-    }
-    __syncthreads();
-  }
-  if (q == 0 && r < nr) {
-    for (uint32_t t = groups * 4u; t < E.T; ++t) {
-      const uint32_t dep = E.depth[t];
-      uint32_t li = 0;
-      for (uint32_t d = 0; d < dep; ++d) {
-        const uint32_t f = E.feat[t * E.D + d];
-        uint32_t right;
-        if (BINNED) {
-          const uint8_t v = s_x[r * S + f];
-          right = (v != 255 && v > E.bin[t * E.D + d]) ? 1u : 0u;
-        } else {
-          const float v = reinterpret_cast<const float*>(s_x)[r * S + f];
-          right = (!isnan(v) && v >= E.thr[t * E.D + d]) ? 1u : 0u;
-        }
-        li = (li << 1) | right;
-      }
-      if (leaf_idx) leaf_idx[static_cast<size_t>(row0 + r) * E.T + t] = li;
-      sum = __fmaf_rn(E.weight[t], E.leaves[static_cast<size_t>(t) * LP + li], sum);
-    }
-    float v = __fadd_rn(sum, E.base);
-    if (transform == 1) v = __fdiv_rn(1.0f, __fadd_rn(1.0f, expf(-v)));
-    out[row0 + r] = v;
+static CopyPool& copy_pool() {
+  static CopyPool pool(7);
+  return pool;
+}
+
+static std::vector<CopyPool::Piece> split_copy(void* dst, const void* src, size_t bytes) {
+  constexpr size_t kPiece = size_t(1) << 20;
+  std::vector<CopyPool::Piece> v;
+  for (size_t o = 0; o < bytes; o += kPiece)
+    v.emplace_back(static_cast<char*>(dst) + o, static_cast<const char*>(src) + o,
+                   std::min(kPiece, bytes - o));
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Pinned host memory handed out by tc_host_alloc: rows in it are transferred
+// to the device without a staging copy.
+static std::mutex g_host_mu;
+static std::map<uintptr_t, size_t> g_host_ranges;  // start -> bytes
+
+static bool host_reachable(const void* p, size_t bytes) {
+  const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  std::lock_guard<std::mutex> lk(g_host_mu);
+  auto it = g_host_ranges.upper_bound(a);
+  if (it == g_host_ranges.begin()) return false;
+  --it;
+  return a >= it->first && a + bytes <= it->first + it->second;
+}
+
+// The kernels address pinned host memory by its host address; refuse memory
+// whose device address differs (rather than let a kernel read elsewhere).
+static void check_same_address(void* p) {
+  void* d = nullptr;
+  TC_HIP(hipHostGetDevicePointer(&d, p, 0));
+  if (d != p) {
+    (void)hipHostFree(p);
+    throw Error(TC_EDEVICE, "pinned host memory has a device address other than its host address");
   }
 }
 
-// A18 inference: the layout of k_predict, with K outputs per row.  Tree
-// lane q of row r sums w_t * leaf_t[idx][c] over its trees t = q, q + Q, ...
-// (fused multiply-adds, tree order) into K partial sums; lane 0 then adds
-// base + the Q partials in lane order, and optionally applies softmax (max
-// shift, exp, class-order sum, divide: CrossEntropyLoss::softmax,
-// loss.cpp:273-285).  As in k_predict, a row's value depends only on the
-// row and the model.
-template <bool BINNED>
-__global__ void __launch_bounds__(256) k_predict_multi(EnsDev E, const void* __restrict__ Xv,
-                                                       uint32_t n, uint32_t F, uint32_t ldc,
-                                                       uint32_t R, int transform,
-                                                       float* __restrict__ out,
-                                                       uint32_t* __restrict__ leaf_idx) {
-  extern __shared__ unsigned char s_raw[];
-  const uint32_t Q = 256u / R, K = E.K;
-  const uint32_t S = BINNED ? (F | 3u) + 1u : (F | 1u);
-  float* s_acc = reinterpret_cast<float*>(s_raw);  // [Q][R][K] partial sums
-  unsigned char* s_x = s_raw + sizeof(float) * 256 * K;
-  const uint32_t tid = threadIdx.x, q = tid / R, r = tid % R;
-  const uint32_t row0 = blockIdx.x * R;
-  const uint32_t nr = min(R, n - row0);
-  if (BINNED) {
-    const uint8_t* B = static_cast<const uint8_t*>(Xv);  // column-major [F][ldc]
-    for (uint32_t i = tid; i < nr * F; i += 256) {
-      const uint32_t f = i / nr, rr = i % nr;
-      s_x[rr * S + f] = B[static_cast<size_t>(f) * ldc + row0 + rr];
-    }
-  } else {
-    const float* X = static_cast<const float*>(Xv);
-    float* sx = reinterpret_cast<float*>(s_x);
-    const float* src = X + static_cast<size_t>(row0) * F;
-    for (uint32_t i = tid; i < nr * F; i += 256) sx[(i / F) * S + (i % F)] = src[i];
+// Every running serving kernel's host stop word: at process exit they are all
+// raised, so that no persistent kernel outlives a process that did not stop
+// its servers (the runtime's teardown would wait for it).
+static std::mutex g_serve_mu;
+static std::vector<uint32_t*> g_serve_stops;
+static void stop_all_servers() {
+  std::lock_guard<std::mutex> lk(g_serve_mu);
+  for (uint32_t* s : g_serve_stops) __atomic_store_n(s, 1u, __ATOMIC_RELEASE);
+}
+static void register_server(uint32_t* stop_word) {
+  static bool hooked = false;
+  std::lock_guard<std::mutex> lk(g_serve_mu);
+  if (!hooked) {
+    std::atexit(stop_all_servers);
+    hooked = true;
   }
-  __syncthreads();
-  const uint32_t LP = 1u << E.D;
-  float acc[kMaxClasses];
-#pragma unroll
-  for (int c = 0; c < kMaxClasses; ++c) acc[c] = 0.f;
-  if (r < nr) {
-    for (uint32_t t = q; t < E.T; t += Q) {
-      const uint32_t dep = E.depth[t];
-      uint32_t li = 0;
-      for (uint32_t d = 0; d < dep; ++d) {
-        const uint32_t f = E.feat[t * E.D + d];
-        uint32_t right;
-        if (BINNED) {
-          const uint8_t v = s_x[r * S + f];
-          right = (v != 255 && v > E.bin[t * E.D + d]) ? 1u : 0u;
-        } else {
-          const float v = reinterpret_cast<const float*>(s_x)[r * S + f];
-          right = (!isnan(v) && v >= E.thr[t * E.D + d]) ? 1u : 0u;
-        }
-        li = (li << 1) | right;
-      }
-      if (leaf_idx) leaf_idx[static_cast<size_t>(row0 + r) * E.T + t] = li;
-      const float w = E.weight[t];
-      const float* lv = E.leaves + (static_cast<size_t>(t) * LP + li) * K;
-#pragma unroll
-      for (uint32_t c = 0; c < kMaxClasses; ++c) {
-        if (c >= K) break;
-        acc[c] = __fmaf_rn(w, lv[c], acc[c]);
-      }
-    }
-  }
-#pragma unroll
-  for (uint32_t c = 0; c < kMaxClasses; ++c) {
-    if (c >= K) break;
-    s_acc[(q * R + r) * K + c] = acc[c];
-  }
-  __syncthreads();
-  if (q == 0 && r < nr) {
-    float v[kMaxClasses];
-#pragma unroll
-    for (uint32_t c = 0; c < kMaxClasses; ++c) v[c] = 0.f;
-    for (uint32_t qq = 0; qq < Q; ++qq) {  // lane partials in lane order
-#pragma unroll
-      for (uint32_t c = 0; c < kMaxClasses; ++c) {
-        if (c >= K) break;
-        v[c] = __fadd_rn(v[c], s_acc[(qq * R + r) * K + c]);
-      }
-    }
-    float m = -INFINITY;
-#pragma unroll
-    for (uint32_t c = 0; c < kMaxClasses; ++c) {
-      if (c >= K) break;
-      v[c] = __fadd_rn(E.base, v[c]);
-      m = fmaxf(m, v[c]);
-    }
-    if (transform == 1) {
-      float z = 0.f;
-#pragma unroll
-      for (uint32_t c = 0; c < kMaxClasses; ++c) {
-        if (c >= K) break;
-        v[c] = expf(__fsub_rn(v[c], m));
-        z = __fadd_rn(z, v[c]);
-      }
-#pragma unroll
-      for (uint32_t c = 0; c < kMaxClasses; ++c) {
-        if (c >= K) break;
-        v[c] = __fdiv_rn(v[c], z);
-      }
-    }
-    float* o = out + static_cast<size_t>(row0 + r) * K;
-#pragma unroll
-    for (uint32_t c = 0; c < kMaxClasses; ++c) {
-      if (c >= K) break;
-      o[c] = v[c];
-    }
-  }
+  g_serve_stops.push_back(stop_word);
+}
+static void unregister_server(uint32_t* stop_word) {
+  std::lock_guard<std::mutex> lk(g_serve_mu);
+  g_serve_stops.erase(std::remove(g_serve_stops.begin(), g_serve_stops.end(), stop_word),
+                      g_serve_stops.end());
 }
 
 // ---------------------------------------------------------------------------
@@ -302,299 +183,307 @@ static unsigned char* scratch_of(int device) {
   return static_cast<unsigned char*>(g_scratch_dev[device]);
 }
 
-EnsDev ens_dev(const tc_ensemble* e) {
-  EnsDev E;
-  E.feat = e->feat; E.thr = e->thr; E.bin = e->bin; E.depth = e->depth; E.leaves = e->leaves;
-  E.weight = e->weight; E.T = e->T; E.D = e->D; E.K = e->K; E.base = e->base;
-  return E;
-}
+// ---------------------------------------------------------------------------
+// Ensemble model and launch shapes.
 
-// Rows per workgroup and LDS bytes for a feature count and output channels.
-void predict_shape(uint32_t F, uint32_t K, bool binned, uint32_t* R, size_t* lds) {
-  const uint32_t S = binned ? (F | 3u) + 1u : (F | 1u);
-  const size_t elem = binned ? 1 : 4;
-  // K = 1: group values and the tree-by-tree rows' leaf values (5 x 256 floats)
-  const size_t head = sizeof(float) * 256 * (K > 1 ? K : 5u);
-  for (uint32_t r : {64u, 32u, 16u, 8u, 4u, 2u, 1u}) {
-    const size_t bytes = head + static_cast<size_t>(r) * S * elem;
-    if (bytes <= 160 * 1024 || r == 1) {
-      *R = r;
-      *lds = bytes;
-      return;
+static int dmax_of(uint32_t D) { return D <= 6 ? 6 : (D <= 8 ? 8 : 16); }
+
+template <int DMAX>
+static void pack_records(uint32_t T, uint32_t D, const uint16_t* feature, const float* fthr,
+                         const uint8_t* bin, const uint16_t* depth, std::vector<uint32_t>& rec) {
+  using L = TreeRec<DMAX>;
+  rec.assign(static_cast<size_t>(std::max(1u, T)) * L::kDwords, 0u);
+  for (uint32_t t = 0; t < T; ++t) {
+    uint32_t* r = rec.data() + static_cast<size_t>(t) * L::kDwords;
+    const uint32_t dep = depth[t];
+    for (uint32_t d = 0; d < dep; ++d) {
+      const size_t k = static_cast<size_t>(t) * D + d;
+      uint32_t fb;
+      std::memcpy(&fb, fthr + k, 4);
+      r[L::kThr + d] = fb;
+      r[L::kFeat + d / 2] |= static_cast<uint32_t>(feature[k]) << (16 * (d & 1));
+      r[L::kBin + d / 4] |= static_cast<uint32_t>(bin ? bin[k] : 0u) << (8 * (d & 3));
     }
+    r[L::kDepth] = dep;
   }
 }
 
-// First row of a call of n rows that the reference sums tree by tree (see
-// k_predict): past the last full 8-row block; every row of a binned call
-// under 16 rows, or of an ensemble without splits.
+EnsDev ens_dev(const tc_ensemble* e) {
+  EnsDev E;
+  E.rec = e->rec; E.leaves = e->leaves; E.weight = e->weight;
+  E.T = e->T; E.D = e->D; E.K = e->K; E.base = e->base;
+  return E;
+}
+
+// Rows per tile and LDS bytes: as many rows (<= 64) as keep three
+// workgroups per CU, else two, else one.
+void predict_shape(uint32_t F, uint32_t K, bool binned, uint32_t* R, size_t* lds) {
+  const bool multi = K > 1;
+  const size_t head = pred_head_bytes(multi, K);
+  const size_t row = static_cast<size_t>(pred_stride(F, binned)) * (binned ? 1 : 4);
+  constexpr size_t kLds = 160 * 1024, kStatic = 2048;
+  for (uint32_t per : {3u, 2u, 1u}) {
+    const size_t room = kLds / per - kStatic;
+    if (room <= head + row) continue;
+    const size_t r = std::min<size_t>(kPredRowsMax, (room - head) / row);
+    if (r >= 16 || per == 1) {
+      *R = static_cast<uint32_t>(r);
+      *lds = head + r * row;
+      return;
+    }
+  }
+  throw Error(TC_EINVAL, "too many features for one LDS row tile");
+}
+
+// First row of a call of n rows that the reference sums tree by tree: past
+// the last full 8-row block; every row of a binned call under 16 rows, or of
+// an ensemble without splits.
 uint32_t seq_rows_from(const tc_ensemble* e, bool binned, uint32_t n) {
   if (!e->any_split || (binned && n < 16)) return 0u;
   return n / 8u * 8u;
 }
 
-void launch_predict(const tc_ensemble* e, const void* X, bool binned, uint32_t n, uint32_t F,
-                    uint32_t ldc, int transform, float* out, uint32_t* leaf_idx, hipStream_t s,
-                    const uint8_t* seq_flag = nullptr) {
-  if (n == 0) return;
-  const uint32_t seq_from = seq_rows_from(e, binned, n);
+template <int DMAX>
+static void launch_dmax(const tc_ensemble* e, const void* X, bool binned, uint32_t n, uint32_t F,
+                        uint32_t ldc, int transform, float* out, hipStream_t s) {
   uint32_t R;
   size_t lds;
   predict_shape(F, e->K, binned, &R, &lds);
-  TC_REQUIRE(lds <= 160 * 1024, "too many features for one LDS row tile");
   const dim3 grid((n + R - 1) / R);
-  if (e->K > 1) {
-    auto kern = binned ? &k_predict_multi<true> : &k_predict_multi<false>;
+  const uint32_t from = seq_rows_from(e, binned, n);
+  auto pick = [&](auto kern) {
     if (lds > 64 * 1024)
       TC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, ens_dev(e), X, n, F, ldc, R, transform, out,
-                       leaf_idx);
-  } else if (binned) {
-    if (lds > 64 * 1024)
-      TC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_predict<true>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    hipLaunchKernelGGL(k_predict<true>, grid, dim3(256), lds, s, ens_dev(e), X, n, F, ldc, R,
-                       transform, out, leaf_idx, seq_from, seq_flag);
+    hipLaunchKernelGGL(kern, grid, dim3(kPredThreads), lds, s, ens_dev(e), X, n, F, ldc, R,
+                       transform, out, from);
+  };
+  if (e->K > 1) {
+    if (binned) pick(&k_predict<true, DMAX, true>);
+    else pick(&k_predict<false, DMAX, true>);
   } else {
-    if (lds > 64 * 1024)
-      TC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_predict<false>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    hipLaunchKernelGGL(k_predict<false>, grid, dim3(256), lds, s, ens_dev(e), X, n, F, ldc, R,
-                       transform, out, leaf_idx, seq_from, seq_flag);
+    if (binned) pick(&k_predict<true, DMAX, false>);
+    else pick(&k_predict<false, DMAX, false>);
   }
   TC_HIP(hipGetLastError());
 }
 
+void launch_predict(const tc_ensemble* e, const void* X, bool binned, uint32_t n, uint32_t F,
+                    uint32_t ldc, int transform, float* out, hipStream_t s) {
+  if (n == 0) return;
+  switch (e->dmax) {
+    case 6: launch_dmax<6>(e, X, binned, n, F, ldc, transform, out, s); break;
+    case 8: launch_dmax<8>(e, X, binned, n, F, ldc, transform, out, s); break;
+    default: launch_dmax<16>(e, X, binned, n, F, ldc, transform, out, s); break;
+  }
+}
+
+static void launch_leaf_index(const tc_ensemble* e, const float* X, uint32_t n, uint32_t F,
+                              uint32_t* li, hipStream_t s) {
+  const dim3 grid((n + 255) / 256);
+  switch (e->dmax) {
+    case 6: hipLaunchKernelGGL(k_leaf_index<6>, grid, dim3(256), 0, s, ens_dev(e), X, n, F, li); break;
+    case 8: hipLaunchKernelGGL(k_leaf_index<8>, grid, dim3(256), 0, s, ens_dev(e), X, n, F, li); break;
+    default: hipLaunchKernelGGL(k_leaf_index<16>, grid, dim3(256), 0, s, ens_dev(e), X, n, F, li); break;
+  }
+  TC_HIP(hipGetLastError());
+}
+
+// Device buffer and pinned staging of the host-pointer entry points, kept
+// with the ensemble and grown on demand (one host thread per context).
+static float* ens_device_buf(tc_ensemble* e, size_t floats) {
+  if (floats > e->d_cap) {
+    dfree(e->d_buf);
+    e->d_buf = nullptr;
+    e->d_cap = 0;
+    e->d_buf = dmalloc<float>(floats);
+    e->d_cap = floats;
+  }
+  return e->d_buf;
+}
+static float* ens_host_buf(tc_ensemble* e, size_t floats) {
+  if (floats > e->h_cap) {
+    if (e->h_buf) (void)hipHostFree(e->h_buf);
+    e->h_buf = nullptr;
+    e->h_cap = 0;
+    TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&e->h_buf), floats * 4, hipHostMallocDefault));
+    e->h_cap = floats;
+  }
+  return e->h_buf;
+}
+
 // ---------------------------------------------------------------------------
-// Continuous-batching server.
-//
-// Requests enter a queue; a scheduler thread admits them while the rows of
-// admitted-but-unfinished requests stay inside the memory budget, and packs
-// the head of the admitted queue (whole or partial requests) into the next
-// free batch slot.  Each slot owns a stream, pinned staging and device
-// buffers; its batch is copied in, predicted and copied out asynchronously,
-// so up to n_streams batches overlap on the device while new arrivals queue
-// for the next batch.  Completion scatters the slot's rows back into the
-// requests' output buffers.
-struct Request {
-  uint64_t id;
-  const float* X;
-  float* out;
-  uint32_t n;
-  uint32_t next = 0;       // next row to schedule
-  uint32_t done = 0;       // rows completed
-  bool finished = false;
-};
+// The server's device backend.  Everything the host writes and the device
+// reads (the descriptor ring, the staged rows and flags) and the outputs the
+// device writes live in uncached pinned host memory (never held in a GPU
+// cache, so the device cannot read a line older than the host's last write);
+// the persistent kernel indexes it by (slot, row) and checks every
+// host-written index before use.
+template <typename T>
+static T* uncached_alloc(size_t count) {
+  void* p = nullptr;
+  TC_HIP(hipHostMalloc(&p, std::max<size_t>(1, count) * sizeof(T),
+                       hipHostMallocUncached | hipHostMallocMapped));
+  check_same_address(p);
+  return static_cast<T*>(p);
+}
 
-struct Segment {
-  std::shared_ptr<Request> req;
-  uint32_t row0, count, slot_row;
-};
-
-// Host copies of one batch split over a few threads (the caller works too):
-// one thread's memcpy bandwidth, not PCIe, would otherwise bound the server.
-class CopyPool {
+class GpuServeBackend : public ServeBackend {
  public:
-  struct Piece {
-    void* dst;
-    const void* src;
-    size_t bytes;
-  };
-  explicit CopyPool(int helpers) {
-    for (int i = 0; i < helpers; ++i) th_.emplace_back([this] { run(); });
-  }
-  ~CopyPool() {
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      stop_ = true;
+  GpuServeBackend(tc_ensemble* e, const ServeConfig& c, int transform) : ens_(e), cfg_(c) {
+    try {
+      setup(transform);
+    } catch (...) {
+      release();
+      throw;
     }
-    cv_.notify_all();
-    for (auto& t : th_) t.join();
   }
-  // Copies every piece; returns when all are done.
-  void copy(const std::vector<Piece>& pieces) {
-    if (pieces.empty()) return;
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      jobs_ = &pieces;
-      next_ = 0;
-      left_ = pieces.size();
+  ~GpuServeBackend() override { release(); }
+
+  float* stage_x(uint32_t s) override {
+    return sx_ + static_cast<size_t>(s) * cfg_.max_batch_rows * cfg_.F;
+  }
+  uint8_t* stage_flag(uint32_t s) override {
+    return sf_ + static_cast<size_t>(s) * cfg_.max_batch_rows;
+  }
+  float* stage_out(uint32_t s) override {
+    return so_ + static_cast<size_t>(s) * cfg_.max_batch_rows * cfg_.K;
+  }
+
+  // A descriptor position is free once the batch kServeRing before it is done.
+  bool ready(uint64_t seq) override {
+    if (seq < kServeRing) return true;
+    return __atomic_load_n(&host_->done[seq % kServeRing], __ATOMIC_ACQUIRE) >= seq - kServeRing + 1;
+  }
+
+  void publish(uint32_t s, uint64_t seq, uint32_t rows) override {
+    ServeHostDesc& d = host_->ring[seq % kServeRing];
+    d.tile0 = tiles_;
+    d.slot = s;
+    d.rows = rows;
+    __atomic_store_n(&d.seq, seq, __ATOMIC_RELEASE);  // after the other fields
+    tiles_ += (rows + R_ - 1) / R_;
+  }
+
+  bool done(uint32_t, uint64_t seq) override {
+    return __atomic_load_n(&host_->done[seq % kServeRing], __ATOMIC_ACQUIRE) >= seq + 1;
+  }
+
+  std::string health() override {
+    const hipError_t q = hipEventQuery(end_);
+    if (q == hipErrorNotReady) return "";
+    if (q == hipSuccess) {
+      uint32_t fault = 0;
+      (void)hipMemcpy(&fault, &ctl_->fault, 4, hipMemcpyDeviceToHost);
+      if (fault & 2u) return "serving kernel stopped: a batch descriptor failed its checks";
+      if (fault & 1u) return "serving kernel stopped: no published batch held a claimed tile";
+      return "serving kernel ended before the server stopped";
     }
-    cv_.notify_all();
-    work();
-    std::unique_lock<std::mutex> lk(mu_);
-    done_.wait(lk, [this] { return left_ == 0; });
-    jobs_ = nullptr;
+    return std::string("serving kernel failed: ") + hipGetErrorString(q);
+  }
+
+  void heartbeat() override {
+    __atomic_store_n(&host_->heartbeat, ++beats_, __ATOMIC_RELEASE);
+  }
+
+  void copy(const std::vector<std::tuple<void*, const void*, size_t>>& pieces) override {
+    copy_pool().copy(pieces);
+  }
+
+  void shutdown() override {
+    if (!launched_) return;
+    __atomic_store_n(&host_->stop, 1u, __ATOMIC_RELEASE);
+    (void)hipStreamSynchronize(stream_);
+    unregister_server(&host_->stop);
+    launched_ = false;
   }
 
  private:
-  void work() {
-    while (true) {
-      const Piece* pc;
-      {
-        std::lock_guard<std::mutex> lk(mu_);
-        if (!jobs_ || next_ >= jobs_->size()) return;
-        pc = &(*jobs_)[next_++];
-      }
-      std::memcpy(pc->dst, pc->src, pc->bytes);
-      std::lock_guard<std::mutex> lk(mu_);
-      if (--left_ == 0) done_.notify_all();
+  void setup(int transform) {
+    TC_HIP(hipSetDevice(ens_->ctx->device));
+    const uint32_t F = cfg_.F, K = cfg_.K;
+    const size_t rows = static_cast<size_t>(cfg_.n_slots) * cfg_.max_batch_rows;
+    size_t lds;
+    predict_shape(F, K, false, &R_, &lds);
+    host_ = uncached_alloc<ServeHost>(1);
+    std::memset(static_cast<void*>(host_), 0, sizeof(ServeHost));
+    for (auto& d : host_->ring) d.seq = kServeNoBatch;
+    sx_ = uncached_alloc<float>(rows * F);
+    sf_ = uncached_alloc<uint8_t>(rows);
+    so_ = uncached_alloc<float>(rows * K);
+    ctl_ = dmalloc<ServeCtl>(1);
+    TC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    TC_HIP(hipEventCreateWithFlags(&end_, hipEventDisableTiming));
+    TC_HIP(hipMemsetAsync(ctl_, 0, sizeof(ServeCtl), stream_));
+    int cus = 0, rate_khz = 0;
+    TC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ens_->ctx->device));
+    TC_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ens_->ctx->device));
+    if (rate_khz <= 0) rate_khz = 100000;  // the gfx9 wall clock runs at 100 MHz
+    // one worker per CU: the rest of each CU stays free for other work
+    const uint32_t grid = 1u + static_cast<uint32_t>(std::max(1, cus));
+    const unsigned long long hb_ticks = 2000ull * static_cast<unsigned long long>(rate_khz);
+    const EnsDev E = ens_dev(ens_);
+    ServeStage st;
+    st.x = sx_;
+    st.flag = sf_;
+    st.out = so_;
+    st.n_slots = cfg_.n_slots;
+    st.max_rows = cfg_.max_batch_rows;
+    auto go = [&](auto kern) {
+      if (lds > 64 * 1024)
+        TC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(kPredThreads), lds, stream_, E, ctl_, host_, st, F,
+                         R_, transform, hb_ticks);
+    };
+    const bool multi = K > 1;
+    switch (ens_->dmax) {
+      case 6: multi ? go(&k_serve<6, true>) : go(&k_serve<6, false>); break;
+      case 8: multi ? go(&k_serve<8, true>) : go(&k_serve<8, false>); break;
+      default: multi ? go(&k_serve<16, true>) : go(&k_serve<16, false>); break;
     }
+    TC_HIP(hipGetLastError());
+    launched_ = true;
+    register_server(&host_->stop);
+    TC_HIP(hipEventRecord(end_, stream_));
   }
-  void run() {
-    std::unique_lock<std::mutex> lk(mu_);
-    while (true) {
-      cv_.wait(lk, [this] { return stop_ || (jobs_ && next_ < jobs_->size()); });
-      if (stop_) return;
-      lk.unlock();
-      work();
-      lk.lock();
-    }
-  }
-  std::vector<std::thread> th_;
-  std::mutex mu_;
-  std::condition_variable cv_, done_;
-  const std::vector<Piece>* jobs_ = nullptr;
-  size_t next_ = 0, left_ = 0;
-  bool stop_ = false;
-};
 
-struct Slot {
-  hipStream_t stream = nullptr;
-  hipEvent_t done = nullptr;
-  float* h_x = nullptr;    // pinned
-  float* h_out = nullptr;  // pinned
-  uint8_t* h_flag = nullptr;  // pinned: rows the reference sums tree by tree
-  float* d_x = nullptr;
-  float* d_out = nullptr;
-  uint8_t* d_flag = nullptr;
-  bool busy = false;
-  std::vector<Segment> segs;
+  void release() {
+    shutdown();
+    if (end_) (void)hipEventDestroy(end_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+    dfree(ctl_);
+    for (void* p : {static_cast<void*>(sx_), static_cast<void*>(sf_), static_cast<void*>(so_),
+                    static_cast<void*>(host_)})
+      if (p) (void)hipHostFree(p);
+    end_ = nullptr;
+    stream_ = nullptr;
+    ctl_ = nullptr;
+    host_ = nullptr;
+    sx_ = nullptr;
+    sf_ = nullptr;
+    so_ = nullptr;
+  }
+
+  tc_ensemble* ens_;
+  ServeConfig cfg_;
+  uint32_t R_ = 64;
+  ServeHost* host_ = nullptr;
+  ServeCtl* ctl_ = nullptr;
+  float* sx_ = nullptr;
+  uint8_t* sf_ = nullptr;
+  float* so_ = nullptr;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t end_ = nullptr;
+  bool launched_ = false;
+  unsigned long long tiles_ = 0, beats_ = 0;
 };
 
 }  // namespace tc
 
 struct tc_server {
-  tc_ensemble* ens = nullptr;
-  tc_serve_params p{};
-  std::vector<tc::Slot> slots;
-  std::mutex mu;
-  std::condition_variable cv_work, cv_done;
-  std::deque<std::shared_ptr<tc::Request>> pending;   // not yet admitted
-  std::deque<std::shared_ptr<tc::Request>> admitted;  // admitted, rows left to schedule
-  std::unordered_map<uint64_t, std::shared_ptr<tc::Request>> live;
-  uint64_t admitted_bytes = 0;
-  uint64_t next_id = 1;
-  uint64_t n_batches = 0, n_rows = 0;
-  bool stop = false;
-  std::string error;
-  std::unique_ptr<tc::CopyPool> copier;
-  std::thread worker;
+  std::unique_ptr<tc::ServeCore> core;
 };
-
-namespace tc {
-
-static void serve_loop(tc_server* S) {
-  const uint32_t F = S->ens->F;
-  const uint64_t row_bytes = 4ull * F + 4ull;
-  (void)hipSetDevice(S->ens->ctx->device);
-  std::unique_lock<std::mutex> lk(S->mu);
-  while (true) {
-    // admission: move pending requests while the budget allows (a request
-    // larger than the whole budget is admitted alone when nothing else is)
-    while (!S->pending.empty()) {
-      const uint64_t b = row_bytes * S->pending.front()->n;
-      if (S->admitted_bytes + b > S->p.memory_budget && S->admitted_bytes > 0) break;
-      S->admitted_bytes += b;
-      S->admitted.push_back(S->pending.front());
-      S->pending.pop_front();
-    }
-    bool progressed = false;
-    // retire finished slots
-    for (auto& sl : S->slots) {
-      if (!sl.busy) continue;
-      if (hipEventQuery(sl.done) != hipSuccess) continue;
-      for (auto& sg : sl.segs) {
-        std::memcpy(sg.req->out + sg.row0, sl.h_out + sg.slot_row, 4ull * sg.count);
-        sg.req->done += sg.count;
-        if (sg.req->done == sg.req->n) {
-          sg.req->finished = true;
-          S->admitted_bytes -= row_bytes * sg.req->n;
-        }
-      }
-      sl.segs.clear();
-      sl.busy = false;
-      progressed = true;
-      S->cv_done.notify_all();
-    }
-    // fill free slots from the admitted queue; the rows are staged with the
-    // queue unlocked (submitters keep going), in parallel pieces
-    for (auto& sl : S->slots) {
-      if (sl.busy || S->admitted.empty()) continue;
-      uint32_t rows = 0;
-      while (!S->admitted.empty() && rows < S->p.max_batch_rows) {
-        auto& rq = S->admitted.front();
-        const uint32_t take = std::min(rq->n - rq->next, S->p.max_batch_rows - rows);
-        sl.segs.push_back(Segment{rq, rq->next, take, rows});
-        rq->next += take;
-        rows += take;
-        if (rq->next == rq->n) S->admitted.pop_front();
-      }
-      if (rows == 0) continue;
-      lk.unlock();
-      {
-        constexpr size_t kPiece = 1u << 20;
-        std::vector<CopyPool::Piece> pieces;
-        for (const auto& sg : sl.segs) {
-          // a request's own tree-by-tree rows, wherever it lands in the batch
-          const uint32_t from = seq_rows_from(S->ens, false, sg.req->n);
-          for (uint32_t k = 0; k < sg.count; ++k) sl.h_flag[sg.slot_row + k] = sg.row0 + k >= from ? 1 : 0;
-          const char* src = reinterpret_cast<const char*>(sg.req->X + static_cast<size_t>(sg.row0) * F);
-          char* dst = reinterpret_cast<char*>(sl.h_x + static_cast<size_t>(sg.slot_row) * F);
-          const size_t bytes = 4ull * sg.count * F;
-          for (size_t o = 0; o < bytes; o += kPiece)
-            pieces.push_back(CopyPool::Piece{dst + o, src + o, std::min(kPiece, bytes - o)});
-        }
-        S->copier->copy(pieces);
-      }
-      lk.lock();
-      try {
-        TC_HIP(hipMemcpyAsync(sl.d_x, sl.h_x, 4ull * rows * F, hipMemcpyHostToDevice, sl.stream));
-        TC_HIP(hipMemcpyAsync(sl.d_flag, sl.h_flag, rows, hipMemcpyHostToDevice, sl.stream));
-        launch_predict(S->ens, sl.d_x, false, rows, F, 0, S->p.transform, sl.d_out, nullptr, sl.stream,
-                       sl.d_flag);
-        TC_HIP(hipMemcpyAsync(sl.h_out, sl.d_out, 4ull * rows, hipMemcpyDeviceToHost, sl.stream));
-        TC_HIP(hipEventRecord(sl.done, sl.stream));
-      } catch (const std::exception& e) {
-        S->error = e.what();
-        S->stop = true;
-      }
-      sl.busy = true;
-      ++S->n_batches;
-      S->n_rows += rows;
-      progressed = true;
-    }
-    bool any_busy = false;
-    for (auto& sl : S->slots) any_busy |= sl.busy;
-    // on stop, drain what was submitted unless the device failed
-    if (S->stop && !any_busy &&
-        ((S->admitted.empty() && S->pending.empty()) || !S->error.empty()))
-      break;
-    if (!progressed) {
-      if (any_busy) {
-        lk.unlock();
-        std::this_thread::sleep_for(std::chrono::microseconds(20));
-        lk.lock();
-      } else {
-        S->cv_work.wait_for(lk, std::chrono::milliseconds(2));
-      }
-    }
-  }
-  S->cv_done.notify_all();
-}
-
-}  // namespace tc
 
 using namespace tc;
 
@@ -610,6 +499,7 @@ int32_t tc_ensemble_create_multi(tc_ctx* ctx, uint32_t n_trees, uint32_t max_dep
     TC_REQUIRE(max_depth >= 1 && max_depth <= 16, "max_depth must be in [1, 16]");
     TC_REQUIRE(n_classes >= 1 && n_classes <= static_cast<uint32_t>(kMaxClasses),
                "n_classes must be in [1, 16]");
+    TC_REQUIRE(n_features >= 1 && n_features <= 65535, "n_features must be in [1, 65535]");
     const uint32_t K = n_classes;
     *out = nullptr;
     for (uint32_t t = 0; t < n_trees; ++t) {
@@ -619,24 +509,21 @@ int32_t tc_ensemble_create_multi(tc_ctx* ctx, uint32_t n_trees, uint32_t max_dep
     }
     auto* e = new tc_ensemble();
     e->ctx = ctx; e->T = n_trees; e->D = max_depth; e->F = n_features; e->K = K; e->base = base;
+    e->dmax = dmax_of(max_depth);
     for (uint32_t t = 0; t < n_trees; ++t) e->any_split = e->any_split || depth[t] > 0;
     try {
-      const size_t TD = std::max<size_t>(1, static_cast<size_t>(n_trees) * max_depth);
+      std::vector<uint32_t> rec;
+      switch (e->dmax) {
+        case 6: pack_records<6>(n_trees, max_depth, feature, fthr, bin, depth, rec); break;
+        case 8: pack_records<8>(n_trees, max_depth, feature, fthr, bin, depth, rec); break;
+        default: pack_records<16>(n_trees, max_depth, feature, fthr, bin, depth, rec); break;
+      }
       const size_t TL = std::max<size_t>(1, (static_cast<size_t>(n_trees) << max_depth) * K);
-      e->feat = dmalloc<uint16_t>(TD);
-      e->thr = dmalloc<float>(TD);
-      e->bin = dmalloc<uint8_t>(TD);
-      e->depth = dmalloc<uint16_t>(std::max(1u, n_trees));
+      e->rec = dmalloc<uint32_t>(rec.size());
       e->leaves = dmalloc<float>(TL);
       e->weight = dmalloc<float>(std::max(1u, n_trees));
+      TC_HIP(hipMemcpy(e->rec, rec.data(), 4 * rec.size(), hipMemcpyHostToDevice));
       if (n_trees) {
-        const size_t td = static_cast<size_t>(n_trees) * max_depth;
-        std::vector<uint8_t> b(td, 0);
-        if (bin) std::memcpy(b.data(), bin, td);
-        TC_HIP(hipMemcpy(e->feat, feature, td * 2, hipMemcpyHostToDevice));
-        TC_HIP(hipMemcpy(e->thr, fthr, td * 4, hipMemcpyHostToDevice));
-        TC_HIP(hipMemcpy(e->bin, b.data(), td, hipMemcpyHostToDevice));
-        TC_HIP(hipMemcpy(e->depth, depth, n_trees * 2ull, hipMemcpyHostToDevice));
         TC_HIP(hipMemcpy(e->leaves, leaves, (static_cast<size_t>(n_trees) << max_depth) * K * 4,
                          hipMemcpyHostToDevice));
         TC_HIP(hipMemcpy(e->weight, weights, n_trees * 4ull, hipMemcpyHostToDevice));
@@ -660,7 +547,9 @@ int32_t tc_ensemble_create(tc_ctx* ctx, uint32_t n_trees, uint32_t max_depth,
 int32_t tc_ensemble_destroy(tc_ensemble* e) {
   return guard([&] {
     if (!e) return;
-    dfree(e->feat); dfree(e->thr); dfree(e->bin); dfree(e->depth); dfree(e->leaves); dfree(e->weight);
+    if (e->ctx && e->ctx->stream) (void)hipStreamSynchronize(e->ctx->stream);
+    dfree(e->rec); dfree(e->leaves); dfree(e->weight); dfree(e->d_buf);
+    if (e->h_buf) (void)hipHostFree(e->h_buf);
     delete e;
   });
 }
@@ -670,7 +559,7 @@ int32_t tc_predict_dev(tc_ensemble* e, const float* X_dev, uint32_t n, uint32_t 
   return guard([&] {
     TC_REQUIRE(e && (X_dev || n == 0) && (out_dev || n == 0), "tc_predict_dev: NULL argument");
     TC_REQUIRE(F == e->F, "feature count differs from the model's");
-    launch_predict(e, X_dev, false, n, F, 0, transform, out_dev, nullptr, e->ctx->stream);
+    launch_predict(e, X_dev, false, n, F, 0, transform, out_dev, e->ctx->stream);
   });
 }
 
@@ -681,24 +570,24 @@ int32_t tc_predict(tc_ensemble* e, const float* X, uint32_t n, uint32_t F, int32
     TC_REQUIRE(F == e->F, "feature count differs from the model's");
     if (n == 0) return;
     hipStream_t s = e->ctx->stream;
-    float* dx = dmalloc<float>(static_cast<size_t>(n) * F + static_cast<size_t>(n) * e->K);
-    uint32_t* dl = nullptr;
-    try {
-      if (leaf_idx) dl = dmalloc<uint32_t>(static_cast<size_t>(n) * std::max(1u, e->T));
-      TC_HIP(hipMemcpyAsync(dx, X, 4ull * n * F, hipMemcpyHostToDevice, s));
-      launch_predict(e, dx, false, n, F, 0, transform, dx + static_cast<size_t>(n) * F, dl, s);
-      TC_HIP(hipMemcpyAsync(out, dx + static_cast<size_t>(n) * F, 4ull * n * e->K,
-                            hipMemcpyDeviceToHost, s));
-      if (leaf_idx && e->T)
-        TC_HIP(hipMemcpyAsync(leaf_idx, dl, 4ull * n * e->T, hipMemcpyDeviceToHost, s));
-      TC_HIP(hipStreamSynchronize(s));
-    } catch (...) {
-      dfree(dx);
-      dfree(dl);
-      throw;
+    const size_t nx = static_cast<size_t>(n) * F, no = static_cast<size_t>(n) * e->K;
+    const size_t nl = leaf_idx ? static_cast<size_t>(n) * std::max(1u, e->T) : 0;
+    float* d = ens_device_buf(e, nx + no + nl);
+    float* dout = d + nx;
+    uint32_t* dl = reinterpret_cast<uint32_t*>(d + nx + no);
+    if (host_reachable(X, 4 * nx)) {
+      TC_HIP(hipMemcpyAsync(d, X, 4 * nx, hipMemcpyHostToDevice, s));
+    } else {
+      // pageable rows: parallel copy into pinned staging, then one transfer
+      float* h = ens_host_buf(e, nx);
+      copy_pool().copy(split_copy(h, X, 4 * nx));
+      TC_HIP(hipMemcpyAsync(d, h, 4 * nx, hipMemcpyHostToDevice, s));
     }
-    dfree(dx);
-    dfree(dl);
+    launch_predict(e, d, false, n, F, 0, transform, dout, s);
+    if (leaf_idx && e->T) launch_leaf_index(e, d, n, F, dl, s);
+    TC_HIP(hipMemcpyAsync(out, dout, 4 * no, hipMemcpyDeviceToHost, s));
+    if (leaf_idx && e->T) TC_HIP(hipMemcpyAsync(leaf_idx, dl, 4 * nl, hipMemcpyDeviceToHost, s));
+    TC_HIP(hipStreamSynchronize(s));
   });
 }
 
@@ -720,7 +609,7 @@ int32_t tc_predict_noalloc(tc_ensemble* e, const float* X, uint32_t n, uint32_t 
     float* dout = reinterpret_cast<float*>(base + xb);
     hipStream_t s = e->ctx->stream;
     TC_HIP(hipMemcpyAsync(dx, X, 4ull * n * F, hipMemcpyHostToDevice, s));
-    launch_predict(e, dx, false, n, F, 0, transform, dout, nullptr, s);
+    launch_predict(e, dx, false, n, F, 0, transform, dout, s);
     TC_HIP(hipMemcpyAsync(out, dout, ob, hipMemcpyDeviceToHost, s));
     TC_HIP(hipStreamSynchronize(s));
   });
@@ -732,135 +621,110 @@ int32_t tc_predict_binned(tc_ensemble* e, tc_dataset* ds, float* out) {
     TC_REQUIRE(ds->F == e->F, "feature count differs from the model's");
     if (ds->N == 0) return;
     hipStream_t s = e->ctx->stream;
-    float* d = dmalloc<float>(static_cast<size_t>(ds->N) * e->K);
-    try {
-      launch_predict(e, ds->bins_cm, true, ds->N, ds->F, ds->N, 0, d, nullptr, s);
-      TC_HIP(hipMemcpyAsync(out, d, 4ull * ds->N * e->K, hipMemcpyDeviceToHost, s));
-      TC_HIP(hipStreamSynchronize(s));
-    } catch (...) {
-      dfree(d);
-      throw;
+    float* d = ens_device_buf(e, static_cast<size_t>(ds->N) * e->K);
+    launch_predict(e, ds->bins_cm, true, ds->N, ds->F, ds->N, 0, d, s);
+    TC_HIP(hipMemcpyAsync(out, d, 4ull * ds->N * e->K, hipMemcpyDeviceToHost, s));
+    TC_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int32_t tc_host_alloc(size_t bytes, void** out) {
+  return guard([&] {
+    TC_REQUIRE(out, "tc_host_alloc: NULL argument");
+    *out = nullptr;
+    void* p = nullptr;
+    const hipError_t e = hipHostMalloc(&p, std::max<size_t>(1, bytes), hipHostMallocDefault);
+    if (e != hipSuccess) throw Error(TC_ENOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    std::lock_guard<std::mutex> lk(g_host_mu);
+    g_host_ranges[reinterpret_cast<uintptr_t>(p)] = std::max<size_t>(1, bytes);
+    *out = p;
+  });
+}
+
+int32_t tc_host_free(void* p) {
+  return guard([&] {
+    if (!p) return;
+    {
+      std::lock_guard<std::mutex> lk(g_host_mu);
+      auto it = g_host_ranges.find(reinterpret_cast<uintptr_t>(p));
+      TC_REQUIRE(it != g_host_ranges.end(), "tc_host_free: not memory from tc_host_alloc");
+      g_host_ranges.erase(it);
     }
-    dfree(d);
+    TC_HIP(hipHostFree(p));
   });
 }
 
 int32_t tc_serve_start(tc_ensemble* e, const tc_serve_params* p, tc_server** out) {
   return guard([&] {
     TC_REQUIRE(e && p && out, "tc_serve_start: NULL argument");
-    TC_REQUIRE(e->K == 1, "tc_serve_start: serving takes single-output ensembles");
-    TC_REQUIRE(p->max_batch_rows > 0 && p->n_streams > 0 && p->n_streams <= 8,
-               "tc_serve_start: bad parameters");
+    TC_REQUIRE(p->max_batch_rows > 0 && p->max_batch_rows <= (1u << 24) && p->n_streams > 0 &&
+                   p->n_streams <= kServeRing,
+               "tc_serve_start: bad parameters (max_batch_rows in [1, 2^24], n_streams in [1, 8])");
     *out = nullptr;
+    ServeConfig c;
+    c.F = e->F;
+    c.K = e->K;
+    c.max_batch_rows = p->max_batch_rows;
+    c.memory_budget = p->memory_budget;
+    c.n_slots = p->n_streams;
+    c.seq_group = 8;
+    c.seq_all = !e->any_split;
+    c.fail_batch = p->fail_batch;
+    auto be = std::make_unique<GpuServeBackend>(e, c, p->transform);
     auto* S = new tc_server();
-    S->ens = e;
-    S->p = *p;
-    S->slots.resize(p->n_streams);
-    try {
-      for (auto& sl : S->slots) {
-        TC_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
-        TC_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
-        TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_x), 4ull * p->max_batch_rows * e->F,
-                             hipHostMallocDefault));
-        TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_out), 4ull * p->max_batch_rows,
-                             hipHostMallocDefault));
-        TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_flag), p->max_batch_rows,
-                             hipHostMallocDefault));
-        sl.d_x = dmalloc<float>(static_cast<size_t>(p->max_batch_rows) * e->F);
-        sl.d_out = dmalloc<float>(p->max_batch_rows);
-        sl.d_flag = dmalloc<uint8_t>(p->max_batch_rows);
-      }
-    } catch (...) {
-      for (auto& sl : S->slots) {
-        if (sl.stream) (void)hipStreamDestroy(sl.stream);
-        if (sl.done) (void)hipEventDestroy(sl.done);
-        if (sl.h_x) (void)hipHostFree(sl.h_x);
-        if (sl.h_out) (void)hipHostFree(sl.h_out);
-        if (sl.h_flag) (void)hipHostFree(sl.h_flag);
-        dfree(sl.d_x);
-        dfree(sl.d_out);
-        dfree(sl.d_flag);
-      }
-      delete S;
-      throw;
-    }
-    S->copier = std::make_unique<CopyPool>(7);
-    S->worker = std::thread(serve_loop, S);
+    S->core = std::make_unique<ServeCore>(c, std::move(be));
     *out = S;
   });
 }
 
+static int32_t serve_status(int32_t rc, const std::string& err) {
+  return rc == TC_OK ? TC_OK : fail(rc, err);
+}
+
 int32_t tc_serve_submit(tc_server* S, const float* X, uint32_t n, float* out, uint64_t* ticket) {
-  return guard([&] {
-    TC_REQUIRE(S && X && out && ticket && n > 0, "tc_serve_submit: bad arguments");
-    auto rq = std::make_shared<Request>();
-    rq->X = X;
-    rq->out = out;
-    rq->n = n;
-    {
-      std::lock_guard<std::mutex> lk(S->mu);
-      if (S->stop) throw Error(TC_ESTATE, "server stopped: " + S->error);
-      rq->id = S->next_id++;
-      S->pending.push_back(rq);
-      S->live[rq->id] = rq;
-    }
-    S->cv_work.notify_one();
-    *ticket = rq->id;
-  });
+  if (!S || !ticket) return fail(TC_EINVAL, "tc_serve_submit: bad arguments");
+  std::string err;
+  return serve_status(S->core->submit(X, n, out, ticket, &err), "tc_serve_submit: " + err);
 }
 
 int32_t tc_serve_wait(tc_server* S, uint64_t ticket, int64_t timeout_us) {
-  int32_t code = TC_OK;
-  const int32_t r = guard([&] {
-    TC_REQUIRE(S, "NULL server");
-    std::unique_lock<std::mutex> lk(S->mu);
-    auto it = S->live.find(ticket);
-    TC_REQUIRE(it != S->live.end(), "unknown ticket");
-    auto rq = it->second;
-    auto pred = [&] { return rq->finished || (S->stop && !S->error.empty()); };
-    if (timeout_us < 0) {
-      S->cv_done.wait(lk, pred);
-    } else if (!S->cv_done.wait_for(lk, std::chrono::microseconds(timeout_us), pred)) {
-      code = TC_ETIMEOUT;
-      return;
-    }
-    if (!rq->finished) throw Error(TC_EDEVICE, "server failed: " + S->error);
-    S->live.erase(ticket);
-  });
-  if (r != TC_OK) return r;
-  if (code != TC_OK) return fail(code, "tc_serve_wait: timed out");
-  return TC_OK;
+  if (!S) return fail(TC_EINVAL, "NULL server");
+  std::string err;
+  return serve_status(S->core->wait(ticket, timeout_us, &err), "tc_serve_wait: " + err);
+}
+
+int32_t tc_serve_cancel(tc_server* S, uint64_t ticket) {
+  if (!S) return fail(TC_EINVAL, "NULL server");
+  std::string err;
+  return serve_status(S->core->cancel(ticket, &err), "tc_serve_cancel: " + err);
 }
 
 int32_t tc_serve_stats(tc_server* S, uint64_t* batches, uint64_t* rows) {
   return guard([&] {
     TC_REQUIRE(S, "NULL server");
-    std::lock_guard<std::mutex> lk(S->mu);
-    if (batches) *batches = S->n_batches;
-    if (rows) *rows = S->n_rows;
+    const ServeCore::Stats st = S->core->stats();
+    if (batches) *batches = st.batches;
+    if (rows) *rows = st.rows;
+  });
+}
+
+int32_t tc_serve_counters(tc_server* S, tc_serve_counters_t* c) {
+  return guard([&] {
+    TC_REQUIRE(S && c, "tc_serve_counters: NULL argument");
+    const ServeCore::Stats st = S->core->stats();
+    c->batches = st.batches;
+    c->rows = st.rows;
+    c->peak_admitted_bytes = st.peak_admitted_bytes;
+    c->max_batches_in_flight = st.max_inflight;
+    c->cancelled = st.cancelled;
+    c->failed = st.failed;
   });
 }
 
 int32_t tc_serve_stop(tc_server* S) {
   return guard([&] {
     if (!S) return;
-    {
-      std::lock_guard<std::mutex> lk(S->mu);
-      S->stop = true;
-    }
-    S->cv_work.notify_all();
-    if (S->worker.joinable()) S->worker.join();
-    for (auto& sl : S->slots) {
-      (void)hipStreamSynchronize(sl.stream);
-      (void)hipStreamDestroy(sl.stream);
-      (void)hipEventDestroy(sl.done);
-      (void)hipHostFree(sl.h_x);
-      (void)hipHostFree(sl.h_out);
-      (void)hipHostFree(sl.h_flag);
-      dfree(sl.d_x);
-      dfree(sl.d_out);
-      dfree(sl.d_flag);
-    }
+    S->core->stop();
     delete S;
   });
 }

@@ -38,7 +38,8 @@ EXPORTS = [
     "tc_trainer_destroy",
     "tc_ensemble_create", "tc_ensemble_create_multi", "tc_ensemble_destroy", "tc_predict",
     "tc_predict_dev", "tc_predict_noalloc", "tc_predict_binned", "tc_serve_start", "tc_serve_submit",
-    "tc_serve_wait", "tc_serve_stats", "tc_serve_stop",
+    "tc_serve_wait", "tc_serve_cancel", "tc_serve_stats", "tc_serve_counters", "tc_serve_stop",
+    "tc_host_alloc", "tc_host_free",
 ]
 
 
@@ -78,7 +79,13 @@ class TrainHistory(Structure):
 
 class ServeParams(Structure):
     _fields_ = [("max_batch_rows", c_uint32), ("memory_budget", c_uint64),
-                ("n_streams", c_uint32), ("transform", c_int32)]
+                ("n_streams", c_uint32), ("transform", c_int32), ("fail_batch", c_uint64)]
+
+
+class ServeCounters(Structure):
+    _fields_ = [("batches", c_uint64), ("rows", c_uint64), ("peak_admitted_bytes", c_uint64),
+                ("max_batches_in_flight", c_uint64), ("cancelled", c_uint64),
+                ("failed", c_uint64)]
 
 
 def tree_params(max_depth=6, max_bins=255, lambda_l2=1.0, lambda_l1=0.0, max_delta_step=0.0,
@@ -184,7 +191,8 @@ def load(path: str = LIB_PATH):
         "tc_predict_noalloc": [V, V, U, U, I, V],
         "tc_serve_start": [V, POINTER(ServeParams), V], "tc_serve_submit": [V, V, U, V, V],
         "tc_serve_wait": [V, c_uint64, c_int64], "tc_serve_stats": [V, V, V],
-        "tc_serve_stop": [V],
+        "tc_serve_cancel": [V, c_uint64], "tc_serve_counters": [V, POINTER(ServeCounters)],
+        "tc_serve_stop": [V], "tc_host_alloc": [c_size_t, V], "tc_host_free": [V],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -529,17 +537,51 @@ class Ensemble:
         return out
 
 
+class HostBuffer:
+    """Pinned host memory (tc_host_alloc) viewed as a numpy array: rows in it
+    reach the device without a staging copy."""
+
+    def __init__(self, shape, dtype=np.float32):
+        self.L = load()
+        dt = np.dtype(dtype)
+        n = int(np.prod(shape)) * dt.itemsize
+        self.ptr = c_void_p()
+        check(self.L.tc_host_alloc(max(n, 1), byref(self.ptr)))
+        buf = (ctypes.c_char * max(n, 1)).from_address(self.ptr.value)
+        self.array = np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape)
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            check(self.L.tc_host_free(self.ptr))
+            self.ptr = c_void_p()
+
+
 class Server:
     """Continuous-batching server over a resident ensemble."""
 
     def __init__(self, ens: Ensemble, max_batch_rows=65536, memory_budget=1 << 30, n_streams=2,
-                 transform=0):
+                 transform=0, fail_batch=0):
         self.ens = ens
         self.L = ens.ctx.L
         self.h = c_void_p()
-        p = ServeParams(max_batch_rows, memory_budget, n_streams, transform)
+        p = ServeParams(max_batch_rows, memory_budget, n_streams, transform, fail_batch)
         check(self.L.tc_serve_start(ens.h, byref(p), byref(self.h)))
         self._keep = {}
+
+    def cancel(self, ticket):
+        """True when cancelled; False when the request is already on the device."""
+        rc = self.L.tc_serve_cancel(self.h, ticket)
+        if rc == TC_EBUSY:
+            return False
+        check(rc)
+        self._keep.pop(ticket, None)
+        return True
+
+    def counters(self):
+        c = ServeCounters()
+        check(self.L.tc_serve_counters(self.h, byref(c)))
+        return {k: getattr(c, k) for k, _ in ServeCounters._fields_}
 
     def submit(self, X, out):
         t = c_uint64()
@@ -564,3 +606,9 @@ class Server:
         if self.h:
             check(self.L.tc_serve_stop(self.h))
             self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass
