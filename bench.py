@@ -271,7 +271,7 @@ def run_serve(args):
     base_gaps = rng.exponential(1.0, n_req)  # scaled by 1/QPS; QPS 0 = all at once
 
     def plain(qps):
-        lat = []
+        lat, call = [], []
         t_start = time.perf_counter()
         t_next = t_start
         for x, g in zip(reqs, base_gaps):
@@ -280,9 +280,12 @@ def run_serve(args):
             if now < t_next:
                 time.sleep(t_next - now)
             t_arr = t_next if qps else now
+            t_call = time.perf_counter()
             ens.predict(x, transform=1)
-            lat.append(time.perf_counter() - t_arr)
-        return np.array(lat), time.perf_counter() - t_start
+            t_end = time.perf_counter()
+            lat.append(t_end - t_arr)
+            call.append(t_end - t_call)
+        return np.array(lat), time.perf_counter() - t_start, np.array(call)
 
     def served(qps, check=False):
         srv = capi.Server(ens, max_batch_rows=args.max_batch_rows, memory_budget=1 << 31,
@@ -315,21 +318,24 @@ def run_serve(args):
         done.wait()
         wall = time.perf_counter() - t_start
         b, r = srv.stats()
+        internal = srv.latencies_us() * 1e-6  # submit -> result, timed inside the server
         srv.stop()
         if check:  # each request alone and in the mix: identical bits
             for i in range(0, n_req, max(1, n_req // 20)):
                 assert np.array_equal(outs[i], ens.predict(reqs[i], transform=1))
-        return lat, wall, b
+        return lat, wall, b, internal
 
     rows = int(sizes.sum())
     p = lambda a, q: float(np.percentile(a, q) * 1e3)
     ens.predict(reqs[0], transform=1)  # warm the plain path
-    lat_p, _ = plain(args.qps)
-    lat_s, _, batches = served(args.qps, check=True)
-    log(f"[bench] {args.qps} QPS: plain p50/p99 {p(lat_p, 50):.3f}/{p(lat_p, 99):.3f} ms, "
-        f"served {p(lat_s, 50):.3f}/{p(lat_s, 99):.3f} ms ({batches} batches)")
-    _, wall_ps = plain(0)
-    _, wall_ss, batches_s = served(0)
+    lat_p, _, call_p = plain(args.qps)
+    lat_s, _, batches, int_s = served(args.qps, check=True)
+    log(f"[bench] {args.qps} QPS: plain p50/p99 {p(lat_p, 50):.3f}/{p(lat_p, 99):.3f} ms "
+        f"(call alone {p(call_p, 50):.3f}/{p(call_p, 99):.3f}), served "
+        f"{p(lat_s, 50):.3f}/{p(lat_s, 99):.3f} ms as the client thread sees it, "
+        f"{p(int_s, 50):.3f}/{p(int_s, 99):.3f} ms inside the server ({batches} batches)")
+    _, wall_ps, _ = plain(0)
+    _, wall_ss, batches_s, _ = served(0)
     log(f"[bench] saturation: plain {rows / wall_ps / 1e6:.2f} M rows/s "
         f"({n_req / wall_ps:.0f} req/s), served {rows / wall_ss / 1e6:.2f} M rows/s "
         f"({n_req / wall_ss:.0f} req/s, {batches_s} batches)")
@@ -344,6 +350,8 @@ def run_serve(args):
                    "requests": n_req, "qps": args.qps},
         "extra": {"served_p50_ms": p(lat_s, 50), "served_p99_ms": p(lat_s, 99),
                   "plain_p50_ms": p(lat_p, 50), "plain_p99_ms": p(lat_p, 99),
+                  "served_internal_p50_ms": p(int_s, 50), "served_internal_p99_ms": p(int_s, 99),
+                  "plain_call_p50_ms": p(call_p, 50), "plain_call_p99_ms": p(call_p, 99),
                   "saturation_served_rows_per_s": rows / wall_ss,
                   "saturation_plain_rows_per_s": rows / wall_ps,
                   "saturation_served_req_per_s": n_req / wall_ss,

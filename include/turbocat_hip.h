@@ -320,10 +320,15 @@ int32_t tc_predict_binned(tc_ensemble* ens, tc_dataset* ds, float* out);
 
 /* --------------------------------------------------------------- serving */
 /* Continuous-batching server over a resident ensemble (single-output or
- * K-channel).  No reference counterpart.  One persistent kernel stays
- * resident for the server's life and takes batches from a ring of
- * descriptors in coherent pinned host memory: no launch and no copy call per
- * batch.  Requests are admitted while their rows (4F + 4K bytes each) fit
+ * K-channel).  No reference counterpart.  A serving kernel stays resident
+ * while batches keep coming and takes them from a ring of descriptors in
+ * uncached pinned host memory: no launch and no copy call per batch.  It ends
+ * after 0.5 ms without a batch or 10 ms of running (HIP may put other
+ * streams' work on its hardware queue, which runs in order), and the server
+ * launches it again when a batch waits.  While a server runs, the library's
+ * own frees are deferred to its stop (hipFree waits for an idle device); a
+ * caller's device-wide wait (hipDeviceSynchronize, hipFree) may wait up to
+ * one time slice.  Requests are admitted while their rows (4F + 4K bytes each) fit
  * memory_budget (a request larger than the whole budget is admitted alone);
  * admitted requests are packed, whole or in pieces, into batches of up to
  * max_batch_rows rows; requests that arrive while batches are on the device
@@ -344,6 +349,7 @@ typedef struct {
   uint64_t peak_admitted_bytes;    /* most request bytes admitted at once */
   uint64_t max_batches_in_flight;
   uint64_t cancelled, failed;      /* requests */
+  uint64_t kernel_launches;        /* serving kernel launches (one per busy period or time slice) */
 } tc_serve_counters_t;
 int32_t tc_serve_start(tc_ensemble* ens, const tc_serve_params* p, tc_server** out);
 /* Submit n rows; X and out must stay valid until tc_serve_wait returns for
@@ -357,6 +363,10 @@ int32_t tc_serve_wait(tc_server* s, uint64_t ticket, int64_t timeout_us);
 int32_t tc_serve_cancel(tc_server* s, uint64_t ticket);
 int32_t tc_serve_stats(tc_server* s, uint64_t* batches, uint64_t* rows);
 int32_t tc_serve_counters(tc_server* s, tc_serve_counters_t* out);
+/* Latencies in microseconds of the requests finished so far, from
+ * tc_serve_submit to the result in the caller's buffer, in completion order
+ * (the first 2^20): copies up to cap of them into us; *count = how many. */
+int32_t tc_serve_latencies(tc_server* s, float* us, uint64_t cap, uint64_t* count);
 /* Drains what was submitted (unless the server failed), then stops. */
 int32_t tc_serve_stop(tc_server* s);
 /* Pinned host memory: tc_predict transfers rows from it to the device

@@ -176,6 +176,9 @@ void fake_counters(void* p, uint64_t* out6) {
   out6[0] = s.batches; out6[1] = s.rows; out6[2] = s.peak_admitted_bytes;
   out6[3] = s.max_inflight; out6[4] = s.cancelled; out6[5] = s.failed;
 }
+uint64_t fake_latencies(void* p, float* us, uint64_t cap) {
+  return static_cast<Handle*>(p)->core->latencies(us, cap);
+}
 const char* fake_error(void* p) { return static_cast<Handle*>(p)->err.c_str(); }
 void fake_stop(void* p) {
   auto* h = static_cast<Handle*>(p);

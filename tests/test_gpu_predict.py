@@ -166,6 +166,70 @@ def test_serving_alone_and_in_the_mix(ctx, oracle):
         assert np.array_equal(a, b)
 
 
+def test_library_calls_while_a_server_runs(ctx, oracle):
+    """hipFree waits for the whole device to go idle, which a server's
+    resident kernel never lets it be: the library's frees are deferred while
+    a server runs.  Plain predictions whose cached buffers grow (and free the
+    old ones), an ensemble and a dataset created and destroyed, and device
+    memory allocated and freed, all with a server resident; then the server
+    still answers, bit-identically."""
+    import ctypes
+    from turbocat import capi
+    F = 30
+    m = random_model(20, 6, F, seed=31)
+    ens = ctx.ensemble(m, F)
+    srv = capi.Server(ens, max_batch_rows=4096, memory_budget=1 << 24, n_streams=2)
+    try:
+        for n in (1, 300, 5000, 70000):                   # each size grows the cached buffers
+            X, _ = regression_data(n, F, seed=n)
+            assert np.array_equal(ens.predict(X), oracle.predict_f32(m, X))
+        m2 = random_model(9, 4, F, seed=32)
+        e2 = ctx.ensemble(m2, F)
+        X, _ = regression_data(999, F, seed=5)
+        assert np.array_equal(e2.predict(X), oracle.predict_f32(m2, X))
+        e2.close()
+        ds, _, _ = ctx.quantize_fit(X)
+        ds.close()
+        p = ctypes.c_void_p()
+        capi.check(ctx.L.tc_device_alloc(ctx.h, 1 << 20, ctypes.byref(p)))
+        capi.check(ctx.L.tc_device_free(ctx.h, p))
+        X, _ = regression_data(2500, F, seed=6)
+        out = np.zeros(len(X), np.float32)
+        assert srv.wait(srv.submit(X, out), timeout_us=60_000_000)
+        assert np.array_equal(out, ens.predict(X))
+    finally:
+        srv.stop()
+    X, _ = regression_data(100_000, F, seed=7)              # frees work again after the stop
+    assert np.array_equal(ens.predict(X), oracle.predict_f32(m, X))
+
+
+def test_handles_may_be_destroyed_in_any_order(oracle):
+    """Every object holds what it uses: a context destroyed before its
+    ensemble and dataset, and an ensemble destroyed under a running server,
+    stay usable by their holders until those are destroyed in turn."""
+    from turbocat import capi
+    F = 12
+    c = capi.Context(0)
+    m = random_model(15, 5, F, seed=41)
+    e = c.ensemble(m, F)
+    X, _ = regression_data(3000, F, seed=42)
+    ds, _, _ = c.quantize_fit(X)
+    bins = ds.bins()
+    srv = capi.Server(e, max_batch_rows=1024, memory_budget=1 << 22, n_streams=2)
+    try:
+        c.close()                                       # the ensemble and dataset hold it
+        assert np.array_equal(e.predict(X), oracle.predict_f32(m, X))
+        assert np.array_equal(ds.bins(), bins)
+        ds.close()
+        e_ref = e.predict(X)
+        e.close()                                       # the server holds the ensemble
+        out = np.zeros(len(X), np.float32)
+        assert srv.wait(srv.submit(X, out), timeout_us=60_000_000)
+        assert np.array_equal(out, e_ref)
+    finally:
+        srv.stop()                                      # the last holder: all released
+
+
 def test_serving_c3_shape_alone_and_in_the_mix(ctx):
     """The serving workload's shape: 1000 trees of depth 8 over 200 features,
     requests of 1..65536 rows (log-uniform) submitted from four threads into
@@ -209,8 +273,10 @@ def test_serving_c3_shape_alone_and_in_the_mix(ctx):
         for t in tickets:
             assert srv.wait(t, timeout_us=120_000_000)
         c = srv.counters()
+        lat = srv.latencies_us()
     finally:
         srv.stop()
+    assert len(lat) == len(reqs) and (lat > 0).all()
     row = 4 * F + 4
     assert c["rows"] == sum(sizes) and c["batches"] < len(reqs)
     assert c["peak_admitted_bytes"] <= max(budget, max(sizes) * row)

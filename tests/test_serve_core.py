@@ -10,6 +10,7 @@ import ctypes
 import os
 import subprocess
 import threading
+import time
 
 import numpy as np
 import pytest
@@ -38,6 +39,8 @@ def lib(tmp_path_factory):
         getattr(L, fn).argtypes = args
         getattr(L, fn).restype = ctypes.c_int32
     L.fake_counters.argtypes = [V, V]
+    L.fake_latencies.argtypes = [V, V, U64]
+    L.fake_latencies.restype = U64
     L.fake_error.argtypes = [V]
     L.fake_error.restype = ctypes.c_char_p
     L.fake_stop.argtypes = [V]
@@ -77,6 +80,12 @@ class Fake:
         self.L.fake_counters(self.h, c)
         return dict(zip(("batches", "rows", "peak", "max_inflight", "cancelled", "failed"), c))
 
+    def latencies(self):
+        n = self.L.fake_latencies(self.h, None, 0)
+        out = np.zeros(n, np.float32)
+        self.L.fake_latencies(self.h, out.ctypes.data, n)
+        return out
+
     def stop(self):
         self.L.fake_stop(self.h)
         self.h = None
@@ -101,13 +110,18 @@ def test_alone_and_in_the_mix_from_several_threads(lib, K):
             rc, tickets[i] = srv.submit(reqs[i], outs[i])
             assert rc == OK
 
+    t0 = time.perf_counter()
     th = [threading.Thread(target=client, args=(k,)) for k in range(4)]
     [t.start() for t in th]
     [t.join() for t in th]
     for t in tickets:
         assert srv.wait(t, 30_000_000) == OK
+    wall_us = (time.perf_counter() - t0) * 1e6
     c = srv.counters()
+    lat = srv.latencies()
     srv.stop()
+    assert len(lat) == len(reqs)                                    # one per finished request
+    assert (lat > 0).all() and lat.max() <= wall_us
     assert c["rows"] == sum(sizes) and c["batches"] < len(reqs)     # coalesced
     assert c["peak"] <= 64 << 10                                    # budget held
     for x, o in zip(reqs, outs):
@@ -151,7 +165,9 @@ def test_cancel_and_timeout(lib):
     assert srv.wait(ta, 30_000_000) == OK and srv.wait(tb, 30_000_000) == OK
     assert lib.fake_cancel(srv.h, ta) == EINVAL         # finished and collected
     cnt = srv.counters()
+    n_lat = len(srv.latencies())
     srv.stop()
+    assert n_lat == 2                                    # the cancelled request has none
     assert cnt["cancelled"] == 1 and not oc.any()
     assert np.array_equal(ob, expected(b, 1, 10))
 

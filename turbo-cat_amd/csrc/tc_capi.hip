@@ -1,6 +1,9 @@
 // C-ABI: contexts, errors, device memory plumbing, datasets (A2/A3).
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "tc_internal.h"
 
@@ -15,6 +18,54 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 int32_t fail(int32_t code, const std::string& msg) {
   g_last_error = msg;
   return code;
+}
+
+namespace {
+// Frees are made under this lock, so that no server kernel can start while
+// one is waiting for the device to go idle.
+std::mutex g_free_mu;
+int g_servers = 0;
+std::vector<std::pair<void*, bool>> g_deferred;  // (pointer, pinned host memory)
+}  // namespace
+
+hipError_t free_or_defer(void* p, bool host) {
+  if (!p) return hipSuccess;
+  std::lock_guard<std::mutex> lk(g_free_mu);
+  if (g_servers > 0) {
+    g_deferred.emplace_back(p, host);
+    return hipSuccess;
+  }
+  return host ? hipHostFree(p) : hipFree(p);
+}
+
+void unref(tc_ctx* c) {
+  if (!c || c->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+void unref(tc_dataset* ds) {
+  if (!ds || ds->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+  if (ds->ctx) (void)hipStreamSynchronize(ds->ctx->stream);
+  dfree(ds->bins_rm);
+  dfree(ds->bins_cm);
+  dfree(ds->edges);
+  dfree(ds->n_edges);
+  unref(ds->ctx);
+  delete ds;
+}
+
+void serving_begin() {
+  std::lock_guard<std::mutex> lk(g_free_mu);
+  ++g_servers;
+}
+
+void serving_end() {
+  std::lock_guard<std::mutex> lk(g_free_mu);
+  if (--g_servers > 0) return;
+  for (const auto& d : g_deferred) (void)(d.second ? hipHostFree(d.first) : hipFree(d.first));
+  g_deferred.clear();
 }
 
 }  // namespace tc
@@ -51,12 +102,7 @@ int32_t tc_ctx_create(int32_t device_id, tc_ctx** out) {
 }
 
 int32_t tc_ctx_destroy(tc_ctx* ctx) {
-  return guard([&] {
-    if (!ctx) return;
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
-  });
+  return guard([&] { unref(ctx); });
 }
 
 int32_t tc_ctx_synchronize(tc_ctx* ctx) {
@@ -77,7 +123,7 @@ int32_t tc_device_alloc(tc_ctx* ctx, size_t bytes, void** out) {
 int32_t tc_device_free(tc_ctx* ctx, void* p) {
   return guard([&] {
     TC_REQUIRE(ctx, "NULL ctx");
-    if (p) TC_HIP(hipFree(p));
+    TC_HIP(free_or_defer(p, false));
   });
 }
 
@@ -97,20 +143,11 @@ int32_t tc_memcpy_d2h(tc_ctx* ctx, void* dst, const void* src, size_t bytes) {
   });
 }
 
-static void destroy_dataset(tc_dataset* ds) {
-  if (!ds) return;
-  dfree(ds->bins_rm);
-  dfree(ds->bins_cm);
-  dfree(ds->edges);
-  dfree(ds->n_edges);
-  delete ds;
-}
-
 static tc_dataset* make_dataset(tc_ctx* ctx, const float* X, uint32_t n, uint32_t F,
                                 uint32_t max_bins, std::vector<float>&& edges,
                                 std::vector<uint32_t>&& n_edges) {
   auto* ds = new tc_dataset();
-  ds->ctx = ctx;
+  ds->ctx = ref(ctx);
   ds->N = n;
   ds->F = F;
   ds->max_bins = max_bins;
@@ -120,7 +157,7 @@ static tc_dataset* make_dataset(tc_ctx* ctx, const float* X, uint32_t n, uint32_
     TC_HIP(hipSetDevice(ctx->device));
     bin_on_device(ds, X);
   } catch (...) {
-    destroy_dataset(ds);
+    unref(ds);
     throw;
   }
   return ds;
@@ -160,8 +197,8 @@ int32_t tc_dataset_create(tc_ctx* ctx, const float* X, uint32_t n, uint32_t F,
 int32_t tc_dataset_bins(tc_dataset* ds, uint8_t* bins_out) {
   return guard([&] {
     TC_REQUIRE(ds && bins_out, "tc_dataset_bins: NULL argument");
-    TC_HIP(hipMemcpy(bins_out, ds->bins_cm, static_cast<size_t>(ds->N) * ds->F,
-                     hipMemcpyDeviceToHost));
+    copy_sync(bins_out, ds->bins_cm, static_cast<size_t>(ds->N) * ds->F, hipMemcpyDeviceToHost,
+              ds->ctx->stream);
   });
 }
 
@@ -174,7 +211,7 @@ int32_t tc_dataset_shape(tc_dataset* ds, uint32_t* n, uint32_t* F) {
 }
 
 int32_t tc_dataset_destroy(tc_dataset* ds) {
-  return guard([&] { destroy_dataset(ds); });
+  return guard([&] { unref(ds); });
 }
 
 }  // extern "C"

@@ -2,7 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -10,13 +12,18 @@
 #include "../../include/turbocat_hip.h"
 #include "tc_common.h"
 
+// Handles are reference counted (refs): an object keeps what it uses alive
+// -- every object its context, a trainer its datasets, a server its
+// ensemble -- so the caller may destroy handles in any order.
 struct tc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   double last_loop_ms = 0.0, last_hist_ms = 0.0;
+  std::atomic<int> refs{1};
 };
 
 struct tc_dataset {
+  std::atomic<int> refs{1};
   tc_ctx* ctx = nullptr;
   uint32_t N = 0, F = 0, Fp = 0, max_bins = 255;
   uint8_t* bins_rm = nullptr;   // [N][Fp], Fp = rm_stride(F)
@@ -28,6 +35,7 @@ struct tc_dataset {
 };
 
 struct tc_ensemble {
+  std::atomic<int> refs{1};
   tc_ctx* ctx = nullptr;
   uint32_t T = 0, D = 0, F = 0;
   uint32_t K = 1;               // output channels (classes of a softmax ensemble)
@@ -42,6 +50,7 @@ struct tc_ensemble {
   size_t d_cap = 0;
   float* h_buf = nullptr;       // pinned
   size_t h_cap = 0;
+  std::mutex buf_mu;            // one host-pointer call at a time uses the buffers
 };
 
 namespace tc {
@@ -90,9 +99,32 @@ T* dmalloc(size_t count) {
   return static_cast<T*>(p);
 }
 
-inline void dfree(void* p) {
-  if (p) (void)hipFree(p);
+// hipFree and hipHostFree wait until the whole device is idle, and a device
+// running a server's persistent kernel never is.  Every free of the library
+// goes through these two: while any server runs they are queued, and done
+// when the last server has stopped (its kernel finished).
+hipError_t free_or_defer(void* p, bool host);
+inline void dfree(void* p) { (void)free_or_defer(p, false); }  // device memory from hipMalloc
+inline void hfree(void* p) { (void)free_or_defer(p, true); }   // pinned host memory
+// A blocking copy ordered on stream s only (never on the null stream, so it
+// waits for nothing a server runs).
+inline void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s) {
+  TC_HIP(hipMemcpyAsync(dst, src, bytes, kind, s));
+  TC_HIP(hipStreamSynchronize(s));
 }
+// Reference counts of the handles: ref takes one more, unref drops one and
+// destroys the object with the last.
+template <class H>
+H* ref(H* h) {
+  h->refs.fetch_add(1, std::memory_order_relaxed);
+  return h;
+}
+void unref(tc_ctx* c);
+void unref(tc_dataset* d);
+void unref(tc_ensemble* e);
+// A server's kernel is about to start / has finished.
+void serving_begin();
+void serving_end();
 
 // Address of the allocation-free prediction's device scratch on `device`,
 // taken once per device (the device must be current).

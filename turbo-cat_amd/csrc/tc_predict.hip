@@ -26,6 +26,7 @@ namespace tc {
 class CopyPool {
  public:
   using Piece = std::tuple<void*, const void*, size_t>;
+  static constexpr size_t kInline = size_t(256) << 10;
   explicit CopyPool(int helpers) {
     for (int i = 0; i < helpers; ++i) th_.emplace_back([this] { run(); });
   }
@@ -40,6 +41,12 @@ class CopyPool {
   // Copies every piece; returns when all are done.  One caller at a time.
   void copy(const std::vector<Piece>& pieces) {
     if (pieces.empty()) return;
+    size_t total = 0;
+    for (const Piece& p : pieces) total += std::get<2>(p);
+    if (total <= kInline) {  // small: waking the helpers costs more than the copy
+      for (const Piece& p : pieces) std::memcpy(std::get<0>(p), std::get<1>(p), std::get<2>(p));
+      return;
+    }
     std::lock_guard<std::mutex> one(call_mu_);
     {
       std::lock_guard<std::mutex> lk(mu_);
@@ -121,7 +128,7 @@ static void check_same_address(void* p) {
   void* d = nullptr;
   TC_HIP(hipHostGetDevicePointer(&d, p, 0));
   if (d != p) {
-    (void)hipHostFree(p);
+    hfree(p);
     throw Error(TC_EDEVICE, "pinned host memory has a device address other than its host address");
   }
 }
@@ -303,7 +310,7 @@ static float* ens_device_buf(tc_ensemble* e, size_t floats) {
 }
 static float* ens_host_buf(tc_ensemble* e, size_t floats) {
   if (floats > e->h_cap) {
-    if (e->h_buf) (void)hipHostFree(e->h_buf);
+    if (e->h_buf) hfree(e->h_buf);
     e->h_buf = nullptr;
     e->h_cap = 0;
     TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&e->h_buf), floats * 4, hipHostMallocDefault));
@@ -330,15 +337,19 @@ static T* uncached_alloc(size_t count) {
 
 class GpuServeBackend : public ServeBackend {
  public:
-  GpuServeBackend(tc_ensemble* e, const ServeConfig& c, int transform) : ens_(e), cfg_(c) {
+  GpuServeBackend(tc_ensemble* e, const ServeConfig& c, int transform) : ens_(ref(e)), cfg_(c) {
     try {
       setup(transform);
     } catch (...) {
       release();
+      unref(ens_);
       throw;
     }
   }
-  ~GpuServeBackend() override { release(); }
+  ~GpuServeBackend() override {
+    release();
+    unref(ens_);
+  }
 
   float* stage_x(uint32_t s) override {
     return sx_ + static_cast<size_t>(s) * cfg_.max_batch_rows * cfg_.F;
@@ -357,12 +368,16 @@ class GpuServeBackend : public ServeBackend {
   }
 
   void publish(uint32_t s, uint64_t seq, uint32_t rows) override {
-    ServeHostDesc& d = host_->ring[seq % kServeRing];
+    const uint32_t pos = static_cast<uint32_t>(seq % kServeRing);
+    ServeHostDesc& d = host_->ring[pos];
+    tile0_[pos] = tiles_;
     d.tile0 = tiles_;
     d.slot = s;
     d.rows = rows;
     __atomic_store_n(&d.seq, seq, __ATOMIC_RELEASE);  // after the other fields
     tiles_ += (rows + R_ - 1) / R_;
+    published_ = seq + 1;
+    pump();
   }
 
   bool done(uint32_t, uint64_t seq) override {
@@ -370,41 +385,89 @@ class GpuServeBackend : public ServeBackend {
   }
 
   std::string health() override {
-    const hipError_t q = hipEventQuery(end_);
-    if (q == hipErrorNotReady) return "";
-    if (q == hipSuccess) {
-      uint32_t fault = 0;
-      (void)hipMemcpy(&fault, &ctl_->fault, 4, hipMemcpyDeviceToHost);
-      if (fault & 2u) return "serving kernel stopped: a batch descriptor failed its checks";
-      if (fault & 1u) return "serving kernel stopped: no published batch held a claimed tile";
-      return "serving kernel ended before the server stopped";
-    }
-    return std::string("serving kernel failed: ") + hipGetErrorString(q);
+    if (running_) reap();
+    return error_;
   }
 
   void heartbeat() override {
     __atomic_store_n(&host_->heartbeat, ++beats_, __ATOMIC_RELEASE);
+    if (published_ > __atomic_load_n(&host_->taken, __ATOMIC_ACQUIRE)) pump();
   }
 
   void copy(const std::vector<std::tuple<void*, const void*, size_t>>& pieces) override {
     copy_pool().copy(pieces);
   }
 
+  uint64_t launches() const override { return launches_; }
+
   void shutdown() override {
-    if (!launched_) return;
+    if (!began_) return;
     __atomic_store_n(&host_->stop, 1u, __ATOMIC_RELEASE);
-    (void)hipStreamSynchronize(stream_);
+    (void)hipStreamSynchronize(stream_);  // a running launch ends on the stop word
+    running_ = false;
     unregister_server(&host_->stop);
-    launched_ = false;
+    began_ = false;
+    serving_end();
   }
 
  private:
+  // When a published batch is not taken and no launch is running, launches
+  // the kernel from the first batch not taken.
+  void pump() {
+    if (running_ && !reap()) return;
+    if (!error_.empty()) return;
+    const uint64_t taken = __atomic_load_n(&host_->taken, __ATOMIC_ACQUIRE);
+    if (published_ <= taken) return;
+    try {
+      launch(taken, tile0_[taken % kServeRing]);
+    } catch (const std::exception& ex) {
+      error_ = std::string("serving kernel launch failed: ") + ex.what();
+    }
+  }
+
+  // True once the running launch has ended; a fault it raised, or a launch
+  // failure, becomes the server's error.
+  bool reap() {
+    const hipError_t q = hipEventQuery(end_);
+    if (q == hipErrorNotReady) return false;
+    running_ = false;
+    if (q != hipSuccess) {
+      error_ = std::string("serving kernel failed: ") + hipGetErrorString(q);
+      return true;
+    }
+    uint32_t fault = 0;
+    if (hipMemcpyAsync(&fault, &ctl_->fault, 4, hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+        hipStreamSynchronize(stream_) != hipSuccess)
+      error_ = "serving kernel state could not be read";
+    else if (fault & 2u)
+      error_ = "serving kernel stopped: a batch descriptor failed its checks";
+    else if (fault & 1u)
+      error_ = "serving kernel stopped: no published batch held a claimed tile";
+    return true;
+  }
+
+  void launch(uint64_t batch0, uint64_t tile0) {
+    TC_HIP(hipMemsetAsync(ctl_, 0, sizeof(ServeCtl), stream_));
+    ServeLaunch L;
+    L.batch0 = batch0;
+    L.tile0 = tile0;
+    L.hb_ticks = hb_ticks_;
+    L.idle_ticks = idle_ticks_;
+    L.slice_ticks = slice_ticks_;
+    hipLaunchKernelGGL(kern_, dim3(grid_), dim3(kPredThreads), lds_, stream_, ens_dev(ens_), ctl_,
+                       host_, stage_, cfg_.F, R_, transform_, L);
+    TC_HIP(hipGetLastError());
+    TC_HIP(hipEventRecord(end_, stream_));
+    running_ = true;
+    ++launches_;
+  }
+
   void setup(int transform) {
     TC_HIP(hipSetDevice(ens_->ctx->device));
     const uint32_t F = cfg_.F, K = cfg_.K;
     const size_t rows = static_cast<size_t>(cfg_.n_slots) * cfg_.max_batch_rows;
-    size_t lds;
-    predict_shape(F, K, false, &R_, &lds);
+    predict_shape(F, K, false, &R_, &lds_);
+    transform_ = transform;
     host_ = uncached_alloc<ServeHost>(1);
     std::memset(static_cast<void*>(host_), 0, sizeof(ServeHost));
     for (auto& d : host_->ring) d.seq = kServeNoBatch;
@@ -414,38 +477,33 @@ class GpuServeBackend : public ServeBackend {
     ctl_ = dmalloc<ServeCtl>(1);
     TC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     TC_HIP(hipEventCreateWithFlags(&end_, hipEventDisableTiming));
-    TC_HIP(hipMemsetAsync(ctl_, 0, sizeof(ServeCtl), stream_));
     int cus = 0, rate_khz = 0;
     TC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ens_->ctx->device));
     TC_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ens_->ctx->device));
     if (rate_khz <= 0) rate_khz = 100000;  // the gfx9 wall clock runs at 100 MHz
     // one worker per CU: the rest of each CU stays free for other work
-    const uint32_t grid = 1u + static_cast<uint32_t>(std::max(1, cus));
-    const unsigned long long hb_ticks = 2000ull * static_cast<unsigned long long>(rate_khz);
-    const EnsDev E = ens_dev(ens_);
-    ServeStage st;
-    st.x = sx_;
-    st.flag = sf_;
-    st.out = so_;
-    st.n_slots = cfg_.n_slots;
-    st.max_rows = cfg_.max_batch_rows;
-    auto go = [&](auto kern) {
-      if (lds > 64 * 1024)
-        TC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-      hipLaunchKernelGGL(kern, dim3(grid), dim3(kPredThreads), lds, stream_, E, ctl_, host_, st, F,
-                         R_, transform, hb_ticks);
-    };
+    grid_ = 1u + static_cast<uint32_t>(std::max(1, cus));
+    const unsigned long long per_ms = static_cast<unsigned long long>(rate_khz);
+    hb_ticks_ = 2000ull * per_ms;    // 2 s without a heartbeat: the host is gone
+    idle_ticks_ = per_ms / 2;        // 0.5 ms without a batch
+    slice_ticks_ = 10ull * per_ms;   // 10 ms per launch at most
+    stage_.x = sx_;
+    stage_.flag = sf_;
+    stage_.out = so_;
+    stage_.n_slots = cfg_.n_slots;
+    stage_.max_rows = cfg_.max_batch_rows;
     const bool multi = K > 1;
     switch (ens_->dmax) {
-      case 6: multi ? go(&k_serve<6, true>) : go(&k_serve<6, false>); break;
-      case 8: multi ? go(&k_serve<8, true>) : go(&k_serve<8, false>); break;
-      default: multi ? go(&k_serve<16, true>) : go(&k_serve<16, false>); break;
+      case 6: kern_ = multi ? &k_serve<6, true> : &k_serve<6, false>; break;
+      case 8: kern_ = multi ? &k_serve<8, true> : &k_serve<8, false>; break;
+      default: kern_ = multi ? &k_serve<16, true> : &k_serve<16, false>; break;
     }
-    TC_HIP(hipGetLastError());
-    launched_ = true;
+    if (lds_ > 64 * 1024)
+      TC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern_),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_)));
+    serving_begin();  // from here on, the library's frees wait for the server's stop
+    began_ = true;
     register_server(&host_->stop);
-    TC_HIP(hipEventRecord(end_, stream_));
   }
 
   void release() {
@@ -455,7 +513,7 @@ class GpuServeBackend : public ServeBackend {
     dfree(ctl_);
     for (void* p : {static_cast<void*>(sx_), static_cast<void*>(sf_), static_cast<void*>(so_),
                     static_cast<void*>(host_)})
-      if (p) (void)hipHostFree(p);
+      if (p) hfree(p);
     end_ = nullptr;
     stream_ = nullptr;
     ctl_ = nullptr;
@@ -475,7 +533,19 @@ class GpuServeBackend : public ServeBackend {
   float* so_ = nullptr;
   hipStream_t stream_ = nullptr;
   hipEvent_t end_ = nullptr;
-  bool launched_ = false;
+  using ServeKernel = void (*)(EnsDev, ServeCtl*, ServeHost*, ServeStage, uint32_t, uint32_t, int,
+                               ServeLaunch);
+  ServeKernel kern_ = nullptr;
+  ServeStage stage_{};
+  size_t lds_ = 0;
+  uint32_t grid_ = 1;
+  int transform_ = 0;
+  unsigned long long hb_ticks_ = 0, idle_ticks_ = 0, slice_ticks_ = 0;
+  bool began_ = false;    // serving_begin called, stop word registered
+  bool running_ = false;  // a launch whose end has not been seen
+  std::string error_;
+  uint64_t published_ = 0, launches_ = 0;
+  unsigned long long tile0_[kServeRing] = {};
   unsigned long long tiles_ = 0, beats_ = 0;
 };
 
@@ -484,6 +554,17 @@ class GpuServeBackend : public ServeBackend {
 struct tc_server {
   std::unique_ptr<tc::ServeCore> core;
 };
+
+namespace tc {
+void unref(tc_ensemble* e) {
+  if (!e || e->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+  if (e->ctx) (void)hipStreamSynchronize(e->ctx->stream);
+  dfree(e->rec); dfree(e->leaves); dfree(e->weight); dfree(e->d_buf);
+  hfree(e->h_buf);
+  unref(e->ctx);
+  delete e;
+}
+}  // namespace tc
 
 using namespace tc;
 
@@ -508,7 +589,7 @@ int32_t tc_ensemble_create_multi(tc_ctx* ctx, uint32_t n_trees, uint32_t max_dep
         TC_REQUIRE(feature[t * max_depth + d] < n_features, "split feature out of range");
     }
     auto* e = new tc_ensemble();
-    e->ctx = ctx; e->T = n_trees; e->D = max_depth; e->F = n_features; e->K = K; e->base = base;
+    e->ctx = ref(ctx); e->T = n_trees; e->D = max_depth; e->F = n_features; e->K = K; e->base = base;
     e->dmax = dmax_of(max_depth);
     for (uint32_t t = 0; t < n_trees; ++t) e->any_split = e->any_split || depth[t] > 0;
     try {
@@ -522,11 +603,12 @@ int32_t tc_ensemble_create_multi(tc_ctx* ctx, uint32_t n_trees, uint32_t max_dep
       e->rec = dmalloc<uint32_t>(rec.size());
       e->leaves = dmalloc<float>(TL);
       e->weight = dmalloc<float>(std::max(1u, n_trees));
-      TC_HIP(hipMemcpy(e->rec, rec.data(), 4 * rec.size(), hipMemcpyHostToDevice));
+      hipStream_t s = ctx->stream;
+      copy_sync(e->rec, rec.data(), 4 * rec.size(), hipMemcpyHostToDevice, s);
       if (n_trees) {
-        TC_HIP(hipMemcpy(e->leaves, leaves, (static_cast<size_t>(n_trees) << max_depth) * K * 4,
-                         hipMemcpyHostToDevice));
-        TC_HIP(hipMemcpy(e->weight, weights, n_trees * 4ull, hipMemcpyHostToDevice));
+        copy_sync(e->leaves, leaves, (static_cast<size_t>(n_trees) << max_depth) * K * 4,
+                  hipMemcpyHostToDevice, s);
+        copy_sync(e->weight, weights, n_trees * 4ull, hipMemcpyHostToDevice, s);
       }
     } catch (...) {
       tc_ensemble_destroy(e);
@@ -545,13 +627,7 @@ int32_t tc_ensemble_create(tc_ctx* ctx, uint32_t n_trees, uint32_t max_depth,
 }
 
 int32_t tc_ensemble_destroy(tc_ensemble* e) {
-  return guard([&] {
-    if (!e) return;
-    if (e->ctx && e->ctx->stream) (void)hipStreamSynchronize(e->ctx->stream);
-    dfree(e->rec); dfree(e->leaves); dfree(e->weight); dfree(e->d_buf);
-    if (e->h_buf) (void)hipHostFree(e->h_buf);
-    delete e;
-  });
+  return guard([&] { unref(e); });
 }
 
 int32_t tc_predict_dev(tc_ensemble* e, const float* X_dev, uint32_t n, uint32_t F, int32_t transform,
@@ -569,6 +645,7 @@ int32_t tc_predict(tc_ensemble* e, const float* X, uint32_t n, uint32_t F, int32
     TC_REQUIRE(e && (X || n == 0) && (out || n == 0), "tc_predict: NULL argument");
     TC_REQUIRE(F == e->F, "feature count differs from the model's");
     if (n == 0) return;
+    std::lock_guard<std::mutex> lk(e->buf_mu);
     hipStream_t s = e->ctx->stream;
     const size_t nx = static_cast<size_t>(n) * F, no = static_cast<size_t>(n) * e->K;
     const size_t nl = leaf_idx ? static_cast<size_t>(n) * std::max(1u, e->T) : 0;
@@ -620,6 +697,7 @@ int32_t tc_predict_binned(tc_ensemble* e, tc_dataset* ds, float* out) {
     TC_REQUIRE(e && ds && out, "tc_predict_binned: NULL argument");
     TC_REQUIRE(ds->F == e->F, "feature count differs from the model's");
     if (ds->N == 0) return;
+    std::lock_guard<std::mutex> lk(e->buf_mu);
     hipStream_t s = e->ctx->stream;
     float* d = ens_device_buf(e, static_cast<size_t>(ds->N) * e->K);
     launch_predict(e, ds->bins_cm, true, ds->N, ds->F, ds->N, 0, d, s);
@@ -650,7 +728,7 @@ int32_t tc_host_free(void* p) {
       TC_REQUIRE(it != g_host_ranges.end(), "tc_host_free: not memory from tc_host_alloc");
       g_host_ranges.erase(it);
     }
-    TC_HIP(hipHostFree(p));
+    TC_HIP(free_or_defer(p, true));
   });
 }
 
@@ -718,6 +796,14 @@ int32_t tc_serve_counters(tc_server* S, tc_serve_counters_t* c) {
     c->max_batches_in_flight = st.max_inflight;
     c->cancelled = st.cancelled;
     c->failed = st.failed;
+    c->kernel_launches = st.launches;
+  });
+}
+
+int32_t tc_serve_latencies(tc_server* S, float* us, uint64_t cap, uint64_t* count) {
+  return guard([&] {
+    TC_REQUIRE(S && count, "tc_serve_latencies: NULL argument");
+    *count = S->core->latencies(us, cap);
   });
 }
 

@@ -88,6 +88,38 @@ __host__ __device__ inline uint32_t pred_stride(uint32_t F, bool binned) {
   return binned ? 4u * (((F + 3u) / 4u) | 1u) : (F | 1u);
 }
 
+// Stages nr contiguous rows of F floats (src, global or host memory) into
+// LDS rows of stride S: a flat pass over the nr * F values, eight loads in
+// flight per thread before their stores.
+__device__ __forceinline__ void stage_rows(const float* src_generic, uint32_t nr, uint32_t F,
+                                           float* sx, uint32_t S) {
+  typedef const __attribute__((address_space(1))) float* gptr;
+  const gptr src = (gptr)src_generic;
+  const uint32_t total = nr * F, tid = threadIdx.x;
+  const uint32_t dq = kPredThreads / F, dm = kPredThreads % F;  // the stride in (row, column)
+  uint32_t i = tid, row = tid / F, col = tid % F;
+  while (i < total) {
+    float v[8];
+    uint32_t rr[8], cc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      rr[u] = row;
+      cc[u] = col;
+      v[u] = i < total ? src[i] : 0.f;
+      i += kPredThreads;
+      row += dq;
+      col += dm;
+      if (col >= F) {
+        col -= F;
+        ++row;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (rr[u] < nr) sx[rr[u] * S + cc[u]] = v[u];
+  }
+}
+
 // LDS of the group values / leaf values: [kPredWaves * G][4][64] floats.
 template <int G>
 constexpr uint32_t pred_val_floats() {
@@ -251,10 +283,8 @@ __global__ void __launch_bounds__(kPredThreads)
     for (uint32_t f = w; f < F; f += kPredWaves)
       if (r < nr) s_x[r * S + f] = B[static_cast<size_t>(f) * ldc + row0 + r];
   } else {
-    const float* X = static_cast<const float*>(Xv) + static_cast<size_t>(row0) * F;
-    float* sx = reinterpret_cast<float*>(s_x);
-    for (uint32_t rr = w; rr < nr; rr += kPredWaves)
-      for (uint32_t f = r; f < F; f += 64) sx[rr * S + f] = X[static_cast<size_t>(rr) * F + f];
+    stage_rows(static_cast<const float*>(Xv) + static_cast<size_t>(row0) * F, nr, F,
+               reinterpret_cast<float*>(s_x), S);
   }
   __syncthreads();
   const bool active = r < nr;
@@ -293,7 +323,7 @@ __global__ void __launch_bounds__(256) k_leaf_index(EnsDev E, const float* __res
 }
 
 // ---------------------------------------------------------------------------
-// Persistent serving kernel.
+// Serving kernel: resident while there is work, bounded in lifetime.
 //
 // The host stages each batch in a slot of server-owned pinned host memory
 // (rows, one tree-by-tree flag per row, outputs; slot s holds rows
@@ -303,20 +333,26 @@ __global__ void __launch_bounds__(256) k_leaf_index(EnsDev E, const float* __res
 // Workgroup 0, thread 0 is the dispatcher: it waits for the descriptor of the
 // next batch number, checks it (the expected first tile, slot < n_slots,
 // 1 <= rows <= max_rows; anything else stops the kernel with a fault flag),
-// copies it into device memory and advances the count of published tiles.
-// Every other workgroup claims tiles (R rows) in order with one atomic, waits
-// for its tile to be published, finds its batch (rechecking the tile range,
-// slot and rows), predicts the tile's rows from the slot's staging and writes
-// the outputs there; the last tile of a batch sets the batch's done word in
-// host memory.  The device dereferences no pointer written by the host:
-// every host memory access is an index into the fixed staging arrays,
-// checked against their bounds.  No kernel launch and no copy call per batch.
+// copies it into device memory, advances the count of published tiles and
+// reports the batches taken in host memory (ServeHost::taken).  Every other
+// workgroup claims tiles (R rows) in order with one atomic, waits for its
+// tile to be published, finds its batch (rechecking the tile range, slot and
+// rows), predicts the tile's rows from the slot's staging and writes the
+// outputs there; the last tile of a batch sets the batch's done word in host
+// memory.  The device dereferences no pointer written by the host: every
+// host memory access is an index into the fixed staging arrays, checked
+// against their bounds.  No copy call per batch, and no launch per batch
+// while batches keep coming.
 //
-// Exit: the dispatcher stops on the host's stop word, or when the host's
-// heartbeat has not changed for hb_ticks of the wall clock (the host is
-// gone); it then raises ServeCtl::stop, and a worker whose claimed tile is
-// not published exits on it.  Every published tile is processed before its
-// claimer looks at the stop word again.
+// Lifetime: a hardware queue runs its packets in order, and HIP may place
+// another stream's work on this kernel's queue; so the kernel never stays
+// for good.  The dispatcher stops taking batches when no batch came for
+// idle_ticks, when it has run for slice_ticks, on the host's stop word, or
+// when the host's heartbeat has not changed for hb_ticks; it then raises
+// ServeCtl::stop, every published tile is processed, and the kernel ends.
+// The host launches it again, from the first batch not taken, whenever a
+// published batch is not taken and no launch is running.  Counters in
+// ServeCtl are relative to the launch's first batch and tile.
 constexpr uint32_t kServeRing = 8;
 constexpr unsigned long long kServeNoBatch = ~0ull;  // a descriptor not yet written
 
@@ -328,6 +364,7 @@ struct ServeHostDesc {
 
 struct ServeHost {
   unsigned long long heartbeat;            // changed by the host while it lives
+  unsigned long long taken;                // batches taken by the device (all launches)
   uint32_t stop;
   uint32_t pad;
   unsigned long long done[kServeRing];     // seq + 1 of the last batch finished in each position
@@ -339,14 +376,21 @@ struct ServeDevDesc {
   uint32_t slot, rows, ntiles, pad;
 };
 
+// Device-side state of one launch, zeroed before it.
 struct ServeCtl {
-  unsigned long long next_tile;      // tiles claimed
-  unsigned long long pub_tiles;      // tiles published on the device side
-  unsigned long long pub_batches;    // batches published on the device side
+  unsigned long long next_tile;      // tiles claimed in this launch
+  unsigned long long pub_tiles;      // tiles published in this launch
+  unsigned long long pub_batches;    // batches published in this launch
   uint32_t stop;
   uint32_t fault;                    // 1: a worker found no batch for its tile; 2: a bad descriptor
   uint32_t done_cnt[kServeRing];     // tiles finished of each position's batch
   ServeDevDesc ring[kServeRing];
+};
+
+// Where a launch starts and when its dispatcher stops (wall-clock ticks).
+struct ServeLaunch {
+  unsigned long long batch0, tile0;  // first batch number / tile index of the launch
+  unsigned long long hb_ticks, idle_ticks, slice_ticks;
 };
 
 // Staging of a server: n_slots x max_rows rows (host memory).
@@ -357,30 +401,42 @@ struct ServeStage {
   uint32_t n_slots, max_rows;
 };
 
-__device__ __forceinline__ unsigned long long ld_sys(const unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+// Polling uses relaxed atomic loads (read at the scope's coherence point,
+// no cache invalidation per poll); the acquire is one fence once the polled
+// value says the data is there.
+__device__ __forceinline__ unsigned long long rx_sys(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
-__device__ __forceinline__ uint32_t ld_sys32(const uint32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+__device__ __forceinline__ uint32_t rx_sys32(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
-__device__ __forceinline__ unsigned long long ld_dev(const unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ unsigned long long rx_dev(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ uint32_t ld_dev32(const uint32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ uint32_t rx_dev32(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+__device__ __forceinline__ void acquire_system() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, ""); }
+__device__ __forceinline__ void release_system() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, ""); }
 
-__device__ void serve_dispatch(ServeCtl* ctl, const ServeHost* host, ServeStage st, uint32_t R,
-                               unsigned long long hb_ticks) {
-  unsigned long long pubb = 0, pubt = 0;
-  unsigned long long hb = ld_sys(&host->heartbeat), t_hb = wall_clock64();
-  bool bad = false;
-  while (!bad) {
-    while (true) {  // every batch the host has published, in order
+__device__ void serve_dispatch(ServeCtl* ctl, ServeHost* host, ServeStage st, uint32_t R,
+                               const ServeLaunch L) {
+  unsigned long long pubb = L.batch0, pubt = L.tile0;
+  const unsigned long long t_start = wall_clock64();
+  unsigned long long hb = rx_sys(&host->heartbeat), t_hb = t_start, t_act = t_start;
+  bool bad = false, spent = false;
+  while (!bad && !spent) {
+    while (true) {  // every batch the host has published, in order, within the slice
+      if (wall_clock64() - t_start > L.slice_ticks) {
+        spent = true;
+        break;
+      }
       const ServeHostDesc* hd = &host->ring[pubb % kServeRing];
-      if (ld_sys(&hd->seq) != pubb) break;  // not written yet
-      const unsigned long long tile0 = ld_sys(&hd->tile0);
-      const uint32_t slot = ld_sys32(&hd->slot), rows = ld_sys32(&hd->rows);
+      if (rx_sys(&hd->seq) != pubb) break;  // not written yet
+      acquire_system();                      // the fields written before the batch number
+      const unsigned long long tile0 = rx_sys(&hd->tile0);
+      const uint32_t slot = rx_sys32(&hd->slot), rows = rx_sys32(&hd->rows);
       if (tile0 != pubt || slot >= st.n_slots || rows == 0 || rows > st.max_rows) {
         atomicOr(&ctl->fault, 2u);
         bad = true;
@@ -395,16 +451,21 @@ __device__ void serve_dispatch(ServeCtl* ctl, const ServeHost* host, ServeStage 
       __hip_atomic_store(&dd->seq, pubb, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       pubt += nt;
       ++pubb;
-      __hip_atomic_store(&ctl->pub_batches, pubb, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&ctl->pub_tiles, pubt, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctl->pub_batches, pubb - L.batch0, __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctl->pub_tiles, pubt - L.tile0, __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&host->taken, pubb, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      t_act = wall_clock64();
     }
-    if (bad || ld_sys32(&host->stop)) break;
-    const unsigned long long h = ld_sys(&host->heartbeat), now = wall_clock64();
+    if (bad || spent || rx_sys32(&host->stop)) break;
+    const unsigned long long h = rx_sys(&host->heartbeat), now = wall_clock64();
+    if (now - t_act > L.idle_ticks || now - t_start > L.slice_ticks) break;
     if (h != hb) {
       hb = h;
       t_hb = now;
-    } else if (now - t_hb > hb_ticks) {
-      break;  // the host stopped beating: end rather than spin forever
+    } else if (now - t_hb > L.hb_ticks) {
+      break;  // the host stopped beating: end rather than spin
     }
     __builtin_amdgcn_s_sleep(2);
   }
@@ -414,10 +475,10 @@ __device__ void serve_dispatch(ServeCtl* ctl, const ServeHost* host, ServeStage 
 template <int DMAX, bool MULTI>
 __global__ void __launch_bounds__(kPredThreads)
     k_serve(EnsDev E, ServeCtl* ctl, ServeHost* host, ServeStage st, uint32_t F, uint32_t R,
-            int transform, unsigned long long hb_ticks) {
+            int transform, ServeLaunch L) {
   extern __shared__ __align__(16) unsigned char s_raw[];
   if (blockIdx.x == 0) {
-    if (threadIdx.x == 0) serve_dispatch(ctl, host, st, R, hb_ticks);
+    if (threadIdx.x == 0) serve_dispatch(ctl, host, st, R, L);
     return;
   }
   __shared__ unsigned long long s_t, s_seq_no;
@@ -427,24 +488,35 @@ __global__ void __launch_bounds__(kPredThreads)
   float* sx = reinterpret_cast<float*>(s_raw + pred_head_bytes(MULTI, E.K));
   const uint32_t S = pred_stride(F, false);
   const uint32_t tid = threadIdx.x, w = tid >> 6, r = tid & 63u, K = E.K;
-  unsigned long long cur_b = 0;  // thread 0: the first batch that may hold the next tile
+  unsigned long long cur_b = L.batch0;  // thread 0: the first batch that may hold the next tile
   while (true) {
     if (tid == 0) {
-      const unsigned long long t = atomicAdd(&ctl->next_tile, 1ull);
+      const unsigned long long t = L.tile0 + atomicAdd(&ctl->next_tile, 1ull);
       int ex = 0;
       while (true) {
-        if (t < ld_dev(&ctl->pub_tiles)) break;
-        if (ld_dev32(&ctl->stop)) {
+        if (t < L.tile0 + rx_dev(&ctl->pub_tiles)) break;
+        if (rx_dev32(&ctl->stop)) {
+          // the dispatcher publishes its last tiles before it raises stop:
+          // look again after synchronizing with the stop
+          acquire_agent();
+          if (t < L.tile0 + rx_dev(&ctl->pub_tiles)) break;
           ex = 1;
           break;
         }
         __builtin_amdgcn_s_sleep(20);
       }
       if (!ex) {
+        // the batch's descriptor (dispatcher) and staged rows (host) are
+        // visible from here on
+        acquire_system();
         // batches are published in order with contiguous tile ranges; a
         // position holding a later batch means the earlier one is finished
-        // (the position was reused), so it cannot hold this unprocessed tile
-        const unsigned long long nb = ld_dev(&ctl->pub_batches);
+        // (the position was reused), so it cannot hold this unprocessed tile.
+        // The host reuses a position only after its batch is done, so every
+        // batch before the last kServeRing published is finished: the scan
+        // starts there (a few probes, however long this wave was idle).
+        const unsigned long long nb = L.batch0 + rx_dev(&ctl->pub_batches);
+        if (nb > kServeRing && cur_b < nb - kServeRing) cur_b = nb - kServeRing;
         while (true) {
           if (cur_b >= nb) {  // inconsistent: no published batch holds the tile
             atomicOr(&ctl->fault, 1u);
@@ -452,7 +524,7 @@ __global__ void __launch_bounds__(kPredThreads)
             break;
           }
           const ServeDevDesc* d = &ctl->ring[cur_b % kServeRing];
-          const unsigned long long sq = ld_dev(&d->seq);
+          const unsigned long long sq = rx_dev(&d->seq);
           const unsigned long long t0 = d->tile0;
           const uint32_t nt = d->ntiles, rows = d->rows, slot = d->slot;
           if (sq == cur_b && t >= t0 && t < t0 + nt) {
@@ -473,17 +545,13 @@ __global__ void __launch_bounds__(kPredThreads)
       }
       s_t = t;
       s_exit = ex;
-      // the tile's rows were staged by the host before it published the batch
-      __threadfence_system();
     }
     __syncthreads();
     if (s_exit) return;
     // in bounds: slot < n_slots, rbase + nr <= rows <= max_rows
     const uint32_t rbase = s_k * R, nr = min(R, s_rows - rbase);
     const size_t row0 = static_cast<size_t>(s_slot) * st.max_rows + rbase;
-    const float* xs = st.x + row0 * F;
-    for (uint32_t rr = w; rr < nr; rr += kPredWaves)
-      for (uint32_t f = r; f < F; f += 64) sx[rr * S + f] = xs[static_cast<size_t>(rr) * F + f];
+    stage_rows(st.x + row0 * F, nr, F, sx, S);
     __syncthreads();
     const bool active = r < nr;
     const float* xrow = sx + (active ? r : 0) * S;
@@ -503,8 +571,8 @@ __global__ void __launch_bounds__(kPredThreads)
       const float sum = tile_sum<false, DMAX, kPredG>(E, xrow, r, active, seq, s_val);
       if (w == 0 && active) st.out[row0 + r] = finish_k1(E, sum, transform);
     }
-    // the outputs reach host memory before the batch is marked done
-    __threadfence_system();
+    // each wave's outputs reach host memory before the batch is marked done
+    release_system();
     __syncthreads();
     if (tid == 0) {
       const uint32_t pos = static_cast<uint32_t>(s_seq_no % kServeRing);
@@ -512,7 +580,6 @@ __global__ void __launch_bounds__(kPredThreads)
                                                   __HIP_MEMORY_SCOPE_AGENT);
       if (old + 1 == s_ntiles) {
         __hip_atomic_store(&ctl->done_cnt[pos], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence_system();
         __hip_atomic_store(&host->done[pos], s_seq_no + 1, __ATOMIC_RELEASE,
                            __HIP_MEMORY_SCOPE_SYSTEM);
       }

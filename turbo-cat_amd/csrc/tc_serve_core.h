@@ -60,6 +60,7 @@ class ServeBackend {
     for (const auto& p : pieces) std::memcpy(std::get<0>(p), std::get<1>(p), std::get<2>(p));
   }
   virtual void shutdown() = 0;  // the device side stops; no slot is read afterwards
+  virtual uint64_t launches() const { return 0; }  // device launches so far
 };
 
 enum class ReqState { kPending, kAdmitted, kDone, kFailed, kCancelled };
@@ -83,6 +84,7 @@ class ServeCore {
     rq->out = out;
     rq->n = n;
     rq->bytes = row_bytes() * n;
+    rq->t_submit = std::chrono::steady_clock::now();
     {
       std::lock_guard<std::mutex> lk(mu_);
       if (stop_) { *err = "server stopped" + (error_.empty() ? "" : ": " + error_); return kState; }
@@ -144,11 +146,12 @@ class ServeCore {
   }
 
   struct Stats {
-    uint64_t batches, rows, peak_admitted_bytes, max_inflight, cancelled, failed;
+    uint64_t batches, rows, peak_admitted_bytes, max_inflight, cancelled, failed, launches;
   };
   Stats stats() {
     std::lock_guard<std::mutex> lk(mu_);
-    return Stats{n_batches_, n_rows_, peak_admitted_, max_inflight_, n_cancelled_, n_failed_};
+    return Stats{n_batches_, n_rows_,     peak_admitted_,  max_inflight_,
+                 n_cancelled_, n_failed_, be_->launches()};
   }
 
   // Drains what was submitted (unless the device failed), then stops.
@@ -170,6 +173,16 @@ class ServeCore {
     return error_;
   }
 
+  // Latencies (microseconds) of the requests finished so far, from submit
+  // to the result in the caller's buffer, in completion order (the first
+  // kMaxLatencies).  Copies up to cap of them; returns how many there are.
+  uint64_t latencies(float* out, uint64_t cap) {
+    std::lock_guard<std::mutex> lk(mu_);
+    const uint64_t n = std::min<uint64_t>(cap, latencies_.size());
+    if (out && n) std::memcpy(out, latencies_.data(), 4 * n);
+    return latencies_.size();
+  }
+
  private:
   struct Req {
     uint64_t id = 0;
@@ -177,6 +190,7 @@ class ServeCore {
     float* out = nullptr;
     uint32_t n = 0;
     uint64_t bytes = 0;
+    std::chrono::steady_clock::time_point t_submit;
     uint32_t next = 0;  // next row to place in a batch
     uint32_t done = 0;  // rows completed
     ReqState state = ReqState::kPending;
@@ -203,6 +217,10 @@ class ServeCore {
     rq->state = s;
     rq->error = why;
     if (s == ReqState::kFailed) ++n_failed_;
+    if (s == ReqState::kDone && latencies_.size() < kMaxLatencies)
+      latencies_.push_back(static_cast<float>(
+          std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - rq->t_submit)
+              .count()));
   }
 
   // Fails every request not yet finished.  Called once nothing the device
@@ -376,6 +394,8 @@ class ServeCore {
   uint64_t inflight_ = 0, max_inflight_ = 0;
   bool stop_ = false, stop_failed_ = false, stopped_ = false;
   std::string error_;
+  static constexpr size_t kMaxLatencies = size_t(1) << 20;
+  std::vector<float> latencies_;
   std::string failing_;                          // a batch failed: fail once in-flight drains
   std::vector<std::shared_ptr<Req>> failed_segs_;  // requests of the failed batch
   std::thread worker_;

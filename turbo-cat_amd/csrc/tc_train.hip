@@ -439,7 +439,7 @@ class Trainer {
     StopPoll& operator=(const StopPoll&) = delete;
     ~StopPoll() {
       if (ev) (void)hipEventSynchronize(ev);
-      if (flags) (void)hipHostFree(flags);
+      if (flags) hfree(flags);
       if (ev) (void)hipEventDestroy(ev);
     }
   };
@@ -618,7 +618,7 @@ class Trainer {
         }
       TC_HIP(hipStreamSynchronize(s));
       std::vector<unsigned long long> rows(static_cast<size_t>(nsnap) * njobs_);
-      TC_HIP(hipMemcpy(rows.data(), snap, 8 * rows.size(), hipMemcpyDeviceToHost));
+      copy_sync(rows.data(), snap, 8 * rows.size(), hipMemcpyDeviceToHost, s);
       uint32_t best = 0;
       for (uint32_t c = 0; c < kN; ++c) {
         double t = 0.0, nr = 0.0;
@@ -656,8 +656,8 @@ class Trainer {
   void check_faults() {
     for (uint32_t j = 0; j < njobs_; ++j) {
       uint32_t f = 0;
-      TC_HIP(hipMemcpy(&f, reinterpret_cast<const char*>(h_jobs_[j].st) + offsetof(TreeState, fault),
-                       4, hipMemcpyDeviceToHost));
+      copy_sync(&f, reinterpret_cast<const char*>(h_jobs_[j].st) + offsetof(TreeState, fault), 4,
+                hipMemcpyDeviceToHost, ctx_->stream);
       if (f & 2u)
         throw Error(TC_EDEVICE, "internal consistency check failed in the split scan of job " +
                                     std::to_string(j) + " (more channels than a pass holds)");
@@ -940,12 +940,12 @@ class Trainer {
     allocs_.clear();
     for (auto& S : samplers_) {
       for (int b = 0; b < 2; ++b) {
-        if (S.host[b]) (void)hipHostFree(S.host[b]);
+        if (S.host[b]) hfree(S.host[b]);
         if (S.ev[b]) (void)hipEventDestroy(S.ev[b]);
         S.host[b] = nullptr;
         S.ev[b] = nullptr;
       }
-      if (S.gh_host) (void)hipHostFree(S.gh_host);
+      if (S.gh_host) hfree(S.gh_host);
       S.gh_host = nullptr;
     }
   }
@@ -1005,8 +1005,14 @@ class Trainer {
 }  // namespace tc
 
 struct tc_trainer {
-  tc_ctx* ctx = nullptr;
+  tc_ctx* ctx = nullptr;              // held (a reference)
+  std::vector<tc_dataset*> datasets;  // held, one reference each
   std::unique_ptr<tc::Trainer> tr;
+  ~tc_trainer() {
+    tr.reset();
+    for (tc_dataset* d : datasets) tc::unref(d);
+    tc::unref(ctx);
+  }
 };
 
 namespace tc {
@@ -1162,7 +1168,8 @@ int32_t tc_trainer_create(tc_ctx* ctx, uint32_t n_jobs, tc_dataset* const* ds, c
       specs.push_back(JobSpec{ds[j], y[j], p[j], init_prediction_host(p[j].loss, y[j], ds[j]->N)});
     }
     auto h = std::make_unique<tc_trainer>();
-    h->ctx = ctx;
+    h->ctx = ref(ctx);
+    for (uint32_t j = 0; j < n_jobs; ++j) h->datasets.push_back(ref(ds[j]));
     h->tr = std::make_unique<Trainer>(ctx, specs);
     h->tr->init_gradients();
     TC_HIP(hipStreamSynchronize(ctx->stream));

@@ -38,7 +38,8 @@ EXPORTS = [
     "tc_trainer_destroy",
     "tc_ensemble_create", "tc_ensemble_create_multi", "tc_ensemble_destroy", "tc_predict",
     "tc_predict_dev", "tc_predict_noalloc", "tc_predict_binned", "tc_serve_start", "tc_serve_submit",
-    "tc_serve_wait", "tc_serve_cancel", "tc_serve_stats", "tc_serve_counters", "tc_serve_stop",
+    "tc_serve_wait", "tc_serve_cancel", "tc_serve_stats", "tc_serve_counters",
+    "tc_serve_latencies", "tc_serve_stop",
     "tc_host_alloc", "tc_host_free",
 ]
 
@@ -85,7 +86,7 @@ class ServeParams(Structure):
 class ServeCounters(Structure):
     _fields_ = [("batches", c_uint64), ("rows", c_uint64), ("peak_admitted_bytes", c_uint64),
                 ("max_batches_in_flight", c_uint64), ("cancelled", c_uint64),
-                ("failed", c_uint64)]
+                ("failed", c_uint64), ("kernel_launches", c_uint64)]
 
 
 def tree_params(max_depth=6, max_bins=255, lambda_l2=1.0, lambda_l1=0.0, max_delta_step=0.0,
@@ -192,6 +193,7 @@ def load(path: str = LIB_PATH):
         "tc_serve_start": [V, POINTER(ServeParams), V], "tc_serve_submit": [V, V, U, V, V],
         "tc_serve_wait": [V, c_uint64, c_int64], "tc_serve_stats": [V, V, V],
         "tc_serve_cancel": [V, c_uint64], "tc_serve_counters": [V, POINTER(ServeCounters)],
+        "tc_serve_latencies": [V, V, c_uint64, V],
         "tc_serve_stop": [V], "tc_host_alloc": [c_size_t, V], "tc_host_free": [V],
     }
     for name, args in sig.items():
@@ -577,6 +579,14 @@ class Server:
         check(rc)
         self._keep.pop(ticket, None)
         return True
+
+    def latencies_us(self):
+        """Submit-to-result latency of every request finished so far (us)."""
+        n = c_uint64()
+        check(self.L.tc_serve_latencies(self.h, None, 0, byref(n)))
+        out = np.zeros(n.value, np.float32)
+        check(self.L.tc_serve_latencies(self.h, _p(out), n.value, byref(n)))
+        return out
 
     def counters(self):
         c = ServeCounters()
