@@ -30,6 +30,42 @@ def test_quantize_bitexact(ctx, oracle):
     assert np.array_equal(ds2.bins(), oracle.apply_bins(Xn, oe, on))
 
 
+def test_quantize_edge_cases(ctx, oracle):
+    """Columns at the rule's corners, quantized on the device and by the
+    oracle: no value at all, U = 255, 256, 257 and 511 under the uint8 wrap
+    (A = 255, 0, 1, 255), +-inf, both signed zeros (equal under the reference's
+    comparisons: edges compare equal, bins bit-exact), huge magnitudes, heavy
+    duplicates, a single non-NaN value, and rows wide enough (F = 700) to
+    stream through the pinned staging in several blocks."""
+    rng = np.random.default_rng(21)
+    n, F = 14003, 700
+    X = rng.standard_normal((n, F)).astype(np.float32)
+    X[:, 0] = np.nan
+    for c, U in ((1, 255), (2, 256), (3, 257), (4, 511)):
+        X[:, c] = rng.permutation(np.arange(n) % U).astype(np.float32)
+    X[::3, 5] = np.inf
+    X[1::3, 5] = -np.inf
+    X[:, 6] = np.where(rng.random(n) < 0.5, -0.0, 0.0).astype(np.float32)
+    X[::7, 6] = 1.5
+    X[1::7, 6] = -1.0                                # U = 3: the zero is an edge
+    X[:, 7] = (rng.standard_normal(n) * 1e37).astype(np.float32)
+    X[:, 8] = np.round(rng.standard_normal(n) * 3).astype(np.float32)
+    X[:, 9] = np.nan
+    X[17, 9] = -2.5
+    X[::11, 10] = np.nan
+    for wrap in (True, False):
+        for mb in (255, 16):
+            ds, edges, ne = ctx.quantize_fit(X, max_bins=mb, wrap=wrap)
+            ob, oe, on = oracle.quantize(X, max_bins=mb, wrap=wrap)
+            assert np.array_equal(ne, on), (wrap, mb)
+            for f in range(F):
+                assert np.array_equal(edges[f, :ne[f]], oe[f, :on[f]]), (wrap, mb, f)
+            assert np.array_equal(ds.bins(), ob), (wrap, mb)
+            ds.close()
+    assert on[0] == 1 and on[9] == 0                 # no value: one 0.0 edge; one value: none
+    assert on[6] == 2 and oe[6, 0] == 0.0
+
+
 def test_quantize_fixed_mode(ctx, oracle):
     X, _ = regression_data(5000, 4, seed=5)
     ds, edges, ne = ctx.quantize_fit(X, wrap=False)

@@ -42,7 +42,13 @@ void unref(tc_ctx* c) {
   if (!c || c->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
   (void)hipStreamSynchronize(c->stream);
   (void)hipStreamDestroy(c->stream);
+  for (void* p : c->stage) hfree(p);
   delete c;
+}
+
+void* ctx_stage(tc_ctx* c, int b) {
+  if (!c->stage[b]) TC_HIP(hipHostMalloc(&c->stage[b], tc_ctx::kStageBytes, hipHostMallocDefault));
+  return c->stage[b];
 }
 
 void unref(tc_dataset* ds) {
@@ -170,12 +176,22 @@ int32_t tc_quantize_fit(tc_ctx* ctx, const float* X, uint32_t n, uint32_t F, uin
     TC_REQUIRE(F >= 1, "tc_quantize_fit: no features");
     TC_REQUIRE(max_bins >= 2 && max_bins <= 255, "max_bins must be in [2, 255]");
     *out = nullptr;
-    std::vector<float> e;
-    std::vector<uint32_t> ne;
-    compute_edges(X, n, F, max_bins, wrap != 0, e, ne);
-    if (edges_out) std::memcpy(edges_out, e.data(), sizeof(float) * e.size());
-    if (n_edges_out) std::memcpy(n_edges_out, ne.data(), sizeof(uint32_t) * ne.size());
-    *out = make_dataset(ctx, X, n, F, max_bins, std::move(e), std::move(ne));
+    auto* ds = new tc_dataset();
+    ds->ctx = ref(ctx);
+    ds->N = n;
+    ds->F = F;
+    ds->max_bins = max_bins;
+    try {
+      TC_HIP(hipSetDevice(ctx->device));
+      quantize_on_device(ds, X, wrap != 0);
+    } catch (...) {
+      unref(ds);
+      throw;
+    }
+    if (edges_out) std::memcpy(edges_out, ds->h_edges.data(), sizeof(float) * ds->h_edges.size());
+    if (n_edges_out)
+      std::memcpy(n_edges_out, ds->h_n_edges.data(), sizeof(uint32_t) * ds->h_n_edges.size());
+    *out = ds;
   });
 }
 

@@ -20,6 +20,10 @@ struct tc_ctx {
   hipStream_t stream = nullptr;
   double last_loop_ms = 0.0, last_hist_ms = 0.0;
   std::atomic<int> refs{1};
+  // two pinned host buffers for streaming rows to the device, made on first use
+  static constexpr size_t kStageBytes = size_t(32) << 20;
+  void* stage[2] = {nullptr, nullptr};
+  std::mutex stage_mu;
 };
 
 struct tc_dataset {
@@ -130,10 +134,13 @@ void serving_end();
 // taken once per device (the device must be current).
 void scratch_setup(int device);
 
-// Host-side edge computation shared by fit and tests (dataset.cpp:278-334).
-void compute_edges(const float* X, uint32_t n, uint32_t F, uint32_t max_bins, bool wrap,
-                   std::vector<float>& edges, std::vector<uint32_t>& n_edges);
-// Bin X (host, row-major) on the device into ds (allocates ds buffers).
+// Pinned staging buffer b (0 or 1) of the context, tc_ctx::kStageBytes.
+void* ctx_stage(tc_ctx* c, int b);
+
+// Edges (dataset.cpp:278-334) and bins of X (host, row-major) on the device,
+// into ds (allocates its buffers; ds->h_edges / h_n_edges get the edges).
+void quantize_on_device(tc_dataset* ds, const float* X, bool wrap);
+// Bin X (host, row-major) with ds->h_edges on the device (allocates ds buffers).
 void bin_on_device(tc_dataset* ds, const float* X);
 
 }  // namespace tc

@@ -1,62 +1,322 @@
-// A2/A3: feature quantization.  Edges are the reference's quantile rule
-// (src/core/dataset.cpp:278-334): per feature the non-NaN values are sorted
-// and de-duplicated (U unique), A = min(max_bins, uint8(U)) when reproducing
-// the reference's uint8 wrap (:308), else min(max_bins, U); edges are the
-// unique values at b*U/A for b = 1..A-1.  Bin = number of edges <= value
-// (upper_bound, :336-344); NaN -> max_bins.  Edge selection runs on the host
-// (one sort per feature, parallel over features); binning every value runs
-// on the GPU and writes both layouts the training kernels read.
+// A2/A3: feature quantization, on the device.  Edges follow the reference's
+// quantile rule (src/core/dataset.cpp:278-334): per feature the non-NaN
+// values are sorted and de-duplicated (U unique), A = min(max_bins,
+// uint8(U)) when reproducing the reference's uint8 wrap (:308), else
+// min(max_bins, U); the edges are the unique values at b*U/A for
+// b = 1..A-1; a feature with no value gets one 0.0 edge (:288-295).  Bin =
+// number of edges <= value (upper_bound, :336-344); NaN -> max_bins.
+//
+// X crosses PCIe once, in row blocks through pinned staging; each block is
+// transposed into column-major order-preserving 32-bit keys (k_keys).  Groups
+// of features are sorted at once as 64-bit keys (feature in the high word,
+// rocPRIM's radix sort), k_edges selects each feature's edges from its sorted
+// run, and k_bin_keys bins every value from the unsorted keys into both bin
+// layouts the training kernels read.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <thread>
+#include <rocprim/device/device_radix_sort.hpp>
 
+#include "tc_copy_pool.h"
 #include "tc_device.h"
 #include "tc_internal.h"
 
 namespace tc {
 
-void compute_edges(const float* X, uint32_t n, uint32_t F, uint32_t max_bins, bool wrap,
-                   std::vector<float>& edges, std::vector<uint32_t>& n_edges) {
-  edges.assign(static_cast<size_t>(F) * 255, 0.f);
-  n_edges.assign(F, 0);
-  const uint8_t nb = static_cast<uint8_t>(max_bins);
-  auto one = [&](uint32_t f, std::vector<float>& vals) {
-    vals.clear();
-    for (uint32_t i = 0; i < n; ++i) {
-      const float v = X[static_cast<size_t>(i) * F + f];
-      if (!std::isnan(v)) vals.push_back(v);
+namespace {
+
+// Order-preserving key of a float: two non-NaN values compare as their keys,
+// with -0 taken as +0 (they compare equal in the reference's sort, unique and
+// upper_bound); NaN is kNanKey, above every value.
+constexpr uint32_t kNanKey = 0xFFFFFFFFu;
+__device__ __forceinline__ uint32_t float_key(float v) {
+  if (isnan(v)) return kNanKey;
+  uint32_t u = __float_as_uint(v);
+  if (u == 0x80000000u) u = 0u;
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_float(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+// Row block [nr][F] -> keys [f * ldk + row], through a 64 x 64 LDS tile.
+__global__ void __launch_bounds__(256) k_keys(const float* __restrict__ X, uint32_t nr, uint32_t F,
+                                              uint32_t* __restrict__ keys, size_t ldk) {
+  __shared__ uint32_t t[64][65];
+  const uint32_t r0 = blockIdx.x * 64u, f0 = blockIdx.y * 64u;
+  const uint32_t tx = threadIdx.x & 63u, ty = threadIdx.x >> 6;
+  for (uint32_t i = ty; i < 64u; i += 4u) {
+    const uint32_t r = r0 + i, f = f0 + tx;
+    t[i][tx] = (r < nr && f < F) ? float_key(X[static_cast<size_t>(r) * F + f]) : 0u;
+  }
+  __syncthreads();
+  for (uint32_t i = ty; i < 64u; i += 4u) {
+    const uint32_t f = f0 + i, r = r0 + tx;
+    if (f < F && r < nr) keys[static_cast<size_t>(f) * ldk + r] = t[tx][i];
+  }
+}
+
+// keys of features [0, g) (f * N + i) -> (f << 32) | key, one run per feature.
+__global__ void __launch_bounds__(256) k_compose(const uint32_t* __restrict__ keys, size_t total,
+                                                 uint32_t N, unsigned long long* __restrict__ out) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * 256u + threadIdx.x;
+  if (i >= total) return;
+  out[i] = (static_cast<unsigned long long>(i / N) << 32) | keys[i];
+}
+
+constexpr uint32_t kEdgeThreads = 1024, kEdgePer = 8, kEdgeTile = kEdgeThreads * kEdgePer;
+
+// Exclusive scan of one value per thread over a 1024-thread workgroup;
+// *total = the sum.  s_w: 32 words.
+__device__ inline uint32_t block_excl_scan_1024(uint32_t v, uint32_t* s_w, uint32_t* total) {
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= static_cast<uint32_t>(o)) x += y;
+  }
+  if (lane == 63u) s_w[w] = x;
+  __syncthreads();
+  if (w == 0) {
+    uint32_t t = lane < 16u ? s_w[lane] : 0u;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+      const uint32_t y = __shfl_up(t, o, 64);
+      if (lane >= static_cast<uint32_t>(o)) t += y;
     }
-    float* e = edges.data() + static_cast<size_t>(f) * 255;
-    if (vals.empty()) {  // all NaN: one 0.0 edge, rows in the NaN bin (:288-295)
+    if (lane < 16u) s_w[16u + lane] = t;
+  }
+  __syncthreads();
+  const uint32_t base = w ? s_w[16u + w - 1u] : 0u;
+  *total = s_w[31];
+  __syncthreads();  // s_w is rewritten by the next call
+  return base + x - v;
+}
+
+// Edges of feature f0 + blockIdx.x from its sorted run (N keys, NaN last):
+// pass 1 counts the unique values U and the NaNs; pass 2 walks the run in
+// tiles, gives every unique value its rank r (0-based), and stores the value
+// of each rank r = floor(b * U / A), b = 1..A-1, as edge b - 1.  Those ranks
+// are distinct (A <= U), so a feature has A - 1 edges (none for A <= 1).
+__global__ void __launch_bounds__(kEdgeThreads)
+    k_edges(const unsigned long long* __restrict__ sorted, uint32_t N, uint32_t f0, uint32_t mb,
+            int wrap, float* __restrict__ edges, uint32_t* __restrict__ n_edges) {
+  __shared__ uint32_t s_k[kEdgeThreads * (kEdgePer + 1)];  // a tile, 9 words per thread
+  __shared__ uint32_t s_w[32];
+  const uint32_t g = blockIdx.x, f = f0 + g, tid = threadIdx.x;
+  const unsigned long long* seg = sorted + static_cast<size_t>(g) * N;
+  auto key = [&](uint32_t i) { return static_cast<uint32_t>(seg[i]); };
+  uint32_t heads = 0, nans = 0;
+  for (uint32_t i = tid; i < N; i += kEdgeThreads) {
+    const uint32_t k = key(i);
+    if (k == kNanKey) ++nans;
+    else if (i == 0 || key(i - 1) != k) ++heads;
+  }
+  uint32_t U, nn;
+  (void)block_excl_scan_1024(heads, s_w, &U);
+  (void)block_excl_scan_1024(nans, s_w, &nn);
+  const uint32_t valid = N - nn;
+  float* e = edges + static_cast<size_t>(f) * 255u;
+  if (valid == 0) {  // no value: one 0.0 edge, every row in the NaN bin
+    if (tid == 0) {
       e[0] = 0.f;
       n_edges[f] = 1;
-      return;
     }
-    std::sort(vals.begin(), vals.end());
-    vals.erase(std::unique(vals.begin(), vals.end()), vals.end());
-    const size_t U = vals.size();
-    const uint32_t ucast = wrap ? static_cast<uint32_t>(static_cast<uint8_t>(U))
-                                : static_cast<uint32_t>(std::min<size_t>(U, 255));
-    const uint32_t A = std::min<uint32_t>(nb, ucast);
-    uint32_t ne = 0;
-    for (uint32_t b = 1; b < A; ++b) {
-      const size_t idx = static_cast<size_t>(b) * U / A;
-      if (ne == 0 || vals[idx] > e[ne - 1]) e[ne++] = vals[idx];
-    }
-    n_edges[f] = ne;
-  };
-  const uint32_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  const uint32_t nt = std::min(hw, std::max(1u, F));
-  std::vector<std::thread> pool;
-  for (uint32_t t = 0; t < nt; ++t) {
-    pool.emplace_back([&, t]() {
-      std::vector<float> vals;
-      vals.reserve(n);
-      for (uint32_t f = t; f < F; f += nt) one(f, vals);
-    });
+    return;
   }
-  for (auto& th : pool) th.join();
+  const uint32_t ucast = wrap ? (U & 0xFFu) : min(U, 255u);
+  const uint32_t A = min(mb, ucast);
+  if (tid == 0) n_edges[f] = A > 0 ? A - 1 : 0;
+  if (A <= 1) return;
+  uint32_t run = 0;  // unique values before the tile
+  for (uint32_t t0 = 0; t0 < valid; t0 += kEdgeTile) {
+    for (uint32_t j = tid; j < kEdgeTile; j += kEdgeThreads) {
+      const uint32_t i = t0 + j;
+      s_k[(j / kEdgePer) * (kEdgePer + 1) + j % kEdgePer] = i < valid ? key(i) : kNanKey;
+    }
+    const uint32_t i0 = t0 + tid * kEdgePer;
+    const uint32_t before = (i0 > 0 && i0 - 1 < valid) ? key(i0 - 1) : kNanKey;
+    __syncthreads();
+    const uint32_t* mine = s_k + tid * (kEdgePer + 1);
+    uint32_t c = 0, prev = before;
+#pragma unroll
+    for (uint32_t q = 0; q < kEdgePer; ++q) {
+      const uint32_t i = i0 + q, k = mine[q];
+      if (i < valid && (i == 0 || k != prev)) ++c;
+      prev = k;
+    }
+    uint32_t tot;
+    uint32_t r = run + block_excl_scan_1024(c, s_w, &tot);
+    prev = before;
+#pragma unroll
+    for (uint32_t q = 0; q < kEdgePer; ++q) {
+      const uint32_t i = i0 + q, k = mine[q];
+      if (i < valid && (i == 0 || k != prev)) {
+        // the b with floor(b * U / A) == r, if any: b = ceil(r * A / U)
+        const unsigned long long ra = static_cast<unsigned long long>(r) * A;
+        const unsigned long long b = (ra + U - 1) / U;
+        if (b >= 1 && b < A && (b * U) / A == r) e[b - 1] = key_float(k);
+        ++r;
+      }
+      prev = k;
+    }
+    run += tot;
+    __syncthreads();  // the tile's LDS is rewritten next
+  }
+}
+
+// One thread per row, 16 features per workgroup column: the features' edges
+// as keys in LDS, upper_bound of each key, one 16-byte store into the
+// row-major bins and 16 coalesced byte stores into the column-major bins.
+__global__ void __launch_bounds__(256) k_bin_keys(const uint32_t* __restrict__ keys, uint32_t N,
+                                                  uint32_t F, uint32_t Fp,
+                                                  const float* __restrict__ edges,
+                                                  const uint32_t* __restrict__ n_edges,
+                                                  uint8_t nan_bin, uint8_t* __restrict__ bins_rm,
+                                                  uint8_t* __restrict__ bins_cm) {
+  __shared__ uint32_t s_e[kTileF][256];
+  __shared__ uint32_t s_n[kTileF];
+  const uint32_t f0 = blockIdx.y * kTileF;
+  for (uint32_t i = threadIdx.x; i < kTileF * 255; i += blockDim.x) {
+    const uint32_t k = i / 255, j = i % 255;
+    s_e[k][j] = (f0 + k < F) ? float_key(edges[static_cast<size_t>(f0 + k) * 255 + j]) : 0u;
+  }
+  if (threadIdx.x < kTileF) s_n[threadIdx.x] = (f0 + threadIdx.x < F) ? n_edges[f0 + threadIdx.x] : 0;
+  __syncthreads();
+  const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= N) return;
+  uint32_t packed[4] = {0, 0, 0, 0};
+  for (uint32_t k = 0; k < kTileF; ++k) {
+    const uint32_t f = f0 + k;
+    if (f >= F) break;
+    const uint32_t v = keys[static_cast<size_t>(f) * N + row];
+    uint32_t b;
+    if (v == kNanKey) {
+      b = nan_bin;
+    } else {
+      uint32_t lo = 0, hi = s_n[k];
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (v < s_e[k][mid]) hi = mid; else lo = mid + 1;
+      }
+      b = lo;
+    }
+    packed[k >> 2] |= b << (8 * (k & 3));
+    bins_cm[static_cast<size_t>(f) * N + row] = static_cast<uint8_t>(b);
+  }
+  *reinterpret_cast<uint4*>(bins_rm + static_cast<size_t>(row) * Fp + f0) =
+      make_uint4(packed[0], packed[1], packed[2], packed[3]);
+}
+
+// Features sorted together: about 2^26 keys (512 MB of 64-bit keys and as
+// much again for the sort's output).
+constexpr size_t kSortKeys = size_t(1) << 26;
+
+uint32_t bits_for(uint32_t v) {  // bits to hold 0..v
+  uint32_t b = 0;
+  while (b < 32 && (v >> b) != 0) ++b;
+  return b;
+}
+
+}  // namespace
+
+void quantize_on_device(tc_dataset* ds, const float* X, bool wrap) {
+  const uint32_t N = ds->N, F = ds->F;
+  hipStream_t s = ds->ctx->stream;
+  std::lock_guard<std::mutex> stage_lock(ds->ctx->stage_mu);  // one stream of rows at a time
+  ds->Fp = rm_stride(F);
+  ds->bins_rm = dmalloc<uint8_t>(static_cast<size_t>(N) * ds->Fp);
+  ds->bins_cm = dmalloc<uint8_t>(static_cast<size_t>(N) * F);
+  ds->edges = dmalloc<float>(static_cast<size_t>(F) * 255);
+  ds->n_edges = dmalloc<uint32_t>(F);
+  ds->h_edges.assign(static_cast<size_t>(F) * 255, 0.f);
+  ds->h_n_edges.assign(F, 0);
+  if (N == 0) {  // no rows: every feature is empty (one 0.0 edge)
+    std::fill(ds->h_n_edges.begin(), ds->h_n_edges.end(), 1u);
+    copy_sync(ds->edges, ds->h_edges.data(), 4ull * F * 255, hipMemcpyHostToDevice, s);
+    copy_sync(ds->n_edges, ds->h_n_edges.data(), 4ull * F, hipMemcpyHostToDevice, s);
+    return;
+  }
+  uint32_t* keys = nullptr;
+  unsigned long long *kin = nullptr, *kout = nullptr;
+  void* temp = nullptr;
+  float* dev[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  auto release = [&]() {
+    (void)hipStreamSynchronize(s);
+    for (int b = 0; b < 2; ++b) {
+      if (ev[b]) (void)hipEventDestroy(ev[b]);
+      dfree(dev[b]);
+    }
+    dfree(keys);
+    dfree(kin);
+    dfree(kout);
+    dfree(temp);
+  };
+  try {
+    keys = dmalloc<uint32_t>(static_cast<size_t>(N) * F);
+    // stream X in row blocks: the host copy of block k + 1 into pinned
+    // staging overlaps the transfer and transposition of block k
+    const size_t row_bytes = 4ull * F;
+    const uint32_t blk = static_cast<uint32_t>(
+        std::max<size_t>(1, std::min<size_t>(N, tc_ctx::kStageBytes / row_bytes)));
+    TC_REQUIRE(row_bytes <= tc_ctx::kStageBytes, "quantize: a row exceeds the staging buffer");
+    float* pin[2];
+    for (int b = 0; b < 2; ++b) {
+      pin[b] = reinterpret_cast<float*>(ctx_stage(ds->ctx, b));
+      dev[b] = dmalloc<float>(static_cast<size_t>(blk) * F);
+      TC_HIP(hipEventCreateWithFlags(&ev[b], hipEventDisableTiming));
+    }
+    uint32_t k = 0;
+    for (uint32_t r0 = 0; r0 < N; r0 += blk, ++k) {
+      const uint32_t b = k & 1u, nr = std::min(blk, N - r0);
+      if (k >= 2) TC_HIP(hipEventSynchronize(ev[b]));  // block k - 2 is done with the buffers
+      copy_pool().copy(split_copy(pin[b], X + static_cast<size_t>(r0) * F, row_bytes * nr));
+      TC_HIP(hipMemcpyAsync(dev[b], pin[b], row_bytes * nr, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_keys, dim3((nr + 63) / 64, (F + 63) / 64), dim3(256), 0, s, dev[b], nr, F,
+                         keys + r0, static_cast<size_t>(N));
+      TC_HIP(hipGetLastError());
+      TC_HIP(hipEventRecord(ev[b], s));
+    }
+    // edges, a group of features at a time
+    const uint32_t G = static_cast<uint32_t>(
+        std::max<size_t>(1, std::min<size_t>(F, kSortKeys / std::max<uint32_t>(N, 1))));
+    const size_t cap = static_cast<size_t>(G) * N;
+    kin = dmalloc<unsigned long long>(cap);
+    kout = dmalloc<unsigned long long>(cap);
+    size_t temp_bytes = 0;
+    TC_HIP(rocprim::radix_sort_keys(nullptr, temp_bytes, kin, kout, cap, 0u, 32u + bits_for(G - 1),
+                                    s));
+    temp = dmalloc<uint8_t>(std::max<size_t>(1, temp_bytes));
+    TC_HIP(hipMemsetAsync(ds->edges, 0, 4ull * F * 255, s));
+    for (uint32_t f0 = 0; f0 < F; f0 += G) {
+      const uint32_t g = std::min(G, F - f0);
+      const size_t total = static_cast<size_t>(g) * N;
+      hipLaunchKernelGGL(k_compose, dim3(static_cast<uint32_t>((total + 255) / 256)), dim3(256), 0, s,
+                         keys + static_cast<size_t>(f0) * N, total, N, kin);
+      TC_HIP(hipGetLastError());
+      size_t need = 0;
+      const uint32_t end_bit = 32u + bits_for(g - 1);
+      TC_HIP(rocprim::radix_sort_keys(nullptr, need, kin, kout, total, 0u, end_bit, s));
+      TC_REQUIRE(need <= temp_bytes, "quantize: sort scratch smaller than the sort needs");
+      TC_HIP(rocprim::radix_sort_keys(temp, need, kin, kout, total, 0u, end_bit, s));
+      hipLaunchKernelGGL(k_edges, dim3(g), dim3(kEdgeThreads), 0, s, kout, N, f0, ds->max_bins & 0xFFu,
+                         wrap ? 1 : 0, ds->edges, ds->n_edges);
+      TC_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_bin_keys, dim3((N + 255) / 256, feature_tiles(F)), dim3(256), 0, s, keys, N,
+                       F, ds->Fp, ds->edges, ds->n_edges, static_cast<uint8_t>(ds->max_bins),
+                       ds->bins_rm, ds->bins_cm);
+    TC_HIP(hipGetLastError());
+    TC_HIP(hipMemcpyAsync(ds->h_edges.data(), ds->edges, 4ull * F * 255, hipMemcpyDeviceToHost, s));
+    TC_HIP(hipMemcpyAsync(ds->h_n_edges.data(), ds->n_edges, 4ull * F, hipMemcpyDeviceToHost, s));
+    TC_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    release();
+    throw;
+  }
+  release();
 }
 
 // One thread per row, 16 features per workgroup column (blockIdx.y): edges of
