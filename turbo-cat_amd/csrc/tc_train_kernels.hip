@@ -881,11 +881,10 @@ __device__ bool level_plan(const JobDev& J, const Lists& L, uint32_t job, uint32
 // takes its own item, and workgroup 0 stores the plan's outputs for the
 // later launches (the plan costs less than the launch it replaces).
 template <int U, bool PACKED, bool SINGLE, bool PLAN>
-__global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict__ jobs,
-                                                       const HistWork* __restrict__ work,
-                                                       const uint32_t* __restrict__ count,
-                                                       uint32_t level,
-                                                       HistCell* __restrict__ partials, Lists L) {
+__device__ __forceinline__ void hist_item(const JobDev* __restrict__ jobs,
+                                          const HistWork* __restrict__ work,
+                                          const uint32_t* __restrict__ count, uint32_t level,
+                                          HistCell* __restrict__ partials, const Lists& L) {
   constexpr int kCells = kTileF * 256;
   __shared__ unsigned long long s_a[kCells];
   __shared__ unsigned long long s_b[PACKED ? 1 : kCells];
@@ -1021,6 +1020,29 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(const JobDev* __restrict_
       atomicAdd(reinterpret_cast<unsigned long long*>(&o->h), static_cast<unsigned long long>(hs));
     }
   }
+}
+
+// Listed items (the root, batches; several per CU).  Two 1024-thread
+// workgroups share a CU only while a wave holds at most about 78 SGPRs (64
+// VGPRs, 64 KB LDS each; measured on gfx950 -- above that one runs at a time
+// although the compiler's occupancy estimate still reads eight waves per
+// SIMD), so the registers are capped here.
+template <int U, bool PACKED, bool SINGLE>
+__global__ void __launch_bounds__(kHistThreads) __attribute__((amdgpu_num_sgpr(76), amdgpu_waves_per_eu(8)))
+    k_hist(const JobDev* __restrict__ jobs, const HistWork* __restrict__ work,
+           const uint32_t* __restrict__ count, uint32_t level, HistCell* __restrict__ partials,
+           Lists L) {
+  hist_item<U, PACKED, SINGLE, false>(jobs, work, count, level, partials, L);
+}
+
+// One job below the root: the level planned in the kernel, about one item per
+// CU (no cap: a capped variant spills in its main loop).
+template <int U, bool PACKED>
+__global__ void __launch_bounds__(kHistThreads)
+    k_hist_plan(const JobDev* __restrict__ jobs, const HistWork* __restrict__ work,
+                const uint32_t* __restrict__ count, uint32_t level, HistCell* __restrict__ partials,
+                Lists L) {
+  hist_item<U, PACKED, true, true>(jobs, work, count, level, partials, L);
 }
 
 // ---------------------------------------------------------------------------
