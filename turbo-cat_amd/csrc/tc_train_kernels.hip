@@ -1082,12 +1082,17 @@ __global__ void __launch_bounds__(TH) k_split_scan(const JobDev* __restrict__ jo
   constexpr uint32_t kTasks = NP / 2;  // histogrammed units per pass
   static_assert(!FIXED || kTasks <= NW, "one wave per task");
   static_assert(FIXED ? (TH == 1024u || TH == 512u) : TH == 256u, "thread counts");
-  // s_pref: the pass's left sums per unit and candidate.  FIXED: s_acc (the
+  // s_pref: the pass's left sums per unit and candidate.  FIXED: s_accg/h (the
   // tasks' summed cells) shares its 64 KB; each task's cells are in
   // registers before the first prefix is written.
   __shared__ __align__(16) unsigned char s_raw[NP * 256 * sizeof(float2)];
   float2(*s_pref)[256] = reinterpret_cast<float2(*)[256]>(s_raw);
-  unsigned long long(*s_acc)[256][2] = reinterpret_cast<unsigned long long(*)[256][2]>(s_raw);
+  // the tasks' summed cells, one plane for g and one for h ([task][bin] each):
+  // lanes add consecutive bins, so a wave's 64-bit atomics fall on 32
+  // different bank pairs (interleaved (g, h) cells at four bins per lane put
+  // them on 4 and stalled 70% of the LDS cycles in bank conflicts)
+  unsigned long long* s_accg = reinterpret_cast<unsigned long long*>(s_raw);
+  unsigned long long* s_acch = s_accg + kTasks * 256;
   static_assert(kTasks * 256 * 2 * sizeof(unsigned long long) <= sizeof(s_raw), "task cells fit");
   __shared__ unsigned long long s_red[4];
   __shared__ float s_chain[256];
@@ -1152,7 +1157,10 @@ __global__ void __launch_bounds__(TH) k_split_scan(const JobDev* __restrict__ jo
         s_cpre[tid + 1] = empty ? 0u : 1u + (store ? (rows - 1) / chunk : 0u);
         s_empty[tid] = empty ? 1u : 0u;
       }
-      for (uint32_t i = tid; i < nt * 512u; i += TH) (&s_acc[0][0][0])[i] = 0ull;
+      for (uint32_t i = tid; i < nt * 256u; i += TH) {
+        s_accg[i] = 0ull;
+        s_acch[i] = 0ull;
+      }
       __syncthreads();
       if (!active) return;  // the job's tree is done (uniform)
       if (tid == 0) {
@@ -1169,14 +1177,14 @@ __global__ void __launch_bounds__(TH) k_split_scan(const JobDev* __restrict__ jo
       __syncthreads();
       const uint32_t ne = s_cpre[nt];
       const uint32_t tile = f / kTileF, fi = f % kTileF, TK = J.T * K;
-      auto source = [&](uint32_t e, uint32_t* t_out) {  // lane's 4 cells of source e
+      auto source = [&](uint32_t e, uint32_t* t_out) {  // lane's first cell of source e
         uint32_t t = 0;
         while (s_cpre[t + 1] <= e) ++t;
         *t_out = t;
         const uint32_t j = e - s_cpre[t];
-        if (j == 0) return static_cast<const HistCell*>(cells(level, s_sm[t], t % K) + 4u * lane);
+        if (j == 0) return static_cast<const HistCell*>(cells(level, s_sm[t], t % K) + lane);
         const uint32_t item = s_first[t] + (j - 1) * TK + tile * K + t % K;
-        return partials + (static_cast<size_t>(item) * kTileF + fi) * 256u + 4u * lane;
+        return partials + (static_cast<size_t>(item) * kTileF + fi) * 256u + lane;
       };
       constexpr uint32_t kInFlight = 2;  // sources loaded per wave before adding
       for (uint32_t e0 = wave; e0 < ne; e0 += kInFlight * NW) {
@@ -1187,7 +1195,7 @@ __global__ void __launch_bounds__(TH) k_split_scan(const JobDev* __restrict__ jo
           if (e0 + d * NW < ne) {
             const HistCell* p = source(e0 + d * NW, &tt[d]);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) v[d][i] = p[i];
+            for (int i = 0; i < 4; ++i) v[d][i] = p[64 * i];  // bins lane + 64 i
           }
         }
 #pragma unroll
@@ -1195,8 +1203,9 @@ __global__ void __launch_bounds__(TH) k_split_scan(const JobDev* __restrict__ jo
           if (e0 + d * NW < ne) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-              atomicAdd(&s_acc[tt[d]][4 * lane + i][0], static_cast<unsigned long long>(v[d][i].g));
-              atomicAdd(&s_acc[tt[d]][4 * lane + i][1], static_cast<unsigned long long>(v[d][i].h));
+              const uint32_t c = tt[d] * 256u + lane + 64u * i;
+              atomicAdd(&s_accg[c], static_cast<unsigned long long>(v[d][i].g));
+              atomicAdd(&s_acch[c], static_cast<unsigned long long>(v[d][i].h));
             }
           }
         }
@@ -1227,10 +1236,10 @@ __global__ void __launch_bounds__(TH) k_split_scan(const JobDev* __restrict__ jo
       if (has_task) {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-          cs_[i] = HistCell{static_cast<long long>(s_acc[t][4 * lane + i][0]),
-                            static_cast<long long>(s_acc[t][4 * lane + i][1])};
+          cs_[i] = HistCell{static_cast<long long>(s_accg[t * 256u + 4 * lane + i]),
+                            static_cast<long long>(s_acch[t * 256u + 4 * lane + i])};
       }
-      __syncthreads();  // s_acc's space becomes s_pref
+      __syncthreads();  // the cells' space becomes s_pref
       if (has_task) {
         const uint32_t ch = t % K, nl = task_left(t), sm = s_sm[t];
         HistCell* hs = cells(level, sm, ch);
