@@ -251,17 +251,28 @@ def run_serve(args):
     from turbocat import capi
     F, T, D = 200, args.serve_trees, 8
     rng = np.random.default_rng(3003)
-    log(f"[bench] training the serving ensemble ({args.serve_train_rows} rows, {T} trees depth {D})")
-    X, y = binary_workload(args.serve_train_rows, F, seed=1003, n_uniform=40)
     ctx = capi.Context(0)
-    ds, _, _ = ctx.quantize_fit(X)
-    tr = capi.Trainer(ctx, [ds], [y], capi.train_params(capi.TC_LOSS_LOGLOSS, n_trees=T,
-                                                         learning_rate=0.1, max_depth=D,
-                                                         lambda_l2=1.0))
-    tr.step(T)
-    tr.sync()
-    model = tr.model()
-    tr.close()
+    if args.serve_model == "random":
+        # a synthetic ensemble of the same shape (every tree full depth, random
+        # splits): the same inference cost, no training in the process
+        log(f"[bench] synthetic serving ensemble ({T} trees depth {D})")
+        mr = np.random.default_rng(1003)
+        model = dict(T=T, D=D, feature=mr.integers(0, F, T * D).astype(np.uint16),
+                     fthr=mr.standard_normal(T * D).astype(np.float32),
+                     bin=np.zeros(T * D, np.uint8), depth=np.full(T, D, np.uint16),
+                     leaves=(mr.standard_normal(T << D) * 0.05).astype(np.float32),
+                     weight=np.full(T, 0.1, np.float32), base=0.0)
+    else:
+        log(f"[bench] training the serving ensemble ({args.serve_train_rows} rows, {T} trees depth {D})")
+        X, y = binary_workload(args.serve_train_rows, F, seed=1003, n_uniform=40)
+        ds, _, _ = ctx.quantize_fit(X)
+        tr = capi.Trainer(ctx, [ds], [y], capi.train_params(capi.TC_LOSS_LOGLOSS, n_trees=T,
+                                                             learning_rate=0.1, max_depth=D,
+                                                             lambda_l2=1.0))
+        tr.step(T)
+        tr.sync()
+        model = tr.model()
+        tr.close()
     ens = ctx.ensemble(model, F)
     n_req = args.requests
     sizes = np.floor(np.exp(rng.uniform(0, np.log(65536), n_req))).astype(int)
@@ -347,7 +358,7 @@ def run_serve(args):
         "config": {"workload": f"serving: {T} symmetric trees depth {D} over {F} features, "
                                f"request rows log-uniform [1, 65536]; latency at Poisson "
                                f"{args.qps} QPS, throughput with every request queued at once",
-                   "requests": n_req, "qps": args.qps},
+                   "requests": n_req, "qps": args.qps, "model": args.serve_model},
         "extra": {"served_p50_ms": p(lat_s, 50), "served_p99_ms": p(lat_s, 99),
                   "plain_p50_ms": p(lat_p, 50), "plain_p99_ms": p(lat_p, 99),
                   "served_internal_p50_ms": p(int_s, 50), "served_internal_p99_ms": p(int_s, 99),
@@ -585,6 +596,8 @@ def main():
     ap.add_argument("--requests", type=int, default=400)
     ap.add_argument("--serve-trees", type=int, default=1000)
     ap.add_argument("--serve-train-rows", type=int, default=500_000)
+    ap.add_argument("--serve-model", default="trained", choices=["trained", "random"],
+                    help="serve workload: a trained ensemble, or a synthetic one of the same shape")
     ap.add_argument("--max-batch-rows", type=int, default=32768)
     ap.add_argument("--streams", type=int, default=4)
     ap.add_argument("--jobs", type=int, default=64)

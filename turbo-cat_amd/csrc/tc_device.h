@@ -94,6 +94,8 @@ __device__ inline uint32_t lane63(uint32_t v) {
 __device__ inline uint32_t wave_incl_scan_u32(uint32_t v) {
   return wave_scan_u32(v, [](uint32_t a, uint32_t b) { return a + b; });
 }
+// Sum over the wave, in every lane.
+__device__ inline uint32_t wave_sum_u32(uint32_t v) { return lane63(wave_incl_scan_u32(v)); }
 // Max over the wave, in every lane.
 __device__ inline uint32_t wave_max_u32(uint32_t v) {
   return lane63(wave_scan_u32(v, [](uint32_t a, uint32_t b) { return a > b ? a : b; }));

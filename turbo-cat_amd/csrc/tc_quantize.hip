@@ -64,61 +64,52 @@ __global__ void __launch_bounds__(256) k_compose(const uint32_t* __restrict__ ke
   out[i] = (static_cast<unsigned long long>(i / N) << 32) | keys[i];
 }
 
-constexpr uint32_t kEdgeThreads = 1024, kEdgePer = 8, kEdgeTile = kEdgeThreads * kEdgePer;
+constexpr uint32_t kEdgeThreads = 1024, kEdgeWaves = kEdgeThreads / 64;
 
-// Exclusive scan of one value per thread over a 1024-thread workgroup;
-// *total = the sum.  s_w: 32 words.
-__device__ inline uint32_t block_excl_scan_1024(uint32_t v, uint32_t* s_w, uint32_t* total) {
-  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-  uint32_t x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t y = __shfl_up(x, o, 64);
-    if (lane >= static_cast<uint32_t>(o)) x += y;
-  }
-  if (lane == 63u) s_w[w] = x;
-  __syncthreads();
-  if (w == 0) {
-    uint32_t t = lane < 16u ? s_w[lane] : 0u;
-#pragma unroll
-    for (int o = 1; o < 16; o <<= 1) {
-      const uint32_t y = __shfl_up(t, o, 64);
-      if (lane >= static_cast<uint32_t>(o)) t += y;
-    }
-    if (lane < 16u) s_w[16u + lane] = t;
-  }
-  __syncthreads();
-  const uint32_t base = w ? s_w[16u + w - 1u] : 0u;
-  *total = s_w[31];
-  __syncthreads();  // s_w is rewritten by the next call
-  return base + x - v;
-}
-
-// Edges of feature f0 + blockIdx.x from its sorted run (N keys, NaN last):
-// pass 1 counts the unique values U and the NaNs; pass 2 walks the run in
-// tiles, gives every unique value its rank r (0-based), and stores the value
-// of each rank r = floor(b * U / A), b = 1..A-1, as edge b - 1.  Those ranks
-// are distinct (A <= U), so a feature has A - 1 edges (none for A <= 1).
+// Edges of feature f0 + blockIdx.x from its sorted run (N keys, NaN last).
+// Wave w owns the contiguous range [w R, (w + 1) R) of the run.  Pass 1
+// counts each range's unique values (heads: a non-NaN key other than its
+// predecessor) and NaNs; their prefix gives every wave the rank of its first
+// head, and U.  The selected ranks floor(b U / A), b = 1..A-1, are distinct
+// (A <= U) and increasing; pass 2 walks each range 64 keys at a time, ranks
+// the heads with a ballot and stores the value of each selected rank as edge
+// b - 1.  So a feature has A - 1 edges (none for A <= 1).
 __global__ void __launch_bounds__(kEdgeThreads)
     k_edges(const unsigned long long* __restrict__ sorted, uint32_t N, uint32_t f0, uint32_t mb,
             int wrap, float* __restrict__ edges, uint32_t* __restrict__ n_edges) {
-  __shared__ uint32_t s_k[kEdgeThreads * (kEdgePer + 1)];  // a tile, 9 words per thread
-  __shared__ uint32_t s_w[32];
+  __shared__ uint32_t s_heads[kEdgeWaves], s_nans[kEdgeWaves];
+  __shared__ uint32_t s_sel[256];  // selected ranks
   const uint32_t g = blockIdx.x, f = f0 + g, tid = threadIdx.x;
+  const uint32_t lane = tid & 63u, w = tid >> 6;
   const unsigned long long* seg = sorted + static_cast<size_t>(g) * N;
   auto key = [&](uint32_t i) { return static_cast<uint32_t>(seg[i]); };
+  const uint32_t R = (N + kEdgeWaves - 1) / kEdgeWaves;
+  const uint32_t lo = min(N, w * R), hi = min(N, lo + R);
+  auto head = [&](uint32_t i) {  // i in [lo, hi)
+    const uint32_t k = key(i);
+    return k != kNanKey && (i == 0 || key(i - 1) != k);
+  };
   uint32_t heads = 0, nans = 0;
-  for (uint32_t i = tid; i < N; i += kEdgeThreads) {
+  for (uint32_t i = lo + lane; i < hi; i += 64) {
     const uint32_t k = key(i);
     if (k == kNanKey) ++nans;
     else if (i == 0 || key(i - 1) != k) ++heads;
   }
-  uint32_t U, nn;
-  (void)block_excl_scan_1024(heads, s_w, &U);
-  (void)block_excl_scan_1024(nans, s_w, &nn);
-  const uint32_t valid = N - nn;
+  heads = wave_sum_u32(heads);
+  nans = wave_sum_u32(nans);
+  if (lane == 0) {
+    s_heads[w] = heads;
+    s_nans[w] = nans;
+  }
+  __syncthreads();
+  uint32_t U = 0, nn = 0, first = 0;
+  for (uint32_t v = 0; v < kEdgeWaves; ++v) {
+    if (v == w) first = U;
+    U += s_heads[v];
+    nn += s_nans[v];
+  }
   float* e = edges + static_cast<size_t>(f) * 255u;
-  if (valid == 0) {  // no value: one 0.0 edge, every row in the NaN bin
+  if (nn == N) {  // no value: one 0.0 edge, every row in the NaN bin
     if (tid == 0) {
       e[0] = 0.f;
       n_edges[f] = 1;
@@ -129,40 +120,25 @@ __global__ void __launch_bounds__(kEdgeThreads)
   const uint32_t A = min(mb, ucast);
   if (tid == 0) n_edges[f] = A > 0 ? A - 1 : 0;
   if (A <= 1) return;
-  uint32_t run = 0;  // unique values before the tile
-  for (uint32_t t0 = 0; t0 < valid; t0 += kEdgeTile) {
-    for (uint32_t j = tid; j < kEdgeTile; j += kEdgeThreads) {
-      const uint32_t i = t0 + j;
-      s_k[(j / kEdgePer) * (kEdgePer + 1) + j % kEdgePer] = i < valid ? key(i) : kNanKey;
+  if (tid + 1 < A)
+    s_sel[tid] = static_cast<uint32_t>(static_cast<unsigned long long>(tid + 1) * U / A);
+  __syncthreads();
+  // the first selected rank at or above the wave's first head
+  uint32_t bc = 0;
+  while (bc + 1 < A && s_sel[bc] < first) ++bc;
+  uint32_t r0 = first;
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (uint32_t base = lo; base < hi && bc + 1 < A; base += 64) {
+    const uint32_t i = base + lane;
+    const bool h = i < hi && head(i);
+    const unsigned long long m = __ballot(h);
+    const uint32_t r = r0 + static_cast<uint32_t>(__popcll(m & below));
+    const uint32_t r1 = r0 + static_cast<uint32_t>(__popcll(m));
+    while (bc + 1 < A && s_sel[bc] < r1) {  // selected ranks in this step (uniform)
+      if (h && r == s_sel[bc]) e[bc] = key_float(key(i));
+      ++bc;
     }
-    const uint32_t i0 = t0 + tid * kEdgePer;
-    const uint32_t before = (i0 > 0 && i0 - 1 < valid) ? key(i0 - 1) : kNanKey;
-    __syncthreads();
-    const uint32_t* mine = s_k + tid * (kEdgePer + 1);
-    uint32_t c = 0, prev = before;
-#pragma unroll
-    for (uint32_t q = 0; q < kEdgePer; ++q) {
-      const uint32_t i = i0 + q, k = mine[q];
-      if (i < valid && (i == 0 || k != prev)) ++c;
-      prev = k;
-    }
-    uint32_t tot;
-    uint32_t r = run + block_excl_scan_1024(c, s_w, &tot);
-    prev = before;
-#pragma unroll
-    for (uint32_t q = 0; q < kEdgePer; ++q) {
-      const uint32_t i = i0 + q, k = mine[q];
-      if (i < valid && (i == 0 || k != prev)) {
-        // the b with floor(b * U / A) == r, if any: b = ceil(r * A / U)
-        const unsigned long long ra = static_cast<unsigned long long>(r) * A;
-        const unsigned long long b = (ra + U - 1) / U;
-        if (b >= 1 && b < A && (b * U) / A == r) e[b - 1] = key_float(k);
-        ++r;
-      }
-      prev = k;
-    }
-    run += tot;
-    __syncthreads();  // the tile's LDS is rewritten next
+    r0 = r1;
   }
 }
 
