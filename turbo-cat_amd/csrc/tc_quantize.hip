@@ -85,15 +85,25 @@ __global__ void __launch_bounds__(kEdgeThreads)
   auto key = [&](uint32_t i) { return static_cast<uint32_t>(seg[i]); };
   const uint32_t R = (N + kEdgeWaves - 1) / kEdgeWaves;
   const uint32_t lo = min(N, w * R), hi = min(N, lo + R);
-  auto head = [&](uint32_t i) {  // i in [lo, hi)
-    const uint32_t k = key(i);
-    return k != kNanKey && (i == 0 || key(i - 1) != k);
-  };
+  // eight steps of 64 keys (and their predecessors) in flight per wave
+  constexpr uint32_t kSteps = 8;
   uint32_t heads = 0, nans = 0;
-  for (uint32_t i = lo + lane; i < hi; i += 64) {
-    const uint32_t k = key(i);
-    if (k == kNanKey) ++nans;
-    else if (i == 0 || key(i - 1) != k) ++heads;
+  for (uint32_t b0 = lo; b0 < hi; b0 += 64 * kSteps) {
+    uint32_t k[kSteps], kp[kSteps];
+#pragma unroll
+    for (uint32_t q = 0; q < kSteps; ++q) {
+      const uint32_t i = b0 + 64 * q + lane;
+      k[q] = i < hi ? key(i) : kNanKey;
+      kp[q] = (i < hi && i > 0) ? key(i - 1) : kNanKey;
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < kSteps; ++q) {
+      const uint32_t i = b0 + 64 * q + lane;
+      if (i < hi) {
+        if (k[q] == kNanKey) ++nans;
+        else if (i == 0 || kp[q] != k[q]) ++heads;
+      }
+    }
   }
   heads = wave_sum_u32(heads);
   nans = wave_sum_u32(nans);
@@ -128,17 +138,27 @@ __global__ void __launch_bounds__(kEdgeThreads)
   while (bc + 1 < A && s_sel[bc] < first) ++bc;
   uint32_t r0 = first;
   const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (uint32_t base = lo; base < hi && bc + 1 < A; base += 64) {
-    const uint32_t i = base + lane;
-    const bool h = i < hi && head(i);
-    const unsigned long long m = __ballot(h);
-    const uint32_t r = r0 + static_cast<uint32_t>(__popcll(m & below));
-    const uint32_t r1 = r0 + static_cast<uint32_t>(__popcll(m));
-    while (bc + 1 < A && s_sel[bc] < r1) {  // selected ranks in this step (uniform)
-      if (h && r == s_sel[bc]) e[bc] = key_float(key(i));
-      ++bc;
+  for (uint32_t b0 = lo; b0 < hi && bc + 1 < A; b0 += 64 * kSteps) {
+    uint32_t k[kSteps], kp[kSteps];
+#pragma unroll
+    for (uint32_t q = 0; q < kSteps; ++q) {
+      const uint32_t i = b0 + 64 * q + lane;
+      k[q] = i < hi ? key(i) : kNanKey;
+      kp[q] = (i < hi && i > 0) ? key(i - 1) : kNanKey;
     }
-    r0 = r1;
+#pragma unroll
+    for (uint32_t q = 0; q < kSteps; ++q) {
+      const uint32_t i = b0 + 64 * q + lane;
+      const bool h = i < hi && k[q] != kNanKey && (i == 0 || kp[q] != k[q]);
+      const unsigned long long m = __ballot(h);
+      const uint32_t r = r0 + static_cast<uint32_t>(__popcll(m & below));
+      const uint32_t r1 = r0 + static_cast<uint32_t>(__popcll(m));
+      while (bc + 1 < A && s_sel[bc] < r1) {  // selected ranks in this step (uniform)
+        if (h && r == s_sel[bc]) e[bc] = key_float(k[q]);
+        ++bc;
+      }
+      r0 = r1;
+    }
   }
 }
 
