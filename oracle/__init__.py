@@ -57,6 +57,31 @@ def tree_params(max_depth=6, max_bins=255, lambda_l2=1.0, lambda_l1=0.0, max_del
                       min_samples_leaf, min_child_weight, min_split_gain)
 
 
+class RTreeParams(ctypes.Structure):
+    """TreeConfig fields of the regular depthwise tree (config.hpp:26-59)."""
+    _fields_ = [
+        ("max_depth", c_uint32),
+        ("max_leaves", c_uint32),
+        ("max_bins", c_uint32),
+        ("min_samples_leaf", c_uint32),
+        ("min_child_weight", c_float),
+        ("lambda_l2", c_float),
+        ("min_split_gain", c_float),
+        ("gamma", c_float),
+        ("leaf_smooth", c_float),
+        ("max_delta_step", c_float),
+        ("learn_missing_direction", c_int32),
+    ]
+
+
+def rtree_params(max_depth=6, max_leaves=64, max_bins=255, min_samples_leaf=1,
+                 min_child_weight=1.0, lambda_l2=1.0, min_split_gain=0.0, gamma=0.0,
+                 leaf_smooth=0.0, max_delta_step=0.0, learn_missing_direction=True) -> RTreeParams:
+    return RTreeParams(max_depth, max_leaves, max_bins, min_samples_leaf, min_child_weight,
+                       lambda_l2, min_split_gain, gamma, leaf_smooth, max_delta_step,
+                       int(learn_missing_direction))
+
+
 class Oracle:
     def __init__(self, path: str = LIB_ORACLE):
         if not os.path.exists(path):
@@ -104,6 +129,13 @@ class Oracle:
         L.tco_early_stop.restype = c_uint32
         for fn in ("tco_tree_build_f32", "tco_tree_build_f64", "tco_tree_build_mc_f64"):
             getattr(L, fn).restype = c_int
+        L.tco_rtree_build_f64.argtypes = [V, c_uint32, c_uint32, V, V, V, c_uint32,
+                                          POINTER(RTreeParams), c_uint32, V, V, V, V, V, V, V,
+                                          V, V, V, V]
+        L.tco_rtree_build_f64.restype = c_int
+        L.tco_rtree_apply_binned.argtypes = [V, V, V, V, V, V, c_uint32, V, c_uint32, V]
+        L.tco_rpredict_f32.argtypes = [V, V, V, V, V, V, V, V, V, c_uint32, c_uint32, c_float, V,
+                                       c_uint32, c_uint32, V, V, c_int, V]
 
     # ---------------------------------------------------------------- A19
     def compute_loss_f32(self, loss, y, p):
@@ -333,6 +365,68 @@ class Oracle:
         m["base"] = float(base)
         m["weight"][:] = lr
         return m, tp
+
+    # ------------------------------------------------------- regular trees
+    def rtree_f64(self, bins, g, h, params: RTreeParams, rows=None):
+        """One regular depthwise tree (Tree::build, tree.cpp:162-217) in the
+        reference's node numbering; FP64 sums.  Returns a dict of node arrays
+        (flags bit 0 is_leaf, bit 1 default_left) plus n_leaves and depth."""
+        F, n = bins.shape
+        cap = (1 << (params.max_depth + 1)) + 1
+        feat = np.zeros(cap, np.uint16); b = np.zeros(cap, np.uint8); fl = np.zeros(cap, np.uint8)
+        le = np.zeros(cap, np.uint32); ri = np.zeros(cap, np.uint32); val = np.zeros(cap)
+        gn = np.zeros(cap); gap = np.zeros(cap)
+        nn = c_uint32(); nl = c_uint32(); dp = ctypes.c_uint16()
+        r = None if rows is None else np.ascontiguousarray(rows, np.uint32)
+        rc = self.L.tco_rtree_build_f64(_p(np.ascontiguousarray(bins, np.uint8)), n, F,
+                                        _p(np.ascontiguousarray(g, np.float32)),
+                                        _p(np.ascontiguousarray(h, np.float32)), _p(r),
+                                        n if r is None else len(r), ctypes.byref(params), cap,
+                                        _p(feat), _p(b), _p(fl), _p(le), _p(ri), _p(val), _p(gn),
+                                        _p(gap), ctypes.byref(nn), ctypes.byref(nl),
+                                        ctypes.byref(dp))
+        assert rc == 0, "node capacity exceeded"
+        k = nn.value
+        return dict(n_nodes=k, n_leaves=nl.value, depth=dp.value, feature=feat[:k], bin=b[:k],
+                    flags=fl[:k], left=le[:k], right=ri[:k], value=val[:k], gain=gn[:k],
+                    gap=gap[:k])
+
+    def rtree_apply_binned(self, tree, bins):
+        """Tree::predict(dataset, row) per row (NaN follows default_left)."""
+        bins = np.ascontiguousarray(bins, np.uint8)
+        F, n = bins.shape
+        out = np.zeros(n, np.float32)
+        self.L.tco_rtree_apply_binned(_p(np.ascontiguousarray(tree["feature"], np.uint16)),
+                                      _p(np.ascontiguousarray(tree["bin"], np.uint8)),
+                                      _p(np.ascontiguousarray(tree["flags"], np.uint8)),
+                                      _p(np.ascontiguousarray(tree["left"], np.uint32)),
+                                      _p(np.ascontiguousarray(tree["right"], np.uint32)),
+                                      _p(np.ascontiguousarray(tree["value"], np.float32)),
+                                      int(tree["n_nodes"]), _p(bins), n, _p(out))
+        return out
+
+    def rpredict_f32(self, trees, weights, X, edges, n_edges, rule, K=1, tree_class=None,
+                     base=0.0):
+        """Regular-tree ensemble prediction from floats (n x K): rule 1 is
+        predict_raw_fast's binning (binary/regression), rule 0 apply_bins'
+        (multiclass)."""
+        X = np.ascontiguousarray(X, np.float32)
+        n, F = X.shape
+        off = np.zeros(len(trees) + 1, np.uint32)
+        for t, tr in enumerate(trees):
+            off[t + 1] = off[t] + int(tr["n_nodes"])
+        cat = lambda key, dt: np.ascontiguousarray(  # noqa: E731
+            np.concatenate([np.asarray(tr[key])[:int(tr["n_nodes"])] for tr in trees] +
+                           [np.zeros(1)]).astype(dt))
+        out = np.zeros((n, K), np.float32)
+        tc = None if tree_class is None else np.ascontiguousarray(tree_class, np.uint32)
+        self.L.tco_rpredict_f32(_p(off), _p(cat("feature", np.uint16)), _p(cat("bin", np.uint8)),
+                                _p(cat("flags", np.uint8)), _p(cat("left", np.uint32)),
+                                _p(cat("right", np.uint32)), _p(cat("value", np.float32)),
+                                _p(np.ascontiguousarray(weights, np.float32)), _p(tc), len(trees),
+                                K, base, _p(X), n, F, _p(np.ascontiguousarray(edges, np.float32)),
+                                _p(np.ascontiguousarray(n_edges, np.uint32)), rule, _p(out))
+        return out
 
 
 def new_model(T, D):

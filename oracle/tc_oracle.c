@@ -592,3 +592,7 @@ uint32_t tco_early_stop(const float* metric, uint32_t n_iter, uint32_t n_estimat
     if (best) *best = b;
     return kept;
 }
+
+/* ------------------------------------------------ regular depthwise trees */
+
+#include "tc_oracle_regular.inc"

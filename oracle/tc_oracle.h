@@ -161,6 +161,48 @@ float tco_compute_loss_mc_f32(const float* y, const float* p, uint32_t n, uint32
 uint32_t tco_early_stop(const float* metric, uint32_t n_iter, uint32_t n_estimators,
                         uint32_t rounds, int higher_is_better, uint32_t* best_iter, float* best);
 
+
+/* Regular (non-oblivious) depthwise trees, Tree::build (src/tree/tree.cpp:
+ * 162-217, :297-390) with SplitFinder (src/tree/histogram.cpp:569-703).
+ * Parameters of TreeConfig (include/turbocat/config.hpp:26-59). */
+typedef struct {
+    uint32_t max_depth;
+    uint32_t max_leaves;
+    uint32_t max_bins;              /* histogram stride: candidates b < max_bins - 1 */
+    uint32_t min_samples_leaf;
+    float min_child_weight;
+    float lambda_l2;
+    float min_split_gain;
+    float gamma;                    /* subtracted from every candidate's gain */
+    float leaf_smooth;              /* 0: off */
+    float max_delta_step;           /* 0: off */
+    int32_t learn_missing_direction;
+} tco_rtree_params;
+
+/* One regular tree on given gradients, FP64 sums, nodes in the reference's
+ * numbering (add_node order: a split node's two children get consecutive
+ * ids, the smaller child's subtree is built first).  flags: bit 0 is_leaf,
+ * bit 1 default_left.  gap[i] = best - second best valid gain at split node
+ * i (+inf if it had one candidate).  Returns 0, or -1 if more than cap nodes. */
+int tco_rtree_build_f64(const uint8_t* bins, uint32_t n, uint32_t F, const float* g,
+                        const float* h, const uint32_t* root, uint32_t n_root,
+                        const tco_rtree_params* p, uint32_t cap, uint16_t* feature,
+                        uint8_t* bin, uint8_t* flags, uint32_t* left, uint32_t* right,
+                        double* value, double* gain, double* gap, uint32_t* n_nodes,
+                        uint32_t* n_leaves, uint16_t* depth);
+/* Tree::predict(dataset, row) over column-major bins: leaf value per row. */
+void tco_rtree_apply_binned(const uint16_t* feature, const uint8_t* bin, const uint8_t* flags,
+                            const uint32_t* left, const uint32_t* right, const float* value,
+                            uint32_t n_nodes, const uint8_t* bins, uint32_t n, float* out);
+/* Regular-tree ensemble prediction from float rows (rule 0: apply_bins, the
+ * multiclass path; rule 1: predict_raw_fast, the binary/regression path);
+ * out n x K. */
+void tco_rpredict_f32(const uint32_t* tree_off, const uint16_t* feature, const uint8_t* bin,
+                      const uint8_t* flags, const uint32_t* left, const uint32_t* right,
+                      const float* value, const float* weight, const uint32_t* tree_class,
+                      uint32_t T, uint32_t K, float base, const float* X, uint32_t n, uint32_t F,
+                      const float* edges, const uint32_t* n_edges, int rule, float* out);
+
 #ifdef __cplusplus
 }
 #endif
