@@ -318,6 +318,102 @@ int32_t tc_predict_noalloc(tc_ensemble* ens, const float* X, uint32_t n, uint32_
  * 934-962) over a device dataset's bins. */
 int32_t tc_predict_binned(tc_ensemble* ens, tc_dataset* ds, float* out);
 
+/* --------------------------------------------------------- regular trees */
+/* Regular (non-oblivious) depthwise trees: TreeConfig (include/turbocat/
+ * config.hpp:26-59) as Tree::build reads it (src/tree/tree.cpp:162-217,
+ * :297-390; SplitFinder, src/tree/histogram.cpp:569-761).  Every node picks
+ * its own (feature, bin); a node is a leaf at max_depth, under 2 *
+ * min_samples_leaf rows, under 2 * min_child_weight hessian, without a
+ * candidate whose sides both have >= min_samples_leaf rows and >=
+ * min_child_weight hessian and whose gain 0.5 (GL^2/(HL+l) + GR^2/(HR+l) -
+ * G^2/(H+l)) - gamma is >= min_split_gain, or when max_leaves leaves exist at
+ * its visit in the reference's depth-first, smaller-child-first order. */
+typedef struct {
+  uint32_t max_depth;          /* 1..12 */
+  uint32_t max_leaves;         /* TreeConfig::max_leaves (config.hpp:29), >= 1 */
+  uint32_t max_bins;           /* candidates b < max_bins - 1; 255 */
+  float lambda_l2;
+  float min_child_weight;
+  uint32_t min_samples_leaf;
+  float min_split_gain;
+  float gamma;                 /* TreeConfig::gamma (config.hpp:37) */
+  float max_delta_step;        /* > 0: leaf values clamped (tree.cpp:675-678) */
+  float leaf_smooth;           /* > 0: leaf value * count / (count + leaf_smooth) */
+  int32_t learn_missing_direction;  /* TreeConfig::learn_missing_direction (config.hpp:52) */
+} tc_rtree_params;
+
+/* Boosting with regular trees: Booster::train's regular branch (booster.cpp:
+ * 160-232, build_tree :438-493) for TC_LOSS_MSE / TC_LOSS_LOGLOSS, and
+ * Booster::train_multiclass (booster.cpp:1263-1400) for TC_LOSS_SOFTMAX: per
+ * iteration the softmax gradients of every class, then one regular tree per
+ * class on that class's gradients over the iteration's shared row sample.
+ * Sampling as tc_train_params (subsample draw, then the colsample draw). */
+typedef struct {
+  int32_t loss;                /* TC_LOSS_MSE, TC_LOSS_LOGLOSS or TC_LOSS_SOFTMAX */
+  uint32_t n_iterations;       /* BoostingConfig::n_estimators; trees = n_iterations * K */
+  float learning_rate;
+  tc_rtree_params tree;
+  float subsample;             /* in (0, 1): row sample per iteration; 0 or >= 1: every row */
+  float colsample_bytree;      /* in (0, 1): consumes an RNG draw per iteration (trees use every feature) */
+  uint64_t seed;
+  uint32_t n_classes;          /* TC_LOSS_SOFTMAX: K in [3, 16]; otherwise 0 or 1 */
+  int32_t use_graph;           /* 1: one captured hipGraph replay per iteration */
+} tc_rtrain_params;
+
+/* Regular trees in the reference's numbering (Tree::nodes in add_node order;
+ * a split node's children are consecutive ids, tree.cpp:357-363), tree after
+ * tree in slots of node_cap = 2^(max_depth+1) - 1 nodes: node k of tree t at
+ * t * node_cap + k.  flags: bit 0 is_leaf, bit 1 default_left (as Tree::save,
+ * tree.cpp:1768).  Trees are iteration-major, class-minor (the order of
+ * TreeEnsemble::add_tree_for_class, booster.cpp:1386). */
+typedef struct {
+  uint16_t* feature;
+  uint8_t* bin;
+  uint8_t* flags;
+  uint32_t* left;
+  uint32_t* right;
+  float* value;
+  float* gain;
+  uint32_t* n_nodes;           /* per tree */
+  uint32_t* n_leaves;          /* per tree */
+  uint16_t* depth;             /* per tree */
+  float* train_preds;          /* optional: K x n raw training predictions, class-major */
+  float base;                  /* out: base score (0 for softmax) */
+} tc_rmodel;
+
+/* One regular tree on given gradients (Tree::build, tree.cpp:162-217) over the
+ * rows `rows` (NULL: all rows; the root's row sample otherwise); h >= 0.
+ * out: one tree (node_cap slots). */
+int32_t tc_rtree_build(tc_dataset* ds, const float* g, const float* h, const uint32_t* rows,
+                       uint32_t n_rows, const tc_rtree_params* p, tc_rmodel* out);
+/* Boosting with regular trees (see tc_rtrain_params). */
+int32_t tc_rtrain(tc_dataset* ds, const float* y, const tc_rtrain_params* p, tc_rmodel* out);
+
+/* Regular-tree ensemble prediction: TreeEnsemble::predict_batch_optimized /
+ * predict_batch_multiclass_optimized (tree.cpp:1471-1709) over the bins of
+ * rule 0 (Dataset::apply_bins: upper_bound, the multiclass path of
+ * bindings.cpp:311-327) or rule 1 (Booster::predict_raw_fast, booster.cpp:
+ * 680-871, the binary/regression path: up to 16 edges the count of edges <= v,
+ * more the count of edges < v); a row walks every tree (bin != 255 && bin >
+ * split bin: right, so NaN goes left) and adds weight * value into its
+ * class's sum in tree order, then + base[class].  Trees: nodes
+ * tree_off[t] .. tree_off[t+1]-1 with tree-local child ids; tree_class NULL:
+ * every tree adds to class 0.  edges: 255 slots per feature. */
+typedef struct tc_rensemble tc_rensemble;
+int32_t tc_rensemble_create(tc_ctx* ctx, uint32_t n_trees, uint32_t n_classes,
+                            const uint32_t* tree_off, const uint16_t* feature, const uint8_t* bin,
+                            const uint8_t* flags, const uint32_t* left, const uint32_t* right,
+                            const float* value, const float* weights, const uint32_t* tree_class,
+                            const float* base, const float* edges, const uint32_t* n_edges,
+                            uint32_t n_features, int32_t rule, tc_rensemble** out);
+int32_t tc_rensemble_destroy(tc_rensemble* ens);
+/* out: n x K; transform 0 raw, 1 sigmoid (K = 1, loss.cpp:213-215) or softmax
+ * (K > 1, CrossEntropyLoss::transform_to_proba, loss.cpp:320-330). */
+int32_t tc_rpredict(tc_rensemble* ens, const float* X, uint32_t n, uint32_t F, int32_t transform,
+                    float* out);
+int32_t tc_rpredict_dev(tc_rensemble* ens, const float* X_dev, uint32_t n, uint32_t F,
+                        int32_t transform, float* out_dev);
+
 /* --------------------------------------------------------------- serving */
 /* Continuous-batching server over a resident ensemble (single-output or
  * K-channel).  No reference counterpart.  A serving kernel stays resident

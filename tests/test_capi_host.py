@@ -179,7 +179,9 @@ def test_ctypes_structs_match_the_c_header(tmp_path):
     if cc is None:
         pytest.skip("no C compiler")
     structs = {"tc_tree_params": capi.TreeParams, "tc_train_params": capi.TrainParams,
-               "tc_model": capi.Model, "tc_serve_params": capi.ServeParams}
+               "tc_model": capi.Model, "tc_serve_params": capi.ServeParams,
+               "tc_rtree_params": capi.RTreeParams, "tc_rtrain_params": capi.RTrainParams,
+               "tc_rmodel": capi.RModel}
     lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "turbocat_hip.h"',
              'int main(void) {']
     for cname, cls in structs.items():

@@ -1,0 +1,338 @@
+"""GPU regular (non-oblivious) depthwise trees -- tc_rtree_build, tc_rtrain and
+the regular-tree ensemble prediction -- against the CPU oracle
+(tco_rtree_build_f64, tco_rtree_apply_binned, tco_rpredict_f32).
+
+Bit-exact claims: every node's (feature, bin), is_leaf and default_left
+flags and child ids in the reference's numbering (add_node order, the
+max_leaves cut in depth-first smaller-child-first order), tree depth and
+leaf count; predictions of an ensemble of given trees (both binning rules,
+NaN, values equal to an edge); training predictions given the GPU's own
+trees.  Float claims: leaf values and gains within 1e-5 of the FP64 oracle
+(relative, on the scale of the sums they come from).  Near-tie rule: where
+the FP64 gap between a node's best and second-best candidate is below
+NEAR_TIE * max(1, |gain|), either choice is accepted and the rest of the tree
+(whose numbering then differs) is not compared; such events are counted and
+must stay rare."""
+import numpy as np
+import pytest
+
+from conftest import binary_data, multiclass_data, regression_data
+
+pytestmark = pytest.mark.gpu
+
+NEAR_TIE = 1e-5
+
+
+def oparams(p):
+    """The oracle's parameter struct for a capi RTreeParams."""
+    from oracle import rtree_params
+    return rtree_params(max_depth=p.max_depth, max_leaves=p.max_leaves, max_bins=p.max_bins,
+                        min_samples_leaf=p.min_samples_leaf, min_child_weight=p.min_child_weight,
+                        lambda_l2=p.lambda_l2, min_split_gain=p.min_split_gain, gamma=p.gamma,
+                        leaf_smooth=p.leaf_smooth, max_delta_step=p.max_delta_step,
+                        learn_missing_direction=bool(p.learn_missing_direction))
+
+
+def term_scales(ref, bins, g, h, lam, rows=None):
+    """Per node of the oracle's tree, the largest of the gain's terms
+    G^2 / (H + lambda) over the node and its two children (rows routed as the
+    training partition routes them: NaN and bin <= split bin to the left).
+    A float gain carries rounding of about 1e-7 of that, however small the
+    gain itself."""
+    n = bins.shape[1]
+    g64, h64 = g.astype(np.float64), h.astype(np.float64)
+    sc = np.zeros(ref["n_nodes"])
+    if ref["n_nodes"] == 0:
+        return sc
+    root = np.zeros(n, bool)
+    root[np.arange(n) if rows is None else np.asarray(rows, np.int64)] = True
+    term = lambda m: g64[m].sum() ** 2 / (h64[m].sum() + lam)  # noqa: E731
+    stack = [(0, root)]
+    while stack:
+        i, m = stack.pop()
+        if int(ref["flags"][i]) & 1:
+            sc[i] = term(m)
+            continue
+        col = bins[int(ref["feature"][i])]
+        left = m & ((col == 255) | (col <= int(ref["bin"][i])))
+        right = m & ~left
+        sc[i] = max(term(m), term(left), term(right))
+        stack += [(int(ref["left"][i]), left), (int(ref["right"][i]), right)]
+    return sc
+
+
+def compare_rtree(gpu, ref, g, h, p, bins=None, rows=None):
+    """Node by node in id order.  Returns (nodes compared, near-tie hit)."""
+    vscale = float(np.abs(g.astype(np.float64)).sum()) + 1.0
+    sc = term_scales(ref, bins, g, h, p.lambda_l2, rows) if bins is not None else \
+        np.zeros(ref["n_nodes"])
+    n = min(gpu["n_nodes"], ref["n_nodes"])
+    for i in range(n):
+        gl, rl = int(gpu["flags"][i]) & 1, int(ref["flags"][i]) & 1
+        eps = NEAR_TIE * max(1.0, abs(ref["gain"][i])) + 1e-6 * sc[i]
+        if not rl and ref["gap"][i] <= eps:
+            if gl or (gpu["feature"][i], gpu["bin"][i]) != (ref["feature"][i], ref["bin"][i]):
+                return i, True
+        if gl != rl:
+            # a leaf on one side only: the decision must be borderline (the
+            # best gain at min_split_gain)
+            gain = float(ref["gain"][i]) if not rl else float(gpu["gain"][i])
+            assert abs(gain - p.min_split_gain) <= 1e-4 * max(1.0, abs(gain)) + 1e-6 * sc[i], (
+                f"node {i}: leaf on one side only (gpu leaf {gl}, ref leaf {rl}, gain {gain})")
+            return i, True
+        assert int(gpu["flags"][i]) == int(ref["flags"][i]), f"node {i}: flags"
+        if rl:
+            v_ref = float(ref["value"][i])
+            assert abs(float(gpu["value"][i]) - v_ref) <= 1e-5 * max(1.0, abs(v_ref)) + \
+                1e-6 * vscale / (p.lambda_l2 + 1e-3), (f"node {i}: leaf {gpu['value'][i]} vs {v_ref}")
+        else:
+            assert (int(gpu["feature"][i]), int(gpu["bin"][i])) == \
+                (int(ref["feature"][i]), int(ref["bin"][i])), (
+                    f"node {i}: gpu ({gpu['feature'][i]},{gpu['bin'][i]}) ref "
+                    f"({ref['feature'][i]},{ref['bin'][i]}) gap {ref['gap'][i]}")
+            assert (int(gpu["left"][i]), int(gpu["right"][i])) == \
+                (int(ref["left"][i]), int(ref["right"][i])), f"node {i}: children"
+            assert abs(float(gpu["gain"][i]) - float(ref["gain"][i])) <= \
+                1e-4 * max(1.0, abs(float(ref["gain"][i]))) + 1e-6 * sc[i], f"node {i}: gain"
+    assert gpu["n_nodes"] == ref["n_nodes"], (gpu["n_nodes"], ref["n_nodes"])
+    assert gpu["n_leaves"] == ref["n_leaves"] and gpu["depth"] == ref["depth"]
+    return n, False
+
+
+def data_with_nan(n, F, seed, nan_frac=0.0):
+    X, y = regression_data(n, F, seed, low_card_col=1)
+    if nan_frac:
+        rng = np.random.default_rng(seed + 99)
+        X[rng.random((n, F)) < nan_frac] = np.nan
+    return X, y
+
+
+PARAMS = [
+    dict(max_depth=6),
+    dict(max_depth=8, max_leaves=20),                       # the max_leaves cut binds
+    dict(max_depth=5, min_samples_leaf=40, min_child_weight=3.0),
+    dict(max_depth=4, lambda_l2=0.0, min_split_gain=0.5, gamma=0.2),
+    dict(max_depth=5, leaf_smooth=5.0, max_delta_step=0.3),
+    dict(max_depth=3, learn_missing_direction=False, max_bins=16),
+    dict(max_depth=1),
+    dict(max_depth=10, max_leaves=64, min_child_weight=0.5),
+]
+
+
+@pytest.mark.parametrize("pi", range(len(PARAMS)))
+@pytest.mark.parametrize("loss", ["logloss", "mse"])
+def test_rtree_teacher_forced(ctx, oracle, pi, loss):
+    """One tree on the oracle's gradients: the GPU's nodes are the FP64
+    oracle's, in its numbering (NaN in 5% of the values: NaN rows go left at
+    the partition and the learned default_left is recorded)."""
+    from oracle import LOSS_LOGLOSS, LOSS_MSE
+    from turbocat.capi import rtree_params
+    X, yr = data_with_nan(6000, 9, 300 + pi, nan_frac=0.05)
+    lid = LOSS_LOGLOSS if loss == "logloss" else LOSS_MSE
+    y = (yr > 0).astype(np.float32) if loss == "logloss" else yr
+    ds, edges, ne = ctx.quantize_fit(X)
+    bins = ds.bins()
+    p0 = np.full(len(y), oracle.init_prediction(lid, y), np.float32)
+    g, h = oracle.gradients_f32(lid, y, p0)
+    p = rtree_params(**PARAMS[pi])
+    gpu = ds.rtree(g, h, p)
+    ref = oracle.rtree_f64(bins, g, h, oparams(p))
+    n, tie = compare_rtree(gpu, ref, g, h, p, bins)
+    assert n >= 1
+    if PARAMS[pi].get("max_leaves", 64) < (1 << PARAMS[pi]["max_depth"]) and not tie:
+        assert gpu["n_leaves"] >= min(PARAMS[pi]["max_leaves"], 2)
+
+
+def test_rtree_row_sample_and_empty_root(ctx, oracle):
+    """The root's rows as a sample (the trainer's subsample), and no rows at
+    all: a tree without nodes (tree.cpp:172-174)."""
+    from oracle import LOSS_LOGLOSS
+    from turbocat.capi import rtree_params
+    X, yr = data_with_nan(5000, 6, 77, nan_frac=0.02)
+    y = (yr > 0.3).astype(np.float32)
+    ds, _, _ = ctx.quantize_fit(X)
+    bins = ds.bins()
+    g, h = oracle.gradients_f32(LOSS_LOGLOSS, y, np.zeros(len(y), np.float32))
+    rows = np.random.default_rng(5).permutation(len(y))[:3100].astype(np.uint32)
+    p = rtree_params(max_depth=7, max_leaves=30)
+    _, tie = compare_rtree(ds.rtree(g, h, p, rows), oracle.rtree_f64(bins, g, h, oparams(p), rows),
+                           g, h, p, bins, rows)
+    empty = ds.rtree(g, h, p, np.zeros(0, np.uint32))
+    assert empty["n_nodes"] == 0 and empty["n_leaves"] == 0
+
+
+def test_rtree_kat_eight_rows(ctx, oracle):
+    """The 8-row toy (tests/test_turbocat.py:55-56) with mode='small': the root
+    splits (0, 3) with gain 2, both children are pure and become leaves -1 and
+    +1 (every candidate gain there is negative), default_left stays 1."""
+    from turbocat.capi import TC_LOSS_LOGLOSS, rtrain_params
+    X = np.array([-2, -1, 1, 2, -1.5, -0.5, 0.5, 1.5], np.float32).reshape(8, 1)
+    y = np.array([0, 0, 1, 1, 0, 0, 1, 1], np.float32)
+    ds, _, _ = ctx.quantize_fit(X)
+    m = ds.rtrain(y, rtrain_params(TC_LOSS_LOGLOSS, n_iterations=1, learning_rate=1.0, max_depth=3,
+                                   lambda_l2=1.0, min_child_weight=0.01))
+    t = m["trees"][0]
+    assert m["base"] == 0.0
+    assert t["n_nodes"] == 3 and t["n_leaves"] == 2 and t["depth"] == 1
+    assert (int(t["feature"][0]), int(t["bin"][0]), float(t["gain"][0])) == (0, 3, 2.0)
+    assert list(t["flags"]) == [2, 3, 3] and int(t["left"][0]) == 1 and int(t["right"][0]) == 2
+    assert list(t["value"][1:]) == [-1.0, 1.0]
+    assert list(m["train_preds"][0]) == [-1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0]
+
+
+def replay_binary(oracle, bins, y, lid, m, p, lr):
+    """Teacher-forced replay of a binary / regression fit: per tree the
+    oracle's gradients on the replayed predictions, its FP64 tree compared
+    with the GPU's, then the GPU's tree applied (Tree::predict over bins, one
+    fused multiply-add).  Returns (near ties, final predictions)."""
+    preds = np.full(len(y), np.float32(m["base"]), np.float32)
+    ties = 0
+    for t, tree in enumerate(m["trees"]):
+        g, h = oracle.gradients_f32(lid, y, preds)
+        ref = oracle.rtree_f64(bins, g, h, oparams(p))
+        _, tie = compare_rtree(tree, ref, g, h, p, bins)
+        ties += tie
+        preds = oracle.fma_update(preds, np.float32(lr), oracle.rtree_apply_binned(tree, bins))
+    return ties, preds
+
+
+@pytest.mark.parametrize("loss,subsample", [("logloss", 1.0), ("mse", 1.0), ("logloss", 0.7)])
+def test_rtrain_matches_oracle(ctx, oracle, loss, subsample):
+    """Booster::train's regular branch: 8 trees, every tree teacher-forced
+    against the oracle on the replayed predictions; the GPU's training
+    predictions equal the replay bit for bit (n % 8 == 0: every row takes the
+    reference's vectorised gradient).  With subsample the trees grow on the
+    reference's row samples (same engine and draws), which the oracle is
+    given."""
+    from oracle import LOSS_LOGLOSS, LOSS_MSE
+    from turbocat import capi
+    X, yr = data_with_nan(8000, 8, 11, nan_frac=0.01)
+    lid = LOSS_LOGLOSS if loss == "logloss" else LOSS_MSE
+    y = (yr > 0).astype(np.float32) if loss == "logloss" else yr
+    ds, _, _ = ctx.quantize_fit(X)
+    bins = ds.bins()
+    p = capi.rtree_params(max_depth=6, max_leaves=24, lambda_l2=1.0)
+    lr = 0.3
+    tp = capi.rtrain_params(lid, n_iterations=8, learning_rate=lr, subsample=subsample, seed=9)
+    tp.tree = p
+    m = ds.rtrain(y, tp)
+    assert m["base"] == np.float32(oracle.init_prediction(lid, y))
+    if subsample >= 1.0:
+        ties, preds = replay_binary(oracle, bins, y, lid, m, p, lr)
+        assert ties <= 1
+        assert np.array_equal(m["train_preds"][0], preds)
+        return
+    # sampled: the oracle grows each tree on the same rows
+    state = 9
+    preds = np.full(len(y), np.float32(m["base"]), np.float32)
+    ties = 0
+    for tree in m["trees"]:
+        seed, state = capi.next_random(state)
+        rows = capi.random_subsample(len(y), subsample, seed)
+        g, h = oracle.gradients_f32(lid, y, preds)
+        _, tie = compare_rtree(tree, oracle.rtree_f64(bins, g, h, oparams(p), rows), g, h, p, bins,
+                               rows)
+        ties += tie
+        preds = oracle.fma_update(preds, np.float32(lr), oracle.rtree_apply_binned(tree, bins))
+    assert ties <= 1
+    assert np.array_equal(m["train_preds"][0], preds)
+
+
+def test_rtrain_multiclass_matches_oracle(ctx, oracle):
+    """Booster::train_multiclass (booster.cpp:1263-1400): per iteration the
+    softmax gradients, then one regular tree per class on that class's
+    gradients.  Each tree is compared with the oracle's on the oracle's
+    gradients of the replayed predictions (the device's softmax exp is within
+    1e-6 of the reference's); the replayed predictions stay within 1e-5 of
+    the GPU's."""
+    from turbocat import capi
+    K = 3
+    X, y = multiclass_data(6000, 10, K, 21)
+    ds, _, _ = ctx.quantize_fit(X)
+    bins = ds.bins()
+    p = capi.rtree_params(max_depth=5, max_leaves=20, lambda_l2=1.0)
+    lr = 0.2
+    tp = capi.rtrain_params(capi.TC_LOSS_SOFTMAX, n_iterations=4, learning_rate=lr, n_classes=K)
+    tp.tree = p
+    m = ds.rtrain(y, tp)
+    assert m["base"] == 0.0 and len(m["trees"]) == 4 * K
+    preds = np.zeros((len(y), K), np.float32)
+    ties = 0
+    for it in range(4):
+        g, h = oracle.gradients_mc_f32(y, preds)
+        nxt = preds.copy()
+        for k in range(K):
+            tree = m["trees"][it * K + k]
+            gk = np.ascontiguousarray(g[:, k]); hk = np.ascontiguousarray(h[:, k])
+            _, tie = compare_rtree(tree, oracle.rtree_f64(bins, gk, hk, oparams(p)), gk, hk, p, bins)
+            ties += tie
+            nxt[:, k] = oracle.fma_update(preds[:, k].copy(), np.float32(lr),
+                                          oracle.rtree_apply_binned(tree, bins))
+        preds = nxt
+    assert ties <= 1
+    assert np.abs(m["train_preds"].T - preds).max() <= 1e-5
+
+
+@pytest.mark.parametrize("kind", ["binary", "multiclass"])
+def test_rpredict_matches_oracle(ctx, oracle, kind):
+    """tc_rpredict of trained regular trees equals the oracle's
+    tco_rpredict_f32 bit for bit: rule 1 (predict_raw_fast, binary) and rule
+    0 (apply_bins, multiclass with the trees' classes), with NaN values,
+    values equal to an edge and a row count that is not a multiple of 64."""
+    from oracle import LOSS_LOGLOSS
+    from turbocat import capi
+    if kind == "binary":
+        X, yr = data_with_nan(4000, 7, 3, nan_frac=0.03)
+        y = (yr > 0).astype(np.float32)
+        K, loss, rule = 1, LOSS_LOGLOSS, 1
+    else:
+        X, y = multiclass_data(4000, 7, 4, 8)
+        K, loss, rule = 4, capi.TC_LOSS_SOFTMAX, 0
+    ds, edges, ne = ctx.quantize_fit(X)
+    tp = capi.rtrain_params(loss, n_iterations=12, learning_rate=0.2, n_classes=K if K > 1 else 0)
+    tp.tree = capi.rtree_params(max_depth=6, max_leaves=40)
+    m = ds.rtrain(y, tp)
+    rng = np.random.default_rng(4)
+    Xp = rng.standard_normal((1003, X.shape[1])).astype(np.float32)
+    Xp[rng.random(Xp.shape) < 0.03] = np.nan
+    for f in range(X.shape[1]):  # values that equal an edge
+        if ne[f]:
+            Xp[f * 7:(f + 1) * 7, f] = edges[f, rng.integers(0, ne[f], 7)]
+    ens = capi.REnsemble(ctx, m["trees"], m["weight"], edges, ne, X.shape[1], rule=rule, K=K,
+                         tree_class=m["tree_class"] if K > 1 else None, base=m["base"])
+    ref = oracle.rpredict_f32(m["trees"], m["weight"], Xp, edges, ne, rule, K=K,
+                              tree_class=m["tree_class"] if K > 1 else None, base=m["base"])
+    out = ens.predict(Xp)
+    assert np.array_equal(out.reshape(-1, K), ref)
+    prob = ens.predict(Xp, transform=1).reshape(-1, K)
+    if K == 1:
+        assert np.allclose(prob[:, 0], 1 / (1 + np.exp(-ref[:, 0].astype(np.float64))), atol=1e-6)
+    else:
+        e = np.exp(ref - ref.max(1, keepdims=True))
+        assert np.allclose(prob, e / e.sum(1, keepdims=True), atol=1e-6)
+    if rule == 0:
+        # apply_bins bins as training did (upper_bound) and the data has no
+        # NaN, so the training rows through prediction reproduce the training
+        # update's sums (the order of the additions differs: within 1e-5).
+        # Not so under rule 1: with more than 16 edges a value equal to an edge
+        # -- every training value is one -- falls one bin lower there, and NaN
+        # always goes left, as in the reference.
+        tr = ens.predict(X).reshape(-1, K)
+        assert np.abs(tr.T - m["train_preds"]).max() <= 1e-5 * max(1.0, float(np.abs(tr).max()))
+
+
+def test_rtrain_graph_equals_stepwise(ctx):
+    """The per-iteration graph replay and plain launches build identical
+    models."""
+    from turbocat import capi
+    X, y = binary_data(5000, 12, 4)
+    ds, _, _ = ctx.quantize_fit(X)
+    a = []
+    for use_graph in (0, 1):
+        tp = capi.rtrain_params(capi.TC_LOSS_LOGLOSS, n_iterations=5, learning_rate=0.3,
+                                use_graph=use_graph)
+        tp.tree = capi.rtree_params(max_depth=6, max_leaves=20)
+        a.append(ds.rtrain(y, tp))
+    for k in ("feature", "bin", "flags", "left", "right", "value", "n_nodes"):
+        assert np.array_equal(a[0]["raw"][k], a[1]["raw"][k]), k
+    assert np.array_equal(a[0]["train_preds"], a[1]["train_preds"])

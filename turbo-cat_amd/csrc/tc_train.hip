@@ -344,12 +344,12 @@ class Trainer {
         // one job: below the root the histogram launch plans its own level
         const dim3 grid(round8(hist_cap_[d]));
         const bool one = njobs_ == 1, plan = one && d > 0;
-        auto* k = packed ? (plan ? k_hist_plan<kHistUnroll, true>
-                                 : one ? k_hist<kHistUnroll, true, true>
-                                       : k_hist<kHistUnroll, true, false>)
-                         : (plan ? k_hist_plan<kHistUnroll, false>
-                                 : one ? k_hist<kHistUnroll, false, true>
-                                       : k_hist<kHistUnroll, false, false>);
+        auto* k = packed ? (plan ? k_hist_plan<kHistUnroll, 1>
+                                 : one ? k_hist<kHistUnroll, 1, true>
+                                       : k_hist<kHistUnroll, 1, false>)
+                         : (plan ? k_hist_plan<kHistUnroll, 0>
+                                 : one ? k_hist<kHistUnroll, 0, true>
+                                       : k_hist<kHistUnroll, 0, false>);
         hipLaunchKernelGGL(k, grid, dim3(kHistThreads), 0, s, d_jobs_, lists_.hist[d],
                            lists_.hist_count[d], d, partials_, lists_);
       }
@@ -1346,7 +1346,7 @@ int32_t tc_hist_build(tc_dataset* ds, const float* g, const float* h, const uint
       JobDev* dj = static_cast<JobDev*>(al(sizeof(JobDev)));
       TC_HIP(hipMemcpyAsync(dj, &J, sizeof(J), hipMemcpyHostToDevice, s));
       if (cnt)
-        hipLaunchKernelGGL((k_hist<kHistUnroll, false, true>), dim3(round8(cnt)),
+        hipLaunchKernelGGL((k_hist<kHistUnroll, 0, true>), dim3(round8(cnt)),
                            dim3(kHistThreads), 0, s, dj, dw, dc, 0u, static_cast<HistCell*>(nullptr),
                            Lists{});
       hipLaunchKernelGGL(k_hist_to_float, dim3(ceil_div(F * 256ull, 256)), dim3(256), 0, s, J.hist,
@@ -1364,7 +1364,7 @@ int32_t tc_hist_build(tc_dataset* ds, const float* g, const float* h, const uint
         one.gscale = one.ginv = one.hscale = one.hinv = 1.0;
         TC_HIP(hipMemcpyAsync(J.st, &one, sizeof(one), hipMemcpyHostToDevice, s));
         if (cnt)
-          hipLaunchKernelGGL((k_hist<kHistUnroll, false, true>), dim3(round8(cnt)),
+          hipLaunchKernelGGL((k_hist<kHistUnroll, 0, true>), dim3(round8(cnt)),
                              dim3(kHistThreads), 0, s, dj, dw, dc, 0u,
                              static_cast<HistCell*>(nullptr), Lists{});
         TC_HIP(hipGetLastError());
@@ -1521,3 +1521,4 @@ int32_t tc_gradients(tc_ctx* ctx, int32_t loss, const float* y, const float* pre
 }
 
 }  // extern "C"
+#include "tc_rtree.hip"

@@ -861,10 +861,14 @@ __device__ bool level_plan(const JobDev& J, const Lists& L, uint32_t job, uint32
 // layout put the bank at (f + b) mod 16 and measured 63% of LDS cycles in
 // bank-conflict stalls).
 //
-// PACKED (MSE, h == 2 for every row): one atomic per value on a cell holding
+// MODE 1 (MSE, h == 2 for every row): one atomic per value on a cell holding
 // g * 2^16 + 1 -- a signed 48-bit g sum above a 16-bit row count (a work
 // item has fewer than 2^16 rows, so the count never carries); h = 2 * count.
-// Otherwise one atomic on the g plane and one on the h plane.
+// MODE 0: one atomic on the g plane and one on the h plane.  MODE 2 (the
+// regular trees' cells, which need exact row counts): the g plane, and an h
+// plane whose cells hold h * 2^16 + 1 (h >= 0 summed over one item stays
+// inside 47 bits under the packed scale); the flush stores that packed word
+// as the cell's h, which the reader unpacks (store mode only).
 //
 // Training (`partials` given, J.store_hist): a node's last chunk stores its
 // cells into the node's HBM histogram, every other chunk into its slot of
@@ -880,13 +884,14 @@ __device__ bool level_plan(const JobDev& J, const Lists& L, uint32_t job, uint32
 // level from the partition above (level_plan, in the LDS of its cells) and
 // takes its own item, and workgroup 0 stores the plan's outputs for the
 // later launches (the plan costs less than the launch it replaces).
-template <int U, bool PACKED, bool SINGLE, bool PLAN>
+template <int U, int MODE, bool SINGLE, bool PLAN>
 __device__ __forceinline__ void hist_item(const JobDev* __restrict__ jobs,
                                           const HistWork* __restrict__ work,
                                           const uint32_t* __restrict__ count, uint32_t level,
                                           HistCell* __restrict__ partials, const Lists& L) {
   constexpr int kCells = kTileF * 256;
   __shared__ unsigned long long s_a[kCells];
+  constexpr bool PACKED = MODE == 1;
   __shared__ unsigned long long s_b[PACKED ? 1 : kCells];
   static_assert(!PLAN || SINGLE, "the in-kernel plan serves one job");
   static_assert(sizeof(PlanShared) <= sizeof(s_a), "the plan fits the cells' LDS");
@@ -962,9 +967,9 @@ __device__ __forceinline__ void hist_item(const JobDev* __restrict__ jobs,
       const long long gq = __double2ll_rn(static_cast<double>(g[u].x) * gsc);
       const unsigned long long va = PACKED ? (static_cast<unsigned long long>(gq) << 16) + 1ull
                                            : static_cast<unsigned long long>(gq);
-      const unsigned long long vb =
-          PACKED ? 0ull
-                 : static_cast<unsigned long long>(__double2ll_rn(static_cast<double>(g[u].y) * hsc));
+      const unsigned long long hq =
+          static_cast<unsigned long long>(__double2ll_rn(static_cast<double>(g[u].y) * hsc));
+      const unsigned long long vb = PACKED ? 0ull : MODE == 2 ? (hq << 16) + 1ull : hq;
       // rotate the 16 bytes by s: byte k of wr = feature (k + s) % 16
       const uint32_t a0 = v[u].x, a1 = v[u].y, a2 = v[u].z, a3 = v[u].w;
       const uint32_t d0 = q == 0 ? a0 : q == 1 ? a1 : q == 2 ? a2 : a3;
@@ -1027,22 +1032,22 @@ __device__ __forceinline__ void hist_item(const JobDev* __restrict__ jobs,
 // VGPRs, 64 KB LDS each; measured on gfx950 -- above that one runs at a time
 // although the compiler's occupancy estimate still reads eight waves per
 // SIMD), so the registers are capped here.
-template <int U, bool PACKED, bool SINGLE>
+template <int U, int MODE, bool SINGLE>
 __global__ void __launch_bounds__(kHistThreads) __attribute__((amdgpu_num_sgpr(76), amdgpu_waves_per_eu(8)))
     k_hist(const JobDev* __restrict__ jobs, const HistWork* __restrict__ work,
            const uint32_t* __restrict__ count, uint32_t level, HistCell* __restrict__ partials,
            Lists L) {
-  hist_item<U, PACKED, SINGLE, false>(jobs, work, count, level, partials, L);
+  hist_item<U, MODE, SINGLE, false>(jobs, work, count, level, partials, L);
 }
 
 // One job below the root: the level planned in the kernel, about one item per
 // CU (no cap: a capped variant spills in its main loop).
-template <int U, bool PACKED>
+template <int U, int MODE>
 __global__ void __launch_bounds__(kHistThreads)
     k_hist_plan(const JobDev* __restrict__ jobs, const HistWork* __restrict__ work,
                 const uint32_t* __restrict__ count, uint32_t level, HistCell* __restrict__ partials,
                 Lists L) {
-  hist_item<U, PACKED, true, true>(jobs, work, count, level, partials, L);
+  hist_item<U, MODE, true, true>(jobs, work, count, level, partials, L);
 }
 
 // ---------------------------------------------------------------------------

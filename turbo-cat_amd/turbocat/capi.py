@@ -41,6 +41,8 @@ EXPORTS = [
     "tc_serve_wait", "tc_serve_cancel", "tc_serve_stats", "tc_serve_counters",
     "tc_serve_latencies", "tc_serve_stop",
     "tc_host_alloc", "tc_host_free",
+    "tc_rtree_build", "tc_rtrain", "tc_rensemble_create", "tc_rensemble_destroy", "tc_rpredict",
+    "tc_rpredict_dev",
 ]
 
 
@@ -76,6 +78,71 @@ class TrainHistory(Structure):
     _fields_ = [("train_loss", c_void_p), ("valid_loss", c_void_p), ("n_trees_built", c_uint32),
                 ("stopped_early", c_int32), ("best_iteration", c_uint32),
                 ("best_valid_loss", c_float)]
+
+
+class RTreeParams(Structure):
+    _fields_ = [("max_depth", c_uint32), ("max_leaves", c_uint32), ("max_bins", c_uint32),
+                ("lambda_l2", c_float), ("min_child_weight", c_float),
+                ("min_samples_leaf", c_uint32), ("min_split_gain", c_float), ("gamma", c_float),
+                ("max_delta_step", c_float), ("leaf_smooth", c_float),
+                ("learn_missing_direction", c_int32)]
+
+
+class RTrainParams(Structure):
+    _fields_ = [("loss", c_int32), ("n_iterations", c_uint32), ("learning_rate", c_float),
+                ("tree", RTreeParams), ("subsample", c_float), ("colsample_bytree", c_float),
+                ("seed", c_uint64), ("n_classes", c_uint32), ("use_graph", c_int32)]
+
+
+class RModel(Structure):
+    _fields_ = [("feature", c_void_p), ("bin", c_void_p), ("flags", c_void_p), ("left", c_void_p),
+                ("right", c_void_p), ("value", c_void_p), ("gain", c_void_p),
+                ("n_nodes", c_void_p), ("n_leaves", c_void_p), ("depth", c_void_p),
+                ("train_preds", c_void_p), ("base", c_float)]
+
+
+def rtree_params(max_depth=6, max_leaves=64, max_bins=255, lambda_l2=1.0, min_child_weight=1.0,
+                 min_samples_leaf=1, min_split_gain=0.0, gamma=0.0, max_delta_step=0.0,
+                 leaf_smooth=0.0, learn_missing_direction=True) -> RTreeParams:
+    """TreeConfig of a regular tree (include/turbocat/config.hpp:26-59 defaults)."""
+    return RTreeParams(max_depth, max_leaves, max_bins, lambda_l2, min_child_weight,
+                       min_samples_leaf, min_split_gain, gamma, max_delta_step, leaf_smooth,
+                       1 if learn_missing_direction else 0)
+
+
+def rtrain_params(loss=TC_LOSS_MSE, n_iterations=100, learning_rate=0.1, subsample=1.0,
+                  colsample_bytree=1.0, seed=42, n_classes=0, use_graph=1,
+                  **tree) -> RTrainParams:
+    return RTrainParams(loss, n_iterations, learning_rate, rtree_params(**tree), subsample,
+                        colsample_bytree, seed, n_classes, use_graph)
+
+
+def _new_rmodel(n_trees, max_depth, n=0, K=1):
+    NC = (2 << max_depth) - 1
+    a = dict(node_cap=NC, n_trees=n_trees, feature=np.zeros(n_trees * NC, np.uint16),
+             bin=np.zeros(n_trees * NC, np.uint8), flags=np.zeros(n_trees * NC, np.uint8),
+             left=np.zeros(n_trees * NC, np.uint32), right=np.zeros(n_trees * NC, np.uint32),
+             value=np.zeros(n_trees * NC, np.float32), gain=np.zeros(n_trees * NC, np.float32),
+             n_nodes=np.zeros(n_trees, np.uint32), n_leaves=np.zeros(n_trees, np.uint32),
+             depth=np.zeros(n_trees, np.uint16),
+             train_preds=np.zeros((K, n), np.float32) if n else None)
+    m = RModel(*[_p(a[k]) for k in ("feature", "bin", "flags", "left", "right", "value", "gain",
+                                    "n_nodes", "n_leaves", "depth", "train_preds")], 0.0)
+    return m, a
+
+
+def rtrees(a):
+    """Per-tree node dicts (the reference's numbering) of a regular model."""
+    NC, out = a["node_cap"], []
+    for t in range(a["n_trees"]):
+        k = int(a["n_nodes"][t])
+        sl = slice(t * NC, t * NC + k)
+        out.append(dict(feature=a["feature"][sl].copy(), bin=a["bin"][sl].copy(),
+                        flags=a["flags"][sl].copy(), left=a["left"][sl].copy(),
+                        right=a["right"][sl].copy(), value=a["value"][sl].copy(),
+                        gain=a["gain"][sl].copy(), n_nodes=k, n_leaves=int(a["n_leaves"][t]),
+                        depth=int(a["depth"][t])))
+    return out
 
 
 class ServeParams(Structure):
@@ -195,6 +262,11 @@ def load(path: str = LIB_PATH):
         "tc_serve_cancel": [V, c_uint64], "tc_serve_counters": [V, POINTER(ServeCounters)],
         "tc_serve_latencies": [V, V, c_uint64, V],
         "tc_serve_stop": [V], "tc_host_alloc": [c_size_t, V], "tc_host_free": [V],
+        "tc_rtree_build": [V, V, V, V, U, POINTER(RTreeParams), POINTER(RModel)],
+        "tc_rtrain": [V, V, POINTER(RTrainParams), POINTER(RModel)],
+        "tc_rensemble_create": [V, U, U, V, V, V, V, V, V, V, V, V, V, V, V, U, I, V],
+        "tc_rensemble_destroy": [V], "tc_rpredict": [V, V, U, U, I, V],
+        "tc_rpredict_dev": [V, V, U, U, I, V],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -412,6 +484,28 @@ class Dataset:
         a["weight"] = np.full(params.n_trees, params.learning_rate, np.float32)
         return a
 
+    def rtree(self, g, h, params: RTreeParams, rows=None):
+        """One regular tree on given gradients (tc_rtree_build): node dict in
+        the reference's numbering."""
+        m, a = _new_rmodel(1, params.max_depth)
+        r = None if rows is None else np.ascontiguousarray(rows, np.uint32)
+        check(self.ctx.L.tc_rtree_build(self.h, _p(np.ascontiguousarray(g, np.float32)),
+                                        _p(np.ascontiguousarray(h, np.float32)), _p(r),
+                                        0 if r is None else len(r), byref(params), byref(m)))
+        return rtrees(a)[0]
+
+    def rtrain(self, y, params: RTrainParams):
+        """Boosting with regular trees (tc_rtrain).  Returns the model dict:
+        trees (iteration-major, class-minor), weights, classes, base."""
+        K = params.n_classes if params.loss == TC_LOSS_SOFTMAX else 1
+        T = params.n_iterations * K
+        m, a = _new_rmodel(T, params.tree.max_depth, self.n, K)
+        y = np.ascontiguousarray(y, np.float32)
+        check(self.ctx.L.tc_rtrain(self.h, _p(y), byref(params), byref(m)))
+        return dict(trees=rtrees(a), weight=np.full(T, params.learning_rate, np.float32),
+                    tree_class=np.tile(np.arange(K, dtype=np.uint32), params.n_iterations),
+                    K=K, base=float(m.base), train_preds=a["train_preds"], raw=a)
+
     def train_valid(self, y, params: TrainParams, valid=None, y_valid=None,
                     early_stopping_rounds=50):
         """tc_train_valid: returns (model, history); history holds the mean
@@ -537,6 +631,56 @@ class Ensemble:
         out = np.zeros(ds.n, np.float32) if self.K == 1 else np.zeros((ds.n, self.K), np.float32)
         check(self.ctx.L.tc_predict_binned(self.h, ds.h, _p(out)))
         return out
+
+
+class REnsemble:
+    """Regular-tree ensemble on the device (tc_rensemble_create).  trees: node
+    dicts; rule 0 (apply_bins) or 1 (predict_raw_fast)."""
+
+    def __init__(self, ctx: Context, trees, weights, edges, n_edges, n_features, rule=1, K=1,
+                 tree_class=None, base=0.0):
+        self.ctx, self.F, self.K = ctx, n_features, K
+        off = np.zeros(len(trees) + 1, np.uint32)
+        for t, tr in enumerate(trees):
+            off[t + 1] = off[t] + tr["n_nodes"]
+
+        def cat(k, dt):
+            if not trees:
+                return np.zeros(1, dt)
+            return np.ascontiguousarray(np.concatenate([tr[k][:tr["n_nodes"]] for tr in trees]), dt)
+        self._keep = [off, cat("feature", np.uint16), cat("bin", np.uint8), cat("flags", np.uint8),
+                      cat("left", np.uint32), cat("right", np.uint32), cat("value", np.float32)]
+        w = np.ascontiguousarray(weights, np.float32)
+        tc = None if tree_class is None else np.ascontiguousarray(tree_class, np.uint32)
+        b = np.ascontiguousarray(np.broadcast_to(np.asarray(base, np.float32), (K,)), np.float32)
+        e = np.ascontiguousarray(edges, np.float32)
+        ne = np.ascontiguousarray(n_edges, np.uint32)
+        self._keep += [w, tc, b, e, ne]
+        self.h = c_void_p()
+        check(ctx.L.tc_rensemble_create(ctx.h, len(trees), K, *[_p(x) for x in self._keep[:7]],
+                                        _p(w), _p(tc), _p(b), _p(e), _p(ne), n_features, rule,
+                                        byref(self.h)))
+
+    def predict(self, X, transform=0):
+        X = np.ascontiguousarray(X, np.float32)
+        n = X.shape[0]
+        out = np.zeros(n, np.float32) if self.K == 1 else np.zeros((n, self.K), np.float32)
+        check(self.ctx.L.tc_rpredict(self.h, _p(X), n, self.F, transform, _p(out)))
+        return out
+
+    def predict_dev(self, x_dev, n, out_dev, transform=0):
+        check(self.ctx.L.tc_rpredict_dev(self.h, x_dev, n, self.F, transform, out_dev))
+
+    def close(self):
+        if self.h:
+            self.ctx.L.tc_rensemble_destroy(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class HostBuffer:
