@@ -388,6 +388,18 @@ int32_t tc_rtree_build(tc_dataset* ds, const float* g, const float* h, const uin
                        uint32_t n_rows, const tc_rtree_params* p, tc_rmodel* out);
 /* Boosting with regular trees (see tc_rtrain_params). */
 int32_t tc_rtrain(tc_dataset* ds, const float* y, const tc_rtrain_params* p, tc_rmodel* out);
+/* With a validation set and early stopping: after every iteration the new
+ * trees are applied to the validation rows (binned with the training edges,
+ * tc_dataset_create; NaN by each node's default_left), their mean loss taken
+ * (MSE, LogLoss, or softmax cross-entropy -log max(p_label, 1e-7)), and the
+ * fit stops after the iteration `iter` at which iter >= max(50, n_iterations
+ * / 10) and early_stopping_rounds iterations in a row did not improve the
+ * best loss by more than 1e-10 (booster.cpp:306-371, :1400-1430).  Trees of
+ * iterations after the stop have no nodes; history->n_trees_built counts
+ * iterations.  history may be NULL; its train_loss is zero-filled. */
+int32_t tc_rtrain_valid(tc_dataset* ds, const float* y, const tc_rtrain_params* p,
+                        tc_dataset* valid, const float* y_valid, uint32_t early_stopping_rounds,
+                        tc_rmodel* out, tc_train_history* history);
 
 /* Regular-tree ensemble prediction: TreeEnsemble::predict_batch_optimized /
  * predict_batch_multiclass_optimized (tree.cpp:1471-1709) over the bins of

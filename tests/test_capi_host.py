@@ -106,10 +106,12 @@ def test_not_fitted_and_unsupported_options():
         clf.predict_proba(X)
     with pytest.raises(RuntimeError, match="not fitted"):
         turbocat.TurboCatRegressor().predict(X)
-    with pytest.raises(ValueError, match="mode='small'"):
-        turbocat.TurboCatRegressor(mode="small", subsample=1.0).fit(X, y)
-    with pytest.raises(ValueError, match="multiclass"):
-        turbocat.TurboCatClassifier(subsample=1.0).fit(X, np.arange(50) % 3)
+    with pytest.raises(ValueError, match="lossguide"):
+        turbocat.TurboCatRegressor(mode="small", grow_policy="lossguide", subsample=1.0).fit(X, y)
+    with pytest.raises(ValueError, match="use_goss with regular trees"):
+        turbocat.TurboCatClassifier(mode="small", use_goss=True, subsample=1.0).fit(X, y > 0)
+    with pytest.raises(ValueError, match="use_goss with regular trees"):  # multiclass: regular trees
+        turbocat.TurboCatClassifier(use_goss=True, subsample=1.0).fit(X, np.arange(50) % 3)
     with pytest.raises(ValueError, match="loss='huber'"):
         turbocat.TurboCatRegressor(loss="huber").fit(X, y)
     with pytest.raises(ValueError, match="goss_top_rate"):
@@ -202,3 +204,95 @@ def test_ctypes_structs_match_the_c_header(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(cls), cname
         for fname, _ in cls._fields_:
             assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
+
+
+def write_tcat_v3_regular(path, trees, edges, n_edges, task, n_classes, lr, max_depth, lam, base):
+    """TCAT v3 writer for regular trees from the documented layouts
+    (booster.cpp:1049-1109 with use_symmetric 0; Tree::save, tree.cpp:
+    1748-1774: u16 depth, u32 n_classes, u32 n_nodes, u16 n_leaves, u16
+    max_depth, f32 lambda_l2, then per node u16 feature, u8 bin, u8 flags,
+    u16 left, u16 right (TreeIndex, types.hpp:47), f32 value, f32 gain)."""
+    with open(path, "wb") as f:
+        f.write(b"TCAT")
+        f.write(struct.pack("<IfBIfHfB", 3, base, task, n_classes, lr, max_depth, lam, 0))
+        f.write(struct.pack("<Q", len(n_edges)))
+        for k, ne in enumerate(n_edges):
+            f.write(struct.pack("<Q", int(ne)))
+            f.write(np.asarray(edges[k, :ne], "<f4").tobytes())
+        f.write(struct.pack("<Q", len(trees)))
+        f.write(np.full(len(trees), lr, "<f4").tobytes())
+        for t in trees:
+            n = len(t["feature"])
+            f.write(struct.pack("<HIIHHf", t["depth"], 1, n, t["n_leaves"], max_depth, lam))
+            for k in range(n):
+                f.write(struct.pack("<HBBHHff", t["feature"][k], t["bin"][k], t["flags"][k],
+                                    t["left"][k], t["right"][k], t["value"][k], t["gain"][k]))
+
+
+def _regular_trees(seed, n_trees, F):
+    """Random well-formed regular trees in the reference's numbering: a split
+    node's children get the next two ids, and the subtree visited first (left
+    or right at random) takes the ids after them (tree.cpp:357-390)."""
+    rng = np.random.default_rng(seed)
+    trees = []
+    for _ in range(n_trees):
+        recs, nxt, depth = {}, [1], [0]
+
+        def build(i, d):
+            depth[0] = max(depth[0], d)
+            if d < 4 and rng.random() < 0.6:
+                lc, rc = nxt[0], nxt[0] + 1
+                nxt[0] += 2
+                right_first = bool(rng.random() < 0.5)
+                recs[i] = ("split", int(rng.integers(0, F)), int(rng.integers(0, 30)),
+                           float(abs(rng.standard_normal())), lc, rc,
+                           2 if rng.random() < 0.7 else 0)
+                for c in ((rc, lc) if right_first else (lc, rc)):
+                    build(c, d + 1)
+            else:
+                recs[i] = ("leaf", float(rng.standard_normal()))
+        build(0, 0)
+        t = dict(feature=[], bin=[], flags=[], left=[], right=[], value=[], gain=[])
+        for k in range(nxt[0]):
+            r = recs[k]
+            split = r[0] == "split"
+            for key, v in zip(("feature", "bin", "flags", "left", "right", "value", "gain"),
+                              (r[1], r[2], r[6], r[4], r[5], 0.0, r[3]) if split else
+                              (0, 0, 3, 0, 0, r[1], 0.0)):
+                t[key].append(v)
+        t["value"] = np.float32(t["value"])
+        t["gain"] = np.float32(t["gain"])
+        t["depth"] = depth[0]
+        t["n_leaves"] = sum(1 for x in t["flags"] if x & 1)
+        trees.append(t)
+    return trees
+
+
+@pytest.mark.parametrize("task,n_classes", [(2, 2), (0, 2), (1, 3)])
+def test_tcat_v3_regular_round_trip(tmp_path, task, n_classes):
+    """A regular-tree TCAT v3 file (regression, binary, and a 3-class model
+    whose trees are stored iteration-major, class-minor) loads and saves back
+    byte for byte; the loaded model reports its classes and tree count."""
+    import turbocat
+    F = 5
+    trees = _regular_trees(11 + task, 9, F)
+    edges = np.sort(np.random.default_rng(3).standard_normal((F, 255)).astype(np.float32), axis=1)
+    n_edges = np.array([30, 31, 16, 17, 0], np.uint32)
+    src = tmp_path / "ref.tcat"
+    write_tcat_v3_regular(src, trees, edges, n_edges, task, n_classes, 0.1, 6, 1.0,
+                          0.0 if task == 1 else 0.25)
+    model = turbocat.TurboCatRegressor() if task == 2 else turbocat.TurboCatClassifier()
+    model.load(str(src))
+    assert model.n_trees == len(trees)
+    if task != 2:
+        assert model.n_classes_ == n_classes
+    out = tmp_path / "ours.tcat"
+    model.save(str(out))
+    assert open(src, "rb").read() == open(out, "rb").read()
+    imp = model.feature_importance()["gain"]
+    exp = np.zeros(256, np.float64)
+    for t in trees:
+        for k in range(len(t["feature"])):
+            if not t["flags"][k] & 1 and t["gain"][k] > 0:
+                exp[t["feature"][k]] += t["gain"][k]
+    assert np.allclose(imp, exp / exp.sum(), rtol=1e-5)

@@ -336,3 +336,115 @@ def test_rtrain_graph_equals_stepwise(ctx):
     for k in ("feature", "bin", "flags", "left", "right", "value", "n_nodes"):
         assert np.array_equal(a[0]["raw"][k], a[1]["raw"][k]), k
     assert np.array_equal(a[0]["train_preds"], a[1]["train_preds"])
+
+
+@pytest.mark.parametrize("kind", ["regression", "multiclass"])
+def test_rtrain_validation_and_early_stopping(ctx, oracle, kind):
+    """Validation rows after every iteration (booster.cpp:306-371,
+    :1400-1430): the GPU's per-iteration validation losses equal the oracle's
+    replay -- each tree applied to the validation bins (apply_bins, NaN by
+    default_left), one fused multiply-add, Loss::compute_loss -- within 2e-6
+    relative; the iterations built follow the reference's stopping rule on
+    the GPU's own losses; the trees are those of a fit without validation."""
+    from oracle import LOSS_MSE
+    from turbocat import capi
+    rounds = 4
+    if kind == "regression":
+        # NaN-free: with NaN the reference's child sums leave out the NaN rows
+        # that the partition sends left, and MSE at this learning rate diverges
+        # (the oracle's own replay does too)
+        X, y = data_with_nan(6000, 8, 31)
+        Xv, yv = data_with_nan(1500, 8, 32)
+        loss, K = capi.TC_LOSS_MSE, 1
+    else:
+        X, y = multiclass_data(5000, 8, 3, 41)
+        Xv, yv = multiclass_data(1200, 8, 3, 42)
+        loss, K = capi.TC_LOSS_SOFTMAX, 3
+    ds, edges, ne = ctx.quantize_fit(X)
+    vds = ctx.dataset(Xv, edges, ne)
+    vb = vds.bins()
+    tp = capi.rtrain_params(loss, n_iterations=120, learning_rate=0.5, n_classes=K if K > 1 else 0)
+    tp.tree = capi.rtree_params(max_depth=6, max_leaves=40, min_child_weight=0.5)
+    m, h = ds.rtrain_valid(y, tp, vds, yv, early_stopping_rounds=rounds)
+    built = h["n_built"]
+    assert 1 <= built <= 120
+    vl = h["valid_loss"]
+    kept, best_it, _ = oracle.early_stop(vl, 120, rounds)
+    assert (kept, best_it) == (built, h["best_iteration"])
+    if built < 120:
+        assert h["stopped"]
+        assert all(tr["n_nodes"] == 0 for tr in m["trees"][built * K:])
+    # replay of the validation predictions and losses
+    vp = np.full((len(yv), K), np.float32(m["base"]), np.float32)
+    for it in range(built):
+        for k in range(K):
+            tr = m["trees"][it * K + k]
+            vp[:, k] = oracle.fma_update(vp[:, k].copy(), np.float32(tp.learning_rate),
+                                         oracle.rtree_apply_binned(tr, vb))
+        ref = oracle.compute_loss_f32(LOSS_MSE, yv, vp[:, 0]) if K == 1 else \
+            oracle.compute_loss_mc_f32(yv, vp)
+        assert abs(vl[it] - ref) <= 2e-6 * abs(ref), (it, vl[it], ref)
+    # validation does not change the trees
+    plain = ds.rtrain(y, tp)
+    for a, b in zip(m["trees"][:built * K], plain["trees"][:built * K]):
+        for key in ("feature", "bin", "flags", "left", "right", "value"):
+            assert np.array_equal(a[key], b[key]), key
+
+
+def test_dropin_mode_small_and_multiclass(tmp_path):
+    """The drop-in classes: mode='small' grows regular trees and predicts
+    through predict_raw_fast's binning (rule 1); labels with three classes
+    train the reference's multiclass loop and predict through apply_bins
+    (rule 0) and softmax; both equal the C-ABI path bit for bit, survive a
+    TCAT save/load, and fit the training data."""
+    import turbocat
+    from turbocat import capi
+    ctx = capi.Context(0)
+    # continuous features: with more than 16 edges predict_raw_fast bins a value
+    # equal to an edge one lower than training did (std::lower_bound), so on a
+    # low-cardinality feature, every value of which is an edge, the reference's
+    # mode="small" predictions of its own training rows are off (reproduced
+    # bit for bit in test_rpredict_matches_oracle)
+    X, yr = regression_data(3000, 6, 51)
+    y = (yr > 0).astype(np.float32)
+    clf = turbocat.TurboCatClassifier(mode="small", n_estimators=20, learning_rate=0.3,
+                                      max_depth=5, subsample=1.0, colsample_bytree=1.0,
+                                      verbosity=0).fit(X, y)
+    assert clf.n_trees == 20
+    p = clf.predict_proba(X)
+    assert p.shape == (3000, 2) and p.dtype == np.float32 and np.allclose(p.sum(1), 1.0)
+    assert (clf.predict(X) == y).mean() > 0.8
+    ds, edges, ne = ctx.quantize_fit(X)
+    tp = capi.rtrain_params(capi.TC_LOSS_LOGLOSS, n_iterations=20, learning_rate=0.3, seed=42)
+    tp.tree = capi.rtree_params(max_depth=5, max_leaves=64, lambda_l2=1.0)
+    m = ds.rtrain(y, tp)
+    ens = capi.REnsemble(ctx, m["trees"], m["weight"], edges, ne, 6, rule=1, base=m["base"])
+    assert np.array_equal(p[:, 1], ens.predict(X, transform=1))
+    path = tmp_path / "small.tcat"
+    clf.save(str(path))
+    again = turbocat.TurboCatClassifier().load(str(path))
+    assert np.array_equal(again.predict_proba(X), p)
+
+    from conftest import multiclass_data
+    Xm, ym = multiclass_data(3000, 6, 3, 52)
+    mc = turbocat.TurboCatClassifier(n_estimators=15, learning_rate=0.3, max_depth=5,
+                                     subsample=1.0, colsample_bytree=1.0, verbosity=0).fit(Xm, ym)
+    assert mc.n_classes_ == 3 and mc.n_trees == 45
+    pm = mc.predict_proba(Xm)
+    assert pm.shape == (3000, 3) and np.allclose(pm.sum(1), 1.0, atol=1e-6)
+    assert (mc.predict(Xm) == ym).mean() > 0.7
+    dsm, em, nem = ctx.quantize_fit(Xm)
+    tpm = capi.rtrain_params(capi.TC_LOSS_SOFTMAX, n_iterations=15, learning_rate=0.3, n_classes=3)
+    tpm.tree = capi.rtree_params(max_depth=5, max_leaves=64, lambda_l2=1.0)
+    mm = dsm.rtrain(ym, tpm)
+    ensm = capi.REnsemble(ctx, mm["trees"], mm["weight"], em, nem, 6, rule=0, K=3,
+                          tree_class=mm["tree_class"], base=0.0)
+    assert np.array_equal(pm, ensm.predict(Xm, transform=1))
+    path = tmp_path / "multi.tcat"
+    mc.save(str(path))
+    again = turbocat.TurboCatClassifier().load(str(path))
+    assert again.n_classes_ == 3 and np.array_equal(again.predict_proba(Xm), pm)
+    reg = turbocat.TurboCatRegressor(mode="small", n_estimators=30, subsample=1.0,
+                                     colsample_bytree=1.0, verbosity=0).fit(X, yr)
+    assert reg.n_trees == 30
+    assert np.corrcoef(reg.predict(X), yr)[0, 1] > 0.8

@@ -1,14 +1,18 @@
 // pybind11 module `_turbocat`: the reference's binding surface
-// (python/bindings.cpp:37-1225) for the symmetric-tree path, backed by the
-// MI355X booster.  Same class names, keyword arguments, defaults, return
-// types and error behaviour (RuntimeError "Model not fitted. Call fit()
-// first." before fit; ValueError for invalid arguments).  Options whose
-// reference code path is not the symmetric-tree hot path (mode="small",
-// ordered boosting, categorical features, multiclass, early-stopping
-// metrics other than the loss, non-MSE/LogLoss losses) raise ValueError
-// naming the option instead of silently training something else.  A
-// validation set early-stops on its loss (booster.cpp:255-371); use_goss
-// samples each tree's rows by GOSS (booster.cpp:207-212).
+// (python/bindings.cpp:37-1225), backed by the MI355X booster.  Same class
+// names, keyword arguments, defaults, return types and error behaviour
+// (RuntimeError "Model not fitted. Call fit() first." before fit; ValueError
+// for invalid arguments).  mode="auto"/"large" grows symmetric trees,
+// mode="small" regular depthwise trees (bindings.cpp:96-103, :150-158);
+// labels with more than two classes train the reference's multiclass loop
+// (regular per-class trees, booster.cpp:1263-1400) whatever the mode, and
+// predict through those trees.  Options whose reference code path lies
+// outside these (lossguide growth, GradTree, ordered boosting, categorical
+// features, interactions, early-stopping metrics other than the loss, other
+// losses, GOSS/MVS with regular trees) raise ValueError naming the option
+// instead of silently training something else.  A validation set early-stops
+// on its loss (booster.cpp:255-371, :1400-1430); use_goss samples each
+// symmetric tree's rows by GOSS (booster.cpp:207-212).
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
@@ -37,8 +41,10 @@ Matrix as_matrix(const FArray& X) {
 }
 
 void reject(bool cond, const std::string& what) {
-  if (cond) throw std::invalid_argument(what + " is not supported by the MI355X symmetric-tree backend");
+  if (cond) throw std::invalid_argument(what + " is not supported by the MI355X backend");
 }
+
+bool lossguide(const std::string& g) { return g == "lossguide" || g == "leaf" || g == "leafwise"; }
 
 // The validation set of fit (an empty X_val means none; bindings.cpp:261-279).
 Matrix validation(const FArray& X_val, const FArray& y_val, uint32_t F) {
@@ -99,7 +105,10 @@ class TurboCatClassifier {
     auto_interactions_ = auto_interactions;
     p_.early_stopping_rounds = static_cast<uint32_t>(std::max(0, early_stopping_rounds));
     es_metric_ = early_stopping_metric;
-    (void)use_symmetric; (void)grow_policy;
+    grow_policy_ = grow_policy;
+    // mode "small": regular trees, "large": symmetric, "auto": symmetric at fit
+    // (bindings.cpp:80, :99-103, :154-158)
+    p_.use_symmetric = mode == "small" ? false : (mode == "large" || mode == "auto" ? true : use_symmetric);
     (void)cat_encoding; (void)max_interactions; (void)interaction_method; (void)n_threads;
   }
 
@@ -107,7 +116,6 @@ class TurboCatClassifier {
     const Matrix m = as_matrix(X);
     if (y.ndim() != 1) throw std::runtime_error("y must be 1-dimensional");
     if (static_cast<uint32_t>(y.shape(0)) != m.n) throw std::invalid_argument("X and y differ in rows");
-    reject(mode_ == "small", "mode='small' (regular trees)");
     reject(use_gradtree_, "use_gradtree");
     reject(use_ordered_, "use_ordered_boosting");
     reject(auto_interactions_, "auto_interactions");
@@ -122,9 +130,18 @@ class TurboCatClassifier {
     float max_label = 0.f;
     for (uint32_t i = 0; i < m.n; ++i) max_label = std::max(max_label, y.data()[i]);
     n_classes_ = static_cast<uint32_t>(max_label) + 1;  // bindings.cpp:161-167
-    reject(n_classes_ > 2, "multiclass classification");
-    n_classes_ = 2;
-    booster_ = std::make_unique<Booster>(p_);
+    if (n_classes_ <= 2) n_classes_ = 2;
+    reject(n_classes_ > 16, "more than 16 classes");
+    Params p = p_;
+    p.n_classes = n_classes_;
+    if (n_classes_ > 2) {  // CrossEntropy over regular per-class trees (bindings.cpp:193-198)
+      p.task = tcamd::kMulticlass;
+      p.loss = TC_LOSS_SOFTMAX;
+    }
+    const bool regular = !p.use_symmetric || n_classes_ > 2;
+    reject(regular && lossguide(grow_policy_), "grow_policy='" + grow_policy_ + "' (lossguide growth)");
+    reject(regular && p.use_goss, "use_goss with regular trees");
+    booster_ = std::make_unique<Booster>(p);
     if (p_.verbosity > 0)
       std::printf("[TurboCat/MI355X] fit: n_samples=%u, n_features=%u, n_estimators=%u\n", m.n, m.F,
                   p_.n_estimators);
@@ -137,6 +154,15 @@ class TurboCatClassifier {
   py::array_t<float> predict_proba_nobinning_fast(FArray X) {
     fitted();
     const Matrix m = as_matrix(X);
+    if (booster_->outputs() > 1) {  // multiclass: n x K softmax (booster.cpp:1522-1536)
+      const uint32_t K = booster_->outputs();
+      py::array_t<float> out({static_cast<py::ssize_t>(m.n), static_cast<py::ssize_t>(K)});
+      {
+        py::gil_scoped_release nogil;
+        booster_->predict_proba(m.data, m.n, m.F, out.mutable_data());
+      }
+      return out;
+    }
     std::vector<float> p1(m.n);
     {
       py::gil_scoped_release nogil;
@@ -157,6 +183,15 @@ class TurboCatClassifier {
     auto p = proba.unchecked<2>();
     py::array_t<int> out(proba.shape(0));
     auto r = out.mutable_unchecked<1>();
+    if (proba.shape(1) > 2) {  // argmax, the first maximum wins (bindings.cpp:399-411)
+      for (py::ssize_t i = 0; i < proba.shape(0); ++i) {
+        int best = 0;
+        for (py::ssize_t c = 1; c < proba.shape(1); ++c)
+          if (p(i, c) > p(i, best)) best = static_cast<int>(c);
+        r(i) = best;
+      }
+      return out;
+    }
     for (py::ssize_t i = 0; i < proba.shape(0); ++i) r(i) = p(i, 1) >= 0.5f ? 1 : 0;
     return out;
   }
@@ -175,7 +210,7 @@ class TurboCatClassifier {
     auto b = std::make_unique<Booster>(p_);
     b->load(path);
     booster_ = std::move(b);
-    n_classes_ = 2;
+    n_classes_ = booster_->n_classes();
   }
 
   py::dict get_params() const {
@@ -204,7 +239,7 @@ class TurboCatClassifier {
     if (!booster_ || !booster_->fitted()) throw std::runtime_error("Model not fitted. Call fit() first.");
   }
   Params p_;
-  std::string mode_, loss_, es_metric_ = "loss";
+  std::string mode_, loss_, es_metric_ = "loss", grow_policy_ = "depthwise";
   bool use_gradtree_ = false, use_ordered_ = false, auto_interactions_ = false;
   uint32_t n_classes_ = 2;
   std::unique_ptr<Booster> booster_;
@@ -236,17 +271,21 @@ class TurboCatRegressor {
     p_.goss_top_rate = goss_top_rate;
     p_.goss_other_rate = goss_other_rate;
     p_.early_stopping_rounds = static_cast<uint32_t>(std::max(0, early_stopping_rounds));
-    (void)max_leaves;
-    (void)grow_policy; (void)n_threads;
+    if (max_leaves < 1) throw std::invalid_argument("max_leaves must be >= 1");
+    p_.max_leaves = static_cast<uint32_t>(max_leaves);
+    p_.use_symmetric = mode != "small";  // bindings.cpp:584-588, :615-620
+    grow_policy_ = grow_policy;
+    (void)n_threads;
   }
 
   void fit(FArray X, FArray y, FArray X_val, FArray y_val) {
     const Matrix m = as_matrix(X);
     if (y.ndim() != 1) throw std::runtime_error("y must be 1-dimensional");
     if (static_cast<uint32_t>(y.shape(0)) != m.n) throw std::invalid_argument("X and y differ in rows");
-    reject(mode_ == "small", "mode='small' (regular trees)");
     const Matrix v = validation(X_val, y_val, m.F);
     reject(!(loss_ == "mse" || loss_ == "l2"), "loss='" + loss_ + "'");
+    reject(!p_.use_symmetric && lossguide(grow_policy_), "grow_policy='" + grow_policy_ + "' (lossguide growth)");
+    reject(!p_.use_symmetric && p_.use_goss, "use_goss with regular trees");
     booster_ = std::make_unique<Booster>(p_);
     if (p_.verbosity > 0)
       std::printf("[TurboCat/MI355X] fit: n_samples=%u, n_features=%u, n_estimators=%u\n", m.n, m.F,
@@ -299,7 +338,8 @@ class TurboCatRegressor {
 
   std::string debug_info() const {
     if (!booster_ || !booster_->fitted()) return "TurboCatRegressor (not fitted), MI355X backend";
-    return "TurboCatRegressor: " + std::to_string(booster_->n_trees()) + " symmetric trees, depth " +
+    return "TurboCatRegressor: " + std::to_string(booster_->n_trees()) +
+           (booster_->regular() ? " regular trees, max depth " : " symmetric trees, depth ") +
            std::to_string(booster_->depth_slots()) + ", " + std::to_string(booster_->n_features()) +
            " features, base " + std::to_string(booster_->base_prediction()) + ", MI355X backend";
   }
@@ -313,14 +353,14 @@ class TurboCatRegressor {
     if (!booster_ || !booster_->fitted()) throw std::runtime_error("Model not fitted. Call fit() first.");
   }
   Params p_;
-  std::string mode_, loss_, es_metric_ = "loss";
+  std::string mode_, loss_, es_metric_ = "loss", grow_policy_ = "depthwise";
   std::unique_ptr<Booster> booster_;
 };
 
 }  // namespace
 
 PYBIND11_MODULE(_turbocat, m) {
-  m.doc() = "TurboCat symmetric-tree gradient boosting on MI355X (gfx950)";
+  m.doc() = "TurboCat gradient boosting (symmetric and regular trees) on MI355X (gfx950)";
   py::register_exception_translator([](std::exception_ptr p) {
     try {
       if (p) std::rethrow_exception(p);
@@ -329,7 +369,7 @@ PYBIND11_MODULE(_turbocat, m) {
     }
   });
   m.def("print_info", []() {
-    py::print("TurboCat (MI355X / gfx950 backend): symmetric trees, HIP kernels, C-ABI turbocat_hip.h");
+    py::print("TurboCat (MI355X / gfx950 backend): symmetric and regular trees, HIP kernels, C-ABI turbocat_hip.h");
   });
 
   py::class_<TurboCatClassifier>(m, "TurboCatClassifier")

@@ -1,8 +1,10 @@
 // Host C++ booster over the C-ABI: the MI355X counterpart of the reference's
-// Booster for the symmetric-tree path (src/boosting/booster.cpp:47-465,
-// :899-945, :1049-1203).  Training, binning and prediction run on the GPU
-// through include/turbocat_hip.h; this class owns the model on the host,
-// keeps it resident on the device for prediction, and reads/writes the
+// Booster (src/boosting/booster.cpp:47-493, :599-945, :1049-1203, multiclass
+// :1263-1482) for symmetric trees and for regular depthwise trees (mode
+// "small", and every multiclass fit: the reference's train_multiclass always
+// grows regular per-class trees).  Training, binning and prediction run on
+// the GPU through include/turbocat_hip.h; this class owns the model on the
+// host, keeps it resident on the device for prediction, and reads/writes the
 // reference's TCAT v3 model files.
 #pragma once
 #include <cstdint>
@@ -39,6 +41,14 @@ struct Params {
   uint32_t min_samples_leaf = 1;
   float min_child_weight = 1.0f;
   float min_split_gain = 0.0f;
+  // regular trees (TreeConfig::max_leaves, gamma, leaf_smooth,
+  // learn_missing_direction; config.hpp:29-52)
+  bool use_symmetric = true;  // false: regular depthwise trees (mode "small")
+  uint32_t max_leaves = 64;
+  float gamma = 0.0f;
+  float leaf_smooth = 0.0f;
+  bool learn_missing = true;
+  uint32_t n_classes = 2;     // > 2: softmax over regular per-class trees (train_multiclass)
   // run
   int32_t loss = TC_LOSS_MSE;
   Task task = kRegression;
@@ -64,9 +74,12 @@ class Booster {
   // stops on the validation loss (booster.cpp:255-371) and keeps the trees built.
   void fit(const float* X, const float* y, uint32_t n, uint32_t F, const float* Xv = nullptr,
            const float* yv = nullptr, uint32_t nv = 0);
-  // Raw scores + base (predict_raw_nobinning_fast, booster.cpp:899-925).
+  // Raw scores + base (predict_raw_nobinning_fast, booster.cpp:899-925; regular
+  // trees: predict_raw_fast, :680-871).  Multiclass: n x K.
   void predict_raw(const float* X, uint32_t n, uint32_t F, float* out);
-  // sigmoid(raw) (predict_proba_nobinning_fast, booster.cpp:927-945).
+  // sigmoid(raw) (predict_proba_nobinning_fast, booster.cpp:927-945);
+  // multiclass: softmax over the classes of apply_bins + the multiclass
+  // ensemble (predict_proba_multiclass, booster.cpp:1506-1536), n x K.
   void predict_proba(const float* X, uint32_t n, uint32_t F, float* out);
   // Per-feature sum of split gains, 256 entries (symmetric_tree.cpp:892, :1176).
   std::vector<float> feature_importance() const;
@@ -75,6 +88,9 @@ class Booster {
   void load(const std::string& path);         // TCAT v1-3 symmetric (booster.cpp:1111-1203)
 
   bool fitted() const { return fitted_; }
+  bool regular() const { return regular_; }
+  uint32_t n_classes() const { return K_ > 1 ? K_ : 2u; }
+  uint32_t outputs() const { return K_; }
   uint32_t n_trees() const { return T_; }
   uint32_t n_features() const { return F_; }
   float base_prediction() const { return base_; }
@@ -93,9 +109,24 @@ class Booster {
   const std::vector<float>& tree_weights() const { return weights_; }
   const std::vector<std::vector<float>>& bin_edges() const { return edges_; }
 
+  // regular model: tree t's nodes are [tree_off[t], tree_off[t+1]) with
+  // tree-local child ids (the reference's Tree::nodes)
+  const std::vector<uint32_t>& tree_off() const { return roff_; }
+  const std::vector<uint16_t>& node_feature() const { return rfeat_; }
+  const std::vector<uint8_t>& node_bin() const { return rbin_; }
+  const std::vector<uint8_t>& node_flags() const { return rflags_; }
+  const std::vector<uint32_t>& node_left() const { return rleft_; }
+  const std::vector<uint32_t>& node_right() const { return rright_; }
+  const std::vector<float>& node_value() const { return rvalue_; }
+  const std::vector<float>& node_gain() const { return rgain_; }
+  const std::vector<uint32_t>& tree_class() const { return rclass_; }
+
  private:
   void ensure_ensemble();
   void drop_ensemble();
+  void fit_regular(const float* X, const float* y, uint32_t n, uint32_t F, const float* Xv,
+                   const float* yv, uint32_t nv);
+  tc_rensemble* regular_ensemble(int rule);
 
   Params p_;
   tc_ctx* ctx_ = nullptr;
@@ -109,6 +140,14 @@ class Booster {
   std::vector<uint8_t> bin_;
   std::vector<float> fthr_, gain_, leaves_, weights_;
   std::vector<uint16_t> depth_;
+  // regular trees
+  bool regular_ = false;
+  uint32_t K_ = 1;              // outputs: classes of a multiclass model, else 1
+  std::vector<uint32_t> roff_, rleft_, rright_, rclass_;
+  std::vector<uint16_t> rfeat_, rdepth_, rnleaves_;
+  std::vector<uint8_t> rbin_, rflags_;
+  std::vector<float> rvalue_, rgain_;
+  tc_rensemble* rens_[2] = {nullptr, nullptr};  // by binning rule
 };
 
 // One process-wide context per device (the reference binds one Booster to

@@ -724,9 +724,8 @@ __global__ void __launch_bounds__(64) k_rfinish(RJob* __restrict__ jobs, RLists 
 
 // Tree::predict(dataset, row) (tree.cpp:929-950): the binned walk of tree t
 // of job J for one row, NaN (bin 255) by the node's default_left.
-__device__ inline float rtree_row_value(const RJob& J, size_t o, uint32_t row) {
+__device__ inline float rtree_row_value(const RJob& J, size_t o, const uint8_t* br) {
   uint32_t k = 0;
-  const uint8_t* br = J.bins_rm + static_cast<size_t>(row) * J.Fp;
   for (uint32_t step = 0; step <= kRMaxDepth; ++step) {
     const uint8_t fl = J.m_flags[o + k];
     if (fl & 1u) return J.m_value[o + k];
@@ -766,7 +765,8 @@ __global__ void __launch_bounds__(256) k_rupdate(const RJob* __restrict__ jobs, 
         pr = *pp;
         const int t = J.st->tree;
         if (t >= 0 && t < static_cast<int>(J.n_trees) && !J.st->stopped && J.m_nodes[t] > 0)
-          pr = __fmaf_rn(lr, rtree_row_value(J, static_cast<size_t>(t) * J.NC, row), pr);
+          pr = __fmaf_rn(lr, rtree_row_value(J, static_cast<size_t>(t) * J.NC,
+                                             J.bins_rm + static_cast<size_t>(row) * J.Fp), pr);
       }
       *pp = pr;
       s_p[k][tid] = pr;
@@ -827,6 +827,99 @@ __global__ void __launch_bounds__(256) k_rupdate(const RJob* __restrict__ jobs, 
   }
 }
 
+// Validation rows after each iteration (booster.cpp:306-330, :1400-1409):
+// every job's new tree applied to the validation rows' bins (Tree::predict
+// over Dataset::apply_bins with the training edges), v_preds += lr * value
+// (fused), then each row's loss term -- MSE / LogLoss as Loss::compute_loss
+// (loss.cpp:110-121, :336-346), softmax rows -(log max(p_label, 1e-7))
+// (CrossEntropyLoss::compute_loss, loss.cpp:232-244) -- summed per
+// 1024-row tile in double, in a fixed order.
+__global__ void __launch_bounds__(256) k_rvalid(const RJob* __restrict__ jobs, uint32_t njobs,
+                                                int loss, float lr, const uint8_t* __restrict__ vbins,
+                                                uint32_t vFp, uint32_t Nv, const float* __restrict__ vy,
+                                                float* __restrict__ vpreds, double* __restrict__ part) {
+  const RJob& J0 = jobs[0];
+  const int t = J0.st->tree;
+  if (t < 0 || t >= static_cast<int>(J0.n_trees) || J0.st->stopped) return;
+  const uint32_t tid = threadIdx.x, lw = blockIdx.x;
+  __shared__ float s_p[kMaxClasses][256];
+  double acc = 0.0;
+  for (int q = 0; q < kGradTile / 256; ++q) {
+    const uint32_t row = lw * kGradTile + q * 256 + tid;
+    if (row >= Nv) continue;
+    const uint8_t* br = vbins + static_cast<size_t>(row) * vFp;
+    for (uint32_t k = 0; k < njobs; ++k) {
+      const RJob& J = jobs[k];
+      float* pp = vpreds + static_cast<size_t>(k) * Nv + row;
+      float pr = *pp;
+      if (J.m_nodes[t] > 0) pr = __fmaf_rn(lr, rtree_row_value(J, static_cast<size_t>(t) * J.NC, br), pr);
+      *pp = pr;
+      s_p[k][tid] = pr;
+    }
+    if (loss != kSoftmax) {
+      acc += static_cast<double>(row_loss(loss, s_p[0][tid], vy[row]));
+      continue;
+    }
+    float e[kMaxClasses];
+    float m = -INFINITY;
+#pragma unroll
+    for (uint32_t k = 0; k < kMaxClasses; ++k) {
+      e[k] = k < njobs ? s_p[k][tid] : -INFINITY;
+      m = fmaxf(m, e[k]);
+    }
+    float sum = 0.f;
+#pragma unroll
+    for (uint32_t k = 0; k < kMaxClasses; ++k) {
+      if (k >= njobs) break;
+      e[k] = expf(__fsub_rn(e[k], m));
+      sum = __fadd_rn(sum, e[k]);
+    }
+    const uint32_t lab = static_cast<uint32_t>(vy[row]);
+    float pl = 0.f;
+#pragma unroll
+    for (uint32_t k = 0; k < kMaxClasses; ++k)
+      if (k == lab) pl = __fdiv_rn(e[k], sum);
+    acc += static_cast<double>(-logf(fmaxf(pl, 1e-7f)));
+  }
+  tile_sum_store(acc, part + lw);
+}
+
+// After k_rvalid: the iteration's mean validation loss and the reference's
+// early-stopping rule (booster.cpp:339-371 / :1411-1430: an improvement is a
+// loss below the best by more than 1e-10; stop once iter >= max(50, n / 10)
+// and es_rounds iterations in a row did not improve).  On a stop every job is
+// stopped and *built records the iterations built (the stopping one
+// included).
+__global__ void __launch_bounds__(256) k_reval(RJob* __restrict__ jobs, uint32_t njobs,
+                                               uint32_t Nv, const double* __restrict__ part,
+                                               uint32_t ntiles, float* __restrict__ vloss,
+                                               uint32_t es_rounds, uint32_t es_min_iter,
+                                               uint32_t* __restrict__ built) {
+  TreeState* st = jobs[0].st;
+  const int t = st->tree;
+  if (t < 0 || t >= static_cast<int>(jobs[0].n_trees) || st->stopped) return;
+  const uint32_t tid = threadIdx.x;
+  __shared__ double s_w[4];
+  double a = 0.0;
+  for (uint32_t w = tid; w < ntiles; w += 256) a += part[w];
+  a = warp_sum_d(a);
+  if ((tid & 63) == 0) s_w[tid >> 6] = a;
+  __syncthreads();
+  if (tid != 0) return;
+  const float vl = static_cast<float>((((s_w[0] + s_w[1]) + s_w[2]) + s_w[3]) / Nv);
+  vloss[t] = vl;
+  if (vl < __fsub_rn(st->es_best, 1e-10f)) {
+    st->es_best = vl;
+    st->es_best_iter = static_cast<uint32_t>(t);
+    st->es_count = 0;
+  } else {
+    st->es_count += 1;
+  }
+  *built = static_cast<uint32_t>(t) + 1u;
+  if (static_cast<uint32_t>(t) >= es_min_iter && st->es_count >= es_rounds)
+    for (uint32_t k = 0; k < njobs; ++k) jobs[k].st->stopped = 1;
+}
+
 // Teacher forcing: job 0's (g, h) from given arrays, their maxima merged into
 // the tree state for the scales.
 __global__ void __launch_bounds__(256) k_rset_gh(RJob* __restrict__ jobs, const float* g,
@@ -860,6 +953,9 @@ struct RSpec {
   float subsample = 1.f, colsample = 1.f;
   uint64_t seed = 42;
   float base = 0.f;
+  const tc_dataset* valid = nullptr;  // validation rows, binned with the training edges
+  const float* y_valid = nullptr;
+  uint32_t es_rounds = 0;
 };
 
 void check_rtree_params(const tc_rtree_params* p) {
@@ -946,7 +1042,30 @@ class RTrainer {
     if (update)
       hipLaunchKernelGGL(k_rupdate, dim3(grad_wg_), dim3(256), 0, s, d_jobs_, nj_, 1, loss_, s_.lr,
                          d_base_, y_, preds_, gh_);
+    if (update && Nv_) {
+      hipLaunchKernelGGL(k_rvalid, dim3(v_tiles_), dim3(256), 0, s, d_jobs_, nj_, loss_, s_.lr,
+                         s_.valid->bins_rm, s_.valid->Fp, Nv_, vy_, vpreds_, vpart_);
+      hipLaunchKernelGGL(k_reval, dim3(1), dim3(256), 0, s, d_jobs_, nj_, Nv_, vpart_, v_tiles_,
+                         vloss_, s_.es_rounds, std::max(50u, s_.n_iter / 10), built_);
+    }
     TC_HIP(hipGetLastError());
+  }
+
+  // Early-stopping outcome (valid only after the fit has run).
+  void history(tc_train_history* h) {
+    TreeState st{};
+    TC_HIP(hipMemcpy(&st, h_jobs_[0].st, sizeof(st), hipMemcpyDeviceToHost));
+    uint32_t built = s_.n_iter;
+    if (Nv_) TC_HIP(hipMemcpy(&built, built_, 4, hipMemcpyDeviceToHost));
+    h->n_trees_built = built;
+    h->stopped_early = st.stopped ? 1 : 0;
+    h->best_iteration = st.es_best_iter;
+    h->best_valid_loss = Nv_ ? st.es_best : 0.0f;
+    if (h->train_loss) std::fill(h->train_loss, h->train_loss + s_.n_iter, 0.0f);
+    if (h->valid_loss) {
+      std::fill(h->valid_loss, h->valid_loss + s_.n_iter, 0.0f);
+      if (Nv_) TC_HIP(hipMemcpy(h->valid_loss, vloss_, 4ull * built, hipMemcpyDeviceToHost));
+    }
   }
 
   // Booster::train / train_multiclass: per iteration the row sample (drawn on
@@ -969,6 +1088,7 @@ class RTrainer {
     TC_HIP(hipEventRecord(a, s));
     try {
       for (uint32_t it = 0; it < s_.n_iter; ++it) {
+        if (stop_seen(it)) break;
         prepare_sample();
         if (ge) TC_HIP(hipGraphLaunch(ge, s));
         else enqueue_iteration(true);
@@ -1063,6 +1183,25 @@ class RTrainer {
     return p;
   }
 
+  // With a validation set: every 16 iterations the stop flag is copied to
+  // pinned memory; once a copy shows the fit stopped, nothing more is
+  // enqueued (iterations enqueued meanwhile do nothing on the device).
+  bool stop_seen(uint32_t it) {
+    if (!Nv_ || it % 16 != 0) return false;
+    if (stop_pending_ && hipEventQuery(stop_ev_) == hipSuccess) {
+      stop_pending_ = false;
+      if (*stop_host_) return true;
+    }
+    if (!stop_pending_) {
+      TC_HIP(hipMemcpyAsync(stop_host_, reinterpret_cast<const char*>(h_jobs_[0].st) +
+                                            offsetof(TreeState, stopped),
+                            4, hipMemcpyDeviceToHost, ctx_->stream));
+      TC_HIP(hipEventRecord(stop_ev_, ctx_->stream));
+      stop_pending_ = true;
+    }
+    return false;
+  }
+
   // The root's rows: all N (identity) or n rows of the sample buffer.
   void set_root(uint32_t n, bool sampled) {
     for (uint32_t j = 0; j < nj_; ++j) {
@@ -1140,6 +1279,26 @@ class RTrainer {
         TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_[b]), 4ull * N_, hipHostMallocDefault));
         TC_HIP(hipEventCreateWithFlags(&ev_[b], hipEventDisableTiming));
       }
+    if (s_.valid) {  // validation rows ride after each iteration's update
+      TC_REQUIRE(s_.valid->F == F_, "validation set: feature count differs from training");
+      TC_REQUIRE(s_.y_valid != nullptr, "validation set without labels");
+      Nv_ = s_.valid->N;
+    }
+    if (Nv_) {
+      v_tiles_ = ceil_div(Nv_, kGradTile);
+      vy_ = alloc<float>(Nv_);
+      TC_HIP(hipMemcpyAsync(vy_, s_.y_valid, 4ull * Nv_, hipMemcpyHostToDevice, s));
+      std::vector<float> vp(static_cast<size_t>(Nv_) * nj_, s_.base);
+      vpreds_ = alloc<float>(vp.size());
+      TC_HIP(hipMemcpyAsync(vpreds_, vp.data(), 4ull * vp.size(), hipMemcpyHostToDevice, s));
+      TC_HIP(hipStreamSynchronize(s));  // vp leaves scope
+      vpart_ = alloc<double>(v_tiles_);
+      vloss_ = alloc_zero<float>(std::max(1u, s_.n_iter));
+      built_ = alloc_zero<uint32_t>(1);
+      TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&stop_host_), 4, hipHostMallocDefault));
+      *stop_host_ = 0;
+      TC_HIP(hipEventCreateWithFlags(&stop_ev_, hipEventDisableTiming));
+    }
     h_jobs_.resize(nj_);
     h_hjobs_.resize(static_cast<size_t>(D_) * nj_);
     for (uint32_t j = 0; j < nj_; ++j) {
@@ -1234,6 +1393,10 @@ class RTrainer {
     if (ctx_ && ctx_->stream) (void)hipStreamSynchronize(ctx_->stream);
     for (void* p : allocs_) dfree(p);
     allocs_.clear();
+    if (stop_host_) hfree(stop_host_);
+    stop_host_ = nullptr;
+    if (stop_ev_) (void)hipEventDestroy(stop_ev_);
+    stop_ev_ = nullptr;
     for (int b = 0; b < 2; ++b) {
       if (host_[b]) hfree(host_[b]);
       if (ev_[b]) (void)hipEventDestroy(ev_[b]);
@@ -1263,6 +1426,16 @@ class RTrainer {
   uint32_t* rows0_dummy_ = nullptr;
   unsigned long long* hist_rows_ = nullptr;
   std::vector<void*> allocs_;
+  // validation
+  uint32_t Nv_ = 0, v_tiles_ = 0;
+  float* vy_ = nullptr;
+  float* vpreds_ = nullptr;
+  double* vpart_ = nullptr;
+  float* vloss_ = nullptr;
+  uint32_t* built_ = nullptr;
+  uint32_t* stop_host_ = nullptr;
+  hipEvent_t stop_ev_ = nullptr;
+  bool stop_pending_ = false;
   // row sampling
   bool sampling_ = false, colsample_ = false;
   uint32_t n_sample_ = 0;
@@ -1419,9 +1592,16 @@ int32_t tc_rtree_build(tc_dataset* ds, const float* g, const float* h, const uin
 }
 
 int32_t tc_rtrain(tc_dataset* ds, const float* y, const tc_rtrain_params* p, tc_rmodel* out) {
+  return tc_rtrain_valid(ds, y, p, nullptr, nullptr, 0, out, nullptr);
+}
+
+int32_t tc_rtrain_valid(tc_dataset* ds, const float* y, const tc_rtrain_params* p,
+                        tc_dataset* valid, const float* y_valid, uint32_t early_stopping_rounds,
+                        tc_rmodel* out, tc_train_history* history) {
   using namespace tc;
   return guard([&] {
     TC_REQUIRE(ds && y && p && out, "tc_rtrain: NULL argument");
+    TC_REQUIRE(!valid || y_valid, "tc_rtrain_valid: validation set without labels");
     TC_REQUIRE(p->loss == TC_LOSS_MSE || p->loss == TC_LOSS_LOGLOSS || p->loss == TC_LOSS_SOFTMAX,
                "tc_rtrain: unsupported loss");
     TC_REQUIRE(p->learning_rate == p->learning_rate, "learning_rate is NaN");
@@ -1444,14 +1624,24 @@ int32_t tc_rtrain(tc_dataset* ds, const float* y, const tc_rtrain_params* p, tc_
     s.colsample = p->colsample_bytree;
     s.seed = p->seed;
     s.base = init_prediction_host(p->loss, y, ds->N);
+    s.valid = valid;
+    s.y_valid = y_valid;
+    s.es_rounds = early_stopping_rounds;
+    if (valid && p->loss == TC_LOSS_SOFTMAX)
+      for (uint32_t i = 0; i < valid->N; ++i)
+        TC_REQUIRE(y_valid[i] >= 0.f && y_valid[i] < static_cast<float>(p->n_classes) &&
+                       y_valid[i] == std::floor(y_valid[i]),
+                   "validation labels must be class indices 0..K-1");
     if (p->n_iterations == 0) {
       out->base = s.base;
+      if (history) *history = tc_train_history{history->train_loss, history->valid_loss, 0, 0, 0, 0.f};
       return;
     }
     RTrainer tr(ds->ctx, s);
     tr.init_gradients();
     tr.run(p->use_graph != 0);
     tr.download(out, p->n_iterations);
+    if (history) tr.history(history);
   });
 }
 

@@ -1,11 +1,14 @@
-"""TurboCat on MI355X: drop-in ``turbocat`` package for the symmetric-tree path.
+"""TurboCat on MI355X: drop-in ``turbocat`` package.
 
 Mirrors the reference's Python wrappers (python/turbocat/__init__.py:290-874):
 the same class names, constructor keywords and defaults, numpy in / numpy out
-(``predict`` -> int32 labels, ``predict_proba`` -> float32 (n, 2)), and the
-same errors.  Quantization, training and prediction run on the GPU through
-the C-ABI in include/turbocat_hip.h (``_turbocat`` is the pybind11 module over
-it; there is no CPU fallback -- without a gfx950 device the calls raise).
+(``predict`` -> int32 labels, ``predict_proba`` -> float32 (n, n_classes)),
+and the same errors.  mode="auto"/"large" grows symmetric trees, mode="small"
+regular depthwise trees; labels with more than two classes train the
+reference's multiclass loop (one regular tree per class and iteration).
+Quantization, training and prediction run on the GPU through the C-ABI in
+include/turbocat_hip.h (``_turbocat`` is the pybind11 module over it; there is
+no CPU fallback -- without a gfx950 device the calls raise).
 """
 from __future__ import annotations
 
@@ -23,7 +26,8 @@ def _f32(a):
 
 
 class TurboCatClassifier:
-    """Gradient-boosted symmetric trees for binary classification (LogLoss)."""
+    """Gradient-boosted trees for binary (LogLoss) and multiclass (softmax)
+    classification."""
 
     def __init__(self, n_estimators=1000, learning_rate=0.1, max_depth=6, max_bins=255,
                  subsample=0.8, colsample_bytree=0.8, min_child_weight=1.0, lambda_l2=1.0,
@@ -70,7 +74,9 @@ class TurboCatClassifier:
 
     def predict(self, X, threshold=0.5):
         proba = self.predict_proba(X)
-        return (proba[:, 1] >= threshold).astype(np.int32)
+        if proba.shape[1] == 2:
+            return (proba[:, 1] >= threshold).astype(np.int32)
+        return np.argmax(proba, axis=1).astype(np.int32)
 
     def feature_importance(self):
         return self._model.feature_importance()
@@ -99,7 +105,7 @@ class TurboCatClassifier:
 
 
 class TurboCatRegressor:
-    """Gradient-boosted symmetric trees for regression (MSE)."""
+    """Gradient-boosted trees for regression (MSE)."""
 
     def __init__(self, n_estimators=1000, learning_rate=0.1, max_depth=6, loss="mse",
                  mode="auto", subsample=0.8, colsample_bytree=0.8, use_goss=False,

@@ -41,7 +41,7 @@ EXPORTS = [
     "tc_serve_wait", "tc_serve_cancel", "tc_serve_stats", "tc_serve_counters",
     "tc_serve_latencies", "tc_serve_stop",
     "tc_host_alloc", "tc_host_free",
-    "tc_rtree_build", "tc_rtrain", "tc_rensemble_create", "tc_rensemble_destroy", "tc_rpredict",
+    "tc_rtree_build", "tc_rtrain", "tc_rtrain_valid", "tc_rensemble_create", "tc_rensemble_destroy", "tc_rpredict",
     "tc_rpredict_dev",
 ]
 
@@ -264,6 +264,8 @@ def load(path: str = LIB_PATH):
         "tc_serve_stop": [V], "tc_host_alloc": [c_size_t, V], "tc_host_free": [V],
         "tc_rtree_build": [V, V, V, V, U, POINTER(RTreeParams), POINTER(RModel)],
         "tc_rtrain": [V, V, POINTER(RTrainParams), POINTER(RModel)],
+        "tc_rtrain_valid": [V, V, POINTER(RTrainParams), V, V, U, POINTER(RModel),
+                            POINTER(TrainHistory)],
         "tc_rensemble_create": [V, U, U, V, V, V, V, V, V, V, V, V, V, V, V, U, I, V],
         "tc_rensemble_destroy": [V], "tc_rpredict": [V, V, U, U, I, V],
         "tc_rpredict_dev": [V, V, U, U, I, V],
@@ -505,6 +507,25 @@ class Dataset:
         return dict(trees=rtrees(a), weight=np.full(T, params.learning_rate, np.float32),
                     tree_class=np.tile(np.arange(K, dtype=np.uint32), params.n_iterations),
                     K=K, base=float(m.base), train_preds=a["train_preds"], raw=a)
+
+    def rtrain_valid(self, y, params: RTrainParams, valid, y_valid, early_stopping_rounds=50):
+        """tc_rtrain_valid: (model dict as rtrain, history dict)."""
+        K = params.n_classes if params.loss == TC_LOSS_SOFTMAX else 1
+        T = params.n_iterations * K
+        m, a = _new_rmodel(T, params.tree.max_depth, self.n, K)
+        vl = np.zeros(max(params.n_iterations, 1), np.float32)
+        h = TrainHistory(None, _p(vl), 0, 0, 0, 0.0)
+        check(self.ctx.L.tc_rtrain_valid(self.h, _p(np.ascontiguousarray(y, np.float32)),
+                                         byref(params), valid.h,
+                                         _p(np.ascontiguousarray(y_valid, np.float32)),
+                                         early_stopping_rounds, byref(m), byref(h)))
+        model = dict(trees=rtrees(a), weight=np.full(T, params.learning_rate, np.float32),
+                     tree_class=np.tile(np.arange(K, dtype=np.uint32), params.n_iterations),
+                     K=K, base=float(m.base), train_preds=a["train_preds"], raw=a)
+        hist = dict(valid_loss=vl[:h.n_trees_built].copy(), n_built=h.n_trees_built,
+                    stopped=bool(h.stopped_early), best_iteration=h.best_iteration,
+                    best_valid_loss=h.best_valid_loss)
+        return model, hist
 
     def train_valid(self, y, params: TrainParams, valid=None, y_valid=None,
                     early_stopping_rounds=50):
