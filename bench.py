@@ -526,6 +526,117 @@ def run_multiclass(args):
     print(json.dumps(out), flush=True)
 
 
+def run_regular(args):
+    """Regular (non-oblivious) depthwise trees, the reference's mode="small"
+    (binary LogLoss on the default matrix) or, with --rclasses > 2, its
+    multiclass loop (one regular tree per class and iteration, on the
+    multiclass matrix): iterations/s, the histogram kernel against the HBM
+    roofline (algorithmic bytes rows x (F + 8) per histogrammed node), and
+    resident prediction rows/s.  The reference's regular trees cannot be
+    compiled here (its tree.cpp needs Eigen), so the CPU column is the
+    repository's FP64 restatement (oracle/) building one tree on a bounded
+    row sample, single thread."""
+    from turbocat import capi
+    multi = args.rclasses > 2
+    D, lr = args.depth, 0.1
+    if multi:
+        n, F, K = args.mc_rows, args.mc_features, args.rclasses
+        log(f"[bench] generating {n} x {F} multiclass workload, {K} classes")
+        X, y = multiclass_workload(n, F, K, seed=1004)
+        loss = capi.TC_LOSS_SOFTMAX
+    else:
+        n, F, K = args.rows, args.features, 1
+        log(f"[bench] generating {n} x {F} binary workload")
+        X, y = binary_workload(n, F, seed=1001)
+        loss = capi.TC_LOSS_LOGLOSS
+    ctx = capi.Context(0)
+    t0 = time.perf_counter()
+    ds, edges, ne = ctx.quantize_fit(X)
+    quant_s = time.perf_counter() - t0
+
+    def fit(iters, time_hist):
+        prm = capi.rtrain_params(loss, n_iterations=iters, learning_rate=lr,
+                                 n_classes=K if multi else 0, time_hist=time_hist)
+        prm.tree = capi.rtree_params(max_depth=D, max_leaves=args.max_leaves, lambda_l2=1.0)
+        return ds.rtrain(y, prm)
+
+    fit(args.warmup, 0)  # first use: allocations, code objects
+    T_timed = args.steps
+    m = fit(T_timed, 0)
+    loop_ms, _ = ctx.last_train_timing()  # device events around the boosting loop
+    its = T_timed / (loop_ms * 1e-3)
+    R = 5
+    fit(R, 1)
+    _, hist_ms = ctx.last_train_timing()
+    rows, launches = ctx.last_train_hist_rows()
+    alg = rows * (F + 8)
+    achieved = alg / (hist_ms * 1e-3) / 1e9
+    nodes = np.array([t["n_nodes"] for t in m["trees"]])
+    log(f"[bench] {its:.1f} iterations/s ({loop_ms / T_timed:.3f} ms per iteration of {K} "
+        f"tree(s), {nodes.mean():.1f} nodes per tree); histogram {launches} launches, "
+        f"{hist_ms / launches * 1e3:.1f} us avg -> {achieved:.0f} GB/s")
+    import ctypes
+    ens = capi.REnsemble(ctx, m["trees"], m["weight"], edges, ne, F, rule=0 if multi else 1, K=K,
+                         tree_class=m["tree_class"] if multi else None, base=m["base"])
+    dx, do = ctypes.c_void_p(), ctypes.c_void_p()
+    capi.check(ctx.L.tc_device_alloc(ctx.h, X.nbytes, ctypes.byref(dx)))
+    capi.check(ctx.L.tc_device_alloc(ctx.h, 4 * n * K, ctypes.byref(do)))
+    capi.check(ctx.L.tc_memcpy_h2d(ctx.h, dx, X.ctypes.data_as(ctypes.c_void_p), X.nbytes))
+    ens.predict_dev(dx, n, do)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ens.predict_dev(dx, n, do)
+    ctx.sync()
+    pred = n / (time.perf_counter() - t0)
+    capi.check(ctx.L.tc_device_free(ctx.h, dx))
+    capi.check(ctx.L.tc_device_free(ctx.h, do))
+    log(f"[bench] predict {pred / 1e6:.1f} M rows/s resident ({len(m['trees'])} trees)")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import Oracle, rtree_params
+        o = Oracle()
+        ns = min(n, args.cpu_rows)
+        bins = ds.bins()[:, :ns].copy() if ns < n else ds.bins()
+        if multi:
+            g, h = o.gradients_mc_f32(y[:ns], np.zeros((ns, K), np.float32))
+            g, h = np.ascontiguousarray(g[:, 0]), np.ascontiguousarray(h[:, 0])
+        else:
+            p0 = np.full(ns, o.init_prediction(loss, y[:ns]), np.float32)
+            g, h = o.gradients_f32(loss, y[:ns], p0)
+        t0 = time.perf_counter()
+        o.rtree_f64(bins, g, h, rtree_params(max_depth=D, max_leaves=args.max_leaves))
+        sec = time.perf_counter() - t0
+        cpu = dict(value=1.0 / (sec * K), unit="iterations/s", cores=1, kind="port",
+                   host_cpu=host_cpu_model(),
+                   sample=(f"FP64 regular-tree restatement (oracle/, single thread) building one "
+                           f"depth-{D} tree on the first {ns} rows: {sec:.2f} s, x {K} trees per "
+                           f"iteration; for all {n} rows expect about {n / ns:.0f}x longer "
+                           f"(extrapolated, not measured)"))
+    out = {
+        "metric": METRIC, "value": round(its, 3), "unit": "iterations/s", "n_gpus": 1,
+        "steps": T_timed, "warmup": args.warmup, "ms_per_step": round(loop_ms / T_timed, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": (f"multiclass softmax {n} rows x {F} float32 N(0,1) features, "
+                                f"{K} classes, the reference's multiclass loop: one regular "
+                                f"depthwise tree per class and iteration, depth {D}, max_leaves "
+                                f"{args.max_leaves}, lr {lr}, lambda 1") if multi
+                   else (f"binary logloss {n} rows x {F} float32 features "
+                         f"({F - F // 5} N(0,1), {F // 5} U[0,1) with 5% duplicates), regular "
+                         f"depthwise trees (mode='small') depth {D}, max_leaves "
+                         f"{args.max_leaves}, lr {lr}, lambda 1"),
+                   "rows": n, "features": F, "classes": K, "depth": D},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "kernel": "k_hist", "avg_launch_us": round(hist_ms / launches * 1e3, 2),
+                     "algorithmic_bytes_per_launch": int(alg / launches)},
+        "cpu_baseline": cpu,
+        "extra": {"quantize_s": round(quant_s, 3), "predict_rows_per_s_resident": round(pred, 1),
+                  "mean_nodes_per_tree": round(float(nodes.mean()), 2)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_copy(args):
     """Calibration for HBM byte counters: a plain device copy of a known
     size (256 MiB read, 256 MiB written per copy)."""
@@ -578,7 +689,7 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="train",
-                    choices=["train", "serve", "batched", "multiclass", "copy"])
+                    choices=["train", "serve", "batched", "multiclass", "regular", "copy"])
     ap.add_argument("--pmc-summary", metavar="DIR", default=None,
                     help="print per-dispatch averages of the rocprofv3 counter CSVs under DIR")
     ap.add_argument("--kernel", default="k_hist", help="kernel name regex for --pmc-summary")
@@ -606,6 +717,13 @@ def main():
     ap.add_argument("--classes", type=int, default=10)
     ap.add_argument("--mc-cpu-rows", type=int, default=200_000)
     ap.add_argument("--max-job-rows", type=int, default=2_000_000)
+    ap.add_argument("--depth", type=int, default=6, help="regular workload: max_depth")
+    ap.add_argument("--rclasses", type=int, default=0,
+                    help="regular workload: > 2 runs the reference's multiclass loop with this many "
+                         "classes on the multiclass matrix (--mc-rows x --mc-features)")
+    ap.add_argument("--max-leaves", type=int, default=64, help="regular workload: max_leaves")
+    ap.add_argument("--cpu-rows", type=int, default=100_000,
+                    help="regular workload: rows of the CPU restatement's sample")
     args = ap.parse_args()
     if args.pmc_summary:
         pmc_summary(args.pmc_summary, args.kernel)
@@ -615,7 +733,7 @@ def main():
                                    "across GPUs is designed but not implemented)"}))
         sys.exit(2)
     {"train": run_train, "serve": run_serve, "batched": run_batched,
-     "multiclass": run_multiclass, "copy": run_copy}[args.workload](args)
+     "multiclass": run_multiclass, "regular": run_regular, "copy": run_copy}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -247,6 +247,10 @@ int32_t tc_train_batch(tc_ctx* ctx, uint32_t n_jobs, tc_dataset* const* ds,
  * of the last tc_train / tc_train_batch on this context, and of its
  * histogram kernels. */
 int32_t tc_last_train_timing(tc_ctx* ctx, double* loop_ms, double* hist_ms);
+/* Rows fed to histogram launches by the last regular-tree fit on this
+ * context (every tree of every class), and those launches' count: the
+ * algorithmic bytes are rows x (F + 8).  No reference counterpart. */
+int32_t tc_last_train_hist_rows(tc_ctx* ctx, uint64_t* rows, uint32_t* launches);
 
 /* Stepwise form of tc_train / tc_train_batch for callers that drive the
  * boosting loop themselves (benchmarks, schedulers): create once, then each
@@ -358,6 +362,8 @@ typedef struct {
   uint64_t seed;
   uint32_t n_classes;          /* TC_LOSS_SOFTMAX: K in [3, 16]; otherwise 0 or 1 */
   int32_t use_graph;           /* 1: one captured hipGraph replay per iteration */
+  int32_t time_hist;           /* 1: plain launches with HIP events around every histogram
+                                  launch (tc_last_train_timing, tc_last_train_hist_rows) */
 } tc_rtrain_params;
 
 /* Regular trees in the reference's numbering (Tree::nodes in add_node order;

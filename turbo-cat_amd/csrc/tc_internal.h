@@ -19,6 +19,8 @@ struct tc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   double last_loop_ms = 0.0, last_hist_ms = 0.0;
+  uint64_t last_hist_rows = 0;     // regular-tree fits (tc_last_train_hist_rows)
+  uint32_t last_hist_launches = 0;
   std::atomic<int> refs{1};
   // two pinned host buffers for streaming rows to the device, made on first use
   static constexpr size_t kStageBytes = size_t(32) << 20;

@@ -664,55 +664,100 @@ __global__ void __launch_bounds__(kRPlanThreads) k_rplan(RJob* __restrict__ jobs
 // (tree.cpp:310-319), else takes ids n, n + 1 for its children (add_node,
 // :357-363, :648-653) whose smaller one is visited first (:370-390) -- and
 // writes the tree in that numbering (leaf values: make_leaf, :655-681).
-// Also empties the level-0 lists for the next tree.
-__global__ void __launch_bounds__(64) k_rfinish(RJob* __restrict__ jobs, RLists L) {
+// The walk is a chain of dependent reads, so the workgroup first copies what
+// it needs of every grown node into LDS (up to kRFinLds nodes; a larger tree
+// walks the HBM records).  Also empties the level-0 lists for the next tree.
+struct RFin {
+  float G, H;
+  uint32_t C, child;
+  float gain;
+  uint16_t feat, depth;
+  uint8_t bin, split, dl, right_first;
+};
+constexpr uint32_t kRFinLds = 2048;
+
+__device__ inline RFin rfin_of(const RNode& X) {
+  RFin r;
+  r.G = X.G; r.H = X.H; r.C = X.C; r.child = X.child; r.gain = X.gain;
+  r.feat = X.feat; r.depth = X.depth; r.bin = X.bin;
+  r.split = X.state == kRSplit ? 1 : 0;
+  r.dl = X.dl; r.right_first = X.right_first;
+  return r;
+}
+
+__global__ void __launch_bounds__(256) k_rfinish(RJob* __restrict__ jobs, RLists L) {
   RJob& J = jobs[blockIdx.y];
-  if (threadIdx.x != 0) return;
-  *L.hist_count[0] = 0;
-  *L.part_count[0] = 0;
+  __shared__ RFin s_n[kRFinLds];
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) {
+    *L.hist_count[0] = 0;
+    *L.part_count[0] = 0;
+  }
   TreeState* st = J.st;
   const int t = st->tree;
   if (t < 0 || t >= static_cast<int>(J.n_trees) || st->stopped) return;
+  uint32_t ng = 0;  // grown nodes: every level's
+  for (uint32_t d = 0; d <= J.D; ++d) ng = max(ng, J.lvl_begin[d] + J.lvl_count[d]);
+  const bool staged = ng <= kRFinLds;
+  if (staged)
+    for (uint32_t i = tid; i < ng; i += 256) s_n[i] = rfin_of(J.nodes[i]);
+  __syncthreads();
+  if (tid != 0) return;
+  // the job's fields in registers: the stores below could otherwise alias the
+  // job record and force a reload (and a wait) after each of them
   const size_t o = static_cast<size_t>(t) * J.NC;
+  uint16_t* __restrict__ m_feat = J.m_feat + o;
+  uint8_t* __restrict__ m_bin = J.m_bin + o;
+  uint8_t* __restrict__ m_flags = J.m_flags + o;
+  uint32_t* __restrict__ m_left = J.m_left + o;
+  uint32_t* __restrict__ m_right = J.m_right + o;
+  float* __restrict__ m_value = J.m_value + o;
+  float* __restrict__ m_gain = J.m_gain + o;
+  const RNode* __restrict__ nodes = J.nodes;
+  const uint32_t max_leaves = J.max_leaves;
+  const float lam = J.lambda_l2, smooth = J.leaf_smooth, mds = J.max_delta_step;
+  __shared__ uint32_t s_sg[2 * kRMaxDepth + 4], s_sid[2 * kRMaxDepth + 4];  // the DFS stack
   uint32_t next = 0, leaves = 0, depth = 0;
   if (J.lvl_count[0] > 0) {
-    uint32_t stack_g[2 * kRMaxDepth + 4], stack_id[2 * kRMaxDepth + 4];
-    int sp = 0;
-    stack_g[0] = 0;
-    stack_id[0] = 0;
-    sp = 1;
+    int sp = 1;
+    s_sg[0] = 0;
+    s_sid[0] = 0;
     next = 1;
     while (sp > 0) {
       --sp;
-      const uint32_t g = stack_g[sp], id = stack_id[sp];
-      const RNode X = J.nodes[g];
+      const uint32_t g = s_sg[sp], id = s_sid[sp];
+      const RFin X = staged ? s_n[g] : rfin_of(nodes[g]);
       depth = max(depth, static_cast<uint32_t>(X.depth));
-      if (X.state == kRSplit && leaves < J.max_leaves) {
+      if (X.split && leaves < max_leaves) {
         const uint32_t l = next, r = next + 1;
         next += 2;
-        J.m_feat[o + id] = X.feat;
-        J.m_bin[o + id] = X.bin;
-        J.m_flags[o + id] = static_cast<uint8_t>((X.dl & 1u) << 1);
-        J.m_left[o + id] = l;
-        J.m_right[o + id] = r;
-        J.m_value[o + id] = 0.f;
-        J.m_gain[o + id] = X.gain;
+        m_feat[id] = X.feat;
+        m_bin[id] = X.bin;
+        m_flags[id] = static_cast<uint8_t>((X.dl & 1u) << 1);
+        m_left[id] = l;
+        m_right[id] = r;
+        m_value[id] = 0.f;
+        m_gain[id] = X.gain;
         // pushed larger first: the smaller child's subtree is built first
-        if (X.right_first) {
-          stack_g[sp] = X.child; stack_id[sp] = l; ++sp;
-          stack_g[sp] = X.child + 1; stack_id[sp] = r; ++sp;
-        } else {
-          stack_g[sp] = X.child + 1; stack_id[sp] = r; ++sp;
-          stack_g[sp] = X.child; stack_id[sp] = l; ++sp;
-        }
+        const uint32_t a = X.right_first ? X.child : X.child + 1, b = X.right_first ? X.child + 1 : X.child;
+        s_sg[sp] = a; s_sid[sp] = X.right_first ? l : r;
+        s_sg[sp + 1] = b; s_sid[sp + 1] = X.right_first ? r : l;
+        sp += 2;
       } else {
-        J.m_feat[o + id] = 0;
-        J.m_bin[o + id] = 0;
-        J.m_flags[o + id] = 3u;  // is_leaf, default_left of a fresh node
-        J.m_left[o + id] = 0;
-        J.m_right[o + id] = 0;
-        J.m_value[o + id] = rleaf_value(J, X.G, X.H, X.C);
-        J.m_gain[o + id] = 0.f;
+        m_feat[id] = 0;
+        m_bin[id] = 0;
+        m_flags[id] = 3u;  // is_leaf, default_left of a fresh node
+        m_left[id] = 0;
+        m_right[id] = 0;
+        // Tree::make_leaf (tree.cpp:655-681)
+        float v = __fdiv_rn(-X.G, __fadd_rn(X.H, lam));
+        if (smooth > 0.0f) {
+          const float c = static_cast<float>(X.C);
+          v = __fmul_rn(v, __fdiv_rn(c, __fadd_rn(c, smooth)));
+        }
+        if (mds > 0.0f) v = fmaxf(-mds, fminf(mds, v));
+        m_value[id] = v;
+        m_gain[id] = 0.f;
         ++leaves;
       }
     }
@@ -724,7 +769,7 @@ __global__ void __launch_bounds__(64) k_rfinish(RJob* __restrict__ jobs, RLists 
 
 // Tree::predict(dataset, row) (tree.cpp:929-950): the binned walk of tree t
 // of job J for one row, NaN (bin 255) by the node's default_left.
-__device__ inline float rtree_row_value(const RJob& J, size_t o, const uint8_t* br) {
+__device__ __forceinline__ float rtree_row_value(const RJob& J, size_t o, const uint8_t* br) {
   uint32_t k = 0;
   for (uint32_t step = 0; step <= kRMaxDepth; ++step) {
     const uint8_t fl = J.m_flags[o + k];
@@ -734,6 +779,63 @@ __device__ inline float rtree_row_value(const RJob& J, size_t o, const uint8_t* 
     else k = b > J.m_bin[o + k] ? J.m_right[o + k] : J.m_left[o + k];
   }
   return J.m_value[o + k];
+}
+
+// The last tree of every job staged in LDS for the row walks (12 bytes a
+// node; u16 child ids), when the trees hold at most kRWalkLds nodes in all;
+// otherwise the walks read the model arrays.  s_base[k]: job k's first node,
+// s_base[njobs]: the total (0 when no job has a tree to apply).
+struct RWalk {
+  uint32_t a;   // feature | bin << 16 | flags << 24
+  uint32_t lr;  // left | right << 16
+  float v;      // leaf value
+};
+constexpr uint32_t kRWalkLds = 2560;
+
+__device__ inline bool rwalk_stage(const RJob* jobs, uint32_t njobs, RWalk* s_w, uint32_t* s_base) {
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) {
+    uint32_t tot = 0;
+    for (uint32_t k = 0; k < njobs; ++k) {
+      s_base[k] = tot;
+      const RJob& J = jobs[k];
+      const int t = J.st->tree;
+      if (t >= 0 && t < static_cast<int>(J.n_trees) && !J.st->stopped) tot += J.m_nodes[t];
+    }
+    s_base[njobs] = tot;
+  }
+  __syncthreads();
+  const bool staged = s_base[njobs] <= kRWalkLds;
+  if (staged)
+    for (uint32_t k = 0; k < njobs; ++k) {
+      const RJob& J = jobs[k];
+      const uint32_t nn = s_base[k + 1] - s_base[k];
+      const size_t o = static_cast<size_t>(max(J.st->tree, 0)) * J.NC;
+      for (uint32_t i = tid; i < nn; i += blockDim.x) {
+        RWalk w;
+        w.a = J.m_feat[o + i] | (static_cast<uint32_t>(J.m_bin[o + i]) << 16) |
+              (static_cast<uint32_t>(J.m_flags[o + i]) << 24);
+        w.lr = J.m_left[o + i] | (J.m_right[o + i] << 16);
+        w.v = J.m_value[o + i];
+        s_w[s_base[k] + i] = w;
+      }
+    }
+  __syncthreads();
+  return staged;
+}
+
+// Tree::predict(dataset, row) over a staged tree.
+__device__ __forceinline__ float rwalk_lds(const RWalk* nd, const uint8_t* br) {
+  uint32_t k = 0;
+  for (uint32_t step = 0; step <= kRMaxDepth; ++step) {
+    const RWalk x = nd[k];
+    const uint32_t fl = x.a >> 24;
+    if (fl & 1u) return x.v;
+    const uint8_t b = br[x.a & 0xFFFFu];
+    const bool right = b == 255 ? !(fl & 2u) : b > ((x.a >> 16) & 0xFFu);
+    k = right ? (x.lr >> 16) : (x.lr & 0xFFFFu);
+  }
+  return nd[k].v;
 }
 
 // The prediction update and the next gradients, over rows (1024 per
@@ -750,41 +852,66 @@ __global__ void __launch_bounds__(256) k_rupdate(const RJob* __restrict__ jobs, 
   const RJob& J0 = jobs[0];
   const uint32_t N = J0.N, tid = threadIdx.x, lw = blockIdx.x;
   const uint32_t vecN = (N / 8u) * 8u;
+  constexpr int kQ = kGradTile / 256;  // rows per thread: lw * 1024 + q * 256 + tid
+  static_assert(kQ == 4, "four rows per thread");
+  __shared__ RWalk s_w[kRWalkLds];
+  __shared__ uint32_t s_base[kMaxClasses + 1];
+  uint32_t row[kQ];
+#pragma unroll
+  for (int q = 0; q < kQ; ++q) row[q] = lw * kGradTile + q * 256 + tid;
+  // phase 1: every job's prediction of the thread's rows
+  const bool staged = mode == 1 && rwalk_stage(jobs, njobs, s_w, s_base);
+  const uint8_t* br[kQ];
+#pragma unroll
+  for (int q = 0; q < kQ; ++q) br[q] = J0.bins_rm + static_cast<size_t>(min(row[q], N - 1)) * J0.Fp;
+  float p0[kQ];  // job 0's new predictions, kept for a single-output loss's gradients
+  for (uint32_t k = 0; k < njobs; ++k) {
+    const RJob& J = jobs[k];
+    float* pk = preds + static_cast<size_t>(k) * N;
+    float pr[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) pr[q] = mode == 1 && row[q] < N ? pk[row[q]] : base[k];
+    if (mode == 1) {
+      const int t = J.st->tree;
+      if (t >= 0 && t < static_cast<int>(J.n_trees) && !J.st->stopped && J.m_nodes[t] > 0) {
+        float v[kQ];
+        if (staged) {
+#pragma unroll
+          for (int q = 0; q < kQ; ++q) v[q] = rwalk_lds(s_w + s_base[k], br[q]);
+        } else {
+#pragma unroll
+          for (int q = 0; q < kQ; ++q) v[q] = rtree_row_value(J, static_cast<size_t>(t) * J.NC, br[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) pr[q] = __fmaf_rn(lr, v[q], pr[q]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+      if (row[q] < N) pk[row[q]] = pr[q];
+      if (k == 0) p0[q] = pr[q];
+    }
+  }
+  // phase 2: the gradients of each row from its (just written) predictions
   float gmax[kMaxClasses], hmax[kMaxClasses];
 #pragma unroll
   for (uint32_t k = 0; k < kMaxClasses; ++k) gmax[k] = hmax[k] = 0.f;
-  __shared__ float s_p[kMaxClasses][256];  // this thread's row's predictions (own column only)
-  for (int q = 0; q < kGradTile / 256; ++q) {
-    const uint32_t row = lw * kGradTile + q * 256 + tid;
-    if (row >= N) continue;
-    for (uint32_t k = 0; k < njobs; ++k) {  // the tree walks (not unrolled)
-      const RJob& J = jobs[k];
-      float* pp = preds + static_cast<size_t>(k) * N + row;
-      float pr = base[k];
-      if (mode == 1) {
-        pr = *pp;
-        const int t = J.st->tree;
-        if (t >= 0 && t < static_cast<int>(J.n_trees) && !J.st->stopped && J.m_nodes[t] > 0)
-          pr = __fmaf_rn(lr, rtree_row_value(J, static_cast<size_t>(t) * J.NC,
-                                             J.bins_rm + static_cast<size_t>(row) * J.Fp), pr);
-      }
-      *pp = pr;
-      s_p[k][tid] = pr;
-    }
-    float e[kMaxClasses];
 #pragma unroll
-    for (uint32_t k = 0; k < kMaxClasses; ++k) e[k] = k < njobs ? s_p[k][tid] : 0.f;
+  for (int q = 0; q < kQ; ++q) {
+    const uint32_t r = row[q];
+    if (r >= N) continue;
     if (loss != kSoftmax) {
-      const float2 v = row_gradient(loss, e[0], y[row], row < vecN);
-      gh[row] = v;
+      const float2 v = row_gradient(loss, p0[q], y[r], r < vecN);
+      gh[r] = v;
       gmax[0] = fmaxf(gmax[0], fabsf(v.x));
       hmax[0] = fmaxf(hmax[0], fabsf(v.y));
       continue;
     }
+    float e[kMaxClasses];
     float m = -INFINITY;
 #pragma unroll
     for (uint32_t k = 0; k < kMaxClasses; ++k) {
-      if (k >= njobs) break;
+      e[k] = k < njobs ? preds[static_cast<size_t>(k) * N + r] : -INFINITY;
       m = fmaxf(m, e[k]);
     }
     float sum = 0.f;
@@ -794,7 +921,7 @@ __global__ void __launch_bounds__(256) k_rupdate(const RJob* __restrict__ jobs, 
       e[k] = expf(__fsub_rn(e[k], m));
       sum = __fadd_rn(sum, e[k]);
     }
-    const uint32_t lab = static_cast<uint32_t>(y[row]);
+    const uint32_t lab = static_cast<uint32_t>(y[r]);
 #pragma unroll
     for (uint32_t k = 0; k < kMaxClasses; ++k) {
       if (k >= njobs) break;
@@ -802,13 +929,12 @@ __global__ void __launch_bounds__(256) k_rupdate(const RJob* __restrict__ jobs, 
       const float g = __fsub_rn(p, k == lab ? 1.0f : 0.0f);
       const float hh = __fmul_rn(p, __fsub_rn(1.0f, p));
       const float h = hh > 1e-6f ? hh : 1e-6f;
-      gh[static_cast<size_t>(k) * N + row] = make_float2(g, h);
+      gh[static_cast<size_t>(k) * N + r] = make_float2(g, h);
       gmax[k] = fmaxf(gmax[k], fabsf(g));
       hmax[k] = fmaxf(hmax[k], h);
     }
   }
-  // each job's tile maxima (one store of both per workgroup; the LDS of the
-  // reduction is reused job after job)
+  // each job's tile maxima
   __shared__ float s_m[2][kMaxClasses][4];
 #pragma unroll
   for (uint32_t k = 0; k < kMaxClasses; ++k) {
@@ -843,6 +969,9 @@ __global__ void __launch_bounds__(256) k_rvalid(const RJob* __restrict__ jobs, u
   if (t < 0 || t >= static_cast<int>(J0.n_trees) || J0.st->stopped) return;
   const uint32_t tid = threadIdx.x, lw = blockIdx.x;
   __shared__ float s_p[kMaxClasses][256];
+  __shared__ RWalk s_w[kRWalkLds];
+  __shared__ uint32_t s_base[kMaxClasses + 1];
+  const bool staged = rwalk_stage(jobs, njobs, s_w, s_base);
   double acc = 0.0;
   for (int q = 0; q < kGradTile / 256; ++q) {
     const uint32_t row = lw * kGradTile + q * 256 + tid;
@@ -852,7 +981,9 @@ __global__ void __launch_bounds__(256) k_rvalid(const RJob* __restrict__ jobs, u
       const RJob& J = jobs[k];
       float* pp = vpreds + static_cast<size_t>(k) * Nv + row;
       float pr = *pp;
-      if (J.m_nodes[t] > 0) pr = __fmaf_rn(lr, rtree_row_value(J, static_cast<size_t>(t) * J.NC, br), pr);
+      if (J.m_nodes[t] > 0)
+        pr = __fmaf_rn(lr, staged ? rwalk_lds(s_w + s_base[k], br)
+                                  : rtree_row_value(J, static_cast<size_t>(t) * J.NC, br), pr);
       *pp = pr;
       s_p[k][tid] = pr;
     }
@@ -1020,16 +1151,27 @@ class RTrainer {
   }
 
   // One iteration: a tree per job, then (update) predictions and gradients.
-  void enqueue_iteration(bool update) {
+  // hev (optional) receives an event pair around every histogram launch.
+  void enqueue_iteration(bool update, std::vector<std::pair<hipEvent_t, hipEvent_t>>* hev = nullptr) {
     hipStream_t s = ctx_->stream;
     hipLaunchKernelGGL(k_rbegin, dim3(1, nj_), dim3(256), 0, s, d_jobs_, lists_);
     for (uint32_t d = 0; d < D_; ++d) {
       const uint32_t par = d & 1u;
       auto* kh = mode_ == 1 ? (nj_ == 1 ? k_hist<kHistUnroll, 1, true> : k_hist<kHistUnroll, 1, false>)
                             : (nj_ == 1 ? k_hist<kHistUnroll, 2, true> : k_hist<kHistUnroll, 2, false>);
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      if (hev) {
+        TC_HIP(hipEventCreate(&e0));
+        TC_HIP(hipEventCreate(&e1));
+        TC_HIP(hipEventRecord(e0, s));
+      }
       hipLaunchKernelGGL(kh, dim3(round8(hist_cap_[d])), dim3(kHistThreads), 0, s,
                          d_hjobs_ + static_cast<size_t>(d) * nj_, lists_.hist[par], lists_.hist_count[par],
                          0u, partials_, Lists{});
+      if (hev) {
+        TC_HIP(hipEventRecord(e1, s));
+        hev->emplace_back(e0, e1);
+      }
       hipLaunchKernelGGL(k_rsplit, dim3(F_, d == 0 ? 1u : (1u << (d - 1)), nj_), dim3(256), 0, s,
                          d_jobs_, d, partials_);
       hipLaunchKernelGGL(k_rdecide, dim3(std::max(1u, (1u << d) / 4u), nj_), dim3(256), 0, s, d_jobs_,
@@ -1038,7 +1180,7 @@ class RTrainer {
                          lists_.part[par], lists_.part_count[par], d);
       hipLaunchKernelGGL(k_rplan, dim3(1, nj_), dim3(kRPlanThreads), 0, s, d_jobs_, d, lists_);
     }
-    hipLaunchKernelGGL(k_rfinish, dim3(1, nj_), dim3(64), 0, s, d_jobs_, lists_);
+    hipLaunchKernelGGL(k_rfinish, dim3(1, nj_), dim3(256), 0, s, d_jobs_, lists_);
     if (update)
       hipLaunchKernelGGL(k_rupdate, dim3(grad_wg_), dim3(256), 0, s, d_jobs_, nj_, 1, loss_, s_.lr,
                          d_base_, y_, preds_, gh_);
@@ -1070,9 +1212,11 @@ class RTrainer {
 
   // Booster::train / train_multiclass: per iteration the row sample (drawn on
   // the host with the reference's engine), then the iteration's trees.
-  void run(bool graph) {
+  void run(bool graph, bool time_hist = false) {
     hipStream_t s = ctx_->stream;
     hipGraphExec_t ge = nullptr;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> hev;
+    if (time_hist) graph = false;
     if (graph) {
       hipGraph_t g;
       TC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -1091,7 +1235,7 @@ class RTrainer {
         if (stop_seen(it)) break;
         prepare_sample();
         if (ge) TC_HIP(hipGraphLaunch(ge, s));
-        else enqueue_iteration(true);
+        else enqueue_iteration(true, time_hist ? &hev : nullptr);
       }
       TC_HIP(hipEventRecord(b, s));
       TC_HIP(hipStreamSynchronize(s));
@@ -1099,12 +1243,26 @@ class RTrainer {
       if (ge) (void)hipGraphExecDestroy(ge);
       (void)hipEventDestroy(a);
       (void)hipEventDestroy(b);
+      for (auto& e : hev) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+      }
       throw;
     }
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, a, b);
     ctx_->last_loop_ms = ms;
-    ctx_->last_hist_ms = 0.0;
+    double hs = 0.0;
+    for (auto& e : hev) {
+      float x = 0.f;
+      (void)hipEventElapsedTime(&x, e.first, e.second);
+      hs += x;
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+    ctx_->last_hist_ms = hs;
+    ctx_->last_hist_launches = static_cast<uint32_t>(hev.size());
+    ctx_->last_hist_rows = hist_rows();
     if (ge) (void)hipGraphExecDestroy(ge);
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
@@ -1349,7 +1507,7 @@ class RTrainer {
       J.m_nodes = alloc_zero<uint32_t>(s_.n_iter);
       J.m_leaves = alloc_zero<uint32_t>(s_.n_iter);
       J.m_depth = alloc_zero<uint16_t>(s_.n_iter);
-      J.hist_rows_total = j == 0 ? hist_rows_ : nullptr;
+      J.hist_rows_total = hist_rows_;  // every class's trees
       for (uint32_t d = 0; d < D_; ++d) {  // k_hist's record of level d
         JobDev& H = h_hjobs_[static_cast<size_t>(d) * nj_ + j];
         std::memset(&H, 0, sizeof(H));
@@ -1460,6 +1618,11 @@ struct RPNode {
   uint32_t left, right;  // ensemble-global node ids
 };
 
+// One row per lane, 64 rows per workgroup.  STAGED: the tile's rows are
+// copied to LDS first (coalesced; an odd row stride keeps the lanes' reads of
+// one feature on different banks); otherwise (rows too wide for 64 KB) each
+// lane reads its row from HBM.
+template <bool STAGED>
 __global__ void __launch_bounds__(64) k_rpredict(const RPNode* __restrict__ nodes,
                                                  const uint32_t* __restrict__ root,
                                                  const float* __restrict__ weight,
@@ -1469,14 +1632,16 @@ __global__ void __launch_bounds__(64) k_rpredict(const RPNode* __restrict__ node
                                                  uint32_t F, int transform, float* __restrict__ out) {
   extern __shared__ float s_x[];  // 64 rows x (F + 1) floats
   const uint32_t tid = threadIdx.x, r0 = blockIdx.x * 64u, S = F + 1;
-  for (uint32_t i = tid; i < 64u * F; i += 64) {
-    const uint32_t rr = i / F, f = i % F;
-    s_x[rr * S + f] = r0 + rr < n ? X[static_cast<size_t>(r0 + rr) * F + f] : 0.f;
+  if (STAGED) {
+    for (uint32_t i = tid; i < 64u * F; i += 64) {
+      const uint32_t rr = i / F, f = i % F;
+      s_x[rr * S + f] = r0 + rr < n ? X[static_cast<size_t>(r0 + rr) * F + f] : 0.f;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   const uint32_t row = r0 + tid;
   if (row >= n) return;
-  const float* x = s_x + tid * S;
+  const float* x = STAGED ? s_x + tid * S : X + static_cast<size_t>(row) * F;
   float acc[kMaxClasses];
 #pragma unroll
   for (uint32_t c = 0; c < kMaxClasses; ++c) acc[c] = 0.f;
@@ -1556,9 +1721,14 @@ void rpredict_dev(tc_rensemble* e, const float* X, uint32_t n, uint32_t F, int32
   TC_REQUIRE(F == e->F, "tc_rpredict: feature count differs from the ensemble's");
   if (n == 0) return;
   const size_t lds = 64ull * (F + 1) * sizeof(float);
-  TC_REQUIRE(lds <= 64 * 1024, "tc_rpredict: at most 16383 features");
-  hipLaunchKernelGGL(k_rpredict, dim3(ceil_div(n, 64)), dim3(64), lds, e->ctx->stream, e->nodes,
-                     e->root, e->weight, e->tree_class, e->base, e->T, e->K, X, n, F, transform, out);
+  if (lds <= 64 * 1024)
+    hipLaunchKernelGGL(k_rpredict<true>, dim3(ceil_div(n, 64)), dim3(64), lds, e->ctx->stream,
+                       e->nodes, e->root, e->weight, e->tree_class, e->base, e->T, e->K, X, n, F,
+                       transform, out);
+  else
+    hipLaunchKernelGGL(k_rpredict<false>, dim3(ceil_div(n, 64)), dim3(64), 0, e->ctx->stream,
+                       e->nodes, e->root, e->weight, e->tree_class, e->base, e->T, e->K, X, n, F,
+                       transform, out);
   TC_HIP(hipGetLastError());
 }
 
@@ -1639,7 +1809,7 @@ int32_t tc_rtrain_valid(tc_dataset* ds, const float* y, const tc_rtrain_params* 
     }
     RTrainer tr(ds->ctx, s);
     tr.init_gradients();
-    tr.run(p->use_graph != 0);
+    tr.run(p->use_graph != 0, p->time_hist != 0);
     tr.download(out, p->n_iterations);
     if (history) tr.history(history);
   });
@@ -1722,6 +1892,15 @@ int32_t tc_rensemble_create(tc_ctx* ctx, uint32_t n_trees, uint32_t n_classes,
       throw;
     }
     *out = e;
+  });
+}
+
+int32_t tc_last_train_hist_rows(tc_ctx* ctx, uint64_t* rows, uint32_t* launches) {
+  using namespace tc;
+  return guard([&] {
+    TC_REQUIRE(ctx, "NULL context");
+    if (rows) *rows = ctx->last_hist_rows;
+    if (launches) *launches = ctx->last_hist_launches;
   });
 }
 

@@ -42,7 +42,7 @@ EXPORTS = [
     "tc_serve_latencies", "tc_serve_stop",
     "tc_host_alloc", "tc_host_free",
     "tc_rtree_build", "tc_rtrain", "tc_rtrain_valid", "tc_rensemble_create", "tc_rensemble_destroy", "tc_rpredict",
-    "tc_rpredict_dev",
+    "tc_rpredict_dev", "tc_last_train_hist_rows",
 ]
 
 
@@ -91,7 +91,8 @@ class RTreeParams(Structure):
 class RTrainParams(Structure):
     _fields_ = [("loss", c_int32), ("n_iterations", c_uint32), ("learning_rate", c_float),
                 ("tree", RTreeParams), ("subsample", c_float), ("colsample_bytree", c_float),
-                ("seed", c_uint64), ("n_classes", c_uint32), ("use_graph", c_int32)]
+                ("seed", c_uint64), ("n_classes", c_uint32), ("use_graph", c_int32),
+                ("time_hist", c_int32)]
 
 
 class RModel(Structure):
@@ -111,10 +112,10 @@ def rtree_params(max_depth=6, max_leaves=64, max_bins=255, lambda_l2=1.0, min_ch
 
 
 def rtrain_params(loss=TC_LOSS_MSE, n_iterations=100, learning_rate=0.1, subsample=1.0,
-                  colsample_bytree=1.0, seed=42, n_classes=0, use_graph=1,
+                  colsample_bytree=1.0, seed=42, n_classes=0, use_graph=1, time_hist=0,
                   **tree) -> RTrainParams:
     return RTrainParams(loss, n_iterations, learning_rate, rtree_params(**tree), subsample,
-                        colsample_bytree, seed, n_classes, use_graph)
+                        colsample_bytree, seed, n_classes, use_graph, time_hist)
 
 
 def _new_rmodel(n_trees, max_depth, n=0, K=1):
@@ -268,7 +269,7 @@ def load(path: str = LIB_PATH):
                             POINTER(TrainHistory)],
         "tc_rensemble_create": [V, U, U, V, V, V, V, V, V, V, V, V, V, V, V, U, I, V],
         "tc_rensemble_destroy": [V], "tc_rpredict": [V, V, U, U, I, V],
-        "tc_rpredict_dev": [V, V, U, U, I, V],
+        "tc_rpredict_dev": [V, V, U, U, I, V], "tc_last_train_hist_rows": [V, V, V],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -396,6 +397,12 @@ class Context:
             a["weight"] = np.full(params[k].n_trees, params[k].learning_rate, np.float32)
             out.append(a)
         return out
+
+    def last_train_hist_rows(self):
+        """(rows fed to histogram launches, launches) of the last regular-tree fit."""
+        r, k = c_uint64(), c_uint32()
+        check(self.L.tc_last_train_hist_rows(self.h, byref(r), byref(k)))
+        return r.value, k.value
 
     def last_train_timing(self):
         a, b = c_double(), c_double()
