@@ -1,7 +1,8 @@
-/* turbocat_hip.h -- C-ABI of the MI355X (gfx950) TurboCat symmetric-tree hot
- * path: quantization, gradients, histograms, oblivious split scoring, tree
- * build, boosting loop, ensemble prediction, continuous-batching serving and
- * batched multi-job training.
+/* turbocat_hip.h -- C-ABI of the MI355X (gfx950) TurboCat hot path:
+ * quantization, gradients, histograms, oblivious split scoring, tree build,
+ * boosting loop, ensemble prediction, continuous-batching serving and batched
+ * multi-job training for symmetric trees; regular depthwise trees (build,
+ * boosting incl. the reference's multiclass loop, prediction).
  *
  * Conventions
  *   - Every function returns int32_t: TC_OK (0) or a negative TC_E* code;
@@ -25,6 +26,11 @@
  *   Booster::train (symmetric branch)     src/boosting/booster.cpp:47-418
  *   SymmetricEnsemble::predict_batch_raw_fast  include/turbocat/symmetric_tree.hpp:177
  *   SymmetricEnsemble::predict_batch (binned)  include/turbocat/symmetric_tree.hpp:171
+ *   Tree::build (Depthwise)               src/tree/tree.cpp:162-217
+ *   Booster::train (regular branch), Booster::train_multiclass
+ *                                         src/boosting/booster.cpp:438-493, :1263-1400
+ *   TreeEnsemble::predict_batch_optimized / _multiclass_optimized
+ *                                         src/tree/tree.cpp:1471-1709
  */
 #ifndef TURBOCAT_HIP_H
 #define TURBOCAT_HIP_H
