@@ -96,6 +96,7 @@ class Oracle:
             getattr(L, fn).argtypes = [c_int, V, V, V, V, c_uint32]
         L.tco_gradients_mc_f64.argtypes = [V, V, V, V, c_uint32, c_uint32]
         L.tco_gradients_mc_f32.argtypes = [V, V, V, V, c_uint32, c_uint32]
+        L.tco_transform_f32.argtypes = [V, V, c_uint32, c_uint32]
         L.tco_predict_mc_f64.argtypes = [V, V, V, V, V, c_uint32, c_uint32, c_uint32, c_double, V,
                                          c_uint32, c_uint32, V, V]
         L.tco_hist_f32.argtypes = [V, c_uint32, c_uint32, V, V, V, c_uint32, V, V]
@@ -202,6 +203,15 @@ class Oracle:
         g = np.zeros((n, K), np.float32); h = np.zeros((n, K), np.float32)
         self.L.tco_gradients_mc_f32(_p(y), _p(p), _p(g), _p(h), n, K)
         return g, h
+
+    def transform_f32(self, raw, K=1):
+        """Probabilities from raw scores in the reference's float order:
+        sigmoid (K == 1) or per-row softmax (raw (n, K))."""
+        raw = np.ascontiguousarray(raw, np.float32)
+        n = raw.size // K
+        out = np.zeros_like(raw)
+        self.L.tco_transform_f32(_p(raw), _p(out), n, K)
+        return out
 
     def gradients_mc_f64(self, y, p):
         y = np.ascontiguousarray(y, np.float32); p = np.ascontiguousarray(p, np.float32)
@@ -448,6 +458,7 @@ class RefLib:
         L.tcref_dataset_bins.argtypes = [V, V, V, V]
         L.tcref_gradients.argtypes = [c_int, V, V, V, V, c_uint32]
         L.tcref_gradients_mc.argtypes = [V, V, V, V, c_uint32, c_uint32]
+        L.tcref_transform.argtypes = [V, V, c_uint32, c_uint32]
         L.tcref_init_prediction.argtypes = [c_int, V, c_uint32]
         L.tcref_init_prediction.restype = c_float
         L.tcref_compute_loss.argtypes = [c_int, V, V, c_uint32]
@@ -538,6 +549,14 @@ class RefLib:
         g = np.zeros((n, K), np.float32); h = np.zeros((n, K), np.float32)
         self.L.tcref_gradients_mc(_p(y), _p(p), _p(g), _p(h), n, K)
         return g, h
+
+    def transform(self, raw, K=1):
+        """LogLoss::transform_prediction (K == 1) / CrossEntropyLoss::
+        transform_to_proba (K > 1); raw row-major (n, K)."""
+        raw = np.ascontiguousarray(raw, np.float32)
+        out = np.zeros_like(raw)
+        self.L.tcref_transform(_p(raw), _p(out), raw.size // K, K)
+        return out
 
     def predict(self, model, X):
         X = np.ascontiguousarray(X, np.float32)

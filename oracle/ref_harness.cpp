@@ -129,6 +129,21 @@ float tcref_compute_loss_mc(const float* y, const float* p, uint32_t n, uint32_t
     return loss.compute_loss(y, p, n);
 }
 
+// Probabilities from raw scores: LogLoss::transform_prediction (loss.cpp:213-215)
+// for K == 1, CrossEntropyLoss::transform_to_proba (loss.cpp:320-331) for K > 1
+// (raw and out row-major n x K).
+void tcref_transform(const float* raw, float* out, uint32_t n, uint32_t K) {
+    if (K == 1) {
+        LossConfig lc;
+        lc.loss_type = LossType::LogLoss;
+        auto loss = Loss::create(lc, TaskType::BinaryClassification, 2);
+        for (uint32_t i = 0; i < n; ++i) out[i] = loss->transform_prediction(raw[i]);
+    } else {
+        CrossEntropyLoss loss(K);
+        loss.transform_to_proba(raw, out, n);
+    }
+}
+
 float tcref_init_prediction(int loss_id, const float* y, uint32_t n) {
     LossConfig lc;
     lc.loss_type = loss_id == 0 ? LossType::MSE : LossType::LogLoss;

@@ -154,6 +154,31 @@ void tco_gradients_f64(int loss, const float* y, const float* p, double* g, doub
     }
 }
 
+void tco_transform_f32(const float* raw, float* out, uint32_t n, uint32_t K) {
+    /* probabilities from raw scores in the reference's float order.  K == 1:
+     * LogLoss::transform_prediction, the sigmoid 1 / (1 + exp(-x)) (loss.cpp:
+     * 213-215, loss.hpp:99-101); K > 1: CrossEntropyLoss::transform_to_proba
+     * (loss.cpp:320-331), the softmax of loss.cpp:273-285 per row: max, exp of
+     * the shifted scores, sequential sum, divide.  raw / out row-major n x K. */
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = raw + (size_t)i * K;
+        float* o = out + (size_t)i * K;
+        if (K == 1) {
+            o[0] = 1.0f / (1.0f + expf(-r[0]));
+            continue;
+        }
+        float m = r[0];
+        for (uint32_t c = 1; c < K; ++c) m = r[c] > m ? r[c] : m;
+        float e[64];
+        float s = 0.0f;
+        for (uint32_t c = 0; c < K; ++c) {
+            e[c] = expf(r[c] - m);
+            s += e[c];
+        }
+        for (uint32_t c = 0; c < K; ++c) o[c] = e[c] / s;
+    }
+}
+
 void tco_gradients_mc_f32(const float* y, const float* p, float* g, float* h, uint32_t n, uint32_t K) {
     /* compute_multiclass_gradients + softmax in the reference's float order,
      * loss.cpp:273-318: max shift, exp, sequential sum, divide; p and g/h

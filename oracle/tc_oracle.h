@@ -46,6 +46,9 @@ void tco_gradients_f64(int loss, const float* y, const float* p, double* g, doub
 /* A8: softmax gradients, K channels, row-major n x K (src/boosting/loss.cpp:273-318). */
 void tco_gradients_mc_f32(const float* y, const float* p, float* g, float* h, uint32_t n, uint32_t K);
 void tco_gradients_mc_f64(const float* y, const float* p, double* g, double* h, uint32_t n, uint32_t K);
+/* Probabilities from raw scores (row-major n x K): the sigmoid for K == 1
+ * (loss.cpp:213-215), the softmax for K > 1 (loss.cpp:273-285, :320-331). */
+void tco_transform_f32(const float* raw, float* out, uint32_t n, uint32_t K);
 void tco_predict_mc_f64(const uint16_t* feature, const float* fthr, const uint16_t* depth,
                         const float* leaves, const float* weight, uint32_t T, uint32_t D,
                         uint32_t K, double base, const float* X, uint32_t n, uint32_t F,

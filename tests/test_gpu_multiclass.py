@@ -9,9 +9,10 @@ bit-exact with the reference's CrossEntropyLoss (test_oracle.py).
 
 Bit-exact claims: per-level (feature, bin) and per-row leaf indices (outside
 FP64 near-ties), the K = 1 tree equal to tc_tree_build, per-row leaf indices
-of inference.  Float claims: gradients within 1e-6 absolute (probabilities),
-leaf values and raw scores within 1e-5 relative to the FP64 oracle,
-normalized by the sum of |terms|."""
+of inference; softmax gradients, and probabilities from given raw scores
+(glibc's expf restated on the device).  Float claims: leaf values and raw
+scores within 1e-5 relative to the FP64 oracle, normalized by the sum of
+|terms|."""
 import numpy as np
 import pytest
 
@@ -59,7 +60,8 @@ def test_softmax_gradients(ctx, oracle, K):
     y = rng.integers(0, K, n).astype(np.float32)
     g, h = ctx.gradients_multi(y, p)
     rg, rh = oracle.gradients_mc_f32(y, p)
-    assert np.abs(g - rg).max() <= 1e-6 and np.abs(h - rh).max() <= 1e-6
+    # the reference's float order with glibc's expf (restated on the device)
+    assert np.array_equal(g, rg) and np.array_equal(h, rh)
     assert (h >= 1e-6).all()
     assert np.allclose(g.sum(axis=1), 0.0, atol=1e-5)   # sum_c p_c - 1 = 0
 
@@ -132,6 +134,8 @@ def test_predict_multi(ctx, oracle, T, D, K, F, n):
     p = ens.predict(X, transform=1)
     e = np.exp(ref - ref.max(axis=1, keepdims=True))
     assert np.allclose(p, e / e.sum(axis=1, keepdims=True), rtol=1e-5, atol=1e-6)
+    # the softmax of the device's own raw scores: the reference's float order
+    assert np.array_equal(p, oracle.transform_f32(out, K))
     # alone and in a bigger batch: identical bits
     assert np.array_equal(ens.predict(X[5:6]), out[5:6])
 

@@ -44,6 +44,7 @@ def test_predict_bitexact(ctx, oracle, T, D, F, n):
     assert rel_err(out, r64, ab).max() < 1e-5
     p = ens.predict(X, transform=1)
     assert np.allclose(p, 1.0 / (1.0 + np.exp(-r64)), rtol=1e-6, atol=1e-7)
+    assert np.array_equal(p, oracle.transform_f32(ref))  # the reference's sigmoid, bit for bit
 
 
 @pytest.mark.parametrize("n", [1, 7, 8, 13, 16, 23, 517, 3001])

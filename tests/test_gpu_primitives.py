@@ -83,9 +83,9 @@ def test_gradients(ctx, oracle, loss):
     p[:5] = [0.0, 15.5, -16.0, 1e-3, -88.0]
     g, h = ctx.gradients(loss, y, p)
     og, oh = oracle.gradients_f32(loss, y, p)
-    vec = (n // 8) * 8
-    # the reference's vectorized rows: same float operations -> identical bits
-    assert np.array_equal(g[:vec], og[:vec]) and np.array_equal(h[:vec], oh[:vec])
+    # the reference's vectorized rows (same float operations) and its scalar
+    # tail rows (glibc's expf, restated on the device): identical bits
+    assert np.array_equal(g, og) and np.array_equal(h, oh)
     g64, h64 = oracle.gradients_f64(loss, y, p)
     # tolerance scale = sum of |terms|: g = p - y (2(p - y) for MSE), h = p(1 - p)
     if loss == 0:
@@ -95,6 +95,22 @@ def test_gradients(ctx, oracle, loss):
         prob = g64 + y
         assert rel_err(g, g64, np.abs(prob) + np.abs(y)).max() < TOL
         assert rel_err(h, h64, 1.0).max() < TOL
+
+
+def test_gradients_scalar_tail(ctx, oracle):
+    """The rows past the last full 8-row block take the reference's scalar
+    sigmoid, 1 / (1 + exp(-x)) with glibc's expf: calls of 7 rows (all of them
+    tail rows) over the clamped range [-15, 15] and its ends, bit for bit
+    against the oracle (glibc on the host)."""
+    rng = np.random.default_rng(12)
+    p = np.concatenate([rng.uniform(-16, 16, 7 * 300), np.linspace(-15, 15, 7 * 60),
+                        [-15.0, 15.0, -14.999999, 14.999999, 0.0, -0.0, 1e-30]]).astype(np.float32)
+    p = p[: len(p) // 7 * 7]
+    y = (rng.random(len(p)) < 0.5).astype(np.float32)
+    for i in range(0, len(p), 7):
+        g, h = ctx.gradients(1, y[i:i + 7], p[i:i + 7])
+        og, oh = oracle.gradients_f32(1, y[i:i + 7], p[i:i + 7])
+        assert np.array_equal(g, og) and np.array_equal(h, oh), p[i:i + 7]
 
 
 @pytest.mark.parametrize("n,F,seed", [(1000, 5, 1), (50000, 20, 2), (70001, 37, 3)])

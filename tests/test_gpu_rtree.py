@@ -242,9 +242,9 @@ def test_rtrain_multiclass_matches_oracle(ctx, oracle):
     """Booster::train_multiclass (booster.cpp:1263-1400): per iteration the
     softmax gradients, then one regular tree per class on that class's
     gradients.  Each tree is compared with the oracle's on the oracle's
-    gradients of the replayed predictions (the device's softmax exp is within
-    1e-6 of the reference's); the replayed predictions stay within 1e-5 of
-    the GPU's."""
+    gradients of the replayed predictions (the device's softmax is the
+    reference's float order with glibc's expf); the replayed predictions
+    equal the GPU's bit for bit."""
     from turbocat import capi
     K = 3
     X, y = multiclass_data(6000, 10, K, 21)
@@ -270,7 +270,7 @@ def test_rtrain_multiclass_matches_oracle(ctx, oracle):
                                           oracle.rtree_apply_binned(tree, bins))
         preds = nxt
     assert ties <= 1
-    assert np.abs(m["train_preds"].T - preds).max() <= 1e-5
+    assert np.array_equal(m["train_preds"].T, preds)
 
 
 @pytest.mark.parametrize("kind", ["binary", "multiclass"])
@@ -305,6 +305,7 @@ def test_rpredict_matches_oracle(ctx, oracle, kind):
     out = ens.predict(Xp)
     assert np.array_equal(out.reshape(-1, K), ref)
     prob = ens.predict(Xp, transform=1).reshape(-1, K)
+    assert np.array_equal(prob, oracle.transform_f32(ref, K).reshape(-1, K))
     if K == 1:
         assert np.allclose(prob[:, 0], 1 / (1 + np.exp(-ref[:, 0].astype(np.float64))), atol=1e-6)
     else:

@@ -352,6 +352,23 @@ def test_ce_gradients_match_reference(oracle, K):
     assert np.array_equal(g, rg) and np.array_equal(h, rh)
 
 
+@pytest.mark.skipif(not HAVE_REF, reason="oracle/_ref/libtcref.so not built (no reference tree)")
+@pytest.mark.parametrize("K", [1, 2, 3, 10])
+def test_transform_matches_reference(oracle, K):
+    """Probabilities from raw scores: the oracle's sigmoid / softmax against
+    LogLoss::transform_prediction and CrossEntropyLoss::transform_to_proba of
+    the compiled reference, bit for bit (scores past +-88, equal scores and
+    saturated rows included)."""
+    rng = np.random.default_rng(40 + K)
+    raw = (rng.standard_normal((4001, K)) * 6).astype(np.float32)
+    raw[::37] = 0.0
+    raw[1::53, 0] = 95.0
+    raw[2::59, -1] = -110.0
+    ref = RefLib.isolated(lambda: RefLib().transform(raw, K))
+    out = oracle.transform_f32(raw, K)
+    assert np.array_equal(out, ref)
+
+
 # --------------------------------------------------------------- A19: losses, early stopping
 @pytest.mark.skipif(not HAVE_REF, reason="oracle/_ref/libtcref.so not built (no reference tree)")
 @pytest.mark.parametrize("loss", [0, 1])

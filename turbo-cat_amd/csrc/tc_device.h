@@ -1,6 +1,7 @@
 // Device helpers shared by the HIP translation units.
 #pragma once
 #include "tc_common.h"
+#include "tc_expf.h"
 
 namespace tc {
 
@@ -44,7 +45,8 @@ __device__ inline float one_plus_exp_ref(float x) {
 
 // A6/A7 gradient of one row (src/boosting/loss.cpp:123-163, :348-362).
 // `vec_row` selects the reference's AVX2 formula (rows 0 .. 8*floor(N/8)-1);
-// the scalar tail uses the library exp.
+// the scalar tail uses the library exp the reference calls (glibc's expf,
+// ref_expf).
 __device__ inline float2 row_gradient(int loss, float pred, float label, bool vec_row) {
   if (loss == kMSE) {
     return make_float2(__fmul_rn(2.0f, __fsub_rn(pred, label)), 2.0f);
@@ -55,7 +57,7 @@ __device__ inline float2 row_gradient(int loss, float pred, float label, bool ve
   if (vec_row) {
     s = __fdiv_rn(1.0f, one_plus_exp_ref(__fsub_rn(0.0f, x)));
   } else {
-    s = __fdiv_rn(1.0f, __fadd_rn(1.0f, expf(-x)));
+    s = __fdiv_rn(1.0f, __fadd_rn(1.0f, ref_expf(-x)));
   }
   const float g = __fsub_rn(s, label);
   const float hh = __fmul_rn(s, __fsub_rn(1.0f, s));

@@ -239,7 +239,7 @@ __device__ void tile_sum_multi(const EnsDev& E, const void* xrow, uint32_t r, fl
 #pragma unroll
     for (uint32_t c = 0; c < kMaxClasses; ++c) {
       if (c >= K) break;
-      v[c] = expf(__fsub_rn(v[c], m));
+      v[c] = ref_expf(__fsub_rn(v[c], m));
       z = __fadd_rn(z, v[c]);
     }
 #pragma unroll
@@ -252,7 +252,7 @@ __device__ void tile_sum_multi(const EnsDev& E, const void* xrow, uint32_t r, fl
 
 __device__ __forceinline__ float finish_k1(const EnsDev& E, float sum, int transform) {
   float v = __fadd_rn(sum, E.base);  // booster.cpp:918-924
-  if (transform == 1) v = __fdiv_rn(1.0f, __fadd_rn(1.0f, expf(-v)));  // loss.cpp:213-215
+  if (transform == 1) v = __fdiv_rn(1.0f, __fadd_rn(1.0f, ref_expf(-v)));  // loss.cpp:213-215
   return v;
 }
 

@@ -918,7 +918,7 @@ __global__ void __launch_bounds__(256) k_rupdate(const RJob* __restrict__ jobs, 
 #pragma unroll
     for (uint32_t k = 0; k < kMaxClasses; ++k) {
       if (k >= njobs) break;
-      e[k] = expf(__fsub_rn(e[k], m));
+      e[k] = ref_expf(__fsub_rn(e[k], m));
       sum = __fadd_rn(sum, e[k]);
     }
     const uint32_t lab = static_cast<uint32_t>(y[r]);
@@ -1002,7 +1002,7 @@ __global__ void __launch_bounds__(256) k_rvalid(const RJob* __restrict__ jobs, u
 #pragma unroll
     for (uint32_t k = 0; k < kMaxClasses; ++k) {
       if (k >= njobs) break;
-      e[k] = expf(__fsub_rn(e[k], m));
+      e[k] = ref_expf(__fsub_rn(e[k], m));
       sum = __fadd_rn(sum, e[k]);
     }
     const uint32_t lab = static_cast<uint32_t>(vy[row]);
@@ -1669,13 +1669,13 @@ __global__ void __launch_bounds__(64) k_rpredict(const RPNode* __restrict__ node
     m = fmaxf(m, o[c]);
   }
   if (transform && K == 1) {
-    o[0] = __fdiv_rn(1.0f, __fadd_rn(1.0f, expf(-o[0])));  // LogLoss::transform_prediction
+    o[0] = __fdiv_rn(1.0f, __fadd_rn(1.0f, ref_expf(-o[0])));  // LogLoss::transform_prediction
   } else if (transform) {  // CrossEntropyLoss::transform_to_proba: max shift, exp, sum, divide
     float sum = 0.f;
 #pragma unroll
     for (uint32_t c = 0; c < kMaxClasses; ++c) {
       if (c >= K) break;
-      o[c] = expf(__fsub_rn(o[c], m));
+      o[c] = ref_expf(__fsub_rn(o[c], m));
       sum = __fadd_rn(sum, o[c]);
     }
 #pragma unroll

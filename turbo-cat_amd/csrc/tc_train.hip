@@ -75,7 +75,7 @@ __global__ void k_gradients_multi(const float* y, const float* p, float* g, floa
 #pragma unroll
   for (uint32_t c = 0; c < kMaxClasses; ++c) {
     if (c >= K) break;
-    e[c] = expf(__fsub_rn(e[c], m));
+    e[c] = ref_expf(__fsub_rn(e[c], m));
     sum = __fadd_rn(sum, e[c]);
   }
   const uint32_t lab = static_cast<uint32_t>(y[i]);

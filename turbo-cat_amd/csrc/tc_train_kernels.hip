@@ -182,7 +182,7 @@ __device__ inline float row_loss(int loss, float pred, float label) {
     const float d = __fsub_rn(pred, label);
     return __fmul_rn(d, d);
   }
-  float q = __fdiv_rn(1.0f, __fadd_rn(1.0f, expf(-pred)));
+  float q = __fdiv_rn(1.0f, __fadd_rn(1.0f, ref_expf(-pred)));
   q = fmaxf(1e-7f, fminf(1.0f - 1e-7f, q));
   return __fsub_rn(__fmul_rn(-label, logf(q)), __fmul_rn(__fsub_rn(1.0f, label), logf(__fsub_rn(1.0f, q))));
 }
@@ -259,7 +259,7 @@ __device__ void grad_update_softmax(const JobDev& J, uint32_t lw, int mode, int 
 #pragma unroll
       for (uint32_t c = 0; c < kMaxClasses; ++c) {
         if (c >= K) break;
-        e[c] = expf(__fsub_rn(e[c], m));
+        e[c] = ref_expf(__fsub_rn(e[c], m));
         sum = __fadd_rn(sum, e[c]);
       }
       const uint32_t lab = static_cast<uint32_t>(J.y[row]);
