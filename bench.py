@@ -32,6 +32,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "hist GB/s, trees/s train, rows/s predict + p50/p99 serve latency vs plain"
 HBM_PEAK_GBPS = 8000.0
+SAMPLING_TEXT = {"none": "", "random": ", row sampling: random 0.8 per tree",
+                 "goss": ", row sampling: GOSS top 0.2 / other 0.1 per tree",
+                 "mvs": ", row sampling: MVS 0.5 per tree"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_hist_train_1Mx100_d6_traffic.json")
 KERNEL_SOURCES = ["tc_common.h", "tc_train_kernels.hip", "tc_train.hip"]
 
@@ -110,9 +113,10 @@ def host_cpu_model():
     return None
 
 
-def cpu_baseline_train(X, y, loss, depth, lr, lam, budget_s):
+def cpu_baseline_train(X, y, loss, depth, lr, lam, budget_s, sampling="none"):
     """Time the reference's own boosting loop (compiled from its sources) on
-    the same matrix for a bounded number of trees."""
+    the same matrix for a bounded number of trees (with the fit's row
+    sampling: random 0.8, GOSS 0.2 / 0.1 or MVS 0.5)."""
     from oracle import RefLib
     if not RefLib.available():
         return None
@@ -122,12 +126,19 @@ def cpu_baseline_train(X, y, loss, depth, lr, lam, budget_s):
         t0 = time.perf_counter()
         ds = R.dataset(X, y)
         q = time.perf_counter() - t0
+        def fit(T):  # seconds of the boosting loop
+            if sampling in ("none", "random"):
+                return ds.fit(loss, T, depth, lr, lam, 1.0, 0.8 if sampling == "random" else 1.0)[2][1]
+            t = time.perf_counter()
+            if sampling == "goss":
+                ds.fit_goss(loss, T, depth, lr, lam, 1.0, 0.2, 0.1)
+            else:
+                ds.fit_mvs(loss, T, depth, lr, lam, 1.0, 0.5)
+            return time.perf_counter() - t
         # one tree to estimate the rate, then as many as fit the budget
-        _, _, s1 = ds.fit(loss, 1, depth, lr, lam, 1.0)
-        per = max(s1[1], 1e-3)
+        per = max(fit(1), 1e-3)
         T = int(max(2, min(50, (budget_s - q) / per)))
-        _, _, sec = ds.fit(loss, T, depth, lr, lam, 1.0)
-        return dict(quantize_s=q, trees=T, loop_s=sec[1], build_s=sec[2])
+        return dict(quantize_s=q, trees=T, loop_s=fit(T))
 
     r = RefLib.isolated(body)
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -135,7 +146,8 @@ def cpu_baseline_train(X, y, loss, depth, lr, lam, budget_s):
                 host_cpu=host_cpu_model(),
                 sample=(f"reference hot path (oracle/_ref, -O3 -mavx2 -mfma, OpenMP) on the full "
                         f"matrix: compute_bins {r['quantize_s']:.2f} s, then {r['trees']} trees "
-                        f"in {r['loop_s']:.2f} s"),
+                        f"in {r['loop_s']:.2f} s" + SAMPLING_TEXT[sampling]
+                        + (" (the MVS draw on one thread)" if sampling == "mvs" else "")),
                 quantize_s=r["quantize_s"])
 
 
@@ -164,9 +176,12 @@ def run_train(args):
     quant_s = time.perf_counter() - t0
     log(f"[bench] quantize_fit {quant_s:.2f} s; bins per feature {int(ne.min()) + 1}..{int(ne.max()) + 1}")
     R = 5  # trees timed kernel by kernel for the roofline
+    smp = dict(random=dict(subsample=0.8),
+               goss=dict(use_goss=True, goss_top_rate=0.2, goss_other_rate=0.1),
+               mvs=dict(use_mvs=True, mvs_subsample=0.5)).get(args.sampling, {})
     prm = capi.train_params(loss_id, n_trees=args.warmup + T_timed + R,
                             learning_rate=0.1, max_depth=6, lambda_l2=lam,
-                            hist_chunk=args.hist_chunk, hist_deep_items=args.deep_items)
+                            hist_chunk=args.hist_chunk, hist_deep_items=args.deep_items, **smp)
     tr = capi.Trainer(ctx, [ds], [y], prm)
     tr.step(args.warmup)
     tr.sync()
@@ -213,7 +228,7 @@ def run_train(args):
     cpu = None
     if not args.no_cpu:
         log("[bench] timing the reference CPU path (bounded sample)")
-        cpu = cpu_baseline_train(X, y, loss_id, 6, 0.1, lam, args.cpu_budget)
+        cpu = cpu_baseline_train(X, y, loss_id, 6, 0.1, lam, args.cpu_budget, args.sampling)
         if cpu:
             cpu["predict_rows_per_s"] = cpu_predict_rate(model, X)
             log(f"[bench] CPU reference {cpu['value']:.3f} trees/s on {cpu['cores']} threads")
@@ -222,17 +237,18 @@ def run_train(args):
         "steps": T_timed, "warmup": args.warmup, "ms_per_step": round(1e3 * dt / T_timed, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": (f"regression MSE {n} rows x {F} float32 N(0,1) features, "
+        "config": {"workload": ((f"regression MSE {n} rows x {F} float32 N(0,1) features, "
                                 f"y = x0 + 2 sin x1 + 0.5 x2 x3 + N(0, 0.1), 255-bin uint8 "
                                 f"quantization, symmetric trees depth 6, lr 0.1, lambda 0") if mse
                    else (f"binary logloss {n} rows x {F} float32 features "
                          f"({F - F // 5} N(0,1), {F // 5} U[0,1) with 5% duplicates), "
                          f"255-bin uint8 quantization, symmetric trees depth 6, lr 0.1, "
-                         f"lambda 1"), "rows": n, "features": F, "depth": 6,
-                   "trees_timed": T_timed},
+                         f"lambda 1")) + SAMPLING_TEXT[args.sampling], "rows": n, "features": F,
+                   "depth": 6, "trees_timed": T_timed},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": None if mse else measured_traffic(n, F, 6, (root, deep_items)),
+                     "traffic": None if mse or args.sampling != "none"
+                     else measured_traffic(n, F, 6, (root, deep_items)),
                      "kernel": "k_hist", "avg_launch_us": round(hist_ms / launches * 1e3, 2),
                      "algorithmic_bytes_per_launch": int(alg_bytes / launches)},
         "cpu_baseline": cpu,
@@ -701,6 +717,9 @@ def main():
                     help="with --hist-chunk: work items per deeper level (0: --hist-chunk rows)")
     ap.add_argument("--loss", default="logloss", choices=["logloss", "mse"],
                     help="train workload: binary logloss (default) or the MSE regression set")
+    ap.add_argument("--sampling", default="none", choices=["none", "random", "goss", "mvs"],
+                    help="train workload: row sampling per tree (random 0.8, GOSS top 0.2 / other "
+                         "0.1, MVS 0.5; drawn on the host with the reference's routines)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--qps", type=float, default=200.0)

@@ -196,6 +196,29 @@ def test_fit_batched_matches_each_job(ctx, oracle):
         assert np.array_equal(m["train_preds"], preds)
 
 
+def test_sampled_fit_graph_equals_stepwise(ctx):
+    """A plain row sample is shuffled ahead on host threads (a ring of future
+    trees) while the trainer captures and replays its per-tree graph: the
+    graph fit, the fit without a graph and a fit stepped in two calls give the
+    same trees, bit for bit (the draws never disturb a capture)."""
+    from turbocat import capi
+    X, y = binary_data(200000, 12, 8)
+    ds, _, _ = ctx.quantize_fit(X)
+    kw = dict(n_trees=40, learning_rate=0.1, max_depth=6, lambda_l2=1.0, subsample=0.8,
+              colsample_bytree=0.8, seed=5)
+    a = ds.train(y, capi.train_params(capi.TC_LOSS_LOGLOSS, use_graph=1, **kw))
+    b = ds.train(y, capi.train_params(capi.TC_LOSS_LOGLOSS, use_graph=0, **kw))
+    tr = capi.Trainer(ctx, [ds], [y], capi.train_params(capi.TC_LOSS_LOGLOSS, use_graph=1, **kw))
+    tr.step(3)
+    tr.step(37)
+    tr.sync()
+    c = tr.model()
+    tr.close()
+    for k in ("feature", "bin", "fthr", "gain", "depth", "leaves"):
+        assert np.array_equal(a[k], b[k]), k
+        assert np.array_equal(a[k], c[k]), k
+
+
 @pytest.mark.parametrize("loss,subsample,colsample", [(0, 0.8, 0.8), (1, 0.5, 1.0)])
 def test_fit_sampled_teacher_forced(ctx, oracle, loss, subsample, colsample):
     """Row sampling on the device: each tree is grown on the reference's row

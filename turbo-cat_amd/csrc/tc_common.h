@@ -35,6 +35,7 @@ constexpr int kHistThreads = 1024;     // 16 waves; two workgroups per CU (64 KB
 constexpr int kPartChunk = 4096;       // rows per partition work item
 constexpr int kPartThreads = 1024;
 constexpr uint32_t kMaxHistChunk = 32768;  // rows per histogram work item (packed count < 2^16)
+constexpr uint32_t kMaskChunkWords = 256;  // 32-row words of a row-sample mask per workgroup
 constexpr int kScanNodesPerPass = 32;  // node prefixes staged in LDS at a time
 constexpr uint32_t kEmptyKey = 0u;
 
@@ -94,6 +95,10 @@ struct JobDev {
   uint32_t N, F, Fp, T;     // Fp = rm_stride(F), T = feature_tiles(F)
   uint32_t n_root;          // rows of the tree's root list (N, or the row sample)
   uint32_t root_identity;   // 1: the root list is 0..N-1 (no sampling), rows[0] not read
+  const uint32_t* smask;    // a plain row sample: [chunks] rows drawn before each chunk of
+                            // kMaskChunkWords words, then [ceil(N/32)] words (bit r of word
+                            // r/32: row r drawn); k_rows_from_mask makes rows[0] from it.
+                            // null: rows[0] is copied in (GOSS, MVS) or not used
   uint32_t fused_root;      // 1: the gradient update writes the root partials (K = 1, no
                             // sampling, stored histograms); k_root_partials skips the job
   // hyper-parameters

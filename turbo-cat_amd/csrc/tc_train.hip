@@ -8,6 +8,7 @@
 #include <cstddef>
 #include <cstdlib>
 #include <cstring>
+#include <future>
 #include <memory>
 #include <numeric>
 #include <random>
@@ -109,13 +110,12 @@ uint64_t next_random(uint64_t& state) {
 
 // Dataset::random_subsample (dataset.cpp:552-562): the same engine and the
 // same std::shuffle, so the same rows in the same order as the reference.
+// The shuffle runs in place in out (n slots); the sample is its first k.
 uint32_t random_subsample(uint32_t n, float ratio, uint64_t seed, uint32_t* out) {
   const uint32_t k = static_cast<uint32_t>(static_cast<float>(n) * ratio);
-  std::vector<uint32_t> idx(n);
-  std::iota(idx.begin(), idx.end(), 0u);
+  std::iota(out, out + n, 0u);
   std::mt19937_64 rng(seed);
-  std::shuffle(idx.begin(), idx.end(), rng);
-  std::memcpy(out, idx.data(), sizeof(uint32_t) * std::min(k, n));
+  std::shuffle(out, out + n, rng);
   return std::min(k, n);
 }
 
@@ -258,12 +258,27 @@ class Trainer {
   }
   ~Trainer() { release(); }
 
-  // Row sample of the next tree for every sampling job: draw, shuffle on the
-  // host (pinned, double-buffered), copy into rows[0] on the stream.
+  // Row sample of the next tree for every sampling job, drawn on the host.  A
+  // plain row sample is shuffled ahead of time (a ring of up to 12 trees) and
+  // copied as a bitmask that the tree's first launch turns into rows[0]; GOSS
+  // and MVS select on the last tree's gradients into two alternating pinned
+  // buffers, copied into rows[0] on the stream.
   void prepare_samples() {
     for (uint32_t j = 0; j < njobs_; ++j) {
       Sampler& S = samplers_[j];
       if (!S.active && !S.colsample) continue;
+      const uint32_t N = h_jobs_[j].N;
+      if (S.ahead) {  // the shuffle needs no gradients: drawn ahead on host threads
+        while (S.submitted < S.prepared + S.host.size()) submit_draw(j, N);
+        const size_t b = S.prepared % S.host.size();
+        S.draws[b].get();  // rethrows a failed draw
+        ++S.prepared;
+        // the sample's bitmask; the tree's first launch makes the root list from it
+        TC_HIP(hipMemcpyAsync(const_cast<uint32_t*>(h_jobs_[j].smask), S.host[b],
+                              4ull * mask_words(N), hipMemcpyHostToDevice, ctx_->stream));
+        TC_HIP(hipEventRecord(S.ev[b], ctx_->stream));
+        continue;
+      }
       uint64_t seed = 0;
       if (S.active) seed = next_random(S.state);
       if (S.colsample) (void)next_random(S.state);  // feature subset: drawn, unused
@@ -271,7 +286,6 @@ class Trainer {
       const int b = S.next;
       S.next ^= 1;
       TC_HIP(hipEventSynchronize(S.ev[b]));
-      const uint32_t N = h_jobs_[j].N;
       if (S.goss || S.mvs) {  // the gradients the next tree is grown on: every queued tree first
         TC_HIP(hipMemcpyAsync(S.gh_host, h_jobs_[j].gh, sizeof(float2) * N, hipMemcpyDeviceToHost,
                               ctx_->stream));
@@ -292,6 +306,43 @@ class Trainer {
                             hipMemcpyHostToDevice, ctx_->stream));
       TC_HIP(hipEventRecord(S.ev[b], ctx_->stream));
     }
+  }
+
+  // Starts the shuffle of the next tree not yet drawn (seeds in tree order, as
+  // prepare_samples draws them) into its ring slot on a host thread, once the
+  // copy of the tree that used the slot before has left it (that copy was
+  // enqueued a ring's length of trees ago).  The thread makes no HIP call: one
+  // made while this thread captures a tree's graph would void the capture.
+  void submit_draw(uint32_t j, uint32_t N) {
+    Sampler& S = samplers_[j];
+    const uint64_t seed = next_random(S.state);
+    if (S.colsample) (void)next_random(S.state);  // feature subset: drawn, unused
+    const size_t b = S.submitted % S.host.size();
+    TC_HIP(hipEventSynchronize(S.ev[b]));
+    uint32_t* out = S.host[b];
+    uint32_t* sc = S.scratch[b].data();
+    const float ratio = S.ratio;
+    S.draws[b] = std::async(std::launch::async, [=] {
+      const uint32_t k = random_subsample(N, ratio, seed, sc);
+      const uint32_t W = (N + 31u) / 32u, nch = (W + kMaskChunkWords - 1u) / kMaskChunkWords;
+      uint32_t* mask = out + nch;
+      std::memset(mask, 0, 4ull * W);
+      for (uint32_t i = 0; i < k; ++i) mask[sc[i] >> 5] |= 1u << (sc[i] & 31u);
+      uint32_t run = 0;  // rows drawn before each chunk
+      for (uint32_t c = 0; c < nch; ++c) {
+        out[c] = run;
+        const uint32_t w1 = std::min(W, (c + 1u) * kMaskChunkWords);
+        for (uint32_t w = c * kMaskChunkWords; w < w1; ++w)
+          run += static_cast<uint32_t>(__builtin_popcount(mask[w]));
+      }
+    });
+    ++S.submitted;
+  }
+
+  // Words of a row-sample mask of N rows with its per-chunk counts.
+  static size_t mask_words(uint32_t N) {
+    const size_t W = (N + 31u) / 32u;
+    return W + (W + kMaskChunkWords - 1u) / kMaskChunkWords;
   }
 
   bool sampling() const {
@@ -333,6 +384,9 @@ class Trainer {
   void enqueue_tree(std::vector<std::pair<hipEvent_t, hipEvent_t>>* hist_events) {
     hipStream_t s = ctx_->stream;
     const bool packed = packed_hist();
+    if (mask_chunks_)  // plain row samples: the root lists from the copied bitmasks
+      hipLaunchKernelGGL(k_rows_from_mask, dim3(mask_chunks_, njobs_), dim3(kMaskChunkWords), 0, s,
+                         d_jobs_);
     if (root_partials_)  // otherwise the gradient update wrote every job's root partials
       hipLaunchKernelGGL(k_root_partials, dim3(total_grad_wg_), dim3(256), 0, s, d_jobs_,
                          d_job_wg_off_, njobs_);
@@ -793,10 +847,32 @@ class Trainer {
       }
       J.root_identity = SM.active ? 0u : 1u;
       if (SM.active) {
-        for (int b = 0; b < 2; ++b) {
-          TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&SM.host[b]), 4ull * std::max(1u, N),
+        // a plain row sample is drawn ahead, up to 12 trees on as many host
+        // threads (its shuffle and bitmask take about 3 ms for 1M rows; 256 MB
+        // of shuffle scratch over all jobs at most; 1M x 100, 0.8: 4 slots
+        // 1,388, 8 2,239-2,513, 12 2,984 trees/s); GOSS and MVS need each
+        // tree's gradients, so two buffers alternate
+        SM.ahead = !SM.goss && !SM.mvs;
+        const uint64_t per = 4ull * std::max(1u, N) * njobs_;
+        const size_t R = SM.ahead ? static_cast<size_t>(std::clamp<uint64_t>(
+                                        (uint64_t(256) << 20) / per, 2, 12))
+                                  : 2;
+        SM.host.assign(R, nullptr);
+        SM.ev.assign(R, nullptr);
+        SM.draws.resize(R);
+        // ahead: a slot is the bitmask; the shuffle runs in the slot's scratch
+        const size_t words = SM.ahead ? mask_words(N) : std::max(1u, N);
+        if (SM.ahead) SM.scratch.assign(R, std::vector<uint32_t>(std::max(1u, N)));
+        for (size_t b = 0; b < R; ++b) {
+          TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&SM.host[b]), 4ull * words,
                                hipHostMallocDefault));
           TC_HIP(hipEventCreateWithFlags(&SM.ev[b], hipEventDisableTiming));
+        }
+        if (SM.ahead) {
+          J.smask = alloc<uint32_t>(words);
+          mask_chunks_ = std::max<uint32_t>(
+              mask_chunks_, static_cast<uint32_t>(((N + 31u) / 32u + kMaskChunkWords - 1u) /
+                                                  kMaskChunkWords));
         }
       }
       const uint32_t K = channels(S.p);
@@ -944,12 +1020,16 @@ class Trainer {
     for (void* p : allocs_) dfree(p);
     allocs_.clear();
     for (auto& S : samplers_) {
-      for (int b = 0; b < 2; ++b) {
+      for (auto& d : S.draws)  // draws still running write into the buffers
+        if (d.valid()) d.wait();
+      S.draws.clear();
+      for (size_t b = 0; b < S.host.size(); ++b) {
         if (S.host[b]) hfree(S.host[b]);
         if (S.ev[b]) (void)hipEventDestroy(S.ev[b]);
-        S.host[b] = nullptr;
-        S.ev[b] = nullptr;
       }
+      S.host.clear();
+      S.ev.clear();
+      S.scratch.clear();
       if (S.gh_host) hfree(S.gh_host);
       S.gh_host = nullptr;
     }
@@ -965,8 +1045,14 @@ class Trainer {
     float2* gh_host = nullptr;         // pinned copy of the gradients (GOSS, MVS)
     uint32_t n_root_host[2] = {0, 0};  // MVS: each buffer's row count, the source of its copy
     std::vector<float> g;              // their g
-    uint32_t* host[2] = {nullptr, nullptr};
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    // pinned row samples and the events of their copies to the device: a
+    // ring of up to 8 drawn ahead (a plain row sample), else two alternating
+    std::vector<uint32_t*> host;
+    std::vector<hipEvent_t> ev;
+    std::vector<std::future<void>> draws;  // the ring's draws (ahead)
+    std::vector<std::vector<uint32_t>> scratch;  // the ring's shuffles (ahead)
+    bool ahead = false;
+    uint64_t submitted = 0, prepared = 0;  // trees drawn / copied (ahead)
     int next = 0;
   };
 
@@ -985,6 +1071,7 @@ class Trainer {
   std::vector<void*> allocs_;
   hipGraphExec_t graph_exec_ = nullptr;
   std::vector<Sampler> samplers_;
+  uint32_t mask_chunks_ = 0;  // k_rows_from_mask's grid: chunks of the largest sampled job
   bool forced_ = false;
   bool keep_rows_ = false;  // partition the last level too (leaf_index)
   bool eval_ = false;       // some job records losses / early-stops (k_eval per tree)

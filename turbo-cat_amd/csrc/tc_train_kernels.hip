@@ -594,6 +594,35 @@ __device__ void tree_begin(const JobDev& J, uint32_t job, const Lists& L, int pa
   }
 }
 
+// The root list of a plain row sample, ascending: rows[0][0 .. n_root) from
+// the sample's bitmask and the rows drawn before each of its chunks (both
+// made on the host).  A node's row order never changes a histogram (every one
+// is an exact integer sum), and ascending rows turn the root's gathers of bins
+// and gradients into sequential reads.  Workgroup = (chunk, job), thread =
+// one 32-row word.
+__global__ void __launch_bounds__(kMaskChunkWords) k_rows_from_mask(const JobDev* __restrict__ jobs) {
+  const JobDev& J = jobs[blockIdx.y];
+  const uint32_t* sm = J.smask;
+  if (!sm) return;
+  const uint32_t W = (J.N + 31u) / 32u, nch = (W + kMaskChunkWords - 1u) / kMaskChunkWords;
+  const uint32_t c = blockIdx.x;
+  if (c >= nch) return;  // uniform: the grid covers the largest job
+  const uint32_t tid = threadIdx.x, w = c * kMaskChunkWords + tid;
+  uint32_t bits = w < W ? sm[nch + w] : 0u;
+  const uint32_t cnt = __popc(bits);
+  __shared__ uint32_t s_w[kMaskChunkWords / 64u];
+  const uint32_t incl = wave_incl_scan_u32(cnt);
+  if ((tid & 63u) == 63u) s_w[tid >> 6] = incl;
+  __syncthreads();
+  uint32_t pos = sm[c] + incl - cnt;
+  for (uint32_t i = 0; i < (tid >> 6); ++i) pos += s_w[i];
+  while (bits) {
+    if (pos >= J.n_root) break;  // a mask inconsistent with n_root writes nothing past it
+    J.rows[pos++] = w * 32u + static_cast<uint32_t>(__builtin_ctz(bits));
+    bits &= bits - 1u;
+  }
+}
+
 // Start of a tree, over the row grid: per-channel root totals (sum g, sum h)
 // over 1024-position tiles of the root list rows[0][0 .. n_root), double,
 // fixed order within the tile (k_tree_begin adds the tiles in a fixed
