@@ -570,9 +570,13 @@ def run_regular(args):
     ds, edges, ne = ctx.quantize_fit(X)
     quant_s = time.perf_counter() - t0
 
+    if args.sampling in ("goss", "mvs"):
+        raise SystemExit("regular trees take a plain row sample only (--sampling random)")
+    smp = dict(subsample=0.8) if args.sampling == "random" else {}
+
     def fit(iters, time_hist):
         prm = capi.rtrain_params(loss, n_iterations=iters, learning_rate=lr,
-                                 n_classes=K if multi else 0, time_hist=time_hist)
+                                 n_classes=K if multi else 0, time_hist=time_hist, **smp)
         prm.tree = capi.rtree_params(max_depth=D, max_leaves=args.max_leaves, lambda_l2=1.0)
         return ds.rtrain(y, prm)
 
@@ -633,14 +637,14 @@ def run_regular(args):
         "steps": T_timed, "warmup": args.warmup, "ms_per_step": round(loop_ms / T_timed, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": (f"multiclass softmax {n} rows x {F} float32 N(0,1) features, "
+        "config": {"workload": ((f"multiclass softmax {n} rows x {F} float32 N(0,1) features, "
                                 f"{K} classes, the reference's multiclass loop: one regular "
                                 f"depthwise tree per class and iteration, depth {D}, max_leaves "
                                 f"{args.max_leaves}, lr {lr}, lambda 1") if multi
                    else (f"binary logloss {n} rows x {F} float32 features "
                          f"({F - F // 5} N(0,1), {F // 5} U[0,1) with 5% duplicates), regular "
                          f"depthwise trees (mode='small') depth {D}, max_leaves "
-                         f"{args.max_leaves}, lr {lr}, lambda 1"),
+                         f"{args.max_leaves}, lr {lr}, lambda 1")) + SAMPLING_TEXT[args.sampling],
                    "rows": n, "features": F, "classes": K, "depth": D},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
@@ -718,8 +722,9 @@ def main():
     ap.add_argument("--loss", default="logloss", choices=["logloss", "mse"],
                     help="train workload: binary logloss (default) or the MSE regression set")
     ap.add_argument("--sampling", default="none", choices=["none", "random", "goss", "mvs"],
-                    help="train workload: row sampling per tree (random 0.8, GOSS top 0.2 / other "
-                         "0.1, MVS 0.5; drawn on the host with the reference's routines)")
+                    help="train and regular workloads: row sampling per tree (random 0.8, GOSS "
+                         "top 0.2 / other 0.1, MVS 0.5; regular trees: random only; drawn on the "
+                         "host with the reference's routines)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--qps", type=float, default=200.0)

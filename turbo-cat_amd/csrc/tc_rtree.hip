@@ -1154,6 +1154,9 @@ class RTrainer {
   // hev (optional) receives an event pair around every histogram launch.
   void enqueue_iteration(bool update, std::vector<std::pair<hipEvent_t, hipEvent_t>>* hev = nullptr) {
     hipStream_t s = ctx_->stream;
+    if (sampling_)  // the root rows of the iteration's sample
+      hipLaunchKernelGGL(k_rows_from_mask_one, dim3(AheadSampler::mask_chunks(N_)),
+                         dim3(kMaskChunkWords), 0, s, mask_, N_, n_sample_, sample_);
     hipLaunchKernelGGL(k_rbegin, dim3(1, nj_), dim3(256), 0, s, d_jobs_, lists_);
     for (uint32_t d = 0; d < D_; ++d) {
       const uint32_t par = d & 1u;
@@ -1388,18 +1391,16 @@ class RTrainer {
     }
   }
 
+  // The next iteration's row sample (Dataset::random_subsample, tree.cpp:
+  // 186-191), drawn ahead on host threads and copied as a bitmask; the
+  // iteration's first launch makes the root rows from it.
   void prepare_sample() {
     if (!sampling_ && !colsample_) return;
-    uint64_t seed = 0;
-    if (sampling_) seed = next_random(rng_);
-    if (colsample_) (void)next_random(rng_);  // feature subset: drawn, unused (tree.cpp:193-196)
-    if (!sampling_) return;
-    const int b = next_;
-    next_ ^= 1;
-    TC_HIP(hipEventSynchronize(ev_[b]));
-    random_subsample(N_, s_.subsample, seed, host_[b]);
-    TC_HIP(hipMemcpyAsync(sample_, host_[b], 4ull * n_sample_, hipMemcpyHostToDevice, ctx_->stream));
-    TC_HIP(hipEventRecord(ev_[b], ctx_->stream));
+    if (sampling_) {
+      plain_.next(mask_, ctx_->stream);
+      return;
+    }
+    (void)next_random(rng_);  // feature subset: drawn, unused (tree.cpp:193-196)
   }
 
   void setup() {
@@ -1432,11 +1433,10 @@ class RTrainer {
     sample_ = alloc<uint32_t>(N_);
     rows0_dummy_ = alloc<uint32_t>(1);
     hist_rows_ = alloc_zero<unsigned long long>(1);
-    if (sampling_)
-      for (int b = 0; b < 2; ++b) {
-        TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_[b]), 4ull * N_, hipHostMallocDefault));
-        TC_HIP(hipEventCreateWithFlags(&ev_[b], hipEventDisableTiming));
-      }
+    if (sampling_) {
+      plain_.init(N_, s_.subsample, rng_, colsample_, ahead_ring(N_, 1));
+      mask_ = alloc<uint32_t>(AheadSampler::mask_words(N_));
+    }
     if (s_.valid) {  // validation rows ride after each iteration's update
       TC_REQUIRE(s_.valid->F == F_, "validation set: feature count differs from training");
       TC_REQUIRE(s_.y_valid != nullptr, "validation set without labels");
@@ -1555,12 +1555,7 @@ class RTrainer {
     stop_host_ = nullptr;
     if (stop_ev_) (void)hipEventDestroy(stop_ev_);
     stop_ev_ = nullptr;
-    for (int b = 0; b < 2; ++b) {
-      if (host_[b]) hfree(host_[b]);
-      if (ev_[b]) (void)hipEventDestroy(ev_[b]);
-      host_[b] = nullptr;
-      ev_[b] = nullptr;
-    }
+    plain_.release();  // waits for its draws
   }
 
   tc_ctx* ctx_;
@@ -1598,9 +1593,8 @@ class RTrainer {
   bool sampling_ = false, colsample_ = false;
   uint32_t n_sample_ = 0;
   uint64_t rng_ = 42;
-  uint32_t* host_[2] = {nullptr, nullptr};
-  hipEvent_t ev_[2] = {nullptr, nullptr};
-  int next_ = 0;
+  AheadSampler plain_;        // the row samples, drawn ahead
+  uint32_t* mask_ = nullptr;  // the iteration's sample as a bitmask (device)
 };
 
 }  // namespace

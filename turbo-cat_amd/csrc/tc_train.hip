@@ -119,6 +119,121 @@ uint32_t random_subsample(uint32_t n, float ratio, uint64_t seed, uint32_t* out)
   return std::min(k, n);
 }
 
+// Plain row samples (Dataset::random_subsample) of the coming trees, drawn
+// ahead: a ring of slots, each tree's shuffle on a host thread of its own
+// (the seeds are taken in tree order, xorshift64* as Booster::next_random,
+// with the feature-subset draw after each when colsample is on, so every tree
+// draws the reference's rows).  A slot holds the sample as a bitmask of the
+// drawn rows (bit r of word r / 32) preceded by the rows drawn before each
+// chunk of kMaskChunkWords words; the device makes the tree's root list from
+// it (k_rows_from_mask*).  Only the set of rows is the reference's: the list
+// comes out in ascending row order, which changes no histogram (exact integer
+// sums) and makes the root's reads sequential.  The draw threads make no HIP
+// call: one made while the trainer captures a graph would void the capture.
+class AheadSampler {
+ public:
+  AheadSampler() = default;
+  AheadSampler(const AheadSampler&) = delete;
+  AheadSampler& operator=(const AheadSampler&) = delete;
+  ~AheadSampler() { release(); }
+
+  static size_t mask_words(uint32_t N) {
+    const size_t W = (N + 31u) / 32u;
+    return W + (W + kMaskChunkWords - 1u) / kMaskChunkWords;
+  }
+  static uint32_t mask_chunks(uint32_t N) {
+    return ((N + 31u) / 32u + kMaskChunkWords - 1u) / kMaskChunkWords;
+  }
+
+  // ring: slots (trees drawn ahead, one host thread each)
+  void init(uint32_t N, float ratio, uint64_t seed, bool colsample, size_t ring) {
+    N_ = N;
+    ratio_ = ratio;
+    state_ = seed;
+    colsample_ = colsample;
+    host_.assign(ring, nullptr);
+    ev_.assign(ring, nullptr);
+    draws_.resize(ring);
+    scratch_.assign(ring, std::vector<uint32_t>(std::max(1u, N)));
+    for (size_t b = 0; b < ring; ++b) {
+      TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_[b]), 4ull * mask_words(N),
+                           hipHostMallocDefault));
+      TC_HIP(hipEventCreateWithFlags(&ev_[b], hipEventDisableTiming));
+    }
+  }
+
+  // The next tree's bitmask, copied to d_mask on stream s.
+  void next(uint32_t* d_mask, hipStream_t s) {
+    while (submitted_ < prepared_ + host_.size()) submit();
+    const size_t b = prepared_ % host_.size();
+    draws_[b].get();  // rethrows a failed draw
+    ++prepared_;
+    TC_HIP(hipMemcpyAsync(d_mask, host_[b], 4ull * mask_words(N_), hipMemcpyHostToDevice, s));
+    TC_HIP(hipEventRecord(ev_[b], s));
+  }
+
+  void release() {
+    for (auto& d : draws_)  // draws still running write into the buffers
+      if (d.valid()) d.wait();
+    draws_.clear();
+    for (size_t b = 0; b < host_.size(); ++b) {
+      if (host_[b]) hfree(host_[b]);
+      if (ev_[b]) (void)hipEventDestroy(ev_[b]);
+    }
+    host_.clear();
+    ev_.clear();
+    scratch_.clear();
+  }
+
+ private:
+  // Starts the next tree's shuffle into its slot, once the copy of the tree
+  // that used the slot before (a ring's length of trees ago) has left it.
+  void submit() {
+    const uint64_t seed = next_random(state_);
+    if (colsample_) (void)next_random(state_);  // feature subset: drawn, unused
+    const size_t b = submitted_ % host_.size();
+    TC_HIP(hipEventSynchronize(ev_[b]));
+    uint32_t* out = host_[b];
+    uint32_t* sc = scratch_[b].data();
+    const uint32_t N = N_;
+    const float ratio = ratio_;
+    draws_[b] = std::async(std::launch::async, [=] {
+      const uint32_t k = random_subsample(N, ratio, seed, sc);
+      const uint32_t W = (N + 31u) / 32u, nch = mask_chunks(N);
+      uint32_t* mask = out + nch;
+      std::memset(mask, 0, 4ull * W);
+      for (uint32_t i = 0; i < k; ++i) mask[sc[i] >> 5] |= 1u << (sc[i] & 31u);
+      uint32_t run = 0;  // rows drawn before each chunk
+      for (uint32_t c = 0; c < nch; ++c) {
+        out[c] = run;
+        const uint32_t w1 = std::min(W, (c + 1u) * kMaskChunkWords);
+        for (uint32_t w = c * kMaskChunkWords; w < w1; ++w)
+          run += static_cast<uint32_t>(__builtin_popcount(mask[w]));
+      }
+    });
+    ++submitted_;
+  }
+
+  uint32_t N_ = 0;
+  float ratio_ = 1.f;
+  uint64_t state_ = 0;
+  bool colsample_ = false;
+  std::vector<uint32_t*> host_;
+  std::vector<hipEvent_t> ev_;
+  std::vector<std::future<void>> draws_;
+  std::vector<std::vector<uint32_t>> scratch_;
+  uint64_t submitted_ = 0, prepared_ = 0;  // trees drawn / copied
+};
+
+// Slots of an AheadSampler: up to 12 trees ahead (each shuffle and bitmask
+// about 3 ms for 1M rows; 1M x 100, subsample 0.8: 4 slots 1,388, 8
+// 2,239-2,513, 12 2,984 trees/s), at most 256 MB of shuffle scratch over
+// `jobs` samplers of N rows.
+inline size_t ahead_ring(uint32_t N, uint32_t jobs) {
+  const uint64_t per = 4ull * std::max(1u, N) * std::max(1u, jobs);
+  return static_cast<size_t>(std::clamp<uint64_t>((uint64_t(256) << 20) / per, 2, 12));
+}
+
 // Rows Dataset::goss_subsample (dataset.cpp:494-550) keeps for one tree: the
 // uint32(n * top_rate) rows of largest |g| -- std::nth_element over (|g|, row)
 // pairs ordered by |g| alone, so equal |g| fall where the library puts them --
@@ -268,15 +383,9 @@ class Trainer {
       Sampler& S = samplers_[j];
       if (!S.active && !S.colsample) continue;
       const uint32_t N = h_jobs_[j].N;
-      if (S.ahead) {  // the shuffle needs no gradients: drawn ahead on host threads
-        while (S.submitted < S.prepared + S.host.size()) submit_draw(j, N);
-        const size_t b = S.prepared % S.host.size();
-        S.draws[b].get();  // rethrows a failed draw
-        ++S.prepared;
+      if (S.plain) {  // the shuffle needs no gradients: drawn ahead on host threads
         // the sample's bitmask; the tree's first launch makes the root list from it
-        TC_HIP(hipMemcpyAsync(const_cast<uint32_t*>(h_jobs_[j].smask), S.host[b],
-                              4ull * mask_words(N), hipMemcpyHostToDevice, ctx_->stream));
-        TC_HIP(hipEventRecord(S.ev[b], ctx_->stream));
+        S.plain->next(const_cast<uint32_t*>(h_jobs_[j].smask), ctx_->stream);
         continue;
       }
       uint64_t seed = 0;
@@ -286,63 +395,23 @@ class Trainer {
       const int b = S.next;
       S.next ^= 1;
       TC_HIP(hipEventSynchronize(S.ev[b]));
-      if (S.goss || S.mvs) {  // the gradients the next tree is grown on: every queued tree first
-        TC_HIP(hipMemcpyAsync(S.gh_host, h_jobs_[j].gh, sizeof(float2) * N, hipMemcpyDeviceToHost,
+      // GOSS / MVS: the gradients the next tree is grown on (every queued tree first)
+      TC_HIP(hipMemcpyAsync(S.gh_host, h_jobs_[j].gh, sizeof(float2) * N, hipMemcpyDeviceToHost,
+                            ctx_->stream));
+      TC_HIP(hipStreamSynchronize(ctx_->stream));
+      for (uint32_t i = 0; i < N; ++i) S.g[i] = S.gh_host[i].x;
+      if (S.goss) {
+        goss_subsample(S.g.data(), N, S.top_rate, S.other_rate, seed, S.host[b]);
+      } else {  // a sample of its own size each tree: the device reads n_root from the job table
+        h_jobs_[j].n_root = S.n_root_host[b] = mvs_subsample(S.g.data(), N, S.ratio, seed, S.host[b]);
+        TC_HIP(hipMemcpyAsync(reinterpret_cast<char*>(d_jobs_ + j) + offsetof(JobDev, n_root),
+                              &S.n_root_host[b], sizeof(uint32_t), hipMemcpyHostToDevice,
                               ctx_->stream));
-        TC_HIP(hipStreamSynchronize(ctx_->stream));
-        for (uint32_t i = 0; i < N; ++i) S.g[i] = S.gh_host[i].x;
-        if (S.goss) {
-          goss_subsample(S.g.data(), N, S.top_rate, S.other_rate, seed, S.host[b]);
-        } else {  // a sample of its own size each tree: the device reads n_root from the job table
-          h_jobs_[j].n_root = S.n_root_host[b] = mvs_subsample(S.g.data(), N, S.ratio, seed, S.host[b]);
-          TC_HIP(hipMemcpyAsync(reinterpret_cast<char*>(d_jobs_ + j) + offsetof(JobDev, n_root),
-                                &S.n_root_host[b], sizeof(uint32_t), hipMemcpyHostToDevice,
-                                ctx_->stream));
-        }
-      } else {
-        random_subsample(N, S.ratio, seed, S.host[b]);
       }
       TC_HIP(hipMemcpyAsync(h_jobs_[j].rows, S.host[b], 4ull * h_jobs_[j].n_root,
                             hipMemcpyHostToDevice, ctx_->stream));
       TC_HIP(hipEventRecord(S.ev[b], ctx_->stream));
     }
-  }
-
-  // Starts the shuffle of the next tree not yet drawn (seeds in tree order, as
-  // prepare_samples draws them) into its ring slot on a host thread, once the
-  // copy of the tree that used the slot before has left it (that copy was
-  // enqueued a ring's length of trees ago).  The thread makes no HIP call: one
-  // made while this thread captures a tree's graph would void the capture.
-  void submit_draw(uint32_t j, uint32_t N) {
-    Sampler& S = samplers_[j];
-    const uint64_t seed = next_random(S.state);
-    if (S.colsample) (void)next_random(S.state);  // feature subset: drawn, unused
-    const size_t b = S.submitted % S.host.size();
-    TC_HIP(hipEventSynchronize(S.ev[b]));
-    uint32_t* out = S.host[b];
-    uint32_t* sc = S.scratch[b].data();
-    const float ratio = S.ratio;
-    S.draws[b] = std::async(std::launch::async, [=] {
-      const uint32_t k = random_subsample(N, ratio, seed, sc);
-      const uint32_t W = (N + 31u) / 32u, nch = (W + kMaskChunkWords - 1u) / kMaskChunkWords;
-      uint32_t* mask = out + nch;
-      std::memset(mask, 0, 4ull * W);
-      for (uint32_t i = 0; i < k; ++i) mask[sc[i] >> 5] |= 1u << (sc[i] & 31u);
-      uint32_t run = 0;  // rows drawn before each chunk
-      for (uint32_t c = 0; c < nch; ++c) {
-        out[c] = run;
-        const uint32_t w1 = std::min(W, (c + 1u) * kMaskChunkWords);
-        for (uint32_t w = c * kMaskChunkWords; w < w1; ++w)
-          run += static_cast<uint32_t>(__builtin_popcount(mask[w]));
-      }
-    });
-    ++S.submitted;
-  }
-
-  // Words of a row-sample mask of N rows with its per-chunk counts.
-  static size_t mask_words(uint32_t N) {
-    const size_t W = (N + 31u) / 32u;
-    return W + (W + kMaskChunkWords - 1u) / kMaskChunkWords;
   }
 
   bool sampling() const {
@@ -846,33 +915,16 @@ class Trainer {
         SM.g.resize(N);
       }
       J.root_identity = SM.active ? 0u : 1u;
-      if (SM.active) {
-        // a plain row sample is drawn ahead, up to 12 trees on as many host
-        // threads (its shuffle and bitmask take about 3 ms for 1M rows; 256 MB
-        // of shuffle scratch over all jobs at most; 1M x 100, 0.8: 4 slots
-        // 1,388, 8 2,239-2,513, 12 2,984 trees/s); GOSS and MVS need each
-        // tree's gradients, so two buffers alternate
-        SM.ahead = !SM.goss && !SM.mvs;
-        const uint64_t per = 4ull * std::max(1u, N) * njobs_;
-        const size_t R = SM.ahead ? static_cast<size_t>(std::clamp<uint64_t>(
-                                        (uint64_t(256) << 20) / per, 2, 12))
-                                  : 2;
-        SM.host.assign(R, nullptr);
-        SM.ev.assign(R, nullptr);
-        SM.draws.resize(R);
-        // ahead: a slot is the bitmask; the shuffle runs in the slot's scratch
-        const size_t words = SM.ahead ? mask_words(N) : std::max(1u, N);
-        if (SM.ahead) SM.scratch.assign(R, std::vector<uint32_t>(std::max(1u, N)));
-        for (size_t b = 0; b < R; ++b) {
-          TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&SM.host[b]), 4ull * words,
+      if (SM.active && !SM.goss && !SM.mvs) {  // a plain row sample: drawn ahead
+        SM.plain = std::make_unique<AheadSampler>();
+        SM.plain->init(N, SM.ratio, SM.state, SM.colsample, ahead_ring(N, njobs_));
+        J.smask = alloc<uint32_t>(AheadSampler::mask_words(N));
+        mask_chunks_ = std::max(mask_chunks_, AheadSampler::mask_chunks(N));
+      } else if (SM.active) {  // GOSS, MVS: each tree's gradients; two buffers alternate
+        for (int b = 0; b < 2; ++b) {
+          TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&SM.host[b]), 4ull * std::max(1u, N),
                                hipHostMallocDefault));
           TC_HIP(hipEventCreateWithFlags(&SM.ev[b], hipEventDisableTiming));
-        }
-        if (SM.ahead) {
-          J.smask = alloc<uint32_t>(words);
-          mask_chunks_ = std::max<uint32_t>(
-              mask_chunks_, static_cast<uint32_t>(((N + 31u) / 32u + kMaskChunkWords - 1u) /
-                                                  kMaskChunkWords));
         }
       }
       const uint32_t K = channels(S.p);
@@ -1020,16 +1072,13 @@ class Trainer {
     for (void* p : allocs_) dfree(p);
     allocs_.clear();
     for (auto& S : samplers_) {
-      for (auto& d : S.draws)  // draws still running write into the buffers
-        if (d.valid()) d.wait();
-      S.draws.clear();
-      for (size_t b = 0; b < S.host.size(); ++b) {
+      S.plain.reset();  // waits for its draws
+      for (int b = 0; b < 2; ++b) {
         if (S.host[b]) hfree(S.host[b]);
         if (S.ev[b]) (void)hipEventDestroy(S.ev[b]);
+        S.host[b] = nullptr;
+        S.ev[b] = nullptr;
       }
-      S.host.clear();
-      S.ev.clear();
-      S.scratch.clear();
       if (S.gh_host) hfree(S.gh_host);
       S.gh_host = nullptr;
     }
@@ -1045,15 +1094,11 @@ class Trainer {
     float2* gh_host = nullptr;         // pinned copy of the gradients (GOSS, MVS)
     uint32_t n_root_host[2] = {0, 0};  // MVS: each buffer's row count, the source of its copy
     std::vector<float> g;              // their g
-    // pinned row samples and the events of their copies to the device: a
-    // ring of up to 8 drawn ahead (a plain row sample), else two alternating
-    std::vector<uint32_t*> host;
-    std::vector<hipEvent_t> ev;
-    std::vector<std::future<void>> draws;  // the ring's draws (ahead)
-    std::vector<std::vector<uint32_t>> scratch;  // the ring's shuffles (ahead)
-    bool ahead = false;
-    uint64_t submitted = 0, prepared = 0;  // trees drawn / copied (ahead)
+    // GOSS, MVS: pinned row samples and the events of their copies, alternating
+    uint32_t* host[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
     int next = 0;
+    std::unique_ptr<AheadSampler> plain;  // a plain row sample, drawn ahead
   };
 
   tc_ctx* ctx_;

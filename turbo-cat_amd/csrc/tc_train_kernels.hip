@@ -594,18 +594,15 @@ __device__ void tree_begin(const JobDev& J, uint32_t job, const Lists& L, int pa
   }
 }
 
-// The root list of a plain row sample, ascending: rows[0][0 .. n_root) from
-// the sample's bitmask and the rows drawn before each of its chunks (both
-// made on the host).  A node's row order never changes a histogram (every one
-// is an exact integer sum), and ascending rows turn the root's gathers of bins
-// and gradients into sequential reads.  Workgroup = (chunk, job), thread =
-// one 32-row word.
-__global__ void __launch_bounds__(kMaskChunkWords) k_rows_from_mask(const JobDev* __restrict__ jobs) {
-  const JobDev& J = jobs[blockIdx.y];
-  const uint32_t* sm = J.smask;
-  if (!sm) return;
-  const uint32_t W = (J.N + 31u) / 32u, nch = (W + kMaskChunkWords - 1u) / kMaskChunkWords;
-  const uint32_t c = blockIdx.x;
+// The root list of a plain row sample, ascending: rows[0 .. n_root) from the
+// sample's bitmask sm (made on the host: the rows drawn before each chunk of
+// kMaskChunkWords words, then the words; bit r of word r / 32 is row r).  A
+// node's row order never changes a histogram (every one is an exact integer
+// sum), and ascending rows turn the root's gathers of bins and gradients into
+// sequential reads.  One workgroup per chunk c, one thread per 32-row word.
+__device__ inline void rows_from_mask(const uint32_t* __restrict__ sm, uint32_t N, uint32_t n_root,
+                                      uint32_t c, uint32_t* __restrict__ rows) {
+  const uint32_t W = (N + 31u) / 32u, nch = (W + kMaskChunkWords - 1u) / kMaskChunkWords;
   if (c >= nch) return;  // uniform: the grid covers the largest job
   const uint32_t tid = threadIdx.x, w = c * kMaskChunkWords + tid;
   uint32_t bits = w < W ? sm[nch + w] : 0u;
@@ -617,10 +614,23 @@ __global__ void __launch_bounds__(kMaskChunkWords) k_rows_from_mask(const JobDev
   uint32_t pos = sm[c] + incl - cnt;
   for (uint32_t i = 0; i < (tid >> 6); ++i) pos += s_w[i];
   while (bits) {
-    if (pos >= J.n_root) break;  // a mask inconsistent with n_root writes nothing past it
-    J.rows[pos++] = w * 32u + static_cast<uint32_t>(__builtin_ctz(bits));
+    if (pos >= n_root) break;  // a mask inconsistent with n_root writes nothing past it
+    rows[pos++] = w * 32u + static_cast<uint32_t>(__builtin_ctz(bits));
     bits &= bits - 1u;
   }
+}
+
+// Grid (chunk, job): every job with a row-sample mask.
+__global__ void __launch_bounds__(kMaskChunkWords) k_rows_from_mask(const JobDev* __restrict__ jobs) {
+  const JobDev& J = jobs[blockIdx.y];
+  if (J.smask) rows_from_mask(J.smask, J.N, J.n_root, blockIdx.x, J.rows);
+}
+
+// Grid (chunk): one sample (the regular-tree trainer's root rows).
+__global__ void __launch_bounds__(kMaskChunkWords)
+    k_rows_from_mask_one(const uint32_t* __restrict__ sm, uint32_t N, uint32_t n_root,
+                         uint32_t* __restrict__ rows) {
+  rows_from_mask(sm, N, n_root, blockIdx.x, rows);
 }
 
 // Start of a tree, over the row grid: per-channel root totals (sum g, sum h)
