@@ -577,7 +577,8 @@ def run_regular(args):
     def fit(iters, time_hist):
         prm = capi.rtrain_params(loss, n_iterations=iters, learning_rate=lr,
                                  n_classes=K if multi else 0, time_hist=time_hist, **smp)
-        prm.tree = capi.rtree_params(max_depth=D, max_leaves=args.max_leaves, lambda_l2=1.0)
+        prm.tree = capi.rtree_params(max_depth=D, max_leaves=args.max_leaves, lambda_l2=1.0,
+                                     grow_policy=1 if args.grow_policy == "lossguide" else 0)
         return ds.rtrain(y, prm)
 
     fit(args.warmup, 0)  # first use: allocations, code objects
@@ -624,7 +625,8 @@ def run_regular(args):
             p0 = np.full(ns, o.init_prediction(loss, y[:ns]), np.float32)
             g, h = o.gradients_f32(loss, y[:ns], p0)
         t0 = time.perf_counter()
-        o.rtree_f64(bins, g, h, rtree_params(max_depth=D, max_leaves=args.max_leaves))
+        o.rtree_f64(bins, g, h, rtree_params(max_depth=D, max_leaves=args.max_leaves,
+                                             grow_policy=1 if args.grow_policy == "lossguide" else 0))
         sec = time.perf_counter() - t0
         cpu = dict(value=1.0 / (sec * K), unit="iterations/s", cores=1, kind="port",
                    host_cpu=host_cpu_model(),
@@ -639,11 +641,11 @@ def run_regular(args):
         "data": "synthetic",
         "config": {"workload": ((f"multiclass softmax {n} rows x {F} float32 N(0,1) features, "
                                 f"{K} classes, the reference's multiclass loop: one regular "
-                                f"depthwise tree per class and iteration, depth {D}, max_leaves "
+                                f"{args.grow_policy} tree per class and iteration, depth {D}, max_leaves "
                                 f"{args.max_leaves}, lr {lr}, lambda 1") if multi
                    else (f"binary logloss {n} rows x {F} float32 features "
                          f"({F - F // 5} N(0,1), {F // 5} U[0,1) with 5% duplicates), regular "
-                         f"depthwise trees (mode='small') depth {D}, max_leaves "
+                         f"{args.grow_policy} trees (mode='small') depth {D}, max_leaves "
                          f"{args.max_leaves}, lr {lr}, lambda 1")) + SAMPLING_TEXT[args.sampling],
                    "rows": n, "features": F, "classes": K, "depth": D},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
@@ -725,6 +727,8 @@ def main():
                     help="train and regular workloads: row sampling per tree (random 0.8, GOSS "
                          "top 0.2 / other 0.1, MVS 0.5; regular trees: random only; drawn on the "
                          "host with the reference's routines)")
+    ap.add_argument("--grow-policy", default="depthwise", choices=["depthwise", "lossguide"],
+                    help="regular workload: how the trees grow (TreeConfig::grow_policy)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--qps", type=float, default=200.0)

@@ -350,6 +350,9 @@ typedef struct {
   float max_delta_step;        /* > 0: leaf values clamped (tree.cpp:675-678) */
   float leaf_smooth;           /* > 0: leaf value * count / (count + leaf_smooth) */
   int32_t learn_missing_direction;  /* TreeConfig::learn_missing_direction (config.hpp:52) */
+  int32_t grow_policy;         /* TreeConfig::grow_policy (config.hpp:21-32): 0 depthwise
+                                  (Tree::build), 1 lossguide (Tree::build_leafwise,
+                                  tree.cpp:685-925; max_depth 1..11) */
 } tc_rtree_params;
 
 /* Boosting with regular trees: Booster::train's regular branch (booster.cpp:

@@ -58,7 +58,7 @@ def tree_params(max_depth=6, max_bins=255, lambda_l2=1.0, lambda_l1=0.0, max_del
 
 
 class RTreeParams(ctypes.Structure):
-    """TreeConfig fields of the regular depthwise tree (config.hpp:26-59)."""
+    """TreeConfig fields of the regular trees (config.hpp:21-59)."""
     _fields_ = [
         ("max_depth", c_uint32),
         ("max_leaves", c_uint32),
@@ -71,15 +71,18 @@ class RTreeParams(ctypes.Structure):
         ("leaf_smooth", c_float),
         ("max_delta_step", c_float),
         ("learn_missing_direction", c_int32),
+        ("grow_policy", c_int32),
     ]
 
 
 def rtree_params(max_depth=6, max_leaves=64, max_bins=255, min_samples_leaf=1,
                  min_child_weight=1.0, lambda_l2=1.0, min_split_gain=0.0, gamma=0.0,
-                 leaf_smooth=0.0, max_delta_step=0.0, learn_missing_direction=True) -> RTreeParams:
+                 leaf_smooth=0.0, max_delta_step=0.0, learn_missing_direction=True,
+                 grow_policy=0) -> RTreeParams:
+    """grow_policy: 0 depthwise, 1 lossguide (TreeConfig::grow_policy)."""
     return RTreeParams(max_depth, max_leaves, max_bins, min_samples_leaf, min_child_weight,
                        lambda_l2, min_split_gain, gamma, leaf_smooth, max_delta_step,
-                       int(learn_missing_direction))
+                       int(learn_missing_direction), int(grow_policy))
 
 
 class Oracle:
@@ -132,8 +135,10 @@ class Oracle:
             getattr(L, fn).restype = c_int
         L.tco_rtree_build_f64.argtypes = [V, c_uint32, c_uint32, V, V, V, c_uint32,
                                           POINTER(RTreeParams), c_uint32, V, V, V, V, V, V, V,
-                                          V, V, V, V]
+                                          V, V, V, V, V]
         L.tco_rtree_build_f64.restype = c_int
+        L.tco_rqueue_order.argtypes = [V, V, c_uint32, V]
+        L.tco_rqueue_order.restype = c_uint32
         L.tco_rtree_apply_binned.argtypes = [V, V, V, V, V, V, c_uint32, V, c_uint32, V]
         L.tco_rpredict_f32.argtypes = [V, V, V, V, V, V, V, V, V, c_uint32, c_uint32, c_float, V,
                                        c_uint32, c_uint32, V, V, c_int, V]
@@ -378,15 +383,17 @@ class Oracle:
 
     # ------------------------------------------------------- regular trees
     def rtree_f64(self, bins, g, h, params: RTreeParams, rows=None):
-        """One regular depthwise tree (Tree::build, tree.cpp:162-217) in the
+        """One regular tree (Tree::build, tree.cpp:162-217: depthwise, or
+        build_leafwise :685-925 when params.grow_policy is 1) in the
         reference's node numbering; FP64 sums.  Returns a dict of node arrays
-        (flags bit 0 is_leaf, bit 1 default_left) plus n_leaves and depth."""
+        (flags bit 0 is_leaf, bit 1 default_left) plus n_leaves, depth and,
+        for lossguide, hgap (the expansion order's smallest gain margin)."""
         F, n = bins.shape
         cap = (1 << (params.max_depth + 1)) + 1
         feat = np.zeros(cap, np.uint16); b = np.zeros(cap, np.uint8); fl = np.zeros(cap, np.uint8)
         le = np.zeros(cap, np.uint32); ri = np.zeros(cap, np.uint32); val = np.zeros(cap)
         gn = np.zeros(cap); gap = np.zeros(cap)
-        nn = c_uint32(); nl = c_uint32(); dp = ctypes.c_uint16()
+        nn = c_uint32(); nl = c_uint32(); dp = ctypes.c_uint16(); hg = ctypes.c_double()
         r = None if rows is None else np.ascontiguousarray(rows, np.uint32)
         rc = self.L.tco_rtree_build_f64(_p(np.ascontiguousarray(bins, np.uint8)), n, F,
                                         _p(np.ascontiguousarray(g, np.float32)),
@@ -394,12 +401,21 @@ class Oracle:
                                         n if r is None else len(r), ctypes.byref(params), cap,
                                         _p(feat), _p(b), _p(fl), _p(le), _p(ri), _p(val), _p(gn),
                                         _p(gap), ctypes.byref(nn), ctypes.byref(nl),
-                                        ctypes.byref(dp))
+                                        ctypes.byref(dp), ctypes.byref(hg))
         assert rc == 0, "node capacity exceeded"
         k = nn.value
         return dict(n_nodes=k, n_leaves=nl.value, depth=dp.value, feature=feat[:k], bin=b[:k],
                     flags=fl[:k], left=le[:k], right=ri[:k], value=val[:k], gain=gn[:k],
-                    gap=gap[:k])
+                    gap=gap[:k], hgap=hg.value)
+
+    def rqueue_order(self, gains, ops):
+        """Items popped by build_leafwise's candidate queue for a sequence of
+        pushes (item index >= 0) and pops (-1)."""
+        gains = np.ascontiguousarray(gains, np.float64)
+        ops = np.ascontiguousarray(ops, np.int32)
+        out = np.zeros(max(1, len(ops)), np.uint32)
+        k = self.L.tco_rqueue_order(_p(gains), _p(ops), len(ops), _p(out))
+        return out[:k]
 
     def rtree_apply_binned(self, tree, bins):
         """Tree::predict(dataset, row) per row (NaN follows default_left)."""

@@ -180,19 +180,29 @@ typedef struct {
     float leaf_smooth;              /* 0: off */
     float max_delta_step;           /* 0: off */
     int32_t learn_missing_direction;
+    int32_t grow_policy;            /* 0 depthwise (Tree::build), 1 lossguide (build_leafwise) */
 } tco_rtree_params;
 
 /* One regular tree on given gradients, FP64 sums, nodes in the reference's
  * numbering (add_node order: a split node's two children get consecutive
- * ids, the smaller child's subtree is built first).  flags: bit 0 is_leaf,
+ * ids; depthwise: the smaller child's subtree is built first; lossguide: in
+ * the order the priority queue expands the nodes).  flags: bit 0 is_leaf,
  * bit 1 default_left.  gap[i] = best - second best valid gain at split node
- * i (+inf if it had one candidate).  Returns 0, or -1 if more than cap nodes. */
+ * i (+inf if it had one candidate; 0 when the best or a better candidate has
+ * a side whose hessian is min_child_weight within float rounding).  Lossguide also reports hgap: the
+ * smallest gain difference between two queued candidates when one of them
+ * was popped (+inf without such a pair), the margin of the expansion order.
+ * Returns 0, or -1 if more than cap nodes. */
 int tco_rtree_build_f64(const uint8_t* bins, uint32_t n, uint32_t F, const float* g,
                         const float* h, const uint32_t* root, uint32_t n_root,
                         const tco_rtree_params* p, uint32_t cap, uint16_t* feature,
                         uint8_t* bin, uint8_t* flags, uint32_t* left, uint32_t* right,
                         double* value, double* gain, double* gap, uint32_t* n_nodes,
-                        uint32_t* n_leaves, uint16_t* depth);
+                        uint32_t* n_leaves, uint16_t* depth, double* hgap);
+/* build_leafwise's candidate queue alone (std::priority_queue by gain):
+ * ops >= 0 push that item, < 0 pop; popped items in order; returns the pops. */
+uint32_t tco_rqueue_order(const double* gains, const int32_t* ops, uint32_t n_ops,
+                          uint32_t* popped);
 /* Tree::predict(dataset, row) over column-major bins: leaf value per row. */
 void tco_rtree_apply_binned(const uint16_t* feature, const uint8_t* bin, const uint8_t* flags,
                             const uint32_t* left, const uint32_t* right, const float* value,

@@ -106,8 +106,10 @@ def test_not_fitted_and_unsupported_options():
         clf.predict_proba(X)
     with pytest.raises(RuntimeError, match="not fitted"):
         turbocat.TurboCatRegressor().predict(X)
-    with pytest.raises(ValueError, match="lossguide"):
-        turbocat.TurboCatRegressor(mode="small", grow_policy="lossguide", subsample=1.0).fit(X, y)
+    with pytest.raises(ValueError, match="use_ordered_boosting"):
+        turbocat.TurboCatClassifier(use_ordered_boosting=True, subsample=1.0).fit(X, y > 0)
+    with pytest.raises(ValueError, match="use_gradtree"):
+        turbocat.TurboCatClassifier(use_gradtree=True, subsample=1.0).fit(X, y > 0)
     with pytest.raises(ValueError, match="use_goss with regular trees"):
         turbocat.TurboCatClassifier(mode="small", use_goss=True, subsample=1.0).fit(X, y > 0)
     with pytest.raises(ValueError, match="use_goss with regular trees"):  # multiclass: regular trees

@@ -12,7 +12,14 @@ trees.  Float claims: leaf values and gains within 1e-5 of the FP64 oracle
 the FP64 gap between a node's best and second-best candidate is below
 NEAR_TIE * max(1, |gain|), either choice is accepted and the rest of the tree
 (whose numbering then differs) is not compared; such events are counted and
-must stay rare."""
+must stay rare.
+
+Lossguide (Tree::build_leafwise): the same claims, in the expansion order's
+numbering, on cases whose candidate-queue margin (the oracle's hgap: the
+smallest gain difference at a pop) is above the float gains' rounding; with
+room for every leaf the leaf-wise tree equals the depthwise one (compared
+from the root, independent of numbering); boosting fits are compared tree by
+tree the same way."""
 import numpy as np
 import pytest
 
@@ -30,7 +37,8 @@ def oparams(p):
                         min_samples_leaf=p.min_samples_leaf, min_child_weight=p.min_child_weight,
                         lambda_l2=p.lambda_l2, min_split_gain=p.min_split_gain, gamma=p.gamma,
                         leaf_smooth=p.leaf_smooth, max_delta_step=p.max_delta_step,
-                        learn_missing_direction=bool(p.learn_missing_direction))
+                        learn_missing_direction=bool(p.learn_missing_direction),
+                        grow_policy=p.grow_policy)
 
 
 def term_scales(ref, bins, g, h, lam, rows=None):
@@ -449,3 +457,179 @@ def test_dropin_mode_small_and_multiclass(tmp_path):
                                      colsample_bytree=1.0, verbosity=0).fit(X, yr)
     assert reg.n_trees == 30
     assert np.corrcoef(reg.predict(X), yr)[0, 1] > 0.8
+
+
+# ------------------------------------------------------------------ lossguide
+LW_CASES = [  # (tree params, loss, data seed): every choice with a margin above rounding
+    (dict(max_depth=6, max_leaves=16), "mse", 680),
+    (dict(max_depth=4, max_leaves=64), "logloss", 437),
+    (dict(max_depth=4, max_leaves=64), "mse", 402),
+    (dict(max_depth=5, max_leaves=6, min_child_weight=3.0, lambda_l2=0.0, min_split_gain=0.5),
+     "logloss", 410),
+    (dict(max_depth=5, max_leaves=6, min_child_weight=3.0, lambda_l2=0.0, min_split_gain=0.5),
+     "mse", 403),
+    (dict(max_depth=8, max_leaves=10, min_samples_leaf=50), "logloss", 507),
+    (dict(max_depth=8, max_leaves=10, min_samples_leaf=50), "mse", 500),
+    (dict(max_depth=10, max_leaves=8, min_child_weight=5.0), "logloss", 556),
+    (dict(max_depth=10, max_leaves=8, min_child_weight=5.0), "mse", 521),
+    (dict(max_depth=7, max_leaves=6), "mse", 500),
+]
+
+
+def lw_data(ctx, oracle, loss, seed):
+    from oracle import LOSS_LOGLOSS, LOSS_MSE
+    X, yr = data_with_nan(6000, 9, seed, nan_frac=0.05)
+    lid = LOSS_LOGLOSS if loss == "logloss" else LOSS_MSE
+    y = (yr > 0).astype(np.float32) if loss == "logloss" else yr
+    ds, edges, ne = ctx.quantize_fit(X)
+    p0 = np.full(len(y), oracle.init_prediction(lid, y), np.float32)
+    g, h = oracle.gradients_f32(lid, y, p0)
+    return ds, ds.bins(), g, h
+
+
+def lossguide_margin(ref, bins, g, h, p):
+    """The oracle tree's smallest margin, in units of the comparison's
+    rounding allowance: every split's gap to its runner-up (0 at a hessian
+    border), its distance from min_split_gain, and the expansion order's gap
+    (hgap).  Above 1, a float implementation makes the same choices and
+    expands the nodes in the same order."""
+    with np.errstate(all="ignore"):
+        sc = np.nan_to_num(term_scales(ref, bins, g, h, p.lambda_l2))
+    m = ref["hgap"] / (NEAR_TIE * max(1.0, float(np.abs(ref["gain"]).max())) + 1e-6 * float(sc.max()))
+    for i in range(ref["n_nodes"]):
+        if int(ref["flags"][i]) & 1:
+            continue
+        eps = NEAR_TIE * max(1.0, abs(ref["gain"][i])) + 1e-6 * sc[i]
+        m = min(m, ref["gap"][i] / eps, abs(ref["gain"][i] - p.min_split_gain) / (10 * eps))
+    return m
+
+
+@pytest.mark.parametrize("ci", range(len(LW_CASES)))
+def test_rtree_lossguide_teacher_forced(ctx, oracle, ci):
+    """One leaf-wise tree on the oracle's gradients: the GPU's nodes are the
+    FP64 oracle's build_leafwise, numbered in expansion order, on cases whose
+    every choice has a margin (a near-tie anywhere may reorder every later
+    expansion, and the numbering with it)."""
+    from turbocat.capi import rtree_params
+    kw, loss, seed = LW_CASES[ci]
+    ds, bins, g, h = lw_data(ctx, oracle, loss, seed)
+    p = rtree_params(grow_policy=1, **kw)
+    gpu = ds.rtree(g, h, p)
+    ref = oracle.rtree_f64(bins, g, h, oparams(p))
+    assert lossguide_margin(ref, bins, g, h, p) > 1.0
+    n, tie = compare_rtree(gpu, ref, g, h, p, bins)
+    assert not tie and n == ref["n_nodes"]
+
+
+def walk_compare(gpu, ref, p, vscale, i=0, j=0):
+    """Two trees from the root, independent of numbering: the same splits
+    (outside per-node FP64 near-ties) and leaf values within 1e-5.
+    Returns True when a near-tie stopped the walk."""
+    gl, rl = int(gpu["flags"][i]) & 1, int(ref["flags"][j]) & 1
+    if not rl and ref["gap"][j] <= NEAR_TIE * max(1.0, abs(ref["gain"][j])):
+        if gl or (gpu["feature"][i], gpu["bin"][i]) != (ref["feature"][j], ref["bin"][j]):
+            return True
+    assert gl == rl and int(gpu["flags"][i]) == int(ref["flags"][j]), (i, j)
+    if rl:
+        v = float(ref["value"][j])
+        assert abs(float(gpu["value"][i]) - v) <= 1e-5 * max(1.0, abs(v)) + \
+            1e-6 * vscale / (p.lambda_l2 + 1e-3), (i, j)
+        return False
+    assert (int(gpu["feature"][i]), int(gpu["bin"][i])) == (int(ref["feature"][j]), int(ref["bin"][j]))
+    a = walk_compare(gpu, ref, p, vscale, int(gpu["left"][i]), int(ref["left"][j]))
+    b = walk_compare(gpu, ref, p, vscale, int(gpu["right"][i]), int(ref["right"][j]))
+    return a or b
+
+
+@pytest.mark.parametrize("D,seed", [(3, 410), (6, 411), (9, 412)])
+def test_rtree_lossguide_without_a_cut_is_the_depthwise_tree(ctx, oracle, D, seed):
+    """With room for every leaf (max_leaves >= 2^max_depth) a leaf-wise tree
+    grows every splittable node, as the depthwise tree does: the two GPU
+    trees agree from the root down (numbered differently), and the leaf-wise
+    tree matches the oracle's the same way."""
+    from turbocat.capi import rtree_params
+    ds, bins, g, h = lw_data(ctx, oracle, "logloss", seed)
+    kw = dict(max_depth=D, max_leaves=1 << D, min_samples_leaf=10)
+    a = ds.rtree(g, h, rtree_params(**kw))
+    p = rtree_params(grow_policy=1, **kw)
+    b = ds.rtree(g, h, p)
+    assert (a["n_nodes"], a["n_leaves"], a["depth"]) == (b["n_nodes"], b["n_leaves"], b["depth"])
+
+    def canon(t, i=0):
+        if int(t["flags"][i]) & 1:
+            return ("leaf", int(t["flags"][i]), float(t["value"][i]))
+        return (int(t["feature"][i]), int(t["bin"][i]), int(t["flags"][i]),
+                canon(t, int(t["left"][i])), canon(t, int(t["right"][i])))
+    assert canon(a) == canon(b)
+    ref = oracle.rtree_f64(bins, g, h, oparams(p))
+    vscale = float(np.abs(g.astype(np.float64)).sum()) + 1.0
+    assert not walk_compare(b, ref, p, vscale)
+
+
+def test_rtrain_lossguide_matches_oracle(ctx, oracle):
+    """Boosting with leaf-wise trees (Booster::train's regular branch with
+    GrowPolicy::Lossguide): each tree against the oracle's on the replayed
+    predictions, compared from the root; the replay equals the GPU's
+    training predictions bit for bit."""
+    from turbocat import capi
+    from oracle import LOSS_LOGLOSS
+    X, yr = regression_data(8000, 8, 31)
+    y = (yr > 0).astype(np.float32)
+    ds, _, _ = ctx.quantize_fit(X)
+    bins = ds.bins()
+    p = capi.rtree_params(max_depth=5, max_leaves=32, lambda_l2=1.0, grow_policy=1)
+    tp = capi.rtrain_params(capi.TC_LOSS_LOGLOSS, n_iterations=6, learning_rate=0.3)
+    tp.tree = p
+    m = ds.rtrain(y, tp)
+    preds = np.full(len(y), np.float32(m["base"]), np.float32)
+    ties = 0
+    for t in m["trees"]:
+        g, h = oracle.gradients_f32(LOSS_LOGLOSS, y, preds)
+        ref = oracle.rtree_f64(bins, g, h, oparams(p))
+        vscale = float(np.abs(g.astype(np.float64)).sum()) + 1.0
+        ties += walk_compare(t, ref, p, vscale)
+        preds = oracle.fma_update(preds, np.float32(0.3), oracle.rtree_apply_binned(t, bins))
+    assert ties <= 1
+    assert np.array_equal(m["train_preds"][0], preds)
+
+
+def test_rtree_lossguide_parameter_checks(ctx):
+    from turbocat import capi
+    X, y = regression_data(500, 4, 7)
+    ds, _, _ = ctx.quantize_fit(X)
+    g = np.ones(500, np.float32)
+    with pytest.raises(capi.TurboCatError, match="grow_policy"):
+        ds.rtree(g, g, capi.rtree_params(grow_policy=2))
+    with pytest.raises(capi.TurboCatError, match="lossguide"):
+        ds.rtree(g, g, capi.rtree_params(max_depth=12, grow_policy=1))
+
+
+def test_dropin_lossguide(ctx):
+    """grow_policy='lossguide' (also 'leaf', 'leafwise') grows leaf-wise
+    regular trees through the drop-in classes; saving and loading keeps the
+    predictions."""
+    import os
+    import tempfile
+    from turbocat import TurboCatRegressor
+    X, y = regression_data(5000, 6, 52)
+    reg = TurboCatRegressor(n_estimators=30, mode="small", grow_policy="lossguide", max_depth=8,
+                            max_leaves=24, subsample=1.0, colsample_bytree=1.0, verbosity=0)
+    reg.fit(X, y)
+    pr = reg.predict(X)
+    assert 1.0 - np.mean((pr - y) ** 2) / np.var(y) > 0.8
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "m.tcat")
+        reg.save(path)
+        r2 = TurboCatRegressor(mode="small", verbosity=0).load(path)
+        assert np.array_equal(r2.predict(X), pr)
+    # the classifier's grow_policy is a keyword of the pybind class (the
+    # reference's Python wrapper does not forward it), as in the reference
+    from turbocat import _turbocat
+    yc = (y > np.median(y)).astype(np.float32)
+    Xe, ye = np.zeros((0, X.shape[1]), np.float32), np.zeros(0, np.float32)
+    for gp in ("leaf", "leafwise"):
+        clf = _turbocat.TurboCatClassifier(n_estimators=20, mode="small", grow_policy=gp,
+                                           max_depth=6, subsample=1.0, colsample_bytree=1.0,
+                                           verbosity=0)
+        clf.fit(X, yc, Xe, ye, [])
+        assert (np.asarray(clf.predict(X)) == yc).mean() > 0.85

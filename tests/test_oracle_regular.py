@@ -21,7 +21,18 @@ by:
 * the compiled reference itself where the two tree kinds coincide: a depth-1
   regular tree is the symmetric tree's root split (oracle/_ref);
 * planted mistakes that these comparisons must notice.
+
+Lossguide (Tree::build_leafwise, tree.cpp:685-925) is pinned the same way: an
+exact leaf-wise builder below (the candidate of largest gain expanded first;
+children queued left first when they may split, one more leaf would fit and
+their split is valid; depth-limited candidates and the queue's remainder
+become leaves), the invariant that with max_leaves >= 2^max_depth it grows
+the depthwise tree (numbered differently), and the candidate queue's pop
+order against std::priority_queue itself (compiled here with g++), ties
+included.
 """
+import os
+import subprocess
 from fractions import Fraction
 
 import numpy as np
@@ -126,6 +137,106 @@ def assert_same(tree, exact):
             assert (int(tree["feature"][i]), int(tree["bin"][i])) == (nd["feature"], nd["bin"]), f"node {i}"
             assert (int(tree["left"][i]), int(tree["right"][i])) == (nd["left"], nd["right"])
             assert abs(tree["gain"][i] - float(nd["gain"])) <= 1e-9 * max(1.0, abs(float(nd["gain"])))
+
+
+def exact_leafwise(bins, g, h, p, rows=None, mistake=None):
+    """Regular lossguide tree in exact arithmetic (build_leafwise,
+    tree.cpp:685-925).  Returns (nodes, leaves, depth, tie): tie is True when
+    two queued candidates had equal exact gains at a pop (the library queue
+    then decides; pinned separately)."""
+    F, n = bins.shape
+    lam = Fraction(float(p.lambda_l2))
+    gamma = Fraction(float(p.gamma))
+    msg = Fraction(float(p.min_split_gain))
+    mcw = Fraction(float(p.min_child_weight))
+    gF = [Fraction(float(v)) for v in g]
+    hF = [Fraction(float(v)) for v in h]
+    nodes, st = [], dict(leaves=0, depth=0, tie=False)
+
+    def add():
+        nodes.append(dict(feature=0, bin=0, leaf=True, dl=True, left=0, right=0, value=0,
+                          gain=0))
+        return len(nodes) - 1
+
+    def leaf(i, G, H, C):
+        v = -G / (H + lam)
+        if p.leaf_smooth > 0:
+            v = v * (Fraction(C) / (Fraction(C) + Fraction(float(p.leaf_smooth))))
+        if p.max_delta_step > 0:
+            m = Fraction(float(p.max_delta_step))
+            v = max(-m, min(m, v))
+        nodes[i]["value"] = v
+        st["leaves"] += 1
+
+    def find(rs, G, H, C):
+        best = None
+        pg = G * G / (H + lam)
+        for f in range(F):
+            col = bins[f]
+            top = int(max((col[r] for r in rs if col[r] != 255), default=0))
+            for b in range(min(p.max_bins - 1, top + 1)):
+                L = [r for r in rs if col[r] <= b]
+                GL = sum((gF[r] for r in L), Fraction(0))
+                HL = sum((hF[r] for r in L), Fraction(0))
+                CL = len(L)
+                GR, HR, CR = G - GL, H - HL, C - CL
+                if CL < p.min_samples_leaf or HL < mcw or CR < p.min_samples_leaf or HR < mcw:
+                    continue
+                gn = Fraction(1, 2) * (GL * GL / (HL + lam) + GR * GR / (HR + lam) - pg) - gamma
+                if gn >= msg and (best is None or gn > best[0]):
+                    best = (gn, f, b, GL, HL, CL)
+        return best
+
+    rs = list(range(n)) if rows is None else [int(r) for r in rows]
+    if not rs:
+        return nodes, 0, 0, False
+    G = sum((gF[r] for r in rs), Fraction(0))
+    H = sum((hF[r] for r in rs), Fraction(0))
+    root = add()
+    best = find(rs, G, H, len(rs))
+    if best is None:
+        leaf(root, G, H, len(rs))
+        return nodes, st["leaves"], 0, False
+    queue = [dict(node=root, rows=rs, depth=0, G=G, H=H, C=len(rs), best=best)]
+    max_leaves = p.max_leaves
+    while queue and st["leaves"] < max_leaves:
+        gains = sorted((c["best"][0] for c in queue), reverse=True)
+        if len(gains) > 1 and gains[0] == gains[1]:
+            st["tie"] = True
+        k = max(range(len(queue)), key=lambda i: queue[i]["best"][0])
+        if mistake == "min_first":
+            k = min(range(len(queue)), key=lambda i: queue[i]["best"][0])
+        c = queue.pop(k)
+        if c["depth"] >= p.max_depth and mistake != "no_depth_check":
+            leaf(c["node"], c["G"], c["H"], c["C"])
+            continue
+        gn, f, b, GL, HL, CL = c["best"]
+        nd = nodes[c["node"]]
+        nd.update(feature=f, bin=b, leaf=False, gain=gn)
+        col = bins[f]
+        lrows = [r for r in c["rows"] if (col[r] == 255 and nd["dl"]) or (col[r] != 255 and col[r] <= b)]
+        lset = set(lrows)
+        rrows = [r for r in c["rows"] if r not in lset]
+        if p.learn_missing_direction and 0 < len(lrows) < len(c["rows"]):
+            nd["dl"] = len(lrows) >= len(rrows)
+        lc, rc = add(), add()
+        nd["left"], nd["right"] = lc, rc
+        st["depth"] = max(st["depth"], c["depth"] + 1)
+        gate = (lambda: st["leaves"] + 1 < max_leaves) if mistake != "gate_le" else \
+            (lambda: st["leaves"] + 1 <= max_leaves)
+        for cid, rws, G2, H2, C2 in ((lc, lrows, GL, HL, CL),
+                                     (rc, rrows, c["G"] - GL, c["H"] - HL, c["C"] - CL)):
+            can = len(rws) >= 2 * p.min_samples_leaf and H2 >= 2 * mcw
+            if can and gate():
+                b2 = find(rws, G2, H2, C2)
+                if b2 is not None:
+                    queue.append(dict(node=cid, rows=rws, depth=c["depth"] + 1, G=G2, H=H2, C=C2,
+                                      best=b2))
+                    continue
+            leaf(cid, G2, H2, C2)
+    for c in queue:
+        leaf(c["node"], c["G"], c["H"], c["C"])
+    return nodes, st["leaves"], st["depth"], st["tie"]
 
 
 def small_case(seed, n=36, F=3, nan=True):
@@ -307,3 +418,122 @@ def test_rpredict_sum_is_one_fma_per_tree(oracle):
     c2 = np.float32(np.float64(w[0]) * np.float64(vals[0]))
     c2 = np.float32(np.float64(w[2]) * np.float64(vals[2]) + np.float64(c2))
     assert list(out3) == [c0, 0.0, c2]
+
+
+# ------------------------------------------------------------------ lossguide
+LW_PARAMS = [
+    dict(max_depth=3, max_leaves=64),                       # no cut: every splittable node
+    dict(max_depth=4, max_leaves=2),                        # the loop stops at 2 leaves made
+    dict(max_depth=4, max_leaves=4),
+    dict(max_depth=2, max_leaves=5),                        # depth-limited pops
+    dict(max_depth=4, max_leaves=6, min_samples_leaf=4, min_child_weight=0.8),
+    dict(max_depth=3, max_leaves=5, lambda_l2=0.0, min_split_gain=0.05, gamma=0.02),
+    dict(max_depth=3, max_leaves=7, leaf_smooth=2.0, max_delta_step=0.7),
+    dict(max_depth=3, max_leaves=3, learn_missing_direction=False, max_bins=4),
+]
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("pi", range(len(LW_PARAMS)))
+def test_lossguide_matches_exact_builder(oracle, seed, pi):
+    bins, g, h = small_case(seed)
+    p = rtree_params(**{**dict(min_child_weight=0.1, grow_policy=1), **LW_PARAMS[pi]})
+    rows = None
+    if seed % 3 == 2:
+        rows = np.random.default_rng(seed + 50).permutation(bins.shape[1])[:28]
+    nodes, nl, depth, tie = exact_leafwise(bins, g, h, p, rows)
+    assert not tie  # no equal exact gains at a pop on these cases (the queue order is pinned below)
+    assert_same(oracle.rtree_f64(bins, g, h, p, rows), (nodes, nl, depth))
+
+
+@pytest.mark.parametrize("mistake", ["min_first", "no_depth_check", "gate_le"])
+def test_lossguide_comparison_notices_planted_mistakes(oracle, mistake):
+    hits = 0
+    for seed in range(6):
+        bins, g, h = small_case(seed)
+        for kw in LW_PARAMS[1:5]:
+            p = rtree_params(**{**dict(min_child_weight=0.1, grow_policy=1), **kw})
+            wrong = exact_leafwise(bins, g, h, p, mistake=mistake)[:3]
+            try:
+                assert_same(oracle.rtree_f64(bins, g, h, p), wrong)
+            except AssertionError:
+                hits += 1
+    assert hits > 0
+
+
+def canon_tree(t, i=0):
+    """A tree as nested tuples from its root: the same tree in any numbering
+    gives the same value."""
+    if t["flags"][i] & 1:
+        return ("leaf", int(t["flags"][i]), round(float(t["value"][i]), 12))
+    return (int(t["feature"][i]), int(t["bin"][i]), int(t["flags"][i]),
+            canon_tree(t, int(t["left"][i])), canon_tree(t, int(t["right"][i])))
+
+
+@pytest.mark.parametrize("D,msl,mcw", [(1, 1, 1.0), (3, 5, 1.0), (5, 5, 0.5), (6, 20, 3.0)])
+def test_lossguide_without_a_cut_grows_the_depthwise_tree(oracle, D, msl, mcw):
+    """Each node's split depends on its own rows only, so with room for every
+    leaf (max_leaves >= 2^max_depth) the leaf-wise tree is the depthwise one,
+    numbered in expansion order instead of depth first."""
+    rng = np.random.default_rng(D)
+    n, F = 4000, 6
+    bins = rng.integers(0, 40, (F, n)).astype(np.uint8)
+    bins[rng.integers(0, F, 50), rng.integers(0, n, 50)] = 255
+    g = rng.standard_normal(n).astype(np.float32)
+    h = rng.uniform(0.1, 0.6, n).astype(np.float32)
+    kw = dict(max_depth=D, max_leaves=1 << D, min_samples_leaf=msl, min_child_weight=mcw)
+    a = oracle.rtree_f64(bins, g, h, rtree_params(**kw))
+    b = oracle.rtree_f64(bins, g, h, rtree_params(grow_policy=1, **kw))
+    assert (a["n_nodes"], a["n_leaves"], a["depth"]) == (b["n_nodes"], b["n_leaves"], b["depth"])
+    assert canon_tree(a) == canon_tree(b)
+
+
+QUEUE_PROGRAM = r"""
+#include <cstdio>
+#include <queue>
+#include <vector>
+struct Item {
+  double gain;
+  unsigned id;
+  bool operator<(const Item& o) const { return gain < o.gain; }
+};
+int main() {
+  unsigned n_gains, n_ops;
+  if (scanf("%u %u", &n_gains, &n_ops) != 2) return 1;
+  std::vector<double> g(n_gains);
+  for (auto& x : g) if (scanf("%lf", &x) != 1) return 1;
+  std::priority_queue<Item> q;
+  for (unsigned i = 0; i < n_ops; ++i) {
+    int op;
+    if (scanf("%d", &op) != 1) return 1;
+    if (op >= 0) q.push(Item{g[op], static_cast<unsigned>(op)});
+    else if (!q.empty()) { printf("%u\n", q.top().id); q.pop(); }
+  }
+  return 0;
+}
+"""
+
+
+def test_lossguide_queue_order_matches_std_priority_queue(oracle, tmp_path):
+    """The candidate queue's pop order, with many equal gains, against
+    std::priority_queue<LeafCandidate>-style items compiled here."""
+    src, exe = tmp_path / "q.cpp", tmp_path / "q"
+    src.write_text(QUEUE_PROGRAM)
+    subprocess.run(["g++", "-O1", "-std=c++17", str(src), "-o", str(exe)], check=True)
+    for seed in range(20):
+        rng = np.random.default_rng(seed)
+        m = 300
+        gains = rng.integers(0, 4 if seed % 2 else 40, m).astype(np.float64) * 0.5
+        ops, pushed = [], 0
+        while pushed < m:
+            if rng.random() < 0.6:
+                ops.append(pushed)
+                pushed += 1
+            else:
+                ops.append(-1)
+        ops += [-1] * m
+        inp = f"{m} {len(ops)}\n" + " ".join(repr(float(x)) for x in gains) + "\n" + \
+            " ".join(str(o) for o in ops) + "\n"
+        out = subprocess.run([str(exe)], input=inp, capture_output=True, text=True, check=True)
+        ref = np.array([int(x) for x in out.stdout.split()], np.uint32)
+        assert np.array_equal(oracle.rqueue_order(gains, ops), ref), seed

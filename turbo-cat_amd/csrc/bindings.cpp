@@ -3,14 +3,16 @@
 // names, keyword arguments, defaults, return types and error behaviour
 // (RuntimeError "Model not fitted. Call fit() first." before fit; ValueError
 // for invalid arguments).  mode="auto"/"large" grows symmetric trees,
-// mode="small" regular depthwise trees (bindings.cpp:96-103, :150-158);
-// labels with more than two classes train the reference's multiclass loop
-// (regular per-class trees, booster.cpp:1263-1400) whatever the mode, and
-// predict through those trees.  Options whose reference code path lies
-// outside these (lossguide growth, GradTree, ordered boosting, categorical
-// features, interactions, early-stopping metrics other than the loss, other
-// losses, GOSS/MVS with regular trees) raise ValueError naming the option
-// instead of silently training something else.  A validation set early-stops
+// mode="small" regular trees (bindings.cpp:96-103, :150-158), grown depth-wise
+// or, with grow_policy "lossguide" / "leaf" / "leafwise" (bindings.cpp:88-93),
+// leaf-wise; labels with more than two classes train the reference's
+// multiclass loop (regular per-class trees, booster.cpp:1263-1400) whatever
+// the mode, and predict through those trees.  Symmetric trees ignore
+// grow_policy, as in the reference.  Options whose reference code path lies
+// outside these (GradTree, ordered boosting, categorical features,
+// interactions, early-stopping metrics other than the loss, other losses,
+// GOSS/MVS with regular trees) raise ValueError naming the option instead of
+// silently training something else.  A validation set early-stops
 // on its loss (booster.cpp:255-371, :1400-1430); use_goss samples each
 // symmetric tree's rows by GOSS (booster.cpp:207-212).
 #include <pybind11/numpy.h>
@@ -139,7 +141,7 @@ class TurboCatClassifier {
       p.loss = TC_LOSS_SOFTMAX;
     }
     const bool regular = !p.use_symmetric || n_classes_ > 2;
-    reject(regular && lossguide(grow_policy_), "grow_policy='" + grow_policy_ + "' (lossguide growth)");
+    p.lossguide = lossguide(grow_policy_);  // read by regular trees only
     reject(regular && p.use_goss, "use_goss with regular trees");
     booster_ = std::make_unique<Booster>(p);
     if (p_.verbosity > 0)
@@ -284,9 +286,10 @@ class TurboCatRegressor {
     if (static_cast<uint32_t>(y.shape(0)) != m.n) throw std::invalid_argument("X and y differ in rows");
     const Matrix v = validation(X_val, y_val, m.F);
     reject(!(loss_ == "mse" || loss_ == "l2"), "loss='" + loss_ + "'");
-    reject(!p_.use_symmetric && lossguide(grow_policy_), "grow_policy='" + grow_policy_ + "' (lossguide growth)");
     reject(!p_.use_symmetric && p_.use_goss, "use_goss with regular trees");
-    booster_ = std::make_unique<Booster>(p_);
+    Params p = p_;
+    p.lossguide = lossguide(grow_policy_);  // read by regular trees only
+    booster_ = std::make_unique<Booster>(p);
     if (p_.verbosity > 0)
       std::printf("[TurboCat/MI355X] fit: n_samples=%u, n_features=%u, n_estimators=%u\n", m.n, m.F,
                   p_.n_estimators);

@@ -151,8 +151,9 @@ void Booster::fit(const float* X, const float* y, uint32_t n, uint32_t F, const 
 // (tc_rtrain_valid); the model is kept as the reference's Tree::nodes.
 void Booster::fit_regular(const float* X, const float* y, uint32_t n, uint32_t F,
                           const float* Xv, const float* yv, uint32_t nv) {
-  if (p_.max_depth < 1 || p_.max_depth > 12)
-    throw std::invalid_argument("max_depth must be in [1, 12] for regular trees");
+  if (p_.max_depth < 1 || p_.max_depth > (p_.lossguide ? 11u : 12u))
+    throw std::invalid_argument(p_.lossguide ? "max_depth must be in [1, 11] for lossguide trees"
+                                             : "max_depth must be in [1, 12] for regular trees");
   if (p_.max_bins < 2 || p_.max_bins > 255) throw std::invalid_argument("max_bins must be in [2, 255]");
   if (!(p_.subsample > 0.0f && p_.subsample <= 1.0f))
     throw std::invalid_argument("subsample must be in (0, 1]");
@@ -183,6 +184,7 @@ void Booster::fit_regular(const float* X, const float* y, uint32_t n, uint32_t F
   tp.tree.max_delta_step = p_.max_delta_step;
   tp.tree.leaf_smooth = p_.leaf_smooth;
   tp.tree.learn_missing_direction = p_.learn_missing ? 1 : 0;
+  tp.tree.grow_policy = p_.lossguide ? 1 : 0;
   tp.subsample = p_.subsample;
   tp.colsample_bytree = p_.colsample_bytree;
   tp.seed = p_.seed;

@@ -48,6 +48,7 @@ struct Params {
   float gamma = 0.0f;
   float leaf_smooth = 0.0f;
   bool learn_missing = true;
+  bool lossguide = false;     // regular trees grown leaf-wise (GrowPolicy::Lossguide)
   uint32_t n_classes = 2;     // > 2: softmax over regular per-class trees (train_multiclass)
   // run
   int32_t loss = TC_LOSS_MSE;

@@ -68,7 +68,9 @@ struct RJob {
   const uint8_t* bins_rm;   // [N][Fp]
   const uint8_t* bins_cm;   // [F][N]
   uint32_t N, F, Fp, T;     // T = feature tiles
-  uint32_t D, max_leaves, max_bins, min_samples_leaf;
+  uint32_t D, max_leaves, max_bins, min_samples_leaf;  // D: levels grown
+  uint32_t leafwise;        // 1: lossguide (grown to max_depth + 1, k_rleafwise replays)
+  uint32_t max_depth;       // the tree's max_depth (D, or D - 1 for lossguide)
   float lambda_l2, min_child_weight, min_split_gain, gamma, max_delta_step, leaf_smooth;
   int32_t learn_missing;
   int32_t mode;             // k_hist cell mode: 1 packed g + count (MSE), 2 packed h + count
@@ -767,6 +769,153 @@ __global__ void __launch_bounds__(256) k_rfinish(RJob* __restrict__ jobs, RLists
   J.m_depth[t] = static_cast<uint16_t>(depth);
 }
 
+// End of a lossguide tree (Tree::build_leafwise, tree.cpp:685-925), one
+// workgroup per job.  Every node's split depends on its own rows only, so the
+// tree was grown level by level to max_depth + 1 (every node the leaf-wise
+// loop may queue has its split); one thread now replays the reference's loop
+// over the grown nodes: the root is queued if its split is valid; while
+// candidates are queued and fewer than max_leaves leaves exist, the one of
+// largest gain is popped -- a leaf if it is at max_depth, else it splits and
+// its children (ids n, n + 1; left first) are queued when they may split
+// (rows >= 2 min_samples_leaf, hess >= 2 min_child_weight, a valid split: the
+// grown node split) and leaves + 1 < max_leaves, or become leaves; what is
+// left in the queue becomes leaves.  The queue is std::priority_queue's
+// binary max-heap by gain, pushed and popped in libstdc++'s steps, so that
+// equal gains leave it in the reference's order.
+struct RCand {
+  float gain;
+  uint32_t g, id;  // grown node, id in the tree
+};
+constexpr uint32_t kRLeafLds = 1024;    // grown nodes staged in LDS
+constexpr uint32_t kRHeap = 1u << (kRMaxDepth - 1);  // queued at once: at most 2^max_depth
+
+__device__ inline void rq_up(RCand* q, uint32_t hole, RCand v) {
+  while (hole > 0) {  // std::__push_heap: past every parent that compares less
+    const uint32_t parent = (hole - 1) / 2;
+    if (!(q[parent].gain < v.gain)) break;
+    q[hole] = q[parent];
+    hole = parent;
+  }
+  q[hole] = v;
+}
+
+__device__ inline RCand rq_pop(RCand* q, uint32_t* n) {
+  const RCand top = q[0];
+  const uint32_t len = *n - 1;  // the heap without its last slot
+  if (len > 0) {  // std::__pop_heap + std::__adjust_heap
+    const RCand v = q[len];
+    uint32_t hole = 0, second = 0;
+    while (second < (len - 1) / 2) {  // the hole to a leaf along the larger child
+      second = 2 * (second + 1);
+      if (q[second].gain < q[second - 1].gain) --second;
+      q[hole] = q[second];
+      hole = second;
+    }
+    if ((len & 1u) == 0 && second == (len - 2) / 2) {
+      second = 2 * (second + 1);
+      q[hole] = q[second - 1];
+      hole = second - 1;
+    }
+    rq_up(q, hole, v);
+  }
+  *n = len;
+  return top;
+}
+
+__global__ void __launch_bounds__(256) k_rleafwise(RJob* __restrict__ jobs, RLists L) {
+  RJob& J = jobs[blockIdx.y];
+  __shared__ RFin s_n[kRLeafLds];
+  __shared__ RCand s_q[kRHeap];
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) {
+    *L.hist_count[0] = 0;
+    *L.part_count[0] = 0;
+  }
+  TreeState* st = J.st;
+  const int t = st->tree;
+  if (t < 0 || t >= static_cast<int>(J.n_trees) || st->stopped) return;
+  uint32_t ng = 0;  // grown nodes: every level's
+  for (uint32_t d = 0; d <= J.D; ++d) ng = max(ng, J.lvl_begin[d] + J.lvl_count[d]);
+  const bool staged = ng <= kRLeafLds;
+  if (staged)
+    for (uint32_t i = tid; i < ng; i += 256) s_n[i] = rfin_of(J.nodes[i]);
+  __syncthreads();
+  if (tid != 0) return;
+  const size_t o = static_cast<size_t>(t) * J.NC;
+  uint16_t* __restrict__ m_feat = J.m_feat + o;
+  uint8_t* __restrict__ m_bin = J.m_bin + o;
+  uint8_t* __restrict__ m_flags = J.m_flags + o;
+  uint32_t* __restrict__ m_left = J.m_left + o;
+  uint32_t* __restrict__ m_right = J.m_right + o;
+  float* __restrict__ m_value = J.m_value + o;
+  float* __restrict__ m_gain = J.m_gain + o;
+  const RNode* __restrict__ nodes = J.nodes;
+  const uint32_t max_leaves = J.max_leaves, max_depth = J.max_depth, NC = J.NC;
+  const float lam = J.lambda_l2, smooth = J.leaf_smooth, mds = J.max_delta_step;
+  uint32_t next = 0, leaves = 0, depth = 0, qn = 0;
+  auto leaf = [&](uint32_t id, const RFin& X) {  // Tree::make_leaf (tree.cpp:655-681)
+    m_feat[id] = 0;
+    m_bin[id] = 0;
+    m_flags[id] = 3u;  // is_leaf, default_left of a fresh node
+    m_left[id] = 0;
+    m_right[id] = 0;
+    float v = __fdiv_rn(-X.G, __fadd_rn(X.H, lam));
+    if (smooth > 0.0f) {
+      const float c = static_cast<float>(X.C);
+      v = __fmul_rn(v, __fdiv_rn(c, __fadd_rn(c, smooth)));
+    }
+    if (mds > 0.0f) v = fmaxf(-mds, fminf(mds, v));
+    m_value[id] = v;
+    m_gain[id] = 0.f;
+    ++leaves;
+  };
+  if (J.lvl_count[0] > 0) {
+    next = 1;
+    const RFin R = staged ? s_n[0] : rfin_of(nodes[0]);
+    if (R.split) {
+      s_q[0] = RCand{R.gain, 0u, 0u};
+      qn = 1;
+    } else {
+      leaf(0, R);
+    }
+    while (qn > 0 && leaves < max_leaves) {
+      const RCand c = rq_pop(s_q, &qn);
+      const RFin X = staged ? s_n[c.g] : rfin_of(nodes[c.g]);
+      if (X.depth >= max_depth || next + 2 > NC) {  // (the slots fit a depth-max_depth tree)
+        leaf(c.id, X);
+        continue;
+      }
+      const uint32_t l = next, r = next + 1;
+      next += 2;
+      m_feat[c.id] = X.feat;
+      m_bin[c.id] = X.bin;
+      m_flags[c.id] = static_cast<uint8_t>((X.dl & 1u) << 1);
+      m_left[c.id] = l;
+      m_right[c.id] = r;
+      m_value[c.id] = 0.f;
+      m_gain[c.id] = X.gain;
+      depth = max(depth, static_cast<uint32_t>(X.depth) + 1u);
+      for (uint32_t side = 0; side < 2; ++side) {
+        const uint32_t g = X.child + side, id = side ? r : l;
+        const RFin Y = staged ? s_n[g] : rfin_of(nodes[g]);
+        if (Y.split && leaves + 1 < max_leaves && qn < kRHeap) {
+          rq_up(s_q, qn, RCand{Y.gain, g, id});
+          ++qn;
+        } else {
+          leaf(id, Y);
+        }
+      }
+    }
+    while (qn > 0) {
+      const RCand c = rq_pop(s_q, &qn);
+      leaf(c.id, staged ? s_n[c.g] : rfin_of(nodes[c.g]));
+    }
+  }
+  J.m_nodes[t] = next;
+  J.m_leaves[t] = leaves;
+  J.m_depth[t] = static_cast<uint16_t>(depth);
+}
+
 // Tree::predict(dataset, row) (tree.cpp:929-950): the binned walk of tree t
 // of job J for one row, NaN (bin 255) by the node's default_left.
 __device__ __forceinline__ float rtree_row_value(const RJob& J, size_t o, const uint8_t* br) {
@@ -1091,7 +1240,10 @@ struct RSpec {
 
 void check_rtree_params(const tc_rtree_params* p) {
   TC_REQUIRE(p != nullptr, "tree params are NULL");
-  TC_REQUIRE(p->max_depth >= 1 && p->max_depth <= kRMaxDepth, "max_depth must be in [1, 12]");
+  TC_REQUIRE(p->grow_policy == 0 || p->grow_policy == 1, "grow_policy must be 0 or 1");
+  TC_REQUIRE(p->max_depth >= 1 && p->max_depth + (p->grow_policy == 1 ? 1u : 0u) <= kRMaxDepth,
+             p->grow_policy == 1 ? "max_depth must be in [1, 11] for lossguide"
+                                 : "max_depth must be in [1, 12]");
   TC_REQUIRE(p->max_leaves >= 1, "max_leaves must be >= 1");
   TC_REQUIRE((p->max_bins & 0xFFu) >= 2 && p->max_bins <= 255, "max_bins must be in [2, 255]");
   TC_REQUIRE(p->lambda_l2 == p->lambda_l2 && p->min_child_weight == p->min_child_weight &&
@@ -1183,7 +1335,10 @@ class RTrainer {
                          lists_.part[par], lists_.part_count[par], d);
       hipLaunchKernelGGL(k_rplan, dim3(1, nj_), dim3(kRPlanThreads), 0, s, d_jobs_, d, lists_);
     }
-    hipLaunchKernelGGL(k_rfinish, dim3(1, nj_), dim3(256), 0, s, d_jobs_, lists_);
+    if (s_.tree.grow_policy == 1)
+      hipLaunchKernelGGL(k_rleafwise, dim3(1, nj_), dim3(256), 0, s, d_jobs_, lists_);
+    else
+      hipLaunchKernelGGL(k_rfinish, dim3(1, nj_), dim3(256), 0, s, d_jobs_, lists_);
     if (update)
       hipLaunchKernelGGL(k_rupdate, dim3(grad_wg_), dim3(256), 0, s, d_jobs_, nj_, 1, loss_, s_.lr,
                          d_base_, y_, preds_, gh_);
@@ -1409,14 +1564,17 @@ class RTrainer {
     N_ = ds->N;
     F_ = ds->F;
     T_ = feature_tiles(F_);
-    D_ = s_.tree.max_depth;
+    // lossguide: one more level is grown, so that every candidate the
+    // leaf-wise loop may queue (a node at max_depth included) has its split
+    D_ = s_.tree.max_depth + (s_.tree.grow_policy == 1 ? 1u : 0u);
     nj_ = s_.K;
     loss_ = s_.loss;
     TC_REQUIRE(nj_ >= 1 && nj_ <= static_cast<uint32_t>(kMaxClasses), "classes must be in [1, 16]");
     TC_REQUIRE(loss_ == TC_LOSS_SOFTMAX ? nj_ >= 2 : nj_ == 1, "classes do not match the loss");
     TC_REQUIRE(N_ >= 1, "no rows");
     mode_ = loss_ == TC_LOSS_MSE ? 1 : 2;
-    NC_ = (2u << D_) - 1u;
+    NC_ = (2u << s_.tree.max_depth) - 1u;  // node slots of a tree of the model
+    NG_ = (2u << D_) - 1u;                 // grown nodes
     const uint32_t PC = 1u << (D_ - 1);
     grad_wg_ = std::max(1u, ceil_div(N_, kGradTile));
     rng_ = s_.seed;
@@ -1466,6 +1624,8 @@ class RTrainer {
       J.bins_cm = ds->bins_cm;
       J.N = N_; J.F = F_; J.Fp = ds->Fp; J.T = T_;
       J.D = D_;
+      J.leafwise = s_.tree.grow_policy == 1 ? 1u : 0u;
+      J.max_depth = s_.tree.max_depth;
       J.max_leaves = s_.tree.max_leaves;
       J.max_bins = s_.tree.max_bins;
       J.min_samples_leaf = s_.tree.min_samples_leaf;
@@ -1482,7 +1642,7 @@ class RTrainer {
       J.gh = gh_ + static_cast<size_t>(j) * N_;
       J.st = alloc<TreeState>(1);
       hipLaunchKernelGGL(k_init_state, dim3(1), dim3(1), 0, s, J.st);
-      J.nodes = alloc<RNode>(NC_);
+      J.nodes = alloc<RNode>(NG_);
       J.lvl_begin = alloc_zero<uint32_t>(D_ + 2);
       J.lvl_count = alloc_zero<uint32_t>(D_ + 2);
       J.keys = alloc<unsigned long long>(static_cast<size_t>(PC) * F_);
@@ -1560,7 +1720,7 @@ class RTrainer {
 
   tc_ctx* ctx_;
   RSpec s_;
-  uint32_t N_ = 0, F_ = 0, T_ = 0, D_ = 0, nj_ = 1, NC_ = 0, grad_wg_ = 1;
+  uint32_t N_ = 0, F_ = 0, T_ = 0, D_ = 0, nj_ = 1, NC_ = 0, NG_ = 0, grad_wg_ = 1;
   int32_t loss_ = 0;
   int mode_ = 2;
   std::vector<RJob> h_jobs_;

@@ -356,6 +356,9 @@ class ServeCore {
         last_health = now;
         const std::string h = be_->health();
         if (!h.empty()) {
+          // batches the device finished before it failed keep their results:
+          // their done flags may have been set after this iteration's retire
+          retire();
           fail_all(h);
           stop_failed_ = true;
         }

@@ -85,7 +85,7 @@ class RTreeParams(Structure):
                 ("lambda_l2", c_float), ("min_child_weight", c_float),
                 ("min_samples_leaf", c_uint32), ("min_split_gain", c_float), ("gamma", c_float),
                 ("max_delta_step", c_float), ("leaf_smooth", c_float),
-                ("learn_missing_direction", c_int32)]
+                ("learn_missing_direction", c_int32), ("grow_policy", c_int32)]
 
 
 class RTrainParams(Structure):
@@ -104,11 +104,12 @@ class RModel(Structure):
 
 def rtree_params(max_depth=6, max_leaves=64, max_bins=255, lambda_l2=1.0, min_child_weight=1.0,
                  min_samples_leaf=1, min_split_gain=0.0, gamma=0.0, max_delta_step=0.0,
-                 leaf_smooth=0.0, learn_missing_direction=True) -> RTreeParams:
-    """TreeConfig of a regular tree (include/turbocat/config.hpp:26-59 defaults)."""
+                 leaf_smooth=0.0, learn_missing_direction=True, grow_policy=0) -> RTreeParams:
+    """TreeConfig of a regular tree (include/turbocat/config.hpp:21-59 defaults);
+    grow_policy 0 depthwise, 1 lossguide."""
     return RTreeParams(max_depth, max_leaves, max_bins, lambda_l2, min_child_weight,
                        min_samples_leaf, min_split_gain, gamma, max_delta_step, leaf_smooth,
-                       1 if learn_missing_direction else 0)
+                       1 if learn_missing_direction else 0, int(grow_policy))
 
 
 def rtrain_params(loss=TC_LOSS_MSE, n_iterations=100, learning_rate=0.1, subsample=1.0,
