@@ -360,7 +360,10 @@ typedef struct {
  * Booster::train_multiclass (booster.cpp:1263-1400) for TC_LOSS_SOFTMAX: per
  * iteration the softmax gradients of every class, then one regular tree per
  * class on that class's gradients over the iteration's shared row sample.
- * Sampling as tc_train_params (subsample draw, then the colsample draw). */
+ * Sampling as tc_train_params (the row sample's draw, then the colsample
+ * draw): MVS, else GOSS, else the plain row sample (booster.cpp:198-219); a
+ * softmax fit takes GOSS on each row's largest |g| over the classes, else
+ * the plain row sample, and no MVS (booster.cpp:1326-1354). */
 typedef struct {
   int32_t loss;                /* TC_LOSS_MSE, TC_LOSS_LOGLOSS or TC_LOSS_SOFTMAX */
   uint32_t n_iterations;       /* BoostingConfig::n_estimators; trees = n_iterations * K */
@@ -373,6 +376,11 @@ typedef struct {
   int32_t use_graph;           /* 1: one captured hipGraph replay per iteration */
   int32_t time_hist;           /* 1: plain launches with HIP events around every histogram
                                   launch (tc_last_train_timing, tc_last_train_hist_rows) */
+  int32_t use_goss;            /* GOSS row sampling (BoostingConfig, config.hpp:83-85) */
+  float goss_top_rate, goss_other_rate;
+  int32_t use_mvs;             /* MVS row sampling, before GOSS (config.hpp:88-89; binary and
+                                  regression fits only, as in the reference) */
+  float mvs_subsample;
 } tc_rtrain_params;
 
 /* Regular trees in the reference's numbering (Tree::nodes in add_node order;

@@ -110,10 +110,6 @@ def test_not_fitted_and_unsupported_options():
         turbocat.TurboCatClassifier(use_ordered_boosting=True, subsample=1.0).fit(X, y > 0)
     with pytest.raises(ValueError, match="use_gradtree"):
         turbocat.TurboCatClassifier(use_gradtree=True, subsample=1.0).fit(X, y > 0)
-    with pytest.raises(ValueError, match="use_goss with regular trees"):
-        turbocat.TurboCatClassifier(mode="small", use_goss=True, subsample=1.0).fit(X, y > 0)
-    with pytest.raises(ValueError, match="use_goss with regular trees"):  # multiclass: regular trees
-        turbocat.TurboCatClassifier(use_goss=True, subsample=1.0).fit(X, np.arange(50) % 3)
     with pytest.raises(ValueError, match="loss='huber'"):
         turbocat.TurboCatRegressor(loss="huber").fit(X, y)
     with pytest.raises(ValueError, match="goss_top_rate"):

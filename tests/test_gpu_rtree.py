@@ -70,39 +70,48 @@ def term_scales(ref, bins, g, h, lam, rows=None):
 
 
 def compare_rtree(gpu, ref, g, h, p, bins=None, rows=None):
-    """Node by node in id order.  Returns (nodes compared, near-tie hit)."""
+    """The two trees walked together from the root, left subtree first (the
+    order the reference's depthwise recursion numbers and caps nodes in), so
+    the first difference met is the first one the build made; then, with no
+    near-tie met, the numbering itself.  Returns (nodes compared, near-tie
+    hit)."""
     vscale = float(np.abs(g.astype(np.float64)).sum()) + 1.0
     sc = term_scales(ref, bins, g, h, p.lambda_l2, rows) if bins is not None else \
         np.zeros(ref["n_nodes"])
-    n = min(gpu["n_nodes"], ref["n_nodes"])
-    for i in range(n):
-        gl, rl = int(gpu["flags"][i]) & 1, int(ref["flags"][i]) & 1
-        eps = NEAR_TIE * max(1.0, abs(ref["gain"][i])) + 1e-6 * sc[i]
-        if not rl and ref["gap"][i] <= eps:
-            if gl or (gpu["feature"][i], gpu["bin"][i]) != (ref["feature"][i], ref["bin"][i]):
-                return i, True
+    stack, n = ([(0, 0)] if min(gpu["n_nodes"], ref["n_nodes"]) else []), 0
+    while stack:
+        i, j = stack.pop()
+        n += 1
+        gl, rl = int(gpu["flags"][i]) & 1, int(ref["flags"][j]) & 1
+        eps = NEAR_TIE * max(1.0, abs(ref["gain"][j])) + 1e-6 * sc[j]
+        if not rl and ref["gap"][j] <= eps:
+            if gl or (gpu["feature"][i], gpu["bin"][i]) != (ref["feature"][j], ref["bin"][j]):
+                return n, True
         if gl != rl:
             # a leaf on one side only: the decision must be borderline (the
             # best gain at min_split_gain)
-            gain = float(ref["gain"][i]) if not rl else float(gpu["gain"][i])
-            assert abs(gain - p.min_split_gain) <= 1e-4 * max(1.0, abs(gain)) + 1e-6 * sc[i], (
-                f"node {i}: leaf on one side only (gpu leaf {gl}, ref leaf {rl}, gain {gain})")
-            return i, True
-        assert int(gpu["flags"][i]) == int(ref["flags"][i]), f"node {i}: flags"
+            gain = float(ref["gain"][j]) if not rl else float(gpu["gain"][i])
+            assert abs(gain - p.min_split_gain) <= 1e-4 * max(1.0, abs(gain)) + 1e-6 * sc[j], (
+                f"node {j}: leaf on one side only (gpu leaf {gl}, ref leaf {rl}, gain {gain})")
+            return n, True
+        assert int(gpu["flags"][i]) == int(ref["flags"][j]), f"node {j}: flags"
         if rl:
-            v_ref = float(ref["value"][i])
+            v_ref = float(ref["value"][j])
             assert abs(float(gpu["value"][i]) - v_ref) <= 1e-5 * max(1.0, abs(v_ref)) + \
-                1e-6 * vscale / (p.lambda_l2 + 1e-3), (f"node {i}: leaf {gpu['value'][i]} vs {v_ref}")
-        else:
-            assert (int(gpu["feature"][i]), int(gpu["bin"][i])) == \
-                (int(ref["feature"][i]), int(ref["bin"][i])), (
-                    f"node {i}: gpu ({gpu['feature'][i]},{gpu['bin'][i]}) ref "
-                    f"({ref['feature'][i]},{ref['bin'][i]}) gap {ref['gap'][i]}")
-            assert (int(gpu["left"][i]), int(gpu["right"][i])) == \
-                (int(ref["left"][i]), int(ref["right"][i])), f"node {i}: children"
-            assert abs(float(gpu["gain"][i]) - float(ref["gain"][i])) <= \
-                1e-4 * max(1.0, abs(float(ref["gain"][i]))) + 1e-6 * sc[i], f"node {i}: gain"
+                1e-6 * vscale / (p.lambda_l2 + 1e-3), (f"node {j}: leaf {gpu['value'][i]} vs {v_ref}")
+            continue
+        assert (int(gpu["feature"][i]), int(gpu["bin"][i])) == \
+            (int(ref["feature"][j]), int(ref["bin"][j])), (
+                f"node {j}: gpu ({gpu['feature'][i]},{gpu['bin'][i]}) ref "
+                f"({ref['feature'][j]},{ref['bin'][j]}) gap {ref['gap'][j]}")
+        assert abs(float(gpu["gain"][i]) - float(ref["gain"][j])) <= \
+            1e-4 * max(1.0, abs(float(ref["gain"][j]))) + 1e-6 * sc[j], f"node {j}: gain"
+        stack.append((int(gpu["right"][i]), int(ref["right"][j])))
+        stack.append((int(gpu["left"][i]), int(ref["left"][j])))
     assert gpu["n_nodes"] == ref["n_nodes"], (gpu["n_nodes"], ref["n_nodes"])
+    m = int(ref["n_nodes"])
+    for key in ("left", "right", "flags"):
+        assert np.array_equal(np.asarray(gpu[key])[:m], np.asarray(ref[key])[:m]), key
     assert gpu["n_leaves"] == ref["n_leaves"] and gpu["depth"] == ref["depth"]
     return n, False
 
@@ -279,6 +288,99 @@ def test_rtrain_multiclass_matches_oracle(ctx, oracle):
         preds = nxt
     assert ties <= 1
     assert np.array_equal(m["train_preds"].T, preds)
+
+
+@pytest.mark.parametrize("kind", ["goss", "mvs"])
+def test_rtrain_goss_mvs_matches_oracle(ctx, oracle, kind):
+    """Booster::train's regular branch with GOSS or MVS (booster.cpp:198-219):
+    each tree grows on the rows the reference's selection draws from the
+    iteration's gradients (the device's gradients are the oracle's bit for
+    bit, so capi.goss_subsample / mvs_subsample on the oracle's give the
+    same rows), then the colsample draw; every tree teacher-forced against
+    the oracle on those rows, and the replay equals the GPU's training
+    predictions."""
+    from oracle import LOSS_LOGLOSS
+    from turbocat import capi
+    X, yr = data_with_nan(8000, 8, 13, nan_frac=0.01)
+    y = (yr > 0).astype(np.float32)
+    ds, _, _ = ctx.quantize_fit(X)
+    bins = ds.bins()
+    p = capi.rtree_params(max_depth=6, max_leaves=24, lambda_l2=1.0)
+    lr, top, other, ratio = 0.3, 0.2, 0.2, 0.5
+    tp = capi.rtrain_params(capi.TC_LOSS_LOGLOSS, n_iterations=8, learning_rate=lr, seed=9,
+                            colsample_bytree=0.8, use_goss=kind == "goss",
+                            goss_top_rate=top, goss_other_rate=other, use_mvs=kind == "mvs",
+                            mvs_subsample=ratio)
+    tp.tree = p
+    m = ds.rtrain(y, tp)
+    state = 9
+    preds = np.full(len(y), np.float32(m["base"]), np.float32)
+    ties = 0
+    for tree in m["trees"]:
+        g, h = oracle.gradients_f32(LOSS_LOGLOSS, y, preds)
+        seed, state = capi.next_random(state)
+        rows = capi.goss_subsample(g, top, other, seed) if kind == "goss" else \
+            capi.mvs_subsample(g, ratio, seed)
+        _, state = capi.next_random(state)  # colsample: drawn, unused
+        _, tie = compare_rtree(tree, oracle.rtree_f64(bins, g, h, oparams(p), rows), g, h, p, bins,
+                               rows)
+        ties += tie
+        preds = oracle.fma_update(preds, np.float32(lr), oracle.rtree_apply_binned(tree, bins))
+    assert ties <= 1
+    assert np.array_equal(m["train_preds"][0], preds)
+
+
+def test_rtrain_multiclass_goss_matches_oracle(ctx, oracle):
+    """The multiclass loop with GOSS (booster.cpp:1326-1347): one sample per
+    iteration, selected on each row's largest |g| over the classes, shared by
+    the K class trees; each tree against the oracle's on those rows."""
+    from turbocat import capi
+    K = 3
+    X, y = multiclass_data(6000, 10, K, 23)
+    ds, _, _ = ctx.quantize_fit(X)
+    bins = ds.bins()
+    p = capi.rtree_params(max_depth=5, max_leaves=20, lambda_l2=1.0)
+    lr, top, other = 0.2, 0.3, 0.2
+    tp = capi.rtrain_params(capi.TC_LOSS_SOFTMAX, n_iterations=4, learning_rate=lr, n_classes=K,
+                            seed=17, use_goss=True, goss_top_rate=top, goss_other_rate=other,
+                            use_mvs=True)  # MVS is not a multiclass option: GOSS is used
+    tp.tree = p
+    m = ds.rtrain(y, tp)
+    state = 17
+    preds = np.zeros((len(y), K), np.float32)
+    ties = 0
+    for it in range(4):
+        g, h = oracle.gradients_mc_f32(y, preds)
+        gsel = np.abs(g).max(axis=1).astype(np.float32)
+        seed, state = capi.next_random(state)
+        rows = capi.goss_subsample(gsel, top, other, seed)
+        nxt = preds.copy()
+        for k in range(K):
+            tree = m["trees"][it * K + k]
+            gk = np.ascontiguousarray(g[:, k]); hk = np.ascontiguousarray(h[:, k])
+            ref = oracle.rtree_f64(bins, gk, hk, oparams(p), rows)
+            _, tie = compare_rtree(tree, ref, gk, hk, p, bins, rows)
+            ties += tie
+            nxt[:, k] = oracle.fma_update(preds[:, k].copy(), np.float32(lr),
+                                          oracle.rtree_apply_binned(tree, bins))
+        preds = nxt
+    assert ties <= 1
+    assert np.array_equal(m["train_preds"].T, preds)
+
+
+def test_dropin_regular_goss(ctx):
+    """use_goss with mode="small" and with more than two classes trains
+    (the reference samples regular trees' rows the same way)."""
+    from turbocat import TurboCatClassifier
+    X, y = multiclass_data(4000, 8, 3, 29)
+    clf = TurboCatClassifier(n_estimators=15, mode="small", use_goss=True, subsample=1.0,
+                             colsample_bytree=1.0, verbosity=0)
+    clf.fit(X, (y == 1).astype(np.float32))
+    assert clf.predict_proba(X).shape == (4000, 2)
+    mc = TurboCatClassifier(n_estimators=10, use_goss=True, subsample=1.0, colsample_bytree=1.0,
+                            verbosity=0)
+    mc.fit(X, y)
+    assert (mc.predict(X) == y).mean() > 0.6
 
 
 @pytest.mark.parametrize("kind", ["binary", "multiclass"])

@@ -194,11 +194,16 @@ def test_injected_batch_failure_fails_waits_and_stop_returns(lib):
 def test_dead_device_fails_every_unfinished_request(lib):
     F = 2
     reqs = requests([25] * 10, F, 6)
-    srv = Fake(lib, F=F, max_batch_rows=25, budget=1 << 20, n_slots=3, device_fail_after=3)
+    # the device is held until every request is in: once it fails the server
+    # stops itself and refuses later submissions
+    srv = Fake(lib, F=F, max_batch_rows=25, budget=1 << 20, n_slots=3, device_fail_after=3,
+               hold=True)
     outs = [np.zeros((25, 1), np.float32) for _ in reqs]
-    ts = [srv.submit(x, o)[1] for x, o in zip(reqs, outs)]
-    rcs = [srv.wait(t, 30_000_000) for t in ts]
-    assert rcs[:2] == [OK, OK] and all(rc == EDEVICE for rc in rcs[2:])
+    sub = [srv.submit(x, o) for x, o in zip(reqs, outs)]
+    assert all(rc == OK for rc, _ in sub)
+    lib.fake_hold(srv.h, 0)
+    rcs = [srv.wait(t, 30_000_000) for _, t in sub]
+    assert rcs[:2] == [OK, OK] and all(rc == EDEVICE for rc in rcs[2:]), rcs
     assert b"fake device failed" in lib.fake_error(srv.h)
     assert srv.counters()["failed"] == 8
     srv.stop()

@@ -10,11 +10,11 @@
 // the mode, and predict through those trees.  Symmetric trees ignore
 // grow_policy, as in the reference.  Options whose reference code path lies
 // outside these (GradTree, ordered boosting, categorical features,
-// interactions, early-stopping metrics other than the loss, other losses,
-// GOSS/MVS with regular trees) raise ValueError naming the option instead of
-// silently training something else.  A validation set early-stops
-// on its loss (booster.cpp:255-371, :1400-1430); use_goss samples each
-// symmetric tree's rows by GOSS (booster.cpp:207-212).
+// interactions, early-stopping metrics other than the loss, other losses)
+// raise ValueError naming the option instead of silently training something
+// else.  use_goss samples each tree's rows by GOSS, symmetric and regular
+// trees alike (booster.cpp:198-219, :1326-1354).  A validation set early-stops
+// on its loss (booster.cpp:255-371, :1400-1430).
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
@@ -140,9 +140,7 @@ class TurboCatClassifier {
       p.task = tcamd::kMulticlass;
       p.loss = TC_LOSS_SOFTMAX;
     }
-    const bool regular = !p.use_symmetric || n_classes_ > 2;
     p.lossguide = lossguide(grow_policy_);  // read by regular trees only
-    reject(regular && p.use_goss, "use_goss with regular trees");
     booster_ = std::make_unique<Booster>(p);
     if (p_.verbosity > 0)
       std::printf("[TurboCat/MI355X] fit: n_samples=%u, n_features=%u, n_estimators=%u\n", m.n, m.F,
@@ -286,7 +284,6 @@ class TurboCatRegressor {
     if (static_cast<uint32_t>(y.shape(0)) != m.n) throw std::invalid_argument("X and y differ in rows");
     const Matrix v = validation(X_val, y_val, m.F);
     reject(!(loss_ == "mse" || loss_ == "l2"), "loss='" + loss_ + "'");
-    reject(!p_.use_symmetric && p_.use_goss, "use_goss with regular trees");
     Params p = p_;
     p.lossguide = lossguide(grow_policy_);  // read by regular trees only
     booster_ = std::make_unique<Booster>(p);

@@ -187,6 +187,11 @@ void Booster::fit_regular(const float* X, const float* y, uint32_t n, uint32_t F
   tp.tree.grow_policy = p_.lossguide ? 1 : 0;
   tp.subsample = p_.subsample;
   tp.colsample_bytree = p_.colsample_bytree;
+  tp.use_goss = p_.use_goss ? 1 : 0;
+  tp.goss_top_rate = p_.goss_top_rate;
+  tp.goss_other_rate = p_.goss_other_rate;
+  tp.use_mvs = p_.use_mvs ? 1 : 0;
+  tp.mvs_subsample = p_.mvs_subsample;
   tp.seed = p_.seed;
   tp.n_classes = multi ? K : 0u;
   tp.use_graph = p_.use_graph ? 1 : 0;

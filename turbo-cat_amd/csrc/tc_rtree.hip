@@ -1231,6 +1231,8 @@ struct RSpec {
   float lr = 0.1f;
   tc_rtree_params tree{};
   float subsample = 1.f, colsample = 1.f;
+  bool use_goss = false, use_mvs = false;  // MVS, else GOSS, else the plain row sample
+  float goss_top = 0.2f, goss_other = 0.1f, mvs_ratio = 0.5f;
   uint64_t seed = 42;
   float base = 0.f;
   const tc_dataset* valid = nullptr;  // validation rows, binned with the training edges
@@ -1306,7 +1308,7 @@ class RTrainer {
   // hev (optional) receives an event pair around every histogram launch.
   void enqueue_iteration(bool update, std::vector<std::pair<hipEvent_t, hipEvent_t>>* hev = nullptr) {
     hipStream_t s = ctx_->stream;
-    if (sampling_)  // the root rows of the iteration's sample
+    if (sampling_ && !selective_)  // the root rows of the iteration's plain sample
       hipLaunchKernelGGL(k_rows_from_mask_one, dim3(AheadSampler::mask_chunks(N_)),
                          dim3(kMaskChunkWords), 0, s, mask_, N_, n_sample_, sample_);
     hipLaunchKernelGGL(k_rbegin, dim3(1, nj_), dim3(256), 0, s, d_jobs_, lists_);
@@ -1546,16 +1548,48 @@ class RTrainer {
     }
   }
 
-  // The next iteration's row sample (Dataset::random_subsample, tree.cpp:
-  // 186-191), drawn ahead on host threads and copied as a bitmask; the
-  // iteration's first launch makes the root rows from it.
+  // The next iteration's row sample.  A plain one (Dataset::random_subsample)
+  // is drawn ahead on host threads and copied as a bitmask that the
+  // iteration's first launch turns into the root rows.  GOSS and MVS select
+  // on the gradients the iteration's trees grow on (every queued iteration
+  // first): Dataset::goss_subsample on |g| -- a softmax fit's largest |g|
+  // over its classes (booster.cpp:1326-1347) -- or Dataset::mvs_subsample;
+  // the rows and their count are copied to the device.
   void prepare_sample() {
     if (!sampling_ && !colsample_) return;
-    if (sampling_) {
+    if (sampling_ && !selective_) {
       plain_.next(mask_, ctx_->stream);
       return;
     }
-    (void)next_random(rng_);  // feature subset: drawn, unused (tree.cpp:193-196)
+    uint64_t seed = 0;
+    if (sampling_) seed = next_random(rng_);
+    if (colsample_) (void)next_random(rng_);  // feature subset: drawn, unused (tree.cpp:193-196)
+    if (!sampling_) return;
+    hipStream_t s = ctx_->stream;
+    const int b = next_;
+    next_ ^= 1;
+    TC_HIP(hipEventSynchronize(sev_[b]));
+    TC_HIP(hipMemcpyAsync(gh_host_, gh_, sizeof(float2) * N_ * nj_, hipMemcpyDeviceToHost, s));
+    TC_HIP(hipStreamSynchronize(s));
+    for (uint32_t i = 0; i < N_; ++i) {
+      float m = 0.0f;
+      if (nj_ == 1) {
+        m = gh_host_[i].x;
+      } else {
+        for (uint32_t c = 0; c < nj_; ++c)
+          m = std::max(m, std::abs(gh_host_[static_cast<size_t>(c) * N_ + i].x));
+      }
+      gsel_[i] = m;
+    }
+    const uint32_t n = s_.use_mvs
+                           ? mvs_subsample(gsel_.data(), N_, s_.mvs_ratio, seed, shost_[b])
+                           : goss_subsample(gsel_.data(), N_, s_.goss_top, s_.goss_other, seed, shost_[b]);
+    nroot_host_[b] = n;
+    TC_HIP(hipMemcpyAsync(sample_, shost_[b], 4ull * std::max(1u, n), hipMemcpyHostToDevice, s));
+    for (uint32_t j = 0; j < nj_; ++j)
+      TC_HIP(hipMemcpyAsync(reinterpret_cast<char*>(d_jobs_ + j) + offsetof(RJob, n_root),
+                            &nroot_host_[b], sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    TC_HIP(hipEventRecord(sev_[b], s));
   }
 
   void setup() {
@@ -1578,9 +1612,15 @@ class RTrainer {
     const uint32_t PC = 1u << (D_ - 1);
     grad_wg_ = std::max(1u, ceil_div(N_, kGradTile));
     rng_ = s_.seed;
-    sampling_ = s_.subsample > 0.f && s_.subsample < 1.f;
+    selective_ = s_.use_goss || s_.use_mvs;
+    sampling_ = selective_ || (s_.subsample > 0.f && s_.subsample < 1.f);
     colsample_ = s_.colsample > 0.f && s_.colsample < 1.f;
-    n_sample_ = sampling_ ? std::min(N_, static_cast<uint32_t>(static_cast<float>(N_) * s_.subsample)) : N_;
+    // the root's grids hold n_sample_ rows: GOSS's fixed count, every row for
+    // MVS (whose sample size the device reads from the job record)
+    n_sample_ = s_.use_mvs    ? N_
+                : s_.use_goss ? std::min(N_, goss_rows(N_, s_.goss_top, s_.goss_other))
+                : sampling_   ? std::min(N_, static_cast<uint32_t>(static_cast<float>(N_) * s_.subsample))
+                              : N_;
     y_ = alloc_zero<float>(N_);
     if (s_.y) TC_HIP(hipMemcpyAsync(y_, s_.y, 4ull * N_, hipMemcpyHostToDevice, s));
     preds_ = alloc_zero<float>(static_cast<size_t>(N_) * nj_);
@@ -1591,9 +1631,17 @@ class RTrainer {
     sample_ = alloc<uint32_t>(N_);
     rows0_dummy_ = alloc<uint32_t>(1);
     hist_rows_ = alloc_zero<unsigned long long>(1);
-    if (sampling_) {
+    if (sampling_ && !selective_) {
       plain_.init(N_, s_.subsample, rng_, colsample_, ahead_ring(N_, 1));
       mask_ = alloc<uint32_t>(AheadSampler::mask_words(N_));
+    } else if (selective_) {
+      TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&gh_host_), sizeof(float2) * N_ * nj_,
+                           hipHostMallocDefault));
+      for (int b = 0; b < 2; ++b) {
+        TC_HIP(hipHostMalloc(reinterpret_cast<void**>(&shost_[b]), 4ull * N_, hipHostMallocDefault));
+        TC_HIP(hipEventCreateWithFlags(&sev_[b], hipEventDisableTiming));
+      }
+      gsel_.resize(N_);
     }
     if (s_.valid) {  // validation rows ride after each iteration's update
       TC_REQUIRE(s_.valid->F == F_, "validation set: feature count differs from training");
@@ -1716,6 +1764,14 @@ class RTrainer {
     if (stop_ev_) (void)hipEventDestroy(stop_ev_);
     stop_ev_ = nullptr;
     plain_.release();  // waits for its draws
+    for (int b = 0; b < 2; ++b) {
+      if (shost_[b]) hfree(shost_[b]);
+      if (sev_[b]) (void)hipEventDestroy(sev_[b]);
+      shost_[b] = nullptr;
+      sev_[b] = nullptr;
+    }
+    if (gh_host_) hfree(gh_host_);
+    gh_host_ = nullptr;
   }
 
   tc_ctx* ctx_;
@@ -1755,6 +1811,14 @@ class RTrainer {
   uint64_t rng_ = 42;
   AheadSampler plain_;        // the row samples, drawn ahead
   uint32_t* mask_ = nullptr;  // the iteration's sample as a bitmask (device)
+  // GOSS / MVS: the selection on each iteration's gradients (host)
+  bool selective_ = false;
+  float2* gh_host_ = nullptr;           // pinned copy of every class's (g, h)
+  std::vector<float> gsel_;             // the gradient the selection reads per row
+  uint32_t* shost_[2] = {nullptr, nullptr};  // pinned row samples, alternating
+  hipEvent_t sev_[2] = {nullptr, nullptr};
+  uint32_t nroot_host_[2] = {0, 0};    // each buffer's row count, the source of its copy
+  int next_ = 0;
 };
 
 }  // namespace
@@ -1946,6 +2010,19 @@ int32_t tc_rtrain_valid(tc_dataset* ds, const float* y, const tc_rtrain_params* 
     s.tree = p->tree;
     s.subsample = p->subsample;
     s.colsample = p->colsample_bytree;
+    // booster.cpp:198-219 (MVS, else GOSS); the multiclass loop takes GOSS or
+    // the plain sample only (:1326-1354)
+    s.use_mvs = p->use_mvs != 0 && p->loss != TC_LOSS_SOFTMAX;
+    s.use_goss = p->use_goss != 0 && !s.use_mvs;
+    s.goss_top = p->goss_top_rate;
+    s.goss_other = p->goss_other_rate;
+    s.mvs_ratio = p->mvs_subsample;
+    if (s.use_goss)
+      TC_REQUIRE(p->goss_top_rate >= 0.f && p->goss_other_rate >= 0.f &&
+                     p->goss_top_rate + p->goss_other_rate <= 1.f,
+                 "goss rates must be >= 0 with top + other <= 1");
+    if (s.use_mvs) TC_REQUIRE(p->mvs_subsample > 0.f && p->mvs_subsample <= 1.f,
+                              "mvs_subsample must be in (0, 1]");
     s.seed = p->seed;
     s.base = init_prediction_host(p->loss, y, ds->N);
     s.valid = valid;

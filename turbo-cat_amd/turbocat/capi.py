@@ -92,7 +92,8 @@ class RTrainParams(Structure):
     _fields_ = [("loss", c_int32), ("n_iterations", c_uint32), ("learning_rate", c_float),
                 ("tree", RTreeParams), ("subsample", c_float), ("colsample_bytree", c_float),
                 ("seed", c_uint64), ("n_classes", c_uint32), ("use_graph", c_int32),
-                ("time_hist", c_int32)]
+                ("time_hist", c_int32), ("use_goss", c_int32), ("goss_top_rate", c_float),
+                ("goss_other_rate", c_float), ("use_mvs", c_int32), ("mvs_subsample", c_float)]
 
 
 class RModel(Structure):
@@ -114,9 +115,12 @@ def rtree_params(max_depth=6, max_leaves=64, max_bins=255, lambda_l2=1.0, min_ch
 
 def rtrain_params(loss=TC_LOSS_MSE, n_iterations=100, learning_rate=0.1, subsample=1.0,
                   colsample_bytree=1.0, seed=42, n_classes=0, use_graph=1, time_hist=0,
-                  **tree) -> RTrainParams:
+                  use_goss=False, goss_top_rate=0.2, goss_other_rate=0.1, use_mvs=False,
+                  mvs_subsample=0.5, **tree) -> RTrainParams:
     return RTrainParams(loss, n_iterations, learning_rate, rtree_params(**tree), subsample,
-                        colsample_bytree, seed, n_classes, use_graph, time_hist)
+                        colsample_bytree, seed, n_classes, use_graph, time_hist,
+                        1 if use_goss else 0, goss_top_rate, goss_other_rate, 1 if use_mvs else 0,
+                        mvs_subsample)
 
 
 def _new_rmodel(n_trees, max_depth, n=0, K=1):
