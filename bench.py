@@ -570,9 +570,9 @@ def run_regular(args):
     ds, edges, ne = ctx.quantize_fit(X)
     quant_s = time.perf_counter() - t0
 
-    if args.sampling in ("goss", "mvs"):
-        raise SystemExit("regular trees take a plain row sample only (--sampling random)")
-    smp = dict(subsample=0.8) if args.sampling == "random" else {}
+    smp = dict(random=dict(subsample=0.8),
+               goss=dict(use_goss=True, goss_top_rate=0.2, goss_other_rate=0.1),
+               mvs=dict(use_mvs=True, mvs_subsample=0.5)).get(args.sampling, {})
 
     def fit(iters, time_hist):
         prm = capi.rtrain_params(loss, n_iterations=iters, learning_rate=lr,
@@ -725,8 +725,8 @@ def main():
                     help="train workload: binary logloss (default) or the MSE regression set")
     ap.add_argument("--sampling", default="none", choices=["none", "random", "goss", "mvs"],
                     help="train and regular workloads: row sampling per tree (random 0.8, GOSS "
-                         "top 0.2 / other 0.1, MVS 0.5; regular trees: random only; drawn on the "
-                         "host with the reference's routines)")
+                         "top 0.2 / other 0.1, MVS 0.5; drawn on the host with the reference's "
+                         "routines)")
     ap.add_argument("--grow-policy", default="depthwise", choices=["depthwise", "lossguide"],
                     help="regular workload: how the trees grow (TreeConfig::grow_policy)")
     ap.add_argument("--no-cpu", action="store_true")

@@ -1562,8 +1562,13 @@ class RTrainer {
       return;
     }
     uint64_t seed = 0;
-    if (sampling_) seed = next_random(rng_);
-    if (colsample_) (void)next_random(rng_);  // feature subset: drawn, unused (tree.cpp:193-196)
+    const MvsDraws* md = nullptr;
+    if (sampling_ && s_.use_mvs) {  // drawn ahead: the seed and the feature-subset draw are taken there
+      md = &mvs_ahead_.next();
+    } else {
+      if (sampling_) seed = next_random(rng_);
+      if (colsample_) (void)next_random(rng_);  // feature subset: drawn, unused (tree.cpp:193-196)
+    }
     if (!sampling_) return;
     hipStream_t s = ctx_->stream;
     const int b = next_;
@@ -1582,8 +1587,9 @@ class RTrainer {
       gsel_[i] = m;
     }
     const uint32_t n = s_.use_mvs
-                           ? mvs_subsample(gsel_.data(), N_, s_.mvs_ratio, seed, shost_[b])
-                           : goss_subsample(gsel_.data(), N_, s_.goss_top, s_.goss_other, seed, shost_[b]);
+                           ? mvs_select(gsel_.data(), N_, s_.mvs_ratio, *md, shost_[b], mvs_scratch_)
+                           : goss_select(gsel_.data(), N_, s_.goss_top, s_.goss_other, seed, shost_[b],
+                                         goss_pairs_);
     nroot_host_[b] = n;
     TC_HIP(hipMemcpyAsync(sample_, shost_[b], 4ull * std::max(1u, n), hipMemcpyHostToDevice, s));
     for (uint32_t j = 0; j < nj_; ++j)
@@ -1642,6 +1648,7 @@ class RTrainer {
         TC_HIP(hipEventCreateWithFlags(&sev_[b], hipEventDisableTiming));
       }
       gsel_.resize(N_);
+      if (s_.use_mvs) mvs_ahead_.init(N_, rng_, colsample_, ahead_ring(N_, 1));
     }
     if (s_.valid) {  // validation rows ride after each iteration's update
       TC_REQUIRE(s_.valid->F == F_, "validation set: feature count differs from training");
@@ -1764,6 +1771,7 @@ class RTrainer {
     if (stop_ev_) (void)hipEventDestroy(stop_ev_);
     stop_ev_ = nullptr;
     plain_.release();  // waits for its draws
+    mvs_ahead_.release();
     for (int b = 0; b < 2; ++b) {
       if (shost_[b]) hfree(shost_[b]);
       if (sev_[b]) (void)hipEventDestroy(sev_[b]);
@@ -1815,6 +1823,9 @@ class RTrainer {
   bool selective_ = false;
   float2* gh_host_ = nullptr;           // pinned copy of every class's (g, h)
   std::vector<float> gsel_;             // the gradient the selection reads per row
+  MvsAhead mvs_ahead_;                  // MVS: the uniform draws, made ahead
+  MvsScratch mvs_scratch_;
+  std::vector<std::pair<float, uint32_t>> goss_pairs_;  // GOSS: kept across iterations
   uint32_t* shost_[2] = {nullptr, nullptr};  // pinned row samples, alternating
   hipEvent_t sev_[2] = {nullptr, nullptr};
   uint32_t nroot_host_[2] = {0, 0};    // each buffer's row count, the source of its copy

@@ -240,12 +240,13 @@ inline size_t ahead_ring(uint32_t N, uint32_t jobs) {
 // then min(uint32(n * other_rate), n - top) rows of the rest by a partial
 // Fisher-Yates draw from std::mt19937_64.  The same library calls on the same
 // array in the same order as the reference: the same rows in the same order.
-uint32_t goss_subsample(const float* g, uint32_t n, float top_rate, float other_rate,
-                        uint64_t seed, uint32_t* out) {
+// gi: the (|g|, row) pairs' storage, kept across trees.
+uint32_t goss_select(const float* g, uint32_t n, float top_rate, float other_rate, uint64_t seed,
+                     uint32_t* out, std::vector<std::pair<float, uint32_t>>& gi) {
   const uint32_t top_n = static_cast<uint32_t>(static_cast<float>(n) * top_rate);
   uint32_t other_n = static_cast<uint32_t>(static_cast<float>(n) * other_rate);
   if (top_n == 0 && other_n == 0) return 0;
-  std::vector<std::pair<float, uint32_t>> gi(n);
+  gi.resize(n);
   for (uint32_t i = 0; i < n; ++i) gi[i] = {std::abs(g[i]), i};
   if (top_n > 0 && top_n < n)
     std::nth_element(gi.begin(), gi.begin() + top_n, gi.end(),
@@ -268,6 +269,12 @@ uint32_t goss_subsample(const float* g, uint32_t n, float top_rate, float other_
   return k;
 }
 
+uint32_t goss_subsample(const float* g, uint32_t n, float top_rate, float other_rate,
+                        uint64_t seed, uint32_t* out) {
+  std::vector<std::pair<float, uint32_t>> gi;
+  return goss_select(g, n, top_rate, other_rate, seed, out, gi);
+}
+
 // Rows goss_subsample returns for n rows (independent of the gradients).
 uint32_t goss_rows(uint32_t n, float top_rate, float other_rate) {
   const uint32_t top_n = static_cast<uint32_t>(static_cast<float>(n) * top_rate);
@@ -288,20 +295,64 @@ uint32_t goss_rows(uint32_t n, float top_rate, float other_rate) {
 // the same order give the same rows in the same order.  S is summed in row
 // order: the reference's own OpenMP reduction adds per-thread partial sums,
 // which equals this order on one thread.
-uint32_t mvs_subsample(const float* g, uint32_t n, float ratio, uint64_t seed, uint32_t* out) {
+//
+// The draws depend on the seed alone (one per row, in row order), so they are
+// made apart from the selection (MvsDraws), ahead of the tree that uses them
+// (MvsAhead); the rank's value is the one std::nth_element places there,
+// found here by a radix select (the value of a rank does not depend on how
+// it is found).
+struct MvsDraws {
+  std::vector<float> u;  // row i's uniform draw
+  std::mt19937_64 after;  // the engine once the n draws are made
+};
+
+inline void mvs_draw(uint64_t seed, uint32_t n, MvsDraws* d) {
+  d->u.resize(n);
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<float> uniform(0.0f, 1.0f);
+  for (uint32_t i = 0; i < n; ++i) d->u[i] = uniform(rng);
+  d->after = rng;
+}
+
+// The value of ascending rank k (0-based) among n >= k + 1 non-negative
+// floats: two 16-bit counting passes over their bit patterns, which order as
+// the values do.
+inline float kth_nonneg(const float* a, uint32_t n, uint32_t k, std::vector<uint32_t>& cnt) {
+  cnt.assign(65536, 0u);
+  for (uint32_t i = 0; i < n; ++i) ++cnt[__builtin_bit_cast(uint32_t, a[i]) >> 16];
+  uint32_t hi = 0, below = 0;
+  while (below + cnt[hi] <= k) below += cnt[hi++];
+  cnt.assign(65536, 0u);
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t b = __builtin_bit_cast(uint32_t, a[i]);
+    if ((b >> 16) == hi) ++cnt[b & 0xFFFFu];
+  }
+  uint32_t lo = 0;
+  while (below + cnt[lo] <= k) below += cnt[lo++];
+  return __builtin_bit_cast(float, (hi << 16) | lo);
+}
+
+struct MvsScratch {
+  std::vector<float> a, sc;
+  std::vector<uint32_t> cnt;
+};
+
+uint32_t mvs_select(const float* g, uint32_t n, float ratio, const MvsDraws& d, uint32_t* out,
+                    MvsScratch& s) {
   if (n == 0 || !(ratio > 0.0f)) return 0;
   const uint32_t target = std::max(static_cast<uint32_t>(static_cast<float>(n) * ratio), 1u);
-  std::vector<float> a(n);
+  s.a.resize(n);
+  s.sc.resize(n);
+  float* a = s.a.data();
+  float* sc = s.sc.data();
   float gmax = 0.0f;
   for (uint32_t i = 0; i < n; ++i) {
     a[i] = std::abs(g[i]);
     gmax = std::max(gmax, a[i]);
   }
-  std::vector<float> sc(a);
   const uint32_t ti =
       std::min(static_cast<uint32_t>((1.0f - ratio) * static_cast<float>(n)), n - 1);
-  std::nth_element(sc.begin(), sc.begin() + ti, sc.end());
-  float thr = std::max(sc[ti], gmax * 0.01f);
+  float thr = std::max(kth_nonneg(a, n, ti, s.cnt), gmax * 0.01f);
   if (thr < 1e-10f) thr = 1e-10f;
   float ssum = 0.0f;
   for (uint32_t i = 0; i < n; ++i) {
@@ -310,14 +361,14 @@ uint32_t mvs_subsample(const float* g, uint32_t n, float ratio, uint64_t seed, u
   }
   const float fn = static_cast<float>(n);
   const float scale = static_cast<float>(target) / ssum * ssum / fn;
-  std::mt19937_64 rng(seed);
-  std::uniform_real_distribution<float> uniform(0.0f, 1.0f);
+  const float* u = d.u.data();
   uint32_t k = 0;
   for (uint32_t i = 0; i < n; ++i) {
     const float prob = std::min(1.0f, scale * sc[i] / ssum * fn);
-    if (uniform(rng) < prob) out[k++] = i;
+    if (u[i] < prob) out[k++] = i;
   }
   if (k < static_cast<uint32_t>(static_cast<float>(target) * 0.8f)) {
+    std::mt19937_64 rng = d.after;
     std::vector<bool> kept(n, false);
     for (uint32_t i = 0; i < k; ++i) kept[out[i]] = true;
     std::vector<uint32_t> rest;
@@ -330,6 +381,69 @@ uint32_t mvs_subsample(const float* g, uint32_t n, float ratio, uint64_t seed, u
   }
   return k;
 }
+
+uint32_t mvs_subsample(const float* g, uint32_t n, float ratio, uint64_t seed, uint32_t* out) {
+  if (n == 0 || !(ratio > 0.0f)) return 0;
+  MvsDraws d;
+  mvs_draw(seed, n, &d);
+  MvsScratch s;
+  return mvs_select(g, n, ratio, d, out, s);
+}
+
+// MVS draws made ahead on host threads (no HIP call there), a ring of trees
+// at a time; each tree's seed is taken in tree order from the booster's
+// engine state, with the feature-subset draw after it as the training loop
+// takes it (booster.cpp:200-226).
+class MvsAhead {
+ public:
+  MvsAhead() = default;
+  MvsAhead(const MvsAhead&) = delete;
+  MvsAhead& operator=(const MvsAhead&) = delete;
+  ~MvsAhead() { release(); }
+
+  void init(uint32_t N, uint64_t state, bool colsample, size_t ring) {
+    N_ = N;
+    state_ = state;
+    colsample_ = colsample;
+    slots_ = std::vector<MvsDraws>(ring);
+    futs_.resize(ring);
+  }
+
+  // The next tree's draws (valid until the call after next).
+  const MvsDraws& next() {
+    while (submitted_ < prepared_ + slots_.size()) submit();
+    const size_t b = prepared_ % slots_.size();
+    futs_[b].get();  // rethrows a failed draw
+    ++prepared_;
+    return slots_[b];
+  }
+
+  void release() {
+    for (auto& f : futs_)
+      if (f.valid()) f.wait();
+    futs_.clear();
+    slots_.clear();
+  }
+
+ private:
+  // the slot handed out by the previous next() is free again
+  void submit() {
+    const uint64_t seed = next_random(state_);
+    if (colsample_) (void)next_random(state_);  // feature subset: drawn, unused
+    const size_t b = submitted_ % slots_.size();
+    MvsDraws* d = &slots_[b];
+    const uint32_t N = N_;
+    futs_[b] = std::async(std::launch::async, [=] { mvs_draw(seed, N, d); });
+    ++submitted_;
+  }
+
+  uint32_t N_ = 0;
+  uint64_t state_ = 0;
+  bool colsample_ = false;
+  std::vector<MvsDraws> slots_;
+  std::vector<std::future<void>> futs_;
+  size_t submitted_ = 0, prepared_ = 0;
+};
 
 struct JobSpec {
   const tc_dataset* ds;
@@ -389,8 +503,13 @@ class Trainer {
         continue;
       }
       uint64_t seed = 0;
-      if (S.active) seed = next_random(S.state);
-      if (S.colsample) (void)next_random(S.state);  // feature subset: drawn, unused
+      const MvsDraws* md = nullptr;
+      if (S.mvs) {  // drawn ahead: the seed and the feature-subset draw are taken there
+        md = &S.mvs_ahead->next();
+      } else {
+        if (S.active) seed = next_random(S.state);
+        if (S.colsample) (void)next_random(S.state);  // feature subset: drawn, unused
+      }
       if (!S.active) continue;
       const int b = S.next;
       S.next ^= 1;
@@ -401,9 +520,10 @@ class Trainer {
       TC_HIP(hipStreamSynchronize(ctx_->stream));
       for (uint32_t i = 0; i < N; ++i) S.g[i] = S.gh_host[i].x;
       if (S.goss) {
-        goss_subsample(S.g.data(), N, S.top_rate, S.other_rate, seed, S.host[b]);
+        goss_select(S.g.data(), N, S.top_rate, S.other_rate, seed, S.host[b], S.goss_pairs);
       } else {  // a sample of its own size each tree: the device reads n_root from the job table
-        h_jobs_[j].n_root = S.n_root_host[b] = mvs_subsample(S.g.data(), N, S.ratio, seed, S.host[b]);
+        h_jobs_[j].n_root = S.n_root_host[b] = mvs_select(S.g.data(), N, S.ratio, *md, S.host[b],
+                                                          S.mvs_scratch);
         TC_HIP(hipMemcpyAsync(reinterpret_cast<char*>(d_jobs_ + j) + offsetof(JobDev, n_root),
                               &S.n_root_host[b], sizeof(uint32_t), hipMemcpyHostToDevice,
                               ctx_->stream));
@@ -914,6 +1034,10 @@ class Trainer {
                              hipHostMallocDefault));
         SM.g.resize(N);
       }
+      if (SM.mvs) {
+        SM.mvs_ahead = std::make_unique<MvsAhead>();
+        SM.mvs_ahead->init(N, SM.state, SM.colsample, ahead_ring(N, njobs_));
+      }
       J.root_identity = SM.active ? 0u : 1u;
       if (SM.active && !SM.goss && !SM.mvs) {  // a plain row sample: drawn ahead
         SM.plain = std::make_unique<AheadSampler>();
@@ -1099,6 +1223,9 @@ class Trainer {
     hipEvent_t ev[2] = {nullptr, nullptr};
     int next = 0;
     std::unique_ptr<AheadSampler> plain;  // a plain row sample, drawn ahead
+    std::unique_ptr<MvsAhead> mvs_ahead;  // MVS: the uniform draws, made ahead
+    MvsScratch mvs_scratch;
+    std::vector<std::pair<float, uint32_t>> goss_pairs;  // GOSS: kept across trees
   };
 
   tc_ctx* ctx_;
