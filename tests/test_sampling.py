@@ -131,7 +131,7 @@ def test_goss_fit_replays_reference(oracle, loss, colsample, seed):
     (1000, 0.5, 7, "normal"), (5003, 0.3, 2 ** 63 + 9, "ties"), (20000, 0.5, 123, "two"),
     (4000, 0.5, 17, "spiky"), (3000, 0.3, 5, "zeros"), (10, 0.5, 1, "normal"),
     (1, 0.5, 3, "normal"), (777, 1.0, 11, "normal"), (2048, 0.05, 13, "spiky"),
-    (100003, 0.7, 99, "ties")])
+    (100003, 0.7, 99, "ties"), (1048583, 0.5, 31, "ties"), (600011, 0.2, 8, "normal")])
 def test_mvs_subsample_matches_reference(n, ratio, seed, kind):
     """MVS rows equal the compiled reference's Dataset::mvs_subsample on one
     OpenMP thread: the probability pass, and ("spiky": a few huge |g| take

@@ -314,58 +314,108 @@ inline void mvs_draw(uint64_t seed, uint32_t n, MvsDraws* d) {
   d->after = rng;
 }
 
-// The value of ascending rank k (0-based) among n >= k + 1 non-negative
-// floats: two 16-bit counting passes over their bit patterns, which order as
-// the values do.
-inline float kth_nonneg(const float* a, uint32_t n, uint32_t k, std::vector<uint32_t>& cnt) {
-  cnt.assign(65536, 0u);
-  for (uint32_t i = 0; i < n; ++i) ++cnt[__builtin_bit_cast(uint32_t, a[i]) >> 16];
-  uint32_t hi = 0, below = 0;
-  while (below + cnt[hi] <= k) below += cnt[hi++];
-  cnt.assign(65536, 0u);
-  for (uint32_t i = 0; i < n; ++i) {
-    const uint32_t b = __builtin_bit_cast(uint32_t, a[i]);
-    if ((b >> 16) == hi) ++cnt[b & 0xFFFFu];
-  }
-  uint32_t lo = 0;
-  while (below + cnt[lo] <= k) below += cnt[lo++];
-  return __builtin_bit_cast(float, (hi << 16) | lo);
+// f(begin, end, part) over `parts` contiguous row ranges, on host threads
+// (the calling thread takes part 0).
+template <class Fn>
+void row_parts(uint32_t n, uint32_t parts, Fn f) {
+  auto lo = [&](uint32_t t) { return static_cast<uint32_t>(static_cast<uint64_t>(n) * t / parts); };
+  std::vector<std::future<void>> fs;
+  for (uint32_t t = 1; t < parts; ++t) fs.push_back(std::async(std::launch::async, f, lo(t), lo(t + 1), t));
+  f(lo(0), lo(1), 0u);
+  for (auto& x : fs) x.get();
 }
 
 struct MvsScratch {
   std::vector<float> a, sc;
-  std::vector<uint32_t> cnt;
+  std::vector<uint32_t> cnt;               // [part][65536] radix counts
+  std::vector<std::vector<uint32_t>> kept;  // per part: its kept rows
+  std::vector<float> gmax;
 };
 
+// The value of ascending rank k (0-based) among n >= k + 1 non-negative
+// floats: two 16-bit counting passes over their bit patterns, which order as
+// the values do (counts per row range, summed).
+inline float kth_nonneg(const float* a, uint32_t n, uint32_t k, uint32_t parts, MvsScratch& s) {
+  s.cnt.assign(static_cast<size_t>(parts) * 65536u, 0u);
+  row_parts(n, parts, [&](uint32_t b, uint32_t e, uint32_t t) {
+    uint32_t* c = s.cnt.data() + static_cast<size_t>(t) * 65536u;
+    for (uint32_t i = b; i < e; ++i) ++c[__builtin_bit_cast(uint32_t, a[i]) >> 16];
+  });
+  auto total = [&](uint32_t v) {
+    uint32_t x = 0;
+    for (uint32_t t = 0; t < parts; ++t) x += s.cnt[static_cast<size_t>(t) * 65536u + v];
+    return x;
+  };
+  uint32_t hi = 0, below = 0;
+  for (uint32_t c = total(0); below + c <= k; c = total(hi)) {
+    below += c;
+    ++hi;
+  }
+  s.cnt.assign(static_cast<size_t>(parts) * 65536u, 0u);
+  row_parts(n, parts, [&](uint32_t b, uint32_t e, uint32_t t) {
+    uint32_t* c = s.cnt.data() + static_cast<size_t>(t) * 65536u;
+    for (uint32_t i = b; i < e; ++i) {
+      const uint32_t v = __builtin_bit_cast(uint32_t, a[i]);
+      if ((v >> 16) == hi) ++c[v & 0xFFFFu];
+    }
+  });
+  uint32_t lo = 0;
+  for (uint32_t c = total(0); below + c <= k; c = total(lo)) {
+    below += c;
+    ++lo;
+  }
+  return __builtin_bit_cast(float, (hi << 16) | lo);
+}
+
+// The passes whose result does not depend on the order rows are visited in
+// (|g| and its maximum, the rank's value, the floored scores, each row's
+// keep test) run over row ranges on host threads; the score sum is one
+// pass in row order, as the reference's float sum is taken.
 uint32_t mvs_select(const float* g, uint32_t n, float ratio, const MvsDraws& d, uint32_t* out,
                     MvsScratch& s) {
   if (n == 0 || !(ratio > 0.0f)) return 0;
+  const uint32_t parts = std::clamp(n / 131072u, 1u, 8u);
   const uint32_t target = std::max(static_cast<uint32_t>(static_cast<float>(n) * ratio), 1u);
   s.a.resize(n);
   s.sc.resize(n);
+  s.gmax.assign(parts, 0.0f);
   float* a = s.a.data();
   float* sc = s.sc.data();
+  row_parts(n, parts, [&](uint32_t b, uint32_t e, uint32_t t) {
+    float m = 0.0f;
+    for (uint32_t i = b; i < e; ++i) {
+      a[i] = std::abs(g[i]);
+      m = std::max(m, a[i]);
+    }
+    s.gmax[t] = m;
+  });
   float gmax = 0.0f;
-  for (uint32_t i = 0; i < n; ++i) {
-    a[i] = std::abs(g[i]);
-    gmax = std::max(gmax, a[i]);
-  }
+  for (float m : s.gmax) gmax = std::max(gmax, m);
   const uint32_t ti =
       std::min(static_cast<uint32_t>((1.0f - ratio) * static_cast<float>(n)), n - 1);
-  float thr = std::max(kth_nonneg(a, n, ti, s.cnt), gmax * 0.01f);
+  float thr = std::max(kth_nonneg(a, n, ti, parts, s), gmax * 0.01f);
   if (thr < 1e-10f) thr = 1e-10f;
+  row_parts(n, parts, [&](uint32_t b, uint32_t e, uint32_t) {
+    for (uint32_t i = b; i < e; ++i) sc[i] = std::max(a[i], thr);
+  });
   float ssum = 0.0f;
-  for (uint32_t i = 0; i < n; ++i) {
-    sc[i] = std::max(a[i], thr);
-    ssum += sc[i];
-  }
+  for (uint32_t i = 0; i < n; ++i) ssum += sc[i];
   const float fn = static_cast<float>(n);
   const float scale = static_cast<float>(target) / ssum * ssum / fn;
   const float* u = d.u.data();
+  s.kept.resize(parts);
+  row_parts(n, parts, [&](uint32_t b, uint32_t e, uint32_t t) {
+    std::vector<uint32_t>& kt = s.kept[t];
+    kt.clear();
+    for (uint32_t i = b; i < e; ++i) {
+      const float prob = std::min(1.0f, scale * sc[i] / ssum * fn);
+      if (u[i] < prob) kt.push_back(i);
+    }
+  });
   uint32_t k = 0;
-  for (uint32_t i = 0; i < n; ++i) {
-    const float prob = std::min(1.0f, scale * sc[i] / ssum * fn);
-    if (u[i] < prob) out[k++] = i;
+  for (const auto& kt : s.kept) {  // ranges in row order: rows ascending
+    std::copy(kt.begin(), kt.end(), out + k);
+    k += static_cast<uint32_t>(kt.size());
   }
   if (k < static_cast<uint32_t>(static_cast<float>(target) * 0.8f)) {
     std::mt19937_64 rng = d.after;
