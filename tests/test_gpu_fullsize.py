@@ -11,11 +11,11 @@ the checks are properties that hold at any size:
   its first four trees replayed by the oracle and its training predictions equal
   to the oracle's fused updates with the GPU's leaves;
 - batched training, 64 regression jobs of 10,000 to 2,000,000 rows
-  (log-uniform) and 16 to 256 features: every job's model from the batched
-  launches identical to the job trained alone, and the largest job's first
-  tree teacher-forced against the FP64 oracle;
+  (log-uniform) and 16 to 256 features, 200 trees each: every job's model
+  from the batched launches identical to the job trained alone, and the
+  largest job's first tree teacher-forced against the FP64 oracle;
 - multiclass, 4,000,000 rows x 256 features, 10-class softmax, depth 8: a
-  fit identical under two work-item shapes, predictions of all rows equal
+  1,000-tree fit identical under two work-item shapes, predictions of all rows equal
   to predictions of two row halves and to predictions from the training
   bins, class probabilities summing to one.
 
@@ -116,8 +116,10 @@ def test_full_size_multiclass_fit_and_predict(ctx):
     X, y = bench.multiclass_workload(n, F, K, seed=1004)
     ds, edges, ne = ctx.quantize_fit(X)
     try:
-        a = fit(ctx, ds, y, capi.TC_LOSS_SOFTMAX, 2, 8, 0.05, (0, 0), n_classes=K)
-        b = fit(ctx, ds, y, capi.TC_LOSS_SOFTMAX, 2, 8, 0.05, (4096, 512), n_classes=K)
+        # the workload's 1,000 trees, under the autotuned and a fixed work-item shape
+        a = fit(ctx, ds, y, capi.TC_LOSS_SOFTMAX, 1000, 8, 0.05, (0, 0), n_classes=K)
+        print("[fullsize] multiclass: 1000 trees fitted (autotuned shape)", flush=True)
+        b = fit(ctx, ds, y, capi.TC_LOSS_SOFTMAX, 1000, 8, 0.05, (4096, 512), n_classes=K)
         same_model(a, b)
         assert (a["depth"] > 0).all()
         ens = ctx.ensemble(a, F)
@@ -139,7 +141,7 @@ def test_full_size_batched_jobs_equal_each_job_alone(ctx, oracle):
     from turbocat import capi
     from oracle import tree_params as otp
     rng = np.random.default_rng(2002)
-    J, T, D = 64, 4, 6
+    J, T, D = 64, 200, 6  # the batched workload's jobs and trees
     rows = np.floor(np.exp(rng.uniform(np.log(1e4), np.log(2e6), J))).astype(int)
     feats = rng.integers(16, 257, J)
     big = int(np.argmax(rows))
