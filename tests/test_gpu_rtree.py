@@ -290,8 +290,9 @@ def test_rtrain_multiclass_matches_oracle(ctx, oracle):
     assert np.array_equal(m["train_preds"].T, preds)
 
 
+@pytest.mark.parametrize("n", [8000, 7997])  # 7997: LogLoss's scalar tail rows select too
 @pytest.mark.parametrize("kind", ["goss", "mvs"])
-def test_rtrain_goss_mvs_matches_oracle(ctx, oracle, kind):
+def test_rtrain_goss_mvs_matches_oracle(ctx, oracle, kind, n):
     """Booster::train's regular branch with GOSS or MVS (booster.cpp:198-219):
     each tree grows on the rows the reference's selection draws from the
     iteration's gradients (the device's gradients are the oracle's bit for
@@ -301,7 +302,7 @@ def test_rtrain_goss_mvs_matches_oracle(ctx, oracle, kind):
     predictions."""
     from oracle import LOSS_LOGLOSS
     from turbocat import capi
-    X, yr = data_with_nan(8000, 8, 13, nan_frac=0.01)
+    X, yr = data_with_nan(n, 8, 13, nan_frac=0.01)
     y = (yr > 0).astype(np.float32)
     ds, _, _ = ctx.quantize_fit(X)
     bins = ds.bins()
